@@ -1,0 +1,69 @@
+"""Builds libcrn2.so (HIP kernels for gfx950 + host C++) in-tree, and the
+oracle's C restatement used by the tests.
+
+Usage: python -m crunch2_amd.build   (or __graft_entry__.build())
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "crunch2_amd")
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libcrn2.so")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+HOST_SOURCES = ["crn_pack.cpp", "synth.cpp", "crn_api.cpp", "crnd_host.cpp"]
+HIP_SOURCES = ["crnd_gpu.hip"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# -packed-fp32-ops: no v_pk_*_f32 generation; -ffp-contract=off: float
+# expressions round like the reference's x86-64 build (no fused multiply-add).
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + os.path.basename(cmd[-1]))
+    return r
+
+
+def build_oracle():
+    src = os.path.join(ORACLE_DIR, "crnd_oracle.c")
+    _run(["gcc", "-O2", "-std=c99", "-shared", "-fPIC", "-Wall", src, "-o", ORACLE_LIB])
+    return ORACLE_LIB
+
+
+def build_lib():
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for s in HOST_SOURCES:
+        o = os.path.join(BUILD, s + ".o")
+        _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    for s in HIP_SOURCES:
+        o = os.path.join(BUILD, s + ".o")
+        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lgomp"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build():
+    build_oracle()
+    build_lib()
+    shutil.rmtree(BUILD, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    build()
+    print("built", LIB, ORACLE_LIB)

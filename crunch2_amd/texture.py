@@ -1,0 +1,120 @@
+"""Seeded synthetic textures and the CRN writer (host side of libcrn2).
+
+The recipe and writer live in crunch2_amd/csrc/synth.cpp and crn_pack.cpp;
+this module wraps them for the tests and the benchmark.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from ._lib import lib, check, PackInput, FMT_DXT1, FMT_DXT5, FMT_DXN_XY, FMT_DXN_YX, FMT_DXT5A
+
+RGB, RGBA, NORMAL = 0, 1, 2
+
+# palette sizes crnlib derives for quality 128 on a texture of >= 8192 blocks
+# (crnlib/crn_comp.cpp:532-577): colour endpoints/selectors, alpha endpoints/selectors
+Q128_DXT1 = (2413, 2695, 0, 0)
+Q128_DXT5 = (3265, 3549, 1944, 2662)
+Q128_ALPHA = (0, 0, 1944, 2662)
+
+
+def synth(w, h, seed, kind=RGB):
+    out = np.zeros((h, w, 4), np.uint8)
+    check(lib().crn2_synth_rgba(w, h, seed, kind, out.ctypes.data_as(C.c_void_p)), "crn2_synth_rgba")
+    return out
+
+
+def downsample(img):
+    h, w = img.shape[:2]
+    out = np.zeros((max(1, h >> 1), max(1, w >> 1), 4), np.uint8)
+    src = np.ascontiguousarray(img)
+    check(lib().crn2_downsample(src.ctypes.data_as(C.c_void_p), w, h, out.ctypes.data_as(C.c_void_p)))
+    return out
+
+
+def mip_chain(img, levels=None):
+    h, w = img.shape[:2]
+    n = levels or (int(math.floor(math.log2(max(w, h)))) + 1)
+    chain = [np.ascontiguousarray(img)]
+    for _ in range(n - 1):
+        chain.append(downsample(chain[-1]))
+    return chain
+
+
+def compress(faces_levels, fmt, palette_sizes):
+    """faces_levels[f][l] = RGBA8 array of level l of face f; returns .crn bytes."""
+    faces = len(faces_levels)
+    nl = len(faces_levels[0])
+    h, w = faces_levels[0][0].shape[:2]
+    arr = (C.c_void_p * (faces * nl))()
+    keep = []
+    for f in range(faces):
+        for l in range(nl):
+            a = np.ascontiguousarray(faces_levels[f][l], dtype=np.uint8)
+            keep.append(a)
+            arr[f * nl + l] = a.ctypes.data
+    ps = (C.c_uint32 * 4)(*palette_sizes)
+    out = C.c_void_p()
+    size = C.c_uint32()
+    check(lib().crn2_compress_rgba(arr, w, h, nl, faces, fmt, ps, C.byref(out), C.byref(size)), "crn2_compress_rgba")
+    data = C.string_at(out, size.value)
+    lib().crn2_free(out)
+    return data
+
+
+def pack(width, height, levels, faces, fmt, color_endpoints=None, color_selectors=None, alpha_endpoints=None,
+         alpha_selectors=None, ce_idx=None, cs_idx=None, a0e_idx=None, a0s_idx=None, a1e_idx=None, a1s_idx=None):
+    """Writes a .crn from explicit palettes and per-block indices (blocks of the
+    even-padded grid of every level, faces stacked, rows in raster order).
+    Colour selectors are linear 2-bit words, alpha selectors linear 3-bit words."""
+    keep = []
+
+    def ptr(a, dt):
+        if a is None:
+            return None, 0
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data, a.size
+
+    p = PackInput()
+    p.width, p.height, p.levels, p.faces, p.format = width, height, levels, faces, fmt
+    p.color_endpoints, p.n_color_endpoints = ptr(color_endpoints, np.uint32)
+    p.color_selectors_linear, p.n_color_selectors = ptr(color_selectors, np.uint32)
+    p.alpha_endpoints, p.n_alpha_endpoints = ptr(alpha_endpoints, np.uint32)
+    p.alpha_selectors_linear, p.n_alpha_selectors = ptr(alpha_selectors, np.uint64)
+    p.color_endpoint_idx, _ = ptr(ce_idx, np.uint32)
+    p.color_selector_idx, _ = ptr(cs_idx, np.uint32)
+    p.alpha0_endpoint_idx, _ = ptr(a0e_idx, np.uint32)
+    p.alpha0_selector_idx, _ = ptr(a0s_idx, np.uint32)
+    p.alpha1_endpoint_idx, _ = ptr(a1e_idx, np.uint32)
+    p.alpha1_selector_idx, _ = ptr(a1s_idx, np.uint32)
+    out = C.c_void_p()
+    size = C.c_uint32()
+    rc = lib().crn2_pack(C.byref(p), C.byref(out), C.byref(size))
+    if rc != 0:
+        raise RuntimeError("crn2_pack failed: %d" % rc)
+    data = C.string_at(out, size.value)
+    lib().crn2_free(out)
+    return data
+
+
+def padded_grid(width, height, levels):
+    """(gw, gh) of the even-padded block grid of each level."""
+    out = []
+    for l in range(levels):
+        w, h = max(1, width >> l), max(1, height >> l)
+        bx, by = (w + 3) // 4, (h + 3) // 4
+        out.append(((bx + 1) & ~1, (by + 1) & ~1))
+    return out
+
+
+def synth_crn(size, seed, fmt, palette_sizes=None, faces=1, kind=None):
+    """Seeded texture (full mip chain) compressed to .crn in `fmt`."""
+    if kind is None:
+        kind = {FMT_DXT1: RGB, FMT_DXT5: RGBA, FMT_DXT5A: RGBA, FMT_DXN_XY: NORMAL, FMT_DXN_YX: NORMAL}[fmt]
+    if palette_sizes is None:
+        palette_sizes = {FMT_DXT1: Q128_DXT1, FMT_DXT5: Q128_DXT5}.get(fmt, Q128_ALPHA)
+    w, h = size if isinstance(size, tuple) else (size, size)
+    fl = [mip_chain(synth(w, h, seed + 7919 * f, kind)) for f in range(faces)]
+    return compress(fl, fmt, palette_sizes)

@@ -1,0 +1,143 @@
+/*
+ * crn2.h -- C ABI of libcrn2, the MI355X (gfx950) implementation of crnlib's
+ * texture-coding hot path.
+ *
+ * Every entry point is extern "C" and takes plain pointers and sizes.  Host
+ * entry points accept host memory; the *_device entry points accept device
+ * pointers (hipMalloc'ed) and a hipStream_t passed as void*.
+ * Return values: 0 (CRN2_OK) on success, a negative CRN2_E* code on failure,
+ * except where a function is documented to return a count, size or handle.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   crn2_unpack_begin            <- crnd::crnd_unpack_begin        inc/crn_defs.h:174, inc/crn_decomp.h:4404
+ *   crn2_get_data                <- crnd::crnd_get_data            inc/crn_defs.h:178, inc/crn_decomp.h:4422
+ *   crn2_unpack_level            <- crnd::crnd_unpack_level        inc/crn_defs.h:188, inc/crn_decomp.h:4441
+ *   crn2_unpack_level_segmented  <- crnd::crnd_unpack_level_segmented inc/crn_defs.h:196, inc/crn_decomp.h:4457
+ *   crn2_unpack_end              <- crnd::crnd_unpack_end          inc/crn_defs.h:205, inc/crn_decomp.h:4474
+ *   crn2_validate_file           <- crnd::crnd_validate_file       inc/crn_defs.h:139, inc/crn_decomp.h:2672
+ *   crn2_get_texture_info        <- crnd::crnd_get_texture_info    inc/crn_defs.h:143, inc/crn_decomp.h:2737
+ *   crn2_get_level_info          <- crnd::crnd_get_level_info      inc/crn_defs.h:147, inc/crn_decomp.h:2761
+ *   crn2_get_level_data          <- crnd::crnd_get_level_data      inc/crn_defs.h:211, inc/crn_decomp.h:2790
+ *   crn2_get_bytes_per_dxt_block <- crnd::crnd_get_bytes_per_dxt_block inc/crn_defs.h:136, inc/crn_decomp.h:2651
+ *   crn_get_width/height/levels/dxt_format/bytes_per_block/uncompressed_size,
+ *   crn_decompress               <- the extern "C" transcode exports of emscripten/crunch_lib.cpp:32-120
+ */
+#ifndef CRN2_H
+#define CRN2_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRN2_OK 0
+#define CRN2_EINVAL (-1)       /* invalid argument (bad pointer, size, pitch, level) */
+#define CRN2_ENOMEM (-2)       /* host or device allocation failed */
+#define CRN2_EUNSUPPORTED (-3) /* format outside DXT1/DXT5(+swizzles)/DXT5A/DXN */
+#define CRN2_ECORRUPT (-4)     /* stream decodes to an out-of-range palette index */
+#define CRN2_EHIP (-5)         /* a HIP runtime call failed */
+#define CRN2_ENODEV (-6)       /* no gfx950 device / kernels not loadable */
+
+/* crn_format values of inc/crnlib.h:74-110 */
+#define CRN2_FMT_DXT1 0
+#define CRN2_FMT_DXT3 1
+#define CRN2_FMT_DXT5 2
+#define CRN2_FMT_DXT5_CCXY 3
+#define CRN2_FMT_DXT5_XGXR 4
+#define CRN2_FMT_DXT5_XGBR 5
+#define CRN2_FMT_DXT5_AGBR 6
+#define CRN2_FMT_DXN_XY 7
+#define CRN2_FMT_DXN_YX 8
+#define CRN2_FMT_DXT5A 9
+
+/* ------------------------------------------------------------------ info */
+typedef struct crn2_texture_info {
+    uint32_t width, height, levels, faces, bytes_per_block, userdata0, userdata1, format;
+} crn2_texture_info;
+
+typedef struct crn2_level_info {
+    uint32_t width, height, faces, blocks_x, blocks_y, bytes_per_block, format;
+} crn2_level_info;
+
+/* 1 if the header and data CRC16s match and the header fields are in range. */
+int crn2_validate_file(const void* data, uint32_t size);
+int crn2_get_texture_info(const void* data, uint32_t size, crn2_texture_info* info);
+/* bytes_per_block follows crn_decomp.h:2784 (8 only for DXT1 and DXT5A). */
+int crn2_get_level_info(const void* data, uint32_t size, uint32_t level, crn2_level_info* info);
+/* Pointer to the level's compressed bytes inside `data`; *level_size receives its size. */
+const void* crn2_get_level_data(const void* data, uint32_t size, uint32_t level, uint32_t* level_size);
+uint32_t crn2_get_bytes_per_dxt_block(uint32_t format);
+
+/* ------------------------------------------------------------ transcoder */
+typedef struct crn2_unpack_ctx crn2_unpack_ctx;
+
+/* Parses the header and the five prefix-code models, copies the file to the
+ * device and decodes the four palettes there.  `data` must stay valid until
+ * crn2_unpack_end (as crnd_unpack_begin requires).  NULL on failure. */
+crn2_unpack_ctx* crn2_unpack_begin(const void* data, uint32_t size);
+
+/* Same, for a file that already lives in device memory (`data_dev`); the
+ * host copy `data` is still read for the header and models. */
+crn2_unpack_ctx* crn2_unpack_begin_device(const void* data, const void* data_dev, uint32_t size, void* stream);
+
+int crn2_get_data(crn2_unpack_ctx* ctx, const void** data, uint32_t* size);
+
+/* Transcodes one level into the host buffers dst[0..faces-1] with the given
+ * row pitch (0 = minimal; otherwise a multiple of 4 and >= blocks_x * bpb);
+ * dst_size >= pitch * blocks_y.  Argument checks follow crn_decomp.h:4441-4455
+ * and 3563-3575. */
+int crn2_unpack_level(crn2_unpack_ctx* ctx, void** dst, uint32_t dst_size, uint32_t row_pitch, uint32_t level);
+
+/* Segmented variant: the level's compressed bytes are given by the caller. */
+int crn2_unpack_level_segmented(crn2_unpack_ctx* ctx, const void* src, uint32_t src_size, void** dst,
+                                uint32_t dst_size, uint32_t row_pitch, uint32_t level);
+
+/* Device variant: dst_dev[f] are device pointers; work is queued on `stream`
+ * (a hipStream_t, NULL = default stream) and the call returns once queued. */
+int crn2_unpack_level_device(crn2_unpack_ctx* ctx, void** dst_dev, uint32_t dst_size, uint32_t row_pitch,
+                             uint32_t level, void* stream);
+
+/* Transcodes levels [first, first + count) of every face into one device
+ * buffer, levels in order, faces in order within a level, minimal pitch. */
+int crn2_unpack_levels_device(crn2_unpack_ctx* ctx, void* dst_dev, uint64_t dst_size, uint32_t first,
+                              uint32_t count, void* stream);
+
+/* Bytes written by crn2_unpack_levels_device for the given level range. */
+uint64_t crn2_levels_size(crn2_unpack_ctx* ctx, uint32_t first, uint32_t count);
+
+int crn2_unpack_end(crn2_unpack_ctx* ctx);
+
+/* -------------------------------------- emscripten-compatible C exports */
+unsigned int crn_get_width(void* src, unsigned int src_size);
+unsigned int crn_get_height(void* src, unsigned int src_size);
+unsigned int crn_get_levels(void* src, unsigned int src_size);
+unsigned int crn_get_dxt_format(void* src, unsigned int src_size);
+unsigned int crn_get_bytes_per_block(void* src, unsigned int src_size);
+unsigned int crn_get_uncompressed_size(void* src, unsigned int src_size, unsigned int level);
+/* Levels [firstLevel, firstLevel + levelCount) of face 0, packed back to back
+ * at minimal pitch, sizes computed from width >> level exactly as
+ * emscripten/crunch_lib.cpp:91-120 does (no max(1, ...)). */
+void crn_decompress(void* src, unsigned int src_size, void* dst, unsigned int dst_size, unsigned int firstLevel,
+                    unsigned int levelCount);
+
+/* ---------------------------------------------- writer and test textures */
+/* CRC16 of crn_decomp.h:2048-2069. */
+uint16_t crn2_crc16(const void* p, uint32_t len, uint16_t crc);
+/* Seeded synthetic RGBA8 texture: kind 0 = RGB, 1 = RGB + radial alpha, 2 = normal map. */
+int crn2_synth_rgba(uint32_t w, uint32_t h, uint64_t seed, int kind, uint8_t* out);
+/* 2x2 box filter with rounding, w x h -> max(1,w/2) x max(1,h/2). */
+int crn2_downsample(const uint8_t* src, uint32_t w, uint32_t h, uint8_t* dst);
+/* Quantizes RGBA8 levels (levels[f * nlevels + l]) to palettes of at most
+ * palette_sizes = {colour endpoints, colour selectors, alpha endpoints, alpha
+ * selectors} entries and writes a .crn; *out is freed with crn2_free. */
+int crn2_compress_rgba(const uint8_t* const* levels, uint32_t width, uint32_t height, uint32_t nlevels,
+                       uint32_t faces, uint32_t format, const uint32_t* palette_sizes, uint8_t** out,
+                       uint32_t* out_size);
+void crn2_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRN2_H */

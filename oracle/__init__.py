@@ -1,0 +1,165 @@
+"""CPU oracle for the transcoder -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this package, and only as the checker.  It wraps liboracle.so, built from
+oracle/crnd_oracle.c (a plain-C restatement of inc/crn_decomp.h; see the
+file header for the reference lines each part follows).
+
+Parity pins (what checks each part against something other than itself):
+  crc16 .................. CRC-16/GENIBUS catalogue check value and a
+                           bitwise textbook CRC (tests/test_oracle.py)
+  canonical prefix codes . brute-force prefix matching against codewords
+                           assigned by the RFC 1951 3.2.2 procedure
+  model transmission ..... decode of models written by the project's
+                           independent encoder (crunch2_amd/csrc/crn_pack.cpp)
+                           for random code-length vectors, run-code extremes
+  palettes / levels ...... round trip: random palettes and per-block indices
+                           packed by the writer must come back as the DXTn
+                           blocks computed directly in numpy from the same
+                           palettes (selector maps checked against the DXT
+                           code tables)
+The reference itself cannot be built here (inc/crn_decomp.h includes the
+CMake-generated crn_export.h), so no reference binary cross-check exists.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: " + LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        L.orc_unpack_begin.restype = C.c_void_p
+        L.orc_unpack_begin.argtypes = [C.c_void_p, C.c_uint32]
+        L.orc_unpack_level.restype = C.c_int
+        L.orc_unpack_level.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.c_uint32, C.c_uint32]
+        L.orc_unpack_end.argtypes = [C.c_void_p]
+        L.orc_crc16.restype = C.c_uint16
+        L.orc_crc16.argtypes = [C.c_void_p, C.c_uint32, C.c_uint16]
+        L.orc_validate_file.argtypes = [C.c_void_p, C.c_uint32]
+        L.orc_get_info.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]
+        L.orc_palette.restype = C.c_uint32
+        L.orc_palette.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_trace_level.argtypes = [C.c_void_p, C.c_uint32] + [C.c_void_p] * 7
+        L.orc_decode_symbols.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]
+        L.orc_receive_model.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def crc16(data, crc=0):
+    b = np.frombuffer(bytes(data), np.uint8)
+    return int(lib().orc_crc16(_p(b), b.nbytes, crc))
+
+
+def validate_file(data):
+    b = np.frombuffer(bytes(data), np.uint8)
+    return bool(lib().orc_validate_file(_p(b), b.nbytes))
+
+
+def info(data):
+    b = np.frombuffer(bytes(data), np.uint8)
+    out = (C.c_uint32 * 9)()
+    if not lib().orc_get_info(_p(b), b.nbytes, out):
+        return None
+    keys = ("width", "height", "levels", "faces", "format", "n_ce", "n_cs", "n_ae", "n_as")
+    return dict(zip(keys, list(out)))
+
+
+class Context:
+    def __init__(self, data):
+        self.data = np.frombuffer(bytes(data), np.uint8).copy()
+        self.h = lib().orc_unpack_begin(_p(self.data), self.data.nbytes)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def close(self):
+        if self.h:
+            lib().orc_unpack_end(C.c_void_p(self.h))
+            self.h = None
+
+    def unpack_level(self, level, row_pitch=0, dst=None, dst_size=None):
+        inf = info(self.data)
+        w, h = max(1, inf["width"] >> level), max(1, inf["height"] >> level)
+        bpb = 8 if inf["format"] in (0, 9) else 16
+        bx, by = (w + 3) // 4, (h + 3) // 4
+        pitch = row_pitch or bx * bpb
+        if dst is None:
+            dst = [np.zeros(pitch * by, np.uint8) for _ in range(inf["faces"])]
+        if dst_size is None:
+            dst_size = min(d.nbytes for d in dst)
+        ptrs = (C.c_void_p * 6)(*[d.ctypes.data for d in dst])
+        ok = lib().orc_unpack_level(C.c_void_p(self.h), ptrs, dst_size, row_pitch, level)
+        return dst if ok else False
+
+    def palette(self, which):
+        n = lib().orc_palette(C.c_void_p(self.h), which, None)
+        dt, per = {0: (np.uint32, 1), 1: (np.uint32, 1), 2: (np.uint16, 1), 3: (np.uint16, 3)}[which]
+        out = np.zeros(max(1, n * per), dt)
+        lib().orc_palette(C.c_void_p(self.h), which, _p(out))
+        return out[: n * per]
+
+    def trace_level(self, level):
+        """Per-block decoded indices and bit positions for the even-padded grid."""
+        inf = info(self.data)
+        w, h = max(1, inf["width"] >> level), max(1, inf["height"] >> level)
+        gw, gh = (((w + 3) // 4) + 1) & ~1, (((h + 3) // 4) + 1) & ~1
+        n = gw * gh * inf["faces"]
+        arrs = [np.zeros(n, np.uint32) for _ in range(6)]
+        pos = np.zeros(n + 1, np.uint64)
+        ok = lib().orc_trace_level(C.c_void_p(self.h), level, *[_p(a) for a in arrs], _p(pos))
+        if not ok:
+            return None
+        keys = ("ce", "cs", "a0e", "a0s", "a1e", "a1s")
+        d = dict(zip(keys, arrs))
+        d["bitpos"] = pos
+        return d
+
+
+def unpack_all(data):
+    """All levels of all faces as {level: [face arrays]} (None if begin fails)."""
+    with Context(data) as c:
+        if not c.h:
+            return None
+        inf = info(data)
+        out = {}
+        for l in range(inf["levels"]):
+            r = c.unpack_level(l)
+            if r is False:
+                return None
+            out[l] = r
+        return out
+
+
+def decode_symbols(code_sizes, data, count):
+    s = np.ascontiguousarray(code_sizes, np.uint8)
+    d = np.frombuffer(bytes(data), np.uint8)
+    out = np.zeros(count, np.uint32)
+    if not lib().orc_decode_symbols(_p(s), s.size, _p(d), d.nbytes, count, _p(out)):
+        return None
+    return out
+
+
+def receive_model(data):
+    d = np.frombuffer(bytes(data), np.uint8)
+    sizes = np.zeros(8192, np.uint8)
+    n = C.c_uint32()
+    if not lib().orc_receive_model(_p(d), d.nbytes, _p(sizes), C.byref(n)):
+        return None
+    return sizes[: n.value]

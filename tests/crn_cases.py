@@ -1,0 +1,169 @@
+"""Known-answer cases for the transcoder built without the transcoder.
+
+Random palettes and per-block palette indices are packed into a .crn by the
+project's writer; the DXTn bytes each level must transcode to are computed
+here directly from the palettes with the DXT1/DXT5 code tables
+(inc/crn_decomp.h:3264-3269): colour selectors in linear order 0..3 map to
+DXT1 codes {0, 2, 3, 1}; alpha selectors in linear order 0..7 map to DXT5
+codes {0, 2, 3, 4, 5, 6, 7, 1}.  Nothing here calls a decoder.
+"""
+import numpy as np
+
+from crunch2_amd import texture
+from crunch2_amd._lib import FMT_DXT1, FMT_DXT5, FMT_DXN_XY, FMT_DXN_YX, FMT_DXT5A, FMT_DXT5_AGBR
+
+DXT1_FROM_LINEAR = np.array([0, 2, 3, 1], np.uint32)
+DXT5_FROM_LINEAR = np.array([0, 2, 3, 4, 5, 6, 7, 1], np.uint64)
+
+COLOR_FMTS = (FMT_DXT1, FMT_DXT5, FMT_DXT5_AGBR)
+
+
+def kind(fmt):
+    if fmt == FMT_DXT1:
+        return "dxt1"
+    if fmt in (FMT_DXN_XY, FMT_DXN_YX):
+        return "dxn"
+    if fmt == FMT_DXT5A:
+        return "dxt5a"
+    return "dxt5"
+
+
+def bpb(fmt):
+    return 8 if fmt in (FMT_DXT1, FMT_DXT5A) else 16
+
+
+def color_sel_raw(lin):
+    lin = np.asarray(lin, np.uint32)
+    out = np.zeros_like(lin)
+    for i in range(16):
+        out |= DXT1_FROM_LINEAR[(lin >> (2 * i)) & 3] << (2 * i)
+    return out
+
+
+def alpha_sel_raw(lin):
+    lin = np.asarray(lin, np.uint64)
+    out = np.zeros_like(lin)
+    for i in range(16):
+        out |= DXT5_FROM_LINEAR[((lin >> np.uint64(3 * i)) & np.uint64(7)).astype(np.int64)] << np.uint64(3 * i)
+    return out
+
+
+def _indices(rng, n_blocks_grid, gw, n, p_left, p_up):
+    """Index stream with runs (left copies) and vertical copies so that every
+    reference code shows up."""
+    out = np.zeros(n_blocks_grid, np.uint32)
+    for b in range(n_blocks_grid):
+        r = rng.random()
+        if b > 0 and r < p_left:
+            out[b] = out[b - 1]
+        elif b >= gw and r < p_left + p_up:
+            out[b] = out[b - gw]
+        else:
+            out[b] = rng.integers(0, n)
+    return out
+
+
+def make_case(seed, width, height, levels, faces, fmt, n_ce=37, n_cs=53, n_ae=29, n_as=41, p_left=0.35, p_up=0.35):
+    rng = np.random.default_rng(seed)
+    grids = texture.padded_grid(width, height, levels)
+    k = kind(fmt)
+    pal = {}
+    if k in ("dxt1", "dxt5"):
+        pal["ce"] = rng.integers(0, 2 ** 32, n_ce, dtype=np.uint64).astype(np.uint32)
+        pal["cs"] = rng.integers(0, 2 ** 32, n_cs, dtype=np.uint64).astype(np.uint32)
+    if k != "dxt1":
+        pal["ae"] = rng.integers(0, 2 ** 16, n_ae).astype(np.uint32)
+        pal["as"] = rng.integers(0, 2 ** 48, n_as, dtype=np.uint64)
+    idx = {key: [] for key in ("ce", "cs", "a0e", "a0s", "a1e", "a1s")}
+    for (gw, gh) in grids:
+        nb = gw * gh * faces
+        if k in ("dxt1", "dxt5"):
+            idx["ce"].append(_indices(rng, nb, gw, n_ce, p_left, p_up))
+            idx["cs"].append(rng.integers(0, n_cs, nb).astype(np.uint32))
+        if k != "dxt1":
+            idx["a0e"].append(_indices(rng, nb, gw, n_ae, p_left, p_up))
+            idx["a0s"].append(rng.integers(0, n_as, nb).astype(np.uint32))
+        if k == "dxn":
+            idx["a1e"].append(_indices(rng, nb, gw, n_ae, p_left, p_up))
+            idx["a1s"].append(rng.integers(0, n_as, nb).astype(np.uint32))
+    if k == "dxt5":
+        # a block's colour and alpha references are one code: keep them in step
+        for l in range(levels):
+            same = np.zeros(len(idx["ce"][l]), bool)
+            same[1:] = idx["ce"][l][1:] == idx["ce"][l][:-1]
+            a = idx["a0e"][l]
+            for b in np.nonzero(same)[0]:
+                a[b] = a[b - 1]
+    cat = {key: (np.concatenate(v) if v else None) for key, v in idx.items()}
+    data = texture.pack(width, height, levels, faces, fmt,
+                        color_endpoints=pal.get("ce"), color_selectors=pal.get("cs"),
+                        alpha_endpoints=pal.get("ae"), alpha_selectors=pal.get("as"),
+                        ce_idx=cat["ce"], cs_idx=cat["cs"], a0e_idx=cat["a0e"], a0s_idx=cat["a0s"],
+                        a1e_idx=cat["a1e"], a1s_idx=cat["a1s"])
+    expected = expected_levels(width, height, levels, faces, fmt, pal, idx)
+    return data, expected
+
+
+def _alpha_block_words(ae, as_raw):
+    w0 = (ae & 0xFFFF).astype(np.uint64) | ((as_raw & np.uint64(0xFFFF)) << np.uint64(16))
+    w1 = (as_raw >> np.uint64(16)) & np.uint64(0xFFFFFFFF)
+    return w0.astype(np.uint32), w1.astype(np.uint32)
+
+
+def expected_levels(width, height, levels, faces, fmt, pal, idx):
+    """{level: [face uint8 arrays]} at minimal pitch."""
+    k = kind(fmt)
+    grids = texture.padded_grid(width, height, levels)
+    cs_raw = color_sel_raw(pal["cs"]) if "cs" in pal else None
+    as_raw = alpha_sel_raw(pal["as"]) if "as" in pal else None
+    out = {}
+    for l, (gw, gh) in enumerate(grids):
+        w, h = max(1, width >> l), max(1, height >> l)
+        bx, by = (w + 3) // 4, (h + 3) // 4
+        faces_out = []
+        for f in range(faces):
+            sl = slice(f * gw * gh, (f + 1) * gw * gh)
+
+            def grid(key):
+                return idx[key][l][sl].reshape(gh, gw)[:by, :bx].reshape(-1)
+
+            if k == "dxt1":
+                words = np.stack([pal["ce"][grid("ce")], cs_raw[grid("cs")]], 1)
+            elif k == "dxt5a":
+                a0, a1 = _alpha_block_words(pal["ae"][grid("a0e")], as_raw[grid("a0s")])
+                words = np.stack([a0, a1], 1)
+            elif k == "dxn":
+                a0, a1 = _alpha_block_words(pal["ae"][grid("a0e")], as_raw[grid("a0s")])
+                b0, b1 = _alpha_block_words(pal["ae"][grid("a1e")], as_raw[grid("a1s")])
+                words = np.stack([a0, a1, b0, b1], 1)
+            else:
+                a0, a1 = _alpha_block_words(pal["ae"][grid("a0e")], as_raw[grid("a0s")])
+                words = np.stack([a0, a1, pal["ce"][grid("ce")], cs_raw[grid("cs")]], 1)
+            faces_out.append(np.ascontiguousarray(words.astype(np.uint32)).view(np.uint8).reshape(-1))
+        out[l] = faces_out
+    return out
+
+
+# (seed, width, height, levels, faces, fmt, palette sizes) covering odd sizes,
+# single-block levels, cubemaps, every format and palette extremes
+CASES = [
+    (1, 1, 1, 1, 1, FMT_DXT1, {}),
+    (2, 13, 7, 4, 1, FMT_DXT1, {}),
+    (3, 64, 64, 7, 1, FMT_DXT1, {}),
+    (4, 40, 24, 6, 6, FMT_DXT1, {}),
+    (5, 33, 65, 7, 1, FMT_DXT5, {}),
+    (6, 16, 16, 5, 6, FMT_DXT5, {}),
+    (7, 24, 20, 5, 1, FMT_DXN_XY, {}),
+    (8, 20, 12, 4, 1, FMT_DXN_YX, {}),
+    (9, 28, 44, 6, 1, FMT_DXT5A, {}),
+    (10, 32, 32, 6, 1, FMT_DXT5_AGBR, {}),
+    (11, 48, 48, 6, 1, FMT_DXT1, {"n_ce": 1, "n_cs": 1}),
+    (12, 256, 256, 9, 1, FMT_DXT1, {"n_ce": 8192, "n_cs": 8192}),
+    (13, 96, 80, 7, 1, FMT_DXT5, {"n_ae": 8192, "n_as": 8192, "n_ce": 3, "n_cs": 2}),
+    (14, 8, 4096, 13, 1, FMT_DXT1, {}),
+]
+
+
+def case(i):
+    seed, w, h, nl, faces, fmt, kw = CASES[i]
+    return make_case(seed, w, h, nl, faces, fmt, **kw)
