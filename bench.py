@@ -1,0 +1,177 @@
+"""Benchmark: CRN->DXTn transcode of a 4096x4096 RGBA8 texture with its full
+13-level mip chain, in DXT1 and DXT5 (one step = both files, all levels),
+inputs resident in HBM, on one MI355X.
+
+Output: one JSON line (see the keys below).  The CPU baseline is the
+project's C restatement of the transcoder (oracle/) timed on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=1, help="texture pairs (distinct seeds) per step")
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline sample length")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (each GPU would transcode its "
+              "own textures independently); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from crunch2_amd import crnd, texture
+    from crunch2_amd._lib import FMT_DXT1, FMT_DXT5, lib
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+
+    # ---- inputs: seeded textures compressed by the project's CRN writer
+    t0 = time.time()
+    files = []
+    for b in range(args.batch):
+        for fmt in (FMT_DXT1, FMT_DXT5):
+            files.append(texture.synth_crn(args.size, 1 + b, fmt))
+    log("inputs built in %.1fs: %s bytes" % (time.time() - t0, [len(f) for f in files]))
+
+    ctxs, outs, counts = [], [], []
+    texels = 0
+    crn_bytes = 0
+    out_bytes = 0
+    for d in files:
+        ctx = crnd.unpack_begin(d)
+        assert ctx is not None
+        inf = crnd.get_texture_info(d)
+        n = crnd.levels_size(ctx, 0, inf["levels"])
+        outs.append(torch.empty(n, dtype=torch.uint8, device=dev))
+        ctxs.append(ctx)
+        counts.append(inf["levels"])
+        for l in range(inf["levels"]):
+            li = crnd.get_level_info(d, l)
+            texels += li["blocks_x"] * li["blocks_y"] * 16 * li["faces"]
+        crn_bytes += len(d)
+        out_bytes += n
+
+    # one plan: every level of every file in one parse launch + one write launch
+    n = len(ctxs)
+    plan = C.c_void_p()
+    rc = lib().crn2_batch_create((C.c_void_p * n)(*[c.handle for c in ctxs]), n,
+                                 (C.c_void_p * n)(*[o.data_ptr() for o in outs]),
+                                 (C.c_uint64 * n)(*[o.numel() for o in outs]), C.byref(plan))
+    assert rc == 0, rc
+
+    def step():
+        assert lib().crn2_batch_run(plan, C.c_void_p(stream.cuda_stream)) == 0
+
+    # ---- parity spot check before timing (oracle restatement, first file)
+    import oracle
+    ref = oracle.unpack_all(files[0])
+    step()
+    torch.cuda.synchronize()
+    exp = np.concatenate([ref[l][0] for l in sorted(ref)])
+    assert np.array_equal(outs[0].cpu().numpy(), exp), "GPU output differs from the oracle"
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    lib().crn2_profile_reset()
+    lib().crn2_profile_enable(1)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    lib().crn2_profile_enable(0)
+    total_ms = e0.elapsed_time(e1)
+    ms = total_ms / args.steps
+
+    prof = {}
+    for k, name in ((0, "crnd_parse_kernel"), (1, "crnd_write_kernel")):
+        t = C.c_double()
+        n = C.c_uint64()
+        lib().crn2_profile_read(k, C.byref(t), C.byref(n))
+        prof[name] = (t.value / max(1, n.value), n.value)
+    launches_per_step = prof["crnd_parse_kernel"][1] / args.steps
+
+    mtex = texels / (ms * 1e-3) / 1e6
+    alg_bytes = crn_bytes + out_bytes
+    parse_ms, _ = prof["crnd_parse_kernel"]
+    write_ms, _ = prof["crnd_write_kernel"]
+    # algorithmic bytes of one launch pair (all files of a step): compressed in + DXTn out
+    achieved = alg_bytes / ((parse_ms + write_ms) * 1e-3) / 1e9
+
+    # ---- CPU baseline: the C restatement, single thread, same files
+    cpu_t = 0.0
+    cpu_tex = 0
+    reps = 0
+    t0 = time.time()
+    while time.time() - t0 < args.cpu_seconds:
+        for d in files:
+            s = time.perf_counter()
+            r = oracle.unpack_all(d)
+            cpu_t += time.perf_counter() - s
+            assert r is not None
+        cpu_tex += texels
+        reps += 1
+    cpu_mtex = cpu_tex / cpu_t / 1e6
+
+    line = {
+        "metric": "Mtexels/s: CRN->DXTn transcode (DXT1 + DXT5, 4096^2, 13 levels)",
+        "value": round(mtex, 1),
+        "unit": "Mtexels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded 4096^2 RGBA8 texture, full box-filter mip chain, packed to .crn by the "
+                "project's writer at crnlib's quality-128 palette sizes)",
+        "config": {"workload": "CRN->DXTn transcode, %dx%d, 13 levels, DXT1 + DXT5, %d texture pair(s) per step"
+                               % (args.size, args.size, args.batch),
+                   "crn_bytes": crn_bytes, "dxtn_bytes": out_bytes, "texels": texels},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "kernel": "crnd_parse_kernel + crnd_write_kernel (one launch each per step)",
+                     "parse_ms": round(parse_ms, 4), "write_ms": round(write_ms, 4)},
+        "cpu_baseline": {"value": round(cpu_mtex, 1), "unit": "Mtexels/s", "cores": 1, "kind": "port",
+                         "sample": "%d full passes over the same %d files (%.1fs of CPU decode)"
+                                   % (reps, len(files), cpu_t)},
+    }
+    assert lib().crn2_batch_status(plan, C.c_void_p(stream.cuda_stream)) == 0
+    print(json.dumps(line))
+    lib().crn2_batch_destroy(plan)
+    for c in ctxs:
+        crnd.unpack_end(c)
+
+
+if __name__ == "__main__":
+    main()

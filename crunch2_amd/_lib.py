@@ -63,6 +63,14 @@ SIGNATURES = {
                                             C.c_void_p]),
     "crn2_levels_size": (C.c_uint64, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "crn2_unpack_end": (C.c_int, [C.c_void_p]),
+    "crn2_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p),
+                                    C.POINTER(C.c_uint64), C.POINTER(C.c_void_p)]),
+    "crn2_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "crn2_batch_status": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "crn2_batch_destroy": (C.c_int, [C.c_void_p]),
+    "crn2_profile_enable": (None, [C.c_int]),
+    "crn2_profile_reset": (None, []),
+    "crn2_profile_read": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "crn_get_width": (C.c_uint, [C.c_void_p, C.c_uint]),
     "crn_get_height": (C.c_uint, [C.c_void_p, C.c_uint]),
     "crn_get_levels": (C.c_uint, [C.c_void_p, C.c_uint]),
@@ -70,6 +78,11 @@ SIGNATURES = {
     "crn_get_bytes_per_block": (C.c_uint, [C.c_void_p, C.c_uint]),
     "crn_get_uncompressed_size": (C.c_uint, [C.c_void_p, C.c_uint, C.c_uint]),
     "crn_decompress": (None, [C.c_void_p, C.c_uint, C.c_void_p, C.c_uint, C.c_uint, C.c_uint]),
+    "crn2_dxt5a_blocks_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]),
+    "crn2_encode_dxt5a_image_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
+                                                 C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "crn2_dxt1_blocks_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "crn2_crc16": (C.c_uint16, [C.c_void_p, C.c_uint32, C.c_uint16]),
     "crn2_synth_rgba": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_void_p]),
     "crn2_downsample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),

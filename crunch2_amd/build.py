@@ -16,8 +16,9 @@ LIB = os.path.join(PKG, "libcrn2.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
 
+ORACLE_SOURCES = ["crnd_oracle.c", "dxt5a_oracle.c", "dxt1_oracle.c"]
 HOST_SOURCES = ["crn_pack.cpp", "synth.cpp", "crn_api.cpp", "crnd_host.cpp"]
-HIP_SOURCES = ["crnd_gpu.hip"]
+HIP_SOURCES = ["crnd_gpu.hip", "dxt5a_gpu.hip", "dxt1_gpu.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_*_f32 generation; -ffp-contract=off: float
@@ -36,8 +37,8 @@ def _run(cmd):
 
 
 def build_oracle():
-    src = os.path.join(ORACLE_DIR, "crnd_oracle.c")
-    _run(["gcc", "-O2", "-std=c99", "-shared", "-fPIC", "-Wall", src, "-o", ORACLE_LIB])
+    srcs = [os.path.join(ORACLE_DIR, s) for s in ORACLE_SOURCES]
+    _run(["gcc", "-O2", "-std=gnu99", "-shared", "-fPIC", "-Wall", "-ffp-contract=off"] + srcs + ["-lm", "-o", ORACLE_LIB])
     return ORACLE_LIB
 
 

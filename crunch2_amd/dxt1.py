@@ -1,0 +1,51 @@
+"""DXT1 endpoint search on the GPU (crnlib's dxt1_endpoint_optimizer,
+crnlib/crn_dxt1.h:163-352) -- Python face of crn2_dxt1_blocks_device."""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import lib, check
+
+PERCEPTUAL, ALPHA_BLOCKS, GRAYSCALE, FORCE_ALPHA = 1, 2, 4, 8
+UBER, BETTER = 4, 3
+OUT_DTYPE = np.dtype([("error", "<u8"), ("lo_hi", "<u4"), ("selectors", "<u4"), ("flags", "<u4"), ("pad", "<u4")])
+
+
+def encode_blocks_device(rgba_blocks, quality=UBER, flags=PERCEPTUAL | ALPHA_BLOCKS, out=None, stream=None):
+    """rgba_blocks: uint8 CUDA tensor [n, 16, 4] (contiguous). Returns a uint8
+    CUDA tensor [n, 24] of packed results (see include/crn2.h)."""
+    n = rgba_blocks.shape[0]
+    if out is None:
+        out = torch.empty((n, 24), dtype=torch.uint8, device=rgba_blocks.device)
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(lib().crn2_dxt1_blocks_device(C.c_void_p(rgba_blocks.data_ptr()), n, quality, flags,
+                                        C.c_void_p(out.data_ptr()), C.c_void_p(s)), "crn2_dxt1_blocks_device")
+    return out
+
+
+def unpack_results(raw):
+    """uint8 [n, 24] -> (selectors [n,16], error, low, high, alpha_block,
+    reordered, alternate_rounding) numpy arrays, the layout of the oracle."""
+    r = np.ascontiguousarray(raw).view(OUT_DTYPE).reshape(-1)
+    sel = ((r["selectors"][:, None] >> (2 * np.arange(16, dtype=np.uint32))[None, :]) & 3).astype(np.uint8)
+    f = r["flags"]
+    return (sel, r["error"], (r["lo_hi"] & 0xFFFF).astype(np.uint16), (r["lo_hi"] >> 16).astype(np.uint16),
+            (f & 1).astype(np.uint8), ((f >> 1) & 1).astype(np.uint8), ((f >> 2) & 1).astype(np.uint8))
+
+
+def encode_blocks(rgba_blocks, quality=UBER, flags=PERCEPTUAL | ALPHA_BLOCKS):
+    t = torch.from_numpy(np.ascontiguousarray(rgba_blocks, np.uint8).reshape(-1, 16, 4)).cuda()
+    raw = encode_blocks_device(t, quality, flags)
+    torch.cuda.synchronize()
+    return unpack_results(raw.cpu().numpy())
+
+
+def image_blocks(img):
+    """RGBA8 image [h, w, 4] -> [nblocks, 16, 4] with edge clamp (crn_dxt_image.cpp:330-344)."""
+    h, w = img.shape[:2]
+    bx, by = (w + 3) // 4, (h + 3) // 4
+    ys = np.minimum(np.arange(by * 4), h - 1)
+    xs = np.minimum(np.arange(bx * 4), w - 1)
+    p = img[ys][:, xs]
+    return np.ascontiguousarray(p.reshape(by, 4, bx, 4, 4).transpose(0, 2, 1, 3, 4).reshape(-1, 16, 4))

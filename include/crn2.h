@@ -72,8 +72,8 @@ uint32_t crn2_get_bytes_per_dxt_block(uint32_t format);
 /* ------------------------------------------------------------ transcoder */
 typedef struct crn2_unpack_ctx crn2_unpack_ctx;
 
-/* Parses the header and the five prefix-code models, copies the file to the
- * device and decodes the four palettes there.  `data` must stay valid until
+/* Parses the header and the five prefix-code models, decodes the four
+ * palettes and copies file, tables and palettes to the device.  `data` must stay valid until
  * crn2_unpack_end (as crnd_unpack_begin requires).  NULL on failure. */
 crn2_unpack_ctx* crn2_unpack_begin(const void* data, uint32_t size);
 
@@ -108,6 +108,28 @@ uint64_t crn2_levels_size(crn2_unpack_ctx* ctx, uint32_t first, uint32_t count);
 
 int crn2_unpack_end(crn2_unpack_ctx* ctx);
 
+/* ------------------------------------------------------ batched transcode */
+/* A plan that transcodes every level of n textures in one parse launch and
+ * one write launch (the reference transcodes one level per call; there is no
+ * counterpart for the batch).  Texture i is written to dst_dev[i] as
+ * crn2_unpack_levels_device(ctxs[i], dst_dev[i], .., 0, levels) would write
+ * it.  The plan shares each context's scratch: two runs that touch the same
+ * context must not overlap. */
+typedef struct crn2_batch crn2_batch;
+int crn2_batch_create(crn2_unpack_ctx* const* ctxs, uint32_t n, void* const* dst_dev, const uint64_t* dst_size,
+                      crn2_batch** out);
+int crn2_batch_run(crn2_batch* batch, void* stream);
+/* Waits for the stream and returns CRN2_ECORRUPT if any stream of the last
+ * runs decoded an out-of-range palette index (clearing the flag). */
+int crn2_batch_status(crn2_batch* batch, void* stream);
+int crn2_batch_destroy(crn2_batch* batch);
+
+/* Kernel timing with HIP events on the launch stream (kernel 0 = level-stream
+ * parse, 1 = block write): enable, run, then read total ms and launch count. */
+void crn2_profile_enable(int on);
+void crn2_profile_reset(void);
+int crn2_profile_read(int kernel, double* total_ms, uint64_t* launches);
+
 /* -------------------------------------- emscripten-compatible C exports */
 unsigned int crn_get_width(void* src, unsigned int src_size);
 unsigned int crn_get_height(void* src, unsigned int src_size);
@@ -120,6 +142,38 @@ unsigned int crn_get_uncompressed_size(void* src, unsigned int src_size, unsigne
  * emscripten/crunch_lib.cpp:91-120 does (no max(1, ...)). */
 void crn_decompress(void* src, unsigned int src_size, void* dst, unsigned int dst_size, unsigned int firstLevel,
                     unsigned int levelCount);
+
+/* ------------------------------------------- DXT5 alpha endpoint search */
+/* crnlib's dxt5_endpoint_optimizer::compute (crnlib/crn_dxt5a.h:72,
+ * crnlib/crn_dxt5a.cpp:40-196) for many 16-texel blocks at once, on the
+ * device.  values: 16 bytes per block (16-byte aligned); quality 0..4
+ * (crn_dxt_quality); use_both = m_use_both_block_types.  Per block out:
+ * selectors48 (u64: 3-bit DXT5 selector of texel i at bit 3*i), errors (u64,
+ * results::m_error), meta (u32: first | second << 8 | block_type << 16 |
+ * reordered << 24). */
+int crn2_dxt5a_blocks_device(const void* values, uint32_t nblocks, int quality, int use_both, void* selectors48,
+                             void* errors, void* meta, void* stream);
+
+/* Plain DXT5A/DXN-channel encode of an RGBA8 image in device memory, block by
+ * block as dxt_image does for cAlphaDXT5 elements (crnlib/crn_dxt_image.cpp:
+ * 330-344, 1491-1523): 4x4 texels with edge clamp, channel `comp`.  Writes one
+ * 8-byte DXT5 alpha block per block at blocks + i * block_stride_u64 (u64
+ * units; 2 interleaves the X and Y blocks of DXN); errors optional (u64). */
+int crn2_encode_dxt5a_image_device(const void* rgba, uint32_t width, uint32_t height, uint32_t comp, int quality,
+                                   int use_both, void* blocks, uint32_t block_stride_u64, void* errors, void* stream);
+
+/* --------------------------------------------- DXT1 endpoint search */
+/* crnlib's dxt1_endpoint_optimizer::compute (crnlib/crn_dxt1.h:223,
+ * crnlib/crn_dxt1.cpp:2234-2246) for many 16-texel blocks at once, on the
+ * device, endpoint caching off.  rgba: 16 RGBA8 texels per block (64 bytes,
+ * 16-byte aligned); quality 3 (better) or 4 (uber); flags: bit0 perceptual,
+ * bit1 use_alpha_blocks (3-colour blocks), bit2 grayscale_sampling, bit3
+ * force_alpha_blocks.  out: 24 bytes per block = {u64 results::m_error,
+ * u32 m_low_color | m_high_color << 16, u32 selectors (2-bit DXT1 code of
+ * texel i at bit 2i), u32 m_alpha_block | m_reordered << 1 |
+ * m_alternate_rounding << 2 | m_enforce_selector << 3 | m_enforced_selector << 4,
+ * u32 0}. */
+int crn2_dxt1_blocks_device(const void* rgba, uint32_t nblocks, int quality, int flags, void* out, void* stream);
 
 /* ---------------------------------------------- writer and test textures */
 /* CRC16 of crn_decomp.h:2048-2069. */

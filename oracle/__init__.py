@@ -163,3 +163,62 @@ def receive_model(data):
     if not lib().orc_receive_model(_p(d), d.nbytes, _p(sizes), C.byref(n)):
         return None
     return sizes[: n.value]
+
+
+def dxt5a_blocks(values, quality=4, use_both=True):
+    """dxt5_endpoint_optimizer results for blocks of 16 values (uint8 [n,16]):
+    (selectors [n,16] raw DXT5 codes, errors [n] u64, first, second,
+    block_type, reordered)."""
+    v = np.ascontiguousarray(values, np.uint8).reshape(-1, 16)
+    n = v.shape[0]
+    sel = np.zeros((n, 16), np.uint8)
+    err = np.zeros(n, np.uint64)
+    ep = np.zeros(n, np.uint32)
+    L = lib()
+    L.orc_dxt5a_blocks.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    assert L.orc_dxt5a_blocks(_p(v), n, quality, int(use_both), _p(sel), _p(err), _p(ep))
+    return sel, err, ep & 0xFF, (ep >> 8) & 0xFF, (ep >> 16) & 0xFF, (ep >> 24) & 0xFF
+
+
+class Dxt1Result(C.Structure):
+    _fields_ = [("error", C.c_uint64), ("low", C.c_uint16), ("high", C.c_uint16), ("alpha_block", C.c_uint8),
+                ("reordered", C.c_uint8), ("alternate_rounding", C.c_uint8), ("enforce_selector", C.c_uint8),
+                ("enforced_selector", C.c_uint8)]
+
+
+DXT1_PERCEPTUAL, DXT1_ALPHA_BLOCKS, DXT1_GRAYSCALE, DXT1_FORCE_ALPHA = 1, 2, 4, 8
+
+
+def dxt1_blocks(rgba, quality=4, flags=DXT1_PERCEPTUAL | DXT1_ALPHA_BLOCKS):
+    """dxt1_endpoint_optimizer results for blocks of 16 RGBA8 texels
+    ([n,16,4] uint8): (selectors [n,16], error, low, high, alpha_block,
+    reordered, alternate_rounding) as numpy arrays."""
+    px = np.ascontiguousarray(rgba, np.uint8).reshape(-1, 16, 4)
+    n = px.shape[0]
+    sel = np.zeros((n, 16), np.uint8)
+    res = (Dxt1Result * n)()
+    L = lib()
+    L.orc_dxt1_blocks.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    assert L.orc_dxt1_blocks(_p(px), n, quality, flags, _p(sel), res)
+    f = lambda k, dt: np.array([getattr(r, k) for r in res], dt)
+    return (sel, f("error", np.uint64), f("low", np.uint16), f("high", np.uint16), f("alpha_block", np.uint8),
+            f("reordered", np.uint8), f("alternate_rounding", np.uint8))
+
+
+def dxt1_cluster(rgba, quality=4, flags=DXT1_PERCEPTUAL):
+    """One optimizer call over any number of texels ([n,4] uint8)."""
+    px = np.ascontiguousarray(rgba, np.uint8).reshape(-1, 4)
+    sel = np.zeros(px.shape[0], np.uint8)
+    r = Dxt1Result()
+    L = lib()
+    L.orc_dxt1_compute.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    assert L.orc_dxt1_compute(_p(px), px.shape[0], quality, flags, _p(sel), C.byref(r))
+    return sel, r
+
+
+def dxt1_omatch(which):
+    out = np.zeros(512, np.uint8)
+    L = lib()
+    L.orc_dxt1_omatch.argtypes = [C.c_int, C.c_void_p]
+    L.orc_dxt1_omatch(which, _p(out))
+    return out.reshape(256, 2)
