@@ -4,19 +4,24 @@
 // one prefix-coded stream (unpack_dxt1 and friends, crn_decomp.h:3944-4223):
 // which code decodes the next symbol depends on reference codes decoded
 // earlier, and the odd row of each row pair takes its references from the
-// even row above.  A level decodes in one pass from its first bit; work
-// splits across levels and textures, not inside a stream.
-//
-// Device pipeline for a batch of textures ("plan", built once, run many times):
-//   1. crnd_parse_kernel: one workgroup (one wave) per texture, one lane per
-//      level stream.  The texture's prefix-code lookup tables, canonical
-//      symbol orders and the above-row block buffers of all its levels
-//      (crn_decomp.h:3652-3660) are staged in LDS; the compressed bits are
-//      read in 16-byte chunks prefetched two chunks ahead.  Each lane writes
-//      one packed palette-index record per block of the even-padded grid.
-//   2. crnd_write_kernel: one thread per visible block gathers the palette
-//      entries named by its record and writes the 8- or 16-byte DXTn block;
-//      consecutive threads write consecutive blocks of a row (coalesced).
+// even row above, so where a block starts is only known by following the
+// stream from its first bit.  The pipeline keeps that serial part as short as
+// it can be and runs everything else in parallel:
+//   1. crnd_trans_kernel (all bit positions in parallel): for every bit
+//      position, the bits an even-row block pair or an odd-row block would
+//      take if it started there.
+//   1b. crnd_compose_kernel (all bit positions): the same for two pairs / two
+//      blocks (four variants by the two blocks' reference flags).
+//   2. crnd_walk_kernel (one wave per level): follows the stream through the
+//      two-unit table, one lookup per two pairs or two odd-row blocks, in
+//      scalar registers with the nearby entries in vector registers; records
+//      the start bit of every group of 8 pairs / 8 odd-row blocks.
+//   3. crnd_sym_kernel (one thread per group; even rows, then odd rows):
+//      endpoint deltas, selectors and reference codes from the start bits.
+//   4. crnd_index_kernel (one wave per level): running endpoint indices, row
+//      by row, with a wave prefix scan inside each row.
+//   5. crnd_write_kernel (one thread per visible block): palette gather and
+//      the 8- or 16-byte DXTn block write, coalesced.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,7 +39,6 @@ using namespace crn2;
 namespace {
 
 enum : uint32_t { M_REF = 0, M_EP0 = 1, M_SEL0 = 2, M_EP1 = 3, M_SEL1 = 4, M_COUNT = 5 };
-constexpr uint32_t kNoLds = 0xFFFFFFFFu;
 
 // kind 0 = DXT1 (colour), 1 = DXT5 family (alpha + colour), 2 = DXN (two
 // alpha channels), 3 = DXT5A (one alpha channel)
@@ -56,26 +60,48 @@ static bool kind_uses(int kind, uint32_t m)
     if (m == M_EP0 || m == M_SEL0) return kind == 0 || kind == 1;
     return kind != 0;
 }
-
-struct ParseJob {
-    const uint8_t* src;   // level bytes (device); readable up to the next 16-byte boundary past src + src_size + 32
-    uint32_t src_size;
-    uint32_t gw, gh, faces;
-    uint32_t bb_ofs;      // entries into the group's LDS block-buffer region
-    void* rec;            // faces * gh * gw records (uint32 for kinds 0/3, uint64 for kinds 1/2)
-};
-
-struct ParseGroup {
-    const DevModel* models;      // M_COUNT models
-    const uint16_t* sorted;      // M_COUNT x kMaxSyms canonical symbol orders (global copy)
+// ------------------------------------------------------------ shared types
+// Per-texture constants of a launch.
+struct TexInfo {
+    const DevModel* models;      // M_COUNT models (device)
+    const uint16_t* sorted;      // M_COUNT x kMaxSyms canonical symbol orders
     int32_t* status;             // set to CRN2_ECORRUPT on an out-of-range palette index
     uint32_t kind, n_ce, n_cs, n_ae, n_as;
-    uint32_t first_job, njobs;
-    uint32_t lut_ofs[M_COUNT];   // LDS byte offsets of the tables (kNoLds when unused)
-    uint32_t sorted_ofs[M_COUNT];// LDS byte offsets of canonical orders (kNoLds: read the global copy)
-    uint32_t sorted_n[M_COUNT];
-    uint32_t bb_ofs;             // LDS byte offset of the block buffers
 };
+
+// One level stream (all faces of one level of one texture).
+struct LevelJob {
+    const uint8_t* src;          // level bytes; the 4-byte word holding src[0] and every word up to
+                                 // src + src_size are readable
+    uint32_t src_size;
+    uint32_t tex;                // index into the TexInfo array
+    uint32_t gw, gh, faces;      // even-padded block grid
+    uint32_t tlen;               // transition entries: multiple of kTransPerWG, >= 8 * src_size + 1024
+    uint32_t wg_trans;           // first workgroup of the job in crnd_trans_kernel / crnd_compose_kernel
+    uint32_t wg_sym;             // first workgroup of the job in crnd_sym_kernel<false> (even rows)
+    uint32_t wg_odd;             // first workgroup of the job in crnd_sym_kernel<true> (odd rows)
+    uint32_t* T;                 // one-unit transitions, one entry per bit position
+    uint64_t* T2;                // two-unit transitions, one entry per bit position
+    uint32_t* posE;              // start bit of each group of kGroup pairs of the even rows
+    uint32_t* posO;              // start bit of each group of kGroup blocks of the odd rows
+    uint8_t* gvals;              // faces * gh/2 * gw/2 reference group symbols
+    uint64_t* pre;               // faces * gh * gw: decoded symbols of each block
+    void* rec;                   // faces * gh * gw palette-index records (uint32 kinds 0/3, uint64 kinds 1/2)
+};
+
+__device__ inline bool kind_uses_dev(uint32_t kind, uint32_t m)
+{
+    if (m == M_REF) return true;
+    if (m == M_EP0 || m == M_SEL0) return kind == 0 || kind == 1;
+    return kind != 0;
+}
+
+// Models of the endpoint-delta and selector symbols of one block, in stream
+// order (crn_decomp.h:3944-4223): colour then alpha for DXT5, two alpha
+// channels for DXN, one alpha channel for DXT5A.
+__device__ inline uint32_t d_model0(uint32_t kind) { return kind <= 1 ? M_EP0 : M_EP1; }
+__device__ inline uint32_t s_model0(uint32_t kind) { return kind <= 1 ? M_SEL0 : M_SEL1; }
+__device__ inline bool two_channels(uint32_t kind) { return kind == 1 || kind == 2; }
 
 struct LdsTables {
     uint32_t lut[kLutSize];
@@ -85,197 +111,674 @@ struct LdsTables {
     uint32_t pad;
 };
 
-// ---------------------------------------------------------------- bit reader
-// MSB-first reader over 16-byte chunks: `cur` is being consumed word by word,
-// `nxt` is already in flight.  Bytes at or past `end` read as zero
-// (crn_decomp.h:3196-3207); chunks at or past the end are not loaded.
-struct DevBits {
-    const uint4* cp;      // next chunk to load
-    const uint8_t* end;
-    const uint8_t* qaddr; // address of `cur`
-    uint4 cur, nxt;
-    int qi;               // next word of `cur`
-    uint64_t buf;         // valid bits MSB-aligned
-    int cnt;
-
-    __device__ inline uint4 load_chunk(const uint4* p) const
-    {
-        if (reinterpret_cast<const uint8_t*>(p) < end) return *p;
-        return make_uint4(0, 0, 0, 0);
+__device__ inline void stage_tables(LdsTables& t, const DevModel& dm, uint32_t tid, uint32_t nthreads)
+{
+    for (uint32_t i = tid; i < kLutSize; i += nthreads) t.lut[i] = dm.lut[i];
+    if (tid < 17) {
+        t.limit[tid] = dm.limit[tid];
+        t.base[tid] = dm.base[tid];
     }
+    if (tid == 0) t.nsyms = dm.num_syms;
+}
 
-    __device__ inline uint32_t word(int i) const
-    {
-        const uint32_t w = (i == 0) ? cur.x : (i == 1) ? cur.y : (i == 2) ? cur.z : cur.w;
-        const uint8_t* wa = qaddr + 4 * i;
-        uint32_t v = __builtin_bswap32(w);
-        if (wa + 4 > end) v = (wa >= end) ? 0u : (v & (0xFFFFFFFFu << (8 * (4 - (int)(end - wa)))));
-        return v;
+// Prefix decode of the 16 bits `top` (MSB first): the direct table covers
+// codes up to kLutBits bits, longer codes take the canonical walk.  A bit
+// pattern that matches no code gives symbol 0 and consumes nothing
+// (crn_decomp.h:3239-3244).
+__device__ inline uint32_t decode_top(const LdsTables& t, const uint16_t* sorted, uint32_t top, uint32_t& len)
+{
+    const uint32_t e = t.lut[top >> (16 - kLutBits)];
+    if (e) {
+        len = e >> 16;
+        return e & 0xFFFFu;
     }
+    const uint32_t k = top + 1;
+    uint32_t L = kLutBits + 1;
+    while (L <= 16 && k > t.limit[L]) L++;
+    len = 0;
+    if (L > 16) return 0;
+    const uint32_t idx = (uint32_t)(t.base[L] + (int32_t)(top >> (16 - L)));
+    if (idx >= t.nsyms) return 0;
+    len = L;
+    return sorted[idx];
+}
 
-    __device__ inline void refill()
-    {
-        if (qi == 4) {
-            cur = nxt;
-            qaddr += 16;
-            nxt = load_chunk(cp);
-            cp++;
-            qi = 0;
-        }
-        buf |= (uint64_t)word(qi) << (32 - cnt);
-        cnt += 32;
-        qi++;
-    }
-
-    __device__ inline void start(const uint8_t* p, uint32_t size)
-    {
-        const uintptr_t a = reinterpret_cast<uintptr_t>(p);
-        const uint4* c0 = reinterpret_cast<const uint4*>(a & ~(uintptr_t)15);
-        end = p + size;
-        qaddr = reinterpret_cast<const uint8_t*>(c0);
-        cur = load_chunk(c0);
-        nxt = load_chunk(c0 + 1);
-        cp = c0 + 2;
-        qi = (int)((a & 15) >> 2);
-        buf = 0;
-        cnt = 0;
-        refill();
-        const int skip = (int)(a & 3) * 8;
-        buf <<= skip;
-        cnt -= skip;
-        if (cnt <= 32) refill();
-    }
-
-    __device__ inline uint32_t decode(const LdsTables& t, const uint16_t* sorted)
-    {
-        if (cnt <= 32) refill();
-        const uint32_t top = (uint32_t)(buf >> 48);
-        const uint32_t e = t.lut[top >> (16 - kLutBits)];
-        uint32_t sym, len;
-        if (e) {
-            sym = e & 0xFFFFu;
-            len = e >> 16;
-        } else {
-            // code longer than the table or no code at all: canonical walk
-            const uint32_t k = top + 1;
-            len = kLutBits + 1;
-            while (len <= 16 && k > t.limit[len]) len++;
-            if (len > 16) return 0;   // no code: symbol 0, nothing consumed (crn_decomp.h:3239-3244)
-            const uint32_t idx = (uint32_t)(t.base[len] + (int32_t)(top >> (16 - len)));
-            if (idx >= t.nsyms) return 0;
-            sym = sorted[idx];
-        }
-        buf <<= len;
-        cnt -= (int)len;
-        return sym;
-    }
+// Code length only (the symbol value is not needed to find the next symbol).
+struct LenLut {
+    uint8_t len[kLutSize];
+    uint32_t limit[17];
+    int32_t base[17];
+    uint32_t nsyms;
+    uint32_t pad;
 };
 
-__global__ void __launch_bounds__(64) crnd_parse_kernel(const ParseGroup* __restrict__ groups,
-                                                        const ParseJob* __restrict__ jobs)
+__device__ inline uint32_t code_len(const LenLut& t, uint32_t top)
 {
-    extern __shared__ __align__(16) uint8_t lds[];
-    const ParseGroup& g = groups[blockIdx.x];
-    const int kind = (int)g.kind;
+    const uint32_t l = t.len[top >> (16 - kLutBits)];
+    if (l) return l;
+    const uint32_t k = top + 1;
+    uint32_t L = kLutBits + 1;
+    while (L <= 16 && k > t.limit[L]) L++;
+    if (L > 16) return 0;
+    const uint32_t idx = (uint32_t)(t.base[L] + (int32_t)(top >> (16 - L)));
+    return idx < t.nsyms ? L : 0u;
+}
 
-    // ---- stage tables, canonical orders; zero the block buffers
-    for (uint32_t m = 0; m < M_COUNT; m++) {
-        if (g.lut_ofs[m] == kNoLds) continue;
-        LdsTables* t = reinterpret_cast<LdsTables*>(lds + g.lut_ofs[m]);
-        const DevModel& dm = g.models[m];
-        for (uint32_t i = threadIdx.x; i < kLutSize; i += blockDim.x) t->lut[i] = dm.lut[i];
-        if (threadIdx.x < 17) {
-            t->limit[threadIdx.x] = dm.limit[threadIdx.x];
-            t->base[threadIdx.x] = dm.base[threadIdx.x];
-        }
-        if (threadIdx.x == 0) t->nsyms = dm.num_syms;
-        if (g.sorted_ofs[m] != kNoLds) {
-            uint16_t* s = reinterpret_cast<uint16_t*>(lds + g.sorted_ofs[m]);
-            const uint16_t* src = g.sorted + m * kMaxSyms;
-            for (uint32_t i = threadIdx.x; i < g.sorted_n[m]; i += blockDim.x) s[i] = src[i];
-        }
+// ------------------------------------------------- stage 1: transition table
+// For every bit position q of a level stream, one 32-bit entry:
+//   bits  0- 7  bits taken by an even-row block pair starting at q: reference
+//               group symbol, then per block the endpoint deltas (when its
+//               reference code is 0) and the selectors
+//   bits  8-15  the reference group symbol at q
+//   bits 16-23  bits taken by an odd-row block at q with a non-zero reference
+//               code (selectors only)
+//   bits 24-31  bits taken by an odd-row block at q with reference code 0
+//               (endpoint deltas, then selectors)
+// A block takes at most 64 bits and a pair at most 144, so each field fits.
+// Positions at or past the end read zero bits, so every entry from the end
+// of the stream on is the same; the table is computed 1024 entries beyond.
+constexpr uint32_t kTransPerWG = 2048;
+constexpr uint32_t kTransWords = kTransPerWG / 32 + 8;
+
+__device__ inline uint32_t be_bytes_word(const uint8_t* src, uint32_t size, uint32_t b)
+{
+    uint32_t v = 0;
+    for (uint32_t k = 0; k < 4; k++) {
+        v <<= 8;
+        if (b + k < size) v |= src[b + k];
     }
-    uint32_t bb_total = 0;
-    for (uint32_t k = 0; k < g.njobs; k++) bb_total = max(bb_total, jobs[g.first_job + k].bb_ofs + jobs[g.first_job + k].gw);
-    uint32_t* bbase = reinterpret_cast<uint32_t*>(lds + g.bb_ofs);
-    for (uint32_t i = threadIdx.x; i < bb_total; i += blockDim.x) bbase[i] = 0;
+    return v;
+}
+
+__device__ inline uint32_t peek16(const uint32_t* w, uint32_t r)
+{
+    const uint32_t i = r >> 5;
+    const uint64_t v = ((uint64_t)w[i] << 32) | w[i + 1];
+    return (uint32_t)((v << (r & 31)) >> 48);
+}
+
+__device__ inline uint32_t find_job(const LevelJob* jobs, uint32_t njobs, uint32_t wg, bool sym)
+{
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((sym ? jobs[mid].wg_sym : jobs[mid].wg_trans) <= wg) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) crnd_trans_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
+                                                         const TexInfo* __restrict__ texs)
+{
+    __shared__ LdsTables s_ref;
+    __shared__ LenLut s_len[4];   // M_EP0, M_SEL0, M_EP1, M_SEL1
+    __shared__ uint32_t s_words[kTransWords];
+    const uint32_t tid = threadIdx.x;
+    const LevelJob& j = jobs[find_job(jobs, njobs, blockIdx.x, false)];
+    const TexInfo& tx = texs[j.tex];
+    const uint32_t kind = tx.kind;
+    const uint32_t P0 = (blockIdx.x - j.wg_trans) * kTransPerWG;
+
+    stage_tables(s_ref, tx.models[M_REF], tid, 256);
+    for (uint32_t m = 1; m < M_COUNT; m++) {
+        if (!kind_uses_dev(kind, m)) continue;
+        const DevModel& dm = tx.models[m];
+        LenLut& L = s_len[m - 1];
+        for (uint32_t i = tid; i < kLutSize; i += 256) L.len[i] = (uint8_t)(dm.lut[i] >> 16);
+        if (tid < 17) {
+            L.limit[tid] = dm.limit[tid];
+            L.base[tid] = dm.base[tid];
+        }
+        if (tid == 0) L.nsyms = dm.num_syms;
+    }
+    for (uint32_t i = tid; i < kTransWords; i += 256) s_words[i] = be_bytes_word(j.src, j.src_size, P0 / 8 + 4 * i);
     __syncthreads();
-    if (threadIdx.x >= g.njobs) return;
-    const ParseJob j = jobs[g.first_job + threadIdx.x];
 
-    const uint32_t slot_ep0 = (kind == 0 || kind == 1) ? M_EP0 : M_EP1;
-    const uint32_t slot_sel0 = (kind == 0 || kind == 1) ? M_SEL0 : M_SEL1;
-    const LdsTables& T_ref = *reinterpret_cast<const LdsTables*>(lds + g.lut_ofs[M_REF]);
-    const LdsTables& T_ep0 = *reinterpret_cast<const LdsTables*>(lds + g.lut_ofs[slot_ep0]);
-    const LdsTables& T_sel0 = *reinterpret_cast<const LdsTables*>(lds + g.lut_ofs[slot_sel0]);
-    const LdsTables& T_ep1 = *reinterpret_cast<const LdsTables*>(lds + g.lut_ofs[kind == 0 ? M_REF : M_EP1]);
-    const LdsTables& T_sel1 = *reinterpret_cast<const LdsTables*>(lds + g.lut_ofs[kind == 0 ? M_REF : M_SEL1]);
-    auto sorted_of = [&](uint32_t m) -> const uint16_t* {
-        return g.sorted_ofs[m] != kNoLds ? reinterpret_cast<const uint16_t*>(lds + g.sorted_ofs[m])
-                                         : g.sorted + m * kMaxSyms;
+    const LenLut& D0 = s_len[d_model0(kind) - 1];
+    const LenLut& S0 = s_len[s_model0(kind) - 1];
+    const LenLut& D1 = s_len[M_EP1 - 1];
+    const LenLut& S1 = s_len[M_SEL1 - 1];
+    const bool two = two_channels(kind);
+    const uint16_t* ref_sorted = tx.sorted + M_REF * kMaxSyms;
+    auto dlen = [&](uint32_t r) -> uint32_t {
+        uint32_t l = code_len(D0, peek16(s_words, r));
+        if (two) l += code_len(D1, peek16(s_words, r + l));
+        return l;
     };
-    const uint16_t* S_ref = sorted_of(M_REF);
-    const uint16_t* S_ep0 = sorted_of(slot_ep0);
-    const uint16_t* S_sel0 = sorted_of(slot_sel0);
-    const uint16_t* S_ep1 = sorted_of(kind == 0 ? M_REF : M_EP1);
-    const uint16_t* S_sel1 = sorted_of(kind == 0 ? M_REF : M_SEL1);
+    auto slen = [&](uint32_t r) -> uint32_t {
+        uint32_t l = code_len(S0, peek16(s_words, r));
+        if (two) l += code_len(S1, peek16(s_words, r + l));
+        return l;
+    };
+    for (uint32_t k = 0; k < kTransPerWG / 256; k++) {
+        const uint32_t r = tid + 256 * k;
+        uint32_t gl;
+        const uint32_t G = decode_top(s_ref, ref_sorted, peek16(s_words, r), gl) & 0xFFu;
+        uint32_t e = r + gl;
+        if ((G & 3u) == 0) e += dlen(e);
+        e += slen(e);
+        if (((G >> 4) & 3u) == 0) e += dlen(e);
+        e += slen(e);
+        const uint32_t o0 = slen(r);
+        const uint32_t d = dlen(r);
+        const uint32_t o1 = d + slen(r + d);
+        j.T[P0 + r] = (e - r) | (G << 8) | (o0 << 16) | (o1 << 24);
+    }
+}
 
-    DevBits bits;
-    bits.start(j.src, j.src_size);
-    const uint32_t gw = j.gw, gh = j.gh;
-    uint32_t* bb = bbase + j.bb_ofs;
-    // i0: colour endpoint (kinds 0, 1) or alpha0 endpoint (kinds 2, 3);
-    // i1: alpha endpoint (kind 1) or alpha1 endpoint (kind 2)
-    uint32_t i0 = 0, i1 = 0;
-    uint32_t group = 0;
-    bool bad = false;
-    uint64_t blk = 0;
-    const uint32_t n0 = (kind == 0 || kind == 1) ? g.n_ce : g.n_ae;
-    const uint32_t n1 = (kind == 1 || kind == 2) ? g.n_ae : 1u;
-    const uint32_t ns0 = (kind == 0 || kind == 1) ? g.n_cs : g.n_as;
-    const uint32_t ns1 = g.n_as;
-    uint32_t* rec32 = static_cast<uint32_t*>(j.rec);
-    uint64_t* rec64 = static_cast<uint64_t*>(j.rec);
-    const bool two = (kind == 1 || kind == 2);
-    for (uint32_t f = 0; f < j.faces; f++) {
-        for (uint32_t y = 0; y < gh; y++) {
-            for (uint32_t x = 0; x < gw; x++, blk++) {
-                const uint32_t cur = bb[x];
-                uint32_t ref, below;
-                if (!(y & 1)) {
-                    if (!(x & 1)) group = bits.decode(T_ref, S_ref) & 0xFFu;
-                    ref = group & 3u;
-                    below = (group >> 2) & 3u;
-                    group >>= 4;
-                } else {
-                    ref = cur & 3u;
-                    below = ref;
+// ------------------------------------------ stage 1b: two-unit transitions
+// For every bit position q, one 64-bit entry built from the one-unit table:
+//   low word, byte f (f = 0..3): bits taken by two odd-row blocks starting at
+//       q when bit 0 of f says the first carries endpoint deltas and bit 1
+//       the second
+//   high word, bits 0-8: bits taken by two even-row pairs starting at q
+//              bits 9-12: the odd-row flags of those pairs (two per pair:
+//                         block below the pair's first / second block has
+//                         reference code 0)
+//              bits 13-20: bits taken by one even-row pair (row ends)
+__device__ inline uint32_t odd_flags(uint32_t g) { return (uint32_t)((g & 0x0Cu) == 0) | ((uint32_t)((g & 0xC0u) == 0) << 1); }
+
+__global__ void __launch_bounds__(256) crnd_compose_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs)
+{
+    const LevelJob& j = jobs[find_job(jobs, njobs, blockIdx.x, false)];
+    const uint32_t P0 = (blockIdx.x - j.wg_trans) * kTransPerWG;
+    const uint32_t* T = j.T;
+    const uint32_t last = j.tlen - 1;
+    for (uint32_t k = 0; k < kTransPerWG / 256; k++) {
+        const uint32_t q = P0 + threadIdx.x + 256 * k;
+        const uint32_t t0 = T[q];
+        const uint32_t o0 = (t0 >> 16) & 0xFFu, o1 = t0 >> 24;
+        const uint32_t ta = T[min(q + o0, last)], tb = T[min(q + o1, last)];
+        const uint32_t lo = (o0 + ((ta >> 16) & 0xFFu)) | ((o1 + ((tb >> 16) & 0xFFu)) << 8) |
+                            ((o0 + (ta >> 24)) << 16) | ((o1 + (tb >> 24)) << 24);
+        const uint32_t e1 = t0 & 0xFFu;
+        const uint32_t te = T[min(q + e1, last)];
+        const uint32_t e2 = e1 + (te & 0xFFu);
+        const uint32_t fl = odd_flags((t0 >> 8) & 0xFFu) | (odd_flags((te >> 8) & 0xFFu) << 2);
+        const uint32_t hi = e2 | (fl << 9) | (e1 << 13);
+        j.T2[q] = (uint64_t)lo | ((uint64_t)hi << 32);
+    }
+}
+
+// --------------------------------------------------------- stage 2: walker
+// One wave per level stream follows the stream from bit 0 through the
+// two-unit table: an even row takes one step per two block pairs and collects
+// the odd-row flags of its pairs; the odd row then takes one step per two
+// blocks, picking the entry byte its two flags select.  The start bit of every
+// group of kGroup pairs (even rows) or kGroup blocks (odd rows) is recorded.
+//
+// A lone wave issues about one instruction every four cycles, so the loop is
+// kept short: all walker state is wave-uniform (scalar registers), the 64 table
+// entries from `base` sit one per lane in two vector registers and are read
+// with v_readlane at offset o = bit - base, the next 64 are prefetched from
+// LDS, and LDS is refilled from the table in 8 KiB segments loaded one segment
+// ahead.
+constexpr uint32_t kSeg = 1024;               // table entries per LDS segment
+constexpr uint32_t kGroup = 8;                // pairs / blocks per recorded start bit
+constexpr uint32_t kMaxEGroups = 1024;        // even-row groups per row: grids up to 16384 blocks wide
+
+// Global-address-space views: loads and stores through them are global_*
+// instructions, which unlike flat_* ones do not count against the LDS wait
+// counter the walker waits on.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const u32x4 gconst_u32x4;
+typedef __attribute__((address_space(1))) uint32_t guint32;
+
+struct SegRegs {
+    uint4 r[8];
+};
+
+__device__ inline void seg_load(const uint64_t* T2, uint32_t tsegs, uint64_t tlast, uint32_t s, uint32_t lane,
+                                SegRegs& v)
+{
+    if (s < tsegs) {
+        const gconst_u32x4* p = (const gconst_u32x4*)(T2 + (size_t)s * kSeg);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const u32x4 t = p[64 * i + lane];
+            v.r[i] = make_uint4(t.x, t.y, t.z, t.w);
+        }
+    } else {
+        const uint32_t a = (uint32_t)tlast, b = (uint32_t)(tlast >> 32);
+#pragma unroll
+        for (int i = 0; i < 8; i++) v.r[i] = make_uint4(a, b, a, b);
+    }
+}
+
+__device__ inline void seg_store(uint64_t* ring, uint32_t s, uint32_t lane, const SegRegs& v)
+{
+    uint4* p = reinterpret_cast<uint4*>(ring + (s & 3u) * kSeg);
+#pragma unroll
+    for (int i = 0; i < 8; i++) p[64 * i + lane] = v.r[i];
+}
+
+__global__ void __launch_bounds__(64) crnd_walk_kernel(const LevelJob* __restrict__ jobs)
+{
+    __shared__ __align__(16) uint64_t ring[4 * kSeg];
+    __shared__ uint16_t eflags[kMaxEGroups];
+    const LevelJob& j = jobs[blockIdx.x];
+    const uint32_t lane = threadIdx.x;
+    const uint64_t* T2 = j.T2;
+    const uint32_t tsegs = j.tlen / kSeg;
+    const uint64_t tlast = T2[j.tlen - 1];
+    SegRegs pf;
+    seg_load(T2, tsegs, tlast, 0, lane, pf);
+    seg_store(ring, 0, lane, pf);
+    seg_load(T2, tsegs, tlast, 1, lane, pf);
+    seg_store(ring, 1, lane, pf);
+    seg_load(T2, tsegs, tlast, 2, lane, pf);
+    __syncthreads();
+
+    uint32_t base = 0;                                 // table position of lane 0 of the window
+    uint64_t w = ring[lane];
+    uint32_t vlo = (uint32_t)w, vhi = (uint32_t)(w >> 32);
+    w = ring[64 + lane];
+    uint32_t nlo = (uint32_t)w, nhi = (uint32_t)(w >> 32);
+    uint32_t o = 0;                                    // current bit - base
+    // Steps move at most 288 bits; one window advance covers moves that end
+    // in the next window, longer moves reload both windows from LDS.
+#define CRND_ADVANCE()                                                         \
+    do {                                                                       \
+        if (o >= 64) {                                                         \
+            base += 64;                                                        \
+            o -= 64;                                                           \
+            vlo = nlo;                                                         \
+            vhi = nhi;                                                         \
+            if (o >= 64) {                                                     \
+                const uint32_t k_ = o >> 6;                                    \
+                for (uint32_t i_ = 0; i_ < k_; i_++) {                         \
+                    base += 64;                                                \
+                    const uint32_t nb_ = base + 64;                            \
+                    if ((nb_ & (kSeg - 1)) == 0) {                             \
+                        const uint32_t s_ = nb_ / kSeg;                        \
+                        seg_store(ring, s_ + 1, lane, pf);                     \
+                        seg_load(T2, tsegs, tlast, s_ + 2, lane, pf);          \
+                    }                                                          \
+                }                                                              \
+                o -= 64 * k_;                                                  \
+                const uint64_t c_ = ring[(base & (4 * kSeg - 1)) + lane];      \
+                vlo = (uint32_t)c_;                                            \
+                vhi = (uint32_t)(c_ >> 32);                                    \
+            }                                                                  \
+            const uint32_t nb_ = base + 64;                                    \
+            if ((nb_ & (kSeg - 1)) == 0) {                                     \
+                const uint32_t s_ = nb_ / kSeg;                                \
+                seg_store(ring, s_ + 1, lane, pf);                             \
+                seg_load(T2, tsegs, tlast, s_ + 2, lane, pf);                  \
+            }                                                                  \
+            const uint64_t w_ = ring[(nb_ & (4 * kSeg - 1)) + lane];           \
+            nlo = (uint32_t)w_;                                                \
+            nhi = (uint32_t)(w_ >> 32);                                        \
+        }                                                                      \
+    } while (0)
+#define CRND_E2()                                                                          \
+    do {                                                                                   \
+        const uint32_t e_ = (uint32_t)__builtin_amdgcn_readlane((int)vhi, (int)o);         \
+        fw |= ((e_ >> 9) & 15u) << fsh;                                                    \
+        fsh += 4;                                                                          \
+        o += e_ & 511u;                                                                    \
+        CRND_ADVANCE();                                                                    \
+    } while (0)
+#define CRND_O2()                                                                          \
+    do {                                                                                   \
+        const uint32_t e_ = (uint32_t)__builtin_amdgcn_readlane((int)vlo, (int)o);         \
+        o += (e_ >> ((fo & 3u) << 3)) & 255u;                                              \
+        fo >>= 2;                                                                          \
+        CRND_ADVANCE();                                                                    \
+    } while (0)
+
+    const uint32_t gw = j.gw, gh = j.gh, faces = j.faces;
+    const uint32_t npairs = gw >> 1;
+    uint32_t vE = 0, kE = 0, nE = 0;
+    uint32_t vO = 0, kO = 0, nO = 0;
+    guint32* posE = (guint32*)j.posE;
+    guint32* posO = (guint32*)j.posO;
+    for (uint32_t f = 0; f < faces; f++) {
+        for (uint32_t yp = 0; yp < gh; yp += 2) {
+            // even row: groups of kGroup pairs, two pairs a step
+            for (uint32_t u0 = 0, g = 0; u0 < npairs; u0 += kGroup, g++) {
+                vE = (lane == kE) ? base + o : vE;
+                if (++kE == 64) {
+                    posE[nE + lane] = vE;
+                    nE += 64;
+                    kE = 0;
                 }
-                if (ref == 0) {
-                    i0 += bits.decode(T_ep0, S_ep0);
-                    if (i0 >= n0) i0 -= n0;
-                    if (two) {
-                        i1 += bits.decode(T_ep1, S_ep1);
-                        if (i1 >= n1) i1 -= n1;
+                uint32_t fw = 0, fsh = 0;
+                const uint32_t n = min(kGroup, npairs - u0);
+                if (n == kGroup) {
+                    CRND_E2();
+                    CRND_E2();
+                    CRND_E2();
+                    CRND_E2();
+                } else {
+                    for (uint32_t k = 0; k + 2 <= n; k += 2) CRND_E2();
+                    if (n & 1u) {
+                        const uint32_t e_ = (uint32_t)__builtin_amdgcn_readlane((int)vhi, (int)o);
+                        fw |= ((e_ >> 9) & 3u) << fsh;
+                        o += (e_ >> 13) & 255u;
+                        CRND_ADVANCE();
                     }
-                } else if (ref >= 2) {
-                    i0 = (cur >> 2) & 0x3FFFu;
-                    i1 = (cur >> 16) & 0x3FFFu;
                 }
-                bb[x] = below | (i0 << 2) | (i1 << 16);
-                const uint32_t s0 = bits.decode(T_sel0, S_sel0);
-                bad |= (i0 >= n0) | (s0 >= ns0);
-                if (!two) {
-                    rec32[blk] = i0 | (s0 << 16);
+                if (lane == 0) eflags[g] = (uint16_t)fw;
+            }
+            // odd row: groups of kGroup blocks, two blocks a step
+            uint32_t fv = eflags[0];          // flags of the next even-row group (vector register, prefetched)
+            uint32_t fcur = 0;
+            for (uint32_t x0 = 0, og = 0; x0 < gw; x0 += kGroup, og++) {
+                vO = (lane == kO) ? base + o : vO;
+                if (++kO == 64) {
+                    posO[nO + lane] = vO;
+                    nO += 64;
+                    kO = 0;
+                }
+                if (!(og & 1u)) {
+                    fcur = (uint32_t)__builtin_amdgcn_readfirstlane((int)fv);
+                    fv = eflags[min((og >> 1) + 1, kMaxEGroups - 1)];
+                }
+                uint32_t fo = (og & 1u) ? (fcur >> 8) : (fcur & 0xFFu);
+                if (x0 + kGroup <= gw) {
+                    CRND_O2();
+                    CRND_O2();
+                    CRND_O2();
+                    CRND_O2();
                 } else {
-                    const uint32_t s1 = bits.decode(T_sel1, S_sel1);
-                    bad |= (i1 >= n1) | (s1 >= ns1);
-                    rec64[blk] = (uint64_t)(i0 | (s0 << 16)) | ((uint64_t)(i1 | (s1 << 16)) << 32);
+                    for (uint32_t x = x0; x < gw; x += 2) CRND_O2();
                 }
             }
         }
     }
-    if (bad) atomicExch(g.status, CRN2_ECORRUPT);
+#undef CRND_O2
+#undef CRND_E2
+#undef CRND_ADVANCE
+    if (lane < kE) posE[nE + lane] = vE;
+    if (lane < kO) posO[nO + lane] = vO;
+}
+
+// ------------------------------------------------------ stage 3: symbols
+// One thread per recorded group decodes, from the group's start bit, the
+// endpoint deltas and selector symbols of its pairs (even rows: reference group
+// symbol first, kept for the odd row) or blocks (odd rows: reference codes from
+// the even row's group symbols).  Record per block (uint64): bits 0-1 reference
+// code, 2-15 first endpoint delta, 16-29 second endpoint delta, 30-43 first
+// selector, 44-57 second selector.
+
+// MSB-first reader starting at any bit; aligned 32-bit loads, bytes at or past
+// `end` read as zero (crn_decomp.h:3196-3207).
+struct WordBits {
+    const uint32_t* wp;
+    const uint8_t* end;
+    uint64_t buf;
+    uint32_t cnt;
+
+    __device__ inline uint32_t load_word()
+    {
+        const uint8_t* a = reinterpret_cast<const uint8_t*>(wp);
+        uint32_t v = 0;
+        if (a < end) {
+            v = __builtin_bswap32(*wp);
+            const uintptr_t left = (uintptr_t)(end - a);
+            if (left < 4) v &= 0xFFFFFFFFu << (8 * (4 - (uint32_t)left));
+        }
+        wp++;
+        return v;
+    }
+
+    __device__ inline void start(const uint8_t* src, uint32_t size, uint32_t bit)
+    {
+        end = src + size;
+        const uintptr_t ab = reinterpret_cast<uintptr_t>(src) + (bit >> 3);
+        wp = reinterpret_cast<const uint32_t*>(ab & ~(uintptr_t)3);
+        const uint32_t skip = (uint32_t)(ab & 3) * 8 + (bit & 7);
+        const uint32_t w0 = load_word();
+        const uint32_t w1 = load_word();
+        buf = (((uint64_t)w0 << 32) | w1) << skip;
+        cnt = 64 - skip;
+    }
+
+    __device__ inline uint32_t decode(const LdsTables& t, const uint16_t* sorted)
+    {
+        if (cnt <= 32) {
+            buf |= (uint64_t)load_word() << (32 - cnt);
+            cnt += 32;
+        }
+        uint32_t len;
+        const uint32_t sym = decode_top(t, sorted, (uint32_t)(buf >> 48), len);
+        buf <<= len;
+        cnt -= len;
+        return sym;
+    }
+};
+
+constexpr uint32_t kSymGroupsPerWG = 256;
+
+struct SymCtx {
+    const LdsTables* t;
+    const uint16_t* so;
+    uint32_t dm0, sm0;
+    bool two;
+
+    __device__ inline uint64_t block(WordBits& b, uint32_t ref) const
+    {
+        uint32_t d0 = 0, d1 = 0, s1 = 0;
+        if (ref == 0) {
+            d0 = b.decode(t[dm0], so + dm0 * kMaxSyms);
+            if (two) d1 = b.decode(t[M_EP1], so + M_EP1 * kMaxSyms);
+        }
+        const uint32_t s0 = b.decode(t[sm0], so + sm0 * kMaxSyms);
+        if (two) s1 = b.decode(t[M_SEL1], so + M_SEL1 * kMaxSyms);
+        return (uint64_t)ref | ((uint64_t)d0 << 2) | ((uint64_t)d1 << 16) | ((uint64_t)s0 << 30) |
+               ((uint64_t)s1 << 44);
+    }
+};
+
+template <bool ODD>
+__global__ void __launch_bounds__(256) crnd_sym_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
+                                                       const TexInfo* __restrict__ texs)
+{
+    __shared__ LdsTables s_t[M_COUNT];
+    const uint32_t tid = threadIdx.x;
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((ODD ? jobs[mid].wg_odd : jobs[mid].wg_sym) <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const LevelJob& j = jobs[lo];
+    const TexInfo& tx = texs[j.tex];
+    const uint32_t kind = tx.kind;
+    for (uint32_t m = 0; m < M_COUNT; m++)
+        if (kind_uses_dev(kind, m)) stage_tables(s_t[m], tx.models[m], tid, 256);
+    __syncthreads();
+
+    SymCtx sc{s_t, tx.sorted, d_model0(kind), s_model0(kind), two_channels(kind)};
+    const uint32_t gw = j.gw, np = gw >> 1;
+    const uint32_t rowpairs = j.faces * (j.gh >> 1);
+    const uint32_t gpr = ODD ? (gw + kGroup - 1) / kGroup : (np + kGroup - 1) / kGroup;   // groups per row
+    const uint32_t it = (blockIdx.x - (ODD ? j.wg_odd : j.wg_sym)) * kSymGroupsPerWG + tid;
+    if (it >= rowpairs * gpr) return;
+    const uint32_t rp = it / gpr, grp = it - rp * gpr;
+    WordBits b;
+    if (!ODD) {
+        b.start(j.src, j.src_size, j.posE[it]);
+        const uint32_t u0 = grp * kGroup, u1 = min(np, u0 + kGroup);
+        uint64_t* pre = j.pre + (uint64_t)(2 * rp) * gw;
+        uint8_t* gv = j.gvals + (uint64_t)rp * np;
+        for (uint32_t u = u0; u < u1; u++) {
+            const uint32_t G = b.decode(s_t[M_REF], tx.sorted + M_REF * kMaxSyms) & 0xFFu;
+            gv[u] = (uint8_t)G;
+            pre[2 * u] = sc.block(b, G & 3u);
+            pre[2 * u + 1] = sc.block(b, (G >> 4) & 3u);
+        }
+    } else {
+        b.start(j.src, j.src_size, j.posO[it]);
+        const uint32_t x0 = grp * kGroup, x1 = min(gw, x0 + kGroup);
+        uint64_t* pre = j.pre + (uint64_t)(2 * rp + 1) * gw;
+        const uint8_t* gv = j.gvals + (uint64_t)rp * np;
+        for (uint32_t x = x0; x < x1; x++) {
+            const uint32_t G = gv[x >> 1];
+            pre[x] = sc.block(b, (G >> (2 + 4 * (x & 1u))) & 3u);
+        }
+    }
+}
+
+// ------------------------------------------------- stage 4: palette indices
+// Running endpoint indices (crn_decomp.h:3944-4223): reference code 0 adds the
+// decoded delta to the running index (one conditional wrap), 1 repeats it,
+// 2 and 3 take the index of the block above; the running index carries
+// across rows and faces.  One wave per level sweeps the rows in order; within
+// a row each lane owns a run of columns, composes its blocks into one map
+// (add a constant, or set a constant), and a wave prefix scan of the maps
+// gives every lane its incoming index.  A lane's records of the next row are
+// loaded while the current row is processed.
+constexpr uint32_t kIdxRun = 16;              // columns per lane held in registers (rows up to 1024 blocks)
+
+__device__ inline uint32_t add_mod(uint32_t a, uint32_t d, uint32_t n)
+{
+    uint32_t t = a + d;
+    if (t >= n) t -= n;
+    if (t >= n) t %= n;
+    return t;
+}
+
+struct IdxState {
+    uint32_t n0, n1, ns0, ns1;
+    bool two;
+    uint32_t v0, v1;          // running indices entering the row (wave-uniform)
+    bool bad;
+};
+
+// Map of one block: R = 1 sets (A0, A1); R = 0 adds (A0, A1).
+__device__ inline void compose_block(const IdxState& s, uint64_t q, uint32_t abv, uint32_t& R, uint32_t& A0,
+                                     uint32_t& A1)
+{
+    const uint32_t ref = (uint32_t)q & 3u;
+    if (ref == 0) {
+        A0 = add_mod(A0, (uint32_t)(q >> 2) & 0x3FFFu, s.n0);
+        A1 = add_mod(A1, (uint32_t)(q >> 16) & 0x3FFFu, s.n1);
+    } else if (ref >= 2) {
+        R = 1;
+        A0 = abv & 0xFFFFu;
+        A1 = abv >> 16;
+    }
+}
+
+__device__ inline void scan_maps(uint32_t lane, uint32_t n0, uint32_t n1, uint32_t& R, uint32_t& A0, uint32_t& A1)
+{
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t pR = (uint32_t)__shfl_up((int)R, off);
+        const uint32_t pA0 = (uint32_t)__shfl_up((int)A0, off);
+        const uint32_t pA1 = (uint32_t)__shfl_up((int)A1, off);
+        if (lane >= off && !R) {
+            R = pR;
+            A0 = add_mod(pA0, A0, n0);
+            A1 = add_mod(pA1, A1, n1);
+        }
+    }
+    uint32_t eR = (uint32_t)__shfl_up((int)R, 1);
+    uint32_t eA0 = (uint32_t)__shfl_up((int)A0, 1);
+    uint32_t eA1 = (uint32_t)__shfl_up((int)A1, 1);
+    if (lane == 0) eR = eA0 = eA1 = 0;
+    R = eR;
+    A0 = eA0;
+    A1 = eA1;
+}
+
+// Applies one block: updates the running pair (w0, w1), the above-row entry
+// and writes the record.
+__device__ inline void apply_block(IdxState& s, uint64_t q, uint32_t& abv, uint32_t& w0, uint32_t& w1,
+                                   void* rec, uint64_t o)
+{
+    const uint32_t ref = (uint32_t)q & 3u;
+    const uint32_t s0 = (uint32_t)(q >> 30) & 0x3FFFu, s1 = (uint32_t)(q >> 44) & 0x3FFFu;
+    if (ref == 0) {
+        w0 += (uint32_t)(q >> 2) & 0x3FFFu;
+        if (w0 >= s.n0) w0 -= s.n0;
+        s.bad |= w0 >= s.n0;
+        if (s.two) {
+            w1 += (uint32_t)(q >> 16) & 0x3FFFu;
+            if (w1 >= s.n1) w1 -= s.n1;
+            s.bad |= w1 >= s.n1;
+        }
+    } else if (ref >= 2) {
+        w0 = abv & 0xFFFFu;
+        w1 = abv >> 16;
+    }
+    abv = (w0 & 0xFFFFu) | (w1 << 16);
+    s.bad |= s0 >= s.ns0;
+    if (s.two) {
+        s.bad |= s1 >= s.ns1;
+        static_cast<uint64_t*>(rec)[o] =
+            (uint64_t)((w0 & 0xFFFFu) | (s0 << 16)) | ((uint64_t)((w1 & 0xFFFFu) | (s1 << 16)) << 32);
+    } else {
+        static_cast<uint32_t*>(rec)[o] = (w0 & 0xFFFFu) | (s0 << 16);
+    }
+}
+
+__global__ void __launch_bounds__(64) crnd_index_kernel(const LevelJob* __restrict__ jobs,
+                                                        const TexInfo* __restrict__ texs)
+{
+    extern __shared__ uint32_t above[];   // gw entries: index0 | index1 << 16 (rows wider than 64 * kIdxRun)
+    const LevelJob& j = jobs[blockIdx.x];
+    const TexInfo& tx = texs[j.tex];
+    const uint32_t kind = tx.kind;
+    IdxState s;
+    s.two = two_channels(kind);
+    s.n0 = kind <= 1 ? tx.n_ce : tx.n_ae;
+    s.n1 = s.two ? tx.n_ae : 1u;
+    s.ns0 = kind <= 1 ? tx.n_cs : tx.n_as;
+    s.ns1 = tx.n_as;
+    s.v0 = s.v1 = 0;
+    s.bad = false;
+    const uint32_t gw = j.gw;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t c = (gw + 63) / 64;
+    const uint32_t x0 = min(gw, lane * c), x1 = min(gw, x0 + c);
+    const uint32_t last_owner = (gw - 1) / c;
+    const uint32_t rows = j.faces * j.gh;
+    if (c <= kIdxRun) {
+        uint64_t q[kIdxRun], qn[kIdxRun];
+        uint32_t abv[kIdxRun];
+#pragma unroll
+        for (uint32_t i = 0; i < kIdxRun; i++) {
+            abv[i] = 0;
+            q[i] = (x0 + i < x1) ? j.pre[x0 + i] : 0;
+        }
+        for (uint32_t r = 0; r < rows; r++) {
+            const uint64_t* nx = j.pre + (uint64_t)(r + 1) * gw;
+#pragma unroll
+            for (uint32_t i = 0; i < kIdxRun; i++) qn[i] = (r + 1 < rows && x0 + i < x1) ? nx[x0 + i] : 0;
+            uint32_t R = 0, A0 = 0, A1 = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < kIdxRun; i++)
+                if (x0 + i < x1) compose_block(s, q[i], abv[i], R, A0, A1);
+            scan_maps(lane, s.n0, s.n1, R, A0, A1);
+            uint32_t w0 = R ? A0 : add_mod(s.v0, A0, s.n0);
+            uint32_t w1 = R ? A1 : add_mod(s.v1, A1, s.n1);
+            const uint64_t ro = (uint64_t)r * gw;
+#pragma unroll
+            for (uint32_t i = 0; i < kIdxRun; i++)
+                if (x0 + i < x1) apply_block(s, q[i], abv[i], w0, w1, j.rec, ro + x0 + i);
+            s.v0 = (uint32_t)__builtin_amdgcn_readlane((int)w0, (int)last_owner);
+            s.v1 = (uint32_t)__builtin_amdgcn_readlane((int)w1, (int)last_owner);
+#pragma unroll
+            for (uint32_t i = 0; i < kIdxRun; i++) q[i] = qn[i];
+        }
+    } else {
+        for (uint32_t x = x0; x < x1; x++) above[x] = 0;
+        for (uint32_t r = 0; r < rows; r++) {
+            const uint64_t* pr = j.pre + (uint64_t)r * gw;
+            uint32_t R = 0, A0 = 0, A1 = 0;
+            for (uint32_t x = x0; x < x1; x++) compose_block(s, pr[x], above[x], R, A0, A1);
+            scan_maps(lane, s.n0, s.n1, R, A0, A1);
+            uint32_t w0 = R ? A0 : add_mod(s.v0, A0, s.n0);
+            uint32_t w1 = R ? A1 : add_mod(s.v1, A1, s.n1);
+            for (uint32_t x = x0; x < x1; x++) apply_block(s, pr[x], above[x], w0, w1, j.rec, (uint64_t)r * gw + x);
+            s.v0 = (uint32_t)__builtin_amdgcn_readlane((int)w0, (int)last_owner);
+            s.v1 = (uint32_t)__builtin_amdgcn_readlane((int)w1, (int)last_owner);
+        }
+    }
+    if (s.bad) atomicExch(tx.status, CRN2_ECORRUPT);
 }
 
 struct WriteJob {
@@ -367,32 +870,29 @@ __global__ void __launch_bounds__(256) crnd_write_kernel(const WriteJob* __restr
 }
 
 // ----------------------------------------------------------------- profiling
+constexpr int kStages = 7;   // trans, compose, walk, sym (even rows), sym (odd rows), index, write
 struct Profile {
     std::mutex mu;
     bool on = false;
-    struct Ev { hipEvent_t a, b, c; };
+    struct Ev { hipEvent_t e[kStages + 1]; };
     std::vector<Ev> pending;
-    double ms[2] = {0, 0};
-    uint64_t n[2] = {0, 0};
+    double ms[kStages] = {};
+    uint64_t n[kStages] = {};
 };
 static Profile g_prof;
 
 static void profile_collect_locked()
 {
-    for (auto& e : g_prof.pending) {
-        float t0 = 0, t1 = 0;
-        (void)hipEventSynchronize(e.c);
-        if (hipEventElapsedTime(&t0, e.a, e.b) == hipSuccess) {
-            g_prof.ms[0] += t0;
-            g_prof.n[0]++;
+    for (auto& ev : g_prof.pending) {
+        (void)hipEventSynchronize(ev.e[kStages]);
+        for (int s = 0; s < kStages; s++) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, ev.e[s], ev.e[s + 1]) == hipSuccess) {
+                g_prof.ms[s] += t;
+                g_prof.n[s]++;
+            }
         }
-        if (hipEventElapsedTime(&t1, e.b, e.c) == hipSuccess) {
-            g_prof.ms[1] += t1;
-            g_prof.n[1]++;
-        }
-        (void)hipEventDestroy(e.a);
-        (void)hipEventDestroy(e.b);
-        (void)hipEventDestroy(e.c);
+        for (int s = 0; s <= kStages; s++) (void)hipEventDestroy(ev.e[s]);
     }
     g_prof.pending.clear();
 }
@@ -426,13 +926,15 @@ struct crn2_unpack_ctx {
 // A prepared launch over one or more contexts.
 struct crn2_batch {
     uint32_t magic = 0;
-    ParseGroup* d_groups = nullptr;
-    ParseJob* d_jobs = nullptr;
+    LevelJob* d_jobs = nullptr;
+    TexInfo* d_texs = nullptr;
     WriteJob* d_wjobs = nullptr;
     WritePalettes* d_pals = nullptr;
-    uint32_t ngroups = 0, nwjobs = 0;
+    uint8_t* d_scratch = nullptr;   // transition tables, start bits, symbol records
+    uint32_t njobs = 0, nwjobs = 0;
+    uint32_t wg_trans = 0, wg_sym = 0, wg_odd = 0;
+    uint32_t max_gw = 0;
     uint64_t total_blocks = 0;
-    size_t lds_bytes = 0;
     std::vector<int32_t*> status;   // per context
 };
 
@@ -446,20 +948,6 @@ static uint32_t level_dim(uint32_t v, uint32_t l) { return std::max<uint32_t>(1u
 static uint32_t bytes_per_block(uint32_t fmt)
 {
     return (fmt == CRN2_FMT_DXT1 || fmt == CRN2_FMT_DXT5A) ? 8u : 16u;
-}
-
-static size_t max_lds_bytes()
-{
-    static size_t v = 0;
-    if (!v) {
-        int dev = 0;
-        hipDeviceProp_t p;
-        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
-            v = std::max<size_t>(p.sharedMemPerBlock, 65536);
-        else
-            v = 65536;
-    }
-    return v;
 }
 
 #define HIPCHK(x)                                   \
@@ -546,7 +1034,7 @@ static void batch_free(crn2_batch* b)
 {
     if (!b) return;
     b->magic = 0;
-    void* ptrs[] = {b->d_groups, b->d_jobs, b->d_wjobs, b->d_pals};
+    void* ptrs[] = {b->d_jobs, b->d_texs, b->d_wjobs, b->d_pals, b->d_scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -667,51 +1155,41 @@ struct PlanPart {
     std::vector<uint32_t> pitch;                // per level
 };
 
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
 static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
 {
     *out = nullptr;
     crn2_batch* B = new (std::nothrow) crn2_batch();
     if (!B) return CRN2_ENOMEM;
-    std::vector<ParseGroup> groups;
-    std::vector<ParseJob> jobs;
+    std::vector<LevelJob> jobs;
+    std::vector<TexInfo> texs;
     std::vector<WriteJob> wj;
     std::vector<WritePalettes> pals;
     uint64_t total = 0;
-    size_t lds_need = 0;
+    size_t scratch = 0;
+    struct Ofs { size_t T, T2, E, O, G, pre; };
+    std::vector<Ofs> ofs;
+    uint32_t wg_trans = 0, wg_sym = 0, wg_odd = 0, max_gw = 0;
     for (uint32_t pi = 0; pi < parts.size(); pi++) {
         PlanPart& P = parts[pi];
         crn2_unpack_ctx* c = P.c;
         const Header& h = c->h;
-        ParseGroup g;
-        memset(&g, 0, sizeof(g));
-        g.models = c->d_models;
-        g.sorted = c->d_sorted;
-        g.status = c->d_status;
-        g.kind = (uint32_t)c->kind;
-        g.n_ce = c->n_ce;
-        g.n_cs = c->n_cs;
-        g.n_ae = c->n_ae;
-        g.n_as = c->n_as;
-        g.first_job = (uint32_t)jobs.size();
-        g.njobs = P.count;
-        // LDS layout: tables, then block buffers, then canonical orders while they fit
-        size_t ofs = 0;
-        for (uint32_t m = 0; m < M_COUNT; m++) {
-            g.lut_ofs[m] = kNoLds;
-            g.sorted_ofs[m] = kNoLds;
-            g.sorted_n[m] = 0;
-            if (!kind_uses(c->kind, m)) continue;
-            g.lut_ofs[m] = (uint32_t)ofs;
-            ofs += sizeof(LdsTables);
-        }
-        uint32_t bb_entries = 0;
-        g.bb_ofs = (uint32_t)ofs;
-        const uint32_t first_job = (uint32_t)jobs.size();
+        TexInfo t;
+        memset(&t, 0, sizeof(t));
+        t.models = c->d_models;
+        t.sorted = c->d_sorted;
+        t.status = c->d_status;
+        t.kind = (uint32_t)c->kind;
+        t.n_ce = c->n_ce;
+        t.n_cs = c->n_cs;
+        t.n_ae = c->n_ae;
+        t.n_as = c->n_as;
         for (uint32_t i = 0; i < P.count; i++) {
             const uint32_t l = P.first + i;
             const uint32_t bx = (level_dim(h.width, l) + 3) >> 2, by = (level_dim(h.height, l) + 3) >> 2;
             const uint32_t gw = (bx + 1) & ~1u, gh = (by + 1) & ~1u;
-            ParseJob j;
+            LevelJob j;
             memset(&j, 0, sizeof(j));
             if (P.src_override) {
                 j.src = P.src_override;
@@ -721,12 +1199,39 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
                 j.src = c->d_file + h.level_ofs[l];
                 j.src_size = next - h.level_ofs[l];
             }
+            if ((uint64_t)j.src_size * 8 + 4096 >= 0xFFFFFFFFull || gw > 2 * kGroup * kMaxEGroups) {
+                batch_free(B);
+                return CRN2_EUNSUPPORTED;
+            }
+            j.tex = (uint32_t)texs.size();
             j.gw = gw;
             j.gh = gh;
             j.faces = h.faces;
-            j.bb_ofs = bb_entries;
-            bb_entries += gw;
+            j.tlen = (uint32_t)align_up((size_t)j.src_size * 8 + 1024, kTransPerWG);
+            j.wg_trans = wg_trans;
+            wg_trans += j.tlen / kTransPerWG;
+            const uint64_t rowpairs = (uint64_t)h.faces * (gh / 2);
+            const uint64_t gE = rowpairs * ((gw / 2 + kGroup - 1) / kGroup), gO = rowpairs * ((gw + kGroup - 1) / kGroup);
+            j.wg_sym = wg_sym;
+            wg_sym += (uint32_t)((gE + kSymGroupsPerWG - 1) / kSymGroupsPerWG);
+            j.wg_odd = wg_odd;
+            wg_odd += (uint32_t)((gO + kSymGroupsPerWG - 1) / kSymGroupsPerWG);
+            Ofs o;
+            o.T = scratch;
+            scratch = align_up(scratch + (size_t)j.tlen * 4, 256);
+            o.T2 = scratch;
+            scratch = align_up(scratch + (size_t)j.tlen * 8, 256);
+            o.E = scratch;
+            scratch = align_up(scratch + gE * 4, 256);
+            o.O = scratch;
+            scratch = align_up(scratch + gO * 4, 256);
+            o.G = scratch;
+            scratch = align_up(scratch + rowpairs * (gw / 2), 256);
+            o.pre = scratch;
+            scratch = align_up(scratch + rowpairs * 2 * gw * 8, 256);
+            ofs.push_back(o);
             j.rec = static_cast<uint8_t*>(c->d_rec) + c->rec_ofs[l];
+            max_gw = std::max(max_gw, gw);
             jobs.push_back(j);
             WriteJob w;
             memset(&w, 0, sizeof(w));
@@ -743,55 +1248,39 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
             total += (uint64_t)bx * by * h.faces;
             wj.push_back(w);
         }
-        (void)first_job;
-        ofs += (size_t)bb_entries * 4;
-        ofs = (ofs + 15) & ~(size_t)15;
-        for (uint32_t m = 0; m < M_COUNT; m++) {
-            if (!kind_uses(c->kind, m)) continue;
-            const uint32_t n = (uint32_t)c->models[m].sorted.size();
-            const size_t need = ((size_t)n * 2 + 15) & ~(size_t)15;
-            if (ofs + need <= max_lds_bytes()) {
-                g.sorted_ofs[m] = (uint32_t)ofs;
-                g.sorted_n[m] = n;
-                ofs += need;
-            }
-        }
-        if (ofs > max_lds_bytes()) {
-            batch_free(B);
-            return CRN2_EUNSUPPORTED;
-        }
-        lds_need = std::max(lds_need, ofs);
-        groups.push_back(g);
+        texs.push_back(t);
         WritePalettes wp{c->d_ce, c->d_cs, c->d_ae, c->d_as, (uint32_t)c->kind, 0};
         pals.push_back(wp);
         B->status.push_back(c->d_status);
-        if (P.count > 64) {
-            batch_free(B);
-            return CRN2_EINVAL;
-        }
     }
-    B->ngroups = (uint32_t)groups.size();
+    B->njobs = (uint32_t)jobs.size();
     B->nwjobs = (uint32_t)wj.size();
+    B->wg_trans = wg_trans;
+    B->wg_sym = wg_sym;
+    B->wg_odd = wg_odd;
+    B->max_gw = max_gw;
     B->total_blocks = total;
-    B->lds_bytes = lds_need;
-    if (dmalloc(&B->d_groups, groups.size() * sizeof(ParseGroup)) != hipSuccess ||
-        dmalloc(&B->d_jobs, jobs.size() * sizeof(ParseJob)) != hipSuccess ||
+    if (dmalloc(&B->d_jobs, jobs.size() * sizeof(LevelJob)) != hipSuccess ||
+        dmalloc(&B->d_texs, texs.size() * sizeof(TexInfo)) != hipSuccess ||
         dmalloc(&B->d_wjobs, wj.size() * sizeof(WriteJob)) != hipSuccess ||
-        dmalloc(&B->d_pals, pals.size() * sizeof(WritePalettes)) != hipSuccess) {
+        dmalloc(&B->d_pals, pals.size() * sizeof(WritePalettes)) != hipSuccess ||
+        dmalloc(&B->d_scratch, scratch) != hipSuccess) {
         batch_free(B);
         return CRN2_ENOMEM;
     }
-    bool ok = hipMemcpy(B->d_groups, groups.data(), groups.size() * sizeof(ParseGroup), hipMemcpyHostToDevice) == hipSuccess;
-    ok &= hipMemcpy(B->d_jobs, jobs.data(), jobs.size() * sizeof(ParseJob), hipMemcpyHostToDevice) == hipSuccess;
+    for (size_t i = 0; i < jobs.size(); i++) {
+        jobs[i].T = reinterpret_cast<uint32_t*>(B->d_scratch + ofs[i].T);
+        jobs[i].T2 = reinterpret_cast<uint64_t*>(B->d_scratch + ofs[i].T2);
+        jobs[i].gvals = B->d_scratch + ofs[i].G;
+        jobs[i].posE = reinterpret_cast<uint32_t*>(B->d_scratch + ofs[i].E);
+        jobs[i].posO = reinterpret_cast<uint32_t*>(B->d_scratch + ofs[i].O);
+        jobs[i].pre = reinterpret_cast<uint64_t*>(B->d_scratch + ofs[i].pre);
+    }
+    bool ok = hipMemcpy(B->d_jobs, jobs.data(), jobs.size() * sizeof(LevelJob), hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(B->d_texs, texs.data(), texs.size() * sizeof(TexInfo), hipMemcpyHostToDevice) == hipSuccess;
     ok &= hipMemcpy(B->d_wjobs, wj.data(), wj.size() * sizeof(WriteJob), hipMemcpyHostToDevice) == hipSuccess;
     ok &= hipMemcpy(B->d_pals, pals.data(), pals.size() * sizeof(WritePalettes), hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
-        batch_free(B);
-        return CRN2_EHIP;
-    }
-    if (B->lds_bytes > 65536 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&crnd_parse_kernel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)B->lds_bytes) != hipSuccess) {
         batch_free(B);
         return CRN2_EHIP;
     }
@@ -809,14 +1298,34 @@ static int run_plan(crn2_batch* B, hipStream_t stream)
         prof = g_prof.on;
     }
     if (prof) {
-        HIPCHK(hipEventCreate(&ev.a));
-        HIPCHK(hipEventCreate(&ev.b));
-        HIPCHK(hipEventCreate(&ev.c));
-        HIPCHK(hipEventRecord(ev.a, stream));
+        for (int s = 0; s <= kStages; s++) HIPCHK(hipEventCreate(&ev.e[s]));
+        HIPCHK(hipEventRecord(ev.e[0], stream));
     }
-    hipLaunchKernelGGL(crnd_parse_kernel, dim3(B->ngroups), dim3(64), B->lds_bytes, stream, B->d_groups, B->d_jobs);
-    HIPCHK(hipGetLastError());
-    if (prof) HIPCHK(hipEventRecord(ev.b, stream));
+    if (B->njobs) {
+        hipLaunchKernelGGL(crnd_trans_kernel, dim3(B->wg_trans), dim3(256), 0, stream, B->d_jobs, B->njobs, B->d_texs);
+        HIPCHK(hipGetLastError());
+        if (prof) HIPCHK(hipEventRecord(ev.e[1], stream));
+        hipLaunchKernelGGL(crnd_compose_kernel, dim3(B->wg_trans), dim3(256), 0, stream, B->d_jobs, B->njobs);
+        HIPCHK(hipGetLastError());
+        if (prof) HIPCHK(hipEventRecord(ev.e[2], stream));
+        hipLaunchKernelGGL(crnd_walk_kernel, dim3(B->njobs), dim3(64), 0, stream, B->d_jobs);
+        HIPCHK(hipGetLastError());
+        if (prof) HIPCHK(hipEventRecord(ev.e[3], stream));
+        hipLaunchKernelGGL(crnd_sym_kernel<false>, dim3(B->wg_sym), dim3(256), 0, stream, B->d_jobs, B->njobs,
+                           B->d_texs);
+        HIPCHK(hipGetLastError());
+        if (prof) HIPCHK(hipEventRecord(ev.e[4], stream));
+        hipLaunchKernelGGL(crnd_sym_kernel<true>, dim3(B->wg_odd), dim3(256), 0, stream, B->d_jobs, B->njobs,
+                           B->d_texs);
+        HIPCHK(hipGetLastError());
+        if (prof) HIPCHK(hipEventRecord(ev.e[5], stream));
+        hipLaunchKernelGGL(crnd_index_kernel, dim3(B->njobs), dim3(64), (size_t)B->max_gw * 4, stream, B->d_jobs,
+                           B->d_texs);
+        HIPCHK(hipGetLastError());
+        if (prof) HIPCHK(hipEventRecord(ev.e[6], stream));
+    } else if (prof) {
+        for (int k = 1; k < kStages; k++) HIPCHK(hipEventRecord(ev.e[k], stream));
+    }
     if (B->total_blocks) {
         const uint32_t tb = 256;
         const uint64_t nb = (B->total_blocks + tb - 1) / tb;
@@ -825,7 +1334,7 @@ static int run_plan(crn2_batch* B, hipStream_t stream)
         HIPCHK(hipGetLastError());
     }
     if (prof) {
-        HIPCHK(hipEventRecord(ev.c, stream));
+        HIPCHK(hipEventRecord(ev.e[kStages], stream));
         std::lock_guard<std::mutex> lk(g_prof.mu);
         g_prof.pending.push_back(ev);
     }
@@ -1083,13 +1592,15 @@ void crn2_profile_reset(void)
 {
     std::lock_guard<std::mutex> lk(g_prof.mu);
     profile_collect_locked();
-    g_prof.ms[0] = g_prof.ms[1] = 0;
-    g_prof.n[0] = g_prof.n[1] = 0;
+    for (int s = 0; s < kStages; s++) {
+        g_prof.ms[s] = 0;
+        g_prof.n[s] = 0;
+    }
 }
 
 int crn2_profile_read(int kernel, double* total_ms, uint64_t* launches)
 {
-    if (kernel < 0 || kernel > 1 || !total_ms || !launches) return CRN2_EINVAL;
+    if (kernel < 0 || kernel >= kStages || !total_ms || !launches) return CRN2_EINVAL;
     std::lock_guard<std::mutex> lk(g_prof.mu);
     profile_collect_locked();
     *total_ms = g_prof.ms[kernel];

@@ -175,6 +175,24 @@ int crn2_encode_dxt5a_image_device(const void* rgba, uint32_t width, uint32_t he
  * u32 0}. */
 int crn2_dxt1_blocks_device(const void* rgba, uint32_t nblocks, int quality, int flags, void* out, void* stream);
 
+/* ------------------------------------ dxt_hc colour selector assignment */
+/* dxt_hc::create_color_selector_codebook_task (crnlib/crn_dxt_hc.cpp:1306-1360)
+ * for all blocks at once, on the device.  blocks: n_blocks x 16 RGBA8 texels;
+ * endpoint_index: the cluster of each block (two per block, one per 4x2 half,
+ * when subblocks is non-zero); cluster_colors: per cluster its 4 colours
+ * (RGBA8) in selector order (cluster::color_values); selectors: the codebook,
+ * 2 bits per texel, texel p at bits 2p..2p+1; perceptual: 8/25/1 channel
+ * weights, else 1/1/1 (crn_color.h:728-745).  Writes, per block, the first
+ * codebook entry with the smallest error (best_index) and that error
+ * (best_error, optional); adds each block's 16 x 4 distance table into
+ * details[best_index * 64 + 4 * texel + colour] (optional, u32, accumulated,
+ * not cleared) and sets used[best_index] = 1 (optional).  All pointers are
+ * device memory; runs on `stream`. */
+int crn2_hc_color_selectors_device(const void* blocks, const uint32_t* endpoint_index, const void* cluster_colors,
+                                   uint32_t n_blocks, int subblocks, int perceptual, const uint32_t* selectors,
+                                   uint32_t n_selectors, uint32_t* best_index, uint32_t* best_error,
+                                   uint32_t* details, uint8_t* used, void* stream);
+
 /* ---------------------------------------------- writer and test textures */
 /* CRC16 of crn_decomp.h:2048-2069. */
 uint16_t crn2_crc16(const void* p, uint32_t len, uint16_t crc);

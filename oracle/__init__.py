@@ -222,3 +222,26 @@ def dxt1_omatch(which):
     L.orc_dxt1_omatch.argtypes = [C.c_int, C.c_void_p]
     L.orc_dxt1_omatch(which, _p(out))
     return out.reshape(256, 2)
+
+
+def hc_color_selectors(blocks, endpoint_index, cluster_colors, selectors, subblocks=False, perceptual=True):
+    """dxt_hc colour selector assignment (crn_dxt_hc.cpp:1306-1360).
+
+    blocks: (n, 16, 4) uint8; endpoint_index: (n,) or (n, 2) uint32 cluster
+    index per block / half-block; cluster_colors: (k, 4, 4) uint8; selectors:
+    (m,) uint32.  Returns (best_index, best_error, details (m, 64), used (m,))."""
+    L = lib()
+    L.orc_hc_color_selectors.argtypes = [C.c_void_p] * 3 + [C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_uint32] + \
+        [C.c_void_p] * 4
+    blocks = np.ascontiguousarray(blocks, np.uint8)
+    ei = np.ascontiguousarray(endpoint_index, np.uint32)
+    cc = np.ascontiguousarray(cluster_colors, np.uint8)
+    sel = np.ascontiguousarray(selectors, np.uint32)
+    n, m = blocks.shape[0], sel.shape[0]
+    best = np.zeros(n, np.uint32)
+    err = np.zeros(n, np.uint32)
+    det = np.zeros((m, 64), np.uint32)
+    used = np.zeros(m, np.uint8)
+    L.orc_hc_color_selectors(_p(blocks), _p(ei), _p(cc), n, int(bool(subblocks)), int(bool(perceptual)), _p(sel), m,
+                             _p(best), _p(err), _p(det), _p(used))
+    return best, err, det, used
