@@ -86,6 +86,20 @@ SIGNATURES = {
     "crn2_hc_color_selectors_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int,
                                                  C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                                  C.c_void_p, C.c_void_p]),
+    "crn2_mlp_features_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "crn2_gemm_f32_device": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int64, C.c_int64,
+                                       C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int, C.c_int,
+                                       C.c_int, C.c_uint32, C.c_void_p]),
+    "crn2_mlp_bias_act_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_int,
+                                           C.c_void_p]),
+    "crn2_mlp_gelu_back_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "crn2_mlp_loss_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "crn2_adam_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float,
+                                   C.c_float, C.c_float, C.c_uint32, C.c_void_p]),
+    "crn2_mlp_infer_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "crn2_dxt1_refine_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "crn2_dxt1_block_error_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                               C.c_void_p, C.c_void_p]),
     "crn2_crc16": (C.c_uint16, [C.c_void_p, C.c_uint32, C.c_uint16]),
     "crn2_synth_rgba": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_void_p]),
     "crn2_downsample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),

@@ -193,6 +193,53 @@ int crn2_hc_color_selectors_device(const void* blocks, const uint32_t* endpoint_
                                    uint32_t n_selectors, uint32_t* best_index, uint32_t* best_error,
                                    uint32_t* details, uint8_t* used, void* stream);
 
+/* ---------------------------------------- learned DXT1 endpoint proposal */
+/* No reference counterpart: a small network proposes DXT1 endpoints that an
+ * integer search refines; crnlib's dxt1_endpoint_optimizer
+ * (crnlib/crn_dxt1.cpp:2234-2246) stays the fallback and the exact block
+ * error judges every result.  Model: 57 features -> 128 -> 128 -> 7, tanh-form
+ * GELU, bfloat16 weights and layer inputs, float32 sums; parameters theta
+ * (24,839 float32) in the order W1 [57][128], b1, W2 [128][128], b2,
+ * W3 [128][7], b3.  All pointers device memory, work queued on `stream`. */
+/* Features of 16-texel RGBA8 blocks: 48 texel channels / 255 - 0.5, then the
+ * 3x3 colour covariance x 4 (57 floats per block). */
+int crn2_mlp_features_device(const void* rgba, uint32_t n, float* feat, void* stream);
+/* C(m,n) [+]= sum_k A(m,k) B(k,n) in float32 with A(m,k) = A[m*sam + k*sak],
+ * B(k,n) = B[k*sbk + n*sbn], C row-major (ldc); round_a / round_b round the
+ * operands to bfloat16 on load; ksplit > 1 splits K and adds atomically into
+ * C (which must start cleared). */
+int crn2_gemm_f32_device(uint32_t M, uint32_t N, uint32_t K, const float* A, int64_t sam, int64_t sak, const float* B,
+                         int64_t sbk, int64_t sbn, float* C, int64_t ldc, int accumulate, int round_a, int round_b,
+                         uint32_t ksplit, void* stream);
+/* Z += bias (per column); A = bfloat16(gelu(Z)) if use_gelu else A = Z. */
+int crn2_mlp_bias_act_device(float* Z, float* A, const float* bias, uint32_t rows, uint32_t cols, int use_gelu,
+                             void* stream);
+/* dZ = dA * gelu'(Z). */
+int crn2_mlp_gelu_back_device(const float* dA, const float* Z, float* dZ, uint64_t n, void* stream);
+/* Loss of outputs Y (rows x 7) against targets T (rows x 6): batch mean of
+ * squared error of outputs 0-5 plus 0.05 x logistic loss of output 6 against
+ * "every rounded endpoint component within one step"; writes dY, adds the
+ * loss into *loss. */
+int crn2_mlp_loss_device(const float* Y, const float* T, uint32_t rows, float* dY, float* loss, void* stream);
+/* Adam step `step` (1-based) on n parameters. */
+int crn2_adam_device(float* w, const float* g, float* m, float* v, uint32_t n, float lr, float b1, float b2, float eps,
+                     uint32_t step, void* stream);
+/* Network outputs for n RGBA8 blocks on the bfloat16 matrix cores: y holds
+ * 8 floats per block (outputs 0-6, one pad). */
+int crn2_mlp_infer_device(const void* rgba, uint32_t n, const float* theta, float* y, void* stream);
+/* Rounds the proposed endpoints y (8 floats per block, as crn2_mlp_infer_device
+ * writes) to RGB565 and improves them by +-1 steps of one endpoint channel
+ * while the exact error drops (at most max_iters steps).  out: 4 u32 per
+ * block = {c0 | c1 << 16 (c0 > c1: four colours), selectors (2 bits per
+ * texel), perceptual error (8/25/1) under the standard DXT1 decode, steps}. */
+int crn2_dxt1_refine_device(const void* rgba, uint32_t n, const float* y, uint32_t max_iters, uint32_t* out,
+                            void* stream);
+/* The same error for given endpoint words (c0 | c1 << 16 at
+ * endpoints[i * stride_u32]) with the best selectors (written to selectors
+ * when it is not null). */
+int crn2_dxt1_block_error_device(const void* rgba, uint32_t n, const uint32_t* endpoints, uint32_t stride_u32,
+                                 uint32_t* err, uint32_t* selectors, void* stream);
+
 /* ---------------------------------------------- writer and test textures */
 /* CRC16 of crn_decomp.h:2048-2069. */
 uint16_t crn2_crc16(const void* p, uint32_t len, uint16_t crc);
