@@ -25,14 +25,81 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def decode_sse(blocks, ends, sels):
+    """Uniform RGB squared error of DXT1 blocks (standard decode) per block."""
+    import torch
+    c = torch.stack([ends & 0xFFFF, (ends >> 16) & 0xFFFF], 1)
+    r, g, b = (c >> 11) & 31, (c >> 5) & 63, c & 31
+    p = torch.stack([(r << 3) | (r >> 2), (g << 2) | (g >> 4), (b << 3) | (b >> 2)], 2)     # n, 2, 3
+    four = (c[:, 0] > c[:, 1])[:, None]
+    p2 = torch.where(four, (2 * p[:, 0] + p[:, 1]) // 3, (p[:, 0] + p[:, 1]) // 2)
+    p3 = torch.where(four, (p[:, 0] + 2 * p[:, 1]) // 3, torch.zeros_like(p[:, 0]))
+    pal = torch.stack([p[:, 0], p[:, 1], p2, p3], 1)                                            # n, 4, 3
+    idx = (sels[:, None] >> (2 * torch.arange(16, device=sels.device))[None, :]) & 3           # n, 16
+    dec = torch.gather(pal, 1, idx[:, :, None].expand(-1, -1, 3).long())
+    d = dec - blocks[:, :, :3].long()
+    return (d * d).sum((1, 2))
+
+
+def psnr(sse, npix):
+    import math
+    mse = float(sse) / (3.0 * npix)
+    return 99.0 if mse == 0 else 20.0 * math.log10(255.0 / math.sqrt(mse))
+
+
+def encode_leg(size, conf):
+    """DXT1 encode of one size x size RGB texture: the plain GPU search against
+    the learned proposal (+ refinement, + plain fallback below `conf`)."""
+    import numpy as np
+    import torch
+    from crunch2_amd import dxt1, learned, texture
+    from crunch2_amd.train_learned import DEFAULT_OUT
+    if not os.path.exists(DEFAULT_OUT):
+        return {"skipped": "no trained parameters at " + DEFAULT_OUT}
+    theta = torch.from_numpy(np.load(DEFAULT_OUT)).cuda()
+    img = texture.synth(size, size, 4242, texture.RGB)
+    blocks = torch.from_numpy(dxt1.image_blocks(img)).cuda()
+    n = blocks.shape[0]
+    texels = n * 16
+    # warm both paths on a slice
+    dxt1.encode_blocks_device(blocks[:4096].contiguous(), dxt1.UBER, learned.LABEL_FLAGS)
+    learned.encode(theta, blocks[:4096].contiguous(), conf)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    raw = dxt1.encode_blocks_device(blocks, dxt1.UBER, learned.LABEL_FLAGS)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    pend = raw.view(torch.int32).reshape(-1, 6)[:, 2].contiguous()
+    perr, psel = learned.block_errors(blocks, pend)
+    t0 = time.perf_counter()
+    ends, sels, errs, nfb = learned.encode(theta, blocks, conf)
+    torch.cuda.synchronize()
+    t_learn = time.perf_counter() - t0
+    sse_p = int(decode_sse(blocks, pend, psel).sum())
+    sse_l = int(decode_sse(blocks, ends, sels).sum())
+    ps, pl = psnr(sse_p, texels), psnr(sse_l, texels)
+    return {"workload": "DXT1 encode of one %dx%d RGB texture (%d blocks), uber, perceptual, four-colour blocks"
+                        % (size, size, n),
+            "plain_mtexels_s": round(texels / t_plain / 1e6, 1), "learned_mtexels_s": round(texels / t_learn / 1e6, 1),
+            "speedup": round(t_plain / t_learn, 3), "psnr_plain_db": round(ps, 4), "psnr_learned_db": round(pl, 4),
+            "delta_db": round(pl - ps, 4), "blocks_worse_than_plain": int((errs > perr).sum()),
+            "fallback_blocks": nfb, "conf_threshold": conf}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=1, help="texture pairs (distinct seeds) per step")
+    ap.add_argument("--mixed", type=int, default=0,
+                    help="instead of the 4096^2 pair, a mixed set of this many textures (sides log-uniform in "
+                         "[64, 2048], full mips; 60%% DXT1 RGB, 25%% DXT5 RGBA, 15%% DXN normal maps)")
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline sample length")
+    ap.add_argument("--encode-size", type=int, default=8192,
+                    help="side of the RGB texture of the DXT1 encode leg (learned vs plain); 0 skips it")
+    ap.add_argument("--conf", type=float, default=0.0, help="confidence logit below which a block also runs the plain search")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (each GPU would transcode its "
@@ -52,10 +119,21 @@ def main():
     # ---- inputs: seeded textures compressed by the project's CRN writer
     t0 = time.time()
     files = []
-    for b in range(args.batch):
-        for fmt in (FMT_DXT1, FMT_DXT5):
-            files.append(texture.synth_crn(args.size, 1 + b, fmt))
-    log("inputs built in %.1fs: %s bytes" % (time.time() - t0, [len(f) for f in files]))
+    if args.mixed:
+        from crunch2_amd._lib import FMT_DXN_XY
+        rng = np.random.default_rng(2024)
+        for i in range(args.mixed):
+            side = int(round(2 ** rng.uniform(6, 11)))
+            u = rng.random()
+            fmt = FMT_DXT1 if u < 0.60 else (FMT_DXT5 if u < 0.85 else FMT_DXN_XY)
+            files.append(texture.synth_crn((side, side), 5000 + i, fmt))
+        log("mixed set of %d textures built in %.1fs (%d bytes)" % (len(files), time.time() - t0,
+                                                                    sum(len(f) for f in files)))
+    else:
+        for b in range(args.batch):
+            for fmt in (FMT_DXT1, FMT_DXT5):
+                files.append(texture.synth_crn(args.size, 1 + b, fmt))
+        log("inputs built in %.1fs: %s bytes" % (time.time() - t0, [len(f) for f in files]))
 
     ctxs, outs, counts = [], [], []
     texels = 0
@@ -91,7 +169,7 @@ def main():
     ref = oracle.unpack_all(files[0])
     step()
     torch.cuda.synchronize()
-    exp = np.concatenate([ref[l][0] for l in sorted(ref)])
+    exp = np.concatenate([np.concatenate([f.reshape(-1) for f in ref[l]]) for l in sorted(ref)])
     assert np.array_equal(outs[0].cpu().numpy(), exp), "GPU output differs from the oracle"
 
     for _ in range(args.warmup):
@@ -154,8 +232,10 @@ def main():
         "dtype": "u8",
         "data": "synthetic (seeded 4096^2 RGBA8 texture, full box-filter mip chain, packed to .crn by the "
                 "project's writer at crnlib's quality-128 palette sizes)",
-        "config": {"workload": "CRN->DXTn transcode, %dx%d, 13 levels, DXT1 + DXT5, %d texture pair(s) per step"
-                               % (args.size, args.size, args.batch),
+        "config": {"workload": ("CRN->DXTn transcode, mixed set of %d textures (64^2..2048^2, full mips, "
+                                "DXT1/DXT5/DXN_XY)" % args.mixed) if args.mixed else
+                               ("CRN->DXTn transcode, %dx%d, 13 levels, DXT1 + DXT5, %d texture pair(s) per step"
+                                % (args.size, args.size, args.batch)),
                    "crn_bytes": crn_bytes, "dxtn_bytes": out_bytes, "texels": texels},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
@@ -166,6 +246,8 @@ def main():
                                    % (reps, len(files), cpu_t)},
     }
     assert lib().crn2_batch_status(plan, C.c_void_p(stream.cuda_stream)) == 0
+    if args.encode_size:
+        line["encode"] = encode_leg(args.encode_size, args.conf)
     print(json.dumps(line))
     lib().crn2_batch_destroy(plan)
     for c in ctxs:

@@ -66,6 +66,7 @@ SIGNATURES = {
     "crn2_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p),
                                     C.POINTER(C.c_uint64), C.POINTER(C.c_void_p)]),
     "crn2_batch_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "crn2_batch_run_graph": (C.c_int, [C.c_void_p, C.c_void_p]),
     "crn2_batch_status": (C.c_int, [C.c_void_p, C.c_void_p]),
     "crn2_batch_destroy": (C.c_int, [C.c_void_p]),
     "crn2_profile_enable": (None, [C.c_int]),
@@ -86,6 +87,8 @@ SIGNATURES = {
     "crn2_hc_color_selectors_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int,
                                                  C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                                  C.c_void_p, C.c_void_p]),
+    "crn2_autotune_mode": (C.c_int, [C.c_int]),
+    "crn2_autotune_choice": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "crn2_mlp_features_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "crn2_gemm_f32_device": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int64, C.c_int64,
                                        C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64, C.c_int, C.c_int,
@@ -97,7 +100,8 @@ SIGNATURES = {
     "crn2_adam_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float,
                                    C.c_float, C.c_float, C.c_uint32, C.c_void_p]),
     "crn2_mlp_infer_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
-    "crn2_dxt1_refine_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "crn2_dxt1_refine_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                          C.c_void_p]),
     "crn2_dxt1_block_error_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
                                                C.c_void_p, C.c_void_p]),
     "crn2_crc16": (C.c_uint16, [C.c_void_p, C.c_uint32, C.c_uint16]),

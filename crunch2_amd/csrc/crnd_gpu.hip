@@ -936,6 +936,7 @@ struct crn2_batch {
     uint32_t max_gw = 0;
     uint64_t total_blocks = 0;
     std::vector<int32_t*> status;   // per context
+    hipGraphExec_t graph = nullptr; // the plan's launches captured once (crn2_batch_run_graph)
 };
 
 namespace {
@@ -1034,6 +1035,7 @@ static void batch_free(crn2_batch* b)
 {
     if (!b) return;
     b->magic = 0;
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
     void* ptrs[] = {b->d_jobs, b->d_texs, b->d_wjobs, b->d_pals, b->d_scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1289,42 +1291,36 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
     return CRN2_OK;
 }
 
-static int run_plan(crn2_batch* B, hipStream_t stream)
+// Queues the plan's launches on `stream`; with `ev`, records an event before
+// each stage and after the last.
+static int launch_plan(crn2_batch* B, hipStream_t stream, Profile::Ev* ev)
 {
-    Profile::Ev ev{};
-    bool prof;
-    {
-        std::lock_guard<std::mutex> lk(g_prof.mu);
-        prof = g_prof.on;
-    }
-    if (prof) {
-        for (int s = 0; s <= kStages; s++) HIPCHK(hipEventCreate(&ev.e[s]));
-        HIPCHK(hipEventRecord(ev.e[0], stream));
-    }
+    const bool prof = ev != nullptr;
+    if (prof) HIPCHK(hipEventRecord(ev->e[0], stream));
     if (B->njobs) {
         hipLaunchKernelGGL(crnd_trans_kernel, dim3(B->wg_trans), dim3(256), 0, stream, B->d_jobs, B->njobs, B->d_texs);
         HIPCHK(hipGetLastError());
-        if (prof) HIPCHK(hipEventRecord(ev.e[1], stream));
+        if (prof) HIPCHK(hipEventRecord(ev->e[1], stream));
         hipLaunchKernelGGL(crnd_compose_kernel, dim3(B->wg_trans), dim3(256), 0, stream, B->d_jobs, B->njobs);
         HIPCHK(hipGetLastError());
-        if (prof) HIPCHK(hipEventRecord(ev.e[2], stream));
+        if (prof) HIPCHK(hipEventRecord(ev->e[2], stream));
         hipLaunchKernelGGL(crnd_walk_kernel, dim3(B->njobs), dim3(64), 0, stream, B->d_jobs);
         HIPCHK(hipGetLastError());
-        if (prof) HIPCHK(hipEventRecord(ev.e[3], stream));
+        if (prof) HIPCHK(hipEventRecord(ev->e[3], stream));
         hipLaunchKernelGGL(crnd_sym_kernel<false>, dim3(B->wg_sym), dim3(256), 0, stream, B->d_jobs, B->njobs,
                            B->d_texs);
         HIPCHK(hipGetLastError());
-        if (prof) HIPCHK(hipEventRecord(ev.e[4], stream));
+        if (prof) HIPCHK(hipEventRecord(ev->e[4], stream));
         hipLaunchKernelGGL(crnd_sym_kernel<true>, dim3(B->wg_odd), dim3(256), 0, stream, B->d_jobs, B->njobs,
                            B->d_texs);
         HIPCHK(hipGetLastError());
-        if (prof) HIPCHK(hipEventRecord(ev.e[5], stream));
+        if (prof) HIPCHK(hipEventRecord(ev->e[5], stream));
         hipLaunchKernelGGL(crnd_index_kernel, dim3(B->njobs), dim3(64), (size_t)B->max_gw * 4, stream, B->d_jobs,
                            B->d_texs);
         HIPCHK(hipGetLastError());
-        if (prof) HIPCHK(hipEventRecord(ev.e[6], stream));
+        if (prof) HIPCHK(hipEventRecord(ev->e[6], stream));
     } else if (prof) {
-        for (int k = 1; k < kStages; k++) HIPCHK(hipEventRecord(ev.e[k], stream));
+        for (int k = 1; k < kStages; k++) HIPCHK(hipEventRecord(ev->e[k], stream));
     }
     if (B->total_blocks) {
         const uint32_t tb = 256;
@@ -1333,11 +1329,49 @@ static int run_plan(crn2_batch* B, hipStream_t stream)
                            B->total_blocks);
         HIPCHK(hipGetLastError());
     }
-    if (prof) {
-        HIPCHK(hipEventRecord(ev.e[kStages], stream));
+    if (prof) HIPCHK(hipEventRecord(ev->e[kStages], stream));
+    return CRN2_OK;
+}
+
+static int run_plan(crn2_batch* B, hipStream_t stream)
+{
+    bool prof;
+    {
         std::lock_guard<std::mutex> lk(g_prof.mu);
-        g_prof.pending.push_back(ev);
+        prof = g_prof.on;
     }
+    if (!prof) return launch_plan(B, stream, nullptr);
+    Profile::Ev ev{};
+    for (int k = 0; k <= kStages; k++) HIPCHK(hipEventCreate(&ev.e[k]));
+    const int rc = launch_plan(B, stream, &ev);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    g_prof.pending.push_back(ev);
+    return CRN2_OK;
+}
+
+// Captures the plan's launches into a graph once (on a private stream: the
+// caller's may be the legacy default stream, which cannot be captured) and
+// replays it on `stream`.  The launches hold no memset nodes.
+static int run_plan_graph(crn2_batch* B, hipStream_t stream)
+{
+    if (!B->graph) {
+        hipStream_t cs = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        hipGraph_t g = nullptr;
+        int rc = CRN2_OK;
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = CRN2_EHIP;
+        if (!rc) rc = launch_plan(B, cs, nullptr);
+        if (hipStreamEndCapture(cs, &g) != hipSuccess && !rc) rc = CRN2_EHIP;
+        if (!rc && hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0) != hipSuccess) {
+            B->graph = nullptr;
+            rc = CRN2_EHIP;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cs);
+        if (rc) return rc;
+    }
+    HIPCHK(hipGraphLaunch(B->graph, stream));
     return CRN2_OK;
 }
 
@@ -1552,6 +1586,12 @@ int crn2_batch_run(crn2_batch* b, void* stream)
 {
     if (!b || b->magic != kBatchMagic) return CRN2_EINVAL;
     return run_plan(b, static_cast<hipStream_t>(stream));
+}
+
+int crn2_batch_run_graph(crn2_batch* b, void* stream)
+{
+    if (!b || b->magic != kBatchMagic) return CRN2_EINVAL;
+    return run_plan_graph(b, static_cast<hipStream_t>(stream));
 }
 
 int crn2_batch_status(crn2_batch* b, void* stream)

@@ -24,6 +24,7 @@
 
 #include <cstdint>
 
+#include "autotune.h"
 #include "crn2_internal.h"
 
 namespace {
@@ -1032,7 +1033,7 @@ __device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags, uint
     return o;
 }
 
-__global__ void __launch_bounds__(64) dxt1_blocks_kernel(const uint32_t* __restrict__ px, uint32_t nblocks, int quality,
+__global__ void __launch_bounds__(256) dxt1_blocks_kernel(const uint32_t* __restrict__ px, uint32_t nblocks, int quality,
                                                          int flags, Dxt1Out* __restrict__ out)
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1099,8 +1100,14 @@ extern "C" int crn2_dxt1_blocks_device(const void* rgba, uint32_t nblocks, int q
         if (rc) return rc;
         init = true;
     }
-    const uint32_t tb = 64, nb = (nblocks + tb - 1) / tb;
-    hipLaunchKernelGGL(dxt1_blocks_kernel, dim3(nb), dim3(tb), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const uint32_t*>(rgba), nblocks, quality, flags, static_cast<Dxt1Out*>(out));
-    return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // launch shapes (threads per workgroup, one block per lane; identical results): 64, 128, 256
+    auto launch = [&](int shape) -> hipError_t {
+        const uint32_t tb = 64u << shape, nb = (nblocks + tb - 1) / tb;
+        hipLaunchKernelGGL(dxt1_blocks_kernel, dim3(nb), dim3(tb), 0, s, static_cast<const uint32_t*>(rgba), nblocks,
+                           quality, flags, static_cast<Dxt1Out*>(out));
+        return hipGetLastError();
+    };
+    const int shape = crn2::tuner().pick(crn2::TUNE_DXT1, nblocks, 3, 0, s, launch);
+    return launch(shape) == hipSuccess ? CRN2_OK : CRN2_EHIP;
 }

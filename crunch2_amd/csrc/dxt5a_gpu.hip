@@ -22,6 +22,7 @@
 
 #include <cstdint>
 
+#include "autotune.h"
 #include "crn2_internal.h"
 
 namespace {
@@ -222,12 +223,17 @@ extern "C" int crn2_dxt5a_blocks_device(const void* values, uint32_t nblocks, in
     if (!values || !selectors48 || !errors || !meta || quality < 0 || quality > 4) return CRN2_EINVAL;
     if (reinterpret_cast<uintptr_t>(values) & 15) return CRN2_EINVAL;
     if (!nblocks) return CRN2_OK;
-    const uint32_t tb = 256, nb = (nblocks + tb - 1) / tb;
-    hipLaunchKernelGGL(dxt5a_blocks_kernel, dim3(nb), dim3(tb), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const uint8_t*>(values), nblocks, quality, use_both,
-                       static_cast<uint64_t*>(selectors48), static_cast<uint64_t*>(errors),
-                       static_cast<uint32_t*>(meta));
-    return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // launch shapes (threads per workgroup, one block per lane; identical results): 256, 64, 128
+    auto launch = [&](int shape) -> hipError_t {
+        const uint32_t tb = shape == 1 ? 64 : shape == 2 ? 128 : 256, nb = (nblocks + tb - 1) / tb;
+        hipLaunchKernelGGL(dxt5a_blocks_kernel, dim3(nb), dim3(tb), 0, s, static_cast<const uint8_t*>(values),
+                           nblocks, quality, use_both, static_cast<uint64_t*>(selectors48),
+                           static_cast<uint64_t*>(errors), static_cast<uint32_t*>(meta));
+        return hipGetLastError();
+    };
+    const int shape = crn2::tuner().pick(crn2::TUNE_DXT5A, nblocks, 3, 0, s, launch);
+    return launch(shape) == hipSuccess ? CRN2_OK : CRN2_EHIP;
 }
 
 extern "C" int crn2_encode_dxt5a_image_device(const void* rgba, uint32_t width, uint32_t height, uint32_t comp,

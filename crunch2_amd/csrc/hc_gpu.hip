@@ -27,6 +27,7 @@
 
 #include <cstdint>
 
+#include "autotune.h"
 #include "crn2_internal.h"
 
 namespace {
@@ -35,8 +36,6 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 constexpr uint32_t kRowsPerWave = 32;
-constexpr uint32_t kWaves = 4;
-constexpr uint32_t kRowsPerWG = kRowsPerWave * kWaves;
 constexpr uint32_t kSelChunk = 8192;   // selectors staged in LDS at a time (32 KiB)
 
 __device__ inline uint32_t color_distance(bool perceptual, uint32_t a, uint32_t b)
@@ -70,9 +69,10 @@ __device__ inline void e2_pixel(const SelArgs& a, uint32_t row, uint32_t p, uint
     for (uint32_t c = 0; c < 4; c++) e[c] = color_distance(a.perceptual != 0, px, cc[c]);
 }
 
-template <int NL>
-__global__ void __launch_bounds__(256) crn2_hc_sel_gemm_kernel(SelArgs a)
+template <int NL, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) crn2_hc_sel_gemm_kernel(SelArgs a)
 {
+    constexpr uint32_t kRowsPerWG = kRowsPerWave * WAVES;
     __shared__ uint32_t s_sel[kSelChunk];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t g = lane >> 5;                 // K half of the MFMA operand layout
@@ -197,16 +197,60 @@ extern "C" int crn2_hc_color_selectors_device(const void* blocks, const uint32_t
     a.subblocks = subblocks ? 1 : 0;
     a.perceptual = perceptual ? 1 : 0;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const uint32_t wgs = (n_blocks + kRowsPerWG - 1) / kRowsPerWG;
-    if (a.perceptual)
-        hipLaunchKernelGGL(crn2_hc_sel_gemm_kernel<4>, dim3(wgs), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL(crn2_hc_sel_gemm_kernel<3>, dim3(wgs), dim3(256), 0, s, a);
-    if (hipGetLastError() != hipSuccess) return CRN2_EHIP;
+    // launch shapes (waves per workgroup; identical results): 4, 2, 8
+    auto launch = [&](int shape) -> hipError_t {
+        const int waves = shape == 1 ? 2 : shape == 2 ? 8 : 4;
+        const uint32_t rows = kRowsPerWave * waves;
+        const uint32_t wgs = (n_blocks + rows - 1) / rows;
+        if (a.perceptual) {
+            if (waves == 2) hipLaunchKernelGGL((crn2_hc_sel_gemm_kernel<4, 2>), dim3(wgs), dim3(128), 0, s, a);
+            else if (waves == 8) hipLaunchKernelGGL((crn2_hc_sel_gemm_kernel<4, 8>), dim3(wgs), dim3(512), 0, s, a);
+            else hipLaunchKernelGGL((crn2_hc_sel_gemm_kernel<4, 4>), dim3(wgs), dim3(256), 0, s, a);
+        } else {
+            if (waves == 2) hipLaunchKernelGGL((crn2_hc_sel_gemm_kernel<3, 2>), dim3(wgs), dim3(128), 0, s, a);
+            else if (waves == 8) hipLaunchKernelGGL((crn2_hc_sel_gemm_kernel<3, 8>), dim3(wgs), dim3(512), 0, s, a);
+            else hipLaunchKernelGGL((crn2_hc_sel_gemm_kernel<3, 4>), dim3(wgs), dim3(256), 0, s, a);
+        }
+        return hipGetLastError();
+    };
+    const int shape = crn2::tuner().pick(crn2::TUNE_HC_SEL, (uint64_t)n_blocks * n_selectors, 3, 0, s, launch);
+    if (launch(shape) != hipSuccess) return CRN2_EHIP;
     if (details || used) {
         const uint64_t n = (uint64_t)n_blocks * 16;
         hipLaunchKernelGGL(crn2_hc_sel_scatter_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, a);
         if (hipGetLastError() != hipSuccess) return CRN2_EHIP;
     }
+    return CRN2_OK;
+}
+
+namespace crn2 {
+Autotuner& tuner()
+{
+    static Autotuner t;
+    return t;
+}
+}  // namespace crn2
+
+extern "C" int crn2_autotune_mode(int mode)
+{
+    if (mode < 0 || mode > 1) return CRN2_EINVAL;
+    std::lock_guard<std::mutex> lk(crn2::tuner().mu);
+    crn2::tuner().mode = mode;
+    if (!mode) {
+        crn2::tuner().best.clear();
+        crn2::tuner().best_ms.clear();
+    }
+    return CRN2_OK;
+}
+
+extern "C" int crn2_autotune_choice(int kernel, uint64_t work, int* shape, float* ms)
+{
+    if (kernel < 0 || kernel >= crn2::TUNE_COUNT || !shape) return CRN2_EINVAL;
+    std::lock_guard<std::mutex> lk(crn2::tuner().mu);
+    const auto key = std::make_pair(kernel, crn2::Autotuner::size_class(work));
+    auto it = crn2::tuner().best.find(key);
+    if (it == crn2::tuner().best.end()) return CRN2_EINVAL;
+    *shape = it->second;
+    if (ms) *ms = crn2::tuner().best_ms[key];
     return CRN2_OK;
 }

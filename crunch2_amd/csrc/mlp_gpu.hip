@@ -385,22 +385,10 @@ __device__ inline uint32_t pair_error(const int (*px)[3], const int* a, const in
     return dxt1_error(px, lo, hi, nullptr);
 }
 
-// out per block (16 bytes): u32 c0 | c1 << 16, u32 selectors, u32 error, u32 steps
-__global__ void __launch_bounds__(256) refine_kernel(const uint8_t* __restrict__ rgba, uint32_t n,
-                                                     const float* __restrict__ y, uint32_t max_iters,
-                                                     uint32_t* __restrict__ out)
+// +-1 single-channel steps from endpoints e while the exact error drops.
+__device__ inline uint32_t refine_from(const int (*px)[3], int (*e)[3], uint32_t max_iters, uint32_t& steps)
 {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
-    int px[16][3];
-    for (int i = 0; i < 16; i++)
-        for (int k = 0; k < 3; k++) px[i][k] = rgba[(size_t)b * 64 + 4 * i + k];
-    const float* yy = y + (size_t)b * 8;
     const int lim[3] = {31, 63, 31};
-    int e[2][3];
-    for (int s = 0; s < 2; s++)
-        for (int k = 0; k < 3; k++)
-            e[s][k] = (int)fminf(fmaxf(rintf(yy[3 * s + k] * (float)lim[k]), 0.0f), (float)lim[k]);
     uint32_t lo, hi;
     uint32_t best = pair_error(px, e[0], e[1], lo, hi);
     uint32_t it = 0;
@@ -427,13 +415,69 @@ __global__ void __launch_bounds__(256) refine_kernel(const uint8_t* __restrict__
         e[bs][bk] += bd;
         best = cand;
     }
-    uint32_t sel;
-    best = pair_error(px, e[0], e[1], lo, hi);
+    steps += it;
+    return best;
+}
+
+// Start points: bit 0 of `starts` = the network's proposal y (8 floats per
+// block, components scaled to [0, 1]); bit 1 = the block's brightest and
+// darkest texels (luma 2r + 4g + b, first extreme).  Each start is refined and
+// the lower final error kept (the network's on ties).
+// out per block (16 bytes): u32 c0 | c1 << 16, u32 selectors, u32 error, u32 steps
+__global__ void __launch_bounds__(256) refine_kernel(const uint8_t* __restrict__ rgba, uint32_t n,
+                                                     const float* __restrict__ y, uint32_t max_iters, uint32_t starts,
+                                                     uint32_t* __restrict__ out)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    int px[16][3];
+    for (int i = 0; i < 16; i++)
+        for (int k = 0; k < 3; k++) px[i][k] = rgba[(size_t)b * 64 + 4 * i + k];
+    const int lim[3] = {31, 63, 31};
+    uint32_t best = 0xFFFFFFFFu, steps = 0;
+    int keep[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    if (starts & 1u) {
+        const float* yy = y + (size_t)b * 8;
+        int e[2][3];
+        for (int s = 0; s < 2; s++)
+            for (int k = 0; k < 3; k++)
+                e[s][k] = (int)fminf(fmaxf(rintf(yy[3 * s + k] * (float)lim[k]), 0.0f), (float)lim[k]);
+        best = refine_from(px, e, max_iters, steps);
+        for (int s = 0; s < 2; s++)
+            for (int k = 0; k < 3; k++) keep[s][k] = e[s][k];
+    }
+    if (starts & 2u) {
+        int bi = 0, di = 0, bl = -1, dl = 1 << 30;
+        for (int i = 0; i < 16; i++) {
+            const int l = 2 * px[i][0] + 4 * px[i][1] + px[i][2];
+            if (l > bl) {
+                bl = l;
+                bi = i;
+            }
+            if (l < dl) {
+                dl = l;
+                di = i;
+            }
+        }
+        int e[2][3];
+        for (int k = 0; k < 3; k++) {
+            e[0][k] = (int)rintf((float)px[bi][k] * (float)lim[k] / 255.0f);
+            e[1][k] = (int)rintf((float)px[di][k] * (float)lim[k] / 255.0f);
+        }
+        const uint32_t eb = refine_from(px, e, max_iters, steps);
+        if (eb < best) {
+            best = eb;
+            for (int s = 0; s < 2; s++)
+                for (int k = 0; k < 3; k++) keep[s][k] = e[s][k];
+        }
+    }
+    uint32_t lo, hi, sel;
+    pair_error(px, keep[0], keep[1], lo, hi);
     best = dxt1_error(px, lo, hi, &sel);
     out[(size_t)b * 4 + 0] = lo | (hi << 16);
     out[(size_t)b * 4 + 1] = sel;
     out[(size_t)b * 4 + 2] = best;
-    out[(size_t)b * 4 + 3] = it;
+    out[(size_t)b * 4 + 3] = steps;
 }
 
 // Error of given DXT1 blocks (c0 | c1 << 16 per block) under the same decode.
@@ -533,13 +577,13 @@ int crn2_mlp_infer_device(const void* rgba, uint32_t n, const float* theta, floa
     return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
 }
 
-int crn2_dxt1_refine_device(const void* rgba, uint32_t n, const float* y, uint32_t max_iters, uint32_t* out,
-                            void* stream)
+int crn2_dxt1_refine_device(const void* rgba, uint32_t n, const float* y, uint32_t max_iters, uint32_t starts,
+                            uint32_t* out, void* stream)
 {
     if (!n) return CRN2_OK;
-    if (!rgba || !y || !out) return CRN2_EINVAL;
+    if (!rgba || !out || !(starts & 3u) || (starts & ~3u) || ((starts & 1u) && !y)) return CRN2_EINVAL;
     hipLaunchKernelGGL(refine_kernel, dim3(grid1(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const uint8_t*>(rgba), n, y, max_iters, out);
+                       static_cast<const uint8_t*>(rgba), n, y, max_iters, starts, out);
     return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
 }
 

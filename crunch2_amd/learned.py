@@ -141,10 +141,14 @@ def infer(theta, rgba_blocks):
     return y
 
 
-def refine(rgba_blocks, y, max_iters=8):
+START_NET, START_BOX = 1, 2
+
+
+def refine(rgba_blocks, y, max_iters=8, starts=START_NET):
+    """Refinement from the network proposal y and/or the bright/dark texel start."""
     n = rgba_blocks.shape[0]
     out = torch.empty((n, 4), dtype=torch.int32, device=rgba_blocks.device)
-    check(lib().crn2_dxt1_refine_device(_p(rgba_blocks), n, _p(y), max_iters, _p(out), _s()),
+    check(lib().crn2_dxt1_refine_device(_p(rgba_blocks), n, _p(y), max_iters, starts, _p(out), _s()),
           "crn2_dxt1_refine_device")
     return out
 
@@ -186,7 +190,7 @@ def encode(theta, rgba_blocks, conf_threshold=0.0, max_iters=8, guarded=False, q
     Returns (endpoints int32 [n] (c0 | c1 << 16), selectors int32 [n],
     errors int32 [n], number of blocks that ran the plain search)."""
     y = infer(theta, rgba_blocks)
-    r = refine(rgba_blocks, y, max_iters)
+    r = refine(rgba_blocks, y, max_iters, START_NET | START_BOX)
     ends, sels, errs = r[:, 0].clone(), r[:, 1].clone(), r[:, 2].clone()
     if guarded:
         idx = torch.arange(rgba_blocks.shape[0], device=rgba_blocks.device)

@@ -119,6 +119,10 @@ typedef struct crn2_batch crn2_batch;
 int crn2_batch_create(crn2_unpack_ctx* const* ctxs, uint32_t n, void* const* dst_dev, const uint64_t* dst_size,
                       crn2_batch** out);
 int crn2_batch_run(crn2_batch* batch, void* stream);
+/* Same launches, captured into a HIP graph on the first call (private capture
+ * stream) and replayed on `stream` afterwards; for callers that run the same
+ * plan every frame.  Profiling (crn2_profile_enable) does not see graph runs. */
+int crn2_batch_run_graph(crn2_batch* b, void* stream);
 /* Waits for the stream and returns CRN2_ECORRUPT if any stream of the last
  * runs decoded an out-of-range palette index (clearing the flag). */
 int crn2_batch_status(crn2_batch* batch, void* stream);
@@ -193,6 +197,20 @@ int crn2_hc_color_selectors_device(const void* blocks, const uint32_t* endpoint_
                                    uint32_t n_selectors, uint32_t* best_index, uint32_t* best_error,
                                    uint32_t* details, uint8_t* used, void* stream);
 
+/* ------------------------------------------------------------- autotuner */
+/* No reference counterpart (the reference's task pool splits work evenly,
+ * crn_dxt_image.cpp:480-485).  Kernels with several launch shapes of identical
+ * results (crn2_hc_color_selectors_device: waves per workgroup;
+ * crn2_dxt1_blocks_device, crn2_dxt5a_blocks_device: workgroup size) time
+ * every shape on the first call per size class (log2 of the work size) when
+ * mode is 1, and reuse the fastest afterwards; mode 0 (default) runs the
+ * default shape and forgets earlier choices.  kernel: 0 selector assignment,
+ * 1 DXT1, 2 DXT5A. */
+int crn2_autotune_mode(int mode);
+/* The shape chosen for `work` (n_blocks * n_selectors for kernel 0, n_blocks
+ * otherwise) and its time per launch in ms; CRN2_EINVAL if none yet. */
+int crn2_autotune_choice(int kernel, uint64_t work, int* shape, float* ms);
+
 /* ---------------------------------------- learned DXT1 endpoint proposal */
 /* No reference counterpart: a small network proposes DXT1 endpoints that an
  * integer search refines; crnlib's dxt1_endpoint_optimizer
@@ -227,13 +245,15 @@ int crn2_adam_device(float* w, const float* g, float* m, float* v, uint32_t n, f
 /* Network outputs for n RGBA8 blocks on the bfloat16 matrix cores: y holds
  * 8 floats per block (outputs 0-6, one pad). */
 int crn2_mlp_infer_device(const void* rgba, uint32_t n, const float* theta, float* y, void* stream);
-/* Rounds the proposed endpoints y (8 floats per block, as crn2_mlp_infer_device
- * writes) to RGB565 and improves them by +-1 steps of one endpoint channel
- * while the exact error drops (at most max_iters steps).  out: 4 u32 per
- * block = {c0 | c1 << 16 (c0 > c1: four colours), selectors (2 bits per
- * texel), perceptual error (8/25/1) under the standard DXT1 decode, steps}. */
-int crn2_dxt1_refine_device(const void* rgba, uint32_t n, const float* y, uint32_t max_iters, uint32_t* out,
-                            void* stream);
+/* Refines start endpoints by +-1 steps of one endpoint channel while the
+ * exact error drops (at most max_iters steps per start) and keeps the best.
+ * starts: bit 0 = the proposal y (8 floats per block as crn2_mlp_infer_device
+ * writes, rounded to RGB565), bit 1 = the block's brightest and darkest
+ * texels (luma 2r + 4g + b).  out: 4 u32 per block = {c0 | c1 << 16 (c0 > c1:
+ * four colours), selectors (2 bits per texel), perceptual error (8/25/1)
+ * under the standard DXT1 decode, steps taken}. */
+int crn2_dxt1_refine_device(const void* rgba, uint32_t n, const float* y, uint32_t max_iters, uint32_t starts,
+                            uint32_t* out, void* stream);
 /* The same error for given endpoint words (c0 | c1 << 16 at
  * endpoints[i * stride_u32]) with the best selectors (written to selectors
  * when it is not null). */

@@ -135,3 +135,47 @@ def test_full_size_4096_matches_oracle(gpu, fmt):
     got = gpu_all_levels(data)
     for l in ref:
         assert np.array_equal(got[l][0], ref[l][0]), l
+
+
+def test_batch_plan_mixed_textures_and_graph_replay(gpu):
+    """One plan over textures of every format family, odd sizes and a cube
+    map; eager run against the oracle, then the captured graph replayed three
+    times (outputs cleared by eager work in between), each replay bit-exact."""
+    import torch
+    say("batch plan + graph replay")
+    specs = [((256, 256), 31, FMT_DXT1, 1), ((96, 200), 32, FMT_DXT5A, 1), ((64, 64), 33, FMT_DXT5, 6),
+             ((128, 72), 34, FMT_DXN_XY, 1), ((4, 4), 35, FMT_DXT1, 1)]
+    files = [texture.synth_crn(sz, seed, fmt, faces=faces) for sz, seed, fmt, faces in specs]
+    ctxs, outs, exps = [], [], []
+    for d in files:
+        c = crnd.unpack_begin(d)
+        assert c is not None
+        inf = crnd.get_texture_info(d)
+        outs.append(torch.zeros(crnd.levels_size(c, 0, inf["levels"]), dtype=torch.uint8, device="cuda"))
+        ctxs.append(c)
+        ref = oracle.unpack_all(d)
+        exps.append(np.concatenate([np.concatenate([f.reshape(-1) for f in ref[l]]) for l in sorted(ref)]))
+    n = len(ctxs)
+    plan = C.c_void_p()
+    assert lib().crn2_batch_create((C.c_void_p * n)(*[c.handle for c in ctxs]), n,
+                                   (C.c_void_p * n)(*[o.data_ptr() for o in outs]),
+                                   (C.c_uint64 * n)(*[o.numel() for o in outs]), C.byref(plan)) == 0
+    stream = torch.cuda.current_stream()
+    assert lib().crn2_batch_run(plan, C.c_void_p(stream.cuda_stream)) == 0
+    torch.cuda.synchronize()
+    for o, e in zip(outs, exps):
+        assert np.array_equal(o.cpu().numpy(), e)
+    for rep in range(3):
+        for o in outs:
+            o.zero_()
+        junk = torch.randn(1 << 20, device="cuda").sum()
+        torch.cuda.synchronize()
+        assert lib().crn2_batch_run_graph(plan, C.c_void_p(stream.cuda_stream)) == 0
+        torch.cuda.synchronize()
+        for o, e in zip(outs, exps):
+            assert np.array_equal(o.cpu().numpy(), e), rep
+        del junk
+    assert lib().crn2_batch_status(plan, C.c_void_p(stream.cuda_stream)) == 0
+    assert lib().crn2_batch_destroy(plan) == 0
+    for c in ctxs:
+        assert crnd.unpack_end(c)
