@@ -203,20 +203,44 @@ def main():
     # algorithmic bytes of one pass of the pipeline (all files of a step): compressed in + DXTn out
     achieved = alg_bytes / (pipe_ms * 1e-3) / 1e9
 
-    # ---- CPU baseline: the C restatement, single thread, same files
+    # ---- CPU baseline: the C restatement on the host cores, one context per
+    # level of every file (the reference transcoder is single-threaded per
+    # context; its multithreaded use is one context per level or texture), so
+    # the levels of all files decode concurrently; contexts are opened (palettes
+    # decoded) outside the timed region, as are the destination buffers.
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = []
+    for d in files:
+        inf = crnd.get_texture_info(d)
+        for l in range(inf["levels"]):
+            li = crnd.get_level_info(d, l)
+            c = oracle.Context(d)
+            bpb = 8 if inf["format"] in (0, 9) else 16
+            dst = [np.zeros(li["blocks_x"] * li["blocks_y"] * bpb, np.uint8) for _ in range(inf["faces"])]
+            jobs.append((li["blocks_x"] * li["blocks_y"] * inf["faces"], c, l, dst))
+    jobs.sort(key=lambda j: -j[0])
+    threads = min(16, os.cpu_count() or 1, len(jobs))
+
+    def run_job(j):
+        assert j[1].unpack_level(j[2], dst=j[3]) is not False
+
     cpu_t = 0.0
     cpu_tex = 0
     reps = 0
-    t0 = time.time()
-    while time.time() - t0 < args.cpu_seconds:
-        for d in files:
-            s = time.perf_counter()
-            r = oracle.unpack_all(d)
-            cpu_t += time.perf_counter() - s
-            assert r is not None
-        cpu_tex += texels
-        reps += 1
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        while cpu_t < args.cpu_seconds:
+            s0 = time.perf_counter()
+            list(pool.map(run_job, jobs))
+            cpu_t += time.perf_counter() - s0
+            cpu_tex += texels
+            reps += 1
     cpu_mtex = cpu_tex / cpu_t / 1e6
+    s0 = time.perf_counter()
+    for j in jobs:
+        run_job(j)
+    cpu1_mtex = texels / (time.perf_counter() - s0) / 1e6
+    for j in jobs:
+        j[1].close()
 
     line = {
         "metric": "Mtexels/s: CRN->DXTn transcode (DXT1 + DXT5, 4096^2, 13 levels)",
@@ -241,9 +265,10 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
                      "kernel": "transcode pipeline (5 launches per step)",
                      "stage_ms": {k: round(v, 4) for k, v in prof.items()}},
-        "cpu_baseline": {"value": round(cpu_mtex, 1), "unit": "Mtexels/s", "cores": 1, "kind": "port",
-                         "sample": "%d full passes over the same %d files (%.1fs of CPU decode)"
-                                   % (reps, len(files), cpu_t)},
+        "cpu_baseline": {"value": round(cpu_mtex, 1), "unit": "Mtexels/s", "cores": threads, "kind": "port",
+                         "sample": "%d full passes over the same %d files, %d level jobs on %d threads (%.1fs of "
+                                   "wall time); one thread alone: %.1f Mtexels/s"
+                                   % (reps, len(files), len(jobs), threads, cpu_t, cpu1_mtex)},
     }
     assert lib().crn2_batch_status(plan, C.c_void_p(stream.cuda_stream)) == 0
     if args.encode_size:

@@ -376,38 +376,33 @@ __global__ void __launch_bounds__(64) crnd_walk_kernel(const LevelJob* __restric
     w = ring[64 + lane];
     uint32_t nlo = (uint32_t)w, nhi = (uint32_t)(w >> 32);
     uint32_t o = 0;                                    // current bit - base
-    // Steps move at most 288 bits; one window advance covers moves that end
-    // in the next window, longer moves reload both windows from LDS.
+    // Moves the window forward by whole windows until it holds bit base + o.
+    // Each window step checks the segment boundary of its next-window position
+    // once; a single step takes the prefetched window, longer moves (a step
+    // covers at most 288 bits) reload the current window from LDS.
 #define CRND_ADVANCE()                                                         \
     do {                                                                       \
         if (o >= 64) {                                                         \
-            base += 64;                                                        \
-            o -= 64;                                                           \
-            vlo = nlo;                                                         \
-            vhi = nhi;                                                         \
-            if (o >= 64) {                                                     \
-                const uint32_t k_ = o >> 6;                                    \
-                for (uint32_t i_ = 0; i_ < k_; i_++) {                         \
-                    base += 64;                                                \
-                    const uint32_t nb_ = base + 64;                            \
-                    if ((nb_ & (kSeg - 1)) == 0) {                             \
-                        const uint32_t s_ = nb_ / kSeg;                        \
-                        seg_store(ring, s_ + 1, lane, pf);                     \
-                        seg_load(T2, tsegs, tlast, s_ + 2, lane, pf);          \
-                    }                                                          \
+            const uint32_t k_ = o >> 6;                                        \
+            for (uint32_t i_ = 0; i_ < k_; i_++) {                             \
+                base += 64;                                                    \
+                const uint32_t nb_ = base + 64;                                \
+                if ((nb_ & (kSeg - 1)) == 0) {                                 \
+                    const uint32_t s_ = nb_ / kSeg;                            \
+                    seg_store(ring, s_ + 1, lane, pf);                         \
+                    seg_load(T2, tsegs, tlast, s_ + 2, lane, pf);              \
                 }                                                              \
-                o -= 64 * k_;                                                  \
+            }                                                                  \
+            o -= 64 * k_;                                                      \
+            if (k_ == 1) {                                                     \
+                vlo = nlo;                                                     \
+                vhi = nhi;                                                     \
+            } else {                                                           \
                 const uint64_t c_ = ring[(base & (4 * kSeg - 1)) + lane];      \
                 vlo = (uint32_t)c_;                                            \
                 vhi = (uint32_t)(c_ >> 32);                                    \
             }                                                                  \
-            const uint32_t nb_ = base + 64;                                    \
-            if ((nb_ & (kSeg - 1)) == 0) {                                     \
-                const uint32_t s_ = nb_ / kSeg;                                \
-                seg_store(ring, s_ + 1, lane, pf);                             \
-                seg_load(T2, tsegs, tlast, s_ + 2, lane, pf);                  \
-            }                                                                  \
-            const uint64_t w_ = ring[(nb_ & (4 * kSeg - 1)) + lane];           \
+            const uint64_t w_ = ring[((base + 64) & (4 * kSeg - 1)) + lane];   \
             nlo = (uint32_t)w_;                                                \
             nhi = (uint32_t)(w_ >> 32);                                        \
         }                                                                      \
