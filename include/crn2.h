@@ -179,6 +179,20 @@ int crn2_encode_dxt5a_image_device(const void* rgba, uint32_t width, uint32_t he
  * u32 0}. */
 int crn2_dxt1_blocks_device(const void* rgba, uint32_t nblocks, int quality, int flags, void* out, void* stream);
 
+/* The same search for pixel clusters of any size on the path dxt_hc takes for
+ * its endpoint clusters (crn_dxt_hc.cpp:663-754 calling
+ * dxt1_endpoint_optimizer::compute with uber quality and no 3-colour blocks;
+ * crn_dxt1.cpp:1759-1869 evaluators), one wave per cluster.  rgba: all
+ * clusters' texels (RGBA8 words, alpha ignored), cluster c owning texels
+ * [offsets[c], offsets[c+1]); flags bit0 perceptual.  out: 24 bytes per
+ * cluster = {u64 m_error, u32 m_low_color | m_high_color << 16, u32 unique
+ * colours, u32 m_reordered | m_alternate_rounding << 2 | m_enforce_selector
+ * << 3 | m_enforced_selector << 4 | 256 when the cluster has more than 4096
+ * unique colours (not encoded), u32 0}; selectors (optional): the 2-bit
+ * DXT1 code of every texel. */
+int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags, void* out,
+                              uint8_t* selectors, void* stream);
+
 /* ------------------------------------ dxt_hc colour selector assignment */
 /* dxt_hc::create_color_selector_codebook_task (crnlib/crn_dxt_hc.cpp:1306-1360)
  * for all blocks at once, on the device.  blocks: n_blocks x 16 RGBA8 texels;

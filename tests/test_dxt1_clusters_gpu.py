@@ -1,0 +1,63 @@
+"""Wave-per-cluster DXT1 search (dxt_hc endpoint clusters of 1 to thousands of
+texels) against the oracle's optimizer on the same texels: error and
+endpoints per cluster, selectors per texel, and the reported error recomputed
+from the emitted endpoints and selectors."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from crunch2_amd import dxt1, texture
+
+pytestmark = pytest.mark.gpu
+W = np.array([8, 25, 1], np.int64)
+
+
+def clusters(seed, sizes):
+    rng = np.random.default_rng(seed)
+    img = texture.synth(512, 512, seed, texture.RGB).reshape(-1, 4)
+    parts = []
+    for n in sizes:
+        start = int(rng.integers(0, img.shape[0] - n))
+        if rng.random() < 0.5:
+            parts.append(img[start:start + n])                                   # smooth run of texels
+        else:
+            parts.append(img[rng.integers(0, img.shape[0], n)])                   # scattered texels
+    return parts
+
+
+@pytest.mark.parametrize("perceptual", [True, False])
+def test_clusters_match_oracle(gpu, perceptual):
+    rng = np.random.default_rng(5)
+    sizes = [1, 2, 3, 4, 16, 17, 64, 200] + list(rng.integers(16, 2048, 40)) + [4096, 5000]
+    parts = clusters(6 + perceptual, sizes)
+    parts.append(np.repeat(parts[0][:1], 300, axis=0))                            # one colour, many texels
+    px = np.concatenate(parts)
+    ofs = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.int32)
+    print("clusters", len(parts), "texels", len(px), flush=True)
+    raw, sel = dxt1.encode_clusters_device(torch.from_numpy(px).cuda(), torch.from_numpy(ofs).cuda(), perceptual)
+    torch.cuda.synchronize()
+    r = raw.cpu().numpy().view(dxt1.CLUSTER_DTYPE).reshape(-1)
+    sel = sel.cpu().numpy()
+    flags = oracle.DXT1_PERCEPTUAL if perceptual else 0
+    same, same_err = 0, 0
+    for c, p in enumerate(parts):
+        osel, ores = oracle.dxt1_cluster(p, 4, flags)
+        lo, hi = int(r["lo_hi"][c] & 0xFFFF), int(r["lo_hi"][c] >> 16)
+        s = sel[ofs[c]:ofs[c + 1]]
+        e = int(r["error"][c])
+        same_err += e == ores.error
+        same += e == ores.error and lo == ores.low and hi == ores.high and np.array_equal(s, osel)
+        # the reported error is the emitted block's (hc palette with the alternate-rounding flag)
+        alt = int((r["flags"][c] >> 2) & 1)
+
+        def ex(v):
+            return np.array([((v >> 11) & 31) * 33 // 4, ((v >> 5) & 63) * 65 // 16, (v & 31) * 33 // 4], np.int64)
+        c0, c1 = ex(lo), ex(hi)
+        pal = np.stack([c0, c1, (2 * c0 + c1 + alt) // 3, (c0 + 2 * c1 + alt) // 3])
+        d = ((p[:, None, :3].astype(np.int64) - pal[None]) ** 2 * (W if perceptual else 1)).sum(-1)
+        if not (r["flags"][c] >> 3) & 1:
+            assert int(d[np.arange(len(p)), s].sum()) == e, c
+    print("bit-exact clusters %d / %d, error-equal %d" % (same, len(parts), same_err), flush=True)
+    assert same_err >= len(parts) - 1
+    assert same >= len(parts) - 2
