@@ -18,7 +18,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-STAGES = ("crnd_trans_kernel", "crnd_walk_kernel", "crnd_sym_kernel", "crnd_index_kernel", "crnd_write_kernel")
+STAGES = ("crnd_trans_kernel", "crnd_compose_kernel", "crnd_walk_kernel", "crnd_sym_kernel<even>",
+          "crnd_sym_kernel<odd>", "crnd_index_kernel", "crnd_write_kernel")
 
 
 def log(*a):
@@ -263,7 +264,7 @@ def main():
                    "crn_bytes": crn_bytes, "dxtn_bytes": out_bytes, "texels": texels},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                     "kernel": "transcode pipeline (5 launches per step)",
+                     "kernel": "transcode pipeline (7 launches per step)",
                      "stage_ms": {k: round(v, 4) for k, v in prof.items()}},
         "cpu_baseline": {"value": round(cpu_mtex, 1), "unit": "Mtexels/s", "cores": threads, "kind": "port",
                          "sample": "%d full passes over the same %d files, %d level jobs on %d threads (%.1fs of "
