@@ -10,16 +10,19 @@
 //   1. crnd_trans_kernel (all bit positions in parallel): for every bit
 //      position, the bits an even-row block pair or an odd-row block would
 //      take if it started there.
-//   1b. crnd_compose_kernel (all bit positions): the same for two pairs / two
-//      blocks (four variants by the two blocks' reference flags).
-//   2. crnd_walk_kernel (one wave per level): follows the stream through the
-//      two-unit table, one lookup per two pairs or two odd-row blocks, in
-//      scalar registers with the nearby entries in vector registers; records
-//      the start bit of every group of 8 pairs / 8 odd-row blocks.
+//   1b. crnd_compose_kernel (all bit positions): the same for eight pairs
+//      (with their odd-row flags) and for two odd-row blocks (four variants by
+//      the two blocks' reference flags).
+//   2. crnd_walk_kernel (two waves per level): one wave follows the stream
+//      through the composed tables, one lookup per eight pairs or two odd-row
+//      blocks, in scalar registers; the other streams the tables into an LDS
+//      ring ahead of it.  Records the start bit of every group of 8 pairs /
+//      8 odd-row blocks.
 //   3. crnd_sym_kernel (one thread per group; even rows, then odd rows):
 //      endpoint deltas, selectors and reference codes from the start bits.
-//   4. crnd_index_kernel (one wave per level): running endpoint indices, row
-//      by row, with a wave prefix scan inside each row.
+//   4. crnd_index_{chunk,link,final}_kernel: running endpoint indices by
+//      pointer jumping over the blocks' parent links (previous block or block
+//      above), in LDS per chunk of rows, then across chunks.
 //   5. crnd_write_kernel (one thread per visible block): palette gather and
 //      the 8- or 16-byte DXTn block write, coalesced.
 #include <hip/hip_runtime.h>
@@ -81,12 +84,22 @@ struct LevelJob {
     uint32_t wg_sym;             // first workgroup of the job in crnd_sym_kernel<false> (even rows)
     uint32_t wg_odd;             // first workgroup of the job in crnd_sym_kernel<true> (odd rows)
     uint32_t* T;                 // one-unit transitions, one entry per bit position
-    uint64_t* T2;                // two-unit transitions, one entry per bit position
+    uint32_t* O2;                // two odd-row blocks, one entry per bit position
+    uint32_t* E8;                // eight even-row pairs, one entry per bit position
+    uint16_t* X;                 // one even-row pair, one entry per bit position
     uint32_t* posE;              // start bit of each group of kGroup pairs of the even rows
     uint32_t* posO;              // start bit of each group of kGroup blocks of the odd rows
     uint8_t* gvals;              // faces * gh/2 * gw/2 reference group symbols
     uint64_t* pre;               // faces * gh * gw: decoded symbols of each block
     void* rec;                   // faces * gh * gw palette-index records (uint32 kinds 0/3, uint64 kinds 1/2)
+    uint32_t crows;              // block rows per index chunk
+    uint32_t nchunks;            // index chunks of the level
+    uint32_t wg_idx;             // first workgroup of the job in crnd_index_chunk_kernel
+    uint32_t pad;
+    uint64_t bnd_first;          // first compact entry of the job in crnd_index_link_kernel
+    uint64_t blk_first;          // first block of the job in crnd_index_final_kernel
+    uint64_t* ent;               // faces * gh * gw: (parent, index residues) after the chunk pass
+    uint64_t* bnd;               // two buffers of nchunks * gw: the chunks' last rows
 };
 
 __device__ inline bool kind_uses_dev(uint32_t kind, uint32_t m)
@@ -267,16 +280,16 @@ __global__ void __launch_bounds__(256) crnd_trans_kernel(const LevelJob* __restr
     }
 }
 
-// ------------------------------------------ stage 1b: two-unit transitions
-// For every bit position q, one 64-bit entry built from the one-unit table:
-//   low word, byte f (f = 0..3): bits taken by two odd-row blocks starting at
-//       q when bit 0 of f says the first carries endpoint deltas and bit 1
-//       the second
-//   high word, bits 0-8: bits taken by two even-row pairs starting at q
-//              bits 9-12: the odd-row flags of those pairs (two per pair:
-//                         block below the pair's first / second block has
-//                         reference code 0)
-//              bits 13-20: bits taken by one even-row pair (row ends)
+// ------------------------------------------- stage 1b: composed transitions
+// For every bit position q, from the one-unit table:
+//   O2[q], byte f (f = 0..3): bits taken by two odd-row blocks starting at q
+//          when bit 0 of f says the first carries endpoint deltas and bit 1
+//          the second
+//   E8[q], bits 0-10: bits taken by eight even-row pairs starting at q
+//          bits 11-26: the odd-row flags of those pairs, two per pair (the
+//          block below the pair's first / second block has reference code 0)
+//   X[q],  bits 0-7: bits taken by one even-row pair; bits 8-9: its two flags
+// Eight pairs take at most 8 * 144 = 1152 bits, two odd blocks at most 128.
 __device__ inline uint32_t odd_flags(uint32_t g) { return (uint32_t)((g & 0x0Cu) == 0) | ((uint32_t)((g & 0xC0u) == 0) << 1); }
 
 __global__ void __launch_bounds__(256) crnd_compose_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs)
@@ -290,138 +303,181 @@ __global__ void __launch_bounds__(256) crnd_compose_kernel(const LevelJob* __res
         const uint32_t t0 = T[q];
         const uint32_t o0 = (t0 >> 16) & 0xFFu, o1 = t0 >> 24;
         const uint32_t ta = T[min(q + o0, last)], tb = T[min(q + o1, last)];
-        const uint32_t lo = (o0 + ((ta >> 16) & 0xFFu)) | ((o1 + ((tb >> 16) & 0xFFu)) << 8) |
-                            ((o0 + (ta >> 24)) << 16) | ((o1 + (tb >> 24)) << 24);
-        const uint32_t e1 = t0 & 0xFFu;
-        const uint32_t te = T[min(q + e1, last)];
-        const uint32_t e2 = e1 + (te & 0xFFu);
-        const uint32_t fl = odd_flags((t0 >> 8) & 0xFFu) | (odd_flags((te >> 8) & 0xFFu) << 2);
-        const uint32_t hi = e2 | (fl << 9) | (e1 << 13);
-        j.T2[q] = (uint64_t)lo | ((uint64_t)hi << 32);
+        j.O2[q] = (o0 + ((ta >> 16) & 0xFFu)) | ((o1 + ((tb >> 16) & 0xFFu)) << 8) | ((o0 + (ta >> 24)) << 16) |
+                  ((o1 + (tb >> 24)) << 24);
+        uint32_t p = q + (t0 & 0xFFu), fl = odd_flags((t0 >> 8) & 0xFFu);
+#pragma unroll
+        for (uint32_t i = 1; i < 8; i++) {
+            const uint32_t t = T[min(p, last)];
+            fl |= odd_flags((t >> 8) & 0xFFu) << (2 * i);
+            p += t & 0xFFu;
+        }
+        j.E8[q] = (p - q) | (fl << 11);
+        j.X[q] = (uint16_t)((t0 & 0xFFu) | (odd_flags((t0 >> 8) & 0xFFu) << 8));
     }
 }
 
 // --------------------------------------------------------- stage 2: walker
-// One wave per level stream follows the stream from bit 0 through the
-// two-unit table: an even row takes one step per two block pairs and collects
-// the odd-row flags of its pairs; the odd row then takes one step per two
-// blocks, picking the entry byte its two flags select.  The start bit of every
-// group of kGroup pairs (even rows) or kGroup blocks (odd rows) is recorded.
+// One workgroup of two waves per level stream.  Wave 0 follows the stream
+// from bit 0 and records the start bit of every group of kGroup pairs (even
+// rows) or kGroup blocks (odd rows); wave 1 streams the composed tables from
+// memory into an LDS ring ahead of it, so the walking wave never waits on
+// memory and holds no staging registers.
 //
-// A lone wave issues about one instruction every four cycles, so the loop is
-// kept short: all walker state is wave-uniform (scalar registers), the 64 table
-// entries from `base` sit one per lane in two vector registers and are read
-// with v_readlane at offset o = bit - base, the next 64 are prefetched from
-// LDS, and LDS is refilled from the table in 8 KiB segments loaded one segment
-// ahead.
-constexpr uint32_t kSeg = 1024;               // table entries per LDS segment
-constexpr uint32_t kGroup = 8;                // pairs / blocks per recorded start bit
-constexpr uint32_t kMaxEGroups = 1024;        // even-row groups per row: grids up to 16384 blocks wide
+// A lone wave issues about one instruction every four cycles, so the walking
+// loop is kept short and wave-uniform (scalar registers):
+//   even row: one E8 lookup per group of eight pairs (a broadcast LDS read),
+//             one X lookup per pair in a row's last, partial group;
+//   odd row:  one O2 lookup per two blocks, read with v_readlane from the 64
+//             ring entries at `base` held one per lane in a vector register,
+//             the next 64 prefetched into a second one.
+// The walker tells the producer which segment it has reached at the start of
+// a group; the producer never overwrites that segment or a later unread one,
+// and the walker waits (bounded by the producer's progress) only when the ring
+// does not yet hold kNeed entries past the group's start.
+constexpr uint32_t kSeg = 1024;                       // table entries per ring segment
+constexpr uint32_t kRingSegs = 8;
+constexpr uint32_t kRingPos = kSeg * kRingSegs;
+constexpr uint32_t kBatch = 4;                        // segments per producer batch
+constexpr uint32_t kGroup = 8;                        // pairs / blocks per recorded start bit
+constexpr uint32_t kMaxEGroups = 64;                  // even-row groups per row (one per lane): grids up to 1024 blocks wide
+constexpr uint32_t kNeed = 1152 + 128 + 64;           // a group's longest move + the two windows
+// Either wave waits for the other at most this many sleeps (about a quarter
+// of a second; a real wait lasts microseconds).  Past it the producer stops
+// and the walker finishes its loops on whatever the ring holds and reports
+// CRN2_EHIP, so a fault in the hand-off cannot hang the device.
+constexpr uint32_t kSpinCap = 1u << 21;
 
-// Global-address-space views: loads and stores through them are global_*
-// instructions, which unlike flat_* ones do not count against the LDS wait
-// counter the walker waits on.
+struct WalkLds {
+    uint32_t o2[kRingPos];
+    uint32_t e8[kRingPos];
+    uint16_t x[kRingPos];
+    uint32_t ready;       // segments [0, ready) are in the ring
+    uint32_t consumed;    // the walker reads no segment below this one any more
+    uint32_t done;        // the walker has finished
+};
+
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const u32x4 gconst_u32x4;
 typedef __attribute__((address_space(1))) uint32_t guint32;
 
-struct SegRegs {
-    uint4 r[8];
+__device__ inline uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ inline uint32_t lds_get(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+__device__ inline void lds_put(uint32_t* p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+
+// Per lane and segment: 4 + 4 vectors of O2 / E8 entries, 2 of X entries.
+struct SegVecs {
+    u32x4 v[kBatch][10];
 };
 
-__device__ inline void seg_load(const uint64_t* T2, uint32_t tsegs, uint64_t tlast, uint32_t s, uint32_t lane,
-                                SegRegs& v)
+// Segments past the table reload its last one: every entry from the end of
+// the stream on is the same, and the table's last segment lies past the end.
+__device__ inline void producer_load(const LevelJob& j, uint32_t tsegs, uint32_t s0, uint32_t lane, SegVecs& r)
 {
-    if (s < tsegs) {
-        const gconst_u32x4* p = (const gconst_u32x4*)(T2 + (size_t)s * kSeg);
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            const u32x4 t = p[64 * i + lane];
-            v.r[i] = make_uint4(t.x, t.y, t.z, t.w);
+    for (uint32_t b = 0; b < kBatch; b++) {
+        const size_t at = (size_t)min(s0 + b, tsegs - 1) * kSeg;
+        const gconst_u32x4* po = (const gconst_u32x4*)(j.O2 + at);
+        const gconst_u32x4* pe = (const gconst_u32x4*)(j.E8 + at);
+        const gconst_u32x4* px = (const gconst_u32x4*)(j.X + at);
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) {
+            r.v[b][i] = po[64 * i + lane];
+            r.v[b][4 + i] = pe[64 * i + lane];
         }
-    } else {
-        const uint32_t a = (uint32_t)tlast, b = (uint32_t)(tlast >> 32);
-#pragma unroll
-        for (int i = 0; i < 8; i++) v.r[i] = make_uint4(a, b, a, b);
+        r.v[b][8] = px[lane];
+        r.v[b][9] = px[64 + lane];
     }
 }
 
-__device__ inline void seg_store(uint64_t* ring, uint32_t s, uint32_t lane, const SegRegs& v)
+__device__ inline void producer_store(WalkLds& sh, uint32_t s0, uint32_t lane, const SegVecs& r)
 {
-    uint4* p = reinterpret_cast<uint4*>(ring + (s & 3u) * kSeg);
 #pragma unroll
-    for (int i = 0; i < 8; i++) p[64 * i + lane] = v.r[i];
+    for (uint32_t b = 0; b < kBatch; b++) {
+        const uint32_t at = ((s0 + b) % kRingSegs) * kSeg;
+        u32x4* po = reinterpret_cast<u32x4*>(sh.o2 + at);
+        u32x4* pe = reinterpret_cast<u32x4*>(sh.e8 + at);
+        u32x4* px = reinterpret_cast<u32x4*>(sh.x + at);
+#pragma unroll
+        for (uint32_t i = 0; i < 4; i++) {
+            po[64 * i + lane] = r.v[b][i];
+            pe[64 * i + lane] = r.v[b][4 + i];
+        }
+        px[lane] = r.v[b][8];
+        px[64 + lane] = r.v[b][9];
+    }
 }
 
-__global__ void __launch_bounds__(64) crnd_walk_kernel(const LevelJob* __restrict__ jobs)
+// Wave 1: loads batch n + 1 while batch n waits for ring space; stops when
+// the walker says it is done.
+__device__ void walk_producer(const LevelJob& j, WalkLds& sh, uint32_t lane)
 {
-    __shared__ __align__(16) uint64_t ring[4 * kSeg];
-    __shared__ uint16_t eflags[kMaxEGroups];
-    const LevelJob& j = jobs[blockIdx.x];
-    const uint32_t lane = threadIdx.x;
-    const uint64_t* T2 = j.T2;
     const uint32_t tsegs = j.tlen / kSeg;
-    const uint64_t tlast = T2[j.tlen - 1];
-    SegRegs pf;
-    seg_load(T2, tsegs, tlast, 0, lane, pf);
-    seg_store(ring, 0, lane, pf);
-    seg_load(T2, tsegs, tlast, 1, lane, pf);
-    seg_store(ring, 1, lane, pf);
-    seg_load(T2, tsegs, tlast, 2, lane, pf);
-    __syncthreads();
+    SegVecs r;
+    uint32_t s = 0;
+    producer_load(j, tsegs, s, lane, r);
+    for (;;) {
+        for (uint32_t spin = 0;; spin++) {
+            if (uniform(lds_get(&sh.done))) return;
+            if (s + kBatch <= uniform(lds_get(&sh.consumed)) + kRingSegs) break;
+            if (spin >= kSpinCap) return;
+            __builtin_amdgcn_s_sleep(2);
+        }
+        producer_store(sh, s, lane, r);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        lds_put(&sh.ready, s + kBatch);
+        s += kBatch;
+        producer_load(j, tsegs, s, lane, r);
+    }
+}
 
-    uint32_t base = 0;                                 // table position of lane 0 of the window
-    uint64_t w = ring[lane];
-    uint32_t vlo = (uint32_t)w, vhi = (uint32_t)(w >> 32);
-    w = ring[64 + lane];
-    uint32_t nlo = (uint32_t)w, nhi = (uint32_t)(w >> 32);
-    uint32_t o = 0;                                    // current bit - base
-    // Moves the window forward by whole windows until it holds bit base + o.
-    // Each window step checks the segment boundary of its next-window position
-    // once; a single step takes the prefetched window, longer moves (a step
-    // covers at most 288 bits) reload the current window from LDS.
-#define CRND_ADVANCE()                                                         \
-    do {                                                                       \
-        if (o >= 64) {                                                         \
-            const uint32_t k_ = o >> 6;                                        \
-            for (uint32_t i_ = 0; i_ < k_; i_++) {                             \
-                base += 64;                                                    \
-                const uint32_t nb_ = base + 64;                                \
-                if ((nb_ & (kSeg - 1)) == 0) {                                 \
-                    const uint32_t s_ = nb_ / kSeg;                            \
-                    seg_store(ring, s_ + 1, lane, pf);                         \
-                    seg_load(T2, tsegs, tlast, s_ + 2, lane, pf);              \
-                }                                                              \
-            }                                                                  \
-            o -= 64 * k_;                                                      \
-            if (k_ == 1) {                                                     \
-                vlo = nlo;                                                     \
-                vhi = nhi;                                                     \
-            } else {                                                           \
-                const uint64_t c_ = ring[(base & (4 * kSeg - 1)) + lane];      \
-                vlo = (uint32_t)c_;                                            \
-                vhi = (uint32_t)(c_ >> 32);                                    \
-            }                                                                  \
-            const uint64_t w_ = ring[((base + 64) & (4 * kSeg - 1)) + lane];   \
-            nlo = (uint32_t)w_;                                                \
-            nhi = (uint32_t)(w_ >> 32);                                        \
-        }                                                                      \
-    } while (0)
-#define CRND_E2()                                                                          \
-    do {                                                                                   \
-        const uint32_t e_ = (uint32_t)__builtin_amdgcn_readlane((int)vhi, (int)o);         \
-        fw |= ((e_ >> 9) & 15u) << fsh;                                                    \
-        fsh += 4;                                                                          \
-        o += e_ & 511u;                                                                    \
-        CRND_ADVANCE();                                                                    \
-    } while (0)
-#define CRND_O2()                                                                          \
-    do {                                                                                   \
-        const uint32_t e_ = (uint32_t)__builtin_amdgcn_readlane((int)vlo, (int)o);         \
-        o += (e_ >> ((fo & 3u) << 3)) & 255u;                                              \
-        fo >>= 2;                                                                          \
-        CRND_ADVANCE();                                                                    \
-    } while (0)
+// Wave 0.
+__device__ void walk_stream(const LevelJob& j, WalkLds& sh, uint32_t lane, int32_t* status)
+{
+    constexpr uint32_t kMask = kRingPos - 1;
+    uint32_t lim = 0;                 // ring holds table entries [.., lim)
+    bool stuck = false;
+    for (uint32_t spin = 0;; spin++) {
+        lim = uniform(lds_get(&sh.ready)) * kSeg;
+        if (lim >= kNeed) break;
+        if (spin >= kSpinCap) {
+            stuck = true;
+            lim = 0xFFFFFFFFu;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    uint32_t pub = 0;                 // segment last reported to the producer
+    uint32_t vready = lim / kSeg;     // ring fill level read at the last report (vector register, read ahead)
+    uint32_t chk = min(kSeg, lim - kNeed);   // group starts below chk need no ring check
+    // Group start at bit `pos`: report a new segment to the producer, take the
+    // fill level read at the previous report, read the next one ahead, and
+    // wait only if the ring lacks entries the group may need.
+    auto ring_check = [&](uint32_t pos) {
+        const uint32_t seg = pos / kSeg;
+        if (seg != pub) {
+            pub = seg;
+            lds_put(&sh.consumed, seg);
+            lim = max(lim, uniform(vready) * kSeg);
+            vready = lds_get(&sh.ready);
+            __asm__ volatile("" ::: "memory");
+        }
+        if (pos + kNeed > lim) {
+            for (uint32_t spin = 0;; spin++) {
+                lim = uniform(lds_get(&sh.ready)) * kSeg;
+                if (pos + kNeed <= lim) break;
+                if (spin >= kSpinCap) {
+                    stuck = true;
+                    lim = 0xFFFFFFFFu;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        }
+        chk = stuck ? 0xFFFFFFFFu : min((seg + 1) * kSeg, lim - kNeed);
+    };
 
     const uint32_t gw = j.gw, gh = j.gh, faces = j.faces;
     const uint32_t npairs = gw >> 1;
@@ -429,65 +485,109 @@ __global__ void __launch_bounds__(64) crnd_walk_kernel(const LevelJob* __restric
     uint32_t vO = 0, kO = 0, nO = 0;
     guint32* posE = (guint32*)j.posE;
     guint32* posO = (guint32*)j.posO;
+    uint32_t p = 0;                   // current bit
+    uint32_t vflags = 0;              // lane g: odd-row flags of even-row group g of the current row
+    const uint8_t* o2b = reinterpret_cast<const uint8_t*>(sh.o2 + lane);
     for (uint32_t f = 0; f < faces; f++) {
         for (uint32_t yp = 0; yp < gh; yp += 2) {
-            // even row: groups of kGroup pairs, two pairs a step
-            for (uint32_t u0 = 0, g = 0; u0 < npairs; u0 += kGroup, g++) {
-                vE = (lane == kE) ? base + o : vE;
+            // even row: one E8 lookup per full group of eight pairs; lane g
+            // of vflags keeps the odd-row flags of group g
+            uint32_t u0 = 0, g = 0;
+            for (; u0 + kGroup <= npairs; u0 += kGroup, g++) {
+                if (p >= chk) ring_check(p);
+                vE = (lane == kE) ? p : vE;
                 if (++kE == 64) {
                     posE[nE + lane] = vE;
                     nE += 64;
                     kE = 0;
                 }
-                uint32_t fw = 0, fsh = 0;
-                const uint32_t n = min(kGroup, npairs - u0);
-                if (n == kGroup) {
-                    CRND_E2();
-                    CRND_E2();
-                    CRND_E2();
-                    CRND_E2();
-                } else {
-                    for (uint32_t k = 0; k + 2 <= n; k += 2) CRND_E2();
-                    if (n & 1u) {
-                        const uint32_t e_ = (uint32_t)__builtin_amdgcn_readlane((int)vhi, (int)o);
-                        fw |= ((e_ >> 9) & 3u) << fsh;
-                        o += (e_ >> 13) & 255u;
-                        CRND_ADVANCE();
-                    }
-                }
-                if (lane == 0) eflags[g] = (uint16_t)fw;
+                const uint32_t e = uniform(sh.e8[p & kMask]);
+                p += e & 0x7FFu;
+                vflags = (lane == g) ? (e >> 11) : vflags;
             }
-            // odd row: groups of kGroup blocks, two blocks a step
-            uint32_t fv = eflags[0];          // flags of the next even-row group (vector register, prefetched)
-            uint32_t fcur = 0;
-            for (uint32_t x0 = 0, og = 0; x0 < gw; x0 += kGroup, og++) {
-                vO = (lane == kO) ? base + o : vO;
-                if (++kO == 64) {
-                    posO[nO + lane] = vO;
-                    nO += 64;
-                    kO = 0;
+            if (u0 < npairs) {
+                if (p >= chk) ring_check(p);
+                vE = (lane == kE) ? p : vE;
+                if (++kE == 64) {
+                    posE[nE + lane] = vE;
+                    nE += 64;
+                    kE = 0;
                 }
-                if (!(og & 1u)) {
-                    fcur = (uint32_t)__builtin_amdgcn_readfirstlane((int)fv);
-                    fv = eflags[min((og >> 1) + 1, kMaxEGroups - 1)];
+                uint32_t fw = 0;
+                for (uint32_t k = 0; u0 + k < npairs; k++) {
+                    const uint32_t x = uniform(sh.x[p & kMask]);
+                    fw |= (x >> 8) << (2 * k);
+                    p += x & 0xFFu;
                 }
-                uint32_t fo = (og & 1u) ? (fcur >> 8) : (fcur & 0xFFu);
-                if (x0 + kGroup <= gw) {
-                    CRND_O2();
-                    CRND_O2();
-                    CRND_O2();
-                    CRND_O2();
-                } else {
-                    for (uint32_t x = x0; x < gw; x += 2) CRND_O2();
-                }
+                vflags = (lane == g) ? fw : vflags;
             }
+            // odd row: one O2 lookup per two blocks from the lane windows
+            uint32_t base = p & ~63u, o = p & 63u;
+            uint32_t rpb = ((base + 64) & kMask) * 4;   // ring byte offset of the prefetched window
+            uint32_t vcur = sh.o2[(base & kMask) + lane];
+            uint32_t vnxt = *reinterpret_cast<const uint32_t*>(o2b + rpb);
+#define CRND_O2()                                                                       \
+    do {                                                                                \
+        const uint32_t e_ = (uint32_t)__builtin_amdgcn_readlane((int)vcur, (int)o);     \
+        o += (e_ >> ((fo & 3u) << 3)) & 0xFFu;                                          \
+        fo >>= 2;                                                                       \
+        if (o >= 64) {                                                                  \
+            do {                                                                        \
+                base += 64;                                                             \
+                o -= 64;                                                                \
+                vcur = vnxt;                                                            \
+                rpb = (rpb + 256) & (4 * kRingPos - 1);                                 \
+                vnxt = *reinterpret_cast<const uint32_t*>(o2b + rpb);                   \
+            } while (o >= 64);                                                          \
+        }                                                                               \
+    } while (0)
+#define CRND_OGROUP_START(og)                                                           \
+    const uint32_t pos_ = base + o;                                                     \
+    if (pos_ >= chk) ring_check(pos_);                                                  \
+    vO = (lane == kO) ? pos_ : vO;                                                      \
+    if (++kO == 64) {                                                                   \
+        posO[nO + lane] = vO;                                                           \
+        nO += 64;                                                                       \
+        kO = 0;                                                                         \
+    }                                                                                   \
+    uint32_t fo = (uint32_t)__builtin_amdgcn_readlane((int)vflags, (int)((og) >> 1)) >> (((og) & 1u) << 3)
+            const uint32_t nfull = gw / kGroup;
+            uint32_t og = 0;
+            for (; og < nfull; og++) {
+                CRND_OGROUP_START(og);
+                CRND_O2();
+                CRND_O2();
+                CRND_O2();
+                CRND_O2();
+            }
+            if (gw & (kGroup - 1)) {
+                CRND_OGROUP_START(og);
+                for (uint32_t x = og * kGroup; x < gw; x += 2) CRND_O2();
+            }
+#undef CRND_OGROUP_START
+#undef CRND_O2
+            p = base + o;
         }
     }
-#undef CRND_O2
-#undef CRND_E2
-#undef CRND_ADVANCE
     if (lane < kE) posE[nE + lane] = vE;
     if (lane < kO) posO[nO + lane] = vO;
+    lds_put(&sh.done, 1u);
+    if (stuck && lane == 0) atomicExch(status, CRN2_EHIP);
+}
+
+__global__ void __launch_bounds__(128) crnd_walk_kernel(const LevelJob* __restrict__ jobs,
+                                                        const TexInfo* __restrict__ texs)
+{
+    __shared__ __align__(16) WalkLds sh;
+    const LevelJob& j = jobs[blockIdx.x];
+    if (threadIdx.x == 0) {
+        sh.ready = 0;
+        sh.consumed = 0;
+        sh.done = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x >= 64) walk_producer(j, sh, threadIdx.x - 64);
+    else walk_stream(j, sh, threadIdx.x, texs[j.tex].status);
 }
 
 // ------------------------------------------------------ stage 3: symbols
@@ -619,161 +719,194 @@ __global__ void __launch_bounds__(256) crnd_sym_kernel(const LevelJob* __restric
 
 // ------------------------------------------------- stage 4: palette indices
 // Running endpoint indices (crn_decomp.h:3944-4223): reference code 0 adds the
-// decoded delta to the running index (one conditional wrap), 1 repeats it,
-// 2 and 3 take the index of the block above; the running index carries
-// across rows and faces.  One wave per level sweeps the rows in order; within
-// a row each lane owns a run of columns, composes its blocks into one map
-// (add a constant, or set a constant), and a wave prefix scan of the maps
-// gives every lane its incoming index.  A lane's records of the next row are
-// loaded while the current row is processed.
-constexpr uint32_t kIdxRun = 16;              // columns per lane held in registers (rows up to 1024 blocks)
+// decoded delta to the running index with one conditional wrap, 1 repeats it,
+// 2 and 3 take the index of the block above; the running index carries across
+// rows and faces, and the block above row 0 holds index 0.  Every block's
+// index is therefore its parent's plus a constant (mod the palette size): the
+// parent is the previous block in raster order (codes 0, 1) or the block above
+// (2, 3), the root is index 0.  Pointer jumping resolves that forest:
+//   crnd_index_chunk_kernel: per chunk of whole rows (at most kIdxChunk
+//       blocks) in LDS, until every block points at the root or outside the
+//       chunk -- which is always the chunk's row above, the last row of the
+//       previous chunk; that last row is also copied to a compact array;
+//   crnd_index_link_kernel: rounds over the compact last rows, each pointing
+//       one chunk back, until every one points at the root;
+//   crnd_index_final_kernel: one thread per block adds its chunk-level result
+//       to the resolved row it points at, checks ranges and writes its record.
+// With residues kept mod n, a block's index equals the reference's whenever no
+// earlier block went out of range; the first that does (previous index +
+// delta >= 2n after the one wrap, or a selector index past its palette) sees
+// exact values, so the corruption check is exact too.  Records always hold
+// in-range indices, so the block write never reads past a palette.
+constexpr uint32_t kIdxChunk = 4096;
+constexpr uint32_t kRoot = 0xFFFFFFFFu;
 
-__device__ inline uint32_t add_mod(uint32_t a, uint32_t d, uint32_t n)
-{
-    uint32_t t = a + d;
-    if (t >= n) t -= n;
-    if (t >= n) t %= n;
-    return t;
-}
-
-struct IdxState {
+struct IdxSizes {
     uint32_t n0, n1, ns0, ns1;
     bool two;
-    uint32_t v0, v1;          // running indices entering the row (wave-uniform)
-    bool bad;
 };
 
-// Map of one block: R = 1 sets (A0, A1); R = 0 adds (A0, A1).
-__device__ inline void compose_block(const IdxState& s, uint64_t q, uint32_t abv, uint32_t& R, uint32_t& A0,
-                                     uint32_t& A1)
+__device__ inline IdxSizes idx_sizes(const TexInfo& tx)
 {
-    const uint32_t ref = (uint32_t)q & 3u;
-    if (ref == 0) {
-        A0 = add_mod(A0, (uint32_t)(q >> 2) & 0x3FFFu, s.n0);
-        A1 = add_mod(A1, (uint32_t)(q >> 16) & 0x3FFFu, s.n1);
-    } else if (ref >= 2) {
-        R = 1;
-        A0 = abv & 0xFFFFu;
-        A1 = abv >> 16;
-    }
-}
-
-__device__ inline void scan_maps(uint32_t lane, uint32_t n0, uint32_t n1, uint32_t& R, uint32_t& A0, uint32_t& A1)
-{
-    for (uint32_t off = 1; off < 64; off <<= 1) {
-        const uint32_t pR = (uint32_t)__shfl_up((int)R, off);
-        const uint32_t pA0 = (uint32_t)__shfl_up((int)A0, off);
-        const uint32_t pA1 = (uint32_t)__shfl_up((int)A1, off);
-        if (lane >= off && !R) {
-            R = pR;
-            A0 = add_mod(pA0, A0, n0);
-            A1 = add_mod(pA1, A1, n1);
-        }
-    }
-    uint32_t eR = (uint32_t)__shfl_up((int)R, 1);
-    uint32_t eA0 = (uint32_t)__shfl_up((int)A0, 1);
-    uint32_t eA1 = (uint32_t)__shfl_up((int)A1, 1);
-    if (lane == 0) eR = eA0 = eA1 = 0;
-    R = eR;
-    A0 = eA0;
-    A1 = eA1;
-}
-
-// Applies one block: updates the running pair (w0, w1), the above-row entry
-// and writes the record.
-__device__ inline void apply_block(IdxState& s, uint64_t q, uint32_t& abv, uint32_t& w0, uint32_t& w1,
-                                   void* rec, uint64_t o)
-{
-    const uint32_t ref = (uint32_t)q & 3u;
-    const uint32_t s0 = (uint32_t)(q >> 30) & 0x3FFFu, s1 = (uint32_t)(q >> 44) & 0x3FFFu;
-    if (ref == 0) {
-        w0 += (uint32_t)(q >> 2) & 0x3FFFu;
-        if (w0 >= s.n0) w0 -= s.n0;
-        s.bad |= w0 >= s.n0;
-        if (s.two) {
-            w1 += (uint32_t)(q >> 16) & 0x3FFFu;
-            if (w1 >= s.n1) w1 -= s.n1;
-            s.bad |= w1 >= s.n1;
-        }
-    } else if (ref >= 2) {
-        w0 = abv & 0xFFFFu;
-        w1 = abv >> 16;
-    }
-    abv = (w0 & 0xFFFFu) | (w1 << 16);
-    s.bad |= s0 >= s.ns0;
-    if (s.two) {
-        s.bad |= s1 >= s.ns1;
-        static_cast<uint64_t*>(rec)[o] =
-            (uint64_t)((w0 & 0xFFFFu) | (s0 << 16)) | ((uint64_t)((w1 & 0xFFFFu) | (s1 << 16)) << 32);
-    } else {
-        static_cast<uint32_t*>(rec)[o] = (w0 & 0xFFFFu) | (s0 << 16);
-    }
-}
-
-__global__ void __launch_bounds__(64) crnd_index_kernel(const LevelJob* __restrict__ jobs,
-                                                        const TexInfo* __restrict__ texs)
-{
-    extern __shared__ uint32_t above[];   // gw entries: index0 | index1 << 16 (rows wider than 64 * kIdxRun)
-    const LevelJob& j = jobs[blockIdx.x];
-    const TexInfo& tx = texs[j.tex];
+    IdxSizes z;
     const uint32_t kind = tx.kind;
-    IdxState s;
-    s.two = two_channels(kind);
-    s.n0 = kind <= 1 ? tx.n_ce : tx.n_ae;
-    s.n1 = s.two ? tx.n_ae : 1u;
-    s.ns0 = kind <= 1 ? tx.n_cs : tx.n_as;
-    s.ns1 = tx.n_as;
-    s.v0 = s.v1 = 0;
-    s.bad = false;
-    const uint32_t gw = j.gw;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t c = (gw + 63) / 64;
-    const uint32_t x0 = min(gw, lane * c), x1 = min(gw, x0 + c);
-    const uint32_t last_owner = (gw - 1) / c;
-    const uint32_t rows = j.faces * j.gh;
-    if (c <= kIdxRun) {
-        uint64_t q[kIdxRun], qn[kIdxRun];
-        uint32_t abv[kIdxRun];
-#pragma unroll
-        for (uint32_t i = 0; i < kIdxRun; i++) {
-            abv[i] = 0;
-            q[i] = (x0 + i < x1) ? j.pre[x0 + i] : 0;
+    z.two = two_channels(kind);
+    z.n0 = kind <= 1 ? tx.n_ce : tx.n_ae;
+    z.n1 = z.two ? tx.n_ae : 1u;
+    z.ns0 = kind <= 1 ? tx.n_cs : tx.n_as;
+    z.ns1 = z.two ? tx.n_as : 1u;
+    return z;
+}
+
+__device__ inline uint32_t residue(uint32_t d, uint32_t n) { return n ? d % n : 0u; }
+
+// Sum of two (index0, index1) residue pairs, 16 bits each.
+__device__ inline uint32_t add_res(uint32_t a, uint32_t b, uint32_t n0, uint32_t n1)
+{
+    uint32_t x = (a & 0xFFFFu) + (b & 0xFFFFu);
+    if (x >= n0) x -= n0;
+    uint32_t y = (a >> 16) + (b >> 16);
+    if (y >= n1) y -= n1;
+    return x | (y << 16);
+}
+
+__device__ inline uint64_t jump(uint64_t e, uint64_t ep, const IdxSizes& z)
+{
+    return (ep & 0xFFFFFFFFull) | ((uint64_t)add_res((uint32_t)(e >> 32), (uint32_t)(ep >> 32), z.n0, z.n1) << 32);
+}
+
+__global__ void __launch_bounds__(256) crnd_index_chunk_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
+                                                               const TexInfo* __restrict__ texs)
+{
+    __shared__ uint64_t L[2][kIdxChunk];
+    const uint32_t tid = threadIdx.x;
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].wg_idx <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const LevelJob& j = jobs[lo];
+    const IdxSizes z = idx_sizes(texs[j.tex]);
+    const uint32_t gw = j.gw, rows = j.faces * j.gh;
+    const uint32_t c = blockIdx.x - j.wg_idx;
+    const uint32_t r0 = c * j.crows, r1 = min(rows, r0 + j.crows);
+    const uint32_t b0 = r0 * gw, nb = (r1 - r0) * gw;
+    for (uint32_t i = tid; i < nb; i += 256) {
+        const uint32_t b = b0 + i;
+        const uint64_t q = j.pre[b];
+        const uint32_t ref = (uint32_t)q & 3u;
+        uint32_t par, off = 0;
+        if (ref <= 1) {
+            par = b ? b - 1 : kRoot;
+            if (ref == 0)
+                off = residue((uint32_t)(q >> 2) & 0x3FFFu, z.n0) |
+                      (z.two ? residue((uint32_t)(q >> 16) & 0x3FFFu, z.n1) << 16 : 0u);
+        } else {
+            par = b >= gw ? b - gw : kRoot;
         }
-        for (uint32_t r = 0; r < rows; r++) {
-            const uint64_t* nx = j.pre + (uint64_t)(r + 1) * gw;
-#pragma unroll
-            for (uint32_t i = 0; i < kIdxRun; i++) qn[i] = (r + 1 < rows && x0 + i < x1) ? nx[x0 + i] : 0;
-            uint32_t R = 0, A0 = 0, A1 = 0;
-#pragma unroll
-            for (uint32_t i = 0; i < kIdxRun; i++)
-                if (x0 + i < x1) compose_block(s, q[i], abv[i], R, A0, A1);
-            scan_maps(lane, s.n0, s.n1, R, A0, A1);
-            uint32_t w0 = R ? A0 : add_mod(s.v0, A0, s.n0);
-            uint32_t w1 = R ? A1 : add_mod(s.v1, A1, s.n1);
-            const uint64_t ro = (uint64_t)r * gw;
-#pragma unroll
-            for (uint32_t i = 0; i < kIdxRun; i++)
-                if (x0 + i < x1) apply_block(s, q[i], abv[i], w0, w1, j.rec, ro + x0 + i);
-            s.v0 = (uint32_t)__builtin_amdgcn_readlane((int)w0, (int)last_owner);
-            s.v1 = (uint32_t)__builtin_amdgcn_readlane((int)w1, (int)last_owner);
-#pragma unroll
-            for (uint32_t i = 0; i < kIdxRun; i++) q[i] = qn[i];
+        L[0][i] = par | ((uint64_t)off << 32);
+    }
+    __syncthreads();
+    uint32_t cur = 0;
+    for (uint32_t span = 1; span < nb; span <<= 1) {
+        for (uint32_t i = tid; i < nb; i += 256) {
+            uint64_t e = L[cur][i];
+            const uint32_t pa = (uint32_t)e;
+            if (pa != kRoot && pa >= b0) e = jump(e, L[cur][pa - b0], z);
+            L[cur ^ 1][i] = e;
         }
-    } else {
-        for (uint32_t x = x0; x < x1; x++) above[x] = 0;
-        for (uint32_t r = 0; r < rows; r++) {
-            const uint64_t* pr = j.pre + (uint64_t)r * gw;
-            uint32_t R = 0, A0 = 0, A1 = 0;
-            for (uint32_t x = x0; x < x1; x++) compose_block(s, pr[x], above[x], R, A0, A1);
-            scan_maps(lane, s.n0, s.n1, R, A0, A1);
-            uint32_t w0 = R ? A0 : add_mod(s.v0, A0, s.n0);
-            uint32_t w1 = R ? A1 : add_mod(s.v1, A1, s.n1);
-            for (uint32_t x = x0; x < x1; x++) apply_block(s, pr[x], above[x], w0, w1, j.rec, (uint64_t)r * gw + x);
-            s.v0 = (uint32_t)__builtin_amdgcn_readlane((int)w0, (int)last_owner);
-            s.v1 = (uint32_t)__builtin_amdgcn_readlane((int)w1, (int)last_owner);
+        __syncthreads();
+        cur ^= 1;
+    }
+    const uint32_t last = nb - gw;
+    for (uint32_t i = tid; i < nb; i += 256) {
+        const uint64_t e = L[cur][i];
+        j.ent[b0 + i] = e;
+        if (i >= last) {
+            const uint32_t pa = (uint32_t)e;
+            const uint32_t cp = pa == kRoot ? kRoot : (c - 1) * gw + (pa - (r0 - 1) * gw);
+            j.bnd[(uint64_t)c * gw + (i - last)] = cp | (e & 0xFFFFFFFF00000000ull);
         }
     }
-    if (s.bad) atomicExch(tx.status, CRN2_ECORRUPT);
+}
+
+template <typename F>
+__device__ inline uint32_t find_job_u64(const LevelJob* jobs, uint32_t njobs, uint64_t t, F first)
+{
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (first(jobs[mid]) <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One round over the compact last rows: buffer `from` to buffer from ^ 1.
+__global__ void __launch_bounds__(256) crnd_index_link_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
+                                                              const TexInfo* __restrict__ texs, uint64_t total,
+                                                              uint32_t from)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const LevelJob& j = jobs[find_job_u64(jobs, njobs, t, [](const LevelJob& x) { return x.bnd_first; })];
+    const uint64_t k = t - j.bnd_first;
+    const uint64_t n = (uint64_t)j.nchunks * j.gw;
+    const uint64_t* src = j.bnd + from * n;
+    uint64_t* dst = j.bnd + (from ^ 1u) * n;
+    uint64_t e = src[k];
+    const uint32_t pa = (uint32_t)e;
+    if (pa != kRoot) e = jump(e, src[pa], idx_sizes(texs[j.tex]));
+    dst[k] = e;
+}
+
+__global__ void __launch_bounds__(256) crnd_index_final_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
+                                                               const TexInfo* __restrict__ texs, uint64_t total,
+                                                               uint32_t fin)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const LevelJob& j = jobs[find_job_u64(jobs, njobs, t, [](const LevelJob& x) { return x.blk_first; })];
+    const TexInfo& tx = texs[j.tex];
+    const IdxSizes z = idx_sizes(tx);
+    const uint32_t b = (uint32_t)(t - j.blk_first);
+    const uint32_t gw = j.gw;
+    const uint64_t* res = j.bnd + (uint64_t)fin * j.nchunks * gw;
+    auto value = [&](uint32_t blk) -> uint32_t {
+        const uint64_t e = j.ent[blk];
+        const uint32_t pa = (uint32_t)e;
+        uint32_t v = (uint32_t)(e >> 32);
+        if (pa != kRoot) {
+            const uint32_t r = pa / gw;
+            v = add_res(v, (uint32_t)(res[(uint64_t)(r / j.crows) * gw + (pa - r * gw)] >> 32), z.n0, z.n1);
+        }
+        return v;
+    };
+    const uint64_t q = j.pre[b];
+    const uint32_t ref = (uint32_t)q & 3u;
+    const uint32_t v = value(b);
+    bool bad = z.n0 == 0 || z.n1 == 0;
+    if (ref == 0) {
+        const uint32_t pv = b ? value(b - 1) : 0u;
+        bad |= (pv & 0xFFFFu) + ((uint32_t)(q >> 2) & 0x3FFFu) >= 2 * z.n0;
+        if (z.two) bad |= (pv >> 16) + ((uint32_t)(q >> 16) & 0x3FFFu) >= 2 * z.n1;
+    }
+    uint32_t s0 = (uint32_t)(q >> 30) & 0x3FFFu, s1 = (uint32_t)(q >> 44) & 0x3FFFu;
+    if (s0 >= z.ns0) {
+        bad = true;
+        s0 = 0;
+    }
+    if (z.two && s1 >= z.ns1) {
+        bad = true;
+        s1 = 0;
+    }
+    if (z.two)
+        static_cast<uint64_t*>(j.rec)[b] =
+            (uint64_t)((v & 0xFFFFu) | (s0 << 16)) | ((uint64_t)((v >> 16) | (s1 << 16)) << 32);
+    else
+        static_cast<uint32_t*>(j.rec)[b] = (v & 0xFFFFu) | (s0 << 16);
+    if (bad) atomicExch(tx.status, CRN2_ECORRUPT);
 }
 
 struct WriteJob {
@@ -927,8 +1060,10 @@ struct crn2_batch {
     WritePalettes* d_pals = nullptr;
     uint8_t* d_scratch = nullptr;   // transition tables, start bits, symbol records
     uint32_t njobs = 0, nwjobs = 0;
-    uint32_t wg_trans = 0, wg_sym = 0, wg_odd = 0;
+    uint32_t wg_trans = 0, wg_sym = 0, wg_odd = 0, wg_idx = 0;
     uint32_t max_gw = 0;
+    uint32_t link_rounds = 0;                   // rounds of crnd_index_link_kernel
+    uint64_t bnd_total = 0, blk_total = 0;
     uint64_t total_blocks = 0;
     std::vector<int32_t*> status;   // per context
     hipGraphExec_t graph = nullptr; // the plan's launches captured once (crn2_batch_run_graph)
@@ -1115,13 +1250,20 @@ static crn2_unpack_ctx* begin_impl(const void* data, const void* data_dev, uint3
 
     const size_t padded = ((size_t)size + 15) / 16 * 16 + 64;
     if (dmalloc(&c->d_file, padded) != hipSuccess || dmalloc(&c->d_models, sizeof(DevModel) * M_COUNT) != hipSuccess ||
-        dmalloc(&c->d_sorted, sorted.size() * 2) != hipSuccess || dmalloc(&c->d_ce, ce.size() * 4) != hipSuccess ||
-        dmalloc(&c->d_cs, cs.size() * 4) != hipSuccess || dmalloc(&c->d_ae, ae.size() * 2) != hipSuccess ||
-        dmalloc(&c->d_as, as.size() * 2) != hipSuccess || dmalloc(&c->d_status, 4) != hipSuccess ||
+        dmalloc(&c->d_sorted, sorted.size() * 2) != hipSuccess ||
+        dmalloc(&c->d_ce, std::max<size_t>(ce.size(), 1) * 4) != hipSuccess ||
+        dmalloc(&c->d_cs, std::max<size_t>(cs.size(), 1) * 4) != hipSuccess ||
+        dmalloc(&c->d_ae, std::max<size_t>(ae.size(), 1) * 2) != hipSuccess ||
+        dmalloc(&c->d_as, std::max<size_t>(as.size(), 3) * 2) != hipSuccess || dmalloc(&c->d_status, 4) != hipSuccess ||
         dmalloc(&c->d_rec, rec) != hipSuccess)
         return fail();
     bool ok = true;
     ok &= hipMemsetAsync(c->d_file, 0, padded, stream) == hipSuccess;
+    // An empty palette keeps one zero entry: records of a corrupt level still index entry 0.
+    if (ce.empty()) ok &= hipMemsetAsync(c->d_ce, 0, 4, stream) == hipSuccess;
+    if (cs.empty()) ok &= hipMemsetAsync(c->d_cs, 0, 4, stream) == hipSuccess;
+    if (ae.empty()) ok &= hipMemsetAsync(c->d_ae, 0, 2, stream) == hipSuccess;
+    if (as.empty()) ok &= hipMemsetAsync(c->d_as, 0, 6, stream) == hipSuccess;
     if (data_dev)
         ok &= hipMemcpyAsync(c->d_file, data_dev, size, hipMemcpyDeviceToDevice, stream) == hipSuccess;
     else
@@ -1165,9 +1307,10 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
     std::vector<WritePalettes> pals;
     uint64_t total = 0;
     size_t scratch = 0;
-    struct Ofs { size_t T, T2, E, O, G, pre; };
+    struct Ofs { size_t T, O2, E8, X, E, O, G, pre, ent, bnd; };
     std::vector<Ofs> ofs;
-    uint32_t wg_trans = 0, wg_sym = 0, wg_odd = 0, max_gw = 0;
+    uint32_t wg_trans = 0, wg_sym = 0, wg_odd = 0, wg_idx = 0, max_gw = 0, max_chunks = 1;
+    uint64_t bnd_total = 0, blk_total = 0;
     for (uint32_t pi = 0; pi < parts.size(); pi++) {
         PlanPart& P = parts[pi];
         crn2_unpack_ctx* c = P.c;
@@ -1196,7 +1339,8 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
                 j.src = c->d_file + h.level_ofs[l];
                 j.src_size = next - h.level_ofs[l];
             }
-            if ((uint64_t)j.src_size * 8 + 4096 >= 0xFFFFFFFFull || gw > 2 * kGroup * kMaxEGroups) {
+            if ((uint64_t)j.src_size * 8 + 4096 >= 0xFFFFFFFFull || gw > 2 * kGroup * kMaxEGroups ||
+                gw > kIdxChunk || (uint64_t)gw * gh * h.faces >= 0xFFFFFFFFull) {
                 batch_free(B);
                 return CRN2_EUNSUPPORTED;
             }
@@ -1213,11 +1357,25 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
             wg_sym += (uint32_t)((gE + kSymGroupsPerWG - 1) / kSymGroupsPerWG);
             j.wg_odd = wg_odd;
             wg_odd += (uint32_t)((gO + kSymGroupsPerWG - 1) / kSymGroupsPerWG);
+            const uint32_t rows = h.faces * gh;
+            j.crows = kIdxChunk / gw;
+            j.nchunks = (rows + j.crows - 1) / j.crows;
+            j.wg_idx = wg_idx;
+            wg_idx += j.nchunks;
+            j.bnd_first = bnd_total;
+            bnd_total += (uint64_t)j.nchunks * gw;
+            j.blk_first = blk_total;
+            blk_total += (uint64_t)rows * gw;
+            max_chunks = std::max(max_chunks, j.nchunks);
             Ofs o;
             o.T = scratch;
             scratch = align_up(scratch + (size_t)j.tlen * 4, 256);
-            o.T2 = scratch;
-            scratch = align_up(scratch + (size_t)j.tlen * 8, 256);
+            o.O2 = scratch;
+            scratch = align_up(scratch + (size_t)j.tlen * 4, 256);
+            o.E8 = scratch;
+            scratch = align_up(scratch + (size_t)j.tlen * 4, 256);
+            o.X = scratch;
+            scratch = align_up(scratch + (size_t)j.tlen * 2, 256);
             o.E = scratch;
             scratch = align_up(scratch + gE * 4, 256);
             o.O = scratch;
@@ -1226,6 +1384,10 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
             scratch = align_up(scratch + rowpairs * (gw / 2), 256);
             o.pre = scratch;
             scratch = align_up(scratch + rowpairs * 2 * gw * 8, 256);
+            o.ent = scratch;
+            scratch = align_up(scratch + rowpairs * 2 * gw * 8, 256);
+            o.bnd = scratch;
+            scratch = align_up(scratch + (size_t)j.nchunks * gw * 16, 256);
             ofs.push_back(o);
             j.rec = static_cast<uint8_t*>(c->d_rec) + c->rec_ofs[l];
             max_gw = std::max(max_gw, gw);
@@ -1255,7 +1417,12 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
     B->wg_trans = wg_trans;
     B->wg_sym = wg_sym;
     B->wg_odd = wg_odd;
+    B->wg_idx = wg_idx;
     B->max_gw = max_gw;
+    B->bnd_total = bnd_total;
+    B->blk_total = blk_total;
+    B->link_rounds = 0;
+    while ((1u << B->link_rounds) < max_chunks) B->link_rounds++;
     B->total_blocks = total;
     if (dmalloc(&B->d_jobs, jobs.size() * sizeof(LevelJob)) != hipSuccess ||
         dmalloc(&B->d_texs, texs.size() * sizeof(TexInfo)) != hipSuccess ||
@@ -1267,7 +1434,11 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
     }
     for (size_t i = 0; i < jobs.size(); i++) {
         jobs[i].T = reinterpret_cast<uint32_t*>(B->d_scratch + ofs[i].T);
-        jobs[i].T2 = reinterpret_cast<uint64_t*>(B->d_scratch + ofs[i].T2);
+        jobs[i].O2 = reinterpret_cast<uint32_t*>(B->d_scratch + ofs[i].O2);
+        jobs[i].E8 = reinterpret_cast<uint32_t*>(B->d_scratch + ofs[i].E8);
+        jobs[i].X = reinterpret_cast<uint16_t*>(B->d_scratch + ofs[i].X);
+        jobs[i].ent = reinterpret_cast<uint64_t*>(B->d_scratch + ofs[i].ent);
+        jobs[i].bnd = reinterpret_cast<uint64_t*>(B->d_scratch + ofs[i].bnd);
         jobs[i].gvals = B->d_scratch + ofs[i].G;
         jobs[i].posE = reinterpret_cast<uint32_t*>(B->d_scratch + ofs[i].E);
         jobs[i].posO = reinterpret_cast<uint32_t*>(B->d_scratch + ofs[i].O);
@@ -1299,7 +1470,7 @@ static int launch_plan(crn2_batch* B, hipStream_t stream, Profile::Ev* ev)
         hipLaunchKernelGGL(crnd_compose_kernel, dim3(B->wg_trans), dim3(256), 0, stream, B->d_jobs, B->njobs);
         HIPCHK(hipGetLastError());
         if (prof) HIPCHK(hipEventRecord(ev->e[2], stream));
-        hipLaunchKernelGGL(crnd_walk_kernel, dim3(B->njobs), dim3(64), 0, stream, B->d_jobs);
+        hipLaunchKernelGGL(crnd_walk_kernel, dim3(B->njobs), dim3(128), 0, stream, B->d_jobs, B->d_texs);
         HIPCHK(hipGetLastError());
         if (prof) HIPCHK(hipEventRecord(ev->e[3], stream));
         hipLaunchKernelGGL(crnd_sym_kernel<false>, dim3(B->wg_sym), dim3(256), 0, stream, B->d_jobs, B->njobs,
@@ -1310,8 +1481,18 @@ static int launch_plan(crn2_batch* B, hipStream_t stream, Profile::Ev* ev)
                            B->d_texs);
         HIPCHK(hipGetLastError());
         if (prof) HIPCHK(hipEventRecord(ev->e[5], stream));
-        hipLaunchKernelGGL(crnd_index_kernel, dim3(B->njobs), dim3(64), (size_t)B->max_gw * 4, stream, B->d_jobs,
+        hipLaunchKernelGGL(crnd_index_chunk_kernel, dim3(B->wg_idx), dim3(256), 0, stream, B->d_jobs, B->njobs,
                            B->d_texs);
+        HIPCHK(hipGetLastError());
+        const uint32_t lb = (uint32_t)((B->bnd_total + 255) / 256);
+        for (uint32_t r = 0; r < B->link_rounds; r++) {
+            hipLaunchKernelGGL(crnd_index_link_kernel, dim3(lb), dim3(256), 0, stream, B->d_jobs, B->njobs, B->d_texs,
+                               B->bnd_total, r & 1u);
+            HIPCHK(hipGetLastError());
+        }
+        const uint32_t fb = (uint32_t)((B->blk_total + 255) / 256);
+        hipLaunchKernelGGL(crnd_index_final_kernel, dim3(fb), dim3(256), 0, stream, B->d_jobs, B->njobs, B->d_texs,
+                           B->blk_total, B->link_rounds & 1u);
         HIPCHK(hipGetLastError());
         if (prof) HIPCHK(hipEventRecord(ev->e[6], stream));
     } else if (prof) {

@@ -167,3 +167,25 @@ CASES = [
 def case(i):
     seed, w, h, nl, faces, fmt, kw = CASES[i]
     return make_case(seed, w, h, nl, faces, fmt, **kw)
+
+
+def shrink_palette(data, pal, factor, crc16):
+    """Copy of a .crn whose palette `pal` (0 colour endpoints, 1 colour
+    selectors, 2 alpha endpoints, 3 alpha selectors) claims `factor` of its
+    entries, header CRC rewritten: the level streams then reach past the
+    palette (delta wraps and selector indices out of range), which
+    crnd_unpack_level reports as a failure for the levels where it happens
+    (crn_decomp.h:3944-4223)."""
+    d = bytearray(data)
+    o = 39 + 8 * pal
+    n = (d[o] << 8) | d[o + 1]
+    m = max(1, int(n * factor))
+    d[o], d[o + 1] = m >> 8, m & 0xFF
+    hs = (d[2] << 8) | d[3]
+    c = crc16(bytes(d[6:hs]))
+    d[4], d[5] = c >> 8, c & 0xFF
+    return bytes(d)
+
+
+SHRUNK = [(FMT_DXT1, 0, 0.5), (FMT_DXT1, 0, 0.99), (FMT_DXT1, 1, 0.9), (FMT_DXT5, 0, 0.99), (FMT_DXT5, 1, 0.99),
+          (FMT_DXT5, 2, 0.9), (FMT_DXT5, 3, 0.99)]

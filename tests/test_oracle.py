@@ -245,3 +245,23 @@ def test_padded_pitch_keeps_gap_bytes():
     rows = r[0].reshape(2, 48)
     assert np.array_equal(rows[:, :32].reshape(-1), expected[0][0])
     assert (rows[:, 32:] == 0xEE).all()
+
+
+@pytest.mark.parametrize("factor", [0.5, 0.9, 0.99])
+def test_selector_range_check_matches_trace(factor):
+    """With the colour-selector palette cut to m entries, a level fails exactly
+    when one of its blocks uses a selector index >= m; the indices come from
+    the trace of the intact file, the failures from the cut one."""
+    from crunch2_amd._lib import FMT_DXT1 as fmt
+    data = texture.synth_crn((256, 256), 31, fmt)
+    d = crn_cases.shrink_palette(data, 1, factor, oracle.crc16)
+    m = (d[47] << 8) | d[48]
+    assert m < ((data[47] << 8) | data[48])
+    levels = oracle.info(data)["levels"]
+    with oracle.Context(data) as c:
+        used = [int(c.trace_level(l)["cs"].max()) for l in range(levels)]
+    with oracle.Context(d) as c:
+        assert c.h
+        got = [c.unpack_level(l) is not False for l in range(levels)]
+    assert got == [u < m for u in used]
+    assert not all(got)

@@ -179,3 +179,28 @@ def test_batch_plan_mixed_textures_and_graph_replay(gpu):
     assert lib().crn2_batch_destroy(plan) == 0
     for c in ctxs:
         assert crnd.unpack_end(c)
+
+
+@pytest.mark.parametrize("fmt,pal,factor", crn_cases.SHRUNK)
+def test_out_of_range_palette_indices_fail_like_oracle(gpu, fmt, pal, factor):
+    """A palette cut short makes some levels reach past it: each level must
+    fail or succeed as the oracle's does, and match it byte for byte when it
+    succeeds."""
+    say("shrunk palette", fmt, pal, factor)
+    data = crn_cases.shrink_palette(texture.synth_crn((256, 256), 31, fmt), pal, factor, oracle.crc16)
+    levels = crnd.get_texture_info(data)["levels"]
+    ctx = crnd.unpack_begin(data)
+    assert ctx is not None
+    with oracle.Context(data) as oc:
+        assert oc.h
+        fails = 0
+        for l in range(levels):
+            want = oc.unpack_level(l)
+            got = crnd.unpack_level(ctx, l)
+            assert (got is False) == (want is False), l
+            if want is False:
+                fails += 1
+            else:
+                assert np.array_equal(got[0], want[0]), l
+    assert crnd.unpack_end(ctx)
+    assert 0 < fails < levels
