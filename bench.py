@@ -18,8 +18,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-STAGES = ("crnd_trans_kernel", "crnd_compose_kernel", "crnd_walk_kernel", "crnd_sym_kernel<even>",
-          "crnd_sym_kernel<odd>", "crnd_index_kernel", "crnd_write_kernel")
+STAGES = ("crnd_tables_kernel", "crnd_walk_kernel", "crnd_sym_kernel<even>", "crnd_sym_kernel<odd>",
+          "crnd_index_{chunk,link,final}_kernel", "crnd_write_kernel")
 
 
 def log(*a):
@@ -92,10 +92,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=1, help="texture pairs (distinct seeds) per step")
+    ap.add_argument("--batch", type=int, default=1, help="texture pairs (distinct seeds)")
+    ap.add_argument("--copies", type=int, default=16,
+                    help="independent copies of every file per step (separate contexts and outputs), so that one "
+                         "launch sequence moves hundreds of MB")
     ap.add_argument("--mixed", type=int, default=0,
                     help="instead of the 4096^2 pair, a mixed set of this many textures (sides log-uniform in "
                          "[64, 2048], full mips; 60%% DXT1 RGB, 25%% DXT5 RGBA, 15%% DXN normal maps)")
+    ap.add_argument("--streams", type=int, default=2, help="plans (groups of copies) per step, one stream each")
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline sample length")
     ap.add_argument("--encode-size", type=int, default=8192,
@@ -136,34 +140,49 @@ def main():
                 files.append(texture.synth_crn(args.size, 1 + b, fmt))
         log("inputs built in %.1fs: %s bytes" % (time.time() - t0, [len(f) for f in files]))
 
-    ctxs, outs, counts = [], [], []
-    texels = 0
+    ctxs, outs = [], []
+    texels = 0         # of one copy of the files
     crn_bytes = 0
     out_bytes = 0
     for d in files:
-        ctx = crnd.unpack_begin(d)
-        assert ctx is not None
         inf = crnd.get_texture_info(d)
-        n = crnd.levels_size(ctx, 0, inf["levels"])
-        outs.append(torch.empty(n, dtype=torch.uint8, device=dev))
-        ctxs.append(ctx)
-        counts.append(inf["levels"])
         for l in range(inf["levels"]):
             li = crnd.get_level_info(d, l)
             texels += li["blocks_x"] * li["blocks_y"] * 16 * li["faces"]
         crn_bytes += len(d)
-        out_bytes += n
+    for c in range(args.copies):
+        for d in files:
+            ctx = crnd.unpack_begin(d)
+            assert ctx is not None
+            inf = crnd.get_texture_info(d)
+            n = crnd.levels_size(ctx, 0, inf["levels"])
+            outs.append(torch.empty(n, dtype=torch.uint8, device=dev))
+            ctxs.append(ctx)
+            if c == 0:
+                out_bytes += n
 
-    # one plan: every level of every file in one parse launch + one write launch
-    n = len(ctxs)
-    plan = C.c_void_p()
-    rc = lib().crn2_batch_create((C.c_void_p * n)(*[c.handle for c in ctxs]), n,
-                                 (C.c_void_p * n)(*[o.data_ptr() for o in outs]),
-                                 (C.c_uint64 * n)(*[o.numel() for o in outs]), C.byref(plan))
-    assert rc == 0, rc
+    # plans: the copies split into `streams` groups, one plan and one stream
+    # each; a step runs every plan once, the plans overlapping on their streams
+    # (one plan's serial walk beside another's bulk stages)
+    nf = len(files)
+    groups = max(1, min(args.streams, args.copies))
+    per = [(args.copies * (g + 1)) // groups - (args.copies * g) // groups for g in range(groups)]
+    plans, streams = [], []
+    at = 0
+    for g in range(groups):
+        k = per[g] * nf
+        pl = C.c_void_p()
+        rc = lib().crn2_batch_create((C.c_void_p * k)(*[c.handle for c in ctxs[at:at + k]]), k,
+                                     (C.c_void_p * k)(*[o.data_ptr() for o in outs[at:at + k]]),
+                                     (C.c_uint64 * k)(*[o.numel() for o in outs[at:at + k]]), C.byref(pl))
+        assert rc == 0, rc
+        plans.append(pl)
+        streams.append(stream if g == 0 else torch.cuda.Stream(device=dev))
+        at += k
 
     def step():
-        assert lib().crn2_batch_run(plan, C.c_void_p(stream.cuda_stream)) == 0
+        for pl, st in zip(plans, streams):
+            assert lib().crn2_batch_run(pl, C.c_void_p(st.cuda_stream)) == 0
 
     # ---- parity spot check before timing (oracle restatement, first file)
     import oracle
@@ -172,37 +191,54 @@ def main():
     torch.cuda.synchronize()
     exp = np.concatenate([np.concatenate([f.reshape(-1) for f in ref[l]]) for l in sorted(ref)])
     assert np.array_equal(outs[0].cpu().numpy(), exp), "GPU output differs from the oracle"
+    assert np.array_equal(outs[len(outs) - nf].cpu().numpy(), exp), "last copy differs from the oracle"
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-
-    lib().crn2_profile_reset()
-    lib().crn2_profile_enable(1)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
+    t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+
+    # stage times: the first plan alone, profiled (stages one after another)
+    lib().crn2_profile_reset()
+    lib().crn2_profile_enable(1)
+    for _ in range(3):
+        assert lib().crn2_batch_run(plans[0], C.c_void_p(stream.cuda_stream)) == 0
     torch.cuda.synchronize()
     lib().crn2_profile_enable(0)
-    total_ms = e0.elapsed_time(e1)
-    ms = total_ms / args.steps
-
     prof = {}
     for k, name in enumerate(STAGES):
         t = C.c_double()
         n = C.c_uint64()
         lib().crn2_profile_read(k, C.byref(t), C.byref(n))
         prof[name] = t.value / max(1, n.value)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
 
-    mtex = texels / (ms * 1e-3) / 1e6
-    alg_bytes = crn_bytes + out_bytes
-    pipe_ms = sum(prof.values())
-    # algorithmic bytes of one pass of the pipeline (all files of a step): compressed in + DXTn out
-    achieved = alg_bytes / (pipe_ms * 1e-3) / 1e9
+    # latency of one copy alone (a plan over the first copy's contexts)
+    plan1 = C.c_void_p()
+    rc = lib().crn2_batch_create((C.c_void_p * nf)(*[c.handle for c in ctxs[:nf]]), nf,
+                                 (C.c_void_p * nf)(*[o.data_ptr() for o in outs[:nf]]),
+                                 (C.c_uint64 * nf)(*[o.numel() for o in outs[:nf]]), C.byref(plan1))
+    assert rc == 0, rc
+    for _ in range(2):
+        assert lib().crn2_batch_run(plan1, C.c_void_p(stream.cuda_stream)) == 0
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(5):
+        assert lib().crn2_batch_run(plan1, C.c_void_p(stream.cuda_stream)) == 0
+    e1.record(stream)
+    torch.cuda.synchronize()
+    one_ms = e0.elapsed_time(e1) / 5
+    lib().crn2_batch_destroy(plan1)
+
+    mtex = args.copies * texels / (ms * 1e-3) / 1e6
+    alg_bytes = args.copies * (crn_bytes + out_bytes)
+    # algorithmic bytes of one step (all copies): compressed in + DXTn out
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
 
     # ---- CPU baseline: the C restatement on the host cores, one context per
     # level of every file (the reference transcoder is single-threaded per
@@ -259,23 +295,29 @@ def main():
                 "project's writer at crnlib's quality-128 palette sizes)",
         "config": {"workload": ("CRN->DXTn transcode, mixed set of %d textures (64^2..2048^2, full mips, "
                                 "DXT1/DXT5/DXN_XY)" % args.mixed) if args.mixed else
-                               ("CRN->DXTn transcode, %dx%d, 13 levels, DXT1 + DXT5, %d texture pair(s) per step"
-                                % (args.size, args.size, args.batch)),
-                   "crn_bytes": crn_bytes, "dxtn_bytes": out_bytes, "texels": texels},
+                               ("CRN->DXTn transcode, %dx%d, 13 levels, DXT1 + DXT5, %d texture pair(s) x %d "
+                                "copies per step" % (args.size, args.size, args.batch, args.copies)),
+                   "copies": args.copies, "streams": groups, "one_copy_ms": round(one_ms, 3),
+                   "one_copy_mtexels_s": round(texels / (one_ms * 1e-3) / 1e6, 1),
+                   "crn_bytes": args.copies * crn_bytes,
+                   "dxtn_bytes": args.copies * out_bytes, "texels": args.copies * texels},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                     "kernel": "transcode pipeline (7 launches per step)",
-                     "stage_ms": {k: round(v, 4) for k, v in prof.items()}},
+                     "kernel": "transcode pipeline (tables, walk, symbols, indices, write; %d plans of %s "
+                               "copies on %d streams)" % (groups, "/".join(str(x) for x in per), groups),
+                     "stage_ms_one_plan": {k: round(v, 4) for k, v in prof.items()}},
         "cpu_baseline": {"value": round(cpu_mtex, 1), "unit": "Mtexels/s", "cores": threads, "kind": "port",
                          "sample": "%d full passes over the same %d files, %d level jobs on %d threads (%.1fs of "
                                    "wall time); one thread alone: %.1f Mtexels/s"
                                    % (reps, len(files), len(jobs), threads, cpu_t, cpu1_mtex)},
     }
-    assert lib().crn2_batch_status(plan, C.c_void_p(stream.cuda_stream)) == 0
+    for pl in plans:
+        assert lib().crn2_batch_status(pl, C.c_void_p(stream.cuda_stream)) == 0
     if args.encode_size:
         line["encode"] = encode_leg(args.encode_size, args.conf)
     print(json.dumps(line))
-    lib().crn2_batch_destroy(plan)
+    for pl in plans:
+        lib().crn2_batch_destroy(pl)
     for c in ctxs:
         crnd.unpack_end(c)
 

@@ -7,12 +7,11 @@
 // even row above, so where a block starts is only known by following the
 // stream from its first bit.  The pipeline keeps that serial part as short as
 // it can be and runs everything else in parallel:
-//   1. crnd_trans_kernel (all bit positions in parallel): for every bit
-//      position, the bits an even-row block pair or an odd-row block would
-//      take if it started there.
-//   1b. crnd_compose_kernel (all bit positions): the same for eight pairs
-//      (with their odd-row flags) and for two odd-row blocks (four variants by
-//      the two blocks' reference flags).
+//   1. crnd_tables_kernel (all bit positions in parallel): for every bit
+//      position, the bits eight even-row block pairs (with their odd-row
+//      flags), one pair, or two odd-row blocks (four variants by the two
+//      blocks' reference flags) would take if they started there; built in
+//      LDS per tile from per-position code lengths.
 //   2. crnd_walk_kernel (two waves per level): one wave follows the stream
 //      through the composed tables, one lookup per eight pairs or two odd-row
 //      blocks, in scalar registers; the other streams the tables into an LDS
@@ -79,11 +78,10 @@ struct LevelJob {
     uint32_t src_size;
     uint32_t tex;                // index into the TexInfo array
     uint32_t gw, gh, faces;      // even-padded block grid
-    uint32_t tlen;               // transition entries: multiple of kTransPerWG, >= 8 * src_size + 1024
-    uint32_t wg_trans;           // first workgroup of the job in crnd_trans_kernel / crnd_compose_kernel
+    uint32_t tlen;               // table entries: multiple of kTabPerWG, >= 8 * src_size + 1024
+    uint32_t wg_trans;           // first workgroup of the job in crnd_tables_kernel
     uint32_t wg_sym;             // first workgroup of the job in crnd_sym_kernel<false> (even rows)
     uint32_t wg_odd;             // first workgroup of the job in crnd_sym_kernel<true> (odd rows)
-    uint32_t* T;                 // one-unit transitions, one entry per bit position
     uint32_t* O2;                // two odd-row blocks, one entry per bit position
     uint32_t* E8;                // eight even-row pairs, one entry per bit position
     uint16_t* X;                 // one even-row pair, one entry per bit position
@@ -177,21 +175,33 @@ __device__ inline uint32_t code_len(const LenLut& t, uint32_t top)
     return idx < t.nsyms ? L : 0u;
 }
 
-// ------------------------------------------------- stage 1: transition table
-// For every bit position q of a level stream, one 32-bit entry:
-//   bits  0- 7  bits taken by an even-row block pair starting at q: reference
-//               group symbol, then per block the endpoint deltas (when its
-//               reference code is 0) and the selectors
-//   bits  8-15  the reference group symbol at q
-//   bits 16-23  bits taken by an odd-row block at q with a non-zero reference
-//               code (selectors only)
-//   bits 24-31  bits taken by an odd-row block at q with reference code 0
-//               (endpoint deltas, then selectors)
-// A block takes at most 64 bits and a pair at most 144, so each field fits.
-// Positions at or past the end read zero bits, so every entry from the end
-// of the stream on is the same; the table is computed 1024 entries beyond.
-constexpr uint32_t kTransPerWG = 2048;
-constexpr uint32_t kTransWords = kTransPerWG / 32 + 8;
+// ------------------------------------------------ stage 1: transition tables
+// For every bit position q of a level stream (bits at or past the end read as
+// zero, so every entry from the end of the stream on is the same; the tables
+// run at least 1024 entries past it):
+//   O2[q], byte f (f = 0..3): bits taken by two odd-row blocks starting at q
+//          when bit 0 of f says the first carries endpoint deltas and bit 1
+//          the second
+//   E8[q], bits 0-10: bits taken by eight even-row pairs starting at q
+//          bits 11-26: the odd-row flags of those pairs, two per pair (the
+//          block below the pair's first / second block has reference code 0)
+//   X[q],  bits 0-7: bits taken by one even-row pair; bits 8-9: its two flags
+// One workgroup computes kTabPerWG consecutive positions in three LDS passes:
+//   1. per position: the reference group symbol and its code length, and the
+//      code lengths of one block's endpoint deltas and of its selectors
+//      (CW = GL | G << 8 | DL << 16 | SL << 24; two channels: both codes);
+//   2. per position: the one-unit transitions from CW -- bits of an even-row
+//      pair (group symbol, then per block the deltas when its reference code
+//      is 0, then the selectors) and of an odd-row block without / with deltas
+//      (T = pair | G << 8 | odd0 << 16 | odd1 << 24; a pair takes at most 144
+//      bits, a block 64, a code 16);
+//   3. the composed entries above from T (eight pairs reach 1008 entries on).
+// Each pass runs past the tile by what the next one reads.
+constexpr uint32_t kTabPerWG = 4096;
+constexpr uint32_t kTabT = kTabPerWG + 1024;
+constexpr uint32_t kTabCW = kTabT + 160;
+constexpr uint32_t kTabWords = kTabCW / 32 + 3;
+constexpr uint32_t kTabThreads = 512;
 
 __device__ inline uint32_t be_bytes_word(const uint8_t* src, uint32_t size, uint32_t b)
 {
@@ -221,98 +231,86 @@ __device__ inline uint32_t find_job(const LevelJob* jobs, uint32_t njobs, uint32
     return lo;
 }
 
-__global__ void __launch_bounds__(256) crnd_trans_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
-                                                         const TexInfo* __restrict__ texs)
+__device__ inline uint32_t odd_flags(uint32_t g) { return (uint32_t)((g & 0x0Cu) == 0) | ((uint32_t)((g & 0xC0u) == 0) << 1); }
+
+__global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
+                                                                  const TexInfo* __restrict__ texs)
 {
     __shared__ LdsTables s_ref;
     __shared__ LenLut s_len[4];   // M_EP0, M_SEL0, M_EP1, M_SEL1
-    __shared__ uint32_t s_words[kTransWords];
+    __shared__ uint32_t s_words[kTabWords];
+    __shared__ uint32_t s_cw[kTabCW];
+    __shared__ uint32_t s_t[kTabT];
     const uint32_t tid = threadIdx.x;
     const LevelJob& j = jobs[find_job(jobs, njobs, blockIdx.x, false)];
     const TexInfo& tx = texs[j.tex];
     const uint32_t kind = tx.kind;
-    const uint32_t P0 = (blockIdx.x - j.wg_trans) * kTransPerWG;
+    const uint32_t P0 = (blockIdx.x - j.wg_trans) * kTabPerWG;
 
-    stage_tables(s_ref, tx.models[M_REF], tid, 256);
+    stage_tables(s_ref, tx.models[M_REF], tid, kTabThreads);
     for (uint32_t m = 1; m < M_COUNT; m++) {
         if (!kind_uses_dev(kind, m)) continue;
         const DevModel& dm = tx.models[m];
         LenLut& L = s_len[m - 1];
-        for (uint32_t i = tid; i < kLutSize; i += 256) L.len[i] = (uint8_t)(dm.lut[i] >> 16);
+        for (uint32_t i = tid; i < kLutSize; i += kTabThreads) L.len[i] = (uint8_t)(dm.lut[i] >> 16);
         if (tid < 17) {
             L.limit[tid] = dm.limit[tid];
             L.base[tid] = dm.base[tid];
         }
         if (tid == 0) L.nsyms = dm.num_syms;
     }
-    for (uint32_t i = tid; i < kTransWords; i += 256) s_words[i] = be_bytes_word(j.src, j.src_size, P0 / 8 + 4 * i);
+    for (uint32_t i = tid; i < kTabWords; i += kTabThreads) s_words[i] = be_bytes_word(j.src, j.src_size, P0 / 8 + 4 * i);
     __syncthreads();
 
+    // 1. code lengths
     const LenLut& D0 = s_len[d_model0(kind) - 1];
     const LenLut& S0 = s_len[s_model0(kind) - 1];
     const LenLut& D1 = s_len[M_EP1 - 1];
     const LenLut& S1 = s_len[M_SEL1 - 1];
     const bool two = two_channels(kind);
     const uint16_t* ref_sorted = tx.sorted + M_REF * kMaxSyms;
-    auto dlen = [&](uint32_t r) -> uint32_t {
-        uint32_t l = code_len(D0, peek16(s_words, r));
-        if (two) l += code_len(D1, peek16(s_words, r + l));
-        return l;
-    };
-    auto slen = [&](uint32_t r) -> uint32_t {
-        uint32_t l = code_len(S0, peek16(s_words, r));
-        if (two) l += code_len(S1, peek16(s_words, r + l));
-        return l;
-    };
-    for (uint32_t k = 0; k < kTransPerWG / 256; k++) {
-        const uint32_t r = tid + 256 * k;
+    for (uint32_t r = tid; r < kTabCW; r += kTabThreads) {
+        const uint32_t top = peek16(s_words, r);
         uint32_t gl;
-        const uint32_t G = decode_top(s_ref, ref_sorted, peek16(s_words, r), gl) & 0xFFu;
-        uint32_t e = r + gl;
-        if ((G & 3u) == 0) e += dlen(e);
-        e += slen(e);
-        if (((G >> 4) & 3u) == 0) e += dlen(e);
-        e += slen(e);
-        const uint32_t o0 = slen(r);
-        const uint32_t d = dlen(r);
-        const uint32_t o1 = d + slen(r + d);
-        j.T[P0 + r] = (e - r) | (G << 8) | (o0 << 16) | (o1 << 24);
+        const uint32_t G = decode_top(s_ref, ref_sorted, top, gl) & 0xFFu;
+        uint32_t dl = code_len(D0, top);
+        if (two) dl += code_len(D1, peek16(s_words, r + dl));
+        uint32_t sl = code_len(S0, top);
+        if (two) sl += code_len(S1, peek16(s_words, r + sl));
+        s_cw[r] = gl | (G << 8) | (dl << 16) | (sl << 24);
     }
-}
+    __syncthreads();
 
-// ------------------------------------------- stage 1b: composed transitions
-// For every bit position q, from the one-unit table:
-//   O2[q], byte f (f = 0..3): bits taken by two odd-row blocks starting at q
-//          when bit 0 of f says the first carries endpoint deltas and bit 1
-//          the second
-//   E8[q], bits 0-10: bits taken by eight even-row pairs starting at q
-//          bits 11-26: the odd-row flags of those pairs, two per pair (the
-//          block below the pair's first / second block has reference code 0)
-//   X[q],  bits 0-7: bits taken by one even-row pair; bits 8-9: its two flags
-// Eight pairs take at most 8 * 144 = 1152 bits, two odd blocks at most 128.
-__device__ inline uint32_t odd_flags(uint32_t g) { return (uint32_t)((g & 0x0Cu) == 0) | ((uint32_t)((g & 0xC0u) == 0) << 1); }
+    // 2. one-unit transitions
+    for (uint32_t r = tid; r < kTabT; r += kTabThreads) {
+        const uint32_t c = s_cw[r];
+        const uint32_t G = (c >> 8) & 0xFFu;
+        uint32_t e = r + (c & 0xFFu);
+        if ((G & 3u) == 0) e += (s_cw[e] >> 16) & 0xFFu;
+        e += s_cw[e] >> 24;
+        if (((G >> 4) & 3u) == 0) e += (s_cw[e] >> 16) & 0xFFu;
+        e += s_cw[e] >> 24;
+        const uint32_t d = (c >> 16) & 0xFFu;
+        s_t[r] = (e - r) | (G << 8) | ((c >> 24) << 16) | ((d + (s_cw[r + d] >> 24)) << 24);
+    }
+    __syncthreads();
 
-__global__ void __launch_bounds__(256) crnd_compose_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs)
-{
-    const LevelJob& j = jobs[find_job(jobs, njobs, blockIdx.x, false)];
-    const uint32_t P0 = (blockIdx.x - j.wg_trans) * kTransPerWG;
-    const uint32_t* T = j.T;
-    const uint32_t last = j.tlen - 1;
-    for (uint32_t k = 0; k < kTransPerWG / 256; k++) {
-        const uint32_t q = P0 + threadIdx.x + 256 * k;
-        const uint32_t t0 = T[q];
+    // 3. composed entries
+    for (uint32_t r = tid; r < kTabPerWG; r += kTabThreads) {
+        const uint32_t t0 = s_t[r];
         const uint32_t o0 = (t0 >> 16) & 0xFFu, o1 = t0 >> 24;
-        const uint32_t ta = T[min(q + o0, last)], tb = T[min(q + o1, last)];
+        const uint32_t ta = s_t[r + o0], tb = s_t[r + o1];
+        const uint32_t q = P0 + r;
         j.O2[q] = (o0 + ((ta >> 16) & 0xFFu)) | ((o1 + ((tb >> 16) & 0xFFu)) << 8) | ((o0 + (ta >> 24)) << 16) |
                   ((o1 + (tb >> 24)) << 24);
-        uint32_t p = q + (t0 & 0xFFu), fl = odd_flags((t0 >> 8) & 0xFFu);
+        uint32_t p = r + (t0 & 0xFFu), fl = odd_flags((t0 >> 8) & 0xFFu);
 #pragma unroll
         for (uint32_t i = 1; i < 8; i++) {
-            const uint32_t t = T[min(p, last)];
+            const uint32_t t = s_t[p];
             fl |= odd_flags((t >> 8) & 0xFFu) << (2 * i);
             p += t & 0xFFu;
         }
-        j.E8[q] = (p - q) | (fl << 11);
+        j.E8[q] = (p - r) | (fl << 11);
         j.X[q] = (uint16_t)((t0 & 0xFFu) | (odd_flags((t0 >> 8) & 0xFFu) << 8));
     }
 }
@@ -360,6 +358,12 @@ struct WalkLds {
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const u32x4 gconst_u32x4;
 typedef __attribute__((address_space(1))) uint32_t guint32;
+
+// Two-bit flag pairs x = f0 | f1 << 2 | f2 << 4 | f3 << 6 -> bytes f_k * 8.
+__device__ inline uint32_t spread_flags(uint32_t x)
+{
+    return ((x & 3u) | ((x & 0xCu) << 6) | ((x & 0x30u) << 12) | ((x & 0xC0u) << 18)) << 3;
+}
 
 __device__ inline uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 __device__ inline uint32_t lds_get(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
@@ -417,6 +421,7 @@ __device__ void walk_producer(const LevelJob& j, WalkLds& sh, uint32_t lane)
     uint32_t s = 0;
     producer_load(j, tsegs, s, lane, r);
     for (;;) {
+#pragma unroll 1
         for (uint32_t spin = 0;; spin++) {
             if (uniform(lds_get(&sh.done))) return;
             if (s + kBatch <= uniform(lds_get(&sh.consumed)) + kRingSegs) break;
@@ -437,7 +442,8 @@ __device__ void walk_stream(const LevelJob& j, WalkLds& sh, uint32_t lane, int32
     constexpr uint32_t kMask = kRingPos - 1;
     uint32_t lim = 0;                 // ring holds table entries [.., lim)
     bool stuck = false;
-    for (uint32_t spin = 0;; spin++) {
+#pragma unroll 1
+        for (uint32_t spin = 0;; spin++) {
         lim = uniform(lds_get(&sh.ready)) * kSeg;
         if (lim >= kNeed) break;
         if (spin >= kSpinCap) {
@@ -464,7 +470,8 @@ __device__ void walk_stream(const LevelJob& j, WalkLds& sh, uint32_t lane, int32
             __asm__ volatile("" ::: "memory");
         }
         if (pos + kNeed > lim) {
-            for (uint32_t spin = 0;; spin++) {
+#pragma unroll 1
+        for (uint32_t spin = 0;; spin++) {
                 lim = uniform(lds_get(&sh.ready)) * kSeg;
                 if (pos + kNeed <= lim) break;
                 if (spin >= kSpinCap) {
@@ -481,38 +488,30 @@ __device__ void walk_stream(const LevelJob& j, WalkLds& sh, uint32_t lane, int32
 
     const uint32_t gw = j.gw, gh = j.gh, faces = j.faces;
     const uint32_t npairs = gw >> 1;
-    uint32_t vE = 0, kE = 0, nE = 0;
-    uint32_t vO = 0, kO = 0, nO = 0;
+    const uint32_t gE = (npairs + kGroup - 1) / kGroup;   // even-row groups per row (at most 64)
+    const uint32_t gO = (gw + kGroup - 1) / kGroup;       // odd-row groups per row (at most 128)
     guint32* posE = (guint32*)j.posE;
     guint32* posO = (guint32*)j.posO;
     uint32_t p = 0;                   // current bit
-    uint32_t vflags = 0;              // lane g: odd-row flags of even-row group g of the current row
     const uint8_t* o2b = reinterpret_cast<const uint8_t*>(sh.o2 + lane);
+    uint32_t rowpair = 0;
     for (uint32_t f = 0; f < faces; f++) {
-        for (uint32_t yp = 0; yp < gh; yp += 2) {
+        for (uint32_t yp = 0; yp < gh; yp += 2, rowpair++) {
             // even row: one E8 lookup per full group of eight pairs; lane g
-            // of vflags keeps the odd-row flags of group g
+            // keeps group g's start bit (vE) and odd-row flags (vflags)
+            uint32_t vE = 0, vflags = 0;
             uint32_t u0 = 0, g = 0;
             for (; u0 + kGroup <= npairs; u0 += kGroup, g++) {
-                if (p >= chk) ring_check(p);
-                vE = (lane == kE) ? p : vE;
-                if (++kE == 64) {
-                    posE[nE + lane] = vE;
-                    nE += 64;
-                    kE = 0;
-                }
+                if (__builtin_expect(p >= chk, 0)) ring_check(p);
+                const bool mine = lane == g;
+                vE = mine ? p : vE;
                 const uint32_t e = uniform(sh.e8[p & kMask]);
                 p += e & 0x7FFu;
-                vflags = (lane == g) ? (e >> 11) : vflags;
+                vflags = mine ? (e >> 11) : vflags;
             }
             if (u0 < npairs) {
-                if (p >= chk) ring_check(p);
-                vE = (lane == kE) ? p : vE;
-                if (++kE == 64) {
-                    posE[nE + lane] = vE;
-                    nE += 64;
-                    kE = 0;
-                }
+                if (__builtin_expect(p >= chk, 0)) ring_check(p);
+                vE = (lane == g) ? p : vE;
                 uint32_t fw = 0;
                 for (uint32_t k = 0; u0 + k < npairs; k++) {
                     const uint32_t x = uniform(sh.x[p & kMask]);
@@ -521,16 +520,22 @@ __device__ void walk_stream(const LevelJob& j, WalkLds& sh, uint32_t lane, int32
                 }
                 vflags = (lane == g) ? fw : vflags;
             }
-            // odd row: one O2 lookup per two blocks from the lane windows
+            if (lane < gE) posE[rowpair * gE + lane] = vE;
+            // odd row: one O2 lookup per two blocks from the lane windows.  Lane
+            // g holds, for odd groups 2g (vFlo) and 2g + 1 (vFhi), byte k =
+            // (flags of the step's two blocks) * 8: the entry's byte offset.
+            const uint32_t vFlo = spread_flags(vflags & 0xFFu), vFhi = spread_flags((vflags >> 8) & 0xFFu);
             uint32_t base = p & ~63u, o = p & 63u;
             uint32_t rpb = ((base + 64) & kMask) * 4;   // ring byte offset of the prefetched window
             uint32_t vcur = sh.o2[(base & kMask) + lane];
             uint32_t vnxt = *reinterpret_cast<const uint32_t*>(o2b + rpb);
-#define CRND_O2()                                                                       \
+            // Both windows in place before the loop, so that the loop's own
+            // waits cover only the prefetch issued by a window advance.
+            __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+#define CRND_O2(FO, K)                                                                  \
     do {                                                                                \
         const uint32_t e_ = (uint32_t)__builtin_amdgcn_readlane((int)vcur, (int)o);     \
-        o += (e_ >> ((fo & 3u) << 3)) & 0xFFu;                                          \
-        fo >>= 2;                                                                       \
+        o += (e_ >> (((FO) >> (8 * (K))) & 31u)) & 0xFFu;                               \
         if (o >= 64) {                                                                  \
             do {                                                                        \
                 base += 64;                                                             \
@@ -541,36 +546,53 @@ __device__ void walk_stream(const LevelJob& j, WalkLds& sh, uint32_t lane, int32
             } while (o >= 64);                                                          \
         }                                                                               \
     } while (0)
-#define CRND_OGROUP_START(og)                                                           \
+#define CRND_OGROUP_START(VF, VO, H)                                                    \
     const uint32_t pos_ = base + o;                                                     \
-    if (pos_ >= chk) ring_check(pos_);                                                  \
-    vO = (lane == kO) ? pos_ : vO;                                                      \
-    if (++kO == 64) {                                                                   \
-        posO[nO + lane] = vO;                                                           \
-        nO += 64;                                                                       \
-        kO = 0;                                                                         \
-    }                                                                                   \
-    uint32_t fo = (uint32_t)__builtin_amdgcn_readlane((int)vflags, (int)((og) >> 1)) >> (((og) & 1u) << 3)
-            const uint32_t nfull = gw / kGroup;
-            uint32_t og = 0;
-            for (; og < nfull; og++) {
-                CRND_OGROUP_START(og);
-                CRND_O2();
-                CRND_O2();
-                CRND_O2();
-                CRND_O2();
+    if (__builtin_expect(pos_ >= chk, 0)) ring_check(pos_);                                                  \
+    VO = (lane == (H)) ? pos_ : VO;                                                     \
+    const uint32_t fo_ = (uint32_t)__builtin_amdgcn_readlane((int)(VF), (int)(H))
+            uint32_t vOa = 0, vOb = 0;   // lane h: start bits of odd groups 2h, 2h + 1
+            const uint32_t n16 = gw / 16;
+            uint32_t h = 0;
+            for (; h < n16; h++) {
+                {
+                    CRND_OGROUP_START(vFlo, vOa, h);
+                    CRND_O2(fo_, 0);
+                    CRND_O2(fo_, 1);
+                    CRND_O2(fo_, 2);
+                    CRND_O2(fo_, 3);
+                }
+                {
+                    CRND_OGROUP_START(vFhi, vOb, h);
+                    CRND_O2(fo_, 0);
+                    CRND_O2(fo_, 1);
+                    CRND_O2(fo_, 2);
+                    CRND_O2(fo_, 3);
+                }
             }
-            if (gw & (kGroup - 1)) {
-                CRND_OGROUP_START(og);
-                for (uint32_t x = og * kGroup; x < gw; x += 2) CRND_O2();
+            uint32_t rem = gw - 16 * n16;    // blocks left: 0..14, even
+            if (rem >= kGroup) {
+                CRND_OGROUP_START(vFlo, vOa, h);
+                CRND_O2(fo_, 0);
+                CRND_O2(fo_, 1);
+                CRND_O2(fo_, 2);
+                CRND_O2(fo_, 3);
+                rem -= kGroup;
+                if (rem) {
+                    CRND_OGROUP_START(vFhi, vOb, h);
+                    for (uint32_t k = 0; 2 * k < rem; k++) CRND_O2(fo_, k);
+                }
+            } else if (rem) {
+                CRND_OGROUP_START(vFlo, vOa, h);
+                for (uint32_t k = 0; 2 * k < rem; k++) CRND_O2(fo_, k);
             }
 #undef CRND_OGROUP_START
 #undef CRND_O2
+            if (2 * lane < gO) posO[rowpair * gO + 2 * lane] = vOa;
+            if (2 * lane + 1 < gO) posO[rowpair * gO + 2 * lane + 1] = vOb;
             p = base + o;
         }
     }
-    if (lane < kE) posE[nE + lane] = vE;
-    if (lane < kO) posO[nO + lane] = vO;
     lds_put(&sh.done, 1u);
     if (stuck && lane == 0) atomicExch(status, CRN2_EHIP);
 }
@@ -998,7 +1020,7 @@ __global__ void __launch_bounds__(256) crnd_write_kernel(const WriteJob* __restr
 }
 
 // ----------------------------------------------------------------- profiling
-constexpr int kStages = 7;   // trans, compose, walk, sym (even rows), sym (odd rows), index, write
+constexpr int kStages = 6;   // tables, walk, sym (even rows), sym (odd rows), index, write
 struct Profile {
     std::mutex mu;
     bool on = false;
@@ -1307,7 +1329,7 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
     std::vector<WritePalettes> pals;
     uint64_t total = 0;
     size_t scratch = 0;
-    struct Ofs { size_t T, O2, E8, X, E, O, G, pre, ent, bnd; };
+    struct Ofs { size_t O2, E8, X, E, O, G, pre, ent, bnd; };
     std::vector<Ofs> ofs;
     uint32_t wg_trans = 0, wg_sym = 0, wg_odd = 0, wg_idx = 0, max_gw = 0, max_chunks = 1;
     uint64_t bnd_total = 0, blk_total = 0;
@@ -1348,9 +1370,9 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
             j.gw = gw;
             j.gh = gh;
             j.faces = h.faces;
-            j.tlen = (uint32_t)align_up((size_t)j.src_size * 8 + 1024, kTransPerWG);
+            j.tlen = (uint32_t)align_up((size_t)j.src_size * 8 + 1024, kTabPerWG);
             j.wg_trans = wg_trans;
-            wg_trans += j.tlen / kTransPerWG;
+            wg_trans += j.tlen / kTabPerWG;
             const uint64_t rowpairs = (uint64_t)h.faces * (gh / 2);
             const uint64_t gE = rowpairs * ((gw / 2 + kGroup - 1) / kGroup), gO = rowpairs * ((gw + kGroup - 1) / kGroup);
             j.wg_sym = wg_sym;
@@ -1368,8 +1390,6 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
             blk_total += (uint64_t)rows * gw;
             max_chunks = std::max(max_chunks, j.nchunks);
             Ofs o;
-            o.T = scratch;
-            scratch = align_up(scratch + (size_t)j.tlen * 4, 256);
             o.O2 = scratch;
             scratch = align_up(scratch + (size_t)j.tlen * 4, 256);
             o.E8 = scratch;
@@ -1433,7 +1453,6 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
         return CRN2_ENOMEM;
     }
     for (size_t i = 0; i < jobs.size(); i++) {
-        jobs[i].T = reinterpret_cast<uint32_t*>(B->d_scratch + ofs[i].T);
         jobs[i].O2 = reinterpret_cast<uint32_t*>(B->d_scratch + ofs[i].O2);
         jobs[i].E8 = reinterpret_cast<uint32_t*>(B->d_scratch + ofs[i].E8);
         jobs[i].X = reinterpret_cast<uint16_t*>(B->d_scratch + ofs[i].X);
@@ -1464,23 +1483,21 @@ static int launch_plan(crn2_batch* B, hipStream_t stream, Profile::Ev* ev)
     const bool prof = ev != nullptr;
     if (prof) HIPCHK(hipEventRecord(ev->e[0], stream));
     if (B->njobs) {
-        hipLaunchKernelGGL(crnd_trans_kernel, dim3(B->wg_trans), dim3(256), 0, stream, B->d_jobs, B->njobs, B->d_texs);
+        hipLaunchKernelGGL(crnd_tables_kernel, dim3(B->wg_trans), dim3(kTabThreads), 0, stream, B->d_jobs, B->njobs,
+                           B->d_texs);
         HIPCHK(hipGetLastError());
         if (prof) HIPCHK(hipEventRecord(ev->e[1], stream));
-        hipLaunchKernelGGL(crnd_compose_kernel, dim3(B->wg_trans), dim3(256), 0, stream, B->d_jobs, B->njobs);
-        HIPCHK(hipGetLastError());
-        if (prof) HIPCHK(hipEventRecord(ev->e[2], stream));
         hipLaunchKernelGGL(crnd_walk_kernel, dim3(B->njobs), dim3(128), 0, stream, B->d_jobs, B->d_texs);
         HIPCHK(hipGetLastError());
-        if (prof) HIPCHK(hipEventRecord(ev->e[3], stream));
+        if (prof) HIPCHK(hipEventRecord(ev->e[2], stream));
         hipLaunchKernelGGL(crnd_sym_kernel<false>, dim3(B->wg_sym), dim3(256), 0, stream, B->d_jobs, B->njobs,
                            B->d_texs);
         HIPCHK(hipGetLastError());
-        if (prof) HIPCHK(hipEventRecord(ev->e[4], stream));
+        if (prof) HIPCHK(hipEventRecord(ev->e[3], stream));
         hipLaunchKernelGGL(crnd_sym_kernel<true>, dim3(B->wg_odd), dim3(256), 0, stream, B->d_jobs, B->njobs,
                            B->d_texs);
         HIPCHK(hipGetLastError());
-        if (prof) HIPCHK(hipEventRecord(ev->e[5], stream));
+        if (prof) HIPCHK(hipEventRecord(ev->e[4], stream));
         hipLaunchKernelGGL(crnd_index_chunk_kernel, dim3(B->wg_idx), dim3(256), 0, stream, B->d_jobs, B->njobs,
                            B->d_texs);
         HIPCHK(hipGetLastError());
@@ -1494,7 +1511,7 @@ static int launch_plan(crn2_batch* B, hipStream_t stream, Profile::Ev* ev)
         hipLaunchKernelGGL(crnd_index_final_kernel, dim3(fb), dim3(256), 0, stream, B->d_jobs, B->njobs, B->d_texs,
                            B->blk_total, B->link_rounds & 1u);
         HIPCHK(hipGetLastError());
-        if (prof) HIPCHK(hipEventRecord(ev->e[6], stream));
+        if (prof) HIPCHK(hipEventRecord(ev->e[5], stream));
     } else if (prof) {
         for (int k = 1; k < kStages; k++) HIPCHK(hipEventRecord(ev->e[k], stream));
     }

@@ -386,13 +386,12 @@ __device__ inline uint32_t pair_error(const int (*px)[3], const int* a, const in
 }
 
 // +-1 single-channel steps from endpoints e while the exact error drops.
-__device__ inline uint32_t refine_from(const int (*px)[3], int (*e)[3], uint32_t max_iters, uint32_t& steps)
+// Returns true if it stopped at a point no single step improves.
+__device__ inline bool descend(const int (*px)[3], int (*e)[3], uint32_t& best, uint32_t max_iters, uint32_t& steps)
 {
     const int lim[3] = {31, 63, 31};
-    uint32_t lo, hi;
-    uint32_t best = pair_error(px, e[0], e[1], lo, hi);
-    uint32_t it = 0;
-    for (; it < max_iters; it++) {
+    for (;;) {
+        if (steps >= max_iters) return false;
         uint32_t cand = best;
         int bs = -1, bk = 0, bd = 0;
         for (int s = 0; s < 2; s++)
@@ -411,19 +410,157 @@ __device__ inline uint32_t refine_from(const int (*px)[3], int (*e)[3], uint32_t
                         bd = d;
                     }
                 }
-        if (bs < 0) break;
+        if (bs < 0) return true;
         e[bs][bk] += bd;
         best = cand;
+        steps++;
     }
-    steps += it;
+}
+
+// Least-squares endpoints for the selectors of the four-colour block (lo, hi)
+// (colour i = lo + w_i (hi - lo), w in {0, 1/3, 2/3, 1}), per channel in
+// 8-bit space, rounded to 5:6:5.  False if the selectors use one weight only.
+__device__ inline bool lsq_endpoints(const int (*px)[3], uint32_t lo, uint32_t hi, int (*e)[3])
+{
+    uint32_t sel;
+    dxt1_error(px, lo, hi, &sel);
+    const int u_of[4] = {0, 3, 1, 2};   // 3 w by selector
+    int a11 = 0, a12 = 0, a22 = 0, b1[3] = {0, 0, 0}, b2[3] = {0, 0, 0};
+    for (int i = 0; i < 16; i++) {
+        const int u = u_of[(sel >> (2 * i)) & 3u], v = 3 - u;
+        a11 += v * v;
+        a12 += u * v;
+        a22 += u * u;
+        for (int k = 0; k < 3; k++) {
+            b1[k] += v * px[i][k];
+            b2[k] += u * px[i][k];
+        }
+    }
+    const float det = (float)a11 * (float)a22 - (float)a12 * (float)a12;
+    if (det <= 0.0f) return false;
+    const float lim[3] = {31.0f, 63.0f, 31.0f};
+    for (int k = 0; k < 3; k++) {
+        const float A = 3.0f * ((float)a22 * (float)b1[k] - (float)a12 * (float)b2[k]) / det;
+        const float B = 3.0f * ((float)a11 * (float)b2[k] - (float)a12 * (float)b1[k]) / det;
+        e[0][k] = (int)fminf(fmaxf(rintf(A * lim[k] / 255.0f), 0.0f), lim[k]);
+        e[1][k] = (int)fminf(fmaxf(rintf(B * lim[k] / 255.0f), 0.0f), lim[k]);
+    }
+    return true;
+}
+
+// Descent, then while the least-squares refit of the result's selectors
+// lowers the error, descent again from the refit.
+__device__ inline uint32_t refine_from(const int (*px)[3], int (*e)[3], uint32_t max_iters, bool refit,
+                                       uint32_t& steps)
+{
+    uint32_t lo, hi;
+    uint32_t best = pair_error(px, e[0], e[1], lo, hi);
+    uint32_t used = 0;
+    for (int round = 0;; round++) {
+        if (!descend(px, e, best, max_iters, used) || !refit || round == 3) break;
+        pair_error(px, e[0], e[1], lo, hi);
+        if (lo == hi) break;
+        int f[2][3];
+        if (!lsq_endpoints(px, lo, hi, f)) break;
+        const uint32_t ef = pair_error(px, f[0], f[1], lo, hi);
+        if (ef >= best) break;
+        best = ef;
+        for (int s = 0; s < 2; s++)
+            for (int k = 0; k < 3; k++) e[s][k] = f[s][k];
+    }
+    steps += used;
     return best;
 }
 
-// Start points: bit 0 of `starts` = the network's proposal y (8 floats per
-// block, components scaled to [0, 1]); bit 1 = the block's brightest and
-// darkest texels (luma 2r + 4g + b, first extreme).  Each start is refined and
-// the lower final error kept (the network's on ties).
-// out per block (16 bytes): u32 c0 | c1 << 16, u32 selectors, u32 error, u32 steps
+// Best improving move among all 728 combinations of -1 / 0 / +1 on the six
+// endpoint components (refine_solution's both-ends lattice,
+// crnlib/crn_dxt1.cpp:601-643); false if none improves.
+__device__ inline bool box_step(const int (*px)[3], int (*e)[3], uint32_t& best)
+{
+    const int lim[3] = {31, 63, 31};
+    uint32_t cand = best;
+    int bm = -1;
+    for (int m = 0; m < 729; m++) {
+        if (m == 364) continue;   // all zero
+        int f[2][3];
+        int r = m;
+        bool ok = true;
+        for (int s = 0; s < 2; s++)
+            for (int k = 0; k < 3; k++) {
+                const int v = e[s][k] + (r % 3) - 1;
+                r /= 3;
+                ok &= v >= 0 && v <= lim[k];
+                f[s][k] = v;
+            }
+        if (!ok) continue;
+        uint32_t lo, hi;
+        const uint32_t er = pair_error(px, f[0], f[1], lo, hi);
+        if (er < cand) {
+            cand = er;
+            bm = m;
+        }
+    }
+    if (bm < 0) return false;
+    for (int s = 0; s < 2; s++)
+        for (int k = 0; k < 3; k++) {
+            e[s][k] += (bm % 3) - 1;
+            bm /= 3;
+        }
+    best = cand;
+    return true;
+}
+
+// Principal-axis start: the extreme projections of the texels on the
+// dominant eigenvector of their covariance (power iteration).
+__device__ inline void pca_start(const int (*px)[3], int (*e)[3])
+{
+    float m[3] = {0, 0, 0};
+    for (int i = 0; i < 16; i++)
+        for (int k = 0; k < 3; k++) m[k] += (float)px[i][k];
+    for (int k = 0; k < 3; k++) m[k] *= 1.0f / 16.0f;
+    float c[6] = {0, 0, 0, 0, 0, 0};   // xx xy xz yy yz zz
+    for (int i = 0; i < 16; i++) {
+        const float x = px[i][0] - m[0], y = px[i][1] - m[1], z = px[i][2] - m[2];
+        c[0] += x * x;
+        c[1] += x * y;
+        c[2] += x * z;
+        c[3] += y * y;
+        c[4] += y * z;
+        c[5] += z * z;
+    }
+    float v[3] = {1.0f, 1.0f, 1.0f};
+    for (int it = 0; it < 8; it++) {
+        const float a = c[0] * v[0] + c[1] * v[1] + c[2] * v[2];
+        const float b = c[1] * v[0] + c[3] * v[1] + c[4] * v[2];
+        const float d = c[2] * v[0] + c[4] * v[1] + c[5] * v[2];
+        const float mx = fmaxf(fabsf(a), fmaxf(fabsf(b), fabsf(d)));
+        if (mx <= 0.0f) break;
+        v[0] = a / mx;
+        v[1] = b / mx;
+        v[2] = d / mx;
+    }
+    const float n2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    float tmin = 0.0f, tmax = 0.0f;
+    for (int i = 0; i < 16; i++) {
+        const float t = ((px[i][0] - m[0]) * v[0] + (px[i][1] - m[1]) * v[1] + (px[i][2] - m[2]) * v[2]) / n2;
+        tmin = fminf(tmin, t);
+        tmax = fmaxf(tmax, t);
+    }
+    const float lim[3] = {31.0f, 63.0f, 31.0f};
+    for (int k = 0; k < 3; k++) {
+        e[0][k] = (int)fminf(fmaxf(rintf((m[k] + tmax * v[k]) * lim[k] / 255.0f), 0.0f), lim[k]);
+        e[1][k] = (int)fminf(fmaxf(rintf((m[k] + tmin * v[k]) * lim[k] / 255.0f), 0.0f), lim[k]);
+    }
+}
+
+// Start points, each refined, the lowest final error kept (earlier start on
+// ties): bit 0 of `starts` = the network's proposal y (8 floats per block,
+// components scaled to [0, 1]); bit 1 = the block's brightest and darkest
+// texels (luma 2r + 4g + b, first extreme); bit 2 = the principal axis.
+// Bit 3 adds the least-squares refit after each descent; bit 4 polishes the
+// kept result on the both-ends lattice.
+// out per block (16 bytes): u32 c0 | c1 << 16, u32 selectors, u32 error,
+// u32 single steps taken (max_iters per start = stopped early)
 __global__ void __launch_bounds__(256) refine_kernel(const uint8_t* __restrict__ rgba, uint32_t n,
                                                      const float* __restrict__ y, uint32_t max_iters, uint32_t starts,
                                                      uint32_t* __restrict__ out)
@@ -434,17 +571,26 @@ __global__ void __launch_bounds__(256) refine_kernel(const uint8_t* __restrict__
     for (int i = 0; i < 16; i++)
         for (int k = 0; k < 3; k++) px[i][k] = rgba[(size_t)b * 64 + 4 * i + k];
     const int lim[3] = {31, 63, 31};
+    const bool refit = (starts & 8u) != 0;
     uint32_t best = 0xFFFFFFFFu, steps = 0;
     int keep[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    auto consider = [&](int (*e)[3]) {
+        uint32_t st = 0;
+        const uint32_t eb = refine_from(px, e, max_iters, refit, st);
+        steps = max(steps, st);
+        if (eb < best) {
+            best = eb;
+            for (int s = 0; s < 2; s++)
+                for (int k = 0; k < 3; k++) keep[s][k] = e[s][k];
+        }
+    };
     if (starts & 1u) {
         const float* yy = y + (size_t)b * 8;
         int e[2][3];
         for (int s = 0; s < 2; s++)
             for (int k = 0; k < 3; k++)
                 e[s][k] = (int)fminf(fmaxf(rintf(yy[3 * s + k] * (float)lim[k]), 0.0f), (float)lim[k]);
-        best = refine_from(px, e, max_iters, steps);
-        for (int s = 0; s < 2; s++)
-            for (int k = 0; k < 3; k++) keep[s][k] = e[s][k];
+        consider(e);
     }
     if (starts & 2u) {
         int bi = 0, di = 0, bl = -1, dl = 1 << 30;
@@ -464,11 +610,21 @@ __global__ void __launch_bounds__(256) refine_kernel(const uint8_t* __restrict__
             e[0][k] = (int)rintf((float)px[bi][k] * (float)lim[k] / 255.0f);
             e[1][k] = (int)rintf((float)px[di][k] * (float)lim[k] / 255.0f);
         }
-        const uint32_t eb = refine_from(px, e, max_iters, steps);
-        if (eb < best) {
-            best = eb;
-            for (int s = 0; s < 2; s++)
-                for (int k = 0; k < 3; k++) keep[s][k] = e[s][k];
+        consider(e);
+    }
+    if (starts & 4u) {
+        int e[2][3];
+        pca_start(px, e);
+        consider(e);
+    }
+    if (starts & 16u) {
+        // polish the best result on the both-ends lattice, each improvement
+        // followed by the single-step descent (and refit)
+        for (int it = 0; it < 4 && best; it++) {
+            if (!box_step(px, keep, best)) break;
+            uint32_t st = 0;
+            best = refine_from(px, keep, max_iters, refit, st);
+            steps = max(steps, st);
         }
     }
     uint32_t lo, hi, sel;
@@ -581,7 +737,7 @@ int crn2_dxt1_refine_device(const void* rgba, uint32_t n, const float* y, uint32
                             uint32_t* out, void* stream)
 {
     if (!n) return CRN2_OK;
-    if (!rgba || !out || !(starts & 3u) || (starts & ~3u) || ((starts & 1u) && !y)) return CRN2_EINVAL;
+    if (!rgba || !out || !(starts & 7u) || (starts & ~31u) || ((starts & 1u) && !y)) return CRN2_EINVAL;
     hipLaunchKernelGGL(refine_kernel, dim3(grid1(n, 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
                        static_cast<const uint8_t*>(rgba), n, y, max_iters, starts, out);
     return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;

@@ -141,7 +141,7 @@ def infer(theta, rgba_blocks):
     return y
 
 
-START_NET, START_BOX = 1, 2
+START_NET, START_BOX, START_PCA, REFIT, POLISH = 1, 2, 4, 8, 16
 
 
 def refine(rgba_blocks, y, max_iters=8, starts=START_NET):
@@ -183,14 +183,15 @@ def plain_targets(rgba_blocks, quality=dxt1.UBER):
     return torch.cat([first / scale, second / scale], 1).contiguous(), raw
 
 
-def encode(theta, rgba_blocks, conf_threshold=0.0, max_iters=8, guarded=False, quality=dxt1.UBER):
+def encode(theta, rgba_blocks, conf_threshold=0.0, max_iters=8, guarded=False, quality=dxt1.UBER,
+           starts=START_NET | START_BOX | START_PCA | REFIT):
     """Learned DXT1 encode of n blocks.  Blocks whose confidence logit is below
     `conf_threshold` also run the plain search and keep the better of the two;
     with `guarded`, every block does (no block ends worse than plain).
     Returns (endpoints int32 [n] (c0 | c1 << 16), selectors int32 [n],
     errors int32 [n], number of blocks that ran the plain search)."""
     y = infer(theta, rgba_blocks)
-    r = refine(rgba_blocks, y, max_iters, START_NET | START_BOX)
+    r = refine(rgba_blocks, y, max_iters, starts)
     ends, sels, errs = r[:, 0].clone(), r[:, 1].clone(), r[:, 2].clone()
     if guarded:
         idx = torch.arange(rgba_blocks.shape[0], device=rgba_blocks.device)

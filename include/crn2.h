@@ -128,8 +128,10 @@ int crn2_batch_run_graph(crn2_batch* b, void* stream);
 int crn2_batch_status(crn2_batch* batch, void* stream);
 int crn2_batch_destroy(crn2_batch* batch);
 
-/* Kernel timing with HIP events on the launch stream (kernel 0 = level-stream
- * parse, 1 = block write): enable, run, then read total ms and launch count. */
+/* Stage timing with HIP events on the launch stream of crn2_batch_run and the
+ * unpack calls (stage 0 = transition tables, 1 = stream walk, 2 / 3 = symbol
+ * decode of even / odd rows, 4 = palette indices, 5 = block write): enable,
+ * run, then read total ms and launch count. */
 void crn2_profile_enable(int on);
 void crn2_profile_reset(void);
 int crn2_profile_read(int kernel, double* total_ms, uint64_t* launches);
@@ -263,9 +265,17 @@ int crn2_mlp_infer_device(const void* rgba, uint32_t n, const float* theta, floa
  * exact error drops (at most max_iters steps per start) and keeps the best.
  * starts: bit 0 = the proposal y (8 floats per block as crn2_mlp_infer_device
  * writes, rounded to RGB565), bit 1 = the block's brightest and darkest
- * texels (luma 2r + 4g + b).  out: 4 u32 per block = {c0 | c1 << 16 (c0 > c1:
- * four colours), selectors (2 bits per texel), perceptual error (8/25/1)
- * under the standard DXT1 decode, steps taken}. */
+ * texels (luma 2r + 4g + b), bit 2 = the extremes along the texels' principal
+ * axis; bit 3 = after each descent, refit both endpoints by least squares to
+ * the selectors (refine_solution, crnlib/crn_dxt1.cpp:525-698) and descend
+ * again while that lowers the error; bit 4 = then move the kept result to
+ * the best of its 728 neighbours with every endpoint component changed by
+ * -1 / 0 / +1 (refine_solution's both-ends lattice, crn_dxt1.cpp:601-643)
+ * while that lowers the error, descending again after each move.
+ * out: 4 u32 per block = {c0 | c1 << 16
+ * (c0 > c1: four colours), selectors (2 bits per texel), perceptual error
+ * (8/25/1) under the standard DXT1 decode, largest step count of a start
+ * (max_iters: stopped before converging)}. */
 int crn2_dxt1_refine_device(const void* rgba, uint32_t n, const float* y, uint32_t max_iters, uint32_t starts,
                             uint32_t* out, void* stream);
 /* The same error for given endpoint words (c0 | c1 << 16 at
