@@ -48,9 +48,10 @@ def psnr(sse, npix):
     return 99.0 if mse == 0 else 20.0 * math.log10(255.0 / math.sqrt(mse))
 
 
-def encode_leg(size, conf):
+def encode_leg(size, frac):
     """DXT1 encode of one size x size RGB texture: the plain GPU search against
-    the learned proposal (+ refinement, + plain fallback below `conf`)."""
+    the learned proposal (+ refinement, + the plain search for the `frac` of
+    blocks with the lowest confidence)."""
     import numpy as np
     import torch
     from crunch2_amd import dxt1, learned, texture
@@ -64,7 +65,7 @@ def encode_leg(size, conf):
     texels = n * 16
     # warm both paths on a slice
     dxt1.encode_blocks_device(blocks[:4096].contiguous(), dxt1.UBER, learned.LABEL_FLAGS)
-    learned.encode(theta, blocks[:4096].contiguous(), conf)
+    learned.encode(theta, blocks[:4096].contiguous(), fallback_frac=frac)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     raw = dxt1.encode_blocks_device(blocks, dxt1.UBER, learned.LABEL_FLAGS)
@@ -73,7 +74,7 @@ def encode_leg(size, conf):
     pend = raw.view(torch.int32).reshape(-1, 6)[:, 2].contiguous()
     perr, psel = learned.block_errors(blocks, pend)
     t0 = time.perf_counter()
-    ends, sels, errs, nfb = learned.encode(theta, blocks, conf)
+    ends, sels, errs, nfb = learned.encode(theta, blocks, fallback_frac=frac)
     torch.cuda.synchronize()
     t_learn = time.perf_counter() - t0
     sse_p = int(decode_sse(blocks, pend, psel).sum())
@@ -84,7 +85,7 @@ def encode_leg(size, conf):
             "plain_mtexels_s": round(texels / t_plain / 1e6, 1), "learned_mtexels_s": round(texels / t_learn / 1e6, 1),
             "speedup": round(t_plain / t_learn, 3), "psnr_plain_db": round(ps, 4), "psnr_learned_db": round(pl, 4),
             "delta_db": round(pl - ps, 4), "blocks_worse_than_plain": int((errs > perr).sum()),
-            "fallback_blocks": nfb, "conf_threshold": conf}
+            "fallback_blocks": nfb, "fallback_frac": frac}
 
 
 def main():
@@ -104,7 +105,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline sample length")
     ap.add_argument("--encode-size", type=int, default=8192,
                     help="side of the RGB texture of the DXT1 encode leg (learned vs plain); 0 skips it")
-    ap.add_argument("--conf", type=float, default=0.0, help="confidence logit below which a block also runs the plain search")
+    ap.add_argument("--fallback-frac", type=float, default=0.10,
+                    help="fraction of blocks (lowest confidence first) that also run the plain search in the encode leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (each GPU would transcode its "
@@ -314,7 +316,7 @@ def main():
     for pl in plans:
         assert lib().crn2_batch_status(pl, C.c_void_p(stream.cuda_stream)) == 0
     if args.encode_size:
-        line["encode"] = encode_leg(args.encode_size, args.conf)
+        line["encode"] = encode_leg(args.encode_size, args.fallback_frac)
     print(json.dumps(line))
     for pl in plans:
         lib().crn2_batch_destroy(pl)

@@ -166,7 +166,8 @@ __global__ void __launch_bounds__(256) gelu_back_kernel(const float* __restrict_
 // mean of squared error (outputs 0-5) + kConfWeight * logistic loss of
 // output 6 against "all six rounded components within one step".
 __global__ void __launch_bounds__(256) mlp_loss_kernel(const float* __restrict__ Y, const float* __restrict__ T,
-                                                       uint32_t rows, float* __restrict__ dY, float* __restrict__ loss)
+                                                       const float* __restrict__ labels, uint32_t rows,
+                                                       float* __restrict__ dY, float* __restrict__ loss)
 {
     __shared__ float red[256];
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -184,7 +185,7 @@ __global__ void __launch_bounds__(256) mlp_loss_kernel(const float* __restrict__
             const float q = fminf(fmaxf(rintf(y[k] * scale[k]), 0.0f), scale[k]);
             ok &= fabsf(q - rintf(t[k] * scale[k])) <= 1.0f;
         }
-        const float lab = ok ? 1.0f : 0.0f;
+        const float lab = labels ? labels[r] : (ok ? 1.0f : 0.0f);
         const float z = y[6];
         const float s = 1.0f / (1.0f + expf(-z));
         l += kConfWeight * (fmaxf(z, 0.0f) + log1pf(expf(-fabsf(z))) - lab * z);
@@ -375,6 +376,43 @@ __device__ inline uint32_t dxt1_error(const int (*px)[3], uint32_t c0, uint32_t 
 
 __device__ inline uint32_t pack565(const int* c) { return ((uint32_t)c[0] << 11) | ((uint32_t)c[1] << 5) | (uint32_t)c[2]; }
 
+// The same error, but stops summing once it reaches `bound` (returns a value
+// >= bound then): a candidate that cannot win costs only a few texels.
+__device__ inline uint32_t dxt1_error_below(const int (*px)[3], uint32_t c0, uint32_t c1, uint32_t bound)
+{
+    int p[4][3];
+    expand565(c0, p[0]);
+    expand565(c1, p[1]);
+    const bool four = c0 > c1;
+    for (int k = 0; k < 3; k++) {
+        if (four) {
+            p[2][k] = (2 * p[0][k] + p[1][k]) / 3;
+            p[3][k] = (p[0][k] + 2 * p[1][k]) / 3;
+        } else {
+            p[2][k] = (p[0][k] + p[1][k]) / 2;
+            p[3][k] = 0;
+        }
+    }
+    uint32_t err = 0;
+    for (int i = 0; i < 16; i++) {
+        uint32_t best = 0xFFFFFFFFu;
+        for (int s = 0; s < 4; s++) {
+            const int dr = px[i][0] - p[s][0], dg = px[i][1] - p[s][1], db = px[i][2] - p[s][2];
+            best = min(best, (uint32_t)(8 * dr * dr + 25 * dg * dg + db * db));
+        }
+        err += best;
+        if (err >= bound) break;
+    }
+    return err;
+}
+
+// Ordered-pair error of (a, b) as pair_error, bounded as dxt1_error_below.
+__device__ inline uint32_t pair_error_below(const int (*px)[3], const int* a, const int* b, uint32_t bound)
+{
+    const uint32_t ca = pack565(a), cb = pack565(b);
+    return dxt1_error_below(px, ca > cb ? ca : cb, ca > cb ? cb : ca, bound);
+}
+
 // Four-colour error of the unordered endpoint pair (a, b) (equal pairs use the
 // three-colour decode); returns the ordered pair in (lo, hi).
 __device__ inline uint32_t pair_error(const int (*px)[3], const int* a, const int* b, uint32_t& lo, uint32_t& hi)
@@ -400,8 +438,7 @@ __device__ inline bool descend(const int (*px)[3], int (*e)[3], uint32_t& best, 
                     const int v = e[s][k] + d;
                     if (v < 0 || v > lim[k]) continue;
                     e[s][k] = v;
-                    uint32_t l2, h2;
-                    const uint32_t er = pair_error(px, e[0], e[1], l2, h2);
+                    const uint32_t er = pair_error_below(px, e[0], e[1], cand);
                     e[s][k] = v - d;
                     if (er < cand) {
                         cand = er;
@@ -493,8 +530,7 @@ __device__ inline bool box_step(const int (*px)[3], int (*e)[3], uint32_t& best)
                 f[s][k] = v;
             }
         if (!ok) continue;
-        uint32_t lo, hi;
-        const uint32_t er = pair_error(px, f[0], f[1], lo, hi);
+        const uint32_t er = pair_error_below(px, f[0], f[1], cand);
         if (er < cand) {
             cand = er;
             bm = m;
@@ -697,11 +733,12 @@ int crn2_mlp_gelu_back_device(const float* dA, const float* Z, float* dZ, uint64
     return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
 }
 
-int crn2_mlp_loss_device(const float* Y, const float* T, uint32_t rows, float* dY, float* loss, void* stream)
+int crn2_mlp_loss_device(const float* Y, const float* T, const float* labels, uint32_t rows, float* dY, float* loss,
+                         void* stream)
 {
     if (!rows) return CRN2_OK;
     hipLaunchKernelGGL(mlp_loss_kernel, dim3(grid1(rows, 256)), dim3(256), 0, static_cast<hipStream_t>(stream), Y, T,
-                       rows, dY, loss);
+                       labels, rows, dY, loss);
     return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
 }
 

@@ -87,8 +87,9 @@ class Trainer:
             self._bufs[name] = b
         return b
 
-    def gradients(self, X, T):
-        """Loss and flat gradient of the batch (X: n x 57, T: n x 6 float32)."""
+    def gradients(self, X, T, labels=None):
+        """Loss and flat gradient of the batch (X: n x 57, T: n x 6 float32;
+        labels: optional n float32 confidence labels)."""
         n = X.shape[0]
         th, o = self.theta, OFFSETS
         Z1, A1 = self._buf("Z1", (n, N_H)), self._buf("A1", (n, N_H))
@@ -108,7 +109,8 @@ class Trainer:
         check(L.crn2_mlp_bias_act_device(_p(Y), _p(Y), C.c_void_p(th.data_ptr() + 4 * o["b3"]), n, N_OUT, 0, s),
               "bias_act")
         self.loss.zero_()
-        check(L.crn2_mlp_loss_device(_p(Y), _p(T), n, _p(dY), _p(self.loss), s), "loss")
+        lp = C.c_void_p(labels.data_ptr()) if labels is not None else C.c_void_p(0)
+        check(L.crn2_mlp_loss_device(_p(Y), _p(T), lp, n, _p(dY), _p(self.loss), s), "loss")
         # backward
         g = self.g
         g.zero_()
@@ -125,8 +127,8 @@ class Trainer:
         gemm(1, N_H, n, self.ones, 0, 0, dZ, N_H, 1, g, N_H, ksplit=ks, c_ofs=o["b1"])
         return self.loss, g
 
-    def step(self, X, T):
-        loss, g = self.gradients(X, T)
+    def step(self, X, T, labels=None):
+        loss, g = self.gradients(X, T, labels)
         self.step_no += 1
         check(lib().crn2_adam_device(_p(self.theta), _p(g), _p(self.m), _p(self.v), N_PARAMS, self.lr, 0.9, 0.999,
                                      1e-8, self.step_no, _s()), "adam")
@@ -142,6 +144,9 @@ def infer(theta, rgba_blocks):
 
 
 START_NET, START_BOX, START_PCA, REFIT, POLISH = 1, 2, 4, 8, 16
+# refinement of the learned encode: every start, refit and lattice polish
+ENCODE_STARTS = START_NET | START_BOX | START_PCA | REFIT | POLISH
+ENCODE_ITERS = 16
 
 
 def refine(rgba_blocks, y, max_iters=8, starts=START_NET):
@@ -183,18 +188,24 @@ def plain_targets(rgba_blocks, quality=dxt1.UBER):
     return torch.cat([first / scale, second / scale], 1).contiguous(), raw
 
 
-def encode(theta, rgba_blocks, conf_threshold=0.0, max_iters=8, guarded=False, quality=dxt1.UBER,
-           starts=START_NET | START_BOX | START_PCA | REFIT):
+def encode(theta, rgba_blocks, conf_threshold=0.0, max_iters=ENCODE_ITERS, guarded=False, quality=dxt1.UBER,
+           starts=ENCODE_STARTS, fallback_frac=None):
     """Learned DXT1 encode of n blocks.  Blocks whose confidence logit is below
-    `conf_threshold` also run the plain search and keep the better of the two;
-    with `guarded`, every block does (no block ends worse than plain).
+    `conf_threshold` (or, with `fallback_frac`, that fraction of the blocks with
+    the lowest confidence) also run the plain search and keep the better of the
+    two; with `guarded`, every block does (no block ends worse than plain).
     Returns (endpoints int32 [n] (c0 | c1 << 16), selectors int32 [n],
     errors int32 [n], number of blocks that ran the plain search)."""
     y = infer(theta, rgba_blocks)
     r = refine(rgba_blocks, y, max_iters, starts)
     ends, sels, errs = r[:, 0].clone(), r[:, 1].clone(), r[:, 2].clone()
+    n = rgba_blocks.shape[0]
     if guarded:
-        idx = torch.arange(rgba_blocks.shape[0], device=rgba_blocks.device)
+        idx = torch.arange(n, device=rgba_blocks.device)
+    elif fallback_frac is not None:
+        k = int(round(fallback_frac * n))
+        idx = torch.topk(y[:, 6], k, largest=False).indices if k else torch.zeros(0, dtype=torch.long,
+                                                                                 device=rgba_blocks.device)
     else:
         idx = torch.nonzero(y[:, 6] < conf_threshold).flatten()
     if idx.numel():

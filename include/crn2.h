@@ -252,9 +252,10 @@ int crn2_mlp_bias_act_device(float* Z, float* A, const float* bias, uint32_t row
 int crn2_mlp_gelu_back_device(const float* dA, const float* Z, float* dZ, uint64_t n, void* stream);
 /* Loss of outputs Y (rows x 7) against targets T (rows x 6): batch mean of
  * squared error of outputs 0-5 plus 0.05 x logistic loss of output 6 against
- * "every rounded endpoint component within one step"; writes dY, adds the
- * loss into *loss. */
-int crn2_mlp_loss_device(const float* Y, const float* T, uint32_t rows, float* dY, float* loss, void* stream);
+ * labels (one float 0/1 per row), or with labels NULL against "every rounded
+ * endpoint component within one step"; writes dY, adds the loss into *loss. */
+int crn2_mlp_loss_device(const float* Y, const float* T, const float* labels, uint32_t rows, float* dY, float* loss,
+                         void* stream);
 /* Adam step `step` (1-based) on n parameters. */
 int crn2_adam_device(float* w, const float* g, float* m, float* v, uint32_t n, float lr, float b1, float b2, float eps,
                      uint32_t step, void* stream);

@@ -64,6 +64,24 @@ def test_gradients_match_oracle(gpu):
     assert cos > 0.99999 and rel < 1e-3
 
 
+def test_gradients_with_given_labels_match_oracle(gpu):
+    blocks, t = training_batch(512, 8)
+    theta = omlp.init_params(9)
+    lab = (np.random.default_rng(8).random(len(blocks)) < 0.3).astype(np.float32)
+    tr = learned.Trainer(dev(theta))
+    x = learned.features(dev(blocks))
+    loss, g = tr.gradients(x, dev(t), dev(lab))
+    torch.cuda.synchronize()
+    ref_loss, ref_g = omlp.loss_and_grads(theta, x.cpu().numpy(), t, labels=lab)
+    g = g.cpu().numpy()
+    assert abs(float(loss.item()) - ref_loss) <= 1e-4 * max(1.0, abs(ref_loss))
+    cos = float(g @ ref_g / (np.linalg.norm(g) * np.linalg.norm(ref_g)))
+    assert cos > 0.99999 and np.abs(g - ref_g).max() / np.abs(ref_g).max() < 1e-3
+    # the labels reach the confidence output: its bias gradient changes with them
+    _, g2 = tr.gradients(x, dev(t), dev(1.0 - lab))
+    assert abs(float(g2[-1].item()) - float(g[-1])) > 1e-6
+
+
 def test_inference_matches_oracle(gpu):
     blocks, _ = training_batch(700, 4)
     theta = omlp.init_params(11)
