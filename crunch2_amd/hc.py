@@ -30,3 +30,27 @@ def color_selectors(blocks, endpoint_index, cluster_colors, selectors, subblocks
                                                _ptr(err), _ptr(details), _ptr(used), C.c_void_p(stream)),
           "crn2_hc_color_selectors_device")
     return best, err
+
+
+def color_endpoints(cluster_px, cluster_ofs, block_cluster, blocks, block_weights, block_encodings, perceptual=True):
+    """dxt_hc colour endpoint codebook (crn_dxt_hc.cpp:663-754).  cluster_px:
+    uint8 CUDA tensor [t, 4]; cluster_ofs: int32 [k + 1]; block_cluster: int32
+    [n] (-1: no cluster); blocks: uint8 [n, 16, 4]; block_weights: float32 [n];
+    block_encodings: uint8 [n].  Returns (endpoints int32 [k], color_values
+    uint8 [k, 4, 4], block_selectors int64 [n] (zero for blocks of no
+    cluster), clusters not encoded)."""
+    k = cluster_ofs.shape[0] - 1
+    n = blocks.shape[0]
+    dev = blocks.device
+    ends = torch.empty(k, dtype=torch.int32, device=dev)
+    cv = torch.empty((k, 4, 4), dtype=torch.uint8, device=dev)
+    bsel = torch.zeros(n, dtype=torch.int64, device=dev)
+    ne = torch.zeros(1, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    check(lib().crn2_hc_color_endpoints_device(_ptr(cluster_px.contiguous()), _ptr(cluster_ofs.contiguous()), k,
+                                               int(cluster_px.shape[0]), _ptr(block_cluster.contiguous()), n,
+                                               _ptr(blocks.contiguous()), _ptr(block_weights.contiguous()),
+                                               _ptr(block_encodings.contiguous()), int(bool(perceptual)), _ptr(ends),
+                                               _ptr(cv), _ptr(bsel), _ptr(ne), C.c_void_p(stream)),
+          "crn2_hc_color_endpoints_device")
+    return ends, cv, bsel, int(ne.item())

@@ -213,6 +213,27 @@ int crn2_hc_color_selectors_device(const void* blocks, const uint32_t* endpoint_
                                    uint32_t n_selectors, uint32_t* best_index, uint32_t* best_error,
                                    uint32_t* details, uint8_t* used, void* stream);
 
+/* ------------------------------------- dxt_hc colour endpoint codebook */
+/* dxt_hc::determine_color_endpoint_codebook_task (crnlib/crn_dxt_hc.cpp:663-754)
+ * with dxt_endpoint_refiner::refine (crnlib/crn_dxt_endpoint_refiner.cpp:
+ * 36-144, 203-302) for all clusters at once, on the device.  cluster_px: every
+ * cluster's texels (RGBA8 words), cluster c owning [cluster_ofs[c],
+ * cluster_ofs[c+1]) of total_px; block_cluster[b]: the cluster whose block list
+ * holds block b (0xFFFFFFFF: none); blocks: nblocks x 16 RGBA8; block_weights
+ * (float) and block_encodings (0..7) per block (dxt_hc's m_block_weights /
+ * m_block_encodings); perceptual: 8/25/1 weights.  Writes per cluster
+ * endpoints[c] = first | second << 16 (the uber search's endpoints, replaced
+ * by the refiner's when their error is lower) and color_values (4 RGBA8
+ * words, the search's colours in linear order); per block of a cluster
+ * block_selectors[b] = 16 selectors (texel 0 in the top bits) << 32 | weight;
+ * *not_encoded = clusters with more than 4096 distinct colours (left zero).
+ * Allocates stream-ordered scratch (not for graph capture). */
+int crn2_hc_color_endpoints_device(const void* cluster_px, const uint32_t* cluster_ofs, uint32_t nclusters,
+                                   uint32_t total_px, const uint32_t* block_cluster, uint32_t nblocks,
+                                   const void* blocks, const float* block_weights, const uint8_t* block_encodings,
+                                   int perceptual, uint32_t* endpoints, void* color_values, uint64_t* block_selectors,
+                                   uint32_t* not_encoded, void* stream);
+
 /* ------------------------------------------------------------- autotuner */
 /* No reference counterpart (the reference's task pool splits work evenly,
  * crn_dxt_image.cpp:480-485).  Kernels with several launch shapes of identical

@@ -245,3 +245,40 @@ def hc_color_selectors(blocks, endpoint_index, cluster_colors, selectors, subblo
     L.orc_hc_color_selectors(_p(blocks), _p(ei), _p(cc), n, int(bool(subblocks)), int(bool(perceptual)), _p(sel), m,
                              _p(best), _p(err), _p(det), _p(used))
     return best, err, det, used
+
+
+def hc_color_endpoints(cluster_px, cluster_ofs, block_cluster, blocks, block_weights, block_encodings,
+                       perceptual=True):
+    """dxt_hc colour endpoint codebook (crn_dxt_hc.cpp:663-754) with the
+    refiner.  Returns (endpoints u32 [k], color_values u8 [k, 4, 4],
+    block_selectors u64 [n])."""
+    L = lib()
+    L.orc_hc_endpoint_codebook.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                           C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    px = np.ascontiguousarray(cluster_px, np.uint8)
+    ofs = np.ascontiguousarray(cluster_ofs, np.uint32)
+    bc = np.ascontiguousarray(block_cluster, np.uint32)
+    blk = np.ascontiguousarray(blocks, np.uint8)
+    w = np.ascontiguousarray(block_weights, np.float32)
+    enc = np.ascontiguousarray(block_encodings, np.uint8)
+    k, n = ofs.shape[0] - 1, blk.shape[0]
+    ends = np.zeros(k, np.uint32)
+    cv = np.zeros((k, 4, 4), np.uint8)
+    bsel = np.zeros(n, np.uint64)
+    L.orc_hc_endpoint_codebook(_p(px), _p(ofs), k, _p(bc), n, _p(blk), _p(w), _p(enc), int(bool(perceptual)),
+                               _p(ends), _p(cv), _p(bsel))
+    return ends, cv, bsel
+
+
+def hc_refine_dxt1(px, selectors, perceptual=True, error_to_beat=2 ** 64 - 1):
+    """dxt_endpoint_refiner (DXT1 selectors) over texels px ([n, 4] u8) with
+    their selector codes.  Returns (improved, low, high, error)."""
+    L = lib()
+    L.orc_hc_refine_dxt1.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_int, C.c_uint64, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]
+    px = np.ascontiguousarray(px, np.uint8).reshape(-1, 4)
+    sel = np.ascontiguousarray(selectors, np.uint8)
+    lo, hi, err = C.c_uint16(), C.c_uint16(), C.c_uint64()
+    ok = L.orc_hc_refine_dxt1(_p(px), px.shape[0], _p(sel), int(bool(perceptual)), error_to_beat, C.byref(lo),
+                              C.byref(hi), C.byref(err))
+    return bool(ok), lo.value, hi.value, err.value

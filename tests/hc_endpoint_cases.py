@@ -1,0 +1,29 @@
+"""Synthetic inputs shaped like dxt_hc's colour endpoint clusters: blocks of a
+seeded texture, each assigned to one of k clusters of uneven size (a cluster's
+texels are all texels of its blocks, in block order), with per-block weights
+and encodings as dxt_hc keeps them (crn_dxt_hc.cpp:663-754)."""
+import numpy as np
+
+from crunch2_amd import dxt1, texture
+
+
+def clusters(side=128, k=48, seed=0, empty=(3,)):
+    img = texture.synth(side, side, 300 + seed, texture.RGB)
+    blocks = dxt1.image_blocks(img)
+    n = blocks.shape[0]
+    rng = np.random.default_rng(seed)
+    # uneven sizes: cluster of block b from a skewed draw; some clusters empty
+    owner = np.minimum((rng.random(n) ** 2 * k).astype(np.int64), k - 1)
+    for e in empty:
+        owner[owner == e] = (e + 1) % k
+    order = np.argsort(owner, kind="stable")
+    counts = np.bincount(owner, minlength=k)
+    ofs = np.zeros(k + 1, np.uint32)
+    ofs[1:] = np.cumsum(counts * 16)
+    px = blocks[order].reshape(-1, 4)
+    weights = (0.25 + 3.0 * rng.random(n)).astype(np.float32)
+    enc = rng.integers(0, 8, n).astype(np.uint8)
+    # a few blocks in no cluster
+    block_cluster = owner.astype(np.uint32)
+    block_cluster[rng.choice(n, 5, replace=False)] = 0xFFFFFFFF
+    return np.ascontiguousarray(px), ofs, block_cluster, blocks, weights, enc
