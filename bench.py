@@ -18,6 +18,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes one copy of the 4096^2 DXT1 + DXT5 pair moves through the
+# pipeline (per-kernel FETCH_SIZE / WRITE_SIZE, scaled by a device copy of
+# known size): 1.168 GB, 30x the algorithmic bytes, mostly the per-bit tables.
+TRAFFIC_PER_COPY = 1.168e9
 STAGES = ("crnd_tables_kernel", "crnd_walk_kernel", "crnd_sym_kernel<even>", "crnd_sym_kernel<odd>",
           "crnd_index_{chunk,link,final}_kernel", "crnd_write_kernel")
 
@@ -304,7 +308,8 @@ def main():
                    "crn_bytes": args.copies * crn_bytes,
                    "dxtn_bytes": args.copies * out_bytes, "texels": args.copies * texels},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 6),
+                     "traffic": None if args.mixed or args.size != 4096 or args.batch != 1 else TRAFFIC_PER_COPY * args.copies,
                      "kernel": "transcode pipeline (tables, walk, symbols, indices, write; %d plans of %s "
                                "copies on %d streams)" % (groups, "/".join(str(x) for x in per), groups),
                      "stage_ms_one_plan": {k: round(v, 4) for k, v in prof.items()}},

@@ -283,6 +283,324 @@ __global__ void __launch_bounds__(64) hc_ep_refine_kernel(const uint32_t* __rest
     endpoints[c] = err < r.error ? (lo | hi << 16) : r.lo_hi;
 }
 
+// ------------------------------------------------------------ alpha (H.8)
+// dxt_hc::determine_alpha_endpoint_codebook_task (crn_dxt_hc.cpp:1014-1130,
+// without ETC colour blocks) for one alpha channel:
+//   hc_alpha_search_kernel, one wave per cluster: the uber dxt5_endpoint_
+//     optimizer (crn_dxt5a.cpp:40-262, 8-value blocks only) over the cluster's
+//     values -- distinct values in first-appearance order with their counts
+//     (LDS tables over the 256 values), every distinct pair in that order,
+//     then the +-16 box around the current best (centre re-read at every
+//     step); each candidate is scored by all lanes at once (first minimum per
+//     value, wave sum) and kept on a strict improvement, a zero error ends the
+//     search; canonical order with the eight-value invert table;
+//   hc_alpha_blocks_kernel, one thread per block: values in linear order,
+//     encoding weights from the endpoint spread, 3-bit nearest-value selectors;
+//   hc_alpha_refine_kernel, one thread per cluster: the DXT5 refiner
+//     (least squares on the alpha values in texel order, then the canonical
+//     23 x 23 box scored through per-selector histograms).
+struct AlphaRes {
+    uint64_t error;
+    uint32_t first, second, reordered, pad;
+};
+
+__constant__ uint8_t k5_to_linear[8] = {0, 7, 1, 2, 3, 4, 5, 6};
+__constant__ uint8_t k5_from_linear[8] = {0, 2, 3, 4, 5, 6, 7, 1};
+__constant__ uint8_t k5_eight_invert[8] = {1, 0, 7, 6, 5, 4, 3, 2};
+
+__device__ inline void values8(uint32_t l, uint32_t h, uint32_t* v)
+{
+    v[0] = l;
+    v[1] = h;
+    for (uint32_t j = 2; j < 8; j++) v[j] = (l * (8 - j) + h * (j - 1)) / 7;
+}
+
+// dxt5_block::get_block_values (crn_dxt.cpp:369-455)
+__device__ inline void alpha_values_of(uint32_t l, uint32_t h, uint32_t* v)
+{
+    if (l > h) {
+        values8(l, h, v);
+    } else {
+        v[0] = l;
+        v[1] = h;
+        for (uint32_t j = 2; j < 6; j++) v[j] = (l * (6 - j) + h * (j - 1)) / 5;
+        v[6] = 0;
+        v[7] = 255;
+    }
+}
+
+struct AlphaLds {
+    uint32_t count[256];
+    uint32_t first[256];
+    uint32_t uval[256];
+    uint32_t uw[256];
+    uint32_t rank[256];
+    uint8_t trial[256];
+    uint8_t best[256];
+};
+
+__device__ inline uint64_t wave_sum64(uint64_t v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, off), hi = (uint32_t)__shfl_xor((int)(v >> 32), off);
+        v += (uint64_t)lo | ((uint64_t)hi << 32);
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(64) hc_alpha_search_kernel(const uint8_t* __restrict__ vals_all,
+                                                             const uint32_t* __restrict__ ofs, uint32_t nclusters,
+                                                             AlphaRes* __restrict__ res, uint8_t* __restrict__ sel_all)
+{
+    __shared__ AlphaLds L;
+    const uint32_t c = blockIdx.x, lane = threadIdx.x;
+    const uint32_t o = ofs[c], n = ofs[c + 1] - o;
+    if (!n) return;
+    const uint8_t* vals = vals_all + o;
+    for (uint32_t v = lane; v < 256; v += 64) {
+        L.count[v] = 0;
+        L.first[v] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    for (uint32_t i = lane; i < n; i += 64) {
+        atomicAdd(&L.count[vals[i]], 1u);
+        atomicMin(&L.first[vals[i]], i);
+    }
+    __syncthreads();
+    // rank of each present value by first appearance
+    for (uint32_t v = lane; v < 256; v += 64) {
+        if (!L.count[v]) continue;
+        const uint32_t f = L.first[v];
+        uint32_t r = 0;
+        for (uint32_t u = 0; u < 256; u++) r += (L.count[u] && L.first[u] < f) ? 1u : 0u;
+        L.rank[v] = r;
+        L.uval[r] = v;
+        L.uw[r] = L.count[v];
+    }
+    __syncthreads();
+    uint32_t nu = 0;
+    for (uint32_t v = 0; v < 256; v++) nu += L.count[v] ? 1u : 0u;
+    AlphaRes out;
+    out.pad = 0;
+    if (nu == 1) {
+        out.error = 0;
+        out.first = out.second = L.uval[0];
+        out.reordered = 0;
+        if (lane == 0) res[c] = out;
+        for (uint32_t i = lane; i < n; i += 64) sel_all[o + i] = 0;
+        return;
+    }
+    uint64_t best_err = ~0ull;
+    uint32_t best_first = 0, best_second = 0;
+    bool done = false;
+    // evaluate_solution (crn_dxt5a.cpp:198-262) with the 8-value palette only
+    auto evaluate = [&](uint32_t lo, uint32_t hi) {
+        uint32_t pal[8];
+        values8(lo, hi, pal);
+        uint64_t part = 0;
+        for (uint32_t i = lane; i < nu; i += 64) {
+            const int val = (int)L.uval[i], w = (int)L.uw[i];
+            uint32_t best = 0xFFFFFFFFu, bsel = 0;
+            for (uint32_t j = 0; j < 8; j++) {
+                int e = val - (int)pal[j];
+                e = e * e * w;
+                if ((uint32_t)e < best) {
+                    best = (uint32_t)e;
+                    bsel = j;
+                    if (!best) break;
+                }
+            }
+            L.trial[i] = (uint8_t)bsel;
+            part += best;
+        }
+        const uint64_t err = wave_sum64(part);
+        if (err < best_err) {
+            best_err = err;
+            best_first = lo;
+            best_second = hi;
+            for (uint32_t i = lane; i < nu; i += 64) L.best[i] = L.trial[i];
+            if (!err) done = true;
+        }
+    };
+    for (uint32_t i = 0; i + 1 < nu && !done; i++)
+        for (uint32_t j = i + 1; j < nu && !done; j++) evaluate(L.uval[i], L.uval[j]);
+    if (best_err && !done) {
+        for (int dl = -16; dl <= 16 && !done; dl++) {
+            const int l = (int)best_first + dl;
+            if (l < 0) continue;
+            if (l > 255) break;
+            for (int dh = -16; dh <= 16 && !done; dh++) {
+                const int h = (int)best_second + dh;
+                if (h < 0) continue;
+                if (h > 255) break;
+                evaluate((uint32_t)l, (uint32_t)h);
+            }
+        }
+    }
+    // canonical order (crn_dxt5a.cpp:151-184), 8-value blocks
+    uint32_t f = best_first, sc = best_second, reordered = 0;
+    bool zero = f == sc, invert = false;
+    if (!zero && f <= sc) {
+        const uint32_t t = f;
+        f = sc;
+        sc = t;
+        reordered = 1;
+        invert = true;
+    }
+    __syncthreads();
+    for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t u = L.rank[vals[i]];
+        uint32_t s = zero ? 0u : L.best[u];
+        if (invert) s = k5_eight_invert[s];
+        sel_all[o + i] = (uint8_t)s;
+    }
+    out.error = best_err;
+    out.first = f;
+    out.second = sc;
+    out.reordered = reordered;
+    if (lane == 0) res[c] = out;
+}
+
+__global__ void __launch_bounds__(256) hc_alpha_blocks_kernel(const AlphaRes* __restrict__ res,
+                                                              const uint32_t* __restrict__ ofs,
+                                                              const uint32_t* __restrict__ block_cluster,
+                                                              uint32_t nblocks, const uint8_t* __restrict__ block_vals,
+                                                              const uint8_t* __restrict__ encodings,
+                                                              uint64_t* __restrict__ block_selectors)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    const uint32_t c = block_cluster[b];
+    if (c == kNone || ofs[c + 1] == ofs[c]) return;
+    const AlphaRes r = res[c];
+    uint32_t bv[8], av[8];
+    alpha_values_of(r.first, r.second, bv);
+    for (int i = 0; i < 8; i++) av[i] = bv[k5_from_linear[i]];
+    const int delta = (int)r.first - (int)r.second;
+    const uint32_t dd = (uint32_t)(delta * delta >> 3);
+    const uint32_t ew = dd < 1 ? 1u : (dd > 2048 ? 2048u : dd);
+    const uint32_t weight = (uint32_t)((float)ew * (1.15f + (1.0f - 1.15f) * ((float)encodings[b] / 7.0f)));
+    uint64_t selector = 0;
+    const uint8_t* v = block_vals + (size_t)b * 16;
+    for (int p = 0; p < 16; p++) {
+        uint32_t best = 0xFFFFFFFFu, sb = 0;
+        for (int t = 0; t < 8; t++) {
+            const uint32_t s = r.reordered ? 7u - k5_to_linear[t] : k5_to_linear[t];
+            const int d = (int)v[p] - (int)av[s];
+            const uint32_t e = (uint32_t)(d >= 0 ? d : -d);
+            if (e < best) {
+                sb = s;
+                best = e;
+            }
+        }
+        selector = selector << 3 | sb;
+    }
+    block_selectors[b] = selector << 16 | weight;
+}
+
+__global__ void __launch_bounds__(64) hc_alpha_refine_kernel(const uint8_t* __restrict__ vals_all,
+                                                             const uint32_t* __restrict__ ofs, uint32_t nclusters,
+                                                             const AlphaRes* __restrict__ res,
+                                                             const uint8_t* __restrict__ sel_all,
+                                                             uint32_t* __restrict__ endpoints,
+                                                             uint32_t* __restrict__ alpha_values,
+                                                             uint32_t* __restrict__ refined_values)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nclusters) return;
+    const uint32_t o = ofs[c], n = ofs[c + 1] - o;
+    endpoints[c] = 0;
+    alpha_values[2 * c] = alpha_values[2 * c + 1] = 0;
+    refined_values[2 * c] = refined_values[2 * c + 1] = 0;
+    if (!n) return;
+    const AlphaRes r = res[c];
+    uint32_t bv[8], av[8];
+    alpha_values_of(r.first, r.second, bv);
+    for (int i = 0; i < 8; i++) av[i] = bv[k5_from_linear[i]];
+    alpha_values[2 * c] = av[0] | av[1] << 8 | av[2] << 16 | av[3] << 24;
+    alpha_values[2 * c + 1] = av[4] | av[5] << 8 | av[6] << 16 | av[7] << 24;
+    // dxt_endpoint_refiner::refine, DXT5 selectors (crn_dxt_endpoint_refiner.cpp:36-144)
+    const uint8_t* vals = vals_all + o;
+    const uint8_t* sel = sel_all + o;
+    double alpha2_sum = 0.0f, beta2_sum = 0.0f, alphabeta_sum = 0.0f, ax = 0, bx = 0, first = 0;
+    uint64_t hist[8], D2[8], DD[8];
+    for (int s = 0; s < 8; s++) hist[s] = D2[s] = DD[s] = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t s = sel[i] & 7u, a = vals[i];
+        const double k = (double)((float)k5_to_linear[s] * 1.0f / 7.0f);
+        const double alpha = 1.0f - k;
+        const double beta = k;
+        const double x = (double)((float)a / 255.0f);
+        if (!i) first = x;
+        alpha2_sum += alpha * alpha;
+        beta2_sum += beta * beta;
+        alphabeta_sum += alpha * beta;
+        ax += alpha * x;
+        bx += beta * x;
+        hist[s]++;
+        D2[s] += (uint64_t)(a * 2);
+        DD[s] += (uint64_t)(a * a);
+    }
+    double a, b;
+    if (beta2_sum == 0.0f) {
+        a = ax / alpha2_sum;
+        b = 0.0;
+    } else if (alpha2_sum == 0.0f) {
+        a = 0.0;
+        b = bx / beta2_sum;
+    } else {
+        const double factor = alpha2_sum * beta2_sum - alphabeta_sum * alphabeta_sum;
+        if (factor != 0.0f) {
+            a = (ax * beta2_sum - bx * alphabeta_sum) / factor;
+            b = (bx * alpha2_sum - ax * alphabeta_sum) / factor;
+        } else {
+            a = b = first;
+        }
+    }
+    float lf = (float)a, hf = (float)b;
+    lf = lf < 0.0f ? 0.0f : (lf > 1.0f ? 1.0f : lf);
+    hf = hf < 0.0f ? 0.0f : (hf > 1.0f ? 1.0f : hf);
+    // optimize_dxt5 (crn_dxt_endpoint_refiner.cpp:146-201)
+    const uint32_t L0 = (uint32_t)clamp_int((int)(lf * 256.0f), 0, 255);
+    const uint32_t H0 = (uint32_t)clamp_int((int)(hf * 256.0f), 0, 255);
+    auto canon = [](uint32_t L, uint32_t H) -> uint32_t {
+        return L == H ? (H ? ((H - 1) << 8 | L) : 1u) : (L > H ? (H << 8 | L) : (L << 8 | H));
+    };
+    uint64_t rerr = ~0ull;
+    uint32_t rlo = 0, rhi = 0;
+    auto score = [&](uint32_t sol) -> bool {   // true: error reached zero
+        const uint32_t L = sol & 0xFFu, H = sol >> 8;
+        uint32_t v[8];
+        values8(L, H, v);
+        uint64_t error = 0;
+        for (int s = 0; s < 8; s++) error += hist[s] * v[s] * v[s] - D2[s] * v[s] + DD[s];
+        if (error < rerr) {
+            rlo = L;
+            rhi = H;
+            rerr = error;
+            if (!error) return true;
+        }
+        return false;
+    };
+    bool stop = score(canon(L0, H0));
+    const uint32_t minL = L0 <= 11 ? 0 : L0 - 11, maxL = L0 >= 244 ? 255 : L0 + 11;
+    const uint32_t minH = H0 <= 11 ? 0 : H0 - 11, maxH = H0 >= 244 ? 255 : H0 + 11;
+    for (uint32_t L = minL; L <= maxL && !stop; L++)
+        for (uint32_t H = minH; H <= maxH && !stop; H++)
+            if ((maxH < L || L <= H || H < minL) && (L != L0 || H != H0) && (L != H0 || H != L0)) stop = score(canon(L, H));
+    const bool refined = rerr < r.error;
+    uint32_t f = r.first, sc = r.second;
+    if (refined) {
+        f = rlo;
+        sc = rhi;
+        alpha_values_of(f, sc, bv);
+        for (int i = 0; i < 8; i++) av[i] = bv[k5_from_linear[i]];
+    }
+    refined_values[2 * c] = av[0] | av[1] << 8 | av[2] << 16 | av[3] << 24;
+    refined_values[2 * c + 1] = av[4] | av[5] << 8 | av[6] << 16 | av[7] << 24;
+    endpoints[c] = f | sc << 8 | (refined ? 1u : 0u) << 16;
+}
+
 inline uint32_t grid1(uint64_t n, uint32_t tb) { return (uint32_t)((n + tb - 1) / tb); }
 
 }  // namespace
@@ -320,6 +638,46 @@ extern "C" int crn2_hc_color_endpoints_device(const void* cluster_px, const uint
         hipLaunchKernelGGL(hc_ep_refine_kernel, dim3(grid1(nclusters, 64)), dim3(64), 0, s,
                            static_cast<const uint32_t*>(cluster_px), cluster_ofs, nclusters, res, sel, perceptual != 0,
                            endpoints, static_cast<uint32_t*>(color_values), not_encoded);
+        if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
+    }
+    (void)hipFreeAsync(sel, s);
+    (void)hipFreeAsync(res, s);
+    return rc;
+}
+
+extern "C" int crn2_hc_alpha_endpoints_device(const void* cluster_vals, const uint32_t* cluster_ofs, uint32_t nclusters,
+                                              uint32_t total_vals, const uint32_t* block_cluster, uint32_t nblocks,
+                                              const void* block_vals, const uint8_t* block_encodings,
+                                              uint32_t* endpoints, void* alpha_values, void* refined_values,
+                                              uint64_t* block_selectors, void* stream)
+{
+    if (!nclusters) return CRN2_OK;
+    if (!cluster_vals || !cluster_ofs || !endpoints || !alpha_values || !refined_values ||
+        (nblocks && (!block_cluster || !block_vals || !block_encodings || !block_selectors)))
+        return CRN2_EINVAL;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    AlphaRes* res = nullptr;
+    uint8_t* sel = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void**>(&res), (size_t)nclusters * sizeof(AlphaRes), s) != hipSuccess)
+        return CRN2_ENOMEM;
+    if (hipMallocAsync(reinterpret_cast<void**>(&sel), (size_t)total_vals + 1, s) != hipSuccess) {
+        (void)hipFreeAsync(res, s);
+        return CRN2_ENOMEM;
+    }
+    int rc = CRN2_OK;
+    hipLaunchKernelGGL(hc_alpha_search_kernel, dim3(nclusters), dim3(64), 0, s, static_cast<const uint8_t*>(cluster_vals),
+                       cluster_ofs, nclusters, res, sel);
+    if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
+    if (rc == CRN2_OK && nblocks) {
+        hipLaunchKernelGGL(hc_alpha_blocks_kernel, dim3(grid1(nblocks, 256)), dim3(256), 0, s, res, cluster_ofs,
+                           block_cluster, nblocks, static_cast<const uint8_t*>(block_vals), block_encodings,
+                           block_selectors);
+        if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
+    }
+    if (rc == CRN2_OK) {
+        hipLaunchKernelGGL(hc_alpha_refine_kernel, dim3(grid1(nclusters, 64)), dim3(64), 0, s,
+                           static_cast<const uint8_t*>(cluster_vals), cluster_ofs, nclusters, res, sel, endpoints,
+                           static_cast<uint32_t*>(alpha_values), static_cast<uint32_t*>(refined_values));
         if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
     }
     (void)hipFreeAsync(sel, s);

@@ -54,3 +54,26 @@ def color_endpoints(cluster_px, cluster_ofs, block_cluster, blocks, block_weight
                                                _ptr(cv), _ptr(bsel), _ptr(ne), C.c_void_p(stream)),
           "crn2_hc_color_endpoints_device")
     return ends, cv, bsel, int(ne.item())
+
+
+def alpha_endpoints(cluster_vals, cluster_ofs, block_cluster, block_vals, block_encodings):
+    """dxt_hc alpha endpoint codebook for one alpha channel
+    (crn_dxt_hc.cpp:1014-1130).  cluster_vals: uint8 CUDA tensor [t];
+    cluster_ofs: int32 [k + 1]; block_cluster: int32 [n] (-1: none);
+    block_vals: uint8 [n, 16]; block_encodings: uint8 [n].  Returns
+    (endpoints int32 [k] = first | second << 8 | refined << 16, alpha_values
+    uint8 [k, 8], refined_values uint8 [k, 8], block_selectors int64 [n])."""
+    k = cluster_ofs.shape[0] - 1
+    n = block_vals.shape[0]
+    dev = block_vals.device
+    ends = torch.empty(k, dtype=torch.int32, device=dev)
+    av = torch.empty((k, 8), dtype=torch.uint8, device=dev)
+    rv = torch.empty((k, 8), dtype=torch.uint8, device=dev)
+    bsel = torch.zeros(n, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    check(lib().crn2_hc_alpha_endpoints_device(_ptr(cluster_vals.contiguous()), _ptr(cluster_ofs.contiguous()), k,
+                                               int(cluster_vals.shape[0]), _ptr(block_cluster.contiguous()), n,
+                                               _ptr(block_vals.contiguous()), _ptr(block_encodings.contiguous()),
+                                               _ptr(ends), _ptr(av), _ptr(rv), _ptr(bsel), C.c_void_p(stream)),
+          "crn2_hc_alpha_endpoints_device")
+    return ends, av, rv, bsel

@@ -234,6 +234,23 @@ int crn2_hc_color_endpoints_device(const void* cluster_px, const uint32_t* clust
                                    int perceptual, uint32_t* endpoints, void* color_values, uint64_t* block_selectors,
                                    uint32_t* not_encoded, void* stream);
 
+/* dxt_hc::determine_alpha_endpoint_codebook_task (crnlib/crn_dxt_hc.cpp:1014-1130,
+ * without ETC colour blocks) with the DXT5 refiner
+ * (crn_dxt_endpoint_refiner.cpp:36-201) for one alpha channel and all
+ * clusters at once.  cluster_vals: every cluster's alpha values (one byte per
+ * texel), cluster c owning [cluster_ofs[c], cluster_ofs[c+1]) of total_vals;
+ * block_cluster[b]: the cluster whose list holds block b (0xFFFFFFFF: none);
+ * block_vals: nblocks x 16 alpha values of that channel; block_encodings per
+ * block.  Writes per cluster endpoints[c] = first | second << 8 | refined <<
+ * 16, alpha_values and refined_values (8 bytes each: the search's and the
+ * final endpoints' values in linear order); per block of a cluster
+ * block_selectors[b] = 16 3-bit selectors (texel 0 on top) << 16 | weight.
+ * Allocates stream-ordered scratch (not for graph capture). */
+int crn2_hc_alpha_endpoints_device(const void* cluster_vals, const uint32_t* cluster_ofs, uint32_t nclusters,
+                                   uint32_t total_vals, const uint32_t* block_cluster, uint32_t nblocks,
+                                   const void* block_vals, const uint8_t* block_encodings, uint32_t* endpoints,
+                                   void* alpha_values, void* refined_values, uint64_t* block_selectors, void* stream);
+
 /* ------------------------------------------------------------- autotuner */
 /* No reference counterpart (the reference's task pool splits work evenly,
  * crn_dxt_image.cpp:480-485).  Kernels with several launch shapes of identical

@@ -282,3 +282,53 @@ def hc_refine_dxt1(px, selectors, perceptual=True, error_to_beat=2 ** 64 - 1):
     ok = L.orc_hc_refine_dxt1(_p(px), px.shape[0], _p(sel), int(bool(perceptual)), error_to_beat, C.byref(lo),
                               C.byref(hi), C.byref(err))
     return bool(ok), lo.value, hi.value, err.value
+
+
+def hc_alpha_endpoints(cluster_vals, cluster_ofs, block_cluster, block_vals, block_encodings):
+    """dxt_hc alpha endpoint codebook for one channel (crn_dxt_hc.cpp:1014-1130)
+    with the DXT5 refiner.  Returns (endpoints u32 [k], alpha_values u8 [k, 8],
+    refined_values u8 [k, 8], block_selectors u64 [n])."""
+    L = lib()
+    L.orc_hc_alpha_endpoint_codebook.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p]
+    v = np.ascontiguousarray(cluster_vals, np.uint8)
+    ofs = np.ascontiguousarray(cluster_ofs, np.uint32)
+    bc = np.ascontiguousarray(block_cluster, np.uint32)
+    bv = np.ascontiguousarray(block_vals, np.uint8)
+    enc = np.ascontiguousarray(block_encodings, np.uint8)
+    k, n = ofs.shape[0] - 1, bv.shape[0]
+    ends = np.zeros(k, np.uint32)
+    av = np.zeros((k, 8), np.uint8)
+    rv = np.zeros((k, 8), np.uint8)
+    bsel = np.zeros(n, np.uint64)
+    L.orc_hc_alpha_endpoint_codebook(_p(v), _p(ofs), k, _p(bc), n, _p(bv), _p(enc), _p(ends), _p(av), _p(rv),
+                                     _p(bsel))
+    return ends, av, rv, bsel
+
+
+def hc_refine_dxt5(vals, selectors, error_to_beat=2 ** 64 - 1):
+    """dxt_endpoint_refiner (DXT5 selectors) over alpha values with their
+    selector codes.  Returns (improved, low, high, error)."""
+    L = lib()
+    L.orc_hc_refine_dxt5.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]
+    v = np.ascontiguousarray(vals, np.uint8)
+    sel = np.ascontiguousarray(selectors, np.uint8)
+    lo, hi, err = C.c_uint8(), C.c_uint8(), C.c_uint64()
+    ok = L.orc_hc_refine_dxt5(_p(v), v.shape[0], _p(sel), error_to_beat, C.byref(lo), C.byref(hi), C.byref(err))
+    return bool(ok), lo.value, hi.value, err.value
+
+
+def dxt5a_values(values, quality=4, use_both=False):
+    """One dxt5_endpoint_optimizer call over any number of values ([n] u8).
+    Returns (selectors [n], error, first, second, block_type, reordered)."""
+    L = lib()
+    L.orc_dxt5a_compute.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]
+    v = np.ascontiguousarray(values, np.uint8).reshape(-1)
+    sel = np.zeros(v.shape[0], np.uint8)
+    err, f, s, t, r = C.c_uint64(), C.c_uint8(), C.c_uint8(), C.c_uint8(), C.c_uint8()
+    assert L.orc_dxt5a_compute(_p(v), v.shape[0], quality, int(bool(use_both)), _p(sel), C.byref(err), C.byref(f),
+                               C.byref(s), C.byref(t), C.byref(r))
+    return sel, err.value, f.value, s.value, t.value, r.value

@@ -26,6 +26,13 @@
  *     endpoints nudged apart as the reference does), each scored with the
  *     selectors fixed through per-selector histograms; the refined endpoints
  *     replace the optimizer's when their error is below the optimizer's.
+ *   dxt_hc::determine_alpha_endpoint_codebook_task (crnlib/crn_dxt_hc.cpp:1014-1130,
+ *     without ETC colour blocks): the uber dxt5_endpoint_optimizer over the
+ *     cluster's alpha values (dxt5a_oracle.c; 8-value blocks only), the eight
+ *     values in linear order (g_dxt5_from_linear), encoding weights from the
+ *     endpoint spread, the 3-bit nearest-value selectors of every block, and
+ *     the DXT5 refiner (optimize_dxt5, crn_dxt_endpoint_refiner.cpp:146-201:
+ *     least squares on the alpha channel, then the canonical 23 x 23 box).
  * Every float / double expression keeps the reference's types and operation
  * order (build with -ffp-contract=off).
  *
@@ -34,7 +41,9 @@
  * result is no worse than every neighbour it could have reached in one more
  * round; with a cluster that is one 4x4 block the codebook selectors are the
  * nearest-colour selectors recomputed in numpy; weights follow the formula
- * recomputed in numpy.  The optimizer itself is pinned in dxt1_oracle.c.
+ * recomputed in numpy; the same for the alpha task, whose refiner also
+ * recovers exact 8-value palettes.  The optimizers are pinned in
+ * dxt1_oracle.c and dxt5a_oracle.c.
  */
 #include <math.h>
 #include <stdint.h>
@@ -287,6 +296,186 @@ void orc_hc_endpoint_codebook(const uint8_t* cluster_px, const uint32_t* cluster
             second = hi;
         }
         endpoints[c] = (uint32_t)first | (uint32_t)second << 16;
+        free(sel);
+    }
+}
+
+/* ------------------------------------------------------------ alpha (H.8) */
+int orc_dxt5a_compute(const uint8_t* values, uint32_t n, int quality, int use_both, uint8_t* selectors,
+                      uint64_t* error, uint8_t* first, uint8_t* second, uint8_t* block_type, uint8_t* reordered);
+
+static const uint8_t k5_to_linear[8] = {0, 7, 1, 2, 3, 4, 5, 6};
+static const uint8_t k5_from_linear[8] = {0, 2, 3, 4, 5, 6, 7, 1};
+
+/* dxt5_block::get_block_values (crn_dxt.cpp:369-455): 8 values when l > h, else 6 + 0 / 255 */
+static void alpha_values_of(uint32_t l, uint32_t h, uint32_t* v)
+{
+    v[0] = l;
+    v[1] = h;
+    if (l > h) {
+        for (uint32_t j = 2; j < 8; j++) v[j] = (l * (8 - j) + h * (j - 1)) / 7;
+    } else {
+        for (uint32_t j = 2; j < 6; j++) v[j] = (l * (6 - j) + h * (j - 1)) / 5;
+        v[6] = 0;
+        v[7] = 255;
+    }
+}
+
+/* optimize_dxt5 (crn_dxt_endpoint_refiner.cpp:146-201) */
+static void optimize_dxt5(const uint8_t* vals, uint32_t n, const uint8_t* sel, float low, float high, uint8_t* out_lo,
+                          uint8_t* out_hi, uint64_t* out_err)
+{
+    const uint8_t L0 = (uint8_t)clamp_int((int)(low * 256.0f), 0, 255);
+    const uint8_t H0 = (uint8_t)clamp_int((int)(high * 256.0f), 0, 255);
+    uint64_t hist[8] = {0}, D2[8] = {0}, DD[8] = {0};
+    for (uint32_t i = 0; i < n; i++) {
+        const uint8_t a = vals[i], s = sel[i];
+        hist[s]++;
+        D2[s] += (uint64_t)(a * 2);
+        DD[s] += (uint64_t)(a * a);
+    }
+    uint16_t sol[529];
+    uint32_t cnt = 0;
+#define CANON(L, H) ((L) == (H) ? ((H) ? (uint16_t)(((H) - 1) << 8 | (L)) : (uint16_t)1) \
+                                : ((L) > (H) ? (uint16_t)((H) << 8 | (L)) : (uint16_t)((L) << 8 | (H))))
+    sol[cnt++] = CANON(L0, H0);
+    const uint8_t minL = L0 <= 11 ? 0 : L0 - 11, maxL = L0 >= 244 ? 255 : L0 + 11;
+    const uint8_t minH = H0 <= 11 ? 0 : H0 - 11, maxH = H0 >= 244 ? 255 : H0 + 11;
+    for (uint16_t L = minL; L <= maxL; L++)
+        for (uint16_t H = minH; H <= maxH; H++)
+            if ((maxH < L || L <= H || H < minL) && (L != L0 || H != H0) && (L != H0 || H != L0))
+                sol[cnt++] = CANON(L, H);
+#undef CANON
+    *out_err = UINT64_MAX;
+    for (uint32_t i = 0; i < cnt; i++) {
+        const uint32_t L = sol[i] & 0xFF, H = sol[i] >> 8;
+        uint32_t v[8];
+        v[0] = L;
+        v[1] = H;
+        for (uint32_t j = 2; j < 8; j++) v[j] = (L * (8 - j) + H * (j - 1)) / 7;   /* get_block_values8 */
+        uint64_t error = 0;
+        for (uint64_t s = 0; s < 8; s++) error += hist[s] * v[s] * v[s] - D2[s] * v[s] + DD[s];
+        if (error < *out_err) {
+            *out_lo = (uint8_t)L;
+            *out_hi = (uint8_t)H;
+            *out_err = error;
+            if (!error) return;
+        }
+    }
+}
+
+/* dxt_endpoint_refiner::refine with DXT5 selectors (crn_dxt_endpoint_refiner.cpp:36-144):
+ * vals: one alpha value per texel; sel: their DXT5 selector codes. */
+int orc_hc_refine_dxt5(const uint8_t* vals, uint32_t n, const uint8_t* sel, uint64_t error_to_beat, uint8_t* low,
+                       uint8_t* high, uint64_t* error)
+{
+    *low = *high = 0;
+    *error = UINT64_MAX;
+    if (!n) return 0;
+    double alpha2_sum = 0.0f, beta2_sum = 0.0f, alphabeta_sum = 0.0f, ax = 0, bx = 0, first = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const double k = (double)((float)k5_to_linear[sel[i]] * 1.0f / 7.0f);
+        const double alpha = 1.0f - k;
+        const double beta = k;
+        const double x = (double)((float)vals[i] / 255.0f);
+        if (!i) first = x;
+        alpha2_sum += alpha * alpha;
+        beta2_sum += beta * beta;
+        alphabeta_sum += alpha * beta;
+        ax += alpha * x;
+        bx += beta * x;
+    }
+    double a, b;
+    if (beta2_sum == 0.0f) {
+        a = ax / alpha2_sum;
+        b = 0.0;
+    } else if (alpha2_sum == 0.0f) {
+        a = 0.0;
+        b = bx / beta2_sum;
+    } else {
+        const double factor = alpha2_sum * beta2_sum - alphabeta_sum * alphabeta_sum;
+        if (factor != 0.0f) {
+            a = (ax * beta2_sum - bx * alphabeta_sum) / factor;
+            b = (bx * alpha2_sum - ax * alphabeta_sum) / factor;
+        } else {
+            a = b = first;
+        }
+    }
+    float l = (float)a, h = (float)b;
+    l = l < 0.0f ? 0.0f : (l > 1.0f ? 1.0f : l);
+    h = h < 0.0f ? 0.0f : (h > 1.0f ? 1.0f : h);
+    optimize_dxt5(vals, n, sel, l, h, low, high, error);
+    return *error < error_to_beat;
+}
+
+/* determine_alpha_endpoint_codebook_task (crn_dxt_hc.cpp:1014-1130) for one
+ * alpha channel, without ETC colour blocks.  cluster_vals: every cluster's
+ * alpha values, cluster c owning [cluster_ofs[c], cluster_ofs[c+1]);
+ * block_cluster[b]: the cluster whose list holds block b (0xFFFFFFFF: none);
+ * block_vals: nblocks x 16 alpha values (the channel this pass encodes);
+ * block_encodings per block.  Outputs per cluster: endpoints[c] = first |
+ * second << 8 | refined << 16 (refined: the refiner's endpoints replaced the
+ * search's), alpha_values (8: the search's values in linear order),
+ * refined_values (8: those of the final endpoints); per block of a cluster:
+ * block_selectors[b] = 16 3-bit selectors (texel 0 on top) << 16 | weight. */
+void orc_hc_alpha_endpoint_codebook(const uint8_t* cluster_vals, const uint32_t* cluster_ofs, uint32_t nclusters,
+                                    const uint32_t* block_cluster, uint32_t nblocks, const uint8_t* block_vals,
+                                    const uint8_t* block_encodings, uint32_t* endpoints, uint8_t* alpha_values,
+                                    uint8_t* refined_values, uint64_t* block_selectors)
+{
+    for (uint32_t c = 0; c < nclusters; c++) {
+        const uint32_t n = cluster_ofs[c + 1] - cluster_ofs[c];
+        endpoints[c] = 0;
+        memset(alpha_values + 8 * c, 0, 8);
+        memset(refined_values + 8 * c, 0, 8);
+        if (!n) continue;
+        const uint8_t* vals = cluster_vals + cluster_ofs[c];
+        uint8_t* sel = (uint8_t*)malloc(n);
+        uint64_t err;
+        uint8_t first, second, type, reordered;
+        orc_dxt5a_compute(vals, n, 4, 0, sel, &err, &first, &second, &type, &reordered);
+        uint32_t bv[8], av[8];
+        alpha_values_of(first, second, bv);
+        for (int i = 0; i < 8; i++) av[i] = bv[k5_from_linear[i]];
+        for (int i = 0; i < 8; i++) alpha_values[8 * c + i] = (uint8_t)av[i];
+        const int delta = (int)first - (int)second;
+        const uint32_t dd = (uint32_t)(delta * delta >> 3);
+        const uint32_t endpoint_weight = dd < 1 ? 1 : (dd > 2048 ? 2048 : dd);
+        uint32_t encoding_weight[8];
+        for (int i = 0; i < 8; i++)
+            encoding_weight[i] = (uint32_t)((float)endpoint_weight * (1.15f + (1.0f - 1.15f) * ((float)i / 7.0f)));
+        for (uint32_t b = 0; b < nblocks; b++) {
+            if (block_cluster[b] != c) continue;
+            const uint32_t weight = encoding_weight[block_encodings[b]];
+            uint64_t selector = 0;
+            for (int p = 0; p < 16; p++) {
+                uint32_t best = UINT32_MAX;
+                uint8_t sb = 0;
+                for (int t = 0; t < 8; t++) {
+                    const uint8_t s = reordered ? (uint8_t)(7 - k5_to_linear[t]) : k5_to_linear[t];
+                    const int d = (int)block_vals[16 * (size_t)b + p] - (int)av[s];
+                    const uint32_t e = (uint32_t)(d >= 0 ? d : -d);
+                    if (e < best) {
+                        sb = s;
+                        best = e;
+                    }
+                }
+                selector = selector << 3 | sb;
+            }
+            block_selectors[b] = selector << 16 | weight;
+        }
+        uint8_t lo, hi;
+        uint64_t rerr;
+        const int refined = orc_hc_refine_dxt5(vals, n, sel, err, &lo, &hi, &rerr);
+        if (refined) {
+            first = lo;
+            second = hi;
+            alpha_values_of(first, second, bv);
+            for (int i = 0; i < 8; i++) refined_values[8 * c + i] = (uint8_t)bv[k5_from_linear[i]];
+        } else {
+            memcpy(refined_values + 8 * c, alpha_values + 8 * c, 8);
+        }
+        endpoints[c] = (uint32_t)first | (uint32_t)second << 8 | (uint32_t)refined << 16;
         free(sel);
     }
 }

@@ -27,3 +27,23 @@ def clusters(side=128, k=48, seed=0, empty=(3,)):
     block_cluster = owner.astype(np.uint32)
     block_cluster[rng.choice(n, 5, replace=False)] = 0xFFFFFFFF
     return np.ascontiguousarray(px), ofs, block_cluster, blocks, weights, enc
+
+
+def alpha_clusters(side=128, k=40, seed=0):
+    """Alpha channel of a seeded RGBA texture, blocks assigned to k clusters of
+    uneven size; a cluster's values are all values of its blocks."""
+    img = texture.synth(side, side, 500 + seed, texture.RGBA)
+    blocks = dxt1.image_blocks(img)[:, :, 3].copy()
+    n = blocks.shape[0]
+    rng = np.random.default_rng(seed)
+    owner = np.minimum((rng.random(n) ** 2 * k).astype(np.int64), k - 1)
+    owner[owner == 2] = 3
+    order = np.argsort(owner, kind="stable")
+    counts = np.bincount(owner, minlength=k)
+    ofs = np.zeros(k + 1, np.uint32)
+    ofs[1:] = np.cumsum(counts * 16)
+    vals = np.ascontiguousarray(blocks[order].reshape(-1))
+    enc = rng.integers(0, 8, n).astype(np.uint8)
+    block_cluster = owner.astype(np.uint32)
+    block_cluster[rng.choice(n, 5, replace=False)] = 0xFFFFFFFF
+    return vals, ofs, block_cluster, np.ascontiguousarray(blocks), enc

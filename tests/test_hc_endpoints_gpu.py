@@ -38,3 +38,20 @@ def test_endpoint_codebook_matches_oracle(gpu, seed, perceptual, side, k):
     inb = np.array([c != 0xFFFFFFFF and not big[c] for c in bc])
     assert np.array_equal(bsel[inb], rbsel[inb])
     assert (bsel[~inb] == 0).all()
+
+
+@pytest.mark.parametrize("seed,side,k", [(0, 128, 40), (1, 256, 64)])
+def test_alpha_endpoint_codebook_matches_oracle(gpu, seed, side, k):
+    from tests.hc_endpoint_cases import alpha_clusters
+    print("hc alpha endpoints seed", seed, "side", side, "clusters", k, flush=True)
+    vals, ofs, bc, blocks, enc = alpha_clusters(side=side, k=k, seed=seed)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    ends, av, rv, bsel = hc.alpha_endpoints(d(vals), d(ofs.astype(np.int32)), d(bc.view(np.int32)), d(blocks), d(enc))
+    torch.cuda.synchronize()
+    ends, av, rv = ends.cpu().numpy().view(np.uint32), av.cpu().numpy(), rv.cpu().numpy()
+    bsel = bsel.cpu().numpy().view(np.uint64)
+    rends, rav, rrv, rbsel = oracle.hc_alpha_endpoints(vals, ofs, bc, blocks, enc)
+    assert np.array_equal(av, rav)
+    assert np.array_equal(ends, rends), np.nonzero(ends != rends)[0][:10]
+    assert np.array_equal(rv, rrv)
+    assert np.array_equal(bsel, rbsel)

@@ -99,3 +99,76 @@ def test_codebook_selectors_and_weights():
         ok, l, h, e = oracle.hc_refine_dxt1(p, sel, True, r.error)
         assert ends[c] == ((l | h << 16) if ok else (r.low | r.high << 16))
     assert checked > 900
+
+
+TO_LIN5 = [0, 7, 1, 2, 3, 4, 5, 6]
+FROM_LIN5 = [0, 2, 3, 4, 5, 6, 7, 1]
+
+
+def values8(l, h):
+    return np.array([l, h] + [(l * (8 - j) + h * (j - 1)) // 7 for j in range(2, 8)], np.int64)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_dxt5_refiner_recovers_palettes(seed):
+    rng = np.random.default_rng(10 + seed)
+    h = int(rng.integers(0, 250))
+    l = int(rng.integers(h + 2, 256))
+    n = int(rng.integers(16, 400))
+    sel = rng.integers(0, 8, n).astype(np.uint8)
+    sel[:8] = np.arange(8)
+    vals = values8(l, h)[sel].astype(np.uint8)
+    ok, lo, hi, err = oracle.hc_refine_dxt5(vals, sel)
+    assert ok and err == 0
+    assert np.array_equal(values8(lo, hi), values8(l, h))
+
+
+def test_dxt5_refiner_error_is_fixed_selector_error():
+    from tests.hc_endpoint_cases import alpha_clusters
+    vals, ofs, *_ = alpha_clusters(seed=1)
+    rng = np.random.default_rng(4)
+    for c in range(len(ofs) - 1):
+        v = vals[ofs[c]:ofs[c + 1]]
+        if not len(v):
+            continue
+        s = rng.integers(0, 8, len(v)).astype(np.uint8)
+        _, lo, hi, err = oracle.hc_refine_dxt5(v, s)
+        assert err == int(((v.astype(np.int64) - values8(lo, hi)[s]) ** 2).sum())
+
+
+def test_alpha_codebook_selectors_and_weights():
+    from tests.hc_endpoint_cases import alpha_clusters
+    vals, ofs, bc, blocks, enc = alpha_clusters(seed=2)
+    ends, av, rv, bsel = oracle.hc_alpha_endpoints(vals, ofs, bc, blocks, enc)
+    f32 = np.float32
+    checked = 0
+    for c in range(len(ofs) - 1):
+        v = vals[ofs[c]:ofs[c + 1]]
+        mine = np.nonzero(bc == c)[0]
+        if not len(v):
+            assert (bsel[mine] == 0).all() and ends[c] == 0
+            continue
+        sel, err, first, second, _, reord = oracle.dxt5a_values(v)
+        assert np.array_equal(av[c].astype(np.int64), values8(first, second)[FROM_LIN5] if first > second else
+                              np.array([first, second] + [(first * (6 - j) + second * (j - 1)) // 5 for j in range(2, 6)]
+                                       + [0, 255], np.int64)[FROM_LIN5])
+        ok, lo, hi, rerr = oracle.hc_refine_dxt5(v, sel, err)
+        f, s2 = (lo, hi) if ok else (first, second)
+        assert int(ends[c]) == f | s2 << 8 | int(ok) << 16
+        fv = values8(f, s2) if f > s2 else np.array([f, s2] + [(f * (6 - j) + s2 * (j - 1)) // 5 for j in range(2, 6)]
+                                                    + [0, 255], np.int64)
+        assert np.array_equal(rv[c].astype(np.int64), fv[FROM_LIN5])
+        a = av[c].astype(np.int64)
+        d = first - second
+        ew = min(max((d * d) >> 3, 1), 2048)
+        order = [(7 - TO_LIN5[t]) if reord else TO_LIN5[t] for t in range(8)]
+        for b in mine:
+            dist = np.abs(blocks[b][:, None].astype(np.int64) - a[np.array(order)][None, :])
+            s = np.array(order)[np.argmin(dist, 1)]
+            word = 0
+            for x in s:
+                word = (word << 3) | int(x)
+            wt = int(f32(ew) * (f32(1.15) + (f32(1.0) - f32(1.15)) * (f32(enc[b]) / f32(7.0))))
+            assert int(bsel[b]) == (word << 16) | wt, (c, b)
+            checked += 1
+    assert checked > 900
