@@ -172,6 +172,180 @@ __global__ void __launch_bounds__(256) crn2_hc_sel_scatter_kernel(SelArgs a)
     if (a.used && p == 0) a.used[s] = 1;
 }
 
+// ----------------------------------------------------------------------------
+// Alpha selector assignment (dxt_hc::create_alpha_selector_codebook_task,
+// crnlib/crn_dxt_hc.cpp:1516-1586): E3[p][v] = (a_p - value_v)^2 over the 8
+// values of the block's cluster, error(b, s) = sum_p E3[p][sel_s(p)] with 3-bit
+// selectors.  As a product: E3 (n x 128, k = 8p + v) times a one-hot H
+// (128 x m).  E3 < 2^16 splits into three 7-bit limbs; a limb sum is at most
+// 16 * 127, the recombined error below 2^21: exact.
+
+struct AlphaSelArgs {
+    const uint8_t* block_vals;       // n_blocks x 16 values of the channel
+    const uint32_t* endpoint_index;  // n_blocks
+    const uint8_t* values;           // n_clusters x 8 (the assignment's palette)
+    const uint8_t* scatter_values;   // n_clusters x 8 (the totals' palette)
+    const uint64_t* selectors;       // n_selectors, texel p at bits 3p..3p+2
+    uint32_t* best_index;
+    uint32_t* best_error;            // may be null
+    uint32_t* details;               // n_selectors x 128, may be null
+    uint8_t* used;                   // may be null
+    uint32_t n_blocks, n_selectors;
+};
+
+constexpr uint32_t kAlphaSelChunk = 4096;   // u64 selectors staged in LDS at a time (32 KiB)
+
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) crn2_hc_alpha_sel_gemm_kernel(AlphaSelArgs a)
+{
+    constexpr int NL = 3;
+    constexpr uint32_t kRowsPerWG = kRowsPerWave * WAVES;
+    __shared__ uint64_t s_sel[kAlphaSelChunk];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t g = lane >> 5, col = lane & 31u;
+    const uint32_t row0 = blockIdx.x * kRowsPerWG + wave * kRowsPerWave;
+    const uint32_t arow = row0 + col;
+
+    // Fragment t covers k = 32 t + 16 g + j (j = 0..15): texel p = 4t + 2g + j/8,
+    // value v = j % 8; operand word q holds j = 4q..4q+3, i.e. texel
+    // 4t + 2g + q/2 and values 4(q%2)..4(q%2)+3.
+    v4i A[NL][4];
+    {
+        uint32_t vals[8];
+        const uint8_t* bv = nullptr;
+        if (arow < a.n_blocks) {
+            const uint8_t* cv = a.values + (size_t)a.endpoint_index[arow] * 8;
+            for (int v = 0; v < 8; v++) vals[v] = cv[v];
+            bv = a.block_vals + (size_t)arow * 16;
+        }
+        for (uint32_t t = 0; t < 4; t++) {
+            uint32_t w[NL][4];
+            for (uint32_t q = 0; q < 4; q++) {
+                const uint32_t p = 4 * t + 2 * g + (q >> 1);
+                uint32_t e[4] = {0, 0, 0, 0};
+                if (bv) {
+                    const int x = bv[p];
+                    for (uint32_t c = 0; c < 4; c++) {
+                        const int d = x - (int)vals[4 * (q & 1) + c];
+                        e[c] = (uint32_t)(d * d);
+                    }
+                }
+                for (int l = 0; l < NL; l++) {
+                    uint32_t v = 0;
+                    for (uint32_t c = 0; c < 4; c++) v |= ((e[c] >> (7 * l)) & 127u) << (8 * c);
+                    w[l][q] = v;
+                }
+            }
+            for (int l = 0; l < NL; l++) A[l][t] = v4i{(int)w[l][0], (int)w[l][1], (int)w[l][2], (int)w[l][3]};
+        }
+    }
+
+    uint32_t best[16], bt[16];
+    for (int i = 0; i < 16; i++) {
+        best[i] = 0xFFFFFFFFu;
+        bt[i] = 0;
+    }
+    for (uint32_t c0 = 0; c0 < a.n_selectors; c0 += kAlphaSelChunk) {
+        const uint32_t cn = min(kAlphaSelChunk, a.n_selectors - c0);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < cn; i += blockDim.x) s_sel[i] = a.selectors[c0 + i];
+        __syncthreads();
+        for (uint32_t t0 = 0; t0 < cn; t0 += 32) {
+            const bool valid = t0 + col < cn;
+            const uint64_t sel = valid ? s_sel[t0 + col] : 0ull;
+            v16i acc[NL];
+            for (int l = 0; l < NL; l++) acc[l] = v16i{0};
+            for (uint32_t t = 0; t < 4; t++) {
+                uint32_t w[4];
+                for (uint32_t q = 0; q < 4; q++) {
+                    const uint32_t p = 4 * t + 2 * g + (q >> 1);
+                    const uint32_t s3 = (uint32_t)(sel >> (3 * p)) & 7u;
+                    w[q] = (s3 >> 2) == (q & 1) ? 1u << (8 * (s3 & 3u)) : 0u;
+                }
+                const v4i B = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+                for (int l = 0; l < NL; l++) acc[l] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[l][t], B, acc[l], 0, 0, 0);
+            }
+            const uint32_t tile = (c0 + t0) >> 5;
+            for (int i = 0; i < 16; i++) {
+                const uint32_t e = (uint32_t)acc[0][i] + ((uint32_t)acc[1][i] << 7) + ((uint32_t)acc[2][i] << 14);
+                if (valid && e < best[i]) {
+                    best[i] = e;
+                    bt[i] = tile;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < 16; i++) {
+        uint32_t e = best[i];
+        uint32_t idx = bt[i] * 32 + col;
+        for (int off = 1; off < 32; off <<= 1) {
+            const uint32_t oe = (uint32_t)__shfl_xor((int)e, off);
+            const uint32_t oi = (uint32_t)__shfl_xor((int)idx, off);
+            if (oe < e || (oe == e && oi < idx)) {
+                e = oe;
+                idx = oi;
+            }
+        }
+        const uint32_t r = row0 + 8 * (i >> 2) + 4 * g + (i & 3);
+        if (col == 0 && r < a.n_blocks) {
+            a.best_index[r] = idx;
+            if (a.best_error) a.best_error[r] = e;
+        }
+    }
+}
+
+// Adds each block's E3 against its cluster's scatter palette into the totals
+// of its chosen selector (integer atomics: exact, order-independent).
+__global__ void __launch_bounds__(256) crn2_hc_alpha_sel_scatter_kernel(AlphaSelArgs a)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t b = t >> 4;
+    const uint32_t p = (uint32_t)(t & 15u);
+    if (b >= a.n_blocks) return;
+    const int x = a.block_vals[b * 16 + p];
+    const uint8_t* cv = a.scatter_values + (size_t)a.endpoint_index[b] * 8;
+    const uint32_t s = a.best_index[b];
+    if (a.details)
+        for (uint32_t v = 0; v < 8; v++) {
+            const int d = x - (int)cv[v];
+            atomicAdd(&a.details[(size_t)s * 128 + 8 * p + v], (uint32_t)(d * d));
+        }
+    if (a.used && p == 0) a.used[s] = 1;
+}
+
+// Per codebook entry and texel, the value with the smallest accumulated error,
+// chosen by the reference's pairwise tournament (its tie order is not "lowest
+// value first": crn_dxt_hc.cpp:1493-1501 for 2-bit, 1706-1717 for 3-bit).
+__global__ void __launch_bounds__(256) crn2_hc_sel_refine_kernel(const uint32_t* __restrict__ details, uint32_t n,
+                                                                 int bits, void* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (bits == 2) {
+        const uint32_t* e = details + (size_t)i * 64;
+        uint32_t sel = 0;
+        for (uint32_t p = 0; p < 16; p++, e += 4) {
+            const uint32_t s03 = e[3] < e[0] ? 3 : 0;
+            const uint32_t s12 = e[2] < e[1] ? 2 : 1;
+            sel |= (e[s12] < e[s03] ? s12 : s03) << (2 * p);
+        }
+        static_cast<uint32_t*>(out)[i] = sel;
+    } else {
+        const uint32_t* e = details + (size_t)i * 128;
+        uint64_t sel = 0;
+        for (uint32_t p = 0; p < 16; p++, e += 8) {
+            const uint32_t s07 = e[7] < e[0] ? 7 : 0;
+            const uint32_t s12 = e[2] < e[1] ? 2 : 1;
+            const uint32_t s34 = e[4] < e[3] ? 4 : 3;
+            const uint32_t s56 = e[6] < e[5] ? 6 : 5;
+            const uint32_t s02 = e[s12] < e[s07] ? s12 : s07;
+            const uint32_t s36 = e[s56] < e[s34] ? s56 : s34;
+            sel |= (uint64_t)(e[s36] < e[s02] ? s36 : s02) << (3 * p);
+        }
+        static_cast<uint64_t*>(out)[i] = sel;
+    }
+}
+
 }  // namespace
 
 extern "C" int crn2_hc_color_selectors_device(const void* blocks, const uint32_t* endpoint_index,
@@ -221,6 +395,47 @@ extern "C" int crn2_hc_color_selectors_device(const void* blocks, const uint32_t
         if (hipGetLastError() != hipSuccess) return CRN2_EHIP;
     }
     return CRN2_OK;
+}
+
+extern "C" int crn2_hc_alpha_selectors_device(const void* block_vals, const uint32_t* endpoint_index,
+                                              const void* values, const void* scatter_values, uint32_t n_blocks,
+                                              const uint64_t* selectors, uint32_t n_selectors, uint32_t* best_index,
+                                              uint32_t* best_error, uint32_t* details, uint8_t* used, void* stream)
+{
+    if (!n_blocks) return CRN2_OK;
+    if (!block_vals || !endpoint_index || !values || !selectors || !n_selectors || !best_index) return CRN2_EINVAL;
+    AlphaSelArgs a;
+    a.block_vals = static_cast<const uint8_t*>(block_vals);
+    a.endpoint_index = endpoint_index;
+    a.values = static_cast<const uint8_t*>(values);
+    a.scatter_values = static_cast<const uint8_t*>(scatter_values ? scatter_values : values);
+    a.selectors = selectors;
+    a.best_index = best_index;
+    a.best_error = best_error;
+    a.details = details;
+    a.used = used;
+    a.n_blocks = n_blocks;
+    a.n_selectors = n_selectors;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t wgs = (n_blocks + kRowsPerWave * 4 - 1) / (kRowsPerWave * 4);
+    hipLaunchKernelGGL(crn2_hc_alpha_sel_gemm_kernel<4>, dim3(wgs), dim3(256), 0, s, a);
+    if (hipGetLastError() != hipSuccess) return CRN2_EHIP;
+    if (details || used) {
+        const uint64_t n = (uint64_t)n_blocks * 16;
+        hipLaunchKernelGGL(crn2_hc_alpha_sel_scatter_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, a);
+        if (hipGetLastError() != hipSuccess) return CRN2_EHIP;
+    }
+    return CRN2_OK;
+}
+
+extern "C" int crn2_hc_selector_refine_device(const uint32_t* details, uint32_t n_selectors, int bits, void* selectors,
+                                              void* stream)
+{
+    if (!n_selectors) return CRN2_OK;
+    if (!details || !selectors || (bits != 2 && bits != 3)) return CRN2_EINVAL;
+    hipLaunchKernelGGL(crn2_hc_sel_refine_kernel, dim3((n_selectors + 255) / 256), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), details, n_selectors, bits, selectors);
+    return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
 }
 
 namespace crn2 {

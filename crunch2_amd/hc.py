@@ -1,5 +1,6 @@
-"""dxt_hc colour selector assignment on the GPU (crnlib/crn_dxt_hc.cpp:1306-1360)
--- Python face of crn2_hc_color_selectors_device."""
+"""dxt_hc on the GPU (crnlib/crn_dxt_hc.cpp): selector assignment (colour
+:1306-1360, alpha :1516-1586), selector codebook refinement, colour and alpha
+endpoint codebooks -- Python face of the crn2_hc_* entry points."""
 import ctypes as C
 
 import torch
@@ -77,3 +78,51 @@ def alpha_endpoints(cluster_vals, cluster_ofs, block_cluster, block_vals, block_
                                                _ptr(ends), _ptr(av), _ptr(rv), _ptr(bsel), C.c_void_p(stream)),
           "crn2_hc_alpha_endpoints_device")
     return ends, av, rv, bsel
+
+
+def alpha_selectors(block_vals, endpoint_index, values, selectors, scatter_values=None, details=None, used=None):
+    """block_vals: uint8 CUDA tensor [n, 16]; endpoint_index: int32 [n];
+    values / scatter_values: uint8 [k, 8]; selectors: int64 [m] (3 bits per
+    texel).  Returns (best_index int32 [n], best_error int32 [n]); details
+    (int32 [m, 128]) and used (uint8 [m]) are accumulated in place when given."""
+    n = block_vals.shape[0]
+    m = selectors.shape[0]
+    dev = block_vals.device
+    best = torch.empty(n, dtype=torch.int32, device=dev)
+    err = torch.empty(n, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    sv = scatter_values.contiguous() if scatter_values is not None else None
+    check(lib().crn2_hc_alpha_selectors_device(_ptr(block_vals.contiguous()), _ptr(endpoint_index.contiguous()),
+                                               _ptr(values.contiguous()), _ptr(sv), n,
+                                               _ptr(selectors.contiguous()), m, _ptr(best), _ptr(err),
+                                               _ptr(details), _ptr(used), C.c_void_p(stream)),
+          "crn2_hc_alpha_selectors_device")
+    return best, err
+
+
+def selector_refine(details, bits):
+    """details: int32 CUDA tensor of accumulated totals, [m, 64] for bits 2 or
+    [m, 128] for bits 3.  Returns the refined codebook:
+    int32 [m] (bits 2) or int64 [m] (bits 3)."""
+    m = details.shape[0]
+    out = torch.empty(m, dtype=torch.int32 if bits == 2 else torch.int64, device=details.device)
+    stream = torch.cuda.current_stream().cuda_stream
+    check(lib().crn2_hc_selector_refine_device(_ptr(details.contiguous()), m, int(bits), _ptr(out),
+                                               C.c_void_p(stream)), "crn2_hc_selector_refine_device")
+    return out
+
+
+def tree_codebook(vectors, weights, max_splits, leaves=True):
+    """Top-down VQ codebook (crn_tree_clusterizer.h:89-171).  vectors: float32
+    CUDA tensor [n, 6] or [n, 16]; weights: int32 [n] (non-negative).  Returns
+    (codebook float32 [k, dim], vector_leaf int32 [n] or None)."""
+    n, dim = vectors.shape
+    dev = vectors.device
+    cb = torch.empty((max_splits, dim), dtype=torch.float32, device=dev)
+    leaf = torch.empty(n, dtype=torch.int32, device=dev) if leaves else None
+    k = C.c_uint32(0)
+    stream = torch.cuda.current_stream().cuda_stream
+    check(lib().crn2_tree_codebook_device(_ptr(vectors.contiguous()), _ptr(weights.contiguous()), n, dim,
+                                          max_splits, _ptr(cb), C.byref(k), _ptr(leaf), C.c_void_p(stream)),
+          "crn2_tree_codebook_device")
+    return cb[:k.value], leaf

@@ -213,6 +213,50 @@ int crn2_hc_color_selectors_device(const void* blocks, const uint32_t* endpoint_
                                    uint32_t n_selectors, uint32_t* best_index, uint32_t* best_error,
                                    uint32_t* details, uint8_t* used, void* stream);
 
+/* dxt_hc::create_alpha_selector_codebook_task (crnlib/crn_dxt_hc.cpp:1516-1586)
+ * for one alpha channel and all blocks at once, on the device.  block_vals:
+ * n_blocks x 16 values of the channel; endpoint_index: the alpha cluster of
+ * each block; values: per cluster its 8 values in selector order
+ * (alpha_cluster::alpha_values), used to choose the selector;
+ * scatter_values: per cluster the values the totals are taken against
+ * (refined_alpha_values when the refiner replaced the endpoints, else the same
+ * as values; NULL means values); selectors: the codebook, 3 bits per texel,
+ * texel p at bits 3p..3p+2.  Writes per block the first codebook entry with
+ * the smallest error (best_index) and that error (best_error, optional); adds
+ * the block's 16 x 8 table into details[best_index * 128 + 8 * texel + value]
+ * (optional, u32, accumulated, not cleared: call once per channel for DXN)
+ * and sets used[best_index] = 1 (optional). */
+int crn2_hc_alpha_selectors_device(const void* block_vals, const uint32_t* endpoint_index, const void* values,
+                                   const void* scatter_values, uint32_t n_blocks, const uint64_t* selectors,
+                                   uint32_t n_selectors, uint32_t* best_index, uint32_t* best_error,
+                                   uint32_t* details, uint8_t* used, void* stream);
+
+/* The last step of dxt_hc::create_color_selector_codebook (bits 2,
+ * crnlib/crn_dxt_hc.cpp:1491-1503) and create_alpha_selector_codebook (bits 3,
+ * :1702-1719): each codebook entry becomes, per texel, the code with the
+ * smallest accumulated error in details (n_selectors x 16 x 4 or x 8 u32),
+ * ties settled by the reference's pairwise tournament.  Writes selectors as
+ * u32 (bits 2) or u64 (bits 3), texel p at bits p*bits. */
+int crn2_hc_selector_refine_device(const uint32_t* details, uint32_t n_selectors, int bits, void* selectors,
+                                   void* stream);
+
+/* ------------------------------------------------- VQ tree clusterizer */
+/* tree_clusterizer<vecNF>::generate_codebook (crnlib/crn_tree_clusterizer.h:
+ * 89-171, split_node :265-570) as dxt_hc calls it: top-down splits of the
+ * highest-variance node, max_splits - 1 of them at most, on the device.
+ * vectors: n x dim floats (dim 6: endpoint vectors, 16: selector vectors);
+ * weights: n.  Writes the codebook (up to max_splits x dim floats, leaves in
+ * node order), its size (*codebook_size, host memory) and, optionally, the
+ * codebook index of each vector (vector_leaf, n).  Float sums run in a fixed
+ * chunked order (documented with the kernels), and with one thread's
+ * behaviour where the reference splits work across its helper threads.
+ * Returns when the result is complete; allocates stream-ordered scratch (not
+ * for graph capture).  CRN2_EINVAL for zero vectors, splits or total weight;
+ * CRN2_EUNSUPPORTED for another dim. */
+int crn2_tree_codebook_device(const float* vectors, const uint32_t* weights, uint32_t n, uint32_t dim,
+                              uint32_t max_splits, float* codebook, uint32_t* codebook_size, uint32_t* vector_leaf,
+                              void* stream);
+
 /* ------------------------------------- dxt_hc colour endpoint codebook */
 /* dxt_hc::determine_color_endpoint_codebook_task (crnlib/crn_dxt_hc.cpp:663-754)
  * with dxt_endpoint_refiner::refine (crnlib/crn_dxt_endpoint_refiner.cpp:

@@ -247,6 +247,59 @@ def hc_color_selectors(blocks, endpoint_index, cluster_colors, selectors, subblo
     return best, err, det, used
 
 
+def hc_alpha_selectors(block_vals, endpoint_index, values, selectors, scatter_values=None):
+    """dxt_hc alpha selector assignment (crn_dxt_hc.cpp:1516-1586).
+
+    block_vals: (n, 16) uint8; endpoint_index: (n,) uint32; values /
+    scatter_values: (k, 8) uint8; selectors: (m,) uint64 (3 bits per texel).
+    Returns (best_index, best_error, details (m, 128), used (m,))."""
+    L = lib()
+    L.orc_hc_alpha_selectors.argtypes = [C.c_void_p] * 4 + [C.c_uint32, C.c_void_p, C.c_uint32] + [C.c_void_p] * 4
+    bv = np.ascontiguousarray(block_vals, np.uint8)
+    ei = np.ascontiguousarray(endpoint_index, np.uint32)
+    v = np.ascontiguousarray(values, np.uint8)
+    sv = None if scatter_values is None else np.ascontiguousarray(scatter_values, np.uint8)
+    sel = np.ascontiguousarray(selectors, np.uint64)
+    n, m = bv.shape[0], sel.shape[0]
+    best = np.zeros(n, np.uint32)
+    err = np.zeros(n, np.uint32)
+    det = np.zeros((m, 128), np.uint32)
+    used = np.zeros(m, np.uint8)
+    L.orc_hc_alpha_selectors(_p(bv), _p(ei), _p(v), _p(sv) if sv is not None else None, n, _p(sel), m, _p(best),
+                             _p(err), _p(det), _p(used))
+    return best, err, det, used
+
+
+def hc_selector_refine(details, bits):
+    """Per codebook entry the tournament argmin of the accumulated totals
+    (crn_dxt_hc.cpp:1491-1503 for bits 2, 1702-1719 for bits 3)."""
+    L = lib()
+    L.orc_hc_selector_refine.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]
+    d = np.ascontiguousarray(details, np.uint32)
+    n = d.shape[0]
+    out = np.zeros(n, np.uint32 if bits == 2 else np.uint64)
+    L.orc_hc_selector_refine(_p(d), n, bits, _p(out))
+    return out
+
+
+def tree_codebook(vectors, weights, max_splits):
+    """tree_clusterizer::generate_codebook (crn_tree_clusterizer.h:89-171) in
+    the fixed chunked summation order.  vectors: (n, dim) float32, dim <= 16;
+    weights: (n,) uint32.  Returns (codebook (k, dim) float32, vector_leaf (n,))."""
+    L = lib()
+    L.orc_tree_codebook.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                    C.POINTER(C.c_uint32), C.c_void_p]
+    v = np.ascontiguousarray(vectors, np.float32)
+    w = np.ascontiguousarray(weights, np.uint32)
+    n, dim = v.shape
+    cb = np.zeros((max_splits, dim), np.float32)
+    leaf = np.zeros(n, np.uint32)
+    k = C.c_uint32(0)
+    if L.orc_tree_codebook(_p(v), _p(w), n, dim, max_splits, _p(cb), C.byref(k), _p(leaf)) != 0:
+        raise ValueError("orc_tree_codebook: invalid arguments")
+    return cb[:k.value].copy(), leaf
+
+
 def hc_color_endpoints(cluster_px, cluster_ofs, block_cluster, blocks, block_weights, block_encodings,
                        perceptual=True):
     """dxt_hc colour endpoint codebook (crn_dxt_hc.cpp:663-754) with the

@@ -54,3 +54,44 @@ def brute_force(blocks, ei, cols, sel, subblocks=False, perceptual=True):
     det = np.zeros((sel.shape[0], 64), np.int64)
     np.add.at(det, best, e2.reshape(n, 64))
     return best, err[np.arange(n), best], det, e2
+
+
+def make_alpha_case(n_blocks, n_clusters, n_selectors, seed, dup=0.1):
+    """Alpha blocks (one channel of a synthetic texture, with some flat and
+    extreme blocks); per cluster an 8-value DXT5 palette in selector order and
+    a second palette (the refined values) one step away; random 3-bit
+    codebook selectors with a fraction `dup` of exact duplicates."""
+    rng = np.random.default_rng(seed)
+    side = 4 * int(np.ceil(np.sqrt(n_blocks)))
+    vals = np.ascontiguousarray(blocks_from_texture(max(8, side), seed)[:n_blocks, :, 1])
+    vals[::7] = 0
+    vals[3::11] = 255
+    lo = rng.integers(0, 256, n_clusters)
+    hi = rng.integers(0, 256, n_clusters)
+    pal = np.zeros((n_clusters, 8), np.uint8)
+    for i, (a, b) in enumerate(((7, 0), (0, 7), (6, 1), (5, 2), (4, 3), (3, 4), (2, 5), (1, 6))):
+        pal[:, i] = (a * lo + b * hi) // 7
+    pal2 = np.clip(pal.astype(np.int64) + rng.integers(-1, 2, pal.shape), 0, 255).astype(np.uint8)
+    ei = rng.integers(0, n_clusters, n_blocks).astype(np.uint32)
+    sel = rng.integers(0, 2**48, n_selectors, dtype=np.uint64)
+    nd = int(dup * n_selectors)
+    if nd:
+        sel[rng.integers(0, n_selectors, nd)] = sel[rng.integers(0, n_selectors, nd)]
+    return vals, ei, pal, pal2, sel
+
+
+def alpha_brute_force(vals, ei, pal, sel, scatter=None):
+    """Direct sums over the whole block x codebook error matrix: argmin
+    (first minimum), its error, and the 16 x 8 totals per chosen entry."""
+    n = vals.shape[0]
+    e3 = (vals[:, :, None].astype(np.int64) - pal[ei][:, None, :].astype(np.int64)) ** 2     # n,16,8
+    codes = ((sel[:, None] >> (3 * np.arange(16, dtype=np.uint64))[None, :]) & 7).astype(np.int64)
+    err = np.zeros((n, sel.shape[0]), np.int64)
+    for p in range(16):
+        err += e3[:, p, :][:, codes[:, p]]
+    best = err.argmin(1)
+    sp = pal if scatter is None else scatter
+    es = (vals[:, :, None].astype(np.int64) - sp[ei][:, None, :].astype(np.int64)) ** 2
+    det = np.zeros((sel.shape[0], 128), np.int64)
+    np.add.at(det, best, es.reshape(n, 128))
+    return best, err[np.arange(n), best], det
