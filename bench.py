@@ -98,13 +98,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=1, help="texture pairs (distinct seeds)")
-    ap.add_argument("--copies", type=int, default=16,
+    ap.add_argument("--copies", type=int, default=64,
                     help="independent copies of every file per step (separate contexts and outputs), so that one "
                          "launch sequence moves hundreds of MB")
     ap.add_argument("--mixed", type=int, default=0,
                     help="instead of the 4096^2 pair, a mixed set of this many textures (sides log-uniform in "
                          "[64, 2048], full mips; 60%% DXT1 RGB, 25%% DXT5 RGBA, 15%% DXN normal maps)")
-    ap.add_argument("--streams", type=int, default=2, help="plans (groups of copies) per step, one stream each")
+    ap.add_argument("--streams", type=int, default=4, help="plans (groups of copies) per step, one stream each")
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline sample length")
     ap.add_argument("--encode-size", type=int, default=8192,

@@ -126,3 +126,47 @@ def tree_codebook(vectors, weights, max_splits, leaves=True):
                                           max_splits, _ptr(cb), C.byref(k), _ptr(leaf), C.c_void_p(stream)),
           "crn2_tree_codebook_device")
     return cb[:k.value], leaf
+
+
+def color_selector_codebook(block_selectors, codebook_size, blocks, endpoint_index, cluster_colors,
+                            perceptual=True):
+    """dxt_hc colour selector codebook (crn_dxt_hc.cpp:1379-1504).
+    block_selectors: int64 CUDA tensor [n] (selector << 32 | weight); blocks
+    uint8 [n, 16, 4]; endpoint_index int32 [n]; cluster_colors uint8 [k, 4, 4].
+    Returns (selectors int32 [m], selector_index int32 [n], used uint8 [m])."""
+    n = blocks.shape[0]
+    dev = blocks.device
+    sel = torch.empty(codebook_size, dtype=torch.int32, device=dev)
+    idx = torch.empty(n, dtype=torch.int32, device=dev)
+    used = torch.empty(codebook_size, dtype=torch.uint8, device=dev)
+    m = C.c_uint32(0)
+    stream = torch.cuda.current_stream().cuda_stream
+    check(lib().crn2_hc_color_selector_codebook_device(_ptr(block_selectors.contiguous()), n, codebook_size,
+                                                       _ptr(blocks.contiguous()), _ptr(endpoint_index.contiguous()),
+                                                       _ptr(cluster_colors.contiguous()), int(bool(perceptual)),
+                                                       _ptr(sel), C.byref(m), _ptr(idx), _ptr(used),
+                                                       C.c_void_p(stream)),
+          "crn2_hc_color_selector_codebook_device")
+    return sel[:m.value], idx, used[:m.value]
+
+
+def alpha_selector_codebook(block_selectors, codebook_size, block_vals, endpoint_index, alpha_values,
+                            refined_values=None):
+    """dxt_hc alpha selector codebook (crn_dxt_hc.cpp:1588-1720) for one or
+    two channels.  block_selectors int64 [c, n]; block_vals uint8 [c, n, 16];
+    endpoint_index int32 [c, n]; alpha_values / refined_values uint8 [k, 8].
+    Returns (selectors int64 [m], selector_index int32 [c, n], used uint8 [m])."""
+    nc, n = block_selectors.shape
+    dev = block_vals.device
+    sel = torch.empty(codebook_size, dtype=torch.int64, device=dev)
+    idx = torch.empty((nc, n), dtype=torch.int32, device=dev)
+    used = torch.empty(codebook_size, dtype=torch.uint8, device=dev)
+    m = C.c_uint32(0)
+    stream = torch.cuda.current_stream().cuda_stream
+    rv = refined_values.contiguous() if refined_values is not None else None
+    check(lib().crn2_hc_alpha_selector_codebook_device(_ptr(block_selectors.contiguous()), n, nc, codebook_size,
+                                                       _ptr(block_vals.contiguous()), _ptr(endpoint_index.contiguous()),
+                                                       _ptr(alpha_values.contiguous()), _ptr(rv), _ptr(sel),
+                                                       C.byref(m), _ptr(idx), _ptr(used), C.c_void_p(stream)),
+          "crn2_hc_alpha_selector_codebook_device")
+    return sel[:m.value], idx, used[:m.value]

@@ -257,6 +257,37 @@ int crn2_tree_codebook_device(const float* vectors, const uint32_t* weights, uin
                               uint32_t max_splits, float* codebook, uint32_t* codebook_size, uint32_t* vector_leaf,
                               void* stream);
 
+/* ------------------------------------------- dxt_hc selector codebooks */
+/* dxt_hc::create_color_selector_codebook (crnlib/crn_dxt_hc.cpp:1379-1504) on
+ * the device: the blocks' selector words (block_selectors[b] = 16 selectors,
+ * texel 0 in the top bits, << 32 | weight, as crn2_hc_color_endpoints_device
+ * writes them) merged, the tree clusterizer's codebook of at most
+ * codebook_size entries, every block assigned its best entry against its
+ * cluster's colours (endpoint_index, cluster_colors as for
+ * crn2_hc_color_selectors_device), and every entry refined to its texels'
+ * best codes.  Writes selectors (codebook_size capacity; texel p at bits 2p),
+ * *n_selectors (host memory), selector_index per block and used per entry.
+ * Returns when done; allocates stream-ordered scratch (not for graph
+ * capture). */
+int crn2_hc_color_selector_codebook_device(const uint64_t* block_selectors, uint32_t n_blocks, uint32_t codebook_size,
+                                           const void* blocks, const uint32_t* endpoint_index,
+                                           const void* cluster_colors, int perceptual, uint32_t* selectors,
+                                           uint32_t* n_selectors, uint32_t* selector_index, uint8_t* used,
+                                           void* stream);
+
+/* dxt_hc::create_alpha_selector_codebook (crnlib/crn_dxt_hc.cpp:1588-1720) for
+ * one (DXT5, DXT5A) or two (DXN) alpha channels: block_selectors, block_vals,
+ * endpoint_index and selector_index hold n_channels consecutive arrays of
+ * n_blocks entries (words = 16 3-bit selectors, texel 0 on top, << 16 |
+ * weight; 16 values per block); alpha_values / refined_values per alpha
+ * cluster as for crn2_hc_alpha_selectors_device.  Writes selectors (u64, texel
+ * p at bits 3p), *n_selectors (host memory), selector_index and used. */
+int crn2_hc_alpha_selector_codebook_device(const uint64_t* block_selectors, uint32_t n_blocks, uint32_t n_channels,
+                                           uint32_t codebook_size, const void* block_vals,
+                                           const uint32_t* endpoint_index, const void* alpha_values,
+                                           const void* refined_values, uint64_t* selectors, uint32_t* n_selectors,
+                                           uint32_t* selector_index, uint8_t* used, void* stream);
+
 /* ------------------------------------- dxt_hc colour endpoint codebook */
 /* dxt_hc::determine_color_endpoint_codebook_task (crnlib/crn_dxt_hc.cpp:663-754)
  * with dxt_endpoint_refiner::refine (crnlib/crn_dxt_endpoint_refiner.cpp:

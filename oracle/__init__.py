@@ -300,6 +300,72 @@ def tree_codebook(vectors, weights, max_splits):
     return cb[:k.value].copy(), leaf
 
 
+def hc_merge_selectors(words, bits):
+    """The merge that opens create_color_selector_codebook (bits 2,
+    crn_dxt_hc.cpp:1381-1448) and create_alpha_selector_codebook (bits 3,
+    :1590-1655): words = selector << (32 | 16) | weight, sorted ascending,
+    equal selectors merged with weights summed and held at 2^32 - 1; each
+    distinct selector becomes 16 floats, texel t (texel 0 in the top bits) ->
+    (code + 0.5) / 2^bits.  Returns (vectors (m, 16) float32, weights (m,))."""
+    wbits = 32 if bits == 2 else 16
+    words = np.sort(np.asarray(words, np.uint64))
+    sel = words >> np.uint64(wbits)
+    w = (words & np.uint64((1 << wbits) - 1)).astype(np.int64)
+    starts = np.flatnonzero(np.r_[True, sel[1:] != sel[:-1]])
+    usel = sel[starts]
+    uw = np.minimum(np.add.reduceat(w, starts), 2**32 - 1).astype(np.uint32)
+    shifts = np.uint64(bits) * np.arange(16, dtype=np.uint64)
+    codes = ((usel[:, None] >> shifts[None, :]) & np.uint64((1 << bits) - 1)).astype(np.float32)   # [:, p] = texel 15 - p
+    vec = ((codes + np.float32(0.5)) * np.float32(1.0 / (1 << bits)))[:, ::-1]
+    return np.ascontiguousarray(vec, np.float32), uw
+
+
+def hc_quantize_selectors(codebook, bits):
+    """Codebook vectors back to codes: texel t -> (uint)(v * 2^bits) at bits t * bits
+    (crn_dxt_hc.cpp:1455-1463, 1664-1673)."""
+    q = (np.asarray(codebook, np.float32) * np.float32(1 << bits)).astype(np.uint64)
+    out = np.zeros(q.shape[0], np.uint64)
+    for t in range(16):
+        out |= q[:, t] << np.uint64(bits * t)
+    return out.astype(np.uint32) if bits == 2 else out
+
+
+def hc_color_selector_codebook(block_selectors, codebook_size, blocks, endpoint_index, cluster_colors,
+                               perceptual=True):
+    """create_color_selector_codebook (crn_dxt_hc.cpp:1379-1504): merge, tree
+    VQ, quantize, assign, refine.  Returns (selectors (k,) uint32,
+    selector_index (n,), used (k,))."""
+    vec, w = hc_merge_selectors(block_selectors, 2)
+    cb, _ = tree_codebook(vec, w, codebook_size)
+    sel = hc_quantize_selectors(cb, 2)
+    best, _, det, used = hc_color_selectors(blocks, endpoint_index, cluster_colors, sel, False, perceptual)
+    return hc_selector_refine(det, 2), best, used
+
+
+def hc_alpha_selector_codebook(block_selectors, codebook_size, block_vals, endpoint_index, alpha_values,
+                               refined_values=None):
+    """create_alpha_selector_codebook (crn_dxt_hc.cpp:1588-1720) for one or
+    two channels: block_selectors (c, n) words, block_vals (c, n, 16),
+    endpoint_index (c, n).  Returns (selectors (k,) uint64, selector_index
+    (c, n), used (k,))."""
+    bs = np.asarray(block_selectors, np.uint64).reshape(-1)
+    vec, w = hc_merge_selectors(bs, 3)
+    cb, _ = tree_codebook(vec, w, codebook_size)
+    sel = hc_quantize_selectors(cb, 3)
+    bv = np.asarray(block_vals, np.uint8)
+    ei = np.asarray(endpoint_index, np.uint32)
+    nc = bv.shape[0]
+    det = np.zeros((len(sel), 128), np.uint32)
+    used = np.zeros(len(sel), np.uint8)
+    idx = np.zeros(ei.shape, np.uint32)
+    for c in range(nc):
+        b, _, d, u = hc_alpha_selectors(bv[c], ei[c], alpha_values, sel, refined_values)
+        idx[c] = b
+        det += d
+        used |= u
+    return hc_selector_refine(det, 3), idx, used
+
+
 def hc_color_endpoints(cluster_px, cluster_ofs, block_cluster, blocks, block_weights, block_encodings,
                        perceptual=True):
     """dxt_hc colour endpoint codebook (crn_dxt_hc.cpp:663-754) with the

@@ -95,3 +95,44 @@ def alpha_brute_force(vals, ei, pal, sel, scatter=None):
     det = np.zeros((sel.shape[0], 128), np.int64)
     np.add.at(det, best, es.reshape(n, 128))
     return best, err[np.arange(n), best], det
+
+
+def color_codebook_case(n_blocks, n_clusters, seed, optimal=True):
+    """Blocks, cluster colours, per-block cluster and per-block selector words
+    (selector, texel 0 in the top bits) << 32 | weight; with `optimal` each
+    block's selectors are its texels' nearest colours (perceptual)."""
+    blocks, ei, cols, _ = make_case(n_blocks, n_clusters, 1, seed)
+    rng = np.random.default_rng(seed + 1)
+    px = blocks[:, :, :3].astype(np.int64)
+    pal = cols[ei][:, :, :3].astype(np.int64)                       # n,4,3
+    e2 = (((px[:, :, None, :] - pal[:, None, :, :]) ** 2) * np.array([8, 25, 1])).sum(-1)   # n,16,4
+    codes = e2.argmin(2) if optimal else rng.integers(0, 4, (n_blocks, 16))
+    sel = np.zeros(n_blocks, np.uint64)
+    for t in range(16):
+        sel |= codes[:, t].astype(np.uint64) << np.uint64(2 * (15 - t))
+    w = rng.integers(1, 3000, n_blocks).astype(np.uint64)
+    return blocks, ei, cols, (sel << np.uint64(32)) | w, e2
+
+
+def alpha_codebook_case(n_blocks, n_clusters, seed, channels=1, optimal=True):
+    """Per channel: block values, cluster index and selector words (selector,
+    texel 0 on top) << 16 | weight; clusters' 8-value palettes and refined
+    palettes; with `optimal` each block's selectors are its values' nearest
+    palette entries."""
+    rng = np.random.default_rng(seed)
+    vals, eis, words, e3s = [], [], [], []
+    _, _, pal, pal2, _ = make_alpha_case(8, n_clusters, 1, seed)
+    for c in range(channels):
+        v, ei, _, _, _ = make_alpha_case(n_blocks, n_clusters, 1, seed + 10 * c + 1)
+        ei = rng.integers(0, n_clusters, n_blocks).astype(np.uint32)
+        e3 = (v[:, :, None].astype(np.int64) - pal[ei][:, None, :].astype(np.int64)) ** 2
+        codes = e3.argmin(2) if optimal else rng.integers(0, 8, (n_blocks, 16))
+        sel = np.zeros(n_blocks, np.uint64)
+        for t in range(16):
+            sel |= codes[:, t].astype(np.uint64) << np.uint64(3 * (15 - t))
+        w = rng.integers(1, 60000, n_blocks).astype(np.uint64)
+        vals.append(v)
+        eis.append(ei)
+        words.append((sel << np.uint64(16)) | w)
+        e3s.append(e3)
+    return np.stack(vals), np.stack(eis), pal, pal2, np.stack(words), np.stack(e3s)
