@@ -119,6 +119,8 @@ SIGNATURES = {
     "crn2_hc_alpha_selector_codebook_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                                          C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "crn2_hc_endpoint_clusters_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                   C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.c_void_p]),
     "crn2_hc_color_endpoints_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                                                  C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -768,6 +768,7 @@ extern "C" int crn2_tree_codebook_device(const float* vectors, const uint32_t* w
     if (!vectors || !weights || !n || !max_splits || !codebook || !codebook_size) return CRN2_EINVAL;
     hipStream_t st = static_cast<hipStream_t>(stream);
     switch (dim) {
+    case 2: return tree_codebook<2>(vectors, weights, n, max_splits, codebook, codebook_size, vector_leaf, st);
     case 6: return tree_codebook<6>(vectors, weights, n, max_splits, codebook, codebook_size, vector_leaf, st);
     case 16: return tree_codebook<16>(vectors, weights, n, max_splits, codebook, codebook_size, vector_leaf, st);
     default: return CRN2_EUNSUPPORTED;

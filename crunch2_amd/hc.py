@@ -170,3 +170,19 @@ def alpha_selector_codebook(block_selectors, codebook_size, block_vals, endpoint
                                                        C.byref(m), _ptr(idx), _ptr(used), C.c_void_p(stream)),
           "crn2_hc_alpha_selector_codebook_device")
     return sel[:m.value], idx, used[:m.value]
+
+
+def endpoint_clusters(vectors, weights, codebook_size):
+    """dxt_hc endpoint clustering (crn_dxt_hc.cpp:888-972 colour, dim 6;
+    1160-1239 alpha, dim 2).  vectors: float32 CUDA tensor [n, dim]; weights
+    int32 [n].  Returns (cluster int32 [n], codebook float32 [k, dim])."""
+    n, dim = vectors.shape
+    dev = vectors.device
+    cl = torch.empty(n, dtype=torch.int32, device=dev)
+    cb = torch.empty((codebook_size, dim), dtype=torch.float32, device=dev)
+    k = C.c_uint32(0)
+    stream = torch.cuda.current_stream().cuda_stream
+    check(lib().crn2_hc_endpoint_clusters_device(_ptr(vectors.contiguous()), _ptr(weights.contiguous()), n, dim,
+                                                 codebook_size, _ptr(cl), _ptr(cb), C.byref(k), C.c_void_p(stream)),
+          "crn2_hc_endpoint_clusters_device")
+    return cl, cb[:k.value]

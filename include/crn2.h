@@ -244,7 +244,8 @@ int crn2_hc_selector_refine_device(const uint32_t* details, uint32_t n_selectors
 /* tree_clusterizer<vecNF>::generate_codebook (crnlib/crn_tree_clusterizer.h:
  * 89-171, split_node :265-570) as dxt_hc calls it: top-down splits of the
  * highest-variance node, max_splits - 1 of them at most, on the device.
- * vectors: n x dim floats (dim 6: endpoint vectors, 16: selector vectors);
+ * vectors: n x dim floats (dim 2: alpha endpoints, 6: colour endpoints, 16:
+ * selectors);
  * weights: n.  Writes the codebook (up to max_splits x dim floats, leaves in
  * node order), its size (*codebook_size, host memory) and, optionally, the
  * codebook index of each vector (vector_leaf, n).  Float sums run in a fixed
@@ -256,6 +257,21 @@ int crn2_hc_selector_refine_device(const uint32_t* details, uint32_t n_selectors
 int crn2_tree_codebook_device(const float* vectors, const uint32_t* weights, uint32_t n, uint32_t dim,
                               uint32_t max_splits, float* codebook, uint32_t* codebook_size, uint32_t* vector_leaf,
                               void* stream);
+
+/* ----------------------------------------- dxt_hc endpoint clustering */
+/* dxt_hc::determine_color_endpoints (crnlib/crn_dxt_hc.cpp:888-972; dim 6,
+ * the tiles' colour endpoint vectors with weights (uint)(texels * level
+ * weight)) and determine_alpha_endpoints (:1160-1239; dim 2, every alpha
+ * channel's tile endpoints, weights = texels) up to the cluster of every tile,
+ * on the device: the (vector, weight) pairs sorted and merged, the tree
+ * clusterizer's codebook of at most codebook_size entries, and every input
+ * vector's nearest entry (lowest index on ties).  Writes cluster (n),
+ * codebook (optional, codebook_size x dim capacity) and *n_clusters (host
+ * memory).  Vectors hold no NaN or negative zero.  Returns when done;
+ * allocates stream-ordered scratch (not for graph capture). */
+int crn2_hc_endpoint_clusters_device(const float* vectors, const uint32_t* weights, uint32_t n, uint32_t dim,
+                                     uint32_t codebook_size, uint32_t* cluster, float* codebook,
+                                     uint32_t* n_clusters, void* stream);
 
 /* ------------------------------------------- dxt_hc selector codebooks */
 /* dxt_hc::create_color_selector_codebook (crnlib/crn_dxt_hc.cpp:1379-1504) on

@@ -366,6 +366,45 @@ def hc_alpha_selector_codebook(block_selectors, codebook_size, block_vals, endpo
     return hc_selector_refine(det, 3), idx, used
 
 
+def hc_merge_vectors(vectors, weights):
+    """The merge of determine_color_endpoints / determine_alpha_endpoints
+    (crn_dxt_hc.cpp:896-946, 1164-1227): (vector, weight) pairs sorted with
+    vectors in lexicographic order, equal vectors merged with weights summed
+    and held at 2^32 - 1.  Returns (distinct vectors, weights)."""
+    v = np.asarray(vectors, np.float32)
+    w = np.asarray(weights, np.int64)
+    order = np.lexsort([v[:, c] for c in range(v.shape[1] - 1, -1, -1)])
+    vs, ws = v[order], w[order]
+    head = np.r_[True, (vs[1:] != vs[:-1]).any(1)]
+    starts = np.flatnonzero(head)
+    return np.ascontiguousarray(vs[starts]), np.minimum(np.add.reduceat(ws, starts), 2**32 - 1).astype(np.uint32)
+
+
+def hc_nearest(vectors, codebook):
+    """Nearest codebook entry of each vector by the float32 squared distance
+    summed in component order, lowest index on ties
+    (crn_dxt_hc.cpp:836-886, 1130-1158)."""
+    v = np.asarray(vectors, np.float32)
+    cb = np.asarray(codebook, np.float32)
+    out = np.zeros(len(v), np.uint32)
+    for s0 in range(0, len(v), 2048):
+        x = v[s0:s0 + 2048]
+        d = np.zeros((len(x), len(cb)), np.float32)
+        for c in range(v.shape[1]):
+            t = cb[None, :, c] - x[:, None, c]
+            d = d + t * t
+        out[s0:s0 + 2048] = d.argmin(1)
+    return out
+
+
+def hc_endpoint_clusters(vectors, weights, codebook_size):
+    """Merge, tree VQ and nearest-entry assignment of the tiles' endpoint
+    vectors.  Returns (cluster per vector, codebook)."""
+    uv, uw = hc_merge_vectors(vectors, weights)
+    cb, _ = tree_codebook(uv, uw, codebook_size)
+    return hc_nearest(vectors, cb), cb
+
+
 def hc_color_endpoints(cluster_px, cluster_ofs, block_cluster, blocks, block_weights, block_encodings,
                        perceptual=True):
     """dxt_hc colour endpoint codebook (crn_dxt_hc.cpp:663-754) with the

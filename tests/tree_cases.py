@@ -39,3 +39,20 @@ def separated(n, dim, seed, blobs):
     v = (centres[lab] + rng.uniform(-0.02, 0.02, (n, dim))).astype(np.float32)
     w = rng.integers(1, 1000, n).astype(np.uint32)
     return v, w
+
+
+def tile_endpoints(n, dim, seed):
+    """Endpoint vectors as dxt_hc's tiles produce them: quantized colour
+    coordinates (multiples of 1/255, halved / quartered for red / blue in the
+    colour case) with many exact repeats, and tile weights."""
+    rng = np.random.default_rng(seed)
+    base = rng.integers(0, 256, (max(4, n // 6), dim))
+    q = base[rng.integers(0, len(base), n)] + rng.integers(-3, 4, (n, dim)) * (rng.random((n, 1)) < 0.5)
+    v = (np.clip(q, 0, 255) * np.float32(1.0 / 255.0)).astype(np.float32)
+    if dim == 6:
+        v[:, 0] *= np.float32(0.5)
+        v[:, 3] *= np.float32(0.5)
+        v[:, 2] *= np.float32(0.25)
+        v[:, 5] *= np.float32(0.25)
+    w = rng.integers(16, 257, n).astype(np.uint32)
+    return v, w
