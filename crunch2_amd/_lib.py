@@ -121,6 +121,9 @@ SIGNATURES = {
                                                          C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p, C.c_void_p]),
     "crn2_hc_endpoint_clusters_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                                    C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.c_void_p]),
+    "crn2_hc_tiles_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int,
+                                       C.c_uint32, C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "crn2_hc_color_endpoints_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                                                  C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

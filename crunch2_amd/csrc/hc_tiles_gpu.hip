@@ -1,0 +1,776 @@
+// hc_tiles_gpu.hip -- dxt_hc's adaptive tile choice (dxt_hc::determine_tiles_task,
+// crnlib/crn_dxt_hc.cpp:386-561) on gfx950.
+//
+// Each 2x2-block chunk of a level offers nine tiles (the four blocks, the two
+// block columns, the two halves, the whole 8x8) and eight ways of covering
+// the chunk with them.  The reference scores every tile (colour: the error of
+// dxt_fast::compress_color_block without refinement against the 4-colour
+// palette, crn_dxt_fast.cpp:25-764; alpha: the error of
+// dxt5_endpoint_optimizer at normal quality, crn_dxt5a.cpp:40-262), turns the
+// eight coverings' summed errors into PSNR minus a per-level derating and
+// keeps the best covering; each kept tile gets palettized endpoints
+// (palettize_color / palettize_alpha :318-376, with split_vectors,
+// crn_tree_clusterizer.h:573-753).
+//
+// Three kernels, all exact restatements with the reference's operation order
+// (the library is built with -ffp-contract=off):
+//   tile_error_kernel   one thread per (chunk, tile): colour and alpha errors;
+//   tile_choose_kernel  one thread per chunk: PSNR, derating, best covering,
+//                       block encodings, tile indices, kept tiles' sources;
+//   tile_vec_kernel     one thread per kept tile: palettized endpoints.
+// The chunk order and the tile slots (tile_offset walks 4 slots per chunk,
+// backwards on every other row pair of a face) are laid out on the host.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "crn2_internal.h"
+
+namespace {
+
+__constant__ uint8_t c_om5[256][2], c_om6[256][2];
+__constant__ float c_u2f[256];
+
+constexpr uint32_t kOffsets[9] = {0, 16, 32, 48, 0, 32, 64, 96, 64};
+__constant__ uint8_t c_tiles[8][4] = {{8}, {6, 7}, {4, 5}, {6, 1, 3}, {7, 0, 2}, {4, 2, 3}, {5, 0, 1}, {0, 2, 1, 3}};
+__constant__ uint8_t c_tile_map[8][2][2] = {{{0, 0}, {0, 0}}, {{0, 0}, {1, 1}}, {{0, 1}, {0, 1}}, {{0, 0}, {1, 2}},
+                                            {{1, 2}, {0, 0}}, {{0, 1}, {0, 2}}, {{1, 0}, {2, 0}}, {{0, 1}, {2, 3}}};
+
+struct Chunk {
+    uint32_t b, width, level, toff;
+};
+
+struct TileArgs {
+    const uint32_t* blocks;      // RGBA8 words, 16 per block
+    const Chunk* chunks;
+    uint32_t nchunks;
+    int has_color, num_alpha, perceptual;
+    uint32_t alpha_comp[2];
+    float color_der[16][8], alpha_der[8], ratio;
+    uint32_t* terr;              // nchunks x 3 x 9
+    uint8_t* block_encodings;
+    uint32_t* tile_indices;
+    uint32_t* tile_src;          // per slot: b << 4 | t, 0xFFFFFFFF empty
+    float* tile_vec;             // per slot: 10 floats
+    uint32_t* slot_width;        // per slot: the level width of its chunk
+    uint32_t num_blocks;
+};
+
+// Texel i of tile t of the chunk whose top-left block is b (level width w).
+__device__ inline uint32_t tile_texel(const uint32_t* blocks, uint32_t b, uint32_t w, uint32_t t, uint32_t i)
+{
+    const uint32_t idx = kOffsets[t] + i;
+    if (idx < 64) return blocks[(size_t)(b + ((idx & 16) ? w : 0) + ((idx & 32) ? 1 : 0)) * 16 + (idx & 15)];
+    const uint32_t j = idx - 64;
+    return blocks[(size_t)(b + ((j & 32) ? w : 0) + ((j & 4) ? 1 : 0)) * 16 + ((j >> 1) & 12) + (j & 3)];
+}
+
+__device__ inline int ch(uint32_t p, int c) { return (int)((p >> (8 * c)) & 255u); }
+
+__device__ inline int mul_8bit(int a, int b)
+{
+    const int t = a * b + 128;
+    return (t + (t >> 8)) >> 8;
+}
+
+__device__ inline void unpack565(uint32_t v, int* c)
+{
+    const int r = (int)((v >> 11) & 31), g = (int)((v >> 5) & 63), b = (int)(v & 31);
+    c[0] = (r << 3) | (r >> 2);
+    c[1] = (g << 2) | (g >> 4);
+    c[2] = (b << 3) | (b >> 2);
+}
+
+__device__ inline uint32_t pack_color(uint32_t p)
+{
+    return (uint32_t)((mul_8bit(ch(p, 0), 31) << 11) + (mul_8bit(ch(p, 1), 63) << 5) + mul_8bit(ch(p, 2), 31));
+}
+
+__device__ inline void eval_colors(int (&col)[4][3], uint32_t c0, uint32_t c1)
+{
+    unpack565(c0, col[0]);
+    unpack565(c1, col[1]);
+    for (int k = 0; k < 3; k++) {
+        col[2][k] = (col[0][k] * 2 + col[1][k]) / 3;
+        col[3][k] = (col[1][k] * 2 + col[0][k]) / 3;
+    }
+}
+
+template <int N>
+__device__ bool match_block_colors(const uint32_t (&px)[N], int (&col)[4][3], uint8_t (&sel)[N])
+{
+    const int dr = col[0][0] - col[1][0], dg = col[0][1] - col[1][1], db = col[0][2] - col[1][2];
+    int stops[4];
+    for (int i = 0; i < 4; i++) stops[i] = col[i][0] * dr + col[i][1] * dg + col[i][2] * db;
+    const int c0p = (stops[1] + stops[3]) >> 1, half = (stops[3] + stops[2]) >> 1, c3p = (stops[2] + stops[0]) >> 1;
+    bool status = false;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int dot = ch(px[i], 0) * dr + ch(px[i], 1) * dg + ch(px[i], 2) * db;
+        uint8_t s;
+        if (dot < half) s = dot < c0p ? 1 : 3;
+        else s = dot < c3p ? 2 : 0;
+        sel[i] = s;
+        if (s != sel[0]) status = true;
+    }
+    return status;
+}
+
+template <int N>
+__device__ bool optimize_block_colors(const uint32_t (&px)[N], uint32_t& max16, uint32_t& min16, uint32_t (&ave)[3])
+{
+    int mn[3], mx[3];
+    for (int c = 0; c < 3; c++) {
+        int mu = ch(px[0], c), a = mu, b = mu;
+#pragma unroll
+        for (int i = 1; i < N; i++) {
+            const int v = ch(px[i], c);
+            mu += v;
+            a = a < v ? a : v;
+            b = b > v ? b : v;
+        }
+        ave[c] = (uint32_t)((mu + N / 2) / N);
+        mn[c] = a;
+        mx[c] = b;
+    }
+    if (mn[0] == mx[0] && mn[1] == mx[1] && mn[2] == mx[2]) return false;
+    double cov[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const double r = ch(px[i], 0) - (int)ave[0];
+        const double g = ch(px[i], 1) - (int)ave[1];
+        const double b = ch(px[i], 2) - (int)ave[2];
+        cov[0] += r * r;
+        cov[1] += r * g;
+        cov[2] += r * b;
+        cov[3] += g * g;
+        cov[4] += g * b;
+        cov[5] += b * b;
+    }
+    double covf[6];
+    for (int i = 0; i < 6; i++) covf[i] = cov[i] * (double)(1.0f / 255.0f);
+    double vfr = mx[0] - mn[0], vfg = mx[1] - mn[1], vfb = mx[2] - mn[2];
+    for (int it = 0; it < 4; it++) {
+        const double r = vfr * covf[0] + vfg * covf[1] + vfb * covf[2];
+        const double g = vfr * covf[1] + vfg * covf[3] + vfb * covf[4];
+        const double b = vfr * covf[2] + vfg * covf[4] + vfb * covf[5];
+        vfr = r;
+        vfg = g;
+        vfb = b;
+    }
+    double magn = fabs(vfr) > fabs(vfg) ? fabs(vfr) : fabs(vfg);
+    magn = magn > fabs(vfb) ? magn : fabs(vfb);
+    int vr, vg, vb;
+    if (magn < 4.0f) {
+        vr = 148;
+        vg = 300;
+        vb = 58;
+    } else {
+        magn = 512.0f / magn;
+        vfr *= magn;
+        vfg *= magn;
+        vfb *= magn;
+        vr = (int)vfr;
+        vg = (int)vfg;
+        vb = (int)vfb;
+    }
+    int mind = ch(px[0], 0) * vr + ch(px[0], 1) * vg + ch(px[0], 2) * vb, maxd = mind;
+    uint32_t pmin = px[0], pmax = px[0];
+#pragma unroll
+    for (int i = 1; i < N; i++) {
+        const int dot = ch(px[i], 0) * vr + ch(px[i], 1) * vg + ch(px[i], 2) * vb;
+        if (dot < mind) {
+            mind = dot;
+            pmin = px[i];
+        }
+        if (dot > maxd) {
+            maxd = dot;
+            pmax = px[i];
+        }
+    }
+    max16 = pack_color(pmax);
+    min16 = pack_color(pmin);
+    return true;
+}
+
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+template <int N>
+__device__ bool refine_block(const uint32_t (&px)[N], uint32_t& max16, uint32_t& min16, const uint8_t (&sel)[N])
+{
+    const int w1t[4] = {3, 0, 2, 1}, p0[4] = {0, 0, 2, 2}, p1[4] = {0, 9, 1, 4}, p2[4] = {9, 0, 4, 1};
+    double a0 = 0, a1 = 0, a2 = 0, t1r = 0, t1g = 0, t1b = 0, t2r = 0, t2g = 0, t2b = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const double r = ch(px[i], 0), g = ch(px[i], 1), b = ch(px[i], 2);
+        const int st = sel[i], w1 = w1t[st];
+        a0 += p0[st];
+        a1 += p1[st];
+        a2 += p2[st];
+        t1r += w1 * r;
+        t1g += w1 * g;
+        t1b += w1 * b;
+        t2r += r;
+        t2g += g;
+        t2b += b;
+    }
+    t2r = 3 * t2r - t1r;
+    t2g = 3 * t2g - t1g;
+    t2b = 3 * t2b - t1b;
+    const double xx = a2, yy = a1, xy = a0;
+    const double t = xx * yy - xy * xy;
+    if (!yy || !xx || fabs(t) < .0000125f) return false;
+    const double frb = (double)(3.0f * 31.0f / 255.0f) / t;
+    const double fg = frb * (double)(63.0f / 31.0f);
+    const uint32_t om = min16, ox = max16;
+    uint32_t x = (uint32_t)clampi((int)((t1r * yy - t2r * xy) * frb + 0.5f), 0, 31) << 11;
+    x |= (uint32_t)clampi((int)((t1g * yy - t2g * xy) * fg + 0.5f), 0, 63) << 5;
+    x |= (uint32_t)clampi((int)((t1b * yy - t2b * xy) * frb + 0.5f), 0, 31);
+    uint32_t m = (uint32_t)clampi((int)((t2r * xx - t1r * xy) * frb + 0.5f), 0, 31) << 11;
+    m |= (uint32_t)clampi((int)((t2g * xx - t1g * xy) * fg + 0.5f), 0, 63) << 5;
+    m |= (uint32_t)clampi((int)((t2b * xx - t1b * xy) * frb + 0.5f), 0, 31);
+    max16 = x;
+    min16 = m;
+    return om != m || ox != x;
+}
+
+template <int N>
+__device__ bool determine_selectors(const uint32_t (&px)[N], uint32_t min16, uint32_t max16, uint8_t (&sel)[N])
+{
+    if (max16 != min16) {
+        int col[4][3];
+        eval_colors(col, min16, max16);
+        return match_block_colors<N>(px, col, sel);
+    }
+    for (int i = 0; i < N; i++) sel[i] = 0;
+    return false;
+}
+
+// dxt_fast::compress_color_block (refine = false), then the error against the
+// 4-colour palette of the result (crn_dxt_hc.cpp:457-470).
+template <int N>
+__device__ uint32_t tile_color_error(const uint32_t (&px)[N])
+{
+    uint8_t sel[N];
+    uint32_t lo = 0, hi = 0, ave[3];
+    bool solid = false;
+    if (!optimize_block_colors<N>(px, lo, hi, ave)) solid = true;
+    else if (!determine_selectors<N>(px, lo, hi, sel)) solid = true;
+    else if (refine_block<N>(px, lo, hi, sel)) determine_selectors<N>(px, lo, hi, sel);
+    if (solid) {
+        for (int i = 0; i < N; i++) sel[i] = 2;
+        lo = (uint32_t)c_om5[ave[0]][0] << 11 | (uint32_t)c_om6[ave[1]][0] << 5 | c_om5[ave[2]][0];
+        hi = (uint32_t)c_om5[ave[0]][1] << 11 | (uint32_t)c_om6[ave[1]][1] << 5 | c_om5[ave[2]][1];
+    }
+    if (lo < hi) {
+        const uint32_t t = lo;
+        lo = hi;
+        hi = t;
+        for (int i = 0; i < N; i++) sel[i] ^= 1;
+    }
+    int col[4][3];
+    eval_colors(col, lo, hi);
+    uint32_t err = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        for (int c = 0; c < 3; c++) {
+            const uint32_t d = (uint32_t)ch(px[i], c) - (uint32_t)col[sel[i]][c];
+            err += d * d;
+        }
+    return err;
+}
+
+// dxt5_endpoint_optimizer at normal quality, 8-value blocks only: the best
+// error over every pair of distinct values (first-appearance order), per
+// value the nearest palette entry (first of the smallest, stop at zero), with
+// the reference's early exits (crn_dxt5a.cpp:53-103, 198-262).
+template <int N>
+__device__ uint32_t tile_alpha_error(const uint32_t (&px)[N], uint32_t comp)
+{
+    uint8_t uv[N];
+    uint8_t uw[N];
+    uint32_t nu = 0;
+    for (int i = 0; i < N; i++) {
+        const uint32_t v = (px[i] >> (8 * comp)) & 255u;
+        uint32_t k = 0;
+        while (k < nu && uv[k] != v) k++;
+        if (k == nu) {
+            uv[nu] = (uint8_t)v;
+            uw[nu] = 0;
+            nu++;
+        }
+        uw[k]++;
+    }
+    if (nu == 1) return 0;
+    uint64_t best = ~0ull;
+    for (uint32_t i = 0; i + 1 < nu; i++)
+        for (uint32_t j = i + 1; j < nu; j++) {
+            const uint32_t l = uv[i], h = uv[j];
+            uint32_t pal[8];
+            pal[0] = l;
+            pal[1] = h;
+            pal[2] = (l * 6 + h) / 7;
+            pal[3] = (l * 5 + h * 2) / 7;
+            pal[4] = (l * 4 + h * 3) / 7;
+            pal[5] = (l * 3 + h * 4) / 7;
+            pal[6] = (l * 2 + h * 5) / 7;
+            pal[7] = (l + h * 6) / 7;
+            uint64_t err = 0;
+            for (uint32_t u = 0; u < nu; u++) {
+                const int val = uv[u], w = uw[u];
+                uint32_t b = 0xFFFFFFFFu;
+                for (int s = 0; s < 8; s++) {
+                    int e = val - (int)pal[s];
+                    e = e * e * w;
+                    if ((uint32_t)e < b) {
+                        b = (uint32_t)e;
+                        if (!b) break;
+                    }
+                }
+                err += b;
+                if (err > best) break;
+            }
+            if (err < best) {
+                best = err;
+                if (!err) return 0;
+            }
+        }
+    return (uint32_t)best;
+}
+
+template <int N>
+__device__ void tile_errors(const TileArgs& a, const Chunk& c, uint32_t t, uint32_t* out)
+{
+    uint32_t px[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) px[i] = tile_texel(a.blocks, c.b, c.width, t, (uint32_t)i);
+    if (a.has_color) out[t] = tile_color_error<N>(px);
+    for (int k = 0; k < a.num_alpha; k++) out[9 * (1 + k) + t] = tile_alpha_error<N>(px, a.alpha_comp[k]);
+}
+
+__global__ void __launch_bounds__(256) tile_error_kernel(TileArgs a)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nchunks * 9) return;
+    const uint32_t ci = i / 9, t = i % 9;
+    const Chunk c = a.chunks[ci];
+    uint32_t* out = a.terr + (size_t)ci * 27;
+    if (t < 4) tile_errors<16>(a, c, t, out);
+    else if (t < 8) tile_errors<32>(a, c, t, out);
+    else tile_errors<64>(a, c, t, out);
+}
+
+__global__ void __launch_bounds__(256) tile_choose_kernel(TileArgs a)
+{
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= a.nchunks) return;
+    const Chunk c = a.chunks[ci];
+    const uint32_t* te = a.terr + (size_t)ci * 27;
+    float best_q = 0.0f;
+    uint32_t best_e = 0;
+    for (uint32_t e = 0; e < 8; e++) {
+        uint32_t tot[3] = {0, 0, 0};
+        for (uint32_t k = a.has_color ? 0 : 1; k < 1u + (uint32_t)a.num_alpha; k++)
+            for (uint32_t t = 0, s = e + 1; s; s >>= 1, t++) tot[k] += te[9 * k + c_tiles[e][t]];
+        float q = 0;
+        if (a.has_color) {
+            const double psnr = tot[0] ? log10(255.0f / sqrt(tot[0] / 192.0)) * 20.0f : 999999.0f;
+            const double x = psnr - a.color_der[c.level][e];
+            q = (float)(x > 0.0f ? x : 0.0f);
+            if (a.num_alpha) q *= a.ratio;
+        }
+        for (int k = 0; k < a.num_alpha; k++) {
+            const double psnr = tot[1 + k] ? log10(255.0f / sqrt(tot[1 + k] / 64.0)) * 20.0f : 999999.0f;
+            const double x = psnr - a.alpha_der[e];
+            q += (float)(x > 0.0f ? x : 0.0f);
+        }
+        if (q > best_q) {
+            best_q = q;
+            best_e = e;
+        }
+    }
+    for (uint32_t ti = 0, s = best_e + 1; s; s >>= 1, ti++) {
+        a.tile_src[c.toff | ti] = c.b << 4 | c_tiles[best_e][ti];
+        a.slot_width[c.toff | ti] = c.width;
+    }
+    for (uint32_t by = 0; by < 2; by++)
+        for (uint32_t bx = 0; bx < 2; bx++) {
+            const uint32_t blk = c.b + (by ? c.width : 0) + bx;
+            a.block_encodings[blk] = (uint8_t)best_e;
+            a.tile_indices[blk] = c.toff | c_tile_map[best_e][by][bx];
+        }
+}
+
+// ---------------------------------------------------------- palettizing
+
+__device__ inline float sqd(const float* x, const float* y, int d)
+{
+    float s = 0.0f;
+    for (int i = 0; i < d; i++) {
+        const float t = x[i] - y[i];
+        s += t * t;
+    }
+    return s;
+}
+
+__device__ inline float dotv(const float* x, const float* y, int d)
+{
+    float s = x[0] * y[0];
+    for (int i = 1; i < d; i++) s += x[i] * y[i];
+    return s;
+}
+
+// split_vectors (crn_tree_clusterizer.h:573-753), D = 1 or 3, sums in vector order.
+template <int D>
+__device__ void split_vectors(const float* v, const uint32_t* w, uint32_t size, float* result)
+{
+    float wv[64 * D];
+    double wd[64];
+    float c[D];
+    for (int k = 0; k < D; k++) c[k] = 0.0f;
+    double tt = 0.0;
+    uint64_t tw = 0;
+    for (uint32_t i = 0; i < size; i++) {
+        const float fw = (float)w[i];
+        for (int k = 0; k < D; k++) {
+            wv[i * D + k] = v[i * D + k] * fw;
+            c[k] += wv[i * D + k];
+        }
+        tw += w[i];
+        wd[i] = dotv(v + i * D, v + i * D, D) * fw;
+        tt += wd[i];
+    }
+    const float var = (float)(tt - (dotv(c, c, D) / (float)tw));
+    const float inv = 1.0f / (float)tw;
+    for (int k = 0; k < D; k++) c[k] *= inv;
+    for (int k = 0; k < D; k++) result[k] = result[D + k] = c[k];
+    if (var <= 0.0f || size == 1) return;
+    const float *fur = v, *opp = v;
+    double fd = -1.0f, od = -1.0f;
+    for (uint32_t i = 0; i < size; i++) {
+        const double dd = sqd(v + i * D, c, D);
+        if (dd > fd) {
+            fd = dd;
+            fur = v + i * D;
+        }
+    }
+    for (uint32_t i = 0; i < size; i++) {
+        const double dd = sqd(v + i * D, fur, D);
+        if (dd > od) {
+            od = dd;
+            opp = v + i * D;
+        }
+    }
+    float lc[D], rc[D];
+    for (int k = 0; k < D; k++) {
+        lc[k] = (fur[k] + c[k]) * .5f;
+        rc[k] = (opp[k] + c[k]) * .5f;
+    }
+    if (size > 2) {
+        float cov[D][D];
+        for (int x = 0; x < D; x++)
+            for (int y = 0; y < D; y++) cov[x][y] = 0.0f;
+        for (uint32_t i = 0; i < size; i++) {
+            float x[D], y[D];
+            for (int k = 0; k < D; k++) {
+                x[k] = v[i * D + k] - c[k];
+                y[k] = x[k] * (float)w[i];
+            }
+            for (int p = 0; p < D; p++)
+                for (int q = p; q < D; q++) cov[p][q] = cov[p][q] + x[p] * y[q];
+        }
+        const float div = (float)tw;
+        for (int p = 0; p < D; p++)
+            for (int q = p; q < D; q++) {
+                cov[p][q] /= div;
+                cov[q][p] = cov[p][q];
+            }
+        float axis[D];
+        for (int k = 0; k < D; k++) axis[k] = 1.0f;
+        for (int it = 0; it < 10; it++) {
+            float x[D];
+            double ms = 0;
+            for (int i = 0; i < D; i++) {
+                double sum = 0;
+                for (int j = 0; j < D; j++) sum += axis[j] * cov[i][j];
+                x[i] = (float)sum;
+                ms = i ? (ms > sum ? ms : sum) : sum;
+            }
+            if (ms != 0.0f) {
+                const float sc = (float)(1.0f / ms);
+                for (int i = 0; i < D; i++) x[i] *= sc;
+            }
+            for (int i = 0; i < D; i++) axis[i] = x[i];
+        }
+        double n2 = axis[0] * axis[0];
+        for (int i = 1; i < D; i++) n2 += axis[i] * axis[i];
+        if (n2 != 0) {
+            const float sc = (float)(1.0f / sqrt(n2));
+            for (int i = 0; i < D; i++) axis[i] *= sc;
+        }
+        float nl[D], nr[D];
+        for (int k = 0; k < D; k++) nl[k] = nr[k] = 0.0f;
+        double lw = 0.0f, rw = 0.0f;
+        for (uint32_t i = 0; i < size; i++) {
+            float x[D];
+            for (int k = 0; k < D; k++) x[k] = v[i * D + k] - c[k];
+            const float fw = (float)w[i];
+            if (dotv(x, axis, D) < 0.0f) {
+                for (int k = 0; k < D; k++) nl[k] += wv[i * D + k];
+                lw += fw;
+            } else {
+                for (int k = 0; k < D; k++) nr[k] += wv[i * D + k];
+                rw += fw;
+            }
+        }
+        if (lw > 0.0f && rw > 0.0f) {
+            const float sl = (float)(1.0f / lw), sr = (float)(1.0f / rw);
+            for (int k = 0; k < D; k++) {
+                lc[k] = nl[k] * sl;
+                rc[k] = nr[k] * sr;
+            }
+        }
+    }
+    float prev = 1e+10f;
+    for (int loop = 0; loop < 1024; loop++) {
+        float nl[D], nr[D];
+        for (int k = 0; k < D; k++) nl[k] = nr[k] = 0.0f;
+        double lt = 0.0f, rt = 0.0f;
+        uint64_t lw = 0, rw = 0;
+        for (uint32_t i = 0; i < size; i++) {
+            const float* x = v + i * D;
+            if (sqd(lc, x, D) < sqd(rc, x, D)) {
+                for (int k = 0; k < D; k++) nl[k] += wv[i * D + k];
+                lt += wd[i];
+                lw += w[i];
+            } else {
+                for (int k = 0; k < D; k++) nr[k] += wv[i * D + k];
+                rt += wd[i];
+                rw += w[i];
+            }
+        }
+        if (!lw || !rw) return;
+        const float lv = (float)(lt - (dotv(nl, nl, D) / (float)lw));
+        const float rv = (float)(rt - (dotv(nr, nr, D) / (float)rw));
+        const float il = 1.0f / (float)lw, ir = 1.0f / (float)rw;
+        for (int k = 0; k < D; k++) {
+            lc[k] = nl[k] * il;
+            rc[k] = nr[k] * ir;
+        }
+        const float total = lv + rv;
+        if (total < .00001f) break;
+        if (((prev - total) / total) < .00001f) break;
+        prev = total;
+    }
+    for (int k = 0; k < D; k++) {
+        result[k] = lc[k];
+        result[D + k] = rc[k];
+    }
+}
+
+__device__ void sort_u32(uint32_t* x, uint32_t n)
+{
+    for (uint32_t i = 1; i < n; i++) {
+        const uint32_t v = x[i];
+        uint32_t j = i;
+        while (j && x[j - 1] > v) {
+            x[j] = x[j - 1];
+            j--;
+        }
+        x[j] = v;
+    }
+}
+
+__global__ void __launch_bounds__(64) tile_vec_kernel(TileArgs a)
+{
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= a.num_blocks) return;
+    const uint32_t src = a.tile_src[slot];
+    if (src == 0xFFFFFFFFu) return;
+    const uint32_t b = src >> 4, t = src & 15u;
+    const uint32_t width = a.slot_width[slot];
+    const uint32_t n = 16u << (t >> 2);
+    float* out = a.tile_vec + (size_t)slot * 10;
+    uint32_t col[64], wts[64], size;
+    if (a.has_color) {
+        for (uint32_t i = 0; i < n; i++) col[i] = tile_texel(a.blocks, b, width, t, i) & 0xFFFFFFu;
+        // crnlib orders colours as r << 16 | g << 8 | b
+        for (uint32_t i = 0; i < n; i++) col[i] = (col[i] & 255u) << 16 | (col[i] & 0xFF00u) | (col[i] >> 16);
+        sort_u32(col, n);
+        float v[64 * 3];
+        size = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            if (!i || col[i] != col[i - 1]) {
+                v[size * 3] = a.perceptual ? c_u2f[col[i] >> 16] * 0.5f : c_u2f[col[i] >> 16];
+                v[size * 3 + 1] = c_u2f[col[i] >> 8 & 0xFF];
+                v[size * 3 + 2] = a.perceptual ? c_u2f[col[i] & 0xFF] * 0.25f : c_u2f[col[i] & 0xFF];
+                wts[size++] = 1;
+            } else {
+                wts[size - 1]++;
+            }
+        }
+        float r[6];
+        split_vectors<3>(v, wts, size, r);
+        const float l0 = sqrtf(dotv(r, r, 3)), l1 = sqrtf(dotv(r + 3, r + 3, 3));
+        if (l0 > l1) {
+            for (int k = 0; k < 3; k++) {
+                out[k] = r[3 + k];
+                out[3 + k] = r[k];
+            }
+        } else {
+            for (int k = 0; k < 6; k++) out[k] = r[k];
+        }
+    }
+    for (int k = 0; k < a.num_alpha; k++) {
+        for (uint32_t i = 0; i < n; i++) col[i] = (tile_texel(a.blocks, b, width, t, i) >> (8 * a.alpha_comp[k])) & 255u;
+        sort_u32(col, n);
+        float v[64];
+        size = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            if (!i || col[i] != col[i - 1]) {
+                v[size] = c_u2f[col[i]];
+                wts[size++] = 1;
+            } else {
+                wts[size - 1]++;
+            }
+        }
+        float r[2];
+        split_vectors<1>(v, wts, size, r);
+        if (r[1] < r[0]) {
+            out[6 + 2 * k] = r[1];
+            out[7 + 2 * k] = r[0];
+        } else {
+            out[6 + 2 * k] = r[0];
+            out[7 + 2 * k] = r[1];
+        }
+    }
+}
+
+bool g_tables_ready = false;
+
+int init_tables()
+{
+    if (g_tables_ready) return CRN2_OK;
+    static uint8_t t5[256][2], t6[256][2];
+    static float u2f[256];
+    // ryg single-colour tables for 4-colour blocks (crn_ryg_dxt.cpp:58-99)
+    for (int which = 0; which < 2; which++) {
+        const int size = which ? 64 : 32;
+        uint8_t(*t)[2] = which ? t6 : t5;
+        for (int i = 0; i < 256; i++) {
+            int best = 256;
+            for (int mn = 0; mn < size; mn++)
+                for (int mx = 0; mx < size; mx++) {
+                    const int me = size == 32 ? ((mn << 3) | (mn >> 2)) : ((mn << 2) | (mn >> 4));
+                    const int xe = size == 32 ? ((mx << 3) | (mx >> 2)) : ((mx << 2) | (mx >> 4));
+                    int err = abs(((xe * 2 + me) / 3) - i);
+                    err += (abs(xe - me) * 8) >> 8;
+                    if (err < best) {
+                        t[i][0] = (uint8_t)mx;
+                        t[i][1] = (uint8_t)mn;
+                        best = err;
+                    }
+                }
+        }
+    }
+    for (int i = 0; i < 256; i++) u2f[i] = i * 1.0f / 255.0f;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_om5), t5, 512) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(c_om6), t6, 512) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(c_u2f), u2f, sizeof(u2f)) != hipSuccess)
+        return CRN2_EHIP;
+    g_tables_ready = true;
+    return CRN2_OK;
+}
+
+}  // namespace
+
+extern "C" int crn2_hc_tiles_device(const void* blocks, uint32_t num_blocks, uint32_t nlevels,
+                                    const uint32_t* level_desc, uint32_t num_faces,
+                                    int has_color, uint32_t num_alpha, const uint32_t* alpha_comp, int perceptual,
+                                    float color_derating, float alpha_derating, float ratio,
+                                    uint8_t* block_encodings, uint32_t* tile_indices, uint32_t* tile_src,
+                                    float* tile_vec, void* stream)
+{
+    if (!blocks || !num_blocks || !nlevels || nlevels > 16 || !level_desc || !num_faces || num_alpha > 2 ||
+        (!has_color && !num_alpha) || (num_alpha && !alpha_comp) || !block_encodings || !tile_indices ||
+        !tile_src || !tile_vec)
+        return CRN2_EINVAL;
+    const int rc0 = init_tables();
+    if (rc0 != CRN2_OK) return rc0;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    TileArgs a;
+    a.blocks = static_cast<const uint32_t*>(blocks);
+    a.has_color = has_color ? 1 : 0;
+    a.num_alpha = (int)num_alpha;
+    a.perceptual = perceptual ? 1 : 0;
+    a.alpha_comp[0] = num_alpha > 0 ? alpha_comp[0] : 0;
+    a.alpha_comp[1] = num_alpha > 1 ? alpha_comp[1] : 0;
+    if (a.alpha_comp[0] > 3 || a.alpha_comp[1] > 3) return CRN2_EINVAL;
+    a.ratio = ratio;
+    // derating tables (crn_dxt_hc.cpp:122-142)
+    static const uint32_t kTileDerating[8] = {0, 1, 1, 2, 2, 2, 2, 3};
+    for (uint32_t l = 0; l < nlevels; l++) {
+        float d = color_derating;
+        if (l && d > .25f) {
+            const float q = d / powf(3.0f, (float)l);
+            d = .25f > q ? .25f : q;
+        }
+        for (int e = 0; e < 8; e++) a.color_der[l][e] = 0.0f + (d - 0.0f) * (kTileDerating[e] / 3.0f);
+    }
+    for (int e = 0; e < 8; e++) a.alpha_der[e] = 0.0f + (alpha_derating - 0.0f) * (kTileDerating[e] / 3.0f);
+    // chunks in the reference's order with their tile slots (crn_dxt_hc.cpp:404-423)
+    std::vector<Chunk> chunks;
+    chunks.reserve(num_blocks / 4);
+    for (uint32_t l = 0; l < nlevels; l++) {
+        const uint32_t first = level_desc[3 * l], nb = level_desc[3 * l + 1], width = level_desc[3 * l + 2];
+        if (!width || nb % width) return CRN2_EINVAL;
+        const uint32_t height = nb / width, face_h = height / num_faces;
+        if ((width & 1) || (face_h & 1) || face_h * num_faces != height) return CRN2_EUNSUPPORTED;
+        if ((uint64_t)first + nb > num_blocks) return CRN2_EINVAL;
+        uint32_t hface = 0, b = first;
+        for (uint32_t h = 0; h < height; h += 2, hface += 2, b += width) {
+            uint32_t toff = b, tdelta = 4;
+            if (hface == face_h) hface = 0;
+            else if (hface & 2) {
+                tdelta = (uint32_t)-4;
+                toff += (width << 1) + tdelta;
+            }
+            for (uint32_t bn = b + width; b < bn; b += 2, toff += tdelta) chunks.push_back(Chunk{b, width, l, toff});
+        }
+    }
+    if (chunks.empty()) return CRN2_OK;
+    Chunk* dch = nullptr;
+    uint32_t* terr = nullptr;
+    uint32_t* swidth = nullptr;
+    const uint32_t nch = (uint32_t)chunks.size();
+    int rc = CRN2_OK;
+    if (hipMallocAsync((void**)&dch, nch * sizeof(Chunk), st) != hipSuccess ||
+        hipMallocAsync((void**)&terr, (size_t)nch * 27 * sizeof(uint32_t), st) != hipSuccess ||
+        hipMallocAsync((void**)&swidth, (size_t)num_blocks * sizeof(uint32_t), st) != hipSuccess)
+        rc = CRN2_ENOMEM;
+    a.slot_width = swidth;
+    a.chunks = dch;
+    a.nchunks = nch;
+    a.terr = terr;
+    a.block_encodings = block_encodings;
+    a.tile_indices = tile_indices;
+    a.tile_src = tile_src;
+    a.tile_vec = tile_vec;
+    a.num_blocks = num_blocks;
+    if (rc == CRN2_OK && (hipMemcpyAsync(dch, chunks.data(), nch * sizeof(Chunk), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemsetAsync(tile_src, 0xFF, (size_t)num_blocks * sizeof(uint32_t), st) != hipSuccess ||
+        hipMemsetAsync(tile_vec, 0, (size_t)num_blocks * 10 * sizeof(float), st) != hipSuccess))
+        rc = CRN2_EHIP;
+    if (rc == CRN2_OK) {
+        hipLaunchKernelGGL(tile_error_kernel, dim3((nch * 9 + 255) / 256), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(tile_choose_kernel, dim3((nch + 255) / 256), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(tile_vec_kernel, dim3((num_blocks + 63) / 64), dim3(64), 0, st, a);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = CRN2_EHIP;
+    }
+    for (void* p : {(void*)dch, (void*)terr, (void*)swidth})
+        if (p) (void)hipFreeAsync(p, st);
+    (void)hipStreamSynchronize(st);
+    return rc;
+}

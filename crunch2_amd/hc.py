@@ -186,3 +186,26 @@ def endpoint_clusters(vectors, weights, codebook_size):
                                                  codebook_size, _ptr(cl), _ptr(cb), C.byref(k), C.c_void_p(stream)),
           "crn2_hc_endpoint_clusters_device")
     return cl, cb[:k.value]
+
+
+def tiles(blocks, levels, num_faces=1, has_color=True, alpha_comp=(), perceptual=True, color_derating=2.0,
+          alpha_derating=2.0, ratio=3.0):
+    """dxt_hc adaptive tile choice (crn_dxt_hc.cpp:386-561).  blocks: uint8
+    CUDA tensor [n, 16, 4]; levels: list of (first_block, num_blocks,
+    block_width[, weight]).  Returns (block_encodings uint8 [n], tile_indices
+    int32 [n], tile_src int32 [n], tile_vec float32 [n, 10])."""
+    n = blocks.shape[0]
+    dev = blocks.device
+    desc = (C.c_uint32 * (3 * len(levels)))(*[int(x) for lv in levels for x in lv[:3]])
+    ac = (C.c_uint32 * 2)(*(list(alpha_comp) + [0, 0])[:2])
+    enc = torch.empty(n, dtype=torch.uint8, device=dev)
+    ti = torch.empty(n, dtype=torch.int32, device=dev)
+    src = torch.empty(n, dtype=torch.int32, device=dev)
+    vec = torch.empty((n, 10), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    check(lib().crn2_hc_tiles_device(_ptr(blocks.contiguous()), n, len(levels), desc, num_faces,
+                                     int(bool(has_color)), len(alpha_comp), ac, int(bool(perceptual)),
+                                     color_derating, alpha_derating, ratio, _ptr(enc), _ptr(ti), _ptr(src), _ptr(vec),
+                                     C.c_void_p(stream)),
+          "crn2_hc_tiles_device")
+    return enc, ti, src, vec

@@ -258,6 +258,27 @@ int crn2_tree_codebook_device(const float* vectors, const uint32_t* weights, uin
                               uint32_t max_splits, float* codebook, uint32_t* codebook_size, uint32_t* vector_leaf,
                               void* stream);
 
+/* --------------------------------------------- dxt_hc adaptive tiles */
+/* dxt_hc::determine_tiles_task (crnlib/crn_dxt_hc.cpp:386-561, with the
+ * derating tables of dxt_hc::compress :122-142) for every level at once, on
+ * the device.  blocks: num_blocks x 16 RGBA8 texels (all levels, device
+ * memory); level_desc: per level {first_block, num_blocks, block_width}
+ * (host memory; widths and per-face heights in blocks must be even, else
+ * CRN2_EUNSUPPORTED); has_color: DXT1 / DXT5 colour blocks; num_alpha and
+ * alpha_comp (host): the alpha channels scored (DXT5: {3}; DXN: {0, 1});
+ * perceptual; the colour and alpha PSNR deratings and the colour / alpha
+ * weighting ratio (dxt_hc::params, defaults 2, 2, 3).  Writes per block its
+ * encoding (0..7) and tile slot (tile_indices); per slot (num_blocks of them)
+ * tile_src = top-left block << 4 | tile (0..8), or 0xFFFFFFFF when unused,
+ * and tile_vec = 10 floats: the colour endpoints (palettize_color, 6) and
+ * per alpha channel 2 (palettize_alpha).  Returns when done; allocates
+ * stream-ordered scratch (not for graph capture). */
+int crn2_hc_tiles_device(const void* blocks, uint32_t num_blocks, uint32_t nlevels, const uint32_t* level_desc,
+                         uint32_t num_faces, int has_color, uint32_t num_alpha, const uint32_t* alpha_comp,
+                         int perceptual, float color_derating, float alpha_derating, float ratio,
+                         uint8_t* block_encodings, uint32_t* tile_indices, uint32_t* tile_src, float* tile_vec,
+                         void* stream);
+
 /* ----------------------------------------- dxt_hc endpoint clustering */
 /* dxt_hc::determine_color_endpoints (crnlib/crn_dxt_hc.cpp:888-972; dim 6,
  * the tiles' colour endpoint vectors with weights (uint)(texels * level

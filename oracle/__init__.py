@@ -405,6 +405,57 @@ def hc_endpoint_clusters(vectors, weights, codebook_size):
     return hc_nearest(vectors, cb), cb
 
 
+def dxt_fast_color(px):
+    """dxt_fast::compress_color_block without refinement (crn_dxt_fast.cpp:
+    725-764) on n texels (n x 4 uint8).  Returns (low16, high16, selectors)."""
+    L = lib()
+    L.orc_dxt_fast_color.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+    L.orc_dxt_fast_color.restype = C.c_uint32
+    p = np.ascontiguousarray(px, np.uint8)
+    sel = np.zeros(p.shape[0], np.uint8)
+    r = L.orc_dxt_fast_color(_p(p), p.shape[0], _p(sel))
+    return r & 0xFFFF, r >> 16, sel
+
+
+def split_vectors(vectors, weights):
+    """split_vectors (crn_tree_clusterizer.h:573-753) on <= 64 vectors of 1 or
+    3 floats.  Returns the two halves' centroids (2, d)."""
+    L = lib()
+    L.orc_split_vectors.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]
+    v = np.ascontiguousarray(vectors, np.float32)
+    w = np.ascontiguousarray(weights, np.uint32)
+    out = np.zeros((2, v.shape[1]), np.float32)
+    L.orc_split_vectors(_p(v), _p(w), v.shape[0], v.shape[1], _p(out))
+    return out
+
+
+def hc_tiles(blocks, levels, num_faces=1, has_color=True, alpha_comp=(), perceptual=True,
+             color_derating=2.0, alpha_derating=2.0, ratio=3.0):
+    """dxt_hc's adaptive tile choice (crn_dxt_hc.cpp:386-561).  blocks: (n, 16,
+    4) uint8 for every level; levels: list of (first_block, num_blocks,
+    block_width[, weight]) (the weight is not used here).  Returns (block_encodings (n,), tile_indices (n,),
+    tile_src (n,) = top-left block << 4 | tile (0xFFFFFFFF: empty slot),
+    tile_vec (n, 10) = colour endpoints (6) and per alpha channel 2)."""
+    L = lib()
+    L.orc_hc_tiles.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_int,
+                               C.c_uint32, C.c_void_p, C.c_int, C.c_float, C.c_float, C.c_float, C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_void_p]
+    b = np.ascontiguousarray(blocks, np.uint8)
+    n = b.shape[0]
+    desc = np.array([[lv[0], lv[1], lv[2]] for lv in levels], np.uint32)
+    ac = np.array(list(alpha_comp) + [0, 0], np.uint32)[:2]
+    enc = np.zeros(n, np.uint8)
+    ti = np.zeros(n, np.uint32)
+    src = np.zeros(n, np.uint32)
+    vec = np.zeros((n, 10), np.float32)
+    rc = L.orc_hc_tiles(_p(b), n, len(levels), _p(desc), num_faces, int(bool(has_color)), len(alpha_comp),
+                        _p(ac), int(bool(perceptual)), color_derating, alpha_derating, ratio, _p(enc), _p(ti),
+                        _p(src), _p(vec))
+    if rc != 0:
+        raise ValueError("orc_hc_tiles: odd level width or face height")
+    return enc, ti, src, vec
+
+
 def hc_color_endpoints(cluster_px, cluster_ofs, block_cluster, blocks, block_weights, block_encodings,
                        perceptual=True):
     """dxt_hc colour endpoint codebook (crn_dxt_hc.cpp:663-754) with the
