@@ -209,3 +209,35 @@ def tiles(blocks, levels, num_faces=1, has_color=True, alpha_comp=(), perceptual
                                      C.c_void_p(stream)),
           "crn2_hc_tiles_device")
     return enc, ti, src, vec
+
+
+def compress(blocks, levels, num_faces=1, has_color=True, alpha_comp=(), perceptual=True,
+             codebook_sizes=(3072, 3072, 3072, 3072), tile_params=(2.0, 2.0, 3.0)):
+    """dxt_hc::compress (crn_dxt_hc.cpp:97-316) on the GPU.  blocks: uint8
+    CUDA tensor [n, 16, 4]; levels: (first_block, num_blocks, block_width,
+    weight) per level.  Returns (endpoint_indices int16 [n, 4] = colour,
+    alpha0, alpha1, reference; selector_indices int16 [n, 3]; colour
+    endpoints int32; colour selectors int32; alpha endpoints int32; alpha
+    selectors int64) -- the palettes that do not apply are None."""
+    n = blocks.shape[0]
+    dev = blocks.device
+    desc = (C.c_uint32 * (3 * len(levels)))(*[int(x) for lv in levels for x in lv[:3]])
+    wts = (C.c_float * len(levels))(*[float(lv[3]) for lv in levels])
+    ac = (C.c_uint32 * 2)(*(list(alpha_comp) + [0, 0])[:2])
+    sizes = (C.c_uint32 * 4)(*codebook_sizes)
+    tp = (C.c_float * 3)(*tile_params)
+    counts = (C.c_uint32 * 4)()
+    ep = torch.empty((n, 4), dtype=torch.int16, device=dev)
+    sel = torch.empty((n, 3), dtype=torch.int16, device=dev)
+    ce = torch.empty(codebook_sizes[0], dtype=torch.int32, device=dev) if has_color else None
+    cs = torch.empty(codebook_sizes[1], dtype=torch.int32, device=dev) if has_color else None
+    ae = torch.empty(codebook_sizes[2], dtype=torch.int32, device=dev) if alpha_comp else None
+    as_ = torch.empty(codebook_sizes[3], dtype=torch.int64, device=dev) if alpha_comp else None
+    stream = torch.cuda.current_stream().cuda_stream
+    check(lib().crn2_hc_compress_device(_ptr(blocks.contiguous()), n, len(levels), desc, wts, num_faces,
+                                        int(bool(has_color)), len(alpha_comp), ac, int(bool(perceptual)), sizes, tp,
+                                        _ptr(ep), _ptr(sel), _ptr(ce), _ptr(ae), _ptr(cs), _ptr(as_),
+                                        counts, C.c_void_p(stream)),
+          "crn2_hc_compress_device")
+    cut = lambda t, k: None if t is None else t[:k]
+    return ep, sel, cut(ce, counts[0]), cut(cs, counts[1]), cut(ae, counts[2]), cut(as_, counts[3])

@@ -258,6 +258,34 @@ int crn2_tree_codebook_device(const float* vectors, const uint32_t* weights, uin
                               uint32_t max_splits, float* codebook, uint32_t* codebook_size, uint32_t* vector_leaf,
                               void* stream);
 
+/* ------------------------------------------------ dxt_hc, end to end */
+/* dxt_hc::compress (crnlib/crn_dxt_hc.cpp:97-316; bool dxt_hc::compress(
+ * color_quad_u8 (*)[16], vector<endpoint_indices_details>&,
+ * vector<selector_indices_details>&, vector<uint32>& color_endpoints,
+ * vector<uint32>& alpha_endpoints, vector<uint32>& color_selectors,
+ * vector<uint64>& alpha_selectors, const params&), crn_dxt_hc.h:176-184) on
+ * the device, every stage above in order.  blocks: num_blocks x 16 RGBA8
+ * (device); level_desc: per level {first_block, num_blocks, block_width} and
+ * level_weight per level (host; dxt_hc::params::m_levels); num_faces;
+ * has_color (DXT1, DXT5) and num_alpha / alpha_comp (host; DXT5 {3}, DXT5A
+ * {c}, DXN_XY {0, 1}, DXN_YX {1, 0}) as params::m_format implies;
+ * codebook_sizes (host): colour endpoints, colour selectors, alpha endpoints,
+ * alpha selectors; tile_params (host, optional): colour and alpha PSNR
+ * derating and the colour / alpha ratio (default 2, 2, 3).  Writes (device)
+ * endpoint_indices (num_blocks x 4 u16: colour, alpha0, alpha1, reference
+ * code), selector_indices (num_blocks x 3 u16), the four palettes (capacity:
+ * their codebook sizes; the unused ones may be NULL) and counts[4] (host:
+ * entries written to each).  The reference's helper-thread splits are
+ * restated for one thread.  Returns CRN2_EUNSUPPORTED when a colour cluster
+ * holds more than 4,096 distinct colours.  Allocates stream-ordered scratch
+ * (not for graph capture). */
+int crn2_hc_compress_device(const void* blocks, uint32_t num_blocks, uint32_t nlevels, const uint32_t* level_desc,
+                            const float* level_weight, uint32_t num_faces, int has_color, uint32_t num_alpha,
+                            const uint32_t* alpha_comp, int perceptual, const uint32_t* codebook_sizes,
+                            const float* tile_params, uint16_t* endpoint_indices, uint16_t* selector_indices,
+                            uint32_t* color_endpoints, uint32_t* alpha_endpoints, uint32_t* color_selectors,
+                            uint64_t* alpha_selectors, uint32_t* counts, void* stream);
+
 /* --------------------------------------------- dxt_hc adaptive tiles */
 /* dxt_hc::determine_tiles_task (crnlib/crn_dxt_hc.cpp:386-561, with the
  * derating tables of dxt_hc::compress :122-142) for every level at once, on

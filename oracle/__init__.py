@@ -456,6 +456,124 @@ def hc_tiles(blocks, levels, num_faces=1, has_color=True, alpha_comp=(), percept
     return enc, ti, src, vec
 
 
+_TILE_OFFSETS = (0, 16, 32, 48, 0, 32, 64, 96, 64)
+
+
+def _tile_texels(blocks, b, width, t):
+    """The texels of tile t of the chunk whose top-left block is b
+    (crn_dxt_hc.cpp:431-445): the four blocks, then the 8x8 in raster order."""
+    four = np.concatenate([blocks[b], blocks[b + width], blocks[b + 1], blocks[b + width + 1]])
+    q = [blocks[b], blocks[b + 1], blocks[b + width], blocks[b + width + 1]]
+    raster = np.concatenate([np.concatenate([q[2 * (y // 4) + 0][4 * (y % 4):4 * (y % 4) + 4],
+                                             q[2 * (y // 4) + 1][4 * (y % 4):4 * (y % 4) + 4]]) for y in range(8)])
+    both = np.concatenate([four, raster])
+    o = _TILE_OFFSETS[t]
+    return both[o:o + (16 << (t >> 2))]
+
+
+def hc_compress(blocks, levels, num_faces=1, has_color=True, alpha_comp=(), perceptual=True,
+                codebook_sizes=(3072, 3072, 3072, 3072), tile_params=(2.0, 2.0, 3.0)):
+    """dxt_hc::compress (crn_dxt_hc.cpp:97-316) composed from the restated
+    stages.  blocks: (n, 16, 4) uint8; levels: (first_block, num_blocks,
+    block_width, weight) per level.  Returns (endpoint_indices (n, 4) uint16 =
+    colour, alpha0, alpha1, reference; selector_indices (n, 3) uint16;
+    colour endpoints, colour selectors, alpha endpoints, alpha selectors)."""
+    blocks = np.ascontiguousarray(blocks, np.uint8)
+    n = blocks.shape[0]
+    na = len(alpha_comp)
+    enc, tidx, src, tvec = hc_tiles(blocks, levels, num_faces, has_color, alpha_comp, perceptual, *tile_params)
+    level_of = np.zeros(n, np.int64)
+    width_of = np.zeros(n, np.int64)
+    for l, lv in enumerate(levels):
+        level_of[lv[0]:lv[0] + lv[1]] = l
+        width_of[lv[0]:lv[0] + lv[1]] = lv[2]
+    weights = np.array([lv[3] for lv in levels], np.float32)
+    slots = np.flatnonzero(src != 0xFFFFFFFF)
+    pos = np.full(n, -1, np.int64)
+    pos[slots] = np.arange(len(slots))
+    nt = len(slots)
+    tb, tt = src[slots] >> 4, src[slots] & 15
+    size = (16 << (tt >> 2)).astype(np.int64)
+    texels = [_tile_texels(blocks, int(b), int(width_of[b]), int(t)) for b, t in zip(tb, tt)]
+    block_tile = pos[tidx]
+    bw = weights[level_of]
+
+    def lists(cl, k, entries):
+        ofs = np.zeros(k + 1, np.uint32)
+        for e, c in enumerate(cl):
+            ofs[c + 1] += len(entries[e])
+        ofs = np.cumsum(ofs).astype(np.uint32)
+        order = np.argsort(cl, kind="stable")
+        data = np.concatenate([entries[e] for e in order]) if len(order) else np.zeros(0)
+        return ofs, data
+
+    out_ce = out_cs = out_ae = out_as = None
+    ep = np.zeros((n, 4), np.uint16)
+    sel = np.zeros((n, 3), np.uint16)
+    comps = []
+    if has_color:
+        cvec = tvec[slots, :6]
+        cw = (size.astype(np.float32) * weights[level_of[tb]]).astype(np.uint32)
+        ccl, cb = hc_endpoint_clusters(cvec, cw, min(nt, codebook_sizes[0]))
+        k = len(cb)
+        ofs, px = lists(ccl, k, texels)
+        bcc = ccl[block_tile]
+        ends, cv, bsel = hc_color_endpoints(px.reshape(-1, 4), ofs, bcc, blocks, bw, enc, perceptual)
+        csel, cidx, cused = hc_color_selector_codebook(bsel, codebook_sizes[1], blocks, bcc, cv, perceptual)
+        valid = np.diff(ofs.astype(np.int64)) > 0
+        out_ce, ce_map = _dedupe(ends, valid)
+        out_cs, cs_map = _dedupe(csel, cused.astype(bool))
+        comps.append((0, ce_map[bcc], cs_map[cidx]))
+    if na:
+        avec = np.concatenate([tvec[slots, 6 + 2 * a:8 + 2 * a] for a in range(na)])
+        aw = np.tile(size, na).astype(np.uint32)
+        acl, cb = hc_endpoint_clusters(avec, aw, min(nt, codebook_sizes[2]))
+        k = len(cb)
+        entries = [texels[e % nt][:, alpha_comp[e // nt]] for e in range(na * nt)]
+        ofs, vals = lists(acl, k, entries)
+        bac = np.stack([acl[a * nt + block_tile] for a in range(na)])
+        bav = np.stack([blocks[:, :, alpha_comp[a]] for a in range(na)])
+        bsel = []
+        for a in range(na):
+            ends, av, rv, bs = hc_alpha_endpoints(vals, ofs, bac[a], bav[a], enc)
+            bsel.append(bs)
+        asel, aidx, aused = hc_alpha_selector_codebook(np.stack(bsel), codebook_sizes[3], bav, bac, av, rv)
+        valid = np.diff(ofs.astype(np.int64)) > 0
+        out_ae, ae_map = _dedupe(ends & 0xFFFF, valid)
+        out_as, as_map = _dedupe(asel, aused.astype(bool))
+        for a in range(na):
+            comps.append((1 + a, ae_map[bac[a]], as_map[aidx[a]]))
+    for c, e, s_ in comps:
+        ep[:, c] = e
+        sel[:, c] = s_
+    for l, lv in enumerate(levels):
+        first, nb, w = lv[0], lv[1], lv[2]
+        for r in range(nb):
+            b = first + r
+            bx, by = r % w, r // w
+            top = by != 0
+            left = top or bx != 0
+            for c, _, _ in comps:
+                left = left and ep[b, c] == ep[b - 1, c]
+                top = top and ep[b, c] == ep[b - w, c]
+            ep[b, 3] = 1 if left else 2 if top else 0
+    return ep, sel, out_ce, out_cs, out_ae, out_as
+
+
+def _dedupe(keys, valid):
+    """Entries with valid set, without repeats in first-appearance order, and
+    each entry's position in that list (crn_dxt_hc.cpp:210-280)."""
+    out, where, remap = [], {}, np.zeros(len(keys), np.int64)
+    for i, (k, v) in enumerate(zip(keys.tolist(), valid.tolist())):
+        if not v:
+            continue
+        if k not in where:
+            where[k] = len(out)
+            out.append(k)
+        remap[i] = where[k]
+    return np.array(out, keys.dtype), remap
+
+
 def hc_color_endpoints(cluster_px, cluster_ofs, block_cluster, blocks, block_weights, block_encodings,
                        perceptual=True):
     """dxt_hc colour endpoint codebook (crn_dxt_hc.cpp:663-754) with the
