@@ -92,6 +92,81 @@ def encode_leg(size, frac):
             "fallback_blocks": nfb, "fallback_frac": frac}
 
 
+def hc_leg(size, cpu_size):
+    """dxt_hc of one size x size RGB texture (SURVEY config 3: 4096 colour
+    endpoints, 8192 colour selectors, hierarchical, no mips) on the GPU, the
+    colour selector assignment on INT8 matrix cores timed alone, and the
+    oracle's single-thread composition on a cpu_size^2 sample."""
+    import numpy as np
+    import torch
+    from crunch2_amd import dxt1, hc, texture
+    img = texture.synth(size, size, 11, texture.RGB)
+    blocks = dxt1.image_blocks(img)
+    n = blocks.shape[0]
+    levels = [(0, n, size // 4, 1.0)]
+    sizes = (4096, 8192, 3072, 3072)
+    tb = torch.from_numpy(blocks).cuda()
+    hc.compress(tb, levels, codebook_sizes=sizes)       # warm-up
+    reps, t0 = 3, None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = hc.compress(tb, levels, codebook_sizes=sizes)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    ep, sel = out[0].long(), out[1].long()
+    ce, cs = out[2].long() & 0xFFFFFFFF, out[3].long() & 0xFFFFFFFF
+    # decode (colour selectors in linear order -> DXT order) for the PSNR
+    lin2dxt = torch.tensor([0, 2, 3, 1], device=tb.device)
+    e = ce[ep[:, 0]]
+    s_ = cs[sel[:, 0]]
+    codes = (s_[:, None] >> (2 * torch.arange(16, device=tb.device))[None, :]) & 3
+    raw = lin2dxt[codes]
+    packed = torch.zeros_like(s_)
+    for i in range(16):
+        packed |= raw[:, i] << (2 * i)
+    sse = int(decode_sse(tb, e, packed).sum())
+    # the INT8 matrix-core assignment alone, on the final colour selectors against per-block cluster colours
+    used_sel = out[3]
+    m = used_sel.shape[0]
+    best = torch.empty(n, dtype=torch.int32, device=tb.device)
+    cols = torch.zeros((ce.shape[0], 4, 4), dtype=torch.uint8, device=tb.device)
+    c565 = torch.stack([ce & 0xFFFF, ce >> 16], 1)
+    r, g, b = (c565 >> 11) & 31, (c565 >> 5) & 63, c565 & 31
+    p = torch.stack([(r << 3) | (r >> 2), (g << 2) | (g >> 4), (b << 3) | (b >> 2)], 2)
+    lin = torch.stack([p[:, 0], (2 * p[:, 0] + p[:, 1]) // 3, (p[:, 0] + 2 * p[:, 1]) // 3, p[:, 1]], 1)
+    cols[:, :, :3] = lin.to(torch.uint8)
+    ei = out[0][:, 0].to(torch.int32).contiguous()
+    hc.color_selectors(tb, ei, cols, used_sel)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(5):
+        hc.color_selectors(tb, ei, cols, used_sel)
+    ev1.record()
+    torch.cuda.synchronize()
+    sel_ms = ev0.elapsed_time(ev1) / 5
+    ops = 2.0 * n * m * 64 * 4
+    line = {"workload": "dxt_hc of one %dx%d RGB texture (%d blocks), 4096 colour endpoints / 8192 colour "
+                        "selectors, perceptual, no mips" % (size, size, n),
+            "mtexels_s": round(n * 16 / t / 1e6, 2), "ms": round(t * 1e3, 2),
+            "color_endpoints": int(ce.shape[0]), "color_selectors": int(m),
+            "psnr_db": round(psnr(sse, n * 16), 3),
+            "selector_assignment": {"ms": round(sel_ms, 3), "int8_tops": round(ops / (sel_ms * 1e-3) / 1e12, 1),
+                                    "peak_tops": 5000.0, "frac": round(ops / (sel_ms * 1e-3) / 5.0e15, 4),
+                                    "ops": ops}}
+    if cpu_size:
+        import oracle
+        cimg = texture.synth(cpu_size, cpu_size, 11, texture.RGB)
+        cb = dxt1.image_blocks(cimg)
+        t0 = time.perf_counter()
+        oracle.hc_compress(cb, [(0, cb.shape[0], cpu_size // 4, 1.0)], codebook_sizes=sizes)
+        ct = time.perf_counter() - t0
+        line["cpu_port"] = {"mtexels_s": round(cb.shape[0] * 16 / ct / 1e6, 3), "cores": 1,
+                            "sample": "the oracle's dxt_hc composition on a %dx%d texture of the same generator "
+                                      "(%.1fs)" % (cpu_size, cpu_size, ct)}
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +184,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline sample length")
     ap.add_argument("--encode-size", type=int, default=8192,
                     help="side of the RGB texture of the DXT1 encode leg (learned vs plain); 0 skips it")
+    ap.add_argument("--hc-size", type=int, default=2048,
+                    help="side of the RGB texture of the dxt_hc leg; 0 skips it")
+    ap.add_argument("--hc-cpu-size", type=int, default=512,
+                    help="side of the texture the oracle's dxt_hc composition is timed on; 0 skips it")
     ap.add_argument("--fallback-frac", type=float, default=0.10,
                     help="fraction of blocks (lowest confidence first) that also run the plain search in the encode leg")
     args = ap.parse_args()
@@ -322,6 +401,8 @@ def main():
         assert lib().crn2_batch_status(pl, C.c_void_p(stream.cuda_stream)) == 0
     if args.encode_size:
         line["encode"] = encode_leg(args.encode_size, args.fallback_frac)
+    if args.hc_size:
+        line["dxt_hc"] = hc_leg(args.hc_size, args.hc_cpu_size)
     print(json.dumps(line))
     for pl in plans:
         lib().crn2_batch_destroy(pl)

@@ -28,8 +28,10 @@
 
 namespace {
 
-constexpr uint32_t kMaxU = 4096;        // unique colours per cluster
-constexpr uint32_t kHashCap = 8192;     // colour hash slots (power of two, >= 2 kMaxU)
+// Largest cluster: 4096 unique colours.  The kernel comes in three LDS sizes
+// (512, 2048 and 4096 unique colours) so that small clusters, the common case,
+// run six or two waves per CU instead of one.
+constexpr uint32_t kMaxU = 4096;
 
 __device__ __constant__ int16_t c_uber_probe[15] = {0, 1, 2, 3, 5, 7, 9, 10, 13, 15, 19, 27, 43, 59, 91};
 __device__ __constant__ int8_t c_adj[26][3] = {
@@ -39,12 +41,15 @@ __device__ __constant__ int8_t c_adj[26][3] = {
     {0, 1, 1}, {1, 1, 1}};
 __device__ uint8_t g_c_om5[256][2], g_c_om6[256][2];
 
-struct Lds {
-    uint32_t col[kMaxU];     // r | g << 8 | b << 16
-    uint32_t w[kMaxU];
-    uint8_t sel[kMaxU];
-    uint32_t hkey[kHashCap]; // colour | 0xFF000000, 0 = empty
-    uint16_t hval[kHashCap];
+template <uint32_t MaxU>
+struct LdsT {
+    static constexpr uint32_t kCapU = MaxU;        // unique colours
+    static constexpr uint32_t kCapH = 2 * MaxU;    // colour hash slots (power of two)
+    uint32_t col[MaxU];      // r | g << 8 | b << 16
+    uint32_t w[MaxU];
+    uint8_t sel[MaxU];
+    uint32_t hkey[kCapH];    // colour | 0xFF000000, 0 = empty
+    uint16_t hval[kCapH];
     unsigned long long pl[3][64][3];   // error planes: count, sum, sum of squares per quantised value
     uint64_t rlo[32], rhi[32], glo[64], ghi[64], blo[32], bhi[32];
     uint64_t err[4][256];
@@ -149,6 +154,7 @@ __device__ inline Pal palette4(uint32_t lo, uint32_t hi, int alt)
 
 // evaluate_solution_hc (crn_dxt1.cpp:1759-1835): sum over colours of the
 // weighted distance to the nearest of the four colours.
+template <class Lds>
 __device__ bool eval_hc(St& s, const Lds& L, uint32_t lo, uint32_t hi, int alt)
 {
     const Pal p = palette4(lo, hi, alt);
@@ -182,6 +188,7 @@ __device__ bool eval_hc(St& s, const Lds& L, uint32_t lo, uint32_t hi, int alt)
 }
 
 // compute_selectors_hc (crn_dxt1.cpp:1837-1869)
+template <class Lds>
 __device__ void compute_selectors(St& s, Lds& L)
 {
     if (s.sel_valid) return;
@@ -206,6 +213,7 @@ __device__ void compute_selectors(St& s, Lds& L)
 
 // evaluate_solution with the plane lower bound (crn_dxt1.cpp:1312-1340); the
 // bound only skips candidates that cannot win.
+template <class Lds>
 __device__ bool evaluate(St& s, Lds& L, uint32_t lo, uint32_t hi, int alt)
 {
     int ar, ag, ab, br, bg, bb;
@@ -218,6 +226,7 @@ __device__ bool evaluate(St& s, Lds& L, uint32_t lo, uint32_t hi, int alt)
     return eval_hc(s, L, lo, hi, alt);
 }
 
+template <class Lds>
 __device__ inline bool eval_canon(St& s, Lds& L, uint32_t lo, uint32_t hi)
 {
     canon(lo, hi);
@@ -228,6 +237,7 @@ __device__ inline int clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : 
 
 // normalised (and perceptually weighted) colour i minus the mean, as
 // compute_vectors stores it (crn_dxt1.cpp:155-189)
+template <class Lds>
 __device__ inline void norm_of(const St& s, const Lds& L, uint32_t i, float* n, float* nw)
 {
     const uint32_t c = L.col[i];
@@ -238,6 +248,7 @@ __device__ inline void norm_of(const St& s, const Lds& L, uint32_t i, float* n, 
     }
 }
 
+template <class Lds>
 __device__ void compute_vectors(St& s, const Lds& L)
 {
     for (int k = 0; k < 3; k++) s.mean[k] = s.meanw[k] = 0.0f;
@@ -260,6 +271,7 @@ __device__ void compute_vectors(St& s, const Lds& L)
 }
 
 // compute_pca (crn_dxt1.cpp:192-256) over the weighted vectors
+template <class Lds>
 __device__ void compute_pca(St& s, const Lds& L, float* axis)
 {
     double cov[6] = {0, 0, 0, 0, 0, 0};
@@ -342,6 +354,7 @@ __device__ bool ray_unit_box(float* coord, const float* org, const float* dir)
 // refine_solution (crn_dxt1.cpp:525-698), levels 0 and 1.  The sums are of
 // integers times integer weights (exact in double), so a lane-split sum is
 // the reference's value.
+template <class Lds>
 __device__ bool refine(St& s, Lds& L, int level)
 {
     compute_selectors(s, L);
@@ -424,6 +437,7 @@ struct FastRandom {
 
 // try_median4 (crn_dxt1.cpp:1181-1308): 4-means over the colours (sequential
 // float sums), then the six pairs of means and refine_solution.
+template <class Lds>
 __device__ bool try_median4(St& s, Lds& L, const float* lo, const float* hi)
 {
     float means[4][3];
@@ -524,6 +538,7 @@ __device__ bool try_median4(St& s, Lds& L, const float* lo, const float* hi)
 // selector vector as last computed: the hc evaluators leave it untouched, so
 // it may belong to an earlier solution, and it is empty before the first
 // compute_selectors.
+template <class Lds>
 __device__ bool sel_all_equal(const St& s, const Lds& L)
 {
     if (!s.sel_nonempty) return false;
@@ -533,6 +548,7 @@ __device__ bool sel_all_equal(const St& s, const Lds& L)
 }
 
 // try_average_block_as_solid (crn_dxt1.cpp:93-153), uber branch
+template <class Lds>
 __device__ bool try_average_solid(St& s, Lds& L)
 {
     uint64_t tr = 0, tg = 0, tb = 0, tw = 0;
@@ -563,6 +579,7 @@ __device__ bool try_average_solid(St& s, Lds& L)
 }
 
 // compute_endpoint_component_errors (crn_dxt1.cpp:369-418)
+template <class Lds>
 __device__ void comp_errors(St& s, Lds& L, int comp, uint64_t* brem)
 {
     uint64_t W[4] = {0, 0, 0, 0}, WP2[4] = {0, 0, 0, 0}, WPP[4] = {0, 0, 0, 0};
@@ -607,6 +624,7 @@ __device__ void comp_errors(St& s, Lds& L, int comp, uint64_t* brem)
 }
 
 // optimize_endpoint_comps (crn_dxt1.cpp:420-486)
+template <class Lds>
 __device__ void optimize_comps(St& s, Lds& L)
 {
     compute_selectors(s, L);
@@ -662,6 +680,7 @@ __device__ inline void lerp_color(int ar, int ag, int ab, int br, int bg, int bb
 }
 
 // try_combinatorial_encoding (crn_dxt1.cpp:1886-1997): 2 to 4 colours only
+template <class Lds>
 __device__ void try_combinatorial(St& s, Lds& L)
 {
     if (s.nu < 2 || s.nu > 4) return;
@@ -736,6 +755,7 @@ __device__ uint32_t sweep(uint32_t range, const float* base, const float (*axis2
     return n;
 }
 
+template <class Lds>
 __device__ void lattice(St& s, Lds& L, int which)
 {
     int lr, lg, lb;
@@ -767,6 +787,7 @@ __device__ void lattice(St& s, Lds& L, int which)
 }
 
 // optimize_endpoints (crn_dxt1.cpp:703-1067), uber quality
+template <class Lds>
 __device__ void optimize_endpoints(St& s, Lds& L, float* lo, float* hi)
 {
     const float olo[3] = {lo[0], lo[1], lo[2]}, ohi[3] = {hi[0], hi[1], hi[2]};
@@ -815,6 +836,7 @@ __device__ void optimize_endpoints(St& s, Lds& L, float* lo, float* hi)
 }
 
 // handle_multicolor_block (crn_dxt1.cpp:1069-1178)
+template <class Lds>
 __device__ void handle_multicolor(St& s, Lds& L)
 {
     uint32_t npass = 1;
@@ -912,25 +934,44 @@ struct ClusterOut {
     uint32_t pad;
 };
 
+template <class Lds>
 __device__ inline int hash_find(const Lds& L, uint32_t key)
 {
-    uint32_t h = hash32(key) & (kHashCap - 1);
+    uint32_t h = hash32(key) & (Lds::kCapH - 1);
     while (L.hkey[h]) {
         if (L.hkey[h] == key) return L.hval[h];
-        h = (h + 1) & (kHashCap - 1);
+        h = (h + 1) & (Lds::kCapH - 1);
     }
     return -1;
 }
 
+// Clusters by texel count into the three LDS sizes (any order within a class:
+// each cluster's result depends only on its own texels).
+__global__ void __launch_bounds__(256) dxt1_cluster_classes_kernel(const uint32_t* __restrict__ ofs,
+                                                                  uint32_t nclusters, uint32_t* __restrict__ count,
+                                                                  uint32_t* __restrict__ list)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nclusters) return;
+    const uint32_t n = ofs[c + 1] - ofs[c];
+    const uint32_t k = n <= 512 ? 0 : n <= 2048 ? 1 : 2;
+    list[(size_t)k * nclusters + atomicAdd(&count[k], 1u)] = c;
+}
+
+template <uint32_t MaxU>
 __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __restrict__ px,
-                                                          const uint32_t* __restrict__ ofs, uint32_t nclusters,
-                                                          int perceptual, ClusterOut* __restrict__ out,
+                                                          const uint32_t* __restrict__ ofs,
+                                                          const uint32_t* __restrict__ list,
+                                                          const uint32_t* __restrict__ count, int perceptual,
+                                                          ClusterOut* __restrict__ out,
                                                           uint8_t* __restrict__ selectors)
 {
+    typedef LdsT<MaxU> Lds;
+    constexpr uint32_t kHashCap = Lds::kCapH;
     extern __shared__ __align__(16) uint8_t lds_raw[];
     Lds& L = *reinterpret_cast<Lds*>(lds_raw);
-    const uint32_t cl = blockIdx.x;
-    if (cl >= nclusters) return;
+    if (blockIdx.x >= *count) return;
+    const uint32_t cl = list[blockIdx.x];
     const uint32_t lane = lane_id();
     const uint32_t p0 = ofs[cl], p1 = ofs[cl + 1];
     const uint32_t n = p1 - p0;
@@ -955,7 +996,7 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
             while (L.hkey[h] && L.hkey[h] != key) h = (h + 1) & (kHashCap - 1);
             if (L.hkey[h] == key) {
                 L.w[L.hval[h]]++;
-            } else if (s.nu < kMaxU) {
+            } else if (s.nu < Lds::kCapU) {
                 L.hkey[h] = key;
                 L.hval[h] = (uint16_t)s.nu;
                 L.col[s.nu] = key & 0x00FFFFFFu;
@@ -1079,13 +1120,36 @@ extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offse
         }
         if (hipMemcpyToSymbol(HIP_SYMBOL(g_c_om5), t5, 512) != hipSuccess) return CRN2_EHIP;
         if (hipMemcpyToSymbol(HIP_SYMBOL(g_c_om6), t6, 512) != hipSuccess) return CRN2_EHIP;
-        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(Lds)) != hipSuccess)
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<512>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<512>)) != hipSuccess ||
+            hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<2048>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<2048>)) != hipSuccess ||
+            hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<kMaxU>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<kMaxU>)) != hipSuccess)
             return CRN2_EHIP;
         init = true;
     }
-    hipLaunchKernelGGL(dxt1_cluster_kernel, dim3(nclusters), dim3(64), sizeof(Lds), static_cast<hipStream_t>(stream),
-                       static_cast<const uint32_t*>(rgba), offsets, nclusters, flags & 1, static_cast<ClusterOut*>(out),
-                       selectors);
-    return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    uint32_t* scratch = nullptr;      // 3 class counts + 3 lists of nclusters
+    if (hipMallocAsync((void**)&scratch, (3 + 3 * (size_t)nclusters) * sizeof(uint32_t), st) != hipSuccess)
+        return CRN2_ENOMEM;
+    int rc = CRN2_OK;
+    if (hipMemsetAsync(scratch, 0, 3 * sizeof(uint32_t), st) != hipSuccess) rc = CRN2_EHIP;
+    if (rc == CRN2_OK) {
+        uint32_t* list = scratch + 3;
+        const uint32_t* px = static_cast<const uint32_t*>(rgba);
+        ClusterOut* o = static_cast<ClusterOut*>(out);
+        hipLaunchKernelGGL(dxt1_cluster_classes_kernel, dim3((nclusters + 255) / 256), dim3(256), 0, st, offsets,
+                           nclusters, scratch, list);
+        // largest clusters first: they take longest
+        hipLaunchKernelGGL(dxt1_cluster_kernel<kMaxU>, dim3(nclusters), dim3(64), sizeof(LdsT<kMaxU>), st, px, offsets,
+                           list + 2 * (size_t)nclusters, scratch + 2, flags & 1, o, selectors);
+        hipLaunchKernelGGL(dxt1_cluster_kernel<2048>, dim3(nclusters), dim3(64), sizeof(LdsT<2048>), st, px, offsets,
+                           list + (size_t)nclusters, scratch + 1, flags & 1, o, selectors);
+        hipLaunchKernelGGL(dxt1_cluster_kernel<512>, dim3(nclusters), dim3(64), sizeof(LdsT<512>), st, px, offsets,
+                           list, scratch, flags & 1, o, selectors);
+        if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
+    }
+    (void)hipFreeAsync(scratch, st);
+    return rc;
 }

@@ -191,7 +191,8 @@ int crn2_dxt1_blocks_device(const void* rgba, uint32_t nblocks, int quality, int
  * colours, u32 m_reordered | m_alternate_rounding << 2 | m_enforce_selector
  * << 3 | m_enforced_selector << 4 | 256 when the cluster has more than 4096
  * unique colours (not encoded), u32 0}; selectors (optional): the 2-bit
- * DXT1 code of every texel. */
+ * DXT1 code of every texel.  Allocates stream-ordered scratch (not for graph
+ * capture). */
 int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags, void* out,
                               uint8_t* selectors, void* stream);
 
