@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=1, help="texture pairs (distinct seeds)")
-    ap.add_argument("--copies", type=int, default=64,
+    ap.add_argument("--copies", type=int, default=128,
                     help="independent copies of every file per step (separate contexts and outputs), so that one "
                          "launch sequence moves hundreds of MB")
     ap.add_argument("--mixed", type=int, default=0,

@@ -334,9 +334,11 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
 // and the walker waits (bounded by the producer's progress) only when the ring
 // does not yet hold kNeed entries past the group's start.
 constexpr uint32_t kSeg = 1024;                       // table entries per ring segment
-constexpr uint32_t kRingSegs = 8;
-constexpr uint32_t kRingPos = kSeg * kRingSegs;
-constexpr uint32_t kBatch = 4;                        // segments per producer batch
+// The ring holds RS segments (template parameter) and the producer loads RS / 2
+// at a time: 8 (80 KiB of LDS, one walker per CU, the deepest prefetch: the
+// shortest walk of a lone level) or 4 (40 KiB, three walkers per CU: more
+// levels at once when a plan holds hundreds of them).
+constexpr uint32_t kPlanJobsSmallRing = 512;
 constexpr uint32_t kGroup = 8;                        // pairs / blocks per recorded start bit
 constexpr uint32_t kMaxEGroups = 64;                  // even-row groups per row (one per lane): grids up to 1024 blocks wide
 constexpr uint32_t kNeed = 1152 + 128 + 64;           // a group's longest move + the two windows
@@ -346,7 +348,9 @@ constexpr uint32_t kNeed = 1152 + 128 + 64;           // a group's longest move 
 // CRN2_EHIP, so a fault in the hand-off cannot hang the device.
 constexpr uint32_t kSpinCap = 1u << 21;
 
+template <uint32_t RS>
 struct WalkLds {
+    static constexpr uint32_t kRingPos = kSeg * RS;
     uint32_t o2[kRingPos];
     uint32_t e8[kRingPos];
     uint16_t x[kRingPos];
@@ -370,14 +374,17 @@ __device__ inline uint32_t lds_get(const uint32_t* p) { return __hip_atomic_load
 __device__ inline void lds_put(uint32_t* p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
 
 // Per lane and segment: 4 + 4 vectors of O2 / E8 entries, 2 of X entries.
+template <uint32_t RS>
 struct SegVecs {
-    u32x4 v[kBatch][10];
+    u32x4 v[RS / 2][10];
 };
 
 // Segments past the table reload its last one: every entry from the end of
 // the stream on is the same, and the table's last segment lies past the end.
-__device__ inline void producer_load(const LevelJob& j, uint32_t tsegs, uint32_t s0, uint32_t lane, SegVecs& r)
+template <uint32_t RS>
+__device__ inline void producer_load(const LevelJob& j, uint32_t tsegs, uint32_t s0, uint32_t lane, SegVecs<RS>& r)
 {
+    constexpr uint32_t kBatch = RS / 2;
 #pragma unroll
     for (uint32_t b = 0; b < kBatch; b++) {
         const size_t at = (size_t)min(s0 + b, tsegs - 1) * kSeg;
@@ -394,8 +401,10 @@ __device__ inline void producer_load(const LevelJob& j, uint32_t tsegs, uint32_t
     }
 }
 
-__device__ inline void producer_store(WalkLds& sh, uint32_t s0, uint32_t lane, const SegVecs& r)
+template <uint32_t RS>
+__device__ inline void producer_store(WalkLds<RS>& sh, uint32_t s0, uint32_t lane, const SegVecs<RS>& r)
 {
+    constexpr uint32_t kBatch = RS / 2, kRingSegs = RS;
 #pragma unroll
     for (uint32_t b = 0; b < kBatch; b++) {
         const uint32_t at = ((s0 + b) % kRingSegs) * kSeg;
@@ -414,10 +423,12 @@ __device__ inline void producer_store(WalkLds& sh, uint32_t s0, uint32_t lane, c
 
 // Wave 1: loads batch n + 1 while batch n waits for ring space; stops when
 // the walker says it is done.
-__device__ void walk_producer(const LevelJob& j, WalkLds& sh, uint32_t lane)
+template <uint32_t RS>
+__device__ void walk_producer(const LevelJob& j, WalkLds<RS>& sh, uint32_t lane)
 {
+    constexpr uint32_t kBatch = RS / 2, kRingSegs = RS;
     const uint32_t tsegs = j.tlen / kSeg;
-    SegVecs r;
+    SegVecs<RS> r;
     uint32_t s = 0;
     producer_load(j, tsegs, s, lane, r);
     for (;;) {
@@ -437,8 +448,10 @@ __device__ void walk_producer(const LevelJob& j, WalkLds& sh, uint32_t lane)
 }
 
 // Wave 0.
-__device__ void walk_stream(const LevelJob& j, WalkLds& sh, uint32_t lane, int32_t* status)
+template <uint32_t RS>
+__device__ void walk_stream(const LevelJob& j, WalkLds<RS>& sh, uint32_t lane, int32_t* status)
 {
+    constexpr uint32_t kRingPos = kSeg * RS;
     constexpr uint32_t kMask = kRingPos - 1;
     uint32_t lim = 0;                 // ring holds table entries [.., lim)
     bool stuck = false;
@@ -597,10 +610,11 @@ __device__ void walk_stream(const LevelJob& j, WalkLds& sh, uint32_t lane, int32
     if (stuck && lane == 0) atomicExch(status, CRN2_EHIP);
 }
 
+template <uint32_t RS>
 __global__ void __launch_bounds__(128) crnd_walk_kernel(const LevelJob* __restrict__ jobs,
                                                         const TexInfo* __restrict__ texs)
 {
-    __shared__ __align__(16) WalkLds sh;
+    __shared__ __align__(16) WalkLds<RS> sh;
     const LevelJob& j = jobs[blockIdx.x];
     if (threadIdx.x == 0) {
         sh.ready = 0;
@@ -1487,7 +1501,10 @@ static int launch_plan(crn2_batch* B, hipStream_t stream, Profile::Ev* ev)
                            B->d_texs);
         HIPCHK(hipGetLastError());
         if (prof) HIPCHK(hipEventRecord(ev->e[1], stream));
-        hipLaunchKernelGGL(crnd_walk_kernel, dim3(B->njobs), dim3(128), 0, stream, B->d_jobs, B->d_texs);
+        if (B->njobs >= kPlanJobsSmallRing)
+            hipLaunchKernelGGL(crnd_walk_kernel<4>, dim3(B->njobs), dim3(128), 0, stream, B->d_jobs, B->d_texs);
+        else
+            hipLaunchKernelGGL(crnd_walk_kernel<8>, dim3(B->njobs), dim3(128), 0, stream, B->d_jobs, B->d_texs);
         HIPCHK(hipGetLastError());
         if (prof) HIPCHK(hipEventRecord(ev->e[2], stream));
         hipLaunchKernelGGL(crnd_sym_kernel<false>, dim3(B->wg_sym), dim3(256), 0, stream, B->d_jobs, B->njobs,

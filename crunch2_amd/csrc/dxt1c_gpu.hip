@@ -28,9 +28,10 @@
 
 namespace {
 
-// Largest cluster: 4096 unique colours.  The kernel comes in three LDS sizes
-// (512, 2048 and 4096 unique colours) so that small clusters, the common case,
-// run six or two waves per CU instead of one.
+// Largest cluster: 4096 unique colours.  The kernel comes in four LDS sizes
+// (512, 1024, 2048 and 4096 unique colours: 26, 36, 57 and 97 KiB) so that
+// small clusters, the common case, run six, four or two waves per CU instead
+// of one.
 constexpr uint32_t kMaxU = 4096;
 
 __device__ __constant__ int16_t c_uber_probe[15] = {0, 1, 2, 3, 5, 7, 9, 10, 13, 15, 19, 27, 43, 59, 91};
@@ -945,7 +946,7 @@ __device__ inline int hash_find(const Lds& L, uint32_t key)
     return -1;
 }
 
-// Clusters by texel count into the three LDS sizes (any order within a class:
+// Clusters by texel count into the four LDS sizes (any order within a class:
 // each cluster's result depends only on its own texels).
 __global__ void __launch_bounds__(256) dxt1_cluster_classes_kernel(const uint32_t* __restrict__ ofs,
                                                                   uint32_t nclusters, uint32_t* __restrict__ count,
@@ -954,7 +955,7 @@ __global__ void __launch_bounds__(256) dxt1_cluster_classes_kernel(const uint32_
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nclusters) return;
     const uint32_t n = ofs[c + 1] - ofs[c];
-    const uint32_t k = n <= 512 ? 0 : n <= 2048 ? 1 : 2;
+    const uint32_t k = n <= 512 ? 0 : n <= 1024 ? 1 : n <= 2048 ? 2 : 3;
     list[(size_t)k * nclusters + atomicAdd(&count[k], 1u)] = c;
 }
 
@@ -1122,6 +1123,8 @@ extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offse
         if (hipMemcpyToSymbol(HIP_SYMBOL(g_c_om6), t6, 512) != hipSuccess) return CRN2_EHIP;
         if (hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<512>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<512>)) != hipSuccess ||
+            hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<1024>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<1024>)) != hipSuccess ||
             hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<2048>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<2048>)) != hipSuccess ||
             hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<kMaxU>),
@@ -1130,21 +1133,23 @@ extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offse
         init = true;
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
-    uint32_t* scratch = nullptr;      // 3 class counts + 3 lists of nclusters
-    if (hipMallocAsync((void**)&scratch, (3 + 3 * (size_t)nclusters) * sizeof(uint32_t), st) != hipSuccess)
+    uint32_t* scratch = nullptr;      // 4 class counts + 4 lists of nclusters
+    if (hipMallocAsync((void**)&scratch, (4 + 4 * (size_t)nclusters) * sizeof(uint32_t), st) != hipSuccess)
         return CRN2_ENOMEM;
     int rc = CRN2_OK;
-    if (hipMemsetAsync(scratch, 0, 3 * sizeof(uint32_t), st) != hipSuccess) rc = CRN2_EHIP;
+    if (hipMemsetAsync(scratch, 0, 4 * sizeof(uint32_t), st) != hipSuccess) rc = CRN2_EHIP;
     if (rc == CRN2_OK) {
-        uint32_t* list = scratch + 3;
+        uint32_t* list = scratch + 4;
         const uint32_t* px = static_cast<const uint32_t*>(rgba);
         ClusterOut* o = static_cast<ClusterOut*>(out);
         hipLaunchKernelGGL(dxt1_cluster_classes_kernel, dim3((nclusters + 255) / 256), dim3(256), 0, st, offsets,
                            nclusters, scratch, list);
         // largest clusters first: they take longest
         hipLaunchKernelGGL(dxt1_cluster_kernel<kMaxU>, dim3(nclusters), dim3(64), sizeof(LdsT<kMaxU>), st, px, offsets,
-                           list + 2 * (size_t)nclusters, scratch + 2, flags & 1, o, selectors);
+                           list + 3 * (size_t)nclusters, scratch + 3, flags & 1, o, selectors);
         hipLaunchKernelGGL(dxt1_cluster_kernel<2048>, dim3(nclusters), dim3(64), sizeof(LdsT<2048>), st, px, offsets,
+                           list + 2 * (size_t)nclusters, scratch + 2, flags & 1, o, selectors);
+        hipLaunchKernelGGL(dxt1_cluster_kernel<1024>, dim3(nclusters), dim3(64), sizeof(LdsT<1024>), st, px, offsets,
                            list + (size_t)nclusters, scratch + 1, flags & 1, o, selectors);
         hipLaunchKernelGGL(dxt1_cluster_kernel<512>, dim3(nclusters), dim3(64), sizeof(LdsT<512>), st, px, offsets,
                            list, scratch, flags & 1, o, selectors);
