@@ -114,7 +114,7 @@ __device__ inline uint32_t d_model0(uint32_t kind) { return kind <= 1 ? M_EP0 : 
 __device__ inline uint32_t s_model0(uint32_t kind) { return kind <= 1 ? M_SEL0 : M_SEL1; }
 __device__ inline bool two_channels(uint32_t kind) { return kind == 1 || kind == 2; }
 
-struct LdsTables {
+struct alignas(16) LdsTables {
     uint32_t lut[kLutSize];
     uint32_t limit[17];
     int32_t base[17];
@@ -124,7 +124,9 @@ struct LdsTables {
 
 __device__ inline void stage_tables(LdsTables& t, const DevModel& dm, uint32_t tid, uint32_t nthreads)
 {
-    for (uint32_t i = tid; i < kLutSize; i += nthreads) t.lut[i] = dm.lut[i];
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    for (uint32_t i = tid; i < kLutSize / 4; i += nthreads)
+        reinterpret_cast<u4*>(t.lut)[i] = reinterpret_cast<const u4*>(dm.lut)[i];
     if (tid < 17) {
         t.limit[tid] = dm.limit[tid];
         t.base[tid] = dm.base[tid];
@@ -155,7 +157,7 @@ __device__ inline uint32_t decode_top(const LdsTables& t, const uint16_t* sorted
 }
 
 // Code length only (the symbol value is not needed to find the next symbol).
-struct LenLut {
+struct alignas(16) LenLut {
     uint8_t len[kLutSize];
     uint32_t limit[17];
     int32_t base[17];
@@ -252,7 +254,9 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
         if (!kind_uses_dev(kind, m)) continue;
         const DevModel& dm = tx.models[m];
         LenLut& L = s_len[m - 1];
-        for (uint32_t i = tid; i < kLutSize; i += kTabThreads) L.len[i] = (uint8_t)(dm.lut[i] >> 16);
+        typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+        for (uint32_t i = tid; i < kLutSize / 16; i += kTabThreads)
+            reinterpret_cast<u4*>(L.len)[i] = reinterpret_cast<const u4*>(dm.len8)[i];
         if (tid < 17) {
             L.limit[tid] = dm.limit[tid];
             L.base[tid] = dm.base[tid];

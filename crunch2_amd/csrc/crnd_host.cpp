@@ -106,6 +106,7 @@ void to_device_model(const HostModel& m, DevModel& dm)
         const uint32_t idx = (uint32_t)(m.base[L] + (int32_t)(v >> (kLutBits - L)));
         if (idx >= m.num_syms) continue;
         dm.lut[v] = (uint32_t)m.sorted[idx] | (L << 16);
+        dm.len8[v] = (uint8_t)L;
     }
 }
 

@@ -16,11 +16,13 @@ constexpr uint32_t kMaxLevels = 16;
 // the canonical tables for the long-code path.
 struct DevModel {
     uint32_t lut[kLutSize];
+    uint8_t len8[kLutSize];   // lut[i] >> 16: the code length alone, for kernels that need no symbol
     uint32_t limit[17];   // limit[L] = 1 + left-aligned largest code of length L (0 if none)
     int32_t base[17];     // base[L] + code = index into sorted[]
     uint32_t num_syms;
-    uint32_t pad[2];
+    uint32_t pad;         // sizeof a multiple of 16: models in an array stay 16-byte aligned
 };
+static_assert(sizeof(DevModel) % 16 == 0, "DevModel size");
 
 struct Header {
     uint32_t header_size, header_crc, data_size, data_crc;
