@@ -472,37 +472,49 @@ __device__ bool try_average_solid(Block& s)
 // For 16 texels every table entry and every partial sum is below
 // 16 * 255^2 * 4 < 2^32, so 32-bit arithmetic (wrapping in the intermediate
 // W*p*p - WP2*p + WPP steps like the reference's 64-bit version) is exact.
-__device__ void comp_errors(const Block& s, int comp, uint32_t (*err)[256], uint32_t* brem)
+// Per-channel error of one selector class as a function of the class's value
+// p: W p^2 - WP2 p + WPP (exact in 32 bits: at most 16 texels x 255^2), the
+// reference's compute_endpoint_component_errors tables (crn_dxt1.cpp:369-418)
+// evaluated where they are read instead of stored.
+struct QErr {
+    uint32_t W[4], WP2[4], WPP[4];
+    __device__ inline uint32_t at(int sl, uint32_t p) const { return W[sl] * p * p - WP2[sl] * p + WPP[sl]; }
+};
+
+__device__ inline uint32_t expand_comp(int comp, uint32_t c)
 {
-    uint32_t W[4] = {0, 0, 0, 0}, WP2[4] = {0, 0, 0, 0}, WPP[4] = {0, 0, 0, 0};
+    return comp == 1 ? (((c << 2) | (c >> 4)) & 0xFF) : (((c << 3) | (c >> 2)) & 0xFF);
+}
+
+__device__ void comp_errors(const Block& s, int comp, QErr& q, uint32_t* brem)
+{
+    for (int k = 0; k < 4; k++) q.W[k] = q.WP2[k] = q.WPP[k] = 0;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
         if (k >= (int)s.nu) continue;
         const uint32_t p = (uint32_t)(comp == 0 ? s.r[k] : comp == 1 ? s.g[k] : s.b[k]);
         const uint32_t w = s.w[k];
         const uint32_t sl = (s.best.sel >> (2 * k)) & 3u;
-        for (uint32_t q = 0; q < 4; q++)
-            if (q == sl) {
-                W[q] += w;
-                WP2[q] += w * p * 2;
-                WPP[q] += w * p * p;
+        for (uint32_t c = 0; c < 4; c++)
+            if (c == sl) {
+                q.W[c] += w;
+                q.WP2[c] += w * p * 2;
+                q.WPP[c] += w * p * p;
             }
     }
     const uint32_t lim = comp == 1 ? 64 : 32;
     for (int sl = 0; sl < 2; sl++) {
-        uint32_t best = err[sl][0] = WPP[sl];
+        uint32_t best = q.WPP[sl];
         for (uint32_t c = 1; c < lim; c++) {
-            const uint32_t p = comp == 1 ? (((c << 2) | (c >> 4)) & 0xFF) : (((c << 3) | (c >> 2)) & 0xFF);
-            const uint32_t e = W[sl] * p * p - WP2[sl] * p + WPP[sl];
-            err[sl][c] = e;
+            const uint32_t e = q.at(sl, expand_comp(comp, c));
             if (e < best) best = e;
         }
         brem[sl] = best;
     }
     for (int sl = 2; sl < 4; sl++) {
-        uint32_t best = err[sl][0] = WPP[sl], d = W[sl] - WP2[sl], dd = W[sl] << 1, e = WPP[sl] + d;
-        for (uint32_t p = 1; p < 256; p++, d += dd, e += d) {
-            err[sl][p] = e;
+        uint32_t best = q.WPP[sl];
+        for (uint32_t p = 1; p < 256; p++) {
+            const uint32_t e = q.at(sl, p);
             if (e < best) best = e;
         }
         brem[sl] = best;
@@ -510,7 +522,7 @@ __device__ void comp_errors(const Block& s, int comp, uint32_t (*err)[256], uint
     for (int sl = 3; sl; sl--) brem[sl - 1] += brem[sl];
 }
 
-__device__ void optimize_comps(Block& s, uint32_t (*err)[256])
+__device__ void optimize_comps(Block& s)
 {
     compute_selectors(s);
     if (s.best.alpha || !s.best.err) return;
@@ -518,39 +530,38 @@ __device__ void optimize_comps(Block& s, uint32_t (*err)[256])
     unpack565(s.best.lo, 1, slr, slg, slb);
     unpack565(s.best.hi, 1, shr, shg, shb);
     uint32_t brem[4];
+    QErr q;
     for (int comp = 0; comp < 3; comp++) {
         uint32_t p0 = (uint32_t)(comp == 0 ? slr : comp == 1 ? slg : slb);
         uint32_t p1 = (uint32_t)(comp == 0 ? shr : comp == 1 ? shg : shb);
         int low[3], high[3];
         unpack565(s.best.lo, 0, low[0], low[1], low[2]);
         unpack565(s.best.hi, 0, high[0], high[1], high[2]);
-        comp_errors(s, comp, err, brem);
-        const int lc = comp == 0 ? low[0] : comp == 1 ? low[1] : low[2];
-        const int hcv = comp == 0 ? high[0] : comp == 1 ? high[1] : high[2];
-        uint32_t best = err[0][lc] + err[1][hcv] + err[2][(p0 * 2 + p1) / 3] + err[3][(p0 + p1 * 2) / 3];
+        comp_errors(s, comp, q, brem);
+        uint32_t best = q.at(0, p0) + q.at(1, p1) + q.at(2, (p0 * 2 + p1) / 3) + q.at(3, (p0 + p1 * 2) / 3);
         if (brem[0] >= best) continue;
         const uint32_t lim = comp == 1 ? 64 : 32;
         for (uint32_t c0 = 0; c0 < lim; c0++) {
-            uint32_t e0 = err[0][c0];
+            p0 = expand_comp(comp, c0);
+            uint32_t e0 = q.at(0, p0);
             if (e0 + brem[1] >= best) continue;
             if (comp == 0) low[0] = (int)c0; else if (comp == 1) low[1] = (int)c0; else low[2] = (int)c0;
             const uint32_t plow = pack565(low[0], low[1], low[2], 0);
-            p0 = comp == 1 ? (((c0 << 2) | (c0 >> 4)) & 0xFF) : (((c0 << 3) | (c0 >> 2)) & 0xFF);
             for (uint32_t c1 = 0; c1 < lim; c1++) {
-                uint32_t e = e0 + err[1][c1];
+                p1 = expand_comp(comp, c1);
+                uint32_t e = e0 + q.at(1, p1);
                 if (e + brem[2] >= best) continue;
-                p1 = comp == 1 ? (((c1 << 2) | (c1 >> 4)) & 0xFF) : (((c1 << 3) | (c1 >> 2)) & 0xFF);
-                e += err[2][(p0 * 2 + p1) / 3];
+                e += q.at(2, (p0 * 2 + p1) / 3);
                 if (e + brem[3] >= best) continue;
-                e += err[3][(p0 + p1 * 2) / 3];
+                e += q.at(3, (p0 + p1 * 2) / 3);
                 if (e >= best) continue;
                 if (comp == 0) high[0] = (int)c1; else if (comp == 1) high[1] = (int)c1; else high[2] = (int)c1;
                 if (!evaluate(s, plow, pack565(high[0], high[1], high[2], 0), 0)) continue;
                 if (!s.best.err) return;
                 compute_selectors(s);
-                comp_errors(s, comp, err, brem);
-                best = err[0][c0] + err[1][c1] + err[2][(p0 * 2 + p1) / 3] + err[3][(p0 + p1 * 2) / 3];
-                e0 = err[0][c0];
+                comp_errors(s, comp, q, brem);
+                best = q.at(0, p0) + q.at(1, p1) + q.at(2, (p0 * 2 + p1) / 3) + q.at(3, (p0 + p1 * 2) / 3);
+                e0 = q.at(0, p0);
                 if (e0 + brem[1] >= best) break;
             }
         }
@@ -683,7 +694,7 @@ __device__ bool sel_all_equal(const Sol& b)
     return true;
 }
 
-__device__ void optimize_endpoints(Block& s, float* lo, float* hi, uint32_t (*errtab)[256])
+__device__ void optimize_endpoints(Block& s, float* lo, float* hi)
 {
     float olo[3] = {lo[0], lo[1], lo[2]}, ohi[3] = {hi[0], hi[1], hi[2]};
     const int16_t* tab = k_uber_probe;
@@ -732,7 +743,7 @@ __device__ void optimize_endpoints(Block& s, float* lo, float* hi, uint32_t (*er
     if (s.best.err) {
         bool solid = false;
         if (sel_all_equal(s.best)) solid = try_average_solid(s);
-        if (!solid && s.quality == 4) optimize_comps(s, errtab);
+        if (!solid && s.quality == 4) optimize_comps(s);
     }
     if (s.quality == 4 && s.best.err) try_combinatorial(s);
 }
@@ -857,7 +868,7 @@ __device__ bool ray_unit_box(float* coord, const float* org, const float* dir)
     return true;
 }
 
-__device__ void handle_multicolor(Block& s, uint32_t (*errtab)[256])
+__device__ void handle_multicolor(Block& s)
 {
     uint32_t npass = 1;
     float pw[3] = {1.0f, 1.0f, 1.0f};
@@ -943,7 +954,7 @@ __device__ void handle_multicolor(Block& s, uint32_t (*errtab)[256])
         const float nd[3] = {-s.axis[0], -s.axis[1], -s.axis[2]};
         if (ray_unit_box(c3, hi, nd)) { hi[0] = c3[0]; hi[1] = c3[1]; hi[2] = c3[2]; }
     }
-    optimize_endpoints(s, lo, hi, errtab);
+    optimize_endpoints(s, lo, hi);
 }
 
 struct Dxt1Out {
@@ -954,7 +965,7 @@ struct Dxt1Out {
     uint32_t pad;
 };
 
-__device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags, uint32_t (*errtab)[256])
+__device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags)
 {
     Block s;
     s.quality = quality;
@@ -1010,7 +1021,7 @@ __device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags, uint
             evaluate(s, (g_om5_3[r][0] << 11) | (g_om6_3[g][0] << 5) | g_om5_3[b][0],
                      (g_om5_3[r][1] << 11) | (g_om6_3[g][1] << 5) | g_om5_3[b][1], 0);
     } else {
-        handle_multicolor(s, errtab);
+        handle_multicolor(s);
     }
     compute_selectors(s);
     const bool inv = s.best.alpha ? (s.best.lo > s.best.hi) : (s.best.lo < s.best.hi);
@@ -1048,8 +1059,7 @@ __global__ void __launch_bounds__(256) dxt1_blocks_kernel(const uint32_t* __rest
         p[4 * q + 2] = v.z;
         p[4 * q + 3] = v.w;
     }
-    uint32_t errtab[4][256];   // private per lane
-    out[b] = encode_block(p, quality, flags, errtab);
+    out[b] = encode_block(p, quality, flags);
 }
 
 }  // namespace

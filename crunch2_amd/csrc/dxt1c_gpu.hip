@@ -29,9 +29,9 @@
 namespace {
 
 // Largest cluster: 4096 unique colours.  The kernel comes in four LDS sizes
-// (512, 1024, 2048 and 4096 unique colours: 26, 36, 57 and 97 KiB) so that
-// small clusters, the common case, run six, four or two waves per CU instead
-// of one.
+// (512, 1024, 2048 and 4096 unique colours: 18, 28, 49 and 89 KiB) so that
+// small clusters, the common case, run eight, five or three waves per CU
+// instead of one.
 constexpr uint32_t kMaxU = 4096;
 
 __device__ __constant__ int16_t c_uber_probe[15] = {0, 1, 2, 3, 5, 7, 9, 10, 13, 15, 19, 27, 43, 59, 91};
@@ -53,7 +53,6 @@ struct LdsT {
     uint16_t hval[kCapH];
     unsigned long long pl[3][64][3];   // error planes: count, sum, sum of squares per quantised value
     uint64_t rlo[32], rhi[32], glo[64], ghi[64], blo[32], bhi[32];
-    uint64_t err[4][256];
 };
 
 struct Best {
@@ -580,8 +579,21 @@ __device__ bool try_average_solid(St& s, Lds& L)
 }
 
 // compute_endpoint_component_errors (crn_dxt1.cpp:369-418)
+// Per-channel error of one selector class as a function of the class's value
+// p: W p^2 - WP2 p + WPP (the reference's compute_endpoint_component_errors
+// tables, crn_dxt1.cpp:369-418, evaluated where they are read).
+struct QErr {
+    uint64_t W[4], WP2[4], WPP[4];
+    __device__ inline uint64_t at(int sl, uint64_t p) const { return W[sl] * p * p - WP2[sl] * p + WPP[sl]; }
+};
+
+__device__ inline uint32_t expand_comp(int comp, uint32_t c)
+{
+    return comp == 1 ? (((c << 2) | (c >> 4)) & 0xFF) : (((c << 3) | (c >> 2)) & 0xFF);
+}
+
 template <class Lds>
-__device__ void comp_errors(St& s, Lds& L, int comp, uint64_t* brem)
+__device__ void comp_errors(St& s, Lds& L, int comp, QErr& q, uint64_t* brem)
 {
     uint64_t W[4] = {0, 0, 0, 0}, WP2[4] = {0, 0, 0, 0}, WPP[4] = {0, 0, 0, 0};
     for (uint32_t i = lane_id(); i < s.nu; i += 64) {
@@ -596,24 +608,16 @@ __device__ void comp_errors(St& s, Lds& L, int comp, uint64_t* brem)
             }
     }
     for (int k = 0; k < 4; k++) {
-        W[k] = wave_sum(W[k]);
-        WP2[k] = wave_sum(WP2[k]);
-        WPP[k] = wave_sum(WPP[k]);
+        q.W[k] = wave_sum(W[k]);
+        q.WP2[k] = wave_sum(WP2[k]);
+        q.WPP[k] = wave_sum(WPP[k]);
     }
     const uint32_t lane = lane_id();
     const uint32_t lim = comp == 1 ? 64 : 32;
-    for (int sl = 0; sl < 2; sl++)
-        for (uint32_t c = lane; c < lim; c += 64) {
-            const uint64_t p = comp == 1 ? ((c << 2) | (c >> 4)) & 0xFF : ((c << 3) | (c >> 2)) & 0xFF;
-            L.err[sl][c] = c ? W[sl] * p * p - WP2[sl] * p + WPP[sl] : WPP[sl];
-        }
-    for (int sl = 2; sl < 4; sl++)
-        for (uint32_t p = lane; p < 256; p += 64) L.err[sl][p] = W[sl] * p * p - WP2[sl] * p + WPP[sl];
-    __builtin_amdgcn_wave_barrier();
     for (int sl = 0; sl < 4; sl++) {
         const uint32_t n = sl < 2 ? lim : 256;
         uint64_t m = ~0ull;
-        for (uint32_t c = lane; c < n; c += 64) m = min(m, L.err[sl][c]);
+        for (uint32_t c = lane; c < n; c += 64) m = min(m, q.at(sl, sl < 2 ? expand_comp(comp, c) : c));
         for (int off = 32; off > 0; off >>= 1) {
             const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)m, off);
             const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(m >> 32), off);
@@ -634,37 +638,37 @@ __device__ void optimize_comps(St& s, Lds& L)
     unpack565(s.best.lo, 1, slo[0], slo[1], slo[2]);
     unpack565(s.best.hi, 1, shi[0], shi[1], shi[2]);
     uint64_t brem[4];
+    QErr q;
     for (int comp = 0; comp < 3; comp++) {
         uint32_t p0 = (uint32_t)slo[comp], p1 = (uint32_t)shi[comp];
         int low[3], high[3];
         unpack565(s.best.lo, 0, low[0], low[1], low[2]);
         unpack565(s.best.hi, 0, high[0], high[1], high[2]);
-        comp_errors(s, L, comp, brem);
-        uint64_t best = L.err[0][low[comp]] + L.err[1][high[comp]] + L.err[2][(p0 * 2 + p1) / 3] +
-                        L.err[3][(p0 + p1 * 2) / 3];
+        comp_errors(s, L, comp, q, brem);
+        uint64_t best = q.at(0, p0) + q.at(1, p1) + q.at(2, (p0 * 2 + p1) / 3) + q.at(3, (p0 + p1 * 2) / 3);
         if (brem[0] >= best) continue;
         const uint32_t lim = comp == 1 ? 64 : 32;
         for (uint32_t c0 = 0; c0 < lim; c0++) {
-            uint64_t e0 = L.err[0][c0];
+            p0 = expand_comp(comp, c0);
+            uint64_t e0 = q.at(0, p0);
             if (e0 + brem[1] >= best) continue;
             low[comp] = (int)c0;
             const uint32_t plow = pack565(low[0], low[1], low[2], 0);
-            p0 = comp == 1 ? (((c0 << 2) | (c0 >> 4)) & 0xFF) : (((c0 << 3) | (c0 >> 2)) & 0xFF);
             for (uint32_t c1 = 0; c1 < lim; c1++) {
-                uint64_t e = e0 + L.err[1][c1];
+                p1 = expand_comp(comp, c1);
+                uint64_t e = e0 + q.at(1, p1);
                 if (e + brem[2] >= best) continue;
-                p1 = comp == 1 ? (((c1 << 2) | (c1 >> 4)) & 0xFF) : (((c1 << 3) | (c1 >> 2)) & 0xFF);
-                e += L.err[2][(p0 * 2 + p1) / 3];
+                e += q.at(2, (p0 * 2 + p1) / 3);
                 if (e + brem[3] >= best) continue;
-                e += L.err[3][(p0 + p1 * 2) / 3];
+                e += q.at(3, (p0 + p1 * 2) / 3);
                 if (e >= best) continue;
                 high[comp] = (int)c1;
                 if (!evaluate(s, L, plow, pack565(high[0], high[1], high[2], 0), 0)) continue;
                 if (!s.best.err) return;
                 compute_selectors(s, L);
-                comp_errors(s, L, comp, brem);
-                best = L.err[0][c0] + L.err[1][c1] + L.err[2][(p0 * 2 + p1) / 3] + L.err[3][(p0 + p1 * 2) / 3];
-                e0 = L.err[0][c0];
+                comp_errors(s, L, comp, q, brem);
+                best = q.at(0, p0) + q.at(1, p1) + q.at(2, (p0 * 2 + p1) / 3) + q.at(3, (p0 + p1 * 2) / 3);
+                e0 = q.at(0, p0);
                 if (e0 + brem[1] >= best) break;
             }
         }
