@@ -23,6 +23,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "crn2_internal.h"
 
@@ -1095,6 +1096,40 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
 
 }  // namespace
 
+namespace {
+// Helper streams and events for crn2_dxt1_clusters_device, one set per device,
+// kept until the process ends.  Callers hold g_side_mu from the fork to the
+// join, so concurrent calls queue their work on the shared streams in turn.
+std::mutex g_side_mu;
+
+int side_streams(hipStream_t* side, hipEvent_t* fork, hipEvent_t* join)
+{
+    struct Set {
+        bool ready = false;
+        hipStream_t s[3];
+        hipEvent_t fork, join[3];
+    };
+    static Set sets[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return CRN2_EHIP;
+    Set& set = sets[dev];
+    if (!set.ready) {
+        for (int i = 0; i < 3; i++)
+            if (hipStreamCreateWithFlags(&set.s[i], hipStreamNonBlocking) != hipSuccess) return CRN2_EHIP;
+        if (hipEventCreateWithFlags(&set.fork, hipEventDisableTiming) != hipSuccess) return CRN2_EHIP;
+        for (int i = 0; i < 3; i++)
+            if (hipEventCreateWithFlags(&set.join[i], hipEventDisableTiming) != hipSuccess) return CRN2_EHIP;
+        set.ready = true;
+    }
+    for (int i = 0; i < 3; i++) {
+        side[i] = set.s[i];
+        join[i] = set.join[i];
+    }
+    *fork = set.fork;
+    return CRN2_OK;
+}
+}  // namespace
+
 extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags,
                                          void* out, uint8_t* selectors, void* stream)
 {
@@ -1148,16 +1183,31 @@ extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offse
         ClusterOut* o = static_cast<ClusterOut*>(out);
         hipLaunchKernelGGL(dxt1_cluster_classes_kernel, dim3((nclusters + 255) / 256), dim3(256), 0, st, offsets,
                            nclusters, scratch, list);
-        // largest clusters first: they take longest
-        hipLaunchKernelGGL(dxt1_cluster_kernel<kMaxU>, dim3(nclusters), dim3(64), sizeof(LdsT<kMaxU>), st, px, offsets,
-                           list + 3 * (size_t)nclusters, scratch + 3, flags & 1, o, selectors);
-        hipLaunchKernelGGL(dxt1_cluster_kernel<2048>, dim3(nclusters), dim3(64), sizeof(LdsT<2048>), st, px, offsets,
-                           list + 2 * (size_t)nclusters, scratch + 2, flags & 1, o, selectors);
-        hipLaunchKernelGGL(dxt1_cluster_kernel<1024>, dim3(nclusters), dim3(64), sizeof(LdsT<1024>), st, px, offsets,
-                           list + (size_t)nclusters, scratch + 1, flags & 1, o, selectors);
-        hipLaunchKernelGGL(dxt1_cluster_kernel<512>, dim3(nclusters), dim3(64), sizeof(LdsT<512>), st, px, offsets,
-                           list, scratch, flags & 1, o, selectors);
-        if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
+        // The four classes run side by side: the large clusters are few and
+        // long, and alone would leave most CUs idle.  Three helper streams
+        // (default priority, created once per device) fork from `st` and join
+        // back into it.
+        hipStream_t side[3];
+        hipEvent_t fork, join[3];
+        std::lock_guard<std::mutex> lk(g_side_mu);
+        rc = side_streams(side, &fork, join);
+        if (rc == CRN2_OK && hipEventRecord(fork, st) != hipSuccess) rc = CRN2_EHIP;
+        for (int i = 0; rc == CRN2_OK && i < 3; i++)
+            if (hipStreamWaitEvent(side[i], fork, 0) != hipSuccess) rc = CRN2_EHIP;
+        if (rc == CRN2_OK) {
+            hipLaunchKernelGGL(dxt1_cluster_kernel<kMaxU>, dim3(nclusters), dim3(64), sizeof(LdsT<kMaxU>), side[0], px,
+                               offsets, list + 3 * (size_t)nclusters, scratch + 3, flags & 1, o, selectors);
+            hipLaunchKernelGGL(dxt1_cluster_kernel<2048>, dim3(nclusters), dim3(64), sizeof(LdsT<2048>), side[1], px,
+                               offsets, list + 2 * (size_t)nclusters, scratch + 2, flags & 1, o, selectors);
+            hipLaunchKernelGGL(dxt1_cluster_kernel<1024>, dim3(nclusters), dim3(64), sizeof(LdsT<1024>), side[2], px,
+                               offsets, list + (size_t)nclusters, scratch + 1, flags & 1, o, selectors);
+            hipLaunchKernelGGL(dxt1_cluster_kernel<512>, dim3(nclusters), dim3(64), sizeof(LdsT<512>), st, px, offsets,
+                               list, scratch, flags & 1, o, selectors);
+            if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
+            for (int i = 0; i < 3; i++)
+                if (hipEventRecord(join[i], side[i]) != hipSuccess || hipStreamWaitEvent(st, join[i], 0) != hipSuccess)
+                    rc = CRN2_EHIP;
+        }
     }
     (void)hipFreeAsync(scratch, st);
     return rc;
