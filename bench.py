@@ -188,7 +188,7 @@ def main():
                     help="side of the RGB texture of the dxt_hc leg; 0 skips it")
     ap.add_argument("--hc-cpu-size", type=int, default=512,
                     help="side of the texture the oracle's dxt_hc composition is timed on; 0 skips it")
-    ap.add_argument("--fallback-frac", type=float, default=0.10,
+    ap.add_argument("--fallback-frac", type=float, default=0.08,
                     help="fraction of blocks (lowest confidence first) that also run the plain search in the encode leg")
     args = ap.parse_args()
     if args.gpus != 1:
