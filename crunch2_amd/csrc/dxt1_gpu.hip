@@ -1044,7 +1044,7 @@ __device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags)
     return o;
 }
 
-__global__ void __launch_bounds__(256) dxt1_blocks_kernel(const uint32_t* __restrict__ px, uint32_t nblocks, int quality,
+__global__ void __launch_bounds__(256, 2) dxt1_blocks_kernel(const uint32_t* __restrict__ px, uint32_t nblocks, int quality,
                                                          int flags, Dxt1Out* __restrict__ out)
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
