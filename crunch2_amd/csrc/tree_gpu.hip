@@ -115,60 +115,66 @@ __device__ inline void load_vec(const float* __restrict__ p, float (&v)[D])
     for (int k = 0; k < D; k++) v[k] = p[k];
 }
 
-// Shared state of a split workgroup.
-template <int D>
+// Shared state of a split workgroup of GR lane groups (256 lanes each; a
+// group handles one 4096-position chunk at a time).
+template <int D, int GR>
 struct TcShared {
-    static constexpr int kRedF = (CovTable<D>::kEntries < 34 ? CovTable<D>::kEntries : 34) > 2 * D
-                                     ? (CovTable<D>::kEntries < 34 ? CovTable<D>::kEntries : 34)
+    static constexpr int kCovCap = GR == 1 ? 34 : 12;     // covariance entries reduced at once
+    static constexpr int kRedF = (CovTable<D>::kEntries < kCovCap ? CovTable<D>::kEntries : kCovCap) > 2 * D
+                                     ? (CovTable<D>::kEntries < kCovCap ? CovTable<D>::kEntries : kCovCap)
                                      : 2 * D;
-    float red[kRedF][kLanes];
-    double redd[2][kLanes];
+    float red[kRedF][kLanes * GR];
+    double redd[2][kLanes * GR];
     float totf[kRedF > 2 * D ? kRedF : 2 * D];
     double totd[2];
     unsigned long long totu[4];
     float c[D], lc[D], rc[D], cl[D], cr[D], axis[D], seedf[D], seedo[D];
     float covar[D][D];
-    float best_d[kLanes / 64];
-    uint32_t best_p[kLanes / 64];
-    uint32_t wl[kLanes / 64], wr[kLanes / 64];
+    float best_d[kLanes * GR / 64];
+    uint32_t best_p[kLanes * GR / 64];
+    uint32_t wl[kLanes * GR / 64], wr[kLanes * GR / 64];
     uint32_t pos;
     int go;
 };
 
-// Tree-combine the K values each thread left in red[k][tid] and add the
-// chunk's sums to totf / totd (threads k < K / k < KD).
-template <int D>
-__device__ inline void tree_combine(TcShared<D>& s, int K, int KD)
+// Tree-combine, in every lane group, the K values its lanes left in
+// red[k][group * 256 + lane] (and KD doubles), then add the groups' chunk sums
+// to totf / totd in group order (= chunk order); groups >= nv had no chunk.
+template <int D, int GR>
+__device__ inline void tree_combine(TcShared<D, GR>& s, int K, int KD, uint32_t nv)
 {
     const uint32_t tid = threadIdx.x;
     for (uint32_t sh = kLanes / 2; sh; sh >>= 1) {
         __syncthreads();
-        for (uint32_t idx = tid; idx < sh * (uint32_t)K; idx += kLanes) {
-            const uint32_t k = idx / sh, t = idx % sh;
+        for (uint32_t idx = tid; idx < GR * sh * (uint32_t)K; idx += kLanes * GR) {
+            const uint32_t g = idx / (sh * K), r = idx % (sh * K), k = r / sh, t = r % sh + g * kLanes;
             s.red[k][t] = s.red[k][t] + s.red[k][t + sh];
         }
-        for (uint32_t idx = tid; idx < sh * (uint32_t)KD; idx += kLanes) {
-            const uint32_t k = idx / sh, t = idx % sh;
+        for (uint32_t idx = tid; idx < GR * sh * (uint32_t)KD; idx += kLanes * GR) {
+            const uint32_t g = idx / (sh * KD), r = idx % (sh * KD), k = r / sh, t = r % sh + g * kLanes;
             s.redd[k][t] = s.redd[k][t] + s.redd[k][t + sh];
         }
     }
     __syncthreads();
-    if (tid < (uint32_t)K) s.totf[tid] = s.totf[tid] + s.red[tid][0];
-    if (tid < (uint32_t)KD) s.totd[tid] = s.totd[tid] + s.redd[tid][0];
+    if (tid < (uint32_t)K)
+        for (uint32_t g = 0; g < nv; g++) s.totf[tid] = s.totf[tid] + s.red[tid][g * kLanes];
+    if (tid < (uint32_t)KD)
+        for (uint32_t g = 0; g < nv; g++) s.totd[tid] = s.totd[tid] + s.redd[tid][g * kLanes];
     __syncthreads();
 }
 
 // First position (lowest on ties) of the largest sqdist(vec, ref) in [b, e).
-template <int D>
-__device__ inline uint32_t furthest(const TcData& a, TcShared<D>& s, uint32_t b, uint32_t e, const float* ref)
+template <int D, int GR>
+__device__ inline uint32_t furthest(const TcData& a, TcShared<D, GR>& s, uint32_t b, uint32_t e, const float* ref)
 {
+    constexpr uint32_t kT = kLanes * GR;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     float r[D];
 #pragma unroll
     for (int k = 0; k < D; k++) r[k] = ref[k];
     float bd = -1.0f;
     uint32_t bp = 0xFFFFFFFFu;
-    for (uint32_t i = b + tid; i < e; i += kLanes) {
+    for (uint32_t i = b + tid; i < e; i += kT) {
         float v[D];
         load_vec<D>(a.vec + (size_t)a.perm[i] * D, v);
         const float dd = sqdist<D>(v, r);
@@ -193,7 +199,7 @@ __device__ inline uint32_t furthest(const TcData& a, TcShared<D>& s, uint32_t b,
     if (tid == 0) {
         float d0 = s.best_d[0];
         uint32_t p0 = s.best_p[0];
-        for (uint32_t w = 1; w < kLanes / 64; w++)
+        for (uint32_t w = 1; w < kT / 64; w++)
             if (s.best_d[w] > d0 || (s.best_d[w] == d0 && s.best_p[w] < p0)) {
                 d0 = s.best_d[w];
                 p0 = s.best_p[w];
@@ -204,24 +210,30 @@ __device__ inline uint32_t furthest(const TcData& a, TcShared<D>& s, uint32_t b,
     return s.pos;
 }
 
-template <int D>
-__global__ void __launch_bounds__(kLanes) tc_split_kernel(TcData a, const TcJob* __restrict__ jobs,
-                                                          TcOut* __restrict__ outs)
+// One node split per workgroup of GR lane groups: group g takes chunks
+// it * GR + g, it = 0, 1, ...; within a chunk lane l sums positions
+// chunk + l + 256 j, exactly as a single group would.
+template <int D, int GR>
+__global__ void __launch_bounds__(kLanes * GR) tc_split_kernel(TcData a, const TcJob* __restrict__ jobs,
+                                                               TcOut* __restrict__ outs)
 {
-    __shared__ TcShared<D> s;
+    constexpr uint32_t kT = kLanes * GR;
+    __shared__ TcShared<D, GR> s;
     constexpr CovTable<D> ct;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t gl = tid & (kLanes - 1), grp = tid / kLanes;   // lane within its group, group
     const TcJob& job = jobs[blockIdx.x];
     const uint32_t b = job.begin, e = job.end;
+    const uint32_t nchunks = (e - b + kChunk - 1) / kChunk, iters = (nchunks + GR - 1) / GR;
     if (tid < (uint32_t)D) s.c[tid] = job.c[tid];
     __syncthreads();
 
     // seeds
     {
-        const uint32_t pf = furthest<D>(a, s, b, e, s.c);
+        const uint32_t pf = furthest<D, GR>(a, s, b, e, s.c);
         if (tid < (uint32_t)D) s.seedf[tid] = a.vec[(size_t)a.perm[pf] * D + tid];
         __syncthreads();
-        const uint32_t po = furthest<D>(a, s, b, e, s.seedf);
+        const uint32_t po = furthest<D, GR>(a, s, b, e, s.seedf);
         if (tid < (uint32_t)D) {
             s.seedo[tid] = a.vec[(size_t)a.perm[po] * D + tid];
             s.lc[tid] = (s.seedf[tid] + s.c[tid]) * .5f;
@@ -237,19 +249,19 @@ __global__ void __launch_bounds__(kLanes) tc_split_kernel(TcData a, const TcJob*
     if (b + 2 < e) {
         // weighted covariance, upper triangle, in groups of at most kRedF entries
         constexpr int NE = CovTable<D>::kEntries;
-        constexpr int G = TcShared<D>::kRedF;
+        constexpr int G = TcShared<D, GR>::kRedF;
         constexpr int NG = (NE + G - 1) / G;
 #pragma unroll
         for (int g = 0; g < NG; g++) {
             const int g0 = g * G;
             const int gn = NE - g0 < G ? NE - g0 : G;
             if (tid < (uint32_t)G) s.totf[tid] = 0.0f;
-            for (uint32_t c0 = b; c0 < e; c0 += kChunk) {
-                const uint32_t c1 = min(e, c0 + kChunk);
+            for (uint32_t it = 0; it < iters; it++) {
+                const uint32_t c0 = b + (it * GR + grp) * kChunk, c1 = min(e, c0 + kChunk);
                 float acc[G];
 #pragma unroll
                 for (int q = 0; q < G; q++) acc[q] = 0.0f;
-                for (uint32_t i = c0 + tid; i < c1; i += kLanes) {
+                for (uint32_t i = c0 + gl; i < c1; i += kLanes) {
                     const uint32_t vi = a.perm[i];
                     float v[D], w[D];
                     load_vec<D>(a.vec + (size_t)vi * D, v);
@@ -266,7 +278,7 @@ __global__ void __launch_bounds__(kLanes) tc_split_kernel(TcData a, const TcJob*
 #pragma unroll
                 for (int q = 0; q < G; q++)
                     if (q < gn) s.red[q][tid] = acc[q];
-                tree_combine<D>(s, gn, 0);
+                tree_combine<D, GR>(s, gn, 0, min(GR, nchunks - it * GR));
             }
             if (tid == 0) {
                 const float divider = (float)job.w;
@@ -310,13 +322,13 @@ __global__ void __launch_bounds__(kLanes) tc_split_kernel(TcData a, const TcJob*
         float ax[D];
 #pragma unroll
         for (int k = 0; k < D; k++) ax[k] = s.axis[k];
-        for (uint32_t c0 = b; c0 < e; c0 += kChunk) {
-            const uint32_t c1 = min(e, c0 + kChunk);
+        for (uint32_t it = 0; it < iters; it++) {
+            const uint32_t c0 = b + (it * GR + grp) * kChunk, c1 = min(e, c0 + kChunk);
             float fl[D], fr[D];
             double dl = 0.0, dr = 0.0;
 #pragma unroll
             for (int k = 0; k < D; k++) fl[k] = fr[k] = 0.0f;
-            for (uint32_t i = c0 + tid; i < c1; i += kLanes) {
+            for (uint32_t i = c0 + gl; i < c1; i += kLanes) {
                 const uint32_t vi = a.perm[i];
                 float v[D], wv[D];
                 load_vec<D>(a.vec + (size_t)vi * D, v);
@@ -341,7 +353,7 @@ __global__ void __launch_bounds__(kLanes) tc_split_kernel(TcData a, const TcJob*
             }
             s.redd[0][tid] = dl;
             s.redd[1][tid] = dr;
-            tree_combine<D>(s, 2 * D, 2);
+            tree_combine<D, GR>(s, 2 * D, 2, min(GR, nchunks - it * GR));
         }
         if (tid == 0 && s.totd[0] > 0.0f && s.totd[1] > 0.0f) {
             const float sl = (float)(1.0f / s.totd[0]), sr = (float)(1.0f / s.totd[1]);
@@ -371,13 +383,13 @@ __global__ void __launch_bounds__(kLanes) tc_split_kernel(TcData a, const TcJob*
             cr[k] = s.cr[k];
         }
         unsigned long long ul = 0, ur = 0, nl = 0, nr = 0;
-        for (uint32_t c0 = b; c0 < e; c0 += kChunk) {
-            const uint32_t c1 = min(e, c0 + kChunk);
+        for (uint32_t it = 0; it < iters; it++) {
+            const uint32_t c0 = b + (it * GR + grp) * kChunk, c1 = min(e, c0 + kChunk);
             float fl[D], fr[D];
             double dl = 0.0, dr = 0.0;
 #pragma unroll
             for (int k = 0; k < D; k++) fl[k] = fr[k] = 0.0f;
-            for (uint32_t i = c0 + tid; i < c1; i += kLanes) {
+            for (uint32_t i = c0 + gl; i < c1; i += kLanes) {
                 const uint32_t vi = a.perm[i];
                 float v[D], wv[D];
                 load_vec<D>(a.vec + (size_t)vi * D, v);
@@ -405,7 +417,7 @@ __global__ void __launch_bounds__(kLanes) tc_split_kernel(TcData a, const TcJob*
             }
             s.redd[0][tid] = dl;
             s.redd[1][tid] = dr;
-            tree_combine<D>(s, 2 * D, 2);
+            tree_combine<D, GR>(s, 2 * D, 2, min(GR, nchunks - it * GR));
         }
         for (int off = 32; off; off >>= 1) {
             ul += (unsigned long long)__shfl_xor((long long)ul, off);
@@ -463,7 +475,7 @@ __global__ void __launch_bounds__(kLanes) tc_split_kernel(TcData a, const TcJob*
     }
     uint32_t run_l = 0, run_r = 0;
     const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
-    for (uint32_t base = b; base < e; base += kLanes) {
+    for (uint32_t base = b; base < e; base += kT) {
         const uint32_t pos = base + tid;
         const bool valid = pos < e;
         uint32_t vi = 0;
@@ -481,7 +493,7 @@ __global__ void __launch_bounds__(kLanes) tc_split_kernel(TcData a, const TcJob*
         }
         __syncthreads();
         uint32_t ol = run_l, or_ = run_r, tl = 0, tr = 0;
-        for (uint32_t w = 0; w < kLanes / 64; w++) {
+        for (uint32_t w = 0; w < kT / 64; w++) {
             if (w < wave) {
                 ol += s.wl[w];
                 or_ += s.wr[w];
@@ -499,7 +511,7 @@ __global__ void __launch_bounds__(kLanes) tc_split_kernel(TcData a, const TcJob*
     }
     __threadfence_block();
     __syncthreads();
-    for (uint32_t pos = b + tid; pos < e; pos += kLanes)
+    for (uint32_t pos = b + tid; pos < e; pos += kT)
         a.perm[pos] = __hip_atomic_load(&a.tmp[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
     if (tid == 0) {
@@ -536,7 +548,7 @@ __global__ void __launch_bounds__(256) tc_prep_kernel(const float* __restrict__ 
 template <int D>
 __global__ void __launch_bounds__(kLanes) tc_root_kernel(TcData a, uint32_t n, TcRoot* __restrict__ out)
 {
-    __shared__ TcShared<D> s;
+    __shared__ TcShared<D, 1> s;
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     if (tid < (uint32_t)D) s.totf[tid] = 0.0f;
     if (tid == 0) {
@@ -561,7 +573,7 @@ __global__ void __launch_bounds__(kLanes) tc_root_kernel(TcData a, uint32_t n, T
 #pragma unroll
         for (int k = 0; k < D; k++) s.red[k][tid] = f[k];
         s.redd[0][tid] = d;
-        tree_combine<D>(s, D, 1);
+        tree_combine<D, 1>(s, D, 1, 1);
     }
     for (int off = 32; off; off >>= 1) uw += (unsigned long long)__shfl_xor((long long)uw, off);
     if (lane == 0) atomicAdd(&s.totu[0], uw);
@@ -601,6 +613,18 @@ struct Before {
         return a > b;
     }
 };
+
+// Dims whose split kernel also comes in a four-group (1024-thread) variant
+// for large nodes: 2 and 6 (16 floats per lane need too much LDS for it).
+template <int D>
+constexpr bool kWide = D <= 6;
+
+template <int D>
+void launch_wide(const TcData& a, const TcJob* jobs, TcOut* outs, uint32_t n, hipStream_t st)
+{
+    if constexpr (D <= 6)
+        hipLaunchKernelGGL((tc_split_kernel<D, 4>), dim3(n), dim3(kLanes * 4), 0, st, a, jobs, outs);
+}
 
 template <int D>
 int tree_codebook(const float* vectors, const uint32_t* weights, uint32_t n, uint32_t max_splits, float* codebook,
@@ -681,14 +705,35 @@ int tree_codebook(const float* vectors, const uint32_t* weights, uint32_t n, uin
                     jobs.push_back(j);
                     job_node.push_back(order[i]);
                 }
+                // nodes of more than two chunks first, split by workgroups of
+                // four lane groups (same sums, same order); the rest one group each
+                {
+                    std::vector<size_t> idx(jobs.size());
+                    for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+                    std::stable_partition(idx.begin(), idx.end(),
+                                          [&](size_t i) { return jobs[i].end - jobs[i].begin > 2 * kChunk; });
+                    std::vector<TcJob> j2(jobs.size());
+                    std::vector<int> n2(jobs.size());
+                    for (size_t i = 0; i < idx.size(); i++) {
+                        j2[i] = jobs[idx[i]];
+                        n2[i] = job_node[idx[i]];
+                    }
+                    jobs.swap(j2);
+                    job_node.swap(n2);
+                }
+                size_t nbig = 0;
+                while (nbig < jobs.size() && jobs[nbig].end - jobs[nbig].begin > 2 * kChunk) nbig++;
+                if (!kWide<D>) nbig = 0;
                 outs.resize(jobs.size());
                 if (hipMemcpyAsync(djobs, jobs.data(), jobs.size() * sizeof(TcJob), hipMemcpyHostToDevice, st) !=
                     hipSuccess) {
                     fail(CRN2_EHIP);
                     break;
                 }
-                hipLaunchKernelGGL(tc_split_kernel<D>, dim3((uint32_t)jobs.size()), dim3(kLanes), 0, st, a, djobs,
-                                   douts);
+                if (nbig) launch_wide<D>(a, djobs, douts, (uint32_t)nbig, st);
+                if (jobs.size() > nbig)
+                    hipLaunchKernelGGL((tc_split_kernel<D, 1>), dim3((uint32_t)(jobs.size() - nbig)), dim3(kLanes), 0,
+                                       st, a, djobs + nbig, douts + nbig);
                 if (hipGetLastError() != hipSuccess ||
                     hipMemcpyAsync(outs.data(), douts, outs.size() * sizeof(TcOut), hipMemcpyDeviceToHost, st) !=
                         hipSuccess ||
