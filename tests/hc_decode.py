@@ -49,3 +49,25 @@ def decode_alpha(ep, sel, c, alpha_endpoints, alpha_selectors):
 def psnr(a, b):
     mse = np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2)
     return 99.0 if mse == 0 else 10 * np.log10(255.0 ** 2 / mse)
+
+
+def dxt1_texels(raw):
+    """Standard DXT1 decode of a byte array of 8-byte blocks -> (n, 16, 3)."""
+    b = np.frombuffer(bytes(raw), np.uint8).reshape(-1, 8).astype(np.int64)
+    lo = b[:, 0] | (b[:, 1] << 8)
+    hi = b[:, 2] | (b[:, 3] << 8)
+    sel = b[:, 4] | (b[:, 5] << 8) | (b[:, 6] << 16) | (b[:, 7] << 24)
+    pal = dxt1_palette(lo, hi)
+    codes = (sel[:, None] >> (2 * np.arange(16))[None, :]) & 3
+    return np.take_along_axis(pal, codes[:, :, None].repeat(3, 2), 1)
+
+
+def dxt5a_values(raw8):
+    """Decode of the 8-byte alpha half of DXT5 / DXN blocks -> (n, 16)."""
+    b = np.frombuffer(bytes(raw8), np.uint8).reshape(-1, 8).astype(np.int64)
+    lo, hi = b[:, 0], b[:, 1]
+    bits = np.zeros(len(b), np.uint64)
+    for k in range(6):
+        bits |= b[:, 2 + k].astype(np.uint64) << np.uint64(8 * k)
+    codes = ((bits[:, None] >> (3 * np.arange(16, dtype=np.uint64))[None, :]) & np.uint64(7)).astype(np.int64)
+    return np.take_along_axis(dxt5_palette(lo, hi), codes, 1)
