@@ -67,6 +67,8 @@ static bool kind_uses(int kind, uint32_t m)
 struct TexInfo {
     const DevModel* models;      // M_COUNT models (device)
     const uint16_t* sorted;      // M_COUNT x kMaxSyms canonical symbol orders
+    const uint32_t* cw1;         // kLutSize: per 11-bit prefix, GL | F << 8 | DL0 << 16 | SL0 << 24 (see crnd_tables_kernel)
+    const uint16_t* cw2;         // kLutSize: per 11-bit prefix, DL1 | SL1 << 8 (second channel of kinds 1 and 2)
     int32_t* status;             // set to CRN2_ECORRUPT on an out-of-range palette index
     uint32_t kind, n_ce, n_cs, n_ae, n_as;
 };
@@ -99,6 +101,17 @@ struct LevelJob {
     uint64_t* ent;               // faces * gh * gw: (parent, index residues) after the chunk pass
     uint64_t* bnd;               // two buffers of nchunks * gw: the chunks' last rows
 };
+
+// The four facts the transition tables need of a reference group symbol G
+// (four 2-bit reference codes: the pair's two blocks in bits 0-1 and 4-5, the
+// blocks below them in bits 2-3 and 6-7): bit 0 / bit 1 -- the pair's first /
+// second block carries endpoint deltas (its code is 0); bit 2 / bit 3 -- the
+// block below the first / second one does.
+__host__ __device__ inline uint32_t group_flags(uint32_t g)
+{
+    return (uint32_t)((g & 0x03u) == 0) | ((uint32_t)((g & 0x30u) == 0) << 1) | ((uint32_t)((g & 0x0Cu) == 0) << 2) |
+           ((uint32_t)((g & 0xC0u) == 0) << 3);
+}
 
 __device__ inline bool kind_uses_dev(uint32_t kind, uint32_t m)
 {
@@ -156,27 +169,6 @@ __device__ inline uint32_t decode_top(const LdsTables& t, const uint16_t* sorted
     return sorted[idx];
 }
 
-// Code length only (the symbol value is not needed to find the next symbol).
-struct alignas(16) LenLut {
-    uint8_t len[kLutSize];
-    uint32_t limit[17];
-    int32_t base[17];
-    uint32_t nsyms;
-    uint32_t pad;
-};
-
-__device__ inline uint32_t code_len(const LenLut& t, uint32_t top)
-{
-    const uint32_t l = t.len[top >> (16 - kLutBits)];
-    if (l) return l;
-    const uint32_t k = top + 1;
-    uint32_t L = kLutBits + 1;
-    while (L <= 16 && k > t.limit[L]) L++;
-    if (L > 16) return 0;
-    const uint32_t idx = (uint32_t)(t.base[L] + (int32_t)(top >> (16 - L)));
-    return idx < t.nsyms ? L : 0u;
-}
-
 // ------------------------------------------------ stage 1: transition tables
 // For every bit position q of a level stream (bits at or past the end read as
 // zero, so every entry from the end of the stream on is the same; the tables
@@ -191,19 +183,30 @@ __device__ inline uint32_t code_len(const LenLut& t, uint32_t top)
 // One workgroup computes kTabPerWG consecutive positions in three LDS passes:
 //   1. per position: the reference group symbol and its code length, and the
 //      code lengths of one block's endpoint deltas and of its selectors
-//      (CW = GL | G << 8 | DL << 16 | SL << 24; two channels: both codes);
+//      (CW = GL | F << 8 | DL << 16 | SL << 24; two channels: both codes;
+//      F = group_flags(G));
 //   2. per position: the one-unit transitions from CW -- bits of an even-row
 //      pair (group symbol, then per block the deltas when its reference code
 //      is 0, then the selectors) and of an odd-row block without / with deltas
-//      (T = pair | G << 8 | odd0 << 16 | odd1 << 24; a pair takes at most 144
-//      bits, a block 64, a code 16);
+//      (T = pair | odd flags << 8 | odd0 << 16 | odd1 << 24; a pair takes at
+//      most 144 bits, a block 64, a code 16);
 //   3. the composed entries above from T (eight pairs reach 1008 entries on).
-// Each pass runs past the tile by what the next one reads.
+// Each pass runs past the tile by what the next one reads.  Every thread
+// keeps several positions' lookup chains in flight at once (kTabU1..3, the
+// positions tid + 512 k), so that the dependent LDS reads of one position
+// overlap those of the others; the pass-1 code tables and the T array share
+// LDS (the tables are dead once pass 1 ends), 41,600 B per workgroup: three
+// workgroups per CU.
 constexpr uint32_t kTabPerWG = 4096;
 constexpr uint32_t kTabT = kTabPerWG + 1024;
 constexpr uint32_t kTabCW = kTabT + 160;
 constexpr uint32_t kTabWords = kTabCW / 32 + 3;
 constexpr uint32_t kTabThreads = 512;
+constexpr uint32_t kTabU1 = 6;   // pass 1: two rounds of 6 slots cover kTabCW (5,280 = 10.3 x 512)
+constexpr uint32_t kTabU2 = 5;   // pass 2: two rounds of 5 cover kTabT
+constexpr uint32_t kTabU3 = 4;   // pass 3: two rounds of 4 cover kTabPerWG
+static_assert(2 * kTabU1 * kTabThreads >= kTabCW && 2 * kTabU2 * kTabThreads == kTabT &&
+                  2 * kTabU3 * kTabThreads == kTabPerWG, "table pass rounds");
 
 __device__ inline uint32_t be_bytes_word(const uint8_t* src, uint32_t size, uint32_t b)
 {
@@ -222,100 +225,237 @@ __device__ inline uint32_t peek16(const uint32_t* w, uint32_t r)
     return (uint32_t)((v << (r & 31)) >> 48);
 }
 
-__device__ inline uint32_t find_job(const LevelJob* jobs, uint32_t njobs, uint32_t wg, bool sym)
+// Canonical-walk constants of one model for codes longer than kLutBits, with
+// limit[L] for L > kLutBits made non-decreasing (an empty length takes the
+// largest limit below it: the canonical walk passes over it all the same), so
+// that the code length is kLutBits + 1 plus the number of limits the window
+// exceeds -- five broadcast LDS reads and no loop.
+struct alignas(16) CodeLimits {
+    uint32_t limit[17];
+    int32_t base[17];
+    uint32_t nsyms;
+    uint32_t pad;
+};
+
+// Length of the code in the 16-bit window `top` whose 11-bit prefix is no
+// complete code (0: no code matches, crn_decomp.h:3239-3244), and its index
+// into the canonical order.
+__device__ __attribute__((always_inline)) inline uint32_t long_code(const CodeLimits& t, uint32_t top, uint32_t& idx)
 {
-    uint32_t lo = 0, hi = njobs;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if ((sym ? jobs[mid].wg_sym : jobs[mid].wg_trans) <= wg) lo = mid;
-        else hi = mid;
-    }
-    return lo;
+    const uint32_t k = top + 1;
+    uint32_t L = kLutBits + 1;
+#pragma unroll
+    for (uint32_t i = kLutBits + 1; i <= 16; i++) L += k > t.limit[i] ? 1u : 0u;
+    if (L > 16) return 0;
+    idx = (uint32_t)(t.base[L] + (int32_t)(top >> (16 - L)));
+    return idx < t.nsyms ? L : 0u;
 }
 
-__device__ inline uint32_t odd_flags(uint32_t g) { return (uint32_t)((g & 0x0Cu) == 0) | ((uint32_t)((g & 0xC0u) == 0) << 1); }
 
-__global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
+typedef __attribute__((address_space(1))) uint32_t guint32;
+typedef __attribute__((address_space(1))) uint16_t guint16;
+
+constexpr uint32_t kTabRefSyms = 1024;   // reference-model symbols (low bytes) staged in LDS; the rest read from memory
+
+struct TabCodes {
+    uint32_t cw1[kLutSize];   // TexInfo::cw1
+    uint16_t cw2[kLutSize];   // TexInfo::cw2
+    CodeLimits lim[M_COUNT];
+    uint8_t ref_sym[kTabRefSyms];
+    uint32_t words[kTabWords];
+};
+union TabShared {
+    TabCodes c;          // pass 1
+    uint32_t t[kTabT];   // passes 2 and 3
+};
+
+// tab_job[wg] = the level job of tables workgroup wg.
+__global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob* __restrict__ jobs,
+                                                                  const uint32_t* __restrict__ tab_job,
                                                                   const TexInfo* __restrict__ texs)
 {
-    __shared__ LdsTables s_ref;
-    __shared__ LenLut s_len[4];   // M_EP0, M_SEL0, M_EP1, M_SEL1
-    __shared__ uint32_t s_words[kTabWords];
+    __shared__ TabShared s_u;
     __shared__ uint32_t s_cw[kTabCW];
-    __shared__ uint32_t s_t[kTabT];
     const uint32_t tid = threadIdx.x;
-    const LevelJob& j = jobs[find_job(jobs, njobs, blockIdx.x, false)];
+    const LevelJob& j = jobs[tab_job[blockIdx.x]];
     const TexInfo& tx = texs[j.tex];
     const uint32_t kind = tx.kind;
     const uint32_t P0 = (blockIdx.x - j.wg_trans) * kTabPerWG;
 
-    stage_tables(s_ref, tx.models[M_REF], tid, kTabThreads);
-    for (uint32_t m = 1; m < M_COUNT; m++) {
-        if (!kind_uses_dev(kind, m)) continue;
-        const DevModel& dm = tx.models[m];
-        LenLut& L = s_len[m - 1];
+    {
         typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-        for (uint32_t i = tid; i < kLutSize / 16; i += kTabThreads)
-            reinterpret_cast<u4*>(L.len)[i] = reinterpret_cast<const u4*>(dm.len8)[i];
-        if (tid < 17) {
-            L.limit[tid] = dm.limit[tid];
-            L.base[tid] = dm.base[tid];
+        const bool two = two_channels(kind);
+        for (uint32_t i = tid; i < kLutSize / 4; i += kTabThreads)
+            reinterpret_cast<u4*>(s_u.c.cw1)[i] = reinterpret_cast<const u4*>(tx.cw1)[i];
+        if (two)
+            for (uint32_t i = tid; i < kLutSize / 8; i += kTabThreads)
+                reinterpret_cast<u4*>(s_u.c.cw2)[i] = reinterpret_cast<const u4*>(tx.cw2)[i];
+        if (tid < 17 * M_COUNT) {
+            const uint32_t m = tid / 17, L = tid % 17;
+            const DevModel& dm = tx.models[m];
+            uint32_t lim = dm.limit[L];
+            for (uint32_t i = kLutBits + 1; i < L; i++) lim = max(lim, dm.limit[i]);
+            s_u.c.lim[m].limit[L] = lim;
+            s_u.c.lim[m].base[L] = dm.base[L];
         }
-        if (tid == 0) L.nsyms = dm.num_syms;
+        if (tid < M_COUNT) s_u.c.lim[tid].nsyms = tx.models[tid].num_syms;
+        for (uint32_t i = tid; i < kTabRefSyms; i += kTabThreads)
+            s_u.c.ref_sym[i] = (uint8_t)tx.sorted[M_REF * kMaxSyms + i];
     }
-    for (uint32_t i = tid; i < kTabWords; i += kTabThreads) s_words[i] = be_bytes_word(j.src, j.src_size, P0 / 8 + 4 * i);
+    for (uint32_t i = tid; i < kTabWords; i += kTabThreads)
+        s_u.c.words[i] = be_bytes_word(j.src, j.src_size, P0 / 8 + 4 * i);
     __syncthreads();
 
-    // 1. code lengths
-    const LenLut& D0 = s_len[d_model0(kind) - 1];
-    const LenLut& S0 = s_len[s_model0(kind) - 1];
-    const LenLut& D1 = s_len[M_EP1 - 1];
-    const LenLut& S1 = s_len[M_SEL1 - 1];
-    const bool two = two_channels(kind);
-    const uint16_t* ref_sorted = tx.sorted + M_REF * kMaxSyms;
-    for (uint32_t r = tid; r < kTabCW; r += kTabThreads) {
-        const uint32_t top = peek16(s_words, r);
-        uint32_t gl;
-        const uint32_t G = decode_top(s_ref, ref_sorted, top, gl) & 0xFFu;
-        uint32_t dl = code_len(D0, top);
-        if (two) dl += code_len(D1, peek16(s_words, r + dl));
-        uint32_t sl = code_len(S0, top);
-        if (two) sl += code_len(S1, peek16(s_words, r + sl));
-        s_cw[r] = gl | (G << 8) | (dl << 16) | (sl << 24);
-    }
-    __syncthreads();
-
-    // 2. one-unit transitions
-    for (uint32_t r = tid; r < kTabT; r += kTabThreads) {
-        const uint32_t c = s_cw[r];
-        const uint32_t G = (c >> 8) & 0xFFu;
-        uint32_t e = r + (c & 0xFFu);
-        if ((G & 3u) == 0) e += (s_cw[e] >> 16) & 0xFFu;
-        e += s_cw[e] >> 24;
-        if (((G >> 4) & 3u) == 0) e += (s_cw[e] >> 16) & 0xFFu;
-        e += s_cw[e] >> 24;
-        const uint32_t d = (c >> 16) & 0xFFu;
-        s_t[r] = (e - r) | (G << 8) | ((c >> 24) << 16) | ((d + (s_cw[r + d] >> 24)) << 24);
-    }
-    __syncthreads();
-
-    // 3. composed entries
-    for (uint32_t r = tid; r < kTabPerWG; r += kTabThreads) {
-        const uint32_t t0 = s_t[r];
-        const uint32_t o0 = (t0 >> 16) & 0xFFu, o1 = t0 >> 24;
-        const uint32_t ta = s_t[r + o0], tb = s_t[r + o1];
-        const uint32_t q = P0 + r;
-        j.O2[q] = (o0 + ((ta >> 16) & 0xFFu)) | ((o1 + ((tb >> 16) & 0xFFu)) << 8) | ((o0 + (ta >> 24)) << 16) |
-                  ((o1 + (tb >> 24)) << 24);
-        uint32_t p = r + (t0 & 0xFFu), fl = odd_flags((t0 >> 8) & 0xFFu);
+    // 1. code lengths: one combined-table read per position (two for two
+    // channels), the canonical walk only for codes longer than kLutBits
+    {
+        const uint32_t* W = s_u.c.words;
+        const CodeLimits& R = s_u.c.lim[M_REF];
+        const CodeLimits& D0 = s_u.c.lim[d_model0(kind)];
+        const CodeLimits& S0 = s_u.c.lim[s_model0(kind)];
+        const CodeLimits& D1 = s_u.c.lim[M_EP1];
+        const CodeLimits& S1 = s_u.c.lim[M_SEL1];
+        const bool two = two_channels(kind);
+        const uint16_t* ref_sorted = tx.sorted + M_REF * kMaxSyms;
+        constexpr uint32_t U = kTabU1;
+#pragma unroll 1
+        for (uint32_t i0 = 0; i0 < 2 * U; i0 += U) {
+            uint32_t r[U], top[U], c[U];
 #pragma unroll
-        for (uint32_t i = 1; i < 8; i++) {
-            const uint32_t t = s_t[p];
-            fl |= odd_flags((t >> 8) & 0xFFu) << (2 * i);
-            p += t & 0xFFu;
+            for (uint32_t u = 0; u < U; u++) {
+                r[u] = min(tid + (i0 + u) * kTabThreads, kTabCW - 1);
+                top[u] = peek16(W, r[u]);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) c[u] = s_u.c.cw1[top[u] >> (16 - kLutBits)];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                if (__builtin_expect((c[u] & 0xFFu) == 0 || (c[u] & 0xFF0000u) == 0 || (c[u] >> 24) == 0, 0)) {
+                    uint32_t gl = c[u] & 0xFFu, g = (c[u] >> 8) & 0xFFu, dl = (c[u] >> 16) & 0xFFu, sl = c[u] >> 24;
+                    uint32_t idx = 0;
+                    if (!gl) {
+                        gl = long_code(R, top[u], idx);
+                        g = group_flags(!gl ? 0u : idx < kTabRefSyms ? (uint32_t)s_u.c.ref_sym[idx]
+                                                                     : (uint32_t)(ref_sorted[idx] & 0xFFu));
+                    }
+                    if (!dl) dl = long_code(D0, top[u], idx);
+                    if (!sl) sl = long_code(S0, top[u], idx);
+                    c[u] = gl | (g << 8) | (dl << 16) | (sl << 24);
+                }
+            }
+            if (two) {
+                uint32_t ta[U], tb[U], a[U], b[U];
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) {
+                    ta[u] = peek16(W, r[u] + ((c[u] >> 16) & 0xFFu));
+                    tb[u] = peek16(W, r[u] + (c[u] >> 24));
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) {
+                    a[u] = s_u.c.cw2[ta[u] >> (16 - kLutBits)] & 0xFFu;
+                    b[u] = s_u.c.cw2[tb[u] >> (16 - kLutBits)] >> 8;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) {
+                    uint32_t idx = 0;
+                    if (__builtin_expect(!a[u], 0)) a[u] = long_code(D1, ta[u], idx);
+                    if (__builtin_expect(!b[u], 0)) b[u] = long_code(S1, tb[u], idx);
+                    c[u] += (a[u] << 16) + (b[u] << 24);
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++)
+                if (tid + (i0 + u) * kTabThreads < kTabCW) s_cw[r[u]] = c[u];
         }
-        j.E8[q] = (p - r) | (fl << 11);
-        j.X[q] = (uint16_t)((t0 & 0xFFu) | (odd_flags((t0 >> 8) & 0xFFu) << 8));
+    }
+    __syncthreads();
+
+    // 2. one-unit transitions (branch-free: a block without deltas re-reads
+    // the entry it already has)
+    {
+        uint32_t* T = s_u.t;
+        constexpr uint32_t U = kTabU2;
+#pragma unroll 1
+        for (uint32_t i0 = 0; i0 < 2 * U; i0 += U) {
+            uint32_t r[U], c[U], e[U], x[U], od[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                r[u] = tid + (i0 + u) * kTabThreads;
+                c[u] = s_cw[r[u]];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                e[u] = r[u] + (c[u] & 0xFFu);
+                x[u] = s_cw[e[u]];
+                od[u] = s_cw[r[u] + ((c[u] >> 16) & 0xFFu)];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                if (c[u] & 0x100u) e[u] += (x[u] >> 16) & 0xFFu;
+                x[u] = s_cw[e[u]];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                e[u] += x[u] >> 24;
+                x[u] = s_cw[e[u]];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                if (c[u] & 0x200u) e[u] += (x[u] >> 16) & 0xFFu;
+                x[u] = s_cw[e[u]];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                e[u] += x[u] >> 24;
+                const uint32_t d = (c[u] >> 16) & 0xFFu;
+                T[r[u]] = (e[u] - r[u]) | ((c[u] >> 2) & 0x300u) | ((c[u] >> 24) << 16) | ((d + (od[u] >> 24)) << 24);
+            }
+        }
+    }
+    __syncthreads();
+
+    // 3. composed entries (stores at 32-bit offsets from the tile's base)
+    {
+        const uint32_t* T = s_u.t;
+        typedef __attribute__((address_space(1))) uint8_t guint8;
+        guint8* O2 = (guint8*)(j.O2 + P0);
+        guint8* E8 = (guint8*)(j.E8 + P0);
+        guint8* X = (guint8*)(j.X + P0);
+        constexpr uint32_t U = kTabU3;
+#pragma unroll 1
+        for (uint32_t i0 = 0; i0 < 2 * U; i0 += U) {
+            uint32_t r[U], t0[U], ta[U], tb[U], p[U], fl[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                r[u] = tid + (i0 + u) * kTabThreads;
+                t0[u] = T[r[u]];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                ta[u] = T[r[u] + ((t0[u] >> 16) & 0xFFu)];
+                tb[u] = T[r[u] + (t0[u] >> 24)];
+                p[u] = r[u] + (t0[u] & 0xFFu);
+                fl[u] = t0[u] & 0x300u;
+            }
+            // odd flags gathered at bit 8 + 2 i, shifted down once
+#pragma unroll
+            for (uint32_t i = 1; i < 8; i++) {
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) {
+                    const uint32_t t = T[p[u]];
+                    fl[u] |= (t & 0x300u) << (2 * i);
+                    p[u] += t & 0xFFu;
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                const uint32_t o0 = (t0[u] >> 16) & 0xFFu, o1 = t0[u] >> 24;
+                *(guint32*)(O2 + 4 * r[u]) = (o0 + ((ta[u] >> 16) & 0xFFu)) | ((o1 + ((tb[u] >> 16) & 0xFFu)) << 8) |
+                                             ((o0 + (ta[u] >> 24)) << 16) | ((o1 + (tb[u] >> 24)) << 24);
+                *(guint32*)(E8 + 4 * r[u]) = (p[u] - r[u]) | ((fl[u] >> 8) << 11);
+                *(guint16*)(X + 2 * r[u]) = (uint16_t)(t0[u] & 0x3FFu);
+            }
+        }
     }
 }
 
@@ -365,7 +505,6 @@ struct WalkLds {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const u32x4 gconst_u32x4;
-typedef __attribute__((address_space(1))) uint32_t guint32;
 
 // Two-bit flag pairs x = f0 | f1 << 2 | f2 << 4 | f3 << 6 -> bytes f_k * 8.
 __device__ inline uint32_t spread_flags(uint32_t x)
@@ -1078,6 +1217,8 @@ struct crn2_unpack_ctx {
     uint8_t* d_file = nullptr;   // device copy padded by 64 bytes
     DevModel* d_models = nullptr;
     uint16_t* d_sorted = nullptr;
+    uint32_t* d_cw1 = nullptr;   // TexInfo::cw1
+    uint16_t* d_cw2 = nullptr;   // TexInfo::cw2
     uint32_t* d_ce = nullptr;
     uint32_t* d_cs = nullptr;
     uint16_t* d_ae = nullptr;
@@ -1098,6 +1239,7 @@ struct crn2_batch {
     TexInfo* d_texs = nullptr;
     WriteJob* d_wjobs = nullptr;
     WritePalettes* d_pals = nullptr;
+    uint32_t* d_tab_job = nullptr;  // level job of every crnd_tables_kernel workgroup
     uint8_t* d_scratch = nullptr;   // transition tables, start bits, symbol records
     uint32_t njobs = 0, nwjobs = 0;
     uint32_t wg_trans = 0, wg_sym = 0, wg_odd = 0, wg_idx = 0;
@@ -1206,7 +1348,7 @@ static void batch_free(crn2_batch* b)
     if (!b) return;
     b->magic = 0;
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    void* ptrs[] = {b->d_jobs, b->d_texs, b->d_wjobs, b->d_pals, b->d_scratch};
+    void* ptrs[] = {b->d_jobs, b->d_texs, b->d_wjobs, b->d_pals, b->d_tab_job, b->d_scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -1216,7 +1358,7 @@ static void ctx_free_device(crn2_unpack_ctx* c)
 {
     batch_free(c->cached);
     c->cached = nullptr;
-    void* ptrs[] = {c->d_file, c->d_models, c->d_sorted, c->d_ce, c->d_cs, c->d_ae, c->d_as, c->d_status, c->d_rec};
+    void* ptrs[] = {c->d_file, c->d_models, c->d_sorted, c->d_cw1, c->d_cw2, c->d_ce, c->d_cs, c->d_ae, c->d_as, c->d_status, c->d_rec};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -1278,6 +1420,20 @@ static crn2_unpack_ctx* begin_impl(const void* data, const void* data_dev, uint3
             memset(&dms[m], 0, sizeof(DevModel));
         }
     }
+    // Per 11-bit prefix: the reference code's length and group symbol and the
+    // lengths of one block's first-channel delta and selector codes (a zero
+    // field: the code is longer than kLutBits); second-channel lengths apart.
+    std::vector<uint32_t> cw1(kLutSize);
+    std::vector<uint16_t> cw2(kLutSize);
+    {
+        const uint32_t d0 = c->kind <= 1 ? M_EP0 : M_EP1, s0 = c->kind <= 1 ? M_SEL0 : M_SEL1;
+        for (uint32_t v = 0; v < kLutSize; v++) {
+            const uint32_t r = dms[M_REF].lut[v];
+            cw1[v] = (r >> 16) | (group_flags(r & 0xFFu) << 8) | ((uint32_t)dms[d0].len8[v] << 16) |
+                     ((uint32_t)dms[s0].len8[v] << 24);
+            cw2[v] = (uint16_t)(dms[M_EP1].len8[v] | (dms[M_SEL1].len8[v] << 8));
+        }
+    }
     uint64_t rec = 0;
     const uint32_t rec_bytes = (c->kind == 1 || c->kind == 2) ? 8 : 4;
     for (uint32_t l = 0; l < h.levels; l++) {
@@ -1291,6 +1447,7 @@ static crn2_unpack_ctx* begin_impl(const void* data, const void* data_dev, uint3
     const size_t padded = ((size_t)size + 15) / 16 * 16 + 64;
     if (dmalloc(&c->d_file, padded) != hipSuccess || dmalloc(&c->d_models, sizeof(DevModel) * M_COUNT) != hipSuccess ||
         dmalloc(&c->d_sorted, sorted.size() * 2) != hipSuccess ||
+        dmalloc(&c->d_cw1, kLutSize * 4) != hipSuccess || dmalloc(&c->d_cw2, kLutSize * 2) != hipSuccess ||
         dmalloc(&c->d_ce, std::max<size_t>(ce.size(), 1) * 4) != hipSuccess ||
         dmalloc(&c->d_cs, std::max<size_t>(cs.size(), 1) * 4) != hipSuccess ||
         dmalloc(&c->d_ae, std::max<size_t>(ae.size(), 1) * 2) != hipSuccess ||
@@ -1310,6 +1467,8 @@ static crn2_unpack_ctx* begin_impl(const void* data, const void* data_dev, uint3
         ok &= hipMemcpyAsync(c->d_file, data, size, hipMemcpyHostToDevice, stream) == hipSuccess;
     ok &= hipMemcpyAsync(c->d_models, dms.data(), sizeof(DevModel) * M_COUNT, hipMemcpyHostToDevice, stream) == hipSuccess;
     ok &= hipMemcpyAsync(c->d_sorted, sorted.data(), sorted.size() * 2, hipMemcpyHostToDevice, stream) == hipSuccess;
+    ok &= hipMemcpyAsync(c->d_cw1, cw1.data(), kLutSize * 4, hipMemcpyHostToDevice, stream) == hipSuccess;
+    ok &= hipMemcpyAsync(c->d_cw2, cw2.data(), kLutSize * 2, hipMemcpyHostToDevice, stream) == hipSuccess;
     if (!ce.empty()) ok &= hipMemcpyAsync(c->d_ce, ce.data(), ce.size() * 4, hipMemcpyHostToDevice, stream) == hipSuccess;
     if (!cs.empty()) ok &= hipMemcpyAsync(c->d_cs, cs.data(), cs.size() * 4, hipMemcpyHostToDevice, stream) == hipSuccess;
     if (!ae.empty()) ok &= hipMemcpyAsync(c->d_ae, ae.data(), ae.size() * 2, hipMemcpyHostToDevice, stream) == hipSuccess;
@@ -1359,6 +1518,8 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
         memset(&t, 0, sizeof(t));
         t.models = c->d_models;
         t.sorted = c->d_sorted;
+        t.cw1 = c->d_cw1;
+        t.cw2 = c->d_cw2;
         t.status = c->d_status;
         t.kind = (uint32_t)c->kind;
         t.n_ce = c->n_ce;
@@ -1466,6 +1627,7 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
         dmalloc(&B->d_texs, texs.size() * sizeof(TexInfo)) != hipSuccess ||
         dmalloc(&B->d_wjobs, wj.size() * sizeof(WriteJob)) != hipSuccess ||
         dmalloc(&B->d_pals, pals.size() * sizeof(WritePalettes)) != hipSuccess ||
+        dmalloc(&B->d_tab_job, std::max<size_t>(wg_trans, 1) * sizeof(uint32_t)) != hipSuccess ||
         dmalloc(&B->d_scratch, scratch) != hipSuccess) {
         batch_free(B);
         return CRN2_ENOMEM;
@@ -1485,6 +1647,13 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
     ok &= hipMemcpy(B->d_texs, texs.data(), texs.size() * sizeof(TexInfo), hipMemcpyHostToDevice) == hipSuccess;
     ok &= hipMemcpy(B->d_wjobs, wj.data(), wj.size() * sizeof(WriteJob), hipMemcpyHostToDevice) == hipSuccess;
     ok &= hipMemcpy(B->d_pals, pals.data(), pals.size() * sizeof(WritePalettes), hipMemcpyHostToDevice) == hipSuccess;
+    {
+        std::vector<uint32_t> tab_job(wg_trans);
+        for (uint32_t i = 0; i < (uint32_t)jobs.size(); i++)
+            std::fill(tab_job.begin() + jobs[i].wg_trans, tab_job.begin() + jobs[i].wg_trans + jobs[i].tlen / kTabPerWG, i);
+        ok &= hipMemcpy(B->d_tab_job, tab_job.data(), tab_job.size() * sizeof(uint32_t), hipMemcpyHostToDevice) ==
+              hipSuccess;
+    }
     if (!ok) {
         batch_free(B);
         return CRN2_EHIP;
@@ -1501,8 +1670,8 @@ static int launch_plan(crn2_batch* B, hipStream_t stream, Profile::Ev* ev)
     const bool prof = ev != nullptr;
     if (prof) HIPCHK(hipEventRecord(ev->e[0], stream));
     if (B->njobs) {
-        hipLaunchKernelGGL(crnd_tables_kernel, dim3(B->wg_trans), dim3(kTabThreads), 0, stream, B->d_jobs, B->njobs,
-                           B->d_texs);
+        hipLaunchKernelGGL(crnd_tables_kernel, dim3(B->wg_trans), dim3(kTabThreads), 0, stream, B->d_jobs,
+                           B->d_tab_job, B->d_texs);
         HIPCHK(hipGetLastError());
         if (prof) HIPCHK(hipEventRecord(ev->e[1], stream));
         if (B->njobs >= kPlanJobsSmallRing)

@@ -16,7 +16,7 @@ constexpr uint32_t kMaxLevels = 16;
 // the canonical tables for the long-code path.
 struct DevModel {
     uint32_t lut[kLutSize];
-    uint8_t len8[kLutSize];   // lut[i] >> 16: the code length alone, for kernels that need no symbol
+    uint8_t len8[kLutSize];   // lut[i] >> 16: the code length alone (source of the combined prefix tables)
     uint32_t limit[17];   // limit[L] = 1 + left-aligned largest code of length L (0 if none)
     int32_t base[17];     // base[L] + code = index into sorted[]
     uint32_t num_syms;
