@@ -23,7 +23,7 @@ HBM_PEAK_GBS = 8000.0
 # known size): 1.168 GB, 30x the algorithmic bytes, mostly the per-bit tables.
 TRAFFIC_PER_COPY = 1.168e9
 STAGES = ("crnd_tables_kernel", "crnd_walk_kernel", "crnd_sym_kernel<even>", "crnd_sym_kernel<odd>",
-          "crnd_index_{chunk,link,final}_kernel", "crnd_write_kernel")
+          "crnd_index_{chunk,carry,final}_kernel", "crnd_write_kernel")
 
 
 def log(*a):

@@ -19,9 +19,9 @@
 //      8 odd-row blocks.
 //   3. crnd_sym_kernel (one thread per group; even rows, then odd rows):
 //      endpoint deltas, selectors and reference codes from the start bits.
-//   4. crnd_index_{chunk,link,final}_kernel: running endpoint indices by
+//   4. crnd_index_{chunk,carry,final}_kernel: running endpoint indices by
 //      pointer jumping over the blocks' parent links (previous block or block
-//      above), in LDS per chunk of rows, then across chunks.
+//      above) in LDS per chunk of rows, then chunk after chunk per level.
 //   5. crnd_write_kernel (one thread per visible block): palette gather and
 //      the 8- or 16-byte DXTn block write, coalesced.
 #include <hip/hip_runtime.h>
@@ -96,10 +96,9 @@ struct LevelJob {
     uint32_t nchunks;            // index chunks of the level
     uint32_t wg_idx;             // first workgroup of the job in crnd_index_chunk_kernel
     uint32_t pad;
-    uint64_t bnd_first;          // first compact entry of the job in crnd_index_link_kernel
     uint64_t blk_first;          // first block of the job in crnd_index_final_kernel
     uint64_t* ent;               // faces * gh * gw: (parent, index residues) after the chunk pass
-    uint64_t* bnd;               // two buffers of nchunks * gw: the chunks' last rows
+    uint64_t* bnd;               // nchunks * gw: the chunks' last rows (resolved in place by crnd_index_carry_kernel)
 };
 
 // The four facts the transition tables need of a reference group symbol G
@@ -908,8 +907,10 @@ __global__ void __launch_bounds__(256) crnd_sym_kernel(const LevelJob* __restric
 //       blocks) in LDS, until every block points at the root or outside the
 //       chunk -- which is always the chunk's row above, the last row of the
 //       previous chunk; that last row is also copied to a compact array;
-//   crnd_index_link_kernel: rounds over the compact last rows, each pointing
-//       one chunk back, until every one points at the root;
+//   crnd_index_carry_kernel: one workgroup per level of more than one chunk
+//       walks its chunks in order, resolving each chunk's last row from the
+//       resolved last row of the chunk before it (held in LDS; the rows of
+//       the next few chunks are loaded ahead);
 //   crnd_index_final_kernel: one thread per block adds its chunk-level result
 //       to the resolved row it points at, checks ranges and writes its record.
 // With residues kept mod n, a block's index equals the reference's whenever no
@@ -1010,48 +1011,93 @@ __global__ void __launch_bounds__(256) crnd_index_chunk_kernel(const LevelJob* _
     }
 }
 
-template <typename F>
-__device__ inline uint32_t find_job_u64(const LevelJob* jobs, uint32_t njobs, uint64_t t, F first)
+// The job holding thread t of a launch over consecutive ranges (first(job) =
+// the range's first thread): one workgroup-uniform binary search for the
+// workgroup's first thread (scalar loads), then a short forward scan per
+// thread for workgroups that straddle jobs.
+template <typename J, typename F>
+__device__ inline uint32_t find_job_wg(const J* jobs, uint32_t njobs, uint64_t t, F first)
 {
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x;
     uint32_t lo = 0, hi = njobs;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (first(jobs[mid]) <= t) lo = mid;
+        if (first(jobs[mid]) <= t0) lo = mid;
         else hi = mid;
     }
+    while (lo + 1 < njobs && first(jobs[lo + 1]) <= t) lo++;
     return lo;
 }
 
-// One round over the compact last rows: buffer `from` to buffer from ^ 1.
-__global__ void __launch_bounds__(256) crnd_index_link_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
-                                                              const TexInfo* __restrict__ texs, uint64_t total,
-                                                              uint32_t from)
+constexpr uint32_t kMaxIdxGw = 1024;       // widest padded grid a plan accepts
+static_assert(2 * kGroup * kMaxEGroups <= kMaxIdxGw, "index carry row width");
+constexpr uint32_t kCarryThreads = 256;
+constexpr uint32_t kCarryPer = kMaxIdxGw / kCarryThreads;
+constexpr uint32_t kCarryAhead = 4;        // chunks whose last rows are loaded ahead
+
+// carry[wg] = a level job of more than one chunk.  Chunk 0's last row is
+// resolved already (it points at the root); chunk c's entries point into
+// chunk c - 1's last row.  Resolved rows are written back in place as
+// (root, residues).
+__global__ void __launch_bounds__(kCarryThreads) crnd_index_carry_kernel(const LevelJob* __restrict__ jobs,
+                                                                         const uint32_t* __restrict__ carry,
+                                                                         const TexInfo* __restrict__ texs)
 {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const LevelJob& j = jobs[find_job_u64(jobs, njobs, t, [](const LevelJob& x) { return x.bnd_first; })];
-    const uint64_t k = t - j.bnd_first;
-    const uint64_t n = (uint64_t)j.nchunks * j.gw;
-    const uint64_t* src = j.bnd + from * n;
-    uint64_t* dst = j.bnd + (from ^ 1u) * n;
-    uint64_t e = src[k];
-    const uint32_t pa = (uint32_t)e;
-    if (pa != kRoot) e = jump(e, src[pa], idx_sizes(texs[j.tex]));
-    dst[k] = e;
+    typedef __attribute__((address_space(1))) uint64_t guint64;
+    __shared__ uint32_t V[2][kMaxIdxGw];
+    const uint32_t tid = threadIdx.x;
+    const LevelJob& j = jobs[carry[blockIdx.x]];
+    const IdxSizes z = idx_sizes(texs[j.tex]);
+    const uint32_t gw = j.gw, n = j.nchunks;
+    guint64* bnd = (guint64*)j.bnd;
+    for (uint32_t i = tid; i < gw; i += kCarryThreads) V[0][i] = (uint32_t)(bnd[i] >> 32);
+    uint64_t pre[kCarryAhead][kCarryPer];
+#pragma unroll
+    for (uint32_t a = 0; a < kCarryAhead; a++)
+#pragma unroll
+        for (uint32_t k = 0; k < kCarryPer; k++) {
+            const uint32_t c = 1 + a, i = tid + k * kCarryThreads;
+            pre[a][k] = (c < n && i < gw) ? bnd[(uint64_t)c * gw + i] : 0ull;
+        }
+    __syncthreads();
+    uint32_t cur = 0;
+    for (uint32_t c0 = 1; c0 < n; c0 += kCarryAhead) {
+#pragma unroll
+        for (uint32_t a = 0; a < kCarryAhead; a++) {
+            const uint32_t c = c0 + a;
+            if (c < n) {
+                const uint32_t cn = c + kCarryAhead;
+#pragma unroll
+                for (uint32_t k = 0; k < kCarryPer; k++) {
+                    const uint32_t i = tid + k * kCarryThreads;
+                    const uint64_t e = pre[a][k];
+                    pre[a][k] = (cn < n && i < gw) ? bnd[(uint64_t)cn * gw + i] : 0ull;
+                    if (i < gw) {
+                        const uint32_t pa = (uint32_t)e;
+                        uint32_t v = (uint32_t)(e >> 32);
+                        if (pa != kRoot) v = add_res(v, V[cur][pa - (c - 1) * gw], z.n0, z.n1);
+                        V[cur ^ 1u][i] = v;
+                        bnd[(uint64_t)c * gw + i] = ((uint64_t)v << 32) | kRoot;
+                    }
+                }
+            }
+            __syncthreads();
+            cur ^= 1u;
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) crnd_index_final_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
-                                                               const TexInfo* __restrict__ texs, uint64_t total,
-                                                               uint32_t fin)
+                                                               const TexInfo* __restrict__ texs, uint64_t total)
 {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    const LevelJob& j = jobs[find_job_u64(jobs, njobs, t, [](const LevelJob& x) { return x.blk_first; })];
+    const LevelJob& j = jobs[find_job_wg(jobs, njobs, t, [](const LevelJob& x) { return x.blk_first; })];
     const TexInfo& tx = texs[j.tex];
     const IdxSizes z = idx_sizes(tx);
     const uint32_t b = (uint32_t)(t - j.blk_first);
     const uint32_t gw = j.gw;
-    const uint64_t* res = j.bnd + (uint64_t)fin * j.nchunks * gw;
+    const uint64_t* res = j.bnd;
     auto value = [&](uint32_t blk) -> uint32_t {
         const uint64_t e = j.ent[blk];
         const uint32_t pa = (uint32_t)e;
@@ -1113,13 +1159,7 @@ __global__ void __launch_bounds__(256) crnd_write_kernel(const WriteJob* __restr
 {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    uint32_t lo = 0, hi = njobs;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (jobs[mid].first_block <= t) lo = mid;
-        else hi = mid;
-    }
-    const WriteJob& j = jobs[lo];
+    const WriteJob& j = jobs[find_job_wg(jobs, njobs, t, [](const WriteJob& x) { return x.first_block; })];
     const WritePalettes& P = pals[j.pal];
     uint64_t v = t - j.first_block;
     const uint32_t per_face = j.bx * j.by;
@@ -1244,8 +1284,9 @@ struct crn2_batch {
     uint32_t njobs = 0, nwjobs = 0;
     uint32_t wg_trans = 0, wg_sym = 0, wg_odd = 0, wg_idx = 0;
     uint32_t max_gw = 0;
-    uint32_t link_rounds = 0;                   // rounds of crnd_index_link_kernel
-    uint64_t bnd_total = 0, blk_total = 0;
+    uint32_t* d_carry = nullptr;                // level jobs of more than one index chunk (crnd_index_carry_kernel)
+    uint32_t ncarry = 0;
+    uint64_t blk_total = 0;
     uint64_t total_blocks = 0;
     std::vector<int32_t*> status;   // per context
     hipGraphExec_t graph = nullptr; // the plan's launches captured once (crn2_batch_run_graph)
@@ -1348,7 +1389,7 @@ static void batch_free(crn2_batch* b)
     if (!b) return;
     b->magic = 0;
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    void* ptrs[] = {b->d_jobs, b->d_texs, b->d_wjobs, b->d_pals, b->d_tab_job, b->d_scratch};
+    void* ptrs[] = {b->d_jobs, b->d_texs, b->d_wjobs, b->d_pals, b->d_tab_job, b->d_carry, b->d_scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -1508,8 +1549,8 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
     size_t scratch = 0;
     struct Ofs { size_t O2, E8, X, E, O, G, pre, ent, bnd; };
     std::vector<Ofs> ofs;
-    uint32_t wg_trans = 0, wg_sym = 0, wg_odd = 0, wg_idx = 0, max_gw = 0, max_chunks = 1;
-    uint64_t bnd_total = 0, blk_total = 0;
+    uint32_t wg_trans = 0, wg_sym = 0, wg_odd = 0, wg_idx = 0, max_gw = 0;
+    uint64_t blk_total = 0;
     for (uint32_t pi = 0; pi < parts.size(); pi++) {
         PlanPart& P = parts[pi];
         crn2_unpack_ctx* c = P.c;
@@ -1563,11 +1604,8 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
             j.nchunks = (rows + j.crows - 1) / j.crows;
             j.wg_idx = wg_idx;
             wg_idx += j.nchunks;
-            j.bnd_first = bnd_total;
-            bnd_total += (uint64_t)j.nchunks * gw;
             j.blk_first = blk_total;
             blk_total += (uint64_t)rows * gw;
-            max_chunks = std::max(max_chunks, j.nchunks);
             Ofs o;
             o.O2 = scratch;
             scratch = align_up(scratch + (size_t)j.tlen * 4, 256);
@@ -1586,7 +1624,7 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
             o.ent = scratch;
             scratch = align_up(scratch + rowpairs * 2 * gw * 8, 256);
             o.bnd = scratch;
-            scratch = align_up(scratch + (size_t)j.nchunks * gw * 16, 256);
+            scratch = align_up(scratch + (size_t)j.nchunks * gw * 8, 256);
             ofs.push_back(o);
             j.rec = static_cast<uint8_t*>(c->d_rec) + c->rec_ofs[l];
             max_gw = std::max(max_gw, gw);
@@ -1618,16 +1656,18 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
     B->wg_odd = wg_odd;
     B->wg_idx = wg_idx;
     B->max_gw = max_gw;
-    B->bnd_total = bnd_total;
     B->blk_total = blk_total;
-    B->link_rounds = 0;
-    while ((1u << B->link_rounds) < max_chunks) B->link_rounds++;
+    std::vector<uint32_t> carry;
+    for (uint32_t i = 0; i < (uint32_t)jobs.size(); i++)
+        if (jobs[i].nchunks > 1) carry.push_back(i);
+    B->ncarry = (uint32_t)carry.size();
     B->total_blocks = total;
     if (dmalloc(&B->d_jobs, jobs.size() * sizeof(LevelJob)) != hipSuccess ||
         dmalloc(&B->d_texs, texs.size() * sizeof(TexInfo)) != hipSuccess ||
         dmalloc(&B->d_wjobs, wj.size() * sizeof(WriteJob)) != hipSuccess ||
         dmalloc(&B->d_pals, pals.size() * sizeof(WritePalettes)) != hipSuccess ||
         dmalloc(&B->d_tab_job, std::max<size_t>(wg_trans, 1) * sizeof(uint32_t)) != hipSuccess ||
+        dmalloc(&B->d_carry, std::max<size_t>(carry.size(), 1) * sizeof(uint32_t)) != hipSuccess ||
         dmalloc(&B->d_scratch, scratch) != hipSuccess) {
         batch_free(B);
         return CRN2_ENOMEM;
@@ -1653,6 +1693,9 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
             std::fill(tab_job.begin() + jobs[i].wg_trans, tab_job.begin() + jobs[i].wg_trans + jobs[i].tlen / kTabPerWG, i);
         ok &= hipMemcpy(B->d_tab_job, tab_job.data(), tab_job.size() * sizeof(uint32_t), hipMemcpyHostToDevice) ==
               hipSuccess;
+        if (!carry.empty())
+            ok &= hipMemcpy(B->d_carry, carry.data(), carry.size() * sizeof(uint32_t), hipMemcpyHostToDevice) ==
+                  hipSuccess;
     }
     if (!ok) {
         batch_free(B);
@@ -1691,15 +1734,14 @@ static int launch_plan(crn2_batch* B, hipStream_t stream, Profile::Ev* ev)
         hipLaunchKernelGGL(crnd_index_chunk_kernel, dim3(B->wg_idx), dim3(256), 0, stream, B->d_jobs, B->njobs,
                            B->d_texs);
         HIPCHK(hipGetLastError());
-        const uint32_t lb = (uint32_t)((B->bnd_total + 255) / 256);
-        for (uint32_t r = 0; r < B->link_rounds; r++) {
-            hipLaunchKernelGGL(crnd_index_link_kernel, dim3(lb), dim3(256), 0, stream, B->d_jobs, B->njobs, B->d_texs,
-                               B->bnd_total, r & 1u);
+        if (B->ncarry) {
+            hipLaunchKernelGGL(crnd_index_carry_kernel, dim3(B->ncarry), dim3(kCarryThreads), 0, stream, B->d_jobs,
+                               B->d_carry, B->d_texs);
             HIPCHK(hipGetLastError());
         }
         const uint32_t fb = (uint32_t)((B->blk_total + 255) / 256);
         hipLaunchKernelGGL(crnd_index_final_kernel, dim3(fb), dim3(256), 0, stream, B->d_jobs, B->njobs, B->d_texs,
-                           B->blk_total, B->link_rounds & 1u);
+                           B->blk_total);
         HIPCHK(hipGetLastError());
         if (prof) HIPCHK(hipEventRecord(ev->e[5], stream));
     } else if (prof) {
