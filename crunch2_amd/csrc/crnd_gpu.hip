@@ -71,6 +71,11 @@ struct TexInfo {
     const uint16_t* cw2;         // kLutSize: per 11-bit prefix, DL1 | SL1 << 8 (second channel of kinds 1 and 2)
     int32_t* status;             // set to CRN2_ECORRUPT on an out-of-range palette index
     uint32_t kind, n_ce, n_cs, n_ae, n_as;
+    // Per code field (DL0, SL0, DL1, SL1): the non-decreasing canonical limits
+    // of lengths 12..16 (see long_len), and in bit f of lfast whether they
+    // give every long code's length exactly (the canonical walk otherwise).
+    uint32_t lthr[4][5];
+    uint32_t lfast;
 };
 
 // One level stream (all faces of one level of one texture).
@@ -263,6 +268,23 @@ struct TabCodes {
     uint8_t ref_sym[kTabRefSyms];
     uint32_t words[kTabWords];
 };
+
+// Length of the long code in window `top` of code field f: the first length
+// L >= 12 whose non-decreasing limit exceeds the window, 0 past the last one
+// -- five compares against workgroup-uniform limits, no branch and no LDS
+// (with alphabets of thousands of symbols a third or more of all windows
+// start a long selector code, so most waves hold one); 0xFF for a field whose
+// limits are not exact (the canonical walk then decides).
+__device__ __attribute__((always_inline)) inline uint32_t long_len(const TexInfo& tx, uint32_t f, uint32_t top)
+{
+    uint32_t L = kLutBits + 1;
+    L += top >= tx.lthr[f][0] ? 1u : 0u;
+    L += top >= tx.lthr[f][1] ? 1u : 0u;
+    L += top >= tx.lthr[f][2] ? 1u : 0u;
+    L += top >= tx.lthr[f][3] ? 1u : 0u;
+    L = top >= tx.lthr[f][4] ? 0u : L;
+    return (tx.lfast >> f) & 1u ? L : 0xFFu;
+}
 union TabShared {
     TabCodes c;          // pass 1
     uint32_t t[kTabT];   // passes 2 and 3
@@ -327,9 +349,20 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
             }
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) c[u] = s_u.c.cw1[top[u] >> (16 - kLutBits)];
+            // long delta / selector codes by compares (branch-free); the
+            // canonical walk only for a long group code or inexact limits
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) {
-                if (__builtin_expect((c[u] & 0xFFu) == 0 || (c[u] & 0xFF0000u) == 0 || (c[u] >> 24) == 0, 0)) {
+                const uint32_t nd = long_len(tx, 0, top[u]), ns = long_len(tx, 1, top[u]);
+                uint32_t dl = (c[u] >> 16) & 0xFFu, sl = c[u] >> 24;
+                dl = dl ? dl : nd;
+                sl = sl ? sl : ns;
+                c[u] = (c[u] & 0xFFFFu) | (dl << 16) | (sl << 24);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                if (__builtin_expect((c[u] & 0xFFu) == 0 || (c[u] & 0xFF0000u) == 0xFF0000u || (c[u] >> 24) == 0xFFu,
+                                     0)) {
                     uint32_t gl = c[u] & 0xFFu, g = (c[u] >> 8) & 0xFFu, dl = (c[u] >> 16) & 0xFFu, sl = c[u] >> 24;
                     uint32_t idx = 0;
                     if (!gl) {
@@ -337,8 +370,8 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
                         g = group_flags(!gl ? 0u : idx < kTabRefSyms ? (uint32_t)s_u.c.ref_sym[idx]
                                                                      : (uint32_t)(ref_sorted[idx] & 0xFFu));
                     }
-                    if (!dl) dl = long_code(D0, top[u], idx);
-                    if (!sl) sl = long_code(S0, top[u], idx);
+                    if (dl == 0xFFu) dl = long_code(D0, top[u], idx);
+                    if (sl == 0xFFu) sl = long_code(S0, top[u], idx);
                     c[u] = gl | (g << 8) | (dl << 16) | (sl << 24);
                 }
             }
@@ -356,9 +389,15 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < U; u++) {
+                    const uint32_t na = long_len(tx, 2, ta[u]), nb = long_len(tx, 3, tb[u]);
+                    a[u] = a[u] ? a[u] : na;
+                    b[u] = b[u] ? b[u] : nb;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) {
                     uint32_t idx = 0;
-                    if (__builtin_expect(!a[u], 0)) a[u] = long_code(D1, ta[u], idx);
-                    if (__builtin_expect(!b[u], 0)) b[u] = long_code(S1, tb[u], idx);
+                    if (__builtin_expect(a[u] == 0xFFu, 0)) a[u] = long_code(D1, ta[u], idx);
+                    if (__builtin_expect(b[u] == 0xFFu, 0)) b[u] = long_code(S1, tb[u], idx);
                     c[u] += (a[u] << 16) + (b[u] << 24);
                 }
             }
@@ -1259,6 +1298,7 @@ struct crn2_unpack_ctx {
     uint16_t* d_sorted = nullptr;
     uint32_t* d_cw1 = nullptr;   // TexInfo::cw1
     uint16_t* d_cw2 = nullptr;   // TexInfo::cw2
+    uint32_t lthr[4][5] = {}, lfast = 0;   // TexInfo::lthr, lfast
     uint32_t* d_ce = nullptr;
     uint32_t* d_cs = nullptr;
     uint16_t* d_ae = nullptr;
@@ -1475,6 +1515,40 @@ static crn2_unpack_ctx* begin_impl(const void* data, const void* data_dev, uint3
             cw2[v] = (uint16_t)(dms[M_EP1].len8[v] | (dms[M_SEL1].len8[v] << 8));
         }
     }
+    // Long-code limits of the fields DL0, SL0, DL1, SL1 (TexInfo::lthr) and
+    // whether they reproduce the canonical walk (crn_decomp.h:3239-3244) on
+    // every window the direct table leaves open (TexInfo::lfast).
+    {
+        const uint32_t fm[4] = {c->kind <= 1 ? M_EP0 : M_EP1, c->kind <= 1 ? M_SEL0 : M_SEL1, M_EP1, M_SEL1};
+        c->lfast = 0;
+        for (uint32_t f = 0; f < 4; f++) {
+            const DevModel& dm = dms[fm[f]];
+            uint32_t m = 0;
+            for (uint32_t L = kLutBits + 1; L <= 16; L++) {
+                m = std::max(m, dm.limit[L]);
+                c->lthr[f][L - kLutBits - 1] = m;
+            }
+            // The compares give the canonical walk's length for every window
+            // (the limits are its own, made non-decreasing); what remains is
+            // its index check, idx = base[L] + code in [0, num_syms), and idx
+            // grows with the window, so the two ends of each length's windows
+            // (from the first one the direct table leaves open) decide.
+            uint32_t h = 0;
+            while (h < kLutSize && dm.len8[h]) h++;
+            bool exact = true;
+            uint32_t lo = h << (16 - kLutBits);
+            for (uint32_t L = kLutBits + 1; L <= 16; L++) {
+                const uint32_t hi = c->lthr[f][L - kLutBits - 1];
+                if (lo < hi) {
+                    const int64_t i0 = (int64_t)dm.base[L] + (lo >> (16 - L));
+                    const int64_t i1 = (int64_t)dm.base[L] + ((hi - 1) >> (16 - L));
+                    exact &= i0 >= 0 && i1 < (int64_t)dm.num_syms;
+                }
+                lo = std::max(lo, hi);
+            }
+            if (exact) c->lfast |= 1u << f;
+        }
+    }
     uint64_t rec = 0;
     const uint32_t rec_bytes = (c->kind == 1 || c->kind == 2) ? 8 : 4;
     for (uint32_t l = 0; l < h.levels; l++) {
@@ -1561,6 +1635,8 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
         t.sorted = c->d_sorted;
         t.cw1 = c->d_cw1;
         t.cw2 = c->d_cw2;
+        memcpy(t.lthr, c->lthr, sizeof(t.lthr));
+        t.lfast = c->lfast;
         t.status = c->d_status;
         t.kind = (uint32_t)c->kind;
         t.n_ce = c->n_ce;
