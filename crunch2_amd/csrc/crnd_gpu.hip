@@ -994,8 +994,11 @@ __device__ inline uint64_t jump(uint64_t e, uint64_t ep, const IdxSizes& z)
     return (ep & 0xFFFFFFFFull) | ((uint64_t)add_res((uint32_t)(e >> 32), (uint32_t)(ep >> 32), z.n0, z.n1) << 32);
 }
 
-__global__ void __launch_bounds__(256) crnd_index_chunk_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
-                                                               const TexInfo* __restrict__ texs)
+constexpr uint32_t kIdxThreads = 1024;
+constexpr uint32_t kIdxPer = kIdxChunk / kIdxThreads;   // entries per thread, in flight together
+
+__global__ void __launch_bounds__(kIdxThreads) crnd_index_chunk_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
+                                                                       const TexInfo* __restrict__ texs)
 {
     __shared__ uint64_t L[2][kIdxChunk];
     const uint32_t tid = threadIdx.x;
@@ -1011,7 +1014,10 @@ __global__ void __launch_bounds__(256) crnd_index_chunk_kernel(const LevelJob* _
     const uint32_t c = blockIdx.x - j.wg_idx;
     const uint32_t r0 = c * j.crows, r1 = min(rows, r0 + j.crows);
     const uint32_t b0 = r0 * gw, nb = (r1 - r0) * gw;
-    for (uint32_t i = tid; i < nb; i += 256) {
+#pragma unroll
+    for (uint32_t k = 0; k < kIdxPer; k++) {
+        const uint32_t i = tid + k * kIdxThreads;
+        if (i >= nb) continue;
         const uint32_t b = b0 + i;
         const uint64_t q = j.pre[b];
         const uint32_t ref = (uint32_t)q & 3u;
@@ -1029,17 +1035,31 @@ __global__ void __launch_bounds__(256) crnd_index_chunk_kernel(const LevelJob* _
     __syncthreads();
     uint32_t cur = 0;
     for (uint32_t span = 1; span < nb; span <<= 1) {
-        for (uint32_t i = tid; i < nb; i += 256) {
-            uint64_t e = L[cur][i];
-            const uint32_t pa = (uint32_t)e;
-            if (pa != kRoot && pa >= b0) e = jump(e, L[cur][pa - b0], z);
-            L[cur ^ 1][i] = e;
+        uint64_t e[kIdxPer], ep[kIdxPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kIdxPer; k++) {
+            const uint32_t i = min(tid + k * kIdxThreads, nb - 1);
+            e[k] = L[cur][i];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kIdxPer; k++) {
+            const uint32_t pa = (uint32_t)e[k];
+            ep[k] = (pa != kRoot && pa >= b0) ? L[cur][pa - b0] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kIdxPer; k++) {
+            const uint32_t i = tid + k * kIdxThreads;
+            const uint32_t pa = (uint32_t)e[k];
+            if (i < nb) L[cur ^ 1][i] = (pa != kRoot && pa >= b0) ? jump(e[k], ep[k], z) : e[k];
         }
         __syncthreads();
         cur ^= 1;
     }
     const uint32_t last = nb - gw;
-    for (uint32_t i = tid; i < nb; i += 256) {
+#pragma unroll
+    for (uint32_t k = 0; k < kIdxPer; k++) {
+        const uint32_t i = tid + k * kIdxThreads;
+        if (i >= nb) continue;
         const uint64_t e = L[cur][i];
         j.ent[b0 + i] = e;
         if (i >= last) {
@@ -1807,7 +1827,7 @@ static int launch_plan(crn2_batch* B, hipStream_t stream, Profile::Ev* ev)
                            B->d_texs);
         HIPCHK(hipGetLastError());
         if (prof) HIPCHK(hipEventRecord(ev->e[4], stream));
-        hipLaunchKernelGGL(crnd_index_chunk_kernel, dim3(B->wg_idx), dim3(256), 0, stream, B->d_jobs, B->njobs,
+        hipLaunchKernelGGL(crnd_index_chunk_kernel, dim3(B->wg_idx), dim3(kIdxThreads), 0, stream, B->d_jobs, B->njobs,
                            B->d_texs);
         HIPCHK(hipGetLastError());
         if (B->ncarry) {
