@@ -192,15 +192,15 @@ __device__ inline uint32_t decode_top(const LdsTables& t, const uint16_t* sorted
 //   2. per position: the one-unit transitions from CW -- bits of an even-row
 //      pair (group symbol, then per block the deltas when its reference code
 //      is 0, then the selectors) and of an odd-row block without / with deltas
-//      (T = pair | odd flags << 8 | odd0 << 16 | odd1 << 24; a pair takes at
-//      most 144 bits, a block 64, a code 16);
+//      (T = pair | odd flags << 8 | odd0 << 10 in 16 bits, odd1 in 8: a pair
+//      takes at most 144 bits, a block 64 (32 without deltas), a code 16);
 //   3. the composed entries above from T (eight pairs reach 1008 entries on).
 // Each pass runs past the tile by what the next one reads.  Every thread
 // keeps several positions' lookup chains in flight at once (kTabU1..3, the
 // positions tid + 512 k), so that the dependent LDS reads of one position
 // overlap those of the others; the pass-1 code tables and the T array share
-// LDS (the tables are dead once pass 1 ends), 41,600 B per workgroup: three
-// workgroups per CU.
+// LDS (the tables are dead once pass 1 ends), 36,480 B per workgroup: three
+// workgroups per CU beside a stream walker's 40 KiB ring.
 constexpr uint32_t kTabPerWG = 4096;
 constexpr uint32_t kTabT = kTabPerWG + 1024;
 constexpr uint32_t kTabCW = kTabT + 160;
@@ -287,7 +287,10 @@ __device__ __attribute__((always_inline)) inline uint32_t long_len(const TexInfo
 }
 union TabShared {
     TabCodes c;          // pass 1
-    uint32_t t[kTabT];   // passes 2 and 3
+    struct {
+        uint16_t lo[kTabT];   // passes 2 and 3: pair bits | odd flags << 8 | bits of an odd block without deltas << 10
+        uint8_t hi[kTabT];    //                 bits of an odd block with deltas
+    } t;
 };
 
 // tab_job[wg] = the level job of tables workgroup wg.
@@ -411,7 +414,8 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
     // 2. one-unit transitions (branch-free: a block without deltas re-reads
     // the entry it already has)
     {
-        uint32_t* T = s_u.t;
+        uint16_t* T = s_u.t.lo;
+        uint8_t* T8 = s_u.t.hi;
         constexpr uint32_t U = kTabU2;
 #pragma unroll 1
         for (uint32_t i0 = 0; i0 < 2 * U; i0 += U) {
@@ -446,7 +450,8 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
             for (uint32_t u = 0; u < U; u++) {
                 e[u] += x[u] >> 24;
                 const uint32_t d = (c[u] >> 16) & 0xFFu;
-                T[r[u]] = (e[u] - r[u]) | ((c[u] >> 2) & 0x300u) | ((c[u] >> 24) << 16) | ((d + (od[u] >> 24)) << 24);
+                T[r[u]] = (uint16_t)((e[u] - r[u]) | ((c[u] >> 2) & 0x300u) | ((c[u] >> 24) << 10));
+                T8[r[u]] = (uint8_t)(d + (od[u] >> 24));
             }
         }
     }
@@ -454,7 +459,8 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
 
     // 3. composed entries (stores at 32-bit offsets from the tile's base)
     {
-        const uint32_t* T = s_u.t;
+        const uint16_t* T = s_u.t.lo;
+        const uint8_t* T8 = s_u.t.hi;
         typedef __attribute__((address_space(1))) uint8_t guint8;
         guint8* O2 = (guint8*)(j.O2 + P0);
         guint8* E8 = (guint8*)(j.E8 + P0);
@@ -462,16 +468,19 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
         constexpr uint32_t U = kTabU3;
 #pragma unroll 1
         for (uint32_t i0 = 0; i0 < 2 * U; i0 += U) {
-            uint32_t r[U], t0[U], ta[U], tb[U], p[U], fl[U];
+            uint32_t r[U], t0[U], o1[U], ta[U], tb[U], ta8[U], tb8[U], p[U], fl[U];
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) {
                 r[u] = tid + (i0 + u) * kTabThreads;
                 t0[u] = T[r[u]];
+                o1[u] = T8[r[u]];
             }
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) {
-                ta[u] = T[r[u] + ((t0[u] >> 16) & 0xFFu)];
-                tb[u] = T[r[u] + (t0[u] >> 24)];
+                ta[u] = T[r[u] + (t0[u] >> 10)];
+                ta8[u] = T8[r[u] + (t0[u] >> 10)];
+                tb[u] = T[r[u] + o1[u]];
+                tb8[u] = T8[r[u] + o1[u]];
                 p[u] = r[u] + (t0[u] & 0xFFu);
                 fl[u] = t0[u] & 0x300u;
             }
@@ -487,9 +496,9 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
             }
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) {
-                const uint32_t o0 = (t0[u] >> 16) & 0xFFu, o1 = t0[u] >> 24;
-                *(guint32*)(O2 + 4 * r[u]) = (o0 + ((ta[u] >> 16) & 0xFFu)) | ((o1 + ((tb[u] >> 16) & 0xFFu)) << 8) |
-                                             ((o0 + (ta[u] >> 24)) << 16) | ((o1 + (tb[u] >> 24)) << 24);
+                const uint32_t o0 = t0[u] >> 10;
+                *(guint32*)(O2 + 4 * r[u]) = (o0 + (ta[u] >> 10)) | ((o1[u] + (tb[u] >> 10)) << 8) |
+                                             ((o0 + ta8[u]) << 16) | ((o1[u] + tb8[u]) << 24);
                 *(guint32*)(E8 + 4 * r[u]) = (p[u] - r[u]) | ((fl[u] >> 8) << 11);
                 *(guint16*)(X + 2 * r[u]) = (uint16_t)(t0[u] & 0x3FFu);
             }
