@@ -67,15 +67,16 @@ static bool kind_uses(int kind, uint32_t m)
 struct TexInfo {
     const DevModel* models;      // M_COUNT models (device)
     const uint16_t* sorted;      // M_COUNT x kMaxSyms canonical symbol orders
-    const uint32_t* cw1;         // kLutSize: per 11-bit prefix, GL | F << 8 | DL0 << 16 | SL0 << 24 (see crnd_tables_kernel)
-    const uint16_t* cw2;         // kLutSize: per 11-bit prefix, DL1 | SL1 << 8 (second channel of kinds 1 and 2)
+    // Per 11-bit prefix (crnd_tables_kernel): cw1 = GL | F << 8 | DL0 << 16 |
+    // SL0 << 24, cw2 = DL1 | SL1 << 8 (second channel of kinds 1 and 2).  GL
+    // is 0 when the group code is longer than the prefix; a DL / SL field is
+    // the code length that every window with the prefix starts (0: no code
+    // matches) or 0xFF where the windows differ, which the canonical walk
+    // then decides.
+    const uint32_t* cw1;
+    const uint16_t* cw2;
     int32_t* status;             // set to CRN2_ECORRUPT on an out-of-range palette index
     uint32_t kind, n_ce, n_cs, n_ae, n_as;
-    // Per code field (DL0, SL0, DL1, SL1): the non-decreasing canonical limits
-    // of lengths 12..16 (see long_len), and in bit f of lfast whether they
-    // give every long code's length exactly (the canonical walk otherwise).
-    uint32_t lthr[4][5];
-    uint32_t lfast;
 };
 
 // One level stream (all faces of one level of one texture).
@@ -269,22 +270,6 @@ struct TabCodes {
     uint32_t words[kTabWords];
 };
 
-// Length of the long code in window `top` of code field f: the first length
-// L >= 12 whose non-decreasing limit exceeds the window, 0 past the last one
-// -- five compares against workgroup-uniform limits, no branch and no LDS
-// (with alphabets of thousands of symbols a third or more of all windows
-// start a long selector code, so most waves hold one); 0xFF for a field whose
-// limits are not exact (the canonical walk then decides).
-__device__ __attribute__((always_inline)) inline uint32_t long_len(const TexInfo& tx, uint32_t f, uint32_t top)
-{
-    uint32_t L = kLutBits + 1;
-    L += top >= tx.lthr[f][0] ? 1u : 0u;
-    L += top >= tx.lthr[f][1] ? 1u : 0u;
-    L += top >= tx.lthr[f][2] ? 1u : 0u;
-    L += top >= tx.lthr[f][3] ? 1u : 0u;
-    L = top >= tx.lthr[f][4] ? 0u : L;
-    return (tx.lfast >> f) & 1u ? L : 0xFFu;
-}
 union TabShared {
     TabCodes c;          // pass 1
     struct {
@@ -352,16 +337,10 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
             }
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) c[u] = s_u.c.cw1[top[u] >> (16 - kLutBits)];
-            // long delta / selector codes by compares (branch-free); the
-            // canonical walk only for a long group code or inexact limits
-#pragma unroll
-            for (uint32_t u = 0; u < U; u++) {
-                const uint32_t nd = long_len(tx, 0, top[u]), ns = long_len(tx, 1, top[u]);
-                uint32_t dl = (c[u] >> 16) & 0xFFu, sl = c[u] >> 24;
-                dl = dl ? dl : nd;
-                sl = sl ? sl : ns;
-                c[u] = (c[u] & 0xFFFFu) | (dl << 16) | (sl << 24);
-            }
+            // the canonical walk only for a long group code or a prefix whose
+            // windows start delta / selector codes of different lengths (at most
+            // five prefixes per model: where a length's codes end off a
+            // 32-window boundary)
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) {
                 if (__builtin_expect((c[u] & 0xFFu) == 0 || (c[u] & 0xFF0000u) == 0xFF0000u || (c[u] >> 24) == 0xFFu,
@@ -389,12 +368,6 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
                 for (uint32_t u = 0; u < U; u++) {
                     a[u] = s_u.c.cw2[ta[u] >> (16 - kLutBits)] & 0xFFu;
                     b[u] = s_u.c.cw2[tb[u] >> (16 - kLutBits)] >> 8;
-                }
-#pragma unroll
-                for (uint32_t u = 0; u < U; u++) {
-                    const uint32_t na = long_len(tx, 2, ta[u]), nb = long_len(tx, 3, tb[u]);
-                    a[u] = a[u] ? a[u] : na;
-                    b[u] = b[u] ? b[u] : nb;
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < U; u++) {
@@ -1327,7 +1300,6 @@ struct crn2_unpack_ctx {
     uint16_t* d_sorted = nullptr;
     uint32_t* d_cw1 = nullptr;   // TexInfo::cw1
     uint16_t* d_cw2 = nullptr;   // TexInfo::cw2
-    uint32_t lthr[4][5] = {}, lfast = 0;   // TexInfo::lthr, lfast
     uint32_t* d_ce = nullptr;
     uint32_t* d_cs = nullptr;
     uint16_t* d_ae = nullptr;
@@ -1530,52 +1502,62 @@ static crn2_unpack_ctx* begin_impl(const void* data, const void* data_dev, uint3
             memset(&dms[m], 0, sizeof(DevModel));
         }
     }
-    // Per 11-bit prefix: the reference code's length and group symbol and the
-    // lengths of one block's first-channel delta and selector codes (a zero
-    // field: the code is longer than kLutBits); second-channel lengths apart.
+    // Per 11-bit prefix (TexInfo::cw1 / cw2): the reference code's length and
+    // group flags, and per delta / selector field the length every window of
+    // the prefix starts.  Codes longer than the prefix: a model's canonical
+    // limits of lengths 12..16, made non-decreasing, give each window's length
+    // as 12 + the number of limits at or below it (0 from the last limit on),
+    // which is the canonical walk's (crn_decomp.h:3239-3244) whenever its index
+    // check holds for the first and last window of each length -- the index
+    // grows with the window; a prefix whose first and last windows agree then
+    // starts that length throughout, the others (at most one per length
+    // boundary) are marked 0xFF for the walk, and so is every long prefix of a
+    // model whose index check can fail.
+    auto field_lengths = [&](uint32_t m, uint8_t* out) {
+        const DevModel& dm = dms[m];
+        uint32_t lim[5], mx = 0;
+        for (uint32_t L = kLutBits + 1; L <= 16; L++) lim[L - kLutBits - 1] = mx = std::max(mx, dm.limit[L]);
+        auto len_at = [&](uint32_t top) -> uint32_t {
+            if (top >= lim[4]) return 0;
+            uint32_t L = kLutBits + 1;
+            for (uint32_t i = 0; i < 4; i++) L += top >= lim[i] ? 1u : 0u;
+            return L;
+        };
+        uint32_t h = 0;
+        while (h < kLutSize && dm.len8[h]) h++;
+        bool exact = true;
+        uint32_t lo = h << (16 - kLutBits);
+        for (uint32_t L = kLutBits + 1; L <= 16; L++) {
+            const uint32_t hi = lim[L - kLutBits - 1];
+            if (lo < hi) {
+                const int64_t i0 = (int64_t)dm.base[L] + (lo >> (16 - L));
+                const int64_t i1 = (int64_t)dm.base[L] + ((hi - 1) >> (16 - L));
+                exact &= i0 >= 0 && i1 < (int64_t)dm.num_syms;
+            }
+            lo = std::max(lo, hi);
+        }
+        for (uint32_t v = 0; v < kLutSize; v++) {
+            if (dm.len8[v]) {
+                out[v] = dm.len8[v];
+                continue;
+            }
+            const uint32_t w = v << (16 - kLutBits), a = len_at(w), b = len_at(w + (1u << (16 - kLutBits)) - 1);
+            out[v] = (exact && a == b) ? (uint8_t)a : (uint8_t)0xFF;
+        }
+    };
     std::vector<uint32_t> cw1(kLutSize);
     std::vector<uint16_t> cw2(kLutSize);
     {
         const uint32_t d0 = c->kind <= 1 ? M_EP0 : M_EP1, s0 = c->kind <= 1 ? M_SEL0 : M_SEL1;
+        std::vector<uint8_t> fd0(kLutSize), fs0(kLutSize), fd1(kLutSize), fs1(kLutSize);
+        field_lengths(d0, fd0.data());
+        field_lengths(s0, fs0.data());
+        field_lengths(M_EP1, fd1.data());
+        field_lengths(M_SEL1, fs1.data());
         for (uint32_t v = 0; v < kLutSize; v++) {
             const uint32_t r = dms[M_REF].lut[v];
-            cw1[v] = (r >> 16) | (group_flags(r & 0xFFu) << 8) | ((uint32_t)dms[d0].len8[v] << 16) |
-                     ((uint32_t)dms[s0].len8[v] << 24);
-            cw2[v] = (uint16_t)(dms[M_EP1].len8[v] | (dms[M_SEL1].len8[v] << 8));
-        }
-    }
-    // Long-code limits of the fields DL0, SL0, DL1, SL1 (TexInfo::lthr) and
-    // whether they reproduce the canonical walk (crn_decomp.h:3239-3244) on
-    // every window the direct table leaves open (TexInfo::lfast).
-    {
-        const uint32_t fm[4] = {c->kind <= 1 ? M_EP0 : M_EP1, c->kind <= 1 ? M_SEL0 : M_SEL1, M_EP1, M_SEL1};
-        c->lfast = 0;
-        for (uint32_t f = 0; f < 4; f++) {
-            const DevModel& dm = dms[fm[f]];
-            uint32_t m = 0;
-            for (uint32_t L = kLutBits + 1; L <= 16; L++) {
-                m = std::max(m, dm.limit[L]);
-                c->lthr[f][L - kLutBits - 1] = m;
-            }
-            // The compares give the canonical walk's length for every window
-            // (the limits are its own, made non-decreasing); what remains is
-            // its index check, idx = base[L] + code in [0, num_syms), and idx
-            // grows with the window, so the two ends of each length's windows
-            // (from the first one the direct table leaves open) decide.
-            uint32_t h = 0;
-            while (h < kLutSize && dm.len8[h]) h++;
-            bool exact = true;
-            uint32_t lo = h << (16 - kLutBits);
-            for (uint32_t L = kLutBits + 1; L <= 16; L++) {
-                const uint32_t hi = c->lthr[f][L - kLutBits - 1];
-                if (lo < hi) {
-                    const int64_t i0 = (int64_t)dm.base[L] + (lo >> (16 - L));
-                    const int64_t i1 = (int64_t)dm.base[L] + ((hi - 1) >> (16 - L));
-                    exact &= i0 >= 0 && i1 < (int64_t)dm.num_syms;
-                }
-                lo = std::max(lo, hi);
-            }
-            if (exact) c->lfast |= 1u << f;
+            cw1[v] = (r >> 16) | (group_flags(r & 0xFFu) << 8) | ((uint32_t)fd0[v] << 16) | ((uint32_t)fs0[v] << 24);
+            cw2[v] = (uint16_t)(fd1[v] | (fs1[v] << 8));
         }
     }
     uint64_t rec = 0;
@@ -1664,8 +1646,6 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
         t.sorted = c->d_sorted;
         t.cw1 = c->d_cw1;
         t.cw2 = c->d_cw2;
-        memcpy(t.lthr, c->lthr, sizeof(t.lthr));
-        t.lfast = c->lfast;
         t.status = c->d_status;
         t.kind = (uint32_t)c->kind;
         t.n_ce = c->n_ce;
