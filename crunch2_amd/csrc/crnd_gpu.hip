@@ -1202,11 +1202,12 @@ __global__ void __launch_bounds__(256) crnd_write_kernel(const WriteJob* __restr
     if (t >= total) return;
     const WriteJob& j = jobs[find_job_wg(jobs, njobs, t, [](const WriteJob& x) { return x.first_block; })];
     const WritePalettes& P = pals[j.pal];
-    uint64_t v = t - j.first_block;
+    // a job's blocks number fewer than 2^32 (plans refuse larger grids): 32-bit division
+    uint32_t v = (uint32_t)(t - j.first_block);
     const uint32_t per_face = j.bx * j.by;
-    const uint32_t f = (uint32_t)(v / per_face);
-    v -= (uint64_t)f * per_face;
-    const uint32_t y = (uint32_t)(v / j.bx), x = (uint32_t)(v % j.bx);
+    const uint32_t f = v / per_face;
+    v -= f * per_face;
+    const uint32_t y = v / j.bx, x = v - y * j.bx;
     const uint64_t r = (uint64_t)(f * j.gh + y) * j.gw + x;
     uint8_t* out = j.dst[f] + (uint64_t)y * j.pitch;
     if (P.kind == 0 || P.kind == 3) {
