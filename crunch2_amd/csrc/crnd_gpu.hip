@@ -156,7 +156,9 @@ __device__ inline void stage_tables(LdsTables& t, const DevModel& dm, uint32_t t
 // codes up to kLutBits bits, longer codes take the canonical walk.  A bit
 // pattern that matches no code gives symbol 0 and consumes nothing
 // (crn_decomp.h:3239-3244).
-__device__ inline uint32_t decode_top(const LdsTables& t, const uint16_t* sorted, uint32_t top, uint32_t& len)
+typedef __attribute__((address_space(1))) const uint16_t gcuint16;
+
+__device__ inline uint32_t decode_top(const LdsTables& t, gcuint16* sorted, uint32_t top, uint32_t& len)
 {
     const uint32_t e = t.lut[top >> (16 - kLutBits)];
     if (e) {
@@ -797,24 +799,45 @@ __global__ void __launch_bounds__(128) crnd_walk_kernel(const LevelJob* __restri
 // code, 2-15 first endpoint delta, 16-29 second endpoint delta, 30-43 first
 // selector, 44-57 second selector.
 
-// MSB-first reader starting at any bit; aligned 32-bit loads, bytes at or past
-// `end` read as zero (crn_decomp.h:3196-3207).
+// MSB-first reader starting at any bit; bytes at or past `end` read as zero
+// (crn_decomp.h:3196-3207).  The stream is fetched as aligned 16-byte vectors
+// kept three deep in registers (the current one and two ahead) with global
+// (not flat) loads, so a refill rarely waits on memory and no LDS wait waits
+// for a fetch.  A vector is loaded only if it starts before `end`; level
+// streams lie in buffers padded by at least 64 bytes past their last 16-byte
+// boundary.
 struct WordBits {
-    const uint32_t* wp;
+    typedef unsigned int v4 __attribute__((ext_vector_type(4)));
+    const uint8_t* vnext;   // byte address of the next vector to fetch
     const uint8_t* end;
+    v4 q0, q1, q2;
+    const uint8_t* wa;      // byte address of the next word to take
+    uint32_t qi;            // its index in q0
     uint64_t buf;
     uint32_t cnt;
 
-    __device__ inline uint32_t load_word()
+    __device__ inline v4 fetch()
     {
-        const uint8_t* a = reinterpret_cast<const uint8_t*>(wp);
-        uint32_t v = 0;
-        if (a < end) {
-            v = __builtin_bswap32(*wp);
-            const uintptr_t left = (uintptr_t)(end - a);
-            if (left < 4) v &= 0xFFFFFFFFu << (8 * (4 - (uint32_t)left));
+        typedef __attribute__((address_space(1))) const v4 gv4;
+        v4 v = {0u, 0u, 0u, 0u};
+        if (vnext < end) v = *(gv4*)vnext;
+        vnext += 16;
+        return v;
+    }
+
+    __device__ inline uint32_t next_word()
+    {
+        uint32_t v = qi == 0 ? q0.x : qi == 1 ? q0.y : qi == 2 ? q0.z : q0.w;
+        v = __builtin_bswap32(v);
+        if (wa >= end) v = 0;
+        else if (end - wa < 4) v &= 0xFFFFFFFFu << (8 * (4 - (uint32_t)(end - wa)));
+        wa += 4;
+        if (++qi == 4) {
+            qi = 0;
+            q0 = q1;
+            q1 = q2;
+            q2 = fetch();
         }
-        wp++;
         return v;
     }
 
@@ -822,18 +845,23 @@ struct WordBits {
     {
         end = src + size;
         const uintptr_t ab = reinterpret_cast<uintptr_t>(src) + (bit >> 3);
-        wp = reinterpret_cast<const uint32_t*>(ab & ~(uintptr_t)3);
+        vnext = reinterpret_cast<const uint8_t*>(ab & ~(uintptr_t)15);
+        wa = reinterpret_cast<const uint8_t*>(ab & ~(uintptr_t)3);
+        qi = (uint32_t)(ab >> 2) & 3u;
+        q0 = fetch();
+        q1 = fetch();
+        q2 = fetch();
         const uint32_t skip = (uint32_t)(ab & 3) * 8 + (bit & 7);
-        const uint32_t w0 = load_word();
-        const uint32_t w1 = load_word();
+        const uint32_t w0 = next_word();
+        const uint32_t w1 = next_word();
         buf = (((uint64_t)w0 << 32) | w1) << skip;
         cnt = 64 - skip;
     }
 
-    __device__ inline uint32_t decode(const LdsTables& t, const uint16_t* sorted)
+    __device__ inline uint32_t decode(const LdsTables& t, gcuint16* sorted)
     {
         if (cnt <= 32) {
-            buf |= (uint64_t)load_word() << (32 - cnt);
+            buf |= (uint64_t)next_word() << (32 - cnt);
             cnt += 32;
         }
         uint32_t len;
@@ -848,7 +876,7 @@ constexpr uint32_t kSymGroupsPerWG = 256;
 
 struct SymCtx {
     const LdsTables* t;
-    const uint16_t* so;
+    gcuint16* so;
     uint32_t dm0, sm0;
     bool two;
 
@@ -885,34 +913,57 @@ __global__ void __launch_bounds__(256) crnd_sym_kernel(const LevelJob* __restric
         if (kind_uses_dev(kind, m)) stage_tables(s_t[m], tx.models[m], tid, 256);
     __syncthreads();
 
-    SymCtx sc{s_t, tx.sorted, d_model0(kind), s_model0(kind), two_channels(kind)};
+    gcuint16* gsorted = (gcuint16*)tx.sorted;
+    SymCtx sc{s_t, gsorted, d_model0(kind), s_model0(kind), two_channels(kind)};
     const uint32_t gw = j.gw, np = gw >> 1;
     const uint32_t rowpairs = j.faces * (j.gh >> 1);
     const uint32_t gpr = ODD ? (gw + kGroup - 1) / kGroup : (np + kGroup - 1) / kGroup;   // groups per row
     const uint32_t it = (blockIdx.x - (ODD ? j.wg_odd : j.wg_sym)) * kSymGroupsPerWG + tid;
     if (it >= rowpairs * gpr) return;
     const uint32_t rp = it / gpr, grp = it - rp * gpr;
+    // A group's records stay in registers (the unrolled loops index them
+    // statically) and are stored once at its end: vmcnt counts stores too, so
+    // a store inside the loop would make the next stream refill wait for it.
+    typedef __attribute__((address_space(1))) uint64_t guint64;
+    typedef __attribute__((address_space(1))) uint8_t guint8;
     WordBits b;
+    uint64_t rec[2 * kGroup];
     if (!ODD) {
         b.start(j.src, j.src_size, j.posE[it]);
-        const uint32_t u0 = grp * kGroup, u1 = min(np, u0 + kGroup);
-        uint64_t* pre = j.pre + (uint64_t)(2 * rp) * gw;
-        uint8_t* gv = j.gvals + (uint64_t)rp * np;
-        for (uint32_t u = u0; u < u1; u++) {
-            const uint32_t G = b.decode(s_t[M_REF], tx.sorted + M_REF * kMaxSyms) & 0xFFu;
-            gv[u] = (uint8_t)G;
-            pre[2 * u] = sc.block(b, G & 3u);
-            pre[2 * u + 1] = sc.block(b, (G >> 4) & 3u);
+        const uint32_t u0 = grp * kGroup, nu = min(np - u0, kGroup);
+        uint32_t g[kGroup];
+#pragma unroll
+        for (uint32_t k = 0; k < kGroup; k++) {
+            if (k < nu) {
+                g[k] = b.decode(s_t[M_REF], gsorted + M_REF * kMaxSyms) & 0xFFu;
+                rec[2 * k] = sc.block(b, g[k] & 3u);
+                rec[2 * k + 1] = sc.block(b, (g[k] >> 4) & 3u);
+            }
+        }
+        guint64* pre = (guint64*)(j.pre + (uint64_t)(2 * rp) * gw + 2 * u0);
+        guint8* gv = (guint8*)(j.gvals + (uint64_t)rp * np + u0);
+#pragma unroll
+        for (uint32_t k = 0; k < kGroup; k++) {
+            if (k < nu) {
+                pre[2 * k] = rec[2 * k];
+                pre[2 * k + 1] = rec[2 * k + 1];
+                gv[k] = (uint8_t)g[k];
+            }
         }
     } else {
         b.start(j.src, j.src_size, j.posO[it]);
-        const uint32_t x0 = grp * kGroup, x1 = min(gw, x0 + kGroup);
-        uint64_t* pre = j.pre + (uint64_t)(2 * rp + 1) * gw;
-        const uint8_t* gv = j.gvals + (uint64_t)rp * np;
-        for (uint32_t x = x0; x < x1; x++) {
-            const uint32_t G = gv[x >> 1];
-            pre[x] = sc.block(b, (G >> (2 + 4 * (x & 1u))) & 3u);
-        }
+        const uint32_t x0 = grp * kGroup, nx = min(gw - x0, kGroup);
+        const guint8* gv = (const guint8*)(j.gvals + (uint64_t)rp * np + (x0 >> 1));
+        uint32_t g[kGroup / 2];
+#pragma unroll
+        for (uint32_t k = 0; k < kGroup / 2; k++) g[k] = 2 * k < nx ? (uint32_t)gv[k] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kGroup; k++)
+            if (k < nx) rec[k] = sc.block(b, (g[k >> 1] >> (2 + 4 * (k & 1u))) & 3u);
+        guint64* pre = (guint64*)(j.pre + (uint64_t)(2 * rp + 1) * gw + x0);
+#pragma unroll
+        for (uint32_t k = 0; k < kGroup; k++)
+            if (k < nx) pre[k] = rec[k];
     }
 }
 
