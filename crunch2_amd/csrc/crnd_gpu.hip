@@ -918,16 +918,19 @@ __global__ void __launch_bounds__(256) crnd_sym_kernel(const LevelJob* __restric
     const uint32_t gw = j.gw, np = gw >> 1;
     const uint32_t rowpairs = j.faces * (j.gh >> 1);
     const uint32_t gpr = ODD ? (gw + kGroup - 1) / kGroup : (np + kGroup - 1) / kGroup;   // groups per row
+    // odd rows: one thread per two consecutive groups of a row, decoded in one
+    // run from the first group's start bit (the second starts where it ends)
+    const uint32_t tpr = ODD ? (gpr + 1) / 2 : gpr;   // threads per row pair
     const uint32_t it = (blockIdx.x - (ODD ? j.wg_odd : j.wg_sym)) * kSymGroupsPerWG + tid;
-    if (it >= rowpairs * gpr) return;
-    const uint32_t rp = it / gpr, grp = it - rp * gpr;
+    if (it >= rowpairs * tpr) return;
+    const uint32_t rp = it / tpr, grp = ODD ? 2 * (it - rp * tpr) : it - rp * tpr;
     // A group's records stay in registers (the unrolled loops index them
     // statically) and are stored once at its end: vmcnt counts stores too, so
     // a store inside the loop would make the next stream refill wait for it.
     typedef __attribute__((address_space(1))) uint64_t guint64;
     typedef __attribute__((address_space(1))) uint8_t guint8;
     WordBits b;
-    uint64_t rec[2 * kGroup];
+    uint64_t rec[2 * kGroup];   // eight pairs, or two groups of eight odd-row blocks
     if (!ODD) {
         b.start(j.src, j.src_size, j.posE[it]);
         const uint32_t u0 = grp * kGroup, nu = min(np - u0, kGroup);
@@ -951,18 +954,18 @@ __global__ void __launch_bounds__(256) crnd_sym_kernel(const LevelJob* __restric
             }
         }
     } else {
-        b.start(j.src, j.src_size, j.posO[it]);
-        const uint32_t x0 = grp * kGroup, nx = min(gw - x0, kGroup);
+        b.start(j.src, j.src_size, j.posO[rp * gpr + grp]);
+        const uint32_t x0 = grp * kGroup, nx = min(gw - x0, 2 * kGroup);
         const guint8* gv = (const guint8*)(j.gvals + (uint64_t)rp * np + (x0 >> 1));
-        uint32_t g[kGroup / 2];
+        uint32_t g[kGroup];
 #pragma unroll
-        for (uint32_t k = 0; k < kGroup / 2; k++) g[k] = 2 * k < nx ? (uint32_t)gv[k] : 0u;
+        for (uint32_t k = 0; k < kGroup; k++) g[k] = 2 * k < nx ? (uint32_t)gv[k] : 0u;
 #pragma unroll
-        for (uint32_t k = 0; k < kGroup; k++)
+        for (uint32_t k = 0; k < 2 * kGroup; k++)
             if (k < nx) rec[k] = sc.block(b, (g[k >> 1] >> (2 + 4 * (k & 1u))) & 3u);
         guint64* pre = (guint64*)(j.pre + (uint64_t)(2 * rp + 1) * gw + x0);
 #pragma unroll
-        for (uint32_t k = 0; k < kGroup; k++)
+        for (uint32_t k = 0; k < 2 * kGroup; k++)
             if (k < nx) pre[k] = rec[k];
     }
 }
@@ -1735,7 +1738,8 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
             j.wg_sym = wg_sym;
             wg_sym += (uint32_t)((gE + kSymGroupsPerWG - 1) / kSymGroupsPerWG);
             j.wg_odd = wg_odd;
-            wg_odd += (uint32_t)((gO + kSymGroupsPerWG - 1) / kSymGroupsPerWG);
+            const uint64_t tO = rowpairs * ((((gw + kGroup - 1) / kGroup) + 1) / 2);   // two odd groups per thread
+            wg_odd += (uint32_t)((tO + kSymGroupsPerWG - 1) / kSymGroupsPerWG);
             const uint32_t rows = h.faces * gh;
             j.crows = kIdxChunk / gw;
             j.nchunks = (rows + j.crows - 1) / j.crows;
