@@ -202,16 +202,19 @@ __device__ inline uint32_t decode_top(const LdsTables& t, gcuint16* sorted, uint
 // keeps several positions' lookup chains in flight at once (kTabU1..3, the
 // positions tid + 512 k), so that the dependent LDS reads of one position
 // overlap those of the others; the pass-1 code tables and the T array share
-// LDS (the tables are dead once pass 1 ends), 36,480 B per workgroup: three
-// workgroups per CU beside a stream walker's 40 KiB ring.
-constexpr uint32_t kTabPerWG = 4096;
+// LDS (the tables are dead once pass 1 ends), 50,816 B per workgroup: three
+// workgroups per CU, two beside a stream walker's 40 KiB ring.  The tile of
+// 6,144 positions keeps the halo that passes 1 and 2 recompute to 19 % and
+// 17 % (measured on the 4096^2 pair: 4,096 positions 6.04 ms per plan of 32
+// copies, 5,120 6.02, 6,144 5.75, 8,192 6.03 at four waves per SIMD).
+constexpr uint32_t kTabPerWG = 6144;
 constexpr uint32_t kTabT = kTabPerWG + 1024;
 constexpr uint32_t kTabCW = kTabT + 160;
 constexpr uint32_t kTabWords = kTabCW / 32 + 3;
 constexpr uint32_t kTabThreads = 512;
-constexpr uint32_t kTabU1 = 6;   // pass 1: two rounds of 6 slots cover kTabCW (5,280 = 10.3 x 512)
-constexpr uint32_t kTabU2 = 5;   // pass 2: two rounds of 5 cover kTabT
-constexpr uint32_t kTabU3 = 4;   // pass 3: two rounds of 4 cover kTabPerWG
+constexpr uint32_t kTabU1 = 8;   // pass 1: two rounds of 8 slots cover kTabCW (7,328 = 14.3 x 512)
+constexpr uint32_t kTabU2 = 7;   // pass 2: two rounds of 7 cover kTabT
+constexpr uint32_t kTabU3 = 6;   // pass 3: two rounds of 6 cover kTabPerWG
 static_assert(2 * kTabU1 * kTabThreads >= kTabCW && 2 * kTabU2 * kTabThreads == kTabT &&
                   2 * kTabU3 * kTabThreads == kTabPerWG, "table pass rounds");
 
