@@ -23,7 +23,7 @@ HBM_PEAK_GBS = 8000.0
 # known size): 1.070 GB, 27.5x the algorithmic bytes, mostly the per-bit tables.
 TRAFFIC_PER_COPY = 1.070e9
 STAGES = ("crnd_tables_kernel", "crnd_walk_kernel", "crnd_sym_kernel<even>", "crnd_sym_kernel<odd>",
-          "crnd_index_{chunk,carry,final}_kernel", "crnd_write_kernel")
+          "crnd_index_{chunk,carry}_kernel", "crnd_index_final_kernel (+ block write)")
 
 
 def log(*a):

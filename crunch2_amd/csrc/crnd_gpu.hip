@@ -19,11 +19,12 @@
 //      8 odd-row blocks.
 //   3. crnd_sym_kernel (one thread per group; even rows, then odd rows):
 //      endpoint deltas, selectors and reference codes from the start bits.
-//   4. crnd_index_{chunk,carry,final}_kernel: running endpoint indices by
-//      pointer jumping over the blocks' parent links (previous block or block
-//      above) in LDS per chunk of rows, then chunk after chunk per level.
-//   5. crnd_write_kernel (one thread per visible block): palette gather and
-//      the 8- or 16-byte DXTn block write, coalesced.
+//   4. crnd_index_{chunk,carry}_kernel: running endpoint indices by pointer
+//      jumping over the blocks' parent links (previous block or block above)
+//      in LDS per chunk of rows, then chunk after chunk per level.
+//   5. crnd_index_final_kernel (one thread per block): each block's indices
+//      from its chunk's resolved row, the range checks, and for a visible
+//      block the palette gather and the 8- or 16-byte DXTn block write.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -97,7 +98,6 @@ struct LevelJob {
     uint32_t* posO;              // start bit of each group of kGroup blocks of the odd rows
     uint8_t* gvals;              // faces * gh/2 * gw/2 reference group symbols
     uint64_t* pre;               // faces * gh * gw: decoded symbols of each block
-    void* rec;                   // faces * gh * gw palette-index records (uint32 kinds 0/3, uint64 kinds 1/2)
     uint32_t crows;              // block rows per index chunk
     uint32_t nchunks;            // index chunks of the level
     uint32_t wg_idx;             // first workgroup of the job in crnd_index_chunk_kernel
@@ -1185,12 +1185,85 @@ __global__ void __launch_bounds__(kCarryThreads) crnd_index_carry_kernel(const L
     }
 }
 
+struct WriteJob {
+    uint8_t* dst[6];        // per-face destination
+    uint32_t pitch;         // bytes between block rows
+    uint32_t bx, by;        // visible blocks
+    uint32_t gw, gh;        // padded grid
+    uint32_t faces;
+    uint32_t pal;           // palette set index
+};
+
+struct WritePalettes {
+    const uint32_t* ce;
+    const uint32_t* cs;
+    const uint16_t* ae;
+    const uint16_t* as;     // 3 per entry
+    uint32_t kind;
+    uint32_t pad;
+};
+
+// One DXTn block from its palette indices (lo: first channel's endpoint and
+// selector indices, 16 bits each; hi: the second channel's) into row `out`.
+__device__ inline void write_block(const WritePalettes& P, uint8_t* out, uint32_t x, uint32_t pitch, uint32_t lo32,
+                                   uint32_t hi32)
+{
+    if (P.kind == 0 || P.kind == 3) {
+        uint2 o;
+        if (P.kind == 0) {
+            o.x = P.ce[lo32 & 0xFFFFu];
+            o.y = P.cs[lo32 >> 16];
+        } else {
+            const uint16_t* s = P.as + 3 * (lo32 >> 16);
+            o.x = (uint32_t)P.ae[lo32 & 0xFFFFu] | ((uint32_t)s[0] << 16);
+            o.y = (uint32_t)s[1] | ((uint32_t)s[2] << 16);
+        }
+        if ((pitch & 7) == 0) {
+            reinterpret_cast<uint2*>(out)[x] = o;
+        } else {
+            uint32_t* w = reinterpret_cast<uint32_t*>(out + 8 * (uint64_t)x);
+            w[0] = o.x;
+            w[1] = o.y;
+        }
+    } else {
+        uint4 o;
+        if (P.kind == 1) {
+            const uint16_t* s = P.as + 3 * (hi32 >> 16);
+            o.x = (uint32_t)P.ae[hi32 & 0xFFFFu] | ((uint32_t)s[0] << 16);
+            o.y = (uint32_t)s[1] | ((uint32_t)s[2] << 16);
+            o.z = P.ce[lo32 & 0xFFFFu];
+            o.w = P.cs[lo32 >> 16];
+        } else {
+            const uint16_t* s0 = P.as + 3 * (lo32 >> 16);
+            const uint16_t* s1 = P.as + 3 * (hi32 >> 16);
+            o.x = (uint32_t)P.ae[lo32 & 0xFFFFu] | ((uint32_t)s0[0] << 16);
+            o.y = (uint32_t)s0[1] | ((uint32_t)s0[2] << 16);
+            o.z = (uint32_t)P.ae[hi32 & 0xFFFFu] | ((uint32_t)s1[0] << 16);
+            o.w = (uint32_t)s1[1] | ((uint32_t)s1[2] << 16);
+        }
+        if ((pitch & 15) == 0) {
+            reinterpret_cast<uint4*>(out)[x] = o;
+        } else {
+            uint32_t* w = reinterpret_cast<uint32_t*>(out + 16 * (uint64_t)x);
+            w[0] = o.x;
+            w[1] = o.y;
+            w[2] = o.z;
+            w[3] = o.w;
+        }
+    }
+}
+
+// Also stage 5: a visible block's DXTn bytes go straight to the caller's
+// buffer (wjobs[i] belongs to jobs[i]).
 __global__ void __launch_bounds__(256) crnd_index_final_kernel(const LevelJob* __restrict__ jobs, uint32_t njobs,
-                                                               const TexInfo* __restrict__ texs, uint64_t total)
+                                                               const TexInfo* __restrict__ texs, uint64_t total,
+                                                               const WriteJob* __restrict__ wjobs,
+                                                               const WritePalettes* __restrict__ pals)
 {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    const LevelJob& j = jobs[find_job_wg(jobs, njobs, t, [](const LevelJob& x) { return x.blk_first; })];
+    const uint32_t ji = find_job_wg(jobs, njobs, t, [](const LevelJob& x) { return x.blk_first; });
+    const LevelJob& j = jobs[ji];
     const TexInfo& tx = texs[j.tex];
     const IdxSizes z = idx_sizes(tx);
     const uint32_t b = (uint32_t)(t - j.blk_first);
@@ -1224,99 +1297,18 @@ __global__ void __launch_bounds__(256) crnd_index_final_kernel(const LevelJob* _
         bad = true;
         s1 = 0;
     }
-    if (z.two)
-        static_cast<uint64_t*>(j.rec)[b] =
-            (uint64_t)((v & 0xFFFFu) | (s0 << 16)) | ((uint64_t)((v >> 16) | (s1 << 16)) << 32);
-    else
-        static_cast<uint32_t*>(j.rec)[b] = (v & 0xFFFFu) | (s0 << 16);
     if (bad) atomicExch(tx.status, CRN2_ECORRUPT);
+    const WriteJob& w = wjobs[ji];
+    const uint32_t per_face = gw * j.gh;
+    const uint32_t f = b / per_face, rem = b - f * per_face, y = rem / gw, x = rem - y * gw;
+    if (x < w.bx && y < w.by)
+        write_block(pals[w.pal], w.dst[f] + (uint64_t)y * w.pitch, x, w.pitch, (v & 0xFFFFu) | (s0 << 16),
+                    (v >> 16) | (s1 << 16));
 }
 
-struct WriteJob {
-    const void* rec;        // records of the level (padded grid, faces stacked)
-    uint8_t* dst[6];        // per-face destination
-    uint32_t pitch;         // bytes between block rows
-    uint32_t bx, by;        // visible blocks
-    uint32_t gw, gh;        // padded grid
-    uint32_t faces;
-    uint32_t pal;           // palette set index
-    uint64_t first_block;   // prefix of visible blocks over the jobs of the launch
-};
-
-struct WritePalettes {
-    const uint32_t* ce;
-    const uint32_t* cs;
-    const uint16_t* ae;
-    const uint16_t* as;     // 3 per entry
-    uint32_t kind;
-    uint32_t pad;
-};
-
-__global__ void __launch_bounds__(256) crnd_write_kernel(const WriteJob* __restrict__ jobs, uint32_t njobs,
-                                                         const WritePalettes* __restrict__ pals, uint64_t total)
-{
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const WriteJob& j = jobs[find_job_wg(jobs, njobs, t, [](const WriteJob& x) { return x.first_block; })];
-    const WritePalettes& P = pals[j.pal];
-    // a job's blocks number fewer than 2^32 (plans refuse larger grids): 32-bit division
-    uint32_t v = (uint32_t)(t - j.first_block);
-    const uint32_t per_face = j.bx * j.by;
-    const uint32_t f = v / per_face;
-    v -= f * per_face;
-    const uint32_t y = v / j.bx, x = v - y * j.bx;
-    const uint64_t r = (uint64_t)(f * j.gh + y) * j.gw + x;
-    uint8_t* out = j.dst[f] + (uint64_t)y * j.pitch;
-    if (P.kind == 0 || P.kind == 3) {
-        const uint32_t rec = static_cast<const uint32_t*>(j.rec)[r];
-        uint2 o;
-        if (P.kind == 0) {
-            o.x = P.ce[rec & 0xFFFFu];
-            o.y = P.cs[rec >> 16];
-        } else {
-            const uint16_t* s = P.as + 3 * (rec >> 16);
-            o.x = (uint32_t)P.ae[rec & 0xFFFFu] | ((uint32_t)s[0] << 16);
-            o.y = (uint32_t)s[1] | ((uint32_t)s[2] << 16);
-        }
-        if ((j.pitch & 7) == 0) {
-            reinterpret_cast<uint2*>(out)[x] = o;
-        } else {
-            uint32_t* w = reinterpret_cast<uint32_t*>(out + 8 * (uint64_t)x);
-            w[0] = o.x;
-            w[1] = o.y;
-        }
-    } else {
-        const uint64_t rec = static_cast<const uint64_t*>(j.rec)[r];
-        const uint32_t lo32 = (uint32_t)rec, hi32 = (uint32_t)(rec >> 32);
-        uint4 o;
-        if (P.kind == 1) {
-            const uint16_t* s = P.as + 3 * (hi32 >> 16);
-            o.x = (uint32_t)P.ae[hi32 & 0xFFFFu] | ((uint32_t)s[0] << 16);
-            o.y = (uint32_t)s[1] | ((uint32_t)s[2] << 16);
-            o.z = P.ce[lo32 & 0xFFFFu];
-            o.w = P.cs[lo32 >> 16];
-        } else {
-            const uint16_t* s0 = P.as + 3 * (lo32 >> 16);
-            const uint16_t* s1 = P.as + 3 * (hi32 >> 16);
-            o.x = (uint32_t)P.ae[lo32 & 0xFFFFu] | ((uint32_t)s0[0] << 16);
-            o.y = (uint32_t)s0[1] | ((uint32_t)s0[2] << 16);
-            o.z = (uint32_t)P.ae[hi32 & 0xFFFFu] | ((uint32_t)s1[0] << 16);
-            o.w = (uint32_t)s1[1] | ((uint32_t)s1[2] << 16);
-        }
-        if ((j.pitch & 15) == 0) {
-            reinterpret_cast<uint4*>(out)[x] = o;
-        } else {
-            uint32_t* w = reinterpret_cast<uint32_t*>(out + 16 * (uint64_t)x);
-            w[0] = o.x;
-            w[1] = o.y;
-            w[2] = o.z;
-            w[3] = o.w;
-        }
-    }
-}
 
 // ----------------------------------------------------------------- profiling
-constexpr int kStages = 6;   // tables, walk, sym (even rows), sym (odd rows), index, write
+constexpr int kStages = 6;   // tables, walk, sym (even rows), sym (odd rows), index chunks + carry, index final + write
 struct Profile {
     std::mutex mu;
     bool on = false;
@@ -1363,8 +1355,6 @@ struct crn2_unpack_ctx {
     uint16_t* d_ae = nullptr;
     uint16_t* d_as = nullptr;
     int32_t* d_status = nullptr;
-    void* d_rec = nullptr;       // records of every level
-    uint64_t rec_ofs[kMaxLevels + 1] = {};
     uint32_t n_ce = 0, n_cs = 0, n_ae = 0, n_as = 0;
     crn2_batch* cached = nullptr;   // plan of the last crn2_unpack_levels_device / _level_device call
     void* cached_dst = nullptr;
@@ -1380,13 +1370,12 @@ struct crn2_batch {
     WritePalettes* d_pals = nullptr;
     uint32_t* d_tab_job = nullptr;  // level job of every crnd_tables_kernel workgroup
     uint8_t* d_scratch = nullptr;   // transition tables, start bits, symbol records
-    uint32_t njobs = 0, nwjobs = 0;
+    uint32_t njobs = 0;
     uint32_t wg_trans = 0, wg_sym = 0, wg_odd = 0, wg_idx = 0;
     uint32_t max_gw = 0;
     uint32_t* d_carry = nullptr;                // level jobs of more than one index chunk (crnd_index_carry_kernel)
     uint32_t ncarry = 0;
     uint64_t blk_total = 0;
-    uint64_t total_blocks = 0;
     std::vector<int32_t*> status;   // per context
     hipGraphExec_t graph = nullptr; // the plan's launches captured once (crn2_batch_run_graph)
 };
@@ -1498,7 +1487,7 @@ static void ctx_free_device(crn2_unpack_ctx* c)
 {
     batch_free(c->cached);
     c->cached = nullptr;
-    void* ptrs[] = {c->d_file, c->d_models, c->d_sorted, c->d_cw1, c->d_cw2, c->d_ce, c->d_cs, c->d_ae, c->d_as, c->d_status, c->d_rec};
+    void* ptrs[] = {c->d_file, c->d_models, c->d_sorted, c->d_cw1, c->d_cw2, c->d_ce, c->d_cs, c->d_ae, c->d_as, c->d_status};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -1618,15 +1607,6 @@ static crn2_unpack_ctx* begin_impl(const void* data, const void* data_dev, uint3
             cw2[v] = (uint16_t)(fd1[v] | (fs1[v] << 8));
         }
     }
-    uint64_t rec = 0;
-    const uint32_t rec_bytes = (c->kind == 1 || c->kind == 2) ? 8 : 4;
-    for (uint32_t l = 0; l < h.levels; l++) {
-        const uint32_t bx = (level_dim(h.width, l) + 3) >> 2, by = (level_dim(h.height, l) + 3) >> 2;
-        const uint32_t gw = (bx + 1) & ~1u, gh = (by + 1) & ~1u;
-        c->rec_ofs[l] = rec;
-        rec += (uint64_t)gw * gh * h.faces * rec_bytes;
-    }
-    c->rec_ofs[h.levels] = rec;
 
     const size_t padded = ((size_t)size + 15) / 16 * 16 + 64;
     if (dmalloc(&c->d_file, padded) != hipSuccess || dmalloc(&c->d_models, sizeof(DevModel) * M_COUNT) != hipSuccess ||
@@ -1635,8 +1615,7 @@ static crn2_unpack_ctx* begin_impl(const void* data, const void* data_dev, uint3
         dmalloc(&c->d_ce, std::max<size_t>(ce.size(), 1) * 4) != hipSuccess ||
         dmalloc(&c->d_cs, std::max<size_t>(cs.size(), 1) * 4) != hipSuccess ||
         dmalloc(&c->d_ae, std::max<size_t>(ae.size(), 1) * 2) != hipSuccess ||
-        dmalloc(&c->d_as, std::max<size_t>(as.size(), 3) * 2) != hipSuccess || dmalloc(&c->d_status, 4) != hipSuccess ||
-        dmalloc(&c->d_rec, rec) != hipSuccess)
+        dmalloc(&c->d_as, std::max<size_t>(as.size(), 3) * 2) != hipSuccess || dmalloc(&c->d_status, 4) != hipSuccess)
         return fail();
     bool ok = true;
     ok &= hipMemsetAsync(c->d_file, 0, padded, stream) == hipSuccess;
@@ -1688,7 +1667,6 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
     std::vector<TexInfo> texs;
     std::vector<WriteJob> wj;
     std::vector<WritePalettes> pals;
-    uint64_t total = 0;
     size_t scratch = 0;
     struct Ofs { size_t O2, E8, X, E, O, G, pre, ent, bnd; };
     std::vector<Ofs> ofs;
@@ -1770,12 +1748,10 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
             o.bnd = scratch;
             scratch = align_up(scratch + (size_t)j.nchunks * gw * 8, 256);
             ofs.push_back(o);
-            j.rec = static_cast<uint8_t*>(c->d_rec) + c->rec_ofs[l];
             max_gw = std::max(max_gw, gw);
             jobs.push_back(j);
             WriteJob w;
             memset(&w, 0, sizeof(w));
-            w.rec = j.rec;
             for (uint32_t f = 0; f < 6; f++) w.dst[f] = (f < h.faces) ? P.dst[i][f] : nullptr;
             w.pitch = P.pitch[i];
             w.bx = bx;
@@ -1784,8 +1760,6 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
             w.gh = gh;
             w.faces = h.faces;
             w.pal = (uint32_t)pals.size();
-            w.first_block = total;
-            total += (uint64_t)bx * by * h.faces;
             wj.push_back(w);
         }
         texs.push_back(t);
@@ -1794,7 +1768,6 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
         B->status.push_back(c->d_status);
     }
     B->njobs = (uint32_t)jobs.size();
-    B->nwjobs = (uint32_t)wj.size();
     B->wg_trans = wg_trans;
     B->wg_sym = wg_sym;
     B->wg_odd = wg_odd;
@@ -1805,7 +1778,6 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
     for (uint32_t i = 0; i < (uint32_t)jobs.size(); i++)
         if (jobs[i].nchunks > 1) carry.push_back(i);
     B->ncarry = (uint32_t)carry.size();
-    B->total_blocks = total;
     if (dmalloc(&B->d_jobs, jobs.size() * sizeof(LevelJob)) != hipSuccess ||
         dmalloc(&B->d_texs, texs.size() * sizeof(TexInfo)) != hipSuccess ||
         dmalloc(&B->d_wjobs, wj.size() * sizeof(WriteJob)) != hipSuccess ||
@@ -1883,20 +1855,13 @@ static int launch_plan(crn2_batch* B, hipStream_t stream, Profile::Ev* ev)
                                B->d_carry, B->d_texs);
             HIPCHK(hipGetLastError());
         }
+        if (prof) HIPCHK(hipEventRecord(ev->e[5], stream));
         const uint32_t fb = (uint32_t)((B->blk_total + 255) / 256);
         hipLaunchKernelGGL(crnd_index_final_kernel, dim3(fb), dim3(256), 0, stream, B->d_jobs, B->njobs, B->d_texs,
-                           B->blk_total);
+                           B->blk_total, B->d_wjobs, B->d_pals);
         HIPCHK(hipGetLastError());
-        if (prof) HIPCHK(hipEventRecord(ev->e[5], stream));
     } else if (prof) {
         for (int k = 1; k < kStages; k++) HIPCHK(hipEventRecord(ev->e[k], stream));
-    }
-    if (B->total_blocks) {
-        const uint32_t tb = 256;
-        const uint64_t nb = (B->total_blocks + tb - 1) / tb;
-        hipLaunchKernelGGL(crnd_write_kernel, dim3((uint32_t)nb), dim3(tb), 0, stream, B->d_wjobs, B->nwjobs, B->d_pals,
-                           B->total_blocks);
-        HIPCHK(hipGetLastError());
     }
     if (prof) HIPCHK(hipEventRecord(ev->e[kStages], stream));
     return CRN2_OK;

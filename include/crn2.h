@@ -130,8 +130,8 @@ int crn2_batch_destroy(crn2_batch* batch);
 
 /* Stage timing with HIP events on the launch stream of crn2_batch_run and the
  * unpack calls (stage 0 = transition tables, 1 = stream walk, 2 / 3 = symbol
- * decode of even / odd rows, 4 = palette indices, 5 = block write): enable,
- * run, then read total ms and launch count. */
+ * decode of even / odd rows, 4 = palette-index chunks, 5 = palette indices and
+ * block write): enable, run, then read total ms and launch count. */
 void crn2_profile_enable(int on);
 void crn2_profile_reset(void);
 int crn2_profile_read(int kernel, double* total_ms, uint64_t* launches);
