@@ -20,8 +20,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes one copy of the 4096^2 DXT1 + DXT5 pair moves through the
 # pipeline (per-kernel FETCH_SIZE / WRITE_SIZE, scaled by a device copy of
-# known size): 1.070 GB, 27.5x the algorithmic bytes, mostly the per-bit tables.
-TRAFFIC_PER_COPY = 1.070e9
+# known size): 1.035 GB, 26.6x the algorithmic bytes, mostly the per-bit tables.
+TRAFFIC_PER_COPY = 1.035e9
 STAGES = ("crnd_tables_kernel", "crnd_walk_kernel", "crnd_sym_kernel<even>", "crnd_sym_kernel<odd>",
           "crnd_index_{chunk,carry}_kernel", "crnd_index_final_kernel (+ block write)")
 
