@@ -146,6 +146,7 @@ SIGNATURES = {
 # internal (not in include/): the block packer behind crn2_compress_rgba
 INTERNAL = {
     "crn2_pack": (C.c_int, [C.POINTER(PackInput), C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]),
+    "crn2_prefix_lengths": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
 }
 
 _lib = None

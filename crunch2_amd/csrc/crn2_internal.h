@@ -36,6 +36,10 @@ typedef struct crn2_pack_input {
 
 int crn2_pack(const crn2_pack_input* in, uint8_t** out, uint32_t* out_size);
 
+// Test hook of the transcoder's per-prefix code lengths (kLutSize = 2,048
+// bytes out) for the prefix code with the given code sizes.
+int crn2_prefix_lengths(const uint8_t* code_sizes, uint32_t n, uint8_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1551,47 +1551,8 @@ static crn2_unpack_ctx* begin_impl(const void* data, const void* data_dev, uint3
     }
     // Per 11-bit prefix (TexInfo::cw1 / cw2): the reference code's length and
     // group flags, and per delta / selector field the length every window of
-    // the prefix starts.  Codes longer than the prefix: a model's canonical
-    // limits of lengths 12..16, made non-decreasing, give each window's length
-    // as 12 + the number of limits at or below it (0 from the last limit on),
-    // which is the canonical walk's (crn_decomp.h:3239-3244) whenever its index
-    // check holds for the first and last window of each length -- the index
-    // grows with the window; a prefix whose first and last windows agree then
-    // starts that length throughout, the others (at most one per length
-    // boundary) are marked 0xFF for the walk, and so is every long prefix of a
-    // model whose index check can fail.
-    auto field_lengths = [&](uint32_t m, uint8_t* out) {
-        const DevModel& dm = dms[m];
-        uint32_t lim[5], mx = 0;
-        for (uint32_t L = kLutBits + 1; L <= 16; L++) lim[L - kLutBits - 1] = mx = std::max(mx, dm.limit[L]);
-        auto len_at = [&](uint32_t top) -> uint32_t {
-            if (top >= lim[4]) return 0;
-            uint32_t L = kLutBits + 1;
-            for (uint32_t i = 0; i < 4; i++) L += top >= lim[i] ? 1u : 0u;
-            return L;
-        };
-        uint32_t h = 0;
-        while (h < kLutSize && dm.len8[h]) h++;
-        bool exact = true;
-        uint32_t lo = h << (16 - kLutBits);
-        for (uint32_t L = kLutBits + 1; L <= 16; L++) {
-            const uint32_t hi = lim[L - kLutBits - 1];
-            if (lo < hi) {
-                const int64_t i0 = (int64_t)dm.base[L] + (lo >> (16 - L));
-                const int64_t i1 = (int64_t)dm.base[L] + ((hi - 1) >> (16 - L));
-                exact &= i0 >= 0 && i1 < (int64_t)dm.num_syms;
-            }
-            lo = std::max(lo, hi);
-        }
-        for (uint32_t v = 0; v < kLutSize; v++) {
-            if (dm.len8[v]) {
-                out[v] = dm.len8[v];
-                continue;
-            }
-            const uint32_t w = v << (16 - kLutBits), a = len_at(w), b = len_at(w + (1u << (16 - kLutBits)) - 1);
-            out[v] = (exact && a == b) ? (uint8_t)a : (uint8_t)0xFF;
-        }
-    };
+    // the prefix starts (prefix_lengths: 0xFF where the canonical walk decides).
+    auto field_lengths = [&](uint32_t m, uint8_t* out) { prefix_lengths(dms[m], out); };
     std::vector<uint32_t> cw1(kLutSize);
     std::vector<uint16_t> cw2(kLutSize);
     {

@@ -14,7 +14,10 @@
 //                             decode_alpha_selectors 3829-3851
 #include "crnd_host.h"
 
+#include <algorithm>
 #include <cstring>
+
+#include "crn2_internal.h"
 
 namespace crn2 {
 
@@ -110,6 +113,45 @@ void to_device_model(const HostModel& m, DevModel& dm)
     }
 }
 
+void prefix_lengths(const DevModel& dm, uint8_t* out)
+{
+    // The model's canonical limits of lengths 12..16, made non-decreasing,
+    // give each long window's length as 12 + the number of limits at or below
+    // it (0 from the last limit on), the canonical walk's length; the walk's
+    // index, base[L] + code, grows with the window, so its check holds for
+    // every window once it holds for the first and last window of each length
+    // (from the first window the direct table leaves open).
+    uint32_t lim[5], mx = 0;
+    for (uint32_t L = kLutBits + 1; L <= 16; L++) lim[L - kLutBits - 1] = mx = std::max(mx, dm.limit[L]);
+    auto len_at = [&](uint32_t top) -> uint32_t {
+        if (top >= lim[4]) return 0;
+        uint32_t L = kLutBits + 1;
+        for (uint32_t i = 0; i < 4; i++) L += top >= lim[i] ? 1u : 0u;
+        return L;
+    };
+    uint32_t h = 0;
+    while (h < kLutSize && dm.len8[h]) h++;
+    bool exact = true;
+    uint32_t lo = h << (16 - kLutBits);
+    for (uint32_t L = kLutBits + 1; L <= 16; L++) {
+        const uint32_t hi = lim[L - kLutBits - 1];
+        if (lo < hi) {
+            const int64_t i0 = (int64_t)dm.base[L] + (lo >> (16 - L));
+            const int64_t i1 = (int64_t)dm.base[L] + ((hi - 1) >> (16 - L));
+            exact &= i0 >= 0 && i1 < (int64_t)dm.num_syms;
+        }
+        lo = std::max(lo, hi);
+    }
+    for (uint32_t v = 0; v < kLutSize; v++) {
+        if (dm.len8[v]) {
+            out[v] = dm.len8[v];
+            continue;
+        }
+        const uint32_t w = v << (16 - kLutBits), a = len_at(w), b = len_at(w + (1u << (16 - kLutBits)) - 1);
+        out[v] = (exact && a == b) ? (uint8_t)a : (uint8_t)0xFF;
+    }
+}
+
 uint32_t HostBits::get_bits(uint32_t n)
 {
     while (count < (int)n) {
@@ -198,3 +240,15 @@ bool receive_model(HostBits& b, HostModel& m)
 }
 
 }  // namespace crn2
+
+// Test hook: prefix_lengths of the prefix code with the given code sizes.
+extern "C" int crn2_prefix_lengths(const uint8_t* code_sizes, uint32_t n, uint8_t* out)
+{
+    if (!code_sizes || !n || n > crn2::kMaxSyms || !out) return CRN2_EINVAL;
+    crn2::HostModel m;
+    if (!crn2::build_model(m, n, code_sizes)) return CRN2_EINVAL;
+    crn2::DevModel dm;
+    crn2::to_device_model(m, dm);
+    crn2::prefix_lengths(dm, out);
+    return CRN2_OK;
+}

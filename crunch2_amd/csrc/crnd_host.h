@@ -46,6 +46,15 @@ bool parse_header(const uint8_t* d, uint32_t size, Header& h);
 bool build_model(HostModel& m, uint32_t n, const uint8_t* sizes);
 void to_device_model(const HostModel& m, DevModel& dm);
 
+// Per 11-bit prefix v (kLutSize entries): the length of the code that every
+// 16-bit window starting with v starts -- 1..16, or 0 where no code matches
+// (crn_decomp.h:3239-3244) -- or 0xFF where the windows differ or the model's
+// canonical index check could fail for one of them (the canonical walk then
+// decides).  Codes longer than the prefix change length inside a prefix only
+// where a length's codes end off a 32-window boundary: at most five 0xFF
+// prefixes for a model that passes the index check.
+void prefix_lengths(const DevModel& dm, uint8_t* out);
+
 // MSB-first bit reader with the reference's zero-past-the-end semantics.
 struct HostBits {
     const uint8_t* next;
