@@ -228,11 +228,12 @@ __device__ inline uint32_t be_bytes_word(const uint8_t* src, uint32_t size, uint
     return v;
 }
 
-__device__ inline uint32_t peek16(const uint32_t* w, uint32_t r)
+// The 32 stream bits from bit r on (MSB first).
+__device__ inline uint32_t peek32(const uint32_t* w, uint32_t r)
 {
     const uint32_t i = r >> 5;
     const uint64_t v = ((uint64_t)w[i] << 32) | w[i + 1];
-    return (uint32_t)((v << (r & 31)) >> 48);
+    return (uint32_t)((v << (r & 31)) >> 32);
 }
 
 // Canonical-walk constants of one model for codes longer than kLutBits, with
@@ -334,14 +335,14 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
         constexpr uint32_t U = kTabU1;
 #pragma unroll 1
         for (uint32_t i0 = 0; i0 < 2 * U; i0 += U) {
-            uint32_t r[U], top[U], c[U];
+            uint32_t r[U], w32[U], c[U];   // w32: the 32 bits from r on; top = w32 >> 16
 #pragma unroll
             for (uint32_t u = 0; u < U; u++) {
                 r[u] = min(tid + (i0 + u) * kTabThreads, kTabCW - 1);
-                top[u] = peek16(W, r[u]);
+                w32[u] = peek32(W, r[u]);
             }
 #pragma unroll
-            for (uint32_t u = 0; u < U; u++) c[u] = s_u.c.cw1[top[u] >> (16 - kLutBits)];
+            for (uint32_t u = 0; u < U; u++) c[u] = s_u.c.cw1[w32[u] >> (32 - kLutBits)];
             // the canonical walk only for a long group code or a prefix whose
             // windows start delta / selector codes of different lengths (at most
             // five prefixes per model: where a length's codes end off a
@@ -353,12 +354,12 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
                     uint32_t gl = c[u] & 0xFFu, g = (c[u] >> 8) & 0xFFu, dl = (c[u] >> 16) & 0xFFu, sl = c[u] >> 24;
                     uint32_t idx = 0;
                     if (!gl) {
-                        gl = long_code(R, top[u], idx);
+                        gl = long_code(R, w32[u] >> 16, idx);
                         g = group_flags(!gl ? 0u : idx < kTabRefSyms ? (uint32_t)s_u.c.ref_sym[idx]
                                                                      : (uint32_t)(ref_sorted[idx] & 0xFFu));
                     }
-                    if (dl == 0xFFu) dl = long_code(D0, top[u], idx);
-                    if (sl == 0xFFu) sl = long_code(S0, top[u], idx);
+                    if (dl == 0xFFu) dl = long_code(D0, w32[u] >> 16, idx);
+                    if (sl == 0xFFu) sl = long_code(S0, w32[u] >> 16, idx);
                     c[u] = gl | (g << 8) | (dl << 16) | (sl << 24);
                 }
             }
@@ -366,8 +367,10 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
                 uint32_t ta[U], tb[U], a[U], b[U];
 #pragma unroll
                 for (uint32_t u = 0; u < U; u++) {
-                    ta[u] = peek16(W, r[u] + ((c[u] >> 16) & 0xFFu));
-                    tb[u] = peek16(W, r[u] + (c[u] >> 24));
+                    // a first-channel code takes at most 16 bits: the second
+                    // channel's window lies inside the 32 bits read at r
+                    ta[u] = (w32[u] << ((c[u] >> 16) & 0xFFu)) >> 16;
+                    tb[u] = (w32[u] << (c[u] >> 24)) >> 16;
                 }
 #pragma unroll
                 for (uint32_t u = 0; u < U; u++) {
