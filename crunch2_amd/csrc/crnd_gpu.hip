@@ -214,9 +214,13 @@ constexpr uint32_t kTabWords = kTabCW / 32 + 3;
 constexpr uint32_t kTabThreads = 512;
 constexpr uint32_t kTabU1 = 8;   // pass 1: two rounds of 8 slots cover kTabCW (7,328 = 14.3 x 512)
 constexpr uint32_t kTabU2 = 7;   // pass 2: two rounds of 7 cover kTabT
-constexpr uint32_t kTabU3 = 6;   // pass 3: two rounds of 6 cover kTabPerWG
+constexpr uint32_t kTabU3 = 6;   // pass 3b: two rounds of 6 cover kTabPerWG
+constexpr uint32_t kTabE2 = kTabPerWG + 6 * 144;   // two-pair entries an eight-pair chain reads
+constexpr uint32_t kTabU3a = 7;  // pass 3a: two rounds of 7 cover kTabE2
 static_assert(2 * kTabU1 * kTabThreads >= kTabCW && 2 * kTabU2 * kTabThreads == kTabT &&
-                  2 * kTabU3 * kTabThreads == kTabPerWG, "table pass rounds");
+                  2 * kTabU3 * kTabThreads == kTabPerWG && 2 * kTabU3a * kTabThreads >= kTabE2 &&
+                  kTabE2 + 144 <= kTabT && kTabE2 * 2 <= kTabCW * 4,
+              "table pass rounds");
 
 __device__ inline uint32_t be_bytes_word(const uint8_t* src, uint32_t size, uint32_t b)
 {
@@ -438,50 +442,78 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
     }
     __syncthreads();
 
-    // 3. composed entries (stores at 32-bit offsets from the tile's base)
+    // 3. composed entries (stores at 32-bit offsets from the tile's base).
+    // 3a: O2 and X, and two-pair entries E2 = len (9 bits) | four odd flags
+    // << 9 for every position an E8 chain reaches (kTabPerWG + 6 x 144), into
+    // the CW array's space (dead after pass 2); 3b: E8 from four E2 lookups
+    // (three dependent LDS reads instead of seven).
     {
         const uint16_t* T = s_u.t.lo;
         const uint8_t* T8 = s_u.t.hi;
+        uint16_t* E2 = reinterpret_cast<uint16_t*>(s_cw);
         typedef __attribute__((address_space(1))) uint8_t guint8;
         guint8* O2 = (guint8*)(j.O2 + P0);
         guint8* E8 = (guint8*)(j.E8 + P0);
         guint8* X = (guint8*)(j.X + P0);
-        constexpr uint32_t U = kTabU3;
+        {
+            constexpr uint32_t U = kTabU3a;
 #pragma unroll 1
-        for (uint32_t i0 = 0; i0 < 2 * U; i0 += U) {
-            uint32_t r[U], t0[U], o1[U], ta[U], tb[U], ta8[U], tb8[U], p[U], fl[U];
-#pragma unroll
-            for (uint32_t u = 0; u < U; u++) {
-                r[u] = tid + (i0 + u) * kTabThreads;
-                t0[u] = T[r[u]];
-                o1[u] = T8[r[u]];
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < U; u++) {
-                ta[u] = T[r[u] + (t0[u] >> 10)];
-                ta8[u] = T8[r[u] + (t0[u] >> 10)];
-                tb[u] = T[r[u] + o1[u]];
-                tb8[u] = T8[r[u] + o1[u]];
-                p[u] = r[u] + (t0[u] & 0xFFu);
-                fl[u] = t0[u] & 0x300u;
-            }
-            // odd flags gathered at bit 8 + 2 i, shifted down once
-#pragma unroll
-            for (uint32_t i = 1; i < 8; i++) {
+            for (uint32_t i0 = 0; i0 < 2 * U; i0 += U) {
+                uint32_t r[U], t0[U], o1[U], ta[U], tb[U], ta8[U], tb8[U], tp[U];
 #pragma unroll
                 for (uint32_t u = 0; u < U; u++) {
-                    const uint32_t t = T[p[u]];
-                    fl[u] |= (t & 0x300u) << (2 * i);
-                    p[u] += t & 0xFFu;
+                    r[u] = min(tid + (i0 + u) * kTabThreads, kTabE2 - 1);
+                    t0[u] = T[r[u]];
+                    o1[u] = T8[r[u]];
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) {
+                    ta[u] = T[r[u] + (t0[u] >> 10)];
+                    ta8[u] = T8[r[u] + (t0[u] >> 10)];
+                    tb[u] = T[r[u] + o1[u]];
+                    tb8[u] = T8[r[u] + o1[u]];
+                    tp[u] = T[r[u] + (t0[u] & 0xFFu)];
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++) {
+                    const uint32_t q = tid + (i0 + u) * kTabThreads;
+                    if (q < kTabE2)
+                        E2[r[u]] = (uint16_t)(((t0[u] & 0xFFu) + (tp[u] & 0xFFu)) | ((t0[u] & 0x300u) << 1) |
+                                              ((tp[u] & 0x300u) << 3));
+                    if (q < kTabPerWG) {
+                        const uint32_t o0 = t0[u] >> 10;
+                        *(guint32*)(O2 + 4 * r[u]) = (o0 + (ta[u] >> 10)) | ((o1[u] + (tb[u] >> 10)) << 8) |
+                                                     ((o0 + ta8[u]) << 16) | ((o1[u] + tb8[u]) << 24);
+                        *(guint16*)(X + 2 * r[u]) = (uint16_t)(t0[u] & 0x3FFu);
+                    }
                 }
             }
+        }
+        __syncthreads();
+        {
+            constexpr uint32_t U = kTabU3;
+#pragma unroll 1
+            for (uint32_t i0 = 0; i0 < 2 * U; i0 += U) {
+                uint32_t r[U], p[U], fl[U];
 #pragma unroll
-            for (uint32_t u = 0; u < U; u++) {
-                const uint32_t o0 = t0[u] >> 10;
-                *(guint32*)(O2 + 4 * r[u]) = (o0 + (ta[u] >> 10)) | ((o1[u] + (tb[u] >> 10)) << 8) |
-                                             ((o0 + ta8[u]) << 16) | ((o1[u] + tb8[u]) << 24);
-                *(guint32*)(E8 + 4 * r[u]) = (p[u] - r[u]) | ((fl[u] >> 8) << 11);
-                *(guint16*)(X + 2 * r[u]) = (uint16_t)(t0[u] & 0x3FFu);
+                for (uint32_t u = 0; u < U; u++) {
+                    r[u] = tid + (i0 + u) * kTabThreads;
+                    p[u] = r[u];
+                    fl[u] = 0;
+                }
+                // odd flags gathered at bit 9 + 4 i, shifted down once
+#pragma unroll
+                for (uint32_t i = 0; i < 4; i++) {
+#pragma unroll
+                    for (uint32_t u = 0; u < U; u++) {
+                        const uint32_t e = E2[p[u]];
+                        fl[u] |= (e & 0x1E00u) << (4 * i);
+                        p[u] += e & 0x1FFu;
+                    }
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; u++)
+                    *(guint32*)(E8 + 4 * r[u]) = (p[u] - r[u]) | ((fl[u] >> 9) << 11);
             }
         }
     }
