@@ -1317,9 +1317,13 @@ __global__ void __launch_bounds__(256) crnd_index_final_kernel(const LevelJob* _
     const uint64_t q = j.pre[b];
     const uint32_t ref = (uint32_t)q & 3u;
     const uint32_t v = value(b);
+    // the previous block's indices: the previous lane's v (the same job unless
+    // b == 0, where they are 0); lane 0 looks them up
+    uint32_t pv = (uint32_t)__shfl_up((int)v, 1);
     bool bad = z.n0 == 0 || z.n1 == 0;
     if (ref == 0) {
-        const uint32_t pv = b ? value(b - 1) : 0u;
+        if (b == 0) pv = 0u;
+        else if ((threadIdx.x & 63u) == 0) pv = value(b - 1);
         bad |= (pv & 0xFFFFu) + ((uint32_t)(q >> 2) & 0x3FFFu) >= 2 * z.n0;
         if (z.two) bad |= (pv >> 16) + ((uint32_t)(q >> 16) & 0x3FFFu) >= 2 * z.n1;
     }
