@@ -7,6 +7,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "crunch2_amd")
@@ -48,15 +49,14 @@ def build_oracle():
 
 def build_lib():
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
-    for s in HOST_SOURCES:
-        o = os.path.join(BUILD, s + ".o")
-        _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
-        objs.append(o)
-    for s in HIP_SOURCES:
-        o = os.path.join(BUILD, s + ".o")
-        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
-        objs.append(o)
+    jobs = [(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, s), "-o", os.path.join(BUILD, s + ".o")])
+            for s in HOST_SOURCES]
+    jobs += [([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", os.path.join(BUILD, s + ".o")])
+             for s in HIP_SOURCES]
+    objs = [j[-1] for j in jobs]
+    # every source compiles on its own; the largest HIP files take most of a minute
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as pool:
+        list(pool.map(_run, jobs))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lgomp"])
     os.replace(tmp, LIB)
