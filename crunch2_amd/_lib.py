@@ -86,6 +86,8 @@ SIGNATURES = {
     "crn2_dxt1_blocks_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "crn2_dxt1_clusters_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p,
                                             C.c_void_p]),
+    "crn2_dxt1_clusters_noalloc_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
+                                                    C.c_void_p, C.c_void_p]),
     "crn2_hc_color_selectors_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_int,
                                                  C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                                  C.c_void_p, C.c_void_p]),

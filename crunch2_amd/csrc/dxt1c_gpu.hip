@@ -1130,15 +1130,39 @@ int side_streams(hipStream_t* side, hipEvent_t* fork, hipEvent_t* join)
 }
 }  // namespace
 
-extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags,
-                                         void* out, uint8_t* selectors, void* stream)
+namespace {
+// Scratch of crn2_dxt1_clusters_noalloc_device: four class counters, then four
+// lists of at most kNoAllocMaxClusters cluster indices (1 MiB + 16 B on every
+// GPU the library runs on, held until the process ends).  Host code never
+// passes the array's name to a kernel: it takes the device address from
+// hipGetSymbolAddress once per device (noalloc_scratch below).
+constexpr uint32_t kNoAllocMaxClusters = 65536;
+__device__ __align__(256) uint32_t scratch[4 + 4 * kNoAllocMaxClusters];
+
+__global__ void clear_class_counts_kernel(uint32_t* __restrict__ count)
 {
-    if (!rgba || !offsets || !out || (flags & ~1)) return CRN2_EINVAL;
-    if (!nclusters) return CRN2_OK;
-    static bool init = false;
-    if (!init) {
-        // ryg single-colour tables (crn_ryg_dxt.cpp:58-99, 517-535), four-colour
-        static uint8_t t5[256][2], t6[256][2];
+    if (threadIdx.x < 4) count[threadIdx.x] = 0;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_init_mu;
+bool g_init[kMaxDevices];
+uint32_t* g_noalloc_scratch[kMaxDevices];
+
+// Per-device set-up: the single-colour tables in device memory, the kernels'
+// LDS limits and the device address of `scratch`.  Idempotent; runs on the
+// device that is current.
+int clusters_init(int* dev_out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return CRN2_EHIP;
+    *dev_out = dev;
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (g_init[dev]) return CRN2_OK;
+    // ryg single-colour tables (crn_ryg_dxt.cpp:58-99, 517-535), four-colour
+    static uint8_t t5[256][2], t6[256][2];
+    static bool tables = false;
+    if (!tables) {
         for (int which = 0; which < 2; which++) {
             const int size = which ? 64 : 32;
             uint8_t (*t)[2] = which ? t6 : t5;
@@ -1158,57 +1182,100 @@ extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offse
                     }
             }
         }
-        if (hipMemcpyToSymbol(HIP_SYMBOL(g_c_om5), t5, 512) != hipSuccess) return CRN2_EHIP;
-        if (hipMemcpyToSymbol(HIP_SYMBOL(g_c_om6), t6, 512) != hipSuccess) return CRN2_EHIP;
-        if (hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<512>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<512>)) != hipSuccess ||
-            hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<1024>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<1024>)) != hipSuccess ||
-            hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<2048>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<2048>)) != hipSuccess ||
-            hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<kMaxU>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<kMaxU>)) != hipSuccess)
-            return CRN2_EHIP;
-        init = true;
+        tables = true;
     }
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    uint32_t* scratch = nullptr;      // 4 class counts + 4 lists of nclusters
-    if (hipMallocAsync((void**)&scratch, (4 + 4 * (size_t)nclusters) * sizeof(uint32_t), st) != hipSuccess)
-        return CRN2_ENOMEM;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_c_om5), t5, 512) != hipSuccess) return CRN2_EHIP;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_c_om6), t6, 512) != hipSuccess) return CRN2_EHIP;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<512>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<512>)) != hipSuccess ||
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<1024>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<1024>)) != hipSuccess ||
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<2048>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<2048>)) != hipSuccess ||
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<kMaxU>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<kMaxU>)) != hipSuccess)
+        return CRN2_EHIP;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CRN2_EHIP;
+    g_noalloc_scratch[dev] = static_cast<uint32_t*>(p);
+    g_init[dev] = true;
+    return CRN2_OK;
+}
+
+// The search proper.  work: 4 class counters + 4 lists of nclusters entries
+// (device memory).  fork: run the four size classes side by side on helper
+// streams; else every class runs on `st`, in turn (identical results).
+int clusters_run(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags, void* out,
+                 uint8_t* selectors, hipStream_t st, uint32_t* work, bool fork)
+{
+    hipLaunchKernelGGL(clear_class_counts_kernel, dim3(1), dim3(64), 0, st, work);
+    uint32_t* list = work + 4;
+    const uint32_t* px = static_cast<const uint32_t*>(rgba);
+    ClusterOut* o = static_cast<ClusterOut*>(out);
+    hipLaunchKernelGGL(dxt1_cluster_classes_kernel, dim3((nclusters + 255) / 256), dim3(256), 0, st, offsets,
+                       nclusters, work, list);
+    hipStream_t cls[4] = {st, st, st, st};
+    hipStream_t side[3];
+    hipEvent_t fork_ev, join[3];
+    std::unique_lock<std::mutex> lk(g_side_mu, std::defer_lock);
     int rc = CRN2_OK;
-    if (hipMemsetAsync(scratch, 0, 4 * sizeof(uint32_t), st) != hipSuccess) rc = CRN2_EHIP;
-    if (rc == CRN2_OK) {
-        uint32_t* list = scratch + 4;
-        const uint32_t* px = static_cast<const uint32_t*>(rgba);
-        ClusterOut* o = static_cast<ClusterOut*>(out);
-        hipLaunchKernelGGL(dxt1_cluster_classes_kernel, dim3((nclusters + 255) / 256), dim3(256), 0, st, offsets,
-                           nclusters, scratch, list);
+    if (fork) {
         // The four classes run side by side: the large clusters are few and
         // long, and alone would leave most CUs idle.  Three helper streams
         // (default priority, created once per device) fork from `st` and join
         // back into it.
-        hipStream_t side[3];
-        hipEvent_t fork, join[3];
-        std::lock_guard<std::mutex> lk(g_side_mu);
-        rc = side_streams(side, &fork, join);
-        if (rc == CRN2_OK && hipEventRecord(fork, st) != hipSuccess) rc = CRN2_EHIP;
+        lk.lock();
+        rc = side_streams(side, &fork_ev, join);
+        if (rc == CRN2_OK && hipEventRecord(fork_ev, st) != hipSuccess) rc = CRN2_EHIP;
         for (int i = 0; rc == CRN2_OK && i < 3; i++)
-            if (hipStreamWaitEvent(side[i], fork, 0) != hipSuccess) rc = CRN2_EHIP;
-        if (rc == CRN2_OK) {
-            hipLaunchKernelGGL(dxt1_cluster_kernel<kMaxU>, dim3(nclusters), dim3(64), sizeof(LdsT<kMaxU>), side[0], px,
-                               offsets, list + 3 * (size_t)nclusters, scratch + 3, flags & 1, o, selectors);
-            hipLaunchKernelGGL(dxt1_cluster_kernel<2048>, dim3(nclusters), dim3(64), sizeof(LdsT<2048>), side[1], px,
-                               offsets, list + 2 * (size_t)nclusters, scratch + 2, flags & 1, o, selectors);
-            hipLaunchKernelGGL(dxt1_cluster_kernel<1024>, dim3(nclusters), dim3(64), sizeof(LdsT<1024>), side[2], px,
-                               offsets, list + (size_t)nclusters, scratch + 1, flags & 1, o, selectors);
-            hipLaunchKernelGGL(dxt1_cluster_kernel<512>, dim3(nclusters), dim3(64), sizeof(LdsT<512>), st, px, offsets,
-                               list, scratch, flags & 1, o, selectors);
-            if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
-            for (int i = 0; i < 3; i++)
-                if (hipEventRecord(join[i], side[i]) != hipSuccess || hipStreamWaitEvent(st, join[i], 0) != hipSuccess)
-                    rc = CRN2_EHIP;
-        }
+            if (hipStreamWaitEvent(side[i], fork_ev, 0) != hipSuccess) rc = CRN2_EHIP;
+        if (rc != CRN2_OK) return rc;
+        cls[3] = side[0];
+        cls[2] = side[1];
+        cls[1] = side[2];
     }
-    (void)hipFreeAsync(scratch, st);
+    hipLaunchKernelGGL(dxt1_cluster_kernel<kMaxU>, dim3(nclusters), dim3(64), sizeof(LdsT<kMaxU>), cls[3], px,
+                       offsets, list + 3 * (size_t)nclusters, work + 3, flags & 1, o, selectors);
+    hipLaunchKernelGGL(dxt1_cluster_kernel<2048>, dim3(nclusters), dim3(64), sizeof(LdsT<2048>), cls[2], px,
+                       offsets, list + 2 * (size_t)nclusters, work + 2, flags & 1, o, selectors);
+    hipLaunchKernelGGL(dxt1_cluster_kernel<1024>, dim3(nclusters), dim3(64), sizeof(LdsT<1024>), cls[1], px,
+                       offsets, list + (size_t)nclusters, work + 1, flags & 1, o, selectors);
+    hipLaunchKernelGGL(dxt1_cluster_kernel<512>, dim3(nclusters), dim3(64), sizeof(LdsT<512>), cls[0], px, offsets,
+                       list, work, flags & 1, o, selectors);
+    if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
+    if (fork)
+        for (int i = 0; i < 3; i++)
+            if (hipEventRecord(join[i], side[i]) != hipSuccess || hipStreamWaitEvent(st, join[i], 0) != hipSuccess)
+                rc = CRN2_EHIP;
     return rc;
+}
+}  // namespace
+
+extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags,
+                                         void* out, uint8_t* selectors, void* stream)
+{
+    if (!rgba || !offsets || !out || (flags & ~1)) return CRN2_EINVAL;
+    if (!nclusters) return CRN2_OK;
+    int dev = 0;
+    int rc = clusters_init(&dev);
+    if (rc != CRN2_OK) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    uint32_t* work = nullptr;      // 4 class counts + 4 lists of nclusters
+    if (hipMallocAsync((void**)&work, (4 + 4 * (size_t)nclusters) * sizeof(uint32_t), st) != hipSuccess)
+        return CRN2_ENOMEM;
+    rc = clusters_run(rgba, offsets, nclusters, flags, out, selectors, st, work, true);
+    (void)hipFreeAsync(work, st);
+    return rc;
+}
+
+extern "C" int crn2_dxt1_clusters_noalloc_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters,
+                                                 int flags, void* out, uint8_t* selectors, void* stream)
+{
+    if (!rgba || !offsets || !out || (flags & ~1) || nclusters > kNoAllocMaxClusters) return CRN2_EINVAL;
+    if (!nclusters) return CRN2_OK;
+    int dev = 0;
+    const int rc = clusters_init(&dev);
+    if (rc != CRN2_OK) return rc;
+    return clusters_run(rgba, offsets, nclusters, flags, out, selectors, static_cast<hipStream_t>(stream),
+                        g_noalloc_scratch[dev], false);
 }

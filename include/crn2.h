@@ -196,6 +196,19 @@ int crn2_dxt1_blocks_device(const void* rgba, uint32_t nblocks, int quality, int
 int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags, void* out,
                               uint8_t* selectors, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): crn2_dxt1_clusters_device with the same arguments and
+ * results, bit for bit, that allocates nothing.  Its scratch (4 counters and
+ * 4 lists of cluster indices) is a fixed array in device memory of 1 MiB per
+ * GPU, held until the process ends, so nclusters is at most 65,536
+ * (CRN2_EINVAL above).  The four cluster size classes run in turn on `stream`
+ * (no helper streams).  Every call on one GPU uses that GPU's one array:
+ * calls that overlap on one GPU (other streams, other threads) race and must
+ * be serialised by the caller. */
+int crn2_dxt1_clusters_noalloc_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags,
+                                      void* out, uint8_t* selectors, void* stream);
+
 /* ------------------------------------ dxt_hc colour selector assignment */
 /* dxt_hc::create_color_selector_codebook_task (crnlib/crn_dxt_hc.cpp:1306-1360)
  * for all blocks at once, on the device.  blocks: n_blocks x 16 RGBA8 texels;
