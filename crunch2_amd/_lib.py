@@ -84,6 +84,16 @@ SIGNATURES = {
     "crn2_encode_dxt5a_image_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
                                                  C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "crn2_dxt1_blocks_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "crn2_encode_dxt1_image_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_void_p,
+                                                C.c_uint32, C.c_void_p, C.c_void_p]),
+    "crn2_encode_image_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
+                                           C.c_void_p, C.c_void_p]),
+    "crn2_compress_blocks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_void_p]),
+    "crn2_decode_dxt_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                         C.c_uint32, C.c_void_p]),
+    "crn2_dxt_error_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                        C.c_uint32, C.c_void_p, C.c_void_p]),
+    "crn2_decompress_blocks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),
     "crn2_dxt1_clusters_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p,
                                             C.c_void_p]),
     "crn2_dxt1_clusters_noalloc_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
@@ -136,6 +146,18 @@ SIGNATURES = {
     "crn2_hc_alpha_endpoints_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                                                  C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                                  C.c_void_p, C.c_void_p, C.c_void_p]),
+    "crn2_dxt1_blocks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p]),
+    "crn2_dxt1_clusters": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p]),
+    "crn2_dxt5a_blocks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "crn2_encode_image": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_void_p]),
+    "crn2_compress_crn": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_uint32, C.c_int, C.POINTER(C.c_uint32), C.c_void_p, C.c_uint32, C.c_uint32,
+                                    C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]),
+    "crn2_compress_dds": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_uint32, C.c_int, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]),
+    "crn2_write_dds": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]),
+    "crn2_crn_to_dds": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]),
     "crn2_crc16": (C.c_uint16, [C.c_void_p, C.c_uint32, C.c_uint16]),
     "crn2_synth_rgba": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_void_p]),
     "crn2_downsample": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p]),

@@ -51,7 +51,8 @@ bool parse_header(const uint8_t* d, uint32_t size, Header& h)
     }
     h.tables_size = be(d + 65, 2);
     h.tables_ofs = be(d + 67, 3);
-    if (h.header_size < 74 || size < h.data_size) return false;
+    // the CRCs run over [6, header_size) and [header_size, data_size): both must lie inside the buffer
+    if (h.header_size < 74 || size < h.data_size || h.data_size < h.header_size) return false;
     for (uint32_t l = 0; l < kMaxLevels; l++)
         h.level_ofs[l] = (l < h.levels && 70 + 4 * l + 4 <= h.header_size) ? be(d + 70 + 4 * l, 4) : 0;
     return true;

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "autotune.h"
 #include "crn2_internal.h"
@@ -1062,6 +1063,31 @@ __global__ void __launch_bounds__(256, 2) dxt1_blocks_kernel(const uint32_t* __r
     out[b] = encode_block(p, quality, flags);
 }
 
+// Whole image, as dxt_image::init_task and set_block_pixels do for cColorDXT1
+// elements (crnlib/crn_dxt_image.cpp:283-349, 1436-1484): the block's 4x4
+// texels with edge clamp, the search, then the 8-byte dxt1_block (low colour,
+// high colour, 2-bit selectors row by row, texel (x, y) at bit 2 (x + 4 y) of
+// the last four bytes) at blocks[i * stride]; errors (optional) = m_error.
+__global__ void __launch_bounds__(256, 2) dxt1_image_kernel(const uint32_t* __restrict__ rgba, uint32_t width,
+                                                           uint32_t height, uint32_t blocks_x, uint32_t nblocks,
+                                                           int quality, int flags, uint64_t* __restrict__ out,
+                                                           uint32_t stride, uint64_t* __restrict__ err)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    const uint32_t bx = b % blocks_x, by = b / blocks_x;
+    uint32_t p[16];
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+        const uint32_t sy = min(by * 4 + y, height - 1);
+#pragma unroll
+        for (int x = 0; x < 4; x++) p[4 * y + x] = rgba[(size_t)sy * width + min(bx * 4 + x, width - 1)];
+    }
+    const Dxt1Out o = encode_block(p, quality, flags);
+    out[(size_t)b * stride] = (uint64_t)o.lo_hi | ((uint64_t)o.selectors << 32);
+    if (err) err[b] = o.error;
+}
+
 }  // namespace
 
 extern "C" int crn2_dxt1_init(void)
@@ -1098,18 +1124,33 @@ extern "C" int crn2_dxt1_init(void)
     return CRN2_OK;
 }
 
+namespace {
+// crn2_dxt1_init once per device (the tables are device memory of each GPU)
+std::mutex g_tables_mu;
+bool g_tables[64];
+
+int dxt1_tables_ready()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return CRN2_EHIP;
+    std::lock_guard<std::mutex> lk(g_tables_mu);
+    if (!g_tables[dev]) {
+        const int rc = crn2_dxt1_init();
+        if (rc) return rc;
+        g_tables[dev] = true;
+    }
+    return CRN2_OK;
+}
+}  // namespace
+
 extern "C" int crn2_dxt1_blocks_device(const void* rgba, uint32_t nblocks, int quality, int flags, void* out,
                                        void* stream)
 {
     if (!rgba || !out || quality < 3 || quality > 4 || (flags & ~15)) return CRN2_EINVAL;
     if (reinterpret_cast<uintptr_t>(rgba) & 15) return CRN2_EINVAL;
     if (!nblocks) return CRN2_OK;
-    static bool init = false;
-    if (!init) {
-        const int rc = crn2_dxt1_init();
-        if (rc) return rc;
-        init = true;
-    }
+    const int rc = dxt1_tables_ready();
+    if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     // launch shapes (threads per workgroup, one block per lane; identical results): 64, 128, 256
     auto launch = [&](int shape) -> hipError_t {
@@ -1120,4 +1161,22 @@ extern "C" int crn2_dxt1_blocks_device(const void* rgba, uint32_t nblocks, int q
     };
     const int shape = crn2::tuner().pick(crn2::TUNE_DXT1, nblocks, 3, 0, s, launch);
     return launch(shape) == hipSuccess ? CRN2_OK : CRN2_EHIP;
+}
+
+extern "C" int crn2_encode_dxt1_image_device(const void* rgba, uint32_t width, uint32_t height, int quality, int flags,
+                                             void* blocks, uint32_t block_stride_u64, void* errors, void* stream)
+{
+    if (!rgba || !blocks || !width || !height || !block_stride_u64 || quality < 3 || quality > 4 || (flags & ~15))
+        return CRN2_EINVAL;
+    if (reinterpret_cast<uintptr_t>(rgba) & 3) return CRN2_EINVAL;
+    const uint64_t bx = (width + 3) / 4, by = (height + 3) / 4, n = bx * by;
+    if (n > 0xFFFFFFFFull) return CRN2_EINVAL;
+    const int rc = dxt1_tables_ready();
+    if (rc) return rc;
+    const uint32_t tb = 256;
+    hipLaunchKernelGGL(dxt1_image_kernel, dim3((uint32_t)((n + tb - 1) / tb)), dim3(tb), 0,
+                       static_cast<hipStream_t>(stream), static_cast<const uint32_t*>(rgba), width, height,
+                       (uint32_t)bx, (uint32_t)n, quality, flags, static_cast<uint64_t*>(blocks), block_stride_u64,
+                       static_cast<uint64_t*>(errors));
+    return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
 }

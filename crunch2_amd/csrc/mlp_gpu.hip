@@ -756,12 +756,15 @@ int crn2_mlp_infer_device(const void* rgba, uint32_t n, const float* theta, floa
 {
     if (!n) return CRN2_OK;
     if (!rgba || !theta || !y) return CRN2_EINVAL;
-    static bool attr = false;
-    if (!attr) {
+    // the LDS limit is a property of the kernel on each device: set it once per device
+    static bool attr[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return CRN2_EHIP;
+    if (!attr[dev]) {
         if (hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_infer_kernel),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kInferLds) != hipSuccess)
             return CRN2_EHIP;
-        attr = true;
+        attr[dev] = true;
     }
     const uint32_t tiles = (n + 31) / 32;
     const uint32_t wgs = std::min<uint32_t>((tiles + kInferWaves - 1) / kInferWaves, 1024);

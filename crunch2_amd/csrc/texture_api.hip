@@ -1,0 +1,339 @@
+// texture_api.hip -- whole-level DXTn encode (the plain, unclustered path of
+// crnlib's dxt_image) and the host-memory block API on top of it.
+//
+// Reference: dxt_image::init_internal's element layout per format
+// (crnlib/crn_dxt_image.cpp:120-200: DXT1 = colour; DXT3 = DXT3 alpha of
+// channel 3 + colour; DXT5 = DXT5 alpha of channel 3 + colour; DXT5A = DXT5
+// alpha of channel 3; DXN_XY = DXT5 alpha of channels 0 and 1; DXN_YX = of
+// channels 1 and 0), set_block_pixels (:1436-1540: 3-colour DXT1 blocks only
+// for DXT1 itself, dxt3_block::set_alpha's (a * 15 + 128) / 255,
+// crn_dxt.cpp:327-348) and the block compressor of crnlib.cpp:345-410.
+// Every element is produced by the GPU searches (crn2_encode_dxt1_image_device,
+// crn2_encode_dxt5a_image_device) writing into its slot of the interleaved
+// output; DXT3 alpha is one small kernel here.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "crn2_internal.h"
+
+namespace {
+
+__global__ void __launch_bounds__(256) dxt3_alpha_kernel(const uint32_t* __restrict__ rgba, uint32_t width,
+                                                         uint32_t height, uint32_t blocks_x, uint32_t nblocks,
+                                                         uint64_t* __restrict__ out, uint32_t stride)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    const uint32_t bx = b % blocks_x, by = b / blocks_x;
+    uint64_t v = 0;
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+        const uint32_t sy = min(by * 4 + y, height - 1);
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+            const uint32_t a = rgba[(size_t)sy * width + min(bx * 4 + x, width - 1)] >> 24;
+            v |= (uint64_t)((a * 15u + 128u) / 255u) << (4 * (4 * y + x));
+        }
+    }
+    out[(size_t)b * stride] = v;
+}
+
+}  // namespace
+
+extern "C" int crn2_encode_image_device(const void* rgba, uint32_t width, uint32_t height, uint32_t format,
+                                        int quality, int flags, void* blocks, void* stream)
+{
+    if (!rgba || !blocks || !width || !height || quality < 0 || quality > 4 || (flags & ~7)) return CRN2_EINVAL;
+    if (reinterpret_cast<uintptr_t>(rgba) & 3) return CRN2_EINVAL;
+    const uint64_t bx = (width + 3) / 4, n = bx * ((height + 3) / 4);
+    if (n > 0xFFFFFFFFull) return CRN2_EINVAL;
+    const bool perceptual = flags & 1, both = (flags >> 1) & 1, gray = (flags >> 2) & 1;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    uint64_t* out = static_cast<uint64_t*>(blocks);
+    // colour elements: the better and uber searches (the fast evaluator of lower qualities is not built)
+    const bool has_color = format <= CRN2_FMT_DXT5_AGBR;
+    if (has_color && quality < 3) return CRN2_EUNSUPPORTED;
+    const int cq = quality;
+    const int cflags = (perceptual ? 1 : 0) | (gray ? 4 : 0);
+    switch (format) {
+        case CRN2_FMT_DXT1:
+            return crn2_encode_dxt1_image_device(rgba, width, height, cq, cflags | (both ? 2 : 0), out, 1, nullptr,
+                                                 stream);
+        case CRN2_FMT_DXT3: {
+            const uint32_t tb = 256;
+            hipLaunchKernelGGL(dxt3_alpha_kernel, dim3((uint32_t)((n + tb - 1) / tb)), dim3(tb), 0, st,
+                               static_cast<const uint32_t*>(rgba), width, height, (uint32_t)bx, (uint32_t)n, out, 2);
+            if (hipGetLastError() != hipSuccess) return CRN2_EHIP;
+            return crn2_encode_dxt1_image_device(rgba, width, height, cq, cflags, out + 1, 2, nullptr, stream);
+        }
+        case CRN2_FMT_DXT5:
+        case CRN2_FMT_DXT5_CCXY:
+        case CRN2_FMT_DXT5_XGXR:
+        case CRN2_FMT_DXT5_XGBR:
+        case CRN2_FMT_DXT5_AGBR: {
+            const int rc = crn2_encode_dxt5a_image_device(rgba, width, height, 3, quality, both, out, 2, nullptr,
+                                                          stream);
+            if (rc) return rc;
+            return crn2_encode_dxt1_image_device(rgba, width, height, cq, cflags, out + 1, 2, nullptr, stream);
+        }
+        case CRN2_FMT_DXT5A:
+            return crn2_encode_dxt5a_image_device(rgba, width, height, 3, quality, both, out, 1, nullptr, stream);
+        case CRN2_FMT_DXN_XY:
+        case CRN2_FMT_DXN_YX: {
+            const uint32_t c0 = format == CRN2_FMT_DXN_XY ? 0 : 1;
+            const int rc = crn2_encode_dxt5a_image_device(rgba, width, height, c0, quality, both, out, 2, nullptr,
+                                                          stream);
+            if (rc) return rc;
+            return crn2_encode_dxt5a_image_device(rgba, width, height, 1 - c0, quality, both, out + 1, 2, nullptr,
+                                                  stream);
+        }
+        default:
+            return CRN2_EUNSUPPORTED;
+    }
+}
+
+extern "C" int crn2_compress_blocks(const uint32_t* pixels, uint32_t nblocks, uint32_t format, int quality,
+                                    int flags, void* out)
+{
+    if (!pixels || !out) return CRN2_EINVAL;
+    if (!nblocks) return CRN2_OK;
+    if (nblocks > (1u << 26)) return CRN2_EINVAL;
+    if (format > CRN2_FMT_DXT5A) return CRN2_EUNSUPPORTED;
+    const uint32_t bpb = (format == CRN2_FMT_DXT1 || format == CRN2_FMT_DXT5A) ? 8 : 16;
+    // nblocks blocks of 16 consecutive texels = one image 4 texels wide and
+    // 4 * nblocks high, whose block column is the blocks in order
+    void* d_px = nullptr;
+    void* d_out = nullptr;
+    const size_t in = (size_t)nblocks * 64, outb = (size_t)nblocks * bpb;
+    if (hipMalloc(&d_px, in) != hipSuccess) return CRN2_ENOMEM;
+    if (hipMalloc(&d_out, outb) != hipSuccess) {
+        (void)hipFree(d_px);
+        return CRN2_ENOMEM;
+    }
+    int rc = CRN2_OK;
+    if (hipMemcpy(d_px, pixels, in, hipMemcpyHostToDevice) != hipSuccess) rc = CRN2_EHIP;
+    if (rc == CRN2_OK) rc = crn2_encode_image_device(d_px, 4, 4 * nblocks, format, quality, flags, d_out, nullptr);
+    if (rc == CRN2_OK && hipMemcpy(out, d_out, outb, hipMemcpyDeviceToHost) != hipSuccess) rc = CRN2_EHIP;
+    (void)hipFree(d_px);
+    (void)hipFree(d_out);
+    return rc;
+}
+
+namespace {
+// Device staging for the host-memory entry points: frees what it allocated.
+struct Staging {
+    void* p[8] = {};
+    int n = 0;
+    ~Staging()
+    {
+        for (int i = 0; i < n; i++) (void)hipFree(p[i]);
+    }
+    void* alloc(size_t bytes)
+    {
+        void* q = nullptr;
+        if (n >= 8 || hipMalloc(&q, bytes ? bytes : 1) != hipSuccess) return nullptr;
+        p[n++] = q;
+        return q;
+    }
+    void* in(const void* host, size_t bytes)
+    {
+        void* q = alloc(bytes);
+        if (q && bytes && hipMemcpy(q, host, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        return q;
+    }
+};
+inline int out(void* host, const void* dev, size_t bytes)
+{
+    return (!bytes || hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost) == hipSuccess) ? CRN2_OK : CRN2_EHIP;
+}
+}  // namespace
+
+extern "C" int crn2_dxt1_blocks(const void* rgba, uint32_t nblocks, int quality, int flags, void* results)
+{
+    if (!rgba || !results) return CRN2_EINVAL;
+    if (!nblocks) return CRN2_OK;
+    Staging s;
+    void* d_in = s.in(rgba, (size_t)nblocks * 64);
+    void* d_out = s.alloc((size_t)nblocks * 24);
+    if (!d_in || !d_out) return CRN2_ENOMEM;
+    int rc = crn2_dxt1_blocks_device(d_in, nblocks, quality, flags, d_out, nullptr);
+    if (rc == CRN2_OK) rc = out(results, d_out, (size_t)nblocks * 24);
+    return rc;
+}
+
+extern "C" int crn2_dxt1_clusters(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags,
+                                  void* results, uint8_t* selectors)
+{
+    if (!rgba || !offsets || !results) return CRN2_EINVAL;
+    if (!nclusters) return CRN2_OK;
+    const uint32_t total = offsets[nclusters];
+    for (uint32_t c = 0; c < nclusters; c++)
+        if (offsets[c + 1] < offsets[c]) return CRN2_EINVAL;
+    Staging s;
+    void* d_px = s.in(rgba, (size_t)total * 4);
+    void* d_ofs = s.in(offsets, ((size_t)nclusters + 1) * 4);
+    void* d_out = s.alloc((size_t)nclusters * 24);
+    uint8_t* d_sel = selectors ? static_cast<uint8_t*>(s.alloc(total)) : nullptr;
+    if (!d_px || !d_ofs || !d_out || (selectors && !d_sel)) return CRN2_ENOMEM;
+    int rc = crn2_dxt1_clusters_device(d_px, static_cast<const uint32_t*>(d_ofs), nclusters, flags, d_out, d_sel,
+                                       nullptr);
+    if (rc == CRN2_OK && hipDeviceSynchronize() != hipSuccess) rc = CRN2_EHIP;
+    if (rc == CRN2_OK) rc = out(results, d_out, (size_t)nclusters * 24);
+    if (rc == CRN2_OK && selectors) rc = out(selectors, d_sel, total);
+    return rc;
+}
+
+extern "C" int crn2_dxt5a_blocks(const void* values, uint32_t nblocks, int quality, int use_both, void* selectors48,
+                                 void* errors, void* meta)
+{
+    if (!values || !selectors48 || !errors || !meta) return CRN2_EINVAL;
+    if (!nblocks) return CRN2_OK;
+    Staging s;
+    void* d_v = s.in(values, (size_t)nblocks * 16);
+    void* d_s = s.alloc((size_t)nblocks * 8);
+    void* d_e = s.alloc((size_t)nblocks * 8);
+    void* d_m = s.alloc((size_t)nblocks * 4);
+    if (!d_v || !d_s || !d_e || !d_m) return CRN2_ENOMEM;
+    int rc = crn2_dxt5a_blocks_device(d_v, nblocks, quality, use_both, d_s, d_e, d_m, nullptr);
+    if (rc == CRN2_OK) rc = out(selectors48, d_s, (size_t)nblocks * 8);
+    if (rc == CRN2_OK) rc = out(errors, d_e, (size_t)nblocks * 8);
+    if (rc == CRN2_OK) rc = out(meta, d_m, (size_t)nblocks * 4);
+    return rc;
+}
+
+extern "C" int crn2_encode_image(const void* rgba, uint32_t width, uint32_t height, uint32_t format, int quality,
+                                 int flags, void* blocks)
+{
+    if (!rgba || !blocks || !width || !height) return CRN2_EINVAL;
+    if (format > CRN2_FMT_DXT5A) return CRN2_EUNSUPPORTED;
+    const uint32_t bpb = (format == CRN2_FMT_DXT1 || format == CRN2_FMT_DXT5A) ? 8 : 16;
+    const size_t n = (size_t)((width + 3) / 4) * ((height + 3) / 4);
+    Staging s;
+    void* d_in = s.in(rgba, (size_t)width * height * 4);
+    void* d_out = s.alloc(n * bpb);
+    if (!d_in || !d_out) return CRN2_ENOMEM;
+    int rc = crn2_encode_image_device(d_in, width, height, format, quality, flags, d_out, nullptr);
+    if (rc == CRN2_OK) rc = out(blocks, d_out, n * bpb);
+    return rc;
+}
+
+extern "C" int crn2_compress_crn(const uint8_t* const* levels, uint32_t width, uint32_t height, uint32_t nlevels,
+                                 uint32_t faces, uint32_t format, int perceptual, const uint32_t* codebook_sizes,
+                                 const float* tile_params, uint32_t userdata0, uint32_t userdata1, uint8_t** out_file,
+                                 uint32_t* out_size)
+{
+    if (!levels || !codebook_sizes || !out_file || !out_size) return CRN2_EINVAL;
+    *out_file = nullptr;
+    *out_size = 0;
+    if (!width || !height || width > 4096 || height > 4096 || !nlevels || nlevels > 16 || (faces != 1 && faces != 6))
+        return CRN2_EINVAL;
+    int has_color = 0;
+    uint32_t num_alpha = 0, alpha_comp[2] = {3, 0};
+    switch (format) {
+        case CRN2_FMT_DXT1: has_color = 1; break;
+        case CRN2_FMT_DXT5: has_color = 1; num_alpha = 1; break;
+        case CRN2_FMT_DXT5A: num_alpha = 1; break;
+        case CRN2_FMT_DXN_XY: num_alpha = 2; alpha_comp[0] = 0; alpha_comp[1] = 1; break;
+        case CRN2_FMT_DXN_YX: num_alpha = 2; alpha_comp[0] = 1; alpha_comp[1] = 0; break;
+        default: return CRN2_EUNSUPPORTED;
+    }
+    for (uint32_t i = 0; i < faces * nlevels; i++)
+        if (!levels[i]) return CRN2_EINVAL;
+    // crn_comp::quantize_images (crnlib/crn_comp.cpp:706-738): per level the
+    // faces' blocks over a grid padded to a multiple of 8 texels, edge clamp,
+    // level weights min(12, 1.3^level)
+    std::vector<uint32_t> desc(3 * nlevels);
+    std::vector<float> weight(nlevels);
+    size_t total = 0;
+    for (uint32_t l = 0; l < nlevels; l++) {
+        const uint32_t w = std::max<uint32_t>(1, width >> l), h = std::max<uint32_t>(1, height >> l);
+        const uint32_t bw = ((w + 7) & ~7u) >> 2, bh = ((h + 7) & ~7u) >> 2;
+        desc[3 * l] = (uint32_t)total;
+        desc[3 * l + 1] = bw * bh * faces;
+        desc[3 * l + 2] = bw;
+        weight[l] = std::min(12.0f, powf(1.3f, (float)l));
+        total += (size_t)bw * bh * faces;
+    }
+    std::vector<uint32_t> blocks(total * 16);
+    size_t b = 0;
+    for (uint32_t l = 0; l < nlevels; l++) {
+        const uint32_t w = std::max<uint32_t>(1, width >> l), h = std::max<uint32_t>(1, height >> l);
+        const uint32_t bw = desc[3 * l + 2], bh = desc[3 * l + 1] / (bw * faces);
+        for (uint32_t f = 0; f < faces; f++) {
+            const uint32_t* img = reinterpret_cast<const uint32_t*>(levels[f * nlevels + l]);
+            for (uint32_t by = 0; by < bh; by++)
+                for (uint32_t bx = 0; bx < bw; bx++, b++)
+                    for (uint32_t t = 0; t < 16; t++) {
+                        const uint32_t x = std::min(bx * 4 + (t & 3), w - 1), y = std::min(by * 4 + (t >> 2), h - 1);
+                        blocks[b * 16 + t] = img[(size_t)y * w + x];
+                    }
+        }
+    }
+    Staging s;
+    void* d_blocks = s.in(blocks.data(), total * 64);
+    uint16_t* d_ep = static_cast<uint16_t*>(s.alloc(total * 8));
+    uint16_t* d_sel = static_cast<uint16_t*>(s.alloc(total * 6));
+    uint32_t* d_ce = static_cast<uint32_t*>(s.alloc((size_t)std::max(codebook_sizes[0], 1u) * 4));
+    uint32_t* d_ae = static_cast<uint32_t*>(s.alloc((size_t)std::max(codebook_sizes[2], 1u) * 4));
+    uint32_t* d_cs = static_cast<uint32_t*>(s.alloc((size_t)std::max(codebook_sizes[1], 1u) * 4));
+    uint64_t* d_as = static_cast<uint64_t*>(s.alloc((size_t)std::max(codebook_sizes[3], 1u) * 8));
+    if (!d_blocks || !d_ep || !d_sel || !d_ce || !d_ae || !d_cs || !d_as) return CRN2_ENOMEM;
+    uint32_t counts[4] = {0, 0, 0, 0};
+    int rc = crn2_hc_compress_device(d_blocks, (uint32_t)total, nlevels, desc.data(), weight.data(), faces, has_color,
+                                     num_alpha, alpha_comp, perceptual, codebook_sizes, tile_params, d_ep, d_sel,
+                                     d_ce, d_ae, d_cs, d_as, counts, nullptr);
+    if (rc != CRN2_OK) return rc;
+    if (hipDeviceSynchronize() != hipSuccess) return CRN2_EHIP;
+    std::vector<uint16_t> ep(total * 4), sel(total * 3);
+    std::vector<uint32_t> ce(counts[0] + 1), cs(counts[1] + 1), ae(counts[2] + 1);
+    std::vector<uint64_t> as(counts[3] + 1);
+    if (out(ep.data(), d_ep, total * 8) || out(sel.data(), d_sel, total * 6) ||
+        out(ce.data(), d_ce, (size_t)counts[0] * 4) || out(cs.data(), d_cs, (size_t)counts[1] * 4) ||
+        out(ae.data(), d_ae, (size_t)counts[2] * 4) || out(as.data(), d_as, (size_t)counts[3] * 8))
+        return CRN2_EHIP;
+    std::vector<uint32_t> idx[6];
+    for (int k = 0; k < 6; k++) idx[k].resize(total);
+    for (size_t i = 0; i < total; i++) {
+        idx[0][i] = ep[4 * i];        // colour endpoint
+        idx[1][i] = sel[3 * i];       // colour selector
+        idx[2][i] = ep[4 * i + 1];    // alpha 0 endpoint
+        idx[3][i] = sel[3 * i + 1];   // alpha 0 selector
+        idx[4][i] = ep[4 * i + 2];    // alpha 1 endpoint
+        idx[5][i] = sel[3 * i + 2];   // alpha 1 selector
+    }
+    crn2_pack_input in;
+    memset(&in, 0, sizeof(in));
+    in.width = width;
+    in.height = height;
+    in.levels = nlevels;
+    in.faces = faces;
+    in.format = format;
+    in.userdata0 = userdata0;
+    in.userdata1 = userdata1;
+    if (has_color) {
+        in.color_endpoints = ce.data();
+        in.n_color_endpoints = counts[0];
+        in.color_selectors_linear = cs.data();
+        in.n_color_selectors = counts[1];
+        in.color_endpoint_idx = idx[0].data();
+        in.color_selector_idx = idx[1].data();
+    }
+    if (num_alpha) {
+        in.alpha_endpoints = ae.data();
+        in.n_alpha_endpoints = counts[2];
+        in.alpha_selectors_linear = as.data();
+        in.n_alpha_selectors = counts[3];
+        in.alpha0_endpoint_idx = idx[2].data();
+        in.alpha0_selector_idx = idx[3].data();
+        if (num_alpha > 1) {
+            in.alpha1_endpoint_idx = idx[4].data();
+            in.alpha1_selector_idx = idx[5].data();
+        }
+    }
+    return crn2_pack(&in, out_file, out_size);
+}

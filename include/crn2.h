@@ -181,6 +181,52 @@ int crn2_encode_dxt5a_image_device(const void* rgba, uint32_t width, uint32_t he
  * u32 0}. */
 int crn2_dxt1_blocks_device(const void* rgba, uint32_t nblocks, int quality, int flags, void* out, void* stream);
 
+/* Plain DXT1 encode of an RGBA8 image in device memory (width x height,
+ * rows of width texels, 4-byte aligned), block by block as dxt_image does for
+ * cColorDXT1 elements (crnlib/crn_dxt_image.cpp:283-349 gather with edge
+ * clamp, :1436-1484 set_block_pixels): per block the 8-byte dxt1_block (low
+ * colour, high colour, 2-bit selectors row by row) at blocks + i *
+ * block_stride_u64 (u64 units, blocks in raster order), and errors (optional,
+ * u64 results::m_error).  quality and flags as crn2_dxt1_blocks_device. */
+int crn2_encode_dxt1_image_device(const void* rgba, uint32_t width, uint32_t height, int quality, int flags,
+                                  void* blocks, uint32_t block_stride_u64, void* errors, void* stream);
+
+/* One whole level in any of the DXTn formats (CRN2_FMT_DXT1 .. DXT5A), the
+ * unclustered dxt_image path: the element layout of dxt_image::init_internal
+ * (crnlib/crn_dxt_image.cpp:120-200) filled by the GPU searches; the output is
+ * the level's DXTn blocks in raster order at minimal pitch.  quality 0..4
+ * (crn_dxt_quality; colour formats need 3 or 4: CRN2_EUNSUPPORTED below);
+ * flags: bit0 perceptual, bit1 use both block types (3-colour DXT1 blocks for
+ * DXT1 only, 6-value alpha blocks), bit2 grayscale sampling. */
+int crn2_encode_image_device(const void* rgba, uint32_t width, uint32_t height, uint32_t format, int quality,
+                             int flags, void* blocks, void* stream);
+
+/* Host-memory form for any number of 4x4 blocks (16 RGBA8 texels each,
+ * r first): the work of crn_compress_block (inc/crnlib.h:679,
+ * crnlib/crnlib.cpp:345-410) for nblocks blocks in one call; out receives
+ * nblocks DXTn blocks of the format.  Synchronous. */
+int crn2_compress_blocks(const uint32_t* pixels, uint32_t nblocks, uint32_t format, int quality, int flags,
+                         void* out);
+
+/* ------------------------------------------------------------ DXTn decode */
+/* crn_decompress_block (inc/crnlib.h:690, crnlib/crnlib.cpp:420-543) for a
+ * whole level on the device: blocks (rows of (width+3)/4 blocks,
+ * block_row_pitch bytes apart, 0 = minimal) of the format decoded to RGBA8
+ * texels (rgba: height rows of width texels, rgba_row_pitch bytes apart, 0 =
+ * width * 4).  Texels past the image edge are not written.  The swizzled DXT5
+ * formats decode as DXT5; unlike the reference, DXT5 does not fall through
+ * into the DXN case (crnlib.cpp:485-503). */
+int crn2_decode_dxt_device(const void* blocks, uint32_t format, uint32_t width, uint32_t height,
+                           uint32_t block_row_pitch, void* rgba, uint32_t rgba_row_pitch, void* stream);
+/* The squared error of the decoded level against src_rgba, per channel
+ * (r, g, b, a), added into sse[4] (device, u64): the sums of
+ * error_metrics::compute (crnlib/crn_image_utils.cpp:1048-1120). */
+int crn2_dxt_error_device(const void* blocks, uint32_t format, uint32_t width, uint32_t height,
+                          uint32_t block_row_pitch, const void* src_rgba, uint32_t src_row_pitch, uint64_t* sse,
+                          void* stream);
+/* Host-memory form: nblocks consecutive blocks to 16 texels each.  Synchronous. */
+int crn2_decompress_blocks(const void* blocks, uint32_t nblocks, uint32_t format, uint32_t* rgba);
+
 /* The same search for pixel clusters of any size on the path dxt_hc takes for
  * its endpoint clusters (crn_dxt_hc.cpp:663-754 calling
  * dxt1_endpoint_optimizer::compute with uber quality and no 3-colour blocks;
@@ -476,6 +522,44 @@ int crn2_dxt1_refine_device(const void* rgba, uint32_t n, const float* y, uint32
  * when it is not null). */
 int crn2_dxt1_block_error_device(const void* rgba, uint32_t n, const uint32_t* endpoints, uint32_t stride_u32,
                                  uint32_t* err, uint32_t* selectors, void* stream);
+
+/* ------------------------------------------- host-memory (synchronous) */
+/* The same work as the _device entry points of the same name, for callers
+ * with host buffers: inputs staged to the device, results copied back before
+ * the call returns (these back the C++ facades of include/cxx). */
+int crn2_dxt1_blocks(const void* rgba, uint32_t nblocks, int quality, int flags, void* results);
+int crn2_dxt1_clusters(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags, void* results,
+                       uint8_t* selectors);
+int crn2_dxt5a_blocks(const void* values, uint32_t nblocks, int quality, int use_both, void* selectors48,
+                      void* errors, void* meta);
+/* One whole level (rgba: width x height texels) to DXTn blocks as
+ * crn2_encode_image_device. */
+int crn2_encode_image(const void* rgba, uint32_t width, uint32_t height, uint32_t format, int quality, int flags,
+                      void* blocks);
+
+/* ------------------------------------------------------------ whole files */
+/* The CRN path of crn_compress (inc/crnlib.h:609, crnlib/crn_comp.cpp:
+ * 521-742): levels[f * nlevels + l] = RGBA8 level l of face f (max(1, width
+ * >> l) x max(1, height >> l)); the blocks of every level (grid padded to a
+ * multiple of 8 texels, edge clamp, level weights min(12, 1.3^l)) go through
+ * the GPU dxt_hc (crn2_hc_compress_device with codebook_sizes and
+ * tile_params as there) and the project's CRN writer.  Formats DXT1, DXT5,
+ * DXT5A, DXN_XY, DXN_YX.  *out is freed with crn2_free. */
+int crn2_compress_crn(const uint8_t* const* levels, uint32_t width, uint32_t height, uint32_t nlevels, uint32_t faces,
+                      uint32_t format, int perceptual, const uint32_t* codebook_sizes, const float* tile_params,
+                      uint32_t userdata0, uint32_t userdata1, uint8_t** out, uint32_t* out_size);
+/* The plain DDS path of crn_compress: every level of every face through
+ * crn2_encode_image, written by crn2_write_dds. */
+int crn2_compress_dds(const uint8_t* const* levels, uint32_t width, uint32_t height, uint32_t nlevels,
+                      uint32_t faces, uint32_t format, int quality, int flags, uint8_t** out, uint32_t* out_size);
+/* A DDS file as mipmapped_texture::write_dds writes a DXTn texture
+ * (crnlib/crn_mipmapped_texture.cpp:921-1140): level_data[f * levels + l] =
+ * that level's blocks at minimal pitch.  *out is freed with crn2_free. */
+int crn2_write_dds(uint32_t width, uint32_t height, uint32_t levels, uint32_t faces, uint32_t format,
+                   const void* const* level_data, uint8_t** out, uint32_t* out_size);
+/* crn_decompress_crn_to_dds (inc/crnlib.h:620, crnlib/crnlib.cpp:269-291):
+ * every level and face transcoded on the GPU, then written as a DDS file. */
+int crn2_crn_to_dds(const void* crn, uint32_t size, uint8_t** out, uint32_t* out_size);
 
 /* ---------------------------------------------- writer and test textures */
 /* CRC16 of crn_decomp.h:2048-2069. */
