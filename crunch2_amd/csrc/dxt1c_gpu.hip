@@ -939,17 +939,54 @@ struct ClusterOut {
     uint32_t flags;    // reordered | alternate_rounding << 2 | enforce << 3 | enforced << 4 | too_many_colours << 8
     uint32_t pad;
 };
+constexpr uint32_t kNotEncoded = 1u << 8;
+
+// Open-addressing colour hash over L.kCapH slots (a power of two in LDS, any
+// size in the global store): first slot and linear probe.
+template <class Lds>
+__device__ inline uint32_t hash_slot(const Lds& L, uint32_t key)
+{
+    return hash32(key) % L.kCapH;
+}
+
+template <class Lds>
+__device__ inline uint32_t hash_next(const Lds& L, uint32_t h)
+{
+    return h + 1 == L.kCapH ? 0 : h + 1;
+}
 
 template <class Lds>
 __device__ inline int hash_find(const Lds& L, uint32_t key)
 {
-    uint32_t h = hash32(key) & (Lds::kCapH - 1);
+    uint32_t h = hash_slot(L, key);
     while (L.hkey[h]) {
-        if (L.hkey[h] == key) return L.hval[h];
-        h = (h + 1) & (Lds::kCapH - 1);
+        if (L.hkey[h] == key) return (int)L.hval[h];
+        h = hash_next(L, h);
     }
     return -1;
 }
+
+// The colour set of a cluster with more distinct colours than the LDS store
+// holds: colours, weights, selectors and the hash in global memory (a scratch
+// region the cluster owns, sized by its texel count), the per-channel tables
+// in LDS.  Members mirror LdsT's so every search step reads either store.
+struct GStore {
+    uint32_t kCapU, kCapH;
+    uint32_t* col;
+    uint32_t* w;
+    uint8_t* sel;
+    uint32_t* hkey;
+    uint32_t* hval;
+    uint32_t* hcnt;
+    uint32_t* mark;
+    unsigned long long (*pl)[64][3];
+    uint64_t *rlo, *rhi, *glo, *ghi, *blo, *bhi;
+};
+
+struct GTables {
+    unsigned long long pl[3][64][3];
+    uint64_t rlo[32], rhi[32], glo[64], ghi[64], blo[32], bhi[32];
+};
 
 // Clusters by texel count into the four LDS sizes (any order within a class:
 // each cluster's result depends only on its own texels).
@@ -964,65 +1001,16 @@ __global__ void __launch_bounds__(256) dxt1_cluster_classes_kernel(const uint32_
     list[(size_t)k * nclusters + atomicAdd(&count[k], 1u)] = c;
 }
 
-template <uint32_t MaxU>
-__global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __restrict__ px,
-                                                          const uint32_t* __restrict__ ofs,
-                                                          const uint32_t* __restrict__ list,
-                                                          const uint32_t* __restrict__ count, int perceptual,
-                                                          ClusterOut* __restrict__ out,
-                                                          uint8_t* __restrict__ selectors)
+// Everything after the unique colours: error planes (crn_dxt1.cpp:2134-2203),
+// the 1-colour shortcut or handle_multicolor_block, selectors and
+// return_solution, for either colour store.
+template <class Lds>
+__device__ void cluster_search(St& s, Lds& L, const uint32_t* __restrict__ px, uint32_t p0, uint32_t n,
+                               int perceptual, uint32_t cl, ClusterOut* __restrict__ out,
+                               uint8_t* __restrict__ selectors)
 {
-    typedef LdsT<MaxU> Lds;
-    constexpr uint32_t kHashCap = Lds::kCapH;
-    extern __shared__ __align__(16) uint8_t lds_raw[];
-    Lds& L = *reinterpret_cast<Lds*>(lds_raw);
-    if (blockIdx.x >= *count) return;
-    const uint32_t cl = list[blockIdx.x];
     const uint32_t lane = lane_id();
-    const uint32_t p0 = ofs[cl], p1 = ofs[cl + 1];
-    const uint32_t n = p1 - p0;
     ClusterOut o{};
-    if (!n) {
-        if (lane == 0) out[cl] = o;
-        return;
-    }
-    for (uint32_t i = lane; i < kHashCap; i += 64) L.hkey[i] = 0;
-    for (uint32_t i = lane; i < 3 * 64 * 3; i += 64) (&L.pl[0][0][0])[i] = 0;
-    __builtin_amdgcn_wave_barrier();
-    // unique colours in first-appearance order (one lane; the order fixes the float sums)
-    St s;
-    s.perceptual = perceptual;
-    s.nu = 0;
-    s.total_w = n;
-    bool overflow = false;
-    if (lane == 0) {
-        for (uint32_t i = 0; i < n; i++) {
-            const uint32_t key = (px[p0 + i] & 0x00FFFFFFu) | 0xFF000000u;
-            uint32_t h = hash32(key) & (kHashCap - 1);
-            while (L.hkey[h] && L.hkey[h] != key) h = (h + 1) & (kHashCap - 1);
-            if (L.hkey[h] == key) {
-                L.w[L.hval[h]]++;
-            } else if (s.nu < Lds::kCapU) {
-                L.hkey[h] = key;
-                L.hval[h] = (uint16_t)s.nu;
-                L.col[s.nu] = key & 0x00FFFFFFu;
-                L.w[s.nu] = 1;
-                s.nu++;
-            } else {
-                overflow = true;
-                break;
-            }
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-    s.nu = (uint32_t)__shfl((int)s.nu, 0);
-    overflow = __shfl((int)overflow, 0) != 0;
-    if (overflow) {
-        o.flags = 1u << 8;
-        if (lane == 0) out[cl] = o;
-        return;
-    }
-    // error planes (crn_dxt1.cpp:2134-2203)
     for (uint32_t i = lane; i < s.nu; i += 64) {
         const uint32_t c = L.col[i];
         const unsigned long long w = L.w[i];
@@ -1092,6 +1080,161 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
                   ((uint32_t)s.best.enforced << 4);
         out[cl] = o;
     }
+}
+
+template <uint32_t MaxU>
+__global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __restrict__ px,
+                                                          const uint32_t* __restrict__ ofs,
+                                                          const uint32_t* __restrict__ list,
+                                                          const uint32_t* __restrict__ count, int perceptual,
+                                                          ClusterOut* __restrict__ out,
+                                                          uint8_t* __restrict__ selectors)
+{
+    typedef LdsT<MaxU> Lds;
+    constexpr uint32_t kHashCap = Lds::kCapH;
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    Lds& L = *reinterpret_cast<Lds*>(lds_raw);
+    if (blockIdx.x >= *count) return;
+    const uint32_t cl = list[blockIdx.x];
+    const uint32_t lane = lane_id();
+    const uint32_t p0 = ofs[cl], p1 = ofs[cl + 1];
+    const uint32_t n = p1 - p0;
+    ClusterOut o{};
+    if (!n) {
+        if (lane == 0) out[cl] = o;
+        return;
+    }
+    for (uint32_t i = lane; i < kHashCap; i += 64) L.hkey[i] = 0;
+    for (uint32_t i = lane; i < 3 * 64 * 3; i += 64) (&L.pl[0][0][0])[i] = 0;
+    __builtin_amdgcn_wave_barrier();
+    // unique colours in first-appearance order (one lane; the order fixes the float sums)
+    St s;
+    s.perceptual = perceptual;
+    s.nu = 0;
+    s.total_w = n;
+    bool overflow = false;
+    if (lane == 0) {
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t key = (px[p0 + i] & 0x00FFFFFFu) | 0xFF000000u;
+            uint32_t h = hash_slot(L, key);
+            while (L.hkey[h] && L.hkey[h] != key) h = hash_next(L, h);
+            if (L.hkey[h] == key) {
+                L.w[L.hval[h]]++;
+            } else if (s.nu < Lds::kCapU) {
+                L.hkey[h] = key;
+                L.hval[h] = (uint16_t)s.nu;
+                L.col[s.nu] = key & 0x00FFFFFFu;
+                L.w[s.nu] = 1;
+                s.nu++;
+            } else {
+                overflow = true;
+                break;
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    s.nu = (uint32_t)__shfl((int)s.nu, 0);
+    overflow = __shfl((int)overflow, 0) != 0;
+    if (overflow) {
+        // more distinct colours than this store holds: left to dxt1_cluster_big_kernel
+        o.flags = kNotEncoded;
+        if (lane == 0) out[cl] = o;
+        return;
+    }
+    cluster_search(s, L, px, p0, n, perceptual, cl, out, selectors);
+}
+
+// Clusters the LDS pass flagged (more than 4,096 distinct colours), one wave
+// each, with the colour set in global scratch: the cluster's texels own
+// scratch entries [p0, p1) of the texel-sized arrays and [2 p0, 2 p1) of the
+// hash.  The first-appearance order is found in parallel: every texel inserts
+// its colour with compare-and-swap and keeps the smallest texel index and
+// the count per slot, the first texel of each colour is marked, and a prefix
+// count over the marks gives each colour its rank.
+__global__ void __launch_bounds__(64) dxt1_cluster_big_kernel(const uint32_t* __restrict__ px,
+                                                              const uint32_t* __restrict__ ofs, uint32_t nclusters,
+                                                              int perceptual, ClusterOut* __restrict__ out,
+                                                              uint8_t* __restrict__ selectors,
+                                                              uint8_t* __restrict__ work)
+{
+    __shared__ GTables T;
+    const uint32_t cl = blockIdx.x;
+    if (cl >= nclusters || !(out[cl].flags & kNotEncoded)) return;
+    const uint32_t lane = lane_id();
+    const uint32_t p0 = ofs[cl], p1 = ofs[cl + 1], n = p1 - p0, total = ofs[nclusters];
+    GStore L;
+    L.kCapU = n;
+    L.kCapH = 2 * n;
+    // scratch layout, 37 bytes per texel of all clusters (`total`): u32
+    // arrays col, w, mark of `total` entries, hkey, hval, hcnt of 2 * total,
+    // then the u8 selectors of `total`
+    uint32_t* base = reinterpret_cast<uint32_t*>(work);
+    L.col = base + p0;
+    L.w = base + (size_t)total + p0;
+    L.mark = base + 2 * (size_t)total + p0;
+    L.hkey = base + 3 * (size_t)total + 2 * (size_t)p0;
+    L.hval = base + 5 * (size_t)total + 2 * (size_t)p0;
+    L.hcnt = base + 7 * (size_t)total + 2 * (size_t)p0;
+    L.sel = work + 36 * (size_t)total + p0;
+    L.pl = T.pl;
+    L.rlo = T.rlo;
+    L.rhi = T.rhi;
+    L.glo = T.glo;
+    L.ghi = T.ghi;
+    L.blo = T.blo;
+    L.bhi = T.bhi;
+    for (uint32_t i = lane; i < L.kCapH; i += 64) {
+        L.hkey[i] = 0;
+        L.hval[i] = 0xFFFFFFFFu;
+        L.hcnt[i] = 0;
+    }
+    for (uint32_t i = lane; i < n; i += 64) L.mark[i] = 0;
+    for (uint32_t i = lane; i < 3 * 64 * 3; i += 64) (&T.pl[0][0][0])[i] = 0;
+    __threadfence();
+    __syncthreads();
+    for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t key = (px[p0 + i] & 0x00FFFFFFu) | 0xFF000000u;
+        uint32_t h = hash_slot(L, key);
+        for (;;) {
+            const uint32_t prev = atomicCAS(&L.hkey[h], 0u, key);
+            if (prev == 0u || prev == key) {
+                atomicMin(&L.hval[h], i);
+                atomicAdd(&L.hcnt[h], 1u);
+                break;
+            }
+            h = hash_next(L, h);
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    for (uint32_t h = lane; h < L.kCapH; h += 64)
+        if (L.hkey[h]) L.mark[L.hval[h]] = 1;
+    __threadfence();
+    __syncthreads();
+    // ranks in texel order: a running count over the marks, 64 texels per step
+    uint32_t base_rank = 0;
+    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        const bool m = i < n && L.mark[i];
+        const uint64_t bal = __ballot(m);
+        const uint32_t r = base_rank + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        if (m) {
+            const uint32_t key = (px[p0 + i] & 0x00FFFFFFu) | 0xFF000000u;
+            uint32_t h = hash_slot(L, key);
+            while (L.hkey[h] != key) h = hash_next(L, h);
+            L.col[r] = key & 0x00FFFFFFu;
+            L.w[r] = L.hcnt[h];
+            L.hval[h] = r;
+        }
+        base_rank += (uint32_t)__popcll(bal);
+    }
+    __threadfence();
+    __syncthreads();
+    St s;
+    s.perceptual = perceptual;
+    s.nu = base_rank;
+    s.total_w = n;
+    cluster_search(s, L, px, p0, n, perceptual, cl, out, selectors);
 }
 
 }  // namespace
@@ -1249,6 +1392,28 @@ int clusters_run(const void* rgba, const uint32_t* offsets, uint32_t nclusters, 
                 rc = CRN2_EHIP;
     return rc;
 }
+
+// Second pass for clusters of more than 4,096 distinct colours (the LDS pass
+// marks them kNotEncoded): reads the texel count once (the call waits for the
+// stream there), and when some cluster may hold more colours than the largest
+// LDS store, runs dxt1_cluster_big_kernel with stream-ordered global scratch.
+int clusters_big_pass(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags, void* out,
+                      uint8_t* selectors, hipStream_t st)
+{
+    uint32_t total = 0;
+    if (hipMemcpyAsync(&total, offsets + nclusters, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return CRN2_EHIP;
+    if (total <= kMaxU) return CRN2_OK;      // no cluster can hold more colours than the LDS store
+    uint8_t* big = nullptr;
+    if (hipMallocAsync((void**)&big, 37 * (size_t)total, st) != hipSuccess) return CRN2_ENOMEM;
+    hipLaunchKernelGGL(dxt1_cluster_big_kernel, dim3(nclusters), dim3(64), 0, st,
+                       static_cast<const uint32_t*>(rgba), offsets, nclusters, flags & 1,
+                       static_cast<ClusterOut*>(out), selectors, big);
+    const int rc = hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
+    (void)hipFreeAsync(big, st);
+    return rc;
+}
 }  // namespace
 
 extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags,
@@ -1265,6 +1430,7 @@ extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offse
         return CRN2_ENOMEM;
     rc = clusters_run(rgba, offsets, nclusters, flags, out, selectors, st, work, true);
     (void)hipFreeAsync(work, st);
+    if (rc == CRN2_OK) rc = clusters_big_pass(rgba, offsets, nclusters, flags, out, selectors, st);
     return rc;
 }
 

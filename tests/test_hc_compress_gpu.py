@@ -39,3 +39,36 @@ def test_compress_matches_oracle(gpu, fmt, size, nlevels, sizes):
         ce = out[2].cpu().numpy().view(np.uint32)
         cs = out[3].cpu().numpy().view(np.uint32)
         assert psnr(decode_color(ep, sel, ce, cs), blocks[:, :, :3]) > 28.0
+
+
+def test_compress_with_clusters_beyond_the_lds_store(gpu):
+    # eight colour endpoints for a 256x256 texture: every cluster holds
+    # thousands of texels and more than 4,096 distinct colours
+    from crunch2_amd import hc
+    blocks, levels = texture_blocks(256, 77, texture.RGB, 1)
+    sizes = (8, 256, 8, 8)
+    out = hc.compress(torch.from_numpy(blocks).cuda(), levels, codebook_sizes=sizes)
+    torch.cuda.synchronize()
+    ref = oracle.hc_compress(blocks, levels, codebook_sizes=sizes)
+    assert np.array_equal(out[0].cpu().numpy().view(np.uint16), ref[0])
+    assert np.array_equal(out[1].cpu().numpy().view(np.uint16), ref[1])
+    assert np.array_equal(out[2].cpu().numpy().view(np.uint32), ref[2])
+    assert np.array_equal(out[3].cpu().numpy().view(np.uint32), ref[3])
+
+
+def test_compress_at_the_clustered_workload_size(gpu):
+    # the clustered-DXT1 workload: 2048x2048 RGB, 4,096 colour endpoints,
+    # 8,192 colour selectors, one level, against the oracle's composition
+    from crunch2_amd import hc
+    blocks, levels = texture_blocks(2048, 3, texture.RGB, 1)
+    sizes = (4096, 8192, 8, 8)
+    print("hc 2048^2", flush=True)
+    out = hc.compress(torch.from_numpy(blocks).cuda(), levels, codebook_sizes=sizes)
+    torch.cuda.synchronize()
+    ref = oracle.hc_compress(blocks, levels, codebook_sizes=sizes)
+    ep, sel = out[0].cpu().numpy().view(np.uint16), out[1].cpu().numpy().view(np.uint16)
+    print("endpoints", len(ref[2]), "selectors", len(ref[3]), flush=True)
+    assert np.array_equal(ep, ref[0])
+    assert np.array_equal(sel, ref[1])
+    assert np.array_equal(out[2].cpu().numpy().view(np.uint32), ref[2])
+    assert np.array_equal(out[3].cpu().numpy().view(np.uint32), ref[3])
