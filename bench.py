@@ -180,6 +180,9 @@ def main():
                     help="instead of the 4096^2 pair, a mixed set of this many textures (sides log-uniform in "
                          "[64, 2048], full mips; 60%% DXT1 RGB, 25%% DXT5 RGBA, 15%% DXN normal maps)")
     ap.add_argument("--streams", type=int, default=4, help="plans (groups of copies) per step, one stream each")
+    ap.add_argument("--writer", choices=("hc", "simple"), default="hc",
+                    help="how the input .crn files are made: the GPU dxt_hc with the project's CRN writer, or the "
+                         "simple CPU block quantizer")
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU baseline sample length")
     ap.add_argument("--encode-size", type=int, default=8192,
@@ -211,18 +214,16 @@ def main():
     files = []
     if args.mixed:
         from crunch2_amd._lib import FMT_DXN_XY
-        rng = np.random.default_rng(2024)
-        for i in range(args.mixed):
-            side = int(round(2 ** rng.uniform(6, 11)))
-            u = rng.random()
-            fmt = FMT_DXT1 if u < 0.60 else (FMT_DXT5 if u < 0.85 else FMT_DXN_XY)
-            files.append(texture.synth_crn((side, side), 5000 + i, fmt))
+        files = [d for _, _, d in texture.mixed_set(args.mixed, writer=args.writer)]
         log("mixed set of %d textures built in %.1fs (%d bytes)" % (len(files), time.time() - t0,
                                                                     sum(len(f) for f in files)))
     else:
         for b in range(args.batch):
             for fmt in (FMT_DXT1, FMT_DXT5):
-                files.append(texture.synth_crn(args.size, 1 + b, fmt))
+                pal = texture.Q128_DXT1 if fmt == FMT_DXT1 else texture.Q128_DXT5
+                if args.writer == "hc":
+                    pal = tuple(v if v else 8 for v in pal)
+                files.append(texture.synth_crn(args.size, 1 + b, fmt, palette_sizes=pal, writer=args.writer))
         log("inputs built in %.1fs: %s bytes" % (time.time() - t0, [len(f) for f in files]))
 
     ctxs, outs = [], []
@@ -376,8 +377,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
-        "data": "synthetic (seeded 4096^2 RGBA8 texture, full box-filter mip chain, packed to .crn by the "
-                "project's writer at crnlib's quality-128 palette sizes)",
+        "data": ("synthetic (seeded %s, full box-filter mip chains, %s)"
+                 % ("textures of the mixed set" if args.mixed else "4096^2 RGB and RGBA8 textures",
+                    "compressed by the GPU dxt_hc at crnlib's quality-128 palette sizes and packed by the "
+                    "project's writer with crnlib's palette ordering" if args.writer == "hc" else
+                    "packed to .crn by the project's simple block quantizer at crnlib's quality-128 palette sizes")),
         "config": {"workload": ("CRN->DXTn transcode, mixed set of %d textures (64^2..2048^2, full mips, "
                                 "DXT1/DXT5/DXN_XY)" % args.mixed) if args.mixed else
                                ("CRN->DXTn transcode, %dx%d, 13 levels, DXT1 + DXT5, %d texture pair(s) x %d "
@@ -388,7 +392,8 @@ def main():
                    "dxtn_bytes": args.copies * out_bytes, "texels": args.copies * texels},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6),
-                     "traffic": None if args.mixed or args.size != 4096 or args.batch != 1 else TRAFFIC_PER_COPY * args.copies,
+                     "traffic": None if (args.mixed or args.size != 4096 or args.batch != 1 or args.writer != "simple")
+                     else TRAFFIC_PER_COPY * args.copies,
                      "kernel": "transcode pipeline (tables, walk, symbols, indices, write; %d plans of %s "
                                "copies on %d streams)" % (groups, "/".join(str(x) for x in per), groups),
                      "stage_ms_one_plan": {k: round(v, 4) for k, v in prof.items()}},

@@ -41,7 +41,7 @@ class PackInput(C.Structure):
         ("alpha_selectors_linear", C.c_void_p), ("n_alpha_selectors", C.c_uint32),
         ("color_endpoint_idx", C.c_void_p), ("color_selector_idx", C.c_void_p),
         ("alpha0_endpoint_idx", C.c_void_p), ("alpha0_selector_idx", C.c_void_p),
-        ("alpha1_endpoint_idx", C.c_void_p), ("alpha1_selector_idx", C.c_void_p)]
+        ("alpha1_endpoint_idx", C.c_void_p), ("alpha1_selector_idx", C.c_void_p), ("order_palettes", C.c_uint32)]
 
 
 # name -> (restype, argtypes); every function declared in include/crn2.h

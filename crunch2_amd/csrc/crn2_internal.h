@@ -32,6 +32,10 @@ typedef struct crn2_pack_input {
     const uint32_t* alpha0_selector_idx;
     const uint32_t* alpha1_endpoint_idx;
     const uint32_t* alpha1_selector_idx;
+    // non-zero: reorder every palette as crnlib's compressor does before
+    // packing (crn_comp::optimize_color / optimize_alpha and the selector
+    // orders); zero: keep the given order
+    uint32_t order_palettes;
 } crn2_pack_input;
 
 int crn2_pack(const crn2_pack_input* in, uint8_t** out, uint32_t* out_size);

@@ -24,6 +24,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <unordered_map>
 #include <vector>
@@ -306,6 +307,203 @@ static std::vector<uint8_t> pack_alpha_selectors(const std::vector<uint64_t>& li
     return bw.buf;
 }
 
+// ----------------------------------------------------------- palette order
+// The entry order crnlib's compressor gives each palette before packing
+// (crnlib/crn_comp.cpp:767-1352): the new index of every entry.
+//   * selectors: a greedy chain from the all-zero word, each step taking the
+//     remaining word nearest the last one taken (first on ties, remaining
+//     words kept by swap-removal), nearness summed over texel pairs from a
+//     table on the XOR of the two codes (colour: d = 0, 5, 14, 10 per 2-bit
+//     code, optimize_color_selectors :935-986; alpha: d = 0, 2, 3, 3, 5, 5,
+//     4, 4 per 3-bit code, optimize_alpha_selectors :1228-1273);
+//   * endpoints: four candidate orders -- the same greedy chain on squared
+//     endpoint distance from a zero entry (sort_color_endpoints :767-797,
+//     sort_alpha_endpoints :1068-1097), and three chains grown at both ends
+//     from the entry with the most endpoint changes, each step taking the
+//     entry whose similarity to an end times its transition count with the
+//     chain is largest (remap_color_endpoints :799-870 with similarity scale
+//     1 + weight, weight 0, 1/6, 1/2; remap_alpha_endpoints :1099-1160) --
+//     and the one whose packed palette plus coded index deltas is smallest
+//     is kept (optimize_color :988-1047, optimize_alpha :1275-1352).
+// Sums that the reference accumulates in 32-bit unsigned arithmetic wrap the
+// same way here.
+struct ColorEp { int lo[3], hi[3]; };
+
+static uint32_t ep_dist(const ColorEp& a, const ColorEp& b)
+{
+    uint32_t e = 0;
+    for (int k = 0; k < 3; k++) {
+        const int dl = a.lo[k] - b.lo[k], dh = a.hi[k] - b.hi[k];
+        e += (uint32_t)(dl * dl) + (uint32_t)(dh * dh);
+    }
+    return e;
+}
+
+static std::vector<uint16_t> chain_order(uint32_t n, const std::function<uint32_t(uint32_t, uint32_t)>& dist,
+                                         uint32_t origin)
+{
+    // dist(i, j): entry i against entry j; `origin` = n stands for the zero entry
+    std::vector<uint16_t> remap(n), idx(n);
+    for (uint32_t i = 0; i < n; i++) idx[i] = (uint16_t)i;
+    uint32_t sel = origin;
+    for (uint32_t left = n; left;) {
+        uint32_t best = 0, min_err = 0xFFFFFFFFu;
+        for (uint32_t i = 0; i < left; i++) {
+            const uint32_t e = dist(idx[i], sel);
+            if (e < min_err) {
+                min_err = e;
+                best = i;
+            }
+        }
+        sel = idx[best];
+        remap[idx[best]] = (uint16_t)(n - left);
+        left--;
+        idx[best] = idx[left];
+    }
+    return remap;
+}
+
+static std::vector<uint16_t> remap_color_chain(const std::vector<ColorEp>& e, const std::vector<uint32_t>& hist,
+                                               uint32_t n, uint32_t selected, float weight)
+{
+    struct Node { uint32_t index, frequency, front_sim, back_sim; };
+    std::vector<Node> rem(n);
+    for (uint32_t i = 0; i < n; i++) rem[i] = {i, 0, 0, 0};
+    std::vector<uint32_t> chosen(2 * (size_t)n + 1);
+    uint32_t count = n, cf = n, cb = n;
+    chosen[cf] = selected;
+    ColorEp fe = e[selected], be = fe;
+    bool fu = true, bu = true;
+    rem[selected] = rem[--count];
+    const uint32_t* freq = &hist[(size_t)selected * n];
+    const uint32_t base = (uint32_t)(4000 * (1.0f + weight));
+    uint32_t norm = 0;
+    while (count) {
+        uint64_t best_value = 0;
+        uint32_t best = 0;
+        for (uint32_t i = 0; i < count; i++) {
+            Node& nd = rem[i];
+            nd.frequency += freq[nd.index];
+            if (fu) nd.front_sim = base - std::min<uint32_t>(4000, ep_dist(e[nd.index], fe));
+            if (bu) nd.back_sim = base - std::min<uint32_t>(4000, ep_dist(e[nd.index], be));
+            const uint64_t v = (uint64_t)(uint32_t)(std::max(nd.front_sim, nd.back_sim) * (nd.frequency + norm) + 1u);
+            if (v > best_value || (v == best_value && nd.index < selected)) {
+                best_value = v;
+                best = i;
+                selected = nd.index;
+            }
+        }
+        freq = &hist[(size_t)selected * n];
+        uint32_t ff = 0, fb = 0;
+        for (int front = (int)cf, back = (int)cb, scale = back - front; scale > 0; front++, back--, scale -= 2) {
+            ff += (uint32_t)scale * freq[chosen[front]];
+            fb += (uint32_t)scale * freq[chosen[back]];
+        }
+        fu = bu = false;
+        const Node nd = rem[best];
+        norm = nd.frequency << 3;
+        if ((uint64_t)nd.front_sim * ff > (uint64_t)nd.back_sim * fb) {
+            chosen[--cf] = selected;
+            fe = e[selected];
+            fu = true;
+        } else {
+            chosen[++cb] = selected;
+            be = e[selected];
+            bu = true;
+        }
+        rem[best] = rem[--count];
+    }
+    std::vector<uint16_t> remap(n);
+    for (uint32_t i = cf; i <= cb; i++) remap[chosen[i]] = (uint16_t)(i - cf);
+    return remap;
+}
+
+static std::vector<uint16_t> remap_alpha_chain(const std::vector<uint32_t>& ep, const std::vector<uint32_t>& hist,
+                                               uint32_t n, uint32_t selected, float weight)
+{
+    auto dist = [&](uint32_t a, uint32_t b) {
+        const int dl = (int)(ep[a] & 255) - (int)(ep[b] & 255), dh = (int)((ep[a] >> 8) & 255) - (int)((ep[b] >> 8) & 255);
+        return (uint32_t)(dl * dl + dh * dh);
+    };
+    const uint32_t* freq = &hist[(size_t)selected * n];
+    std::vector<uint32_t> chosen_front, chosen_back, remaining, total(n, 0);   // chain = reverse(front) + back
+    chosen_back.push_back(selected);
+    for (uint32_t i = 0; i < n; i++)
+        if (i != selected) {
+            remaining.push_back(i);
+            total[i] = freq[i];
+        }
+    auto chain_at = [&](size_t k) -> uint32_t {
+        return k < chosen_front.size() ? chosen_front[chosen_front.size() - 1 - k]
+                                       : chosen_back[k - chosen_front.size()];
+    };
+    const uint32_t base = (uint32_t)(1000 * (1.0f + weight));
+    uint32_t norm = 0;
+    while (!remaining.empty()) {
+        const uint32_t ef = chain_at(0), eb = chain_at(chosen_front.size() + chosen_back.size() - 1);
+        uint32_t sel_i = 0;
+        uint64_t best = 0, sf = 0, sb = 0;
+        for (uint32_t i = 0; i < remaining.size(); i++) {
+            const uint32_t r = remaining[i];
+            const uint64_t simf = base - std::min<uint32_t>(dist(r, ef), 1000);
+            const uint64_t simb = base - std::min<uint32_t>(dist(r, eb), 1000);
+            const uint64_t v = std::max(simf, simb) * (uint64_t)(uint32_t)(total[r] + norm) + 1;
+            if (v > best) {
+                best = v;
+                sel_i = i;
+                sf = simf;
+                sb = simb;
+            }
+        }
+        selected = remaining[sel_i];
+        freq = &hist[(size_t)selected * n];
+        norm = total[selected];
+        uint32_t ff = 0, fb = 0;
+        const int len = (int)(chosen_front.size() + chosen_back.size());
+        for (int front = 0, back = len - 1, scale = back; scale > 0; front++, back--, scale -= 2) {
+            ff += (uint32_t)scale * freq[chain_at((size_t)front)];
+            fb += (uint32_t)scale * freq[chain_at((size_t)back)];
+        }
+        if (sf * ff > sb * fb) chosen_front.push_back(selected);
+        else chosen_back.push_back(selected);
+        remaining.erase(remaining.begin() + sel_i);
+        for (uint32_t r : remaining) total[r] += freq[r];
+    }
+    std::vector<uint16_t> remap(n);
+    const size_t len = chosen_front.size() + chosen_back.size();
+    for (size_t k = 0; k < len; k++) remap[chain_at(k)] = (uint16_t)k;
+    return remap;
+}
+
+static std::vector<uint16_t> order_color_selectors(const std::vector<uint32_t>& sel)
+{
+    static const uint8_t d[4] = {0, 5, 14, 10};
+    std::vector<uint8_t> D4(256), D8(65536);
+    for (uint32_t i = 0; i < 256; i++) D4[i] = (uint8_t)(d[(i ^ (i >> 4)) & 3] + d[((i >> 2) ^ (i >> 6)) & 3]);
+    for (uint32_t i = 0; i < 65536; i++)
+        D8[i] = (uint8_t)(D4[((i >> 8) & 0xF0) | ((i >> 4) & 0xF)] + D4[((i >> 4) & 0xF0) | (i & 0xF)]);
+    const uint32_t n = (uint32_t)sel.size();
+    return chain_order(n, [&](uint32_t i, uint32_t j) {
+        const uint32_t s = sel[i], t = j == n ? 0u : sel[j];
+        return (uint32_t)D8[((s >> 16) & 0xFF00) | ((t >> 24) & 0xFF)] + D8[((s >> 8) & 0xFF00) | ((t >> 16) & 0xFF)] +
+               D8[(s & 0xFF00) | ((t >> 8) & 0xFF)] + D8[((s << 8) & 0xFF00) | (t & 0xFF)];
+    }, n);
+}
+
+static std::vector<uint16_t> order_alpha_selectors(const std::vector<uint64_t>& sel)
+{
+    static const uint8_t d[8] = {0, 2, 3, 3, 5, 5, 4, 4};
+    std::vector<uint8_t> D6(4096);
+    for (uint32_t i = 0; i < 4096; i++) D6[i] = (uint8_t)(d[(i ^ (i >> 6)) & 7] + d[((i >> 3) ^ (i >> 9)) & 7]);
+    const uint32_t n = (uint32_t)sel.size();
+    return chain_order(n, [&](uint32_t i, uint32_t j) {
+        uint64_t s = sel[i] << 6, t = j == n ? 0ull : sel[j];
+        uint32_t e = 0;
+        for (int k = 0; k < 8; k++, s >>= 6, t >>= 6) e += D6[(s & 0xFC0) | (t & 0x3F)];
+        return e;
+    }, n);
+}
+
 static inline uint16_t crc16_update(uint16_t crc, const uint8_t* p, size_t len)
 {
     uint32_t c = (uint16_t)~crc;
@@ -408,6 +606,154 @@ extern "C" int crn2_pack(const crn2_pack_input* in, uint8_t** out, uint32_t* out
             }
     }
 
+    // working copies of the palettes and per-block indices (reordered below when asked)
+    std::vector<uint32_t> ce, cs, ae, ice, ics, ia[4];
+    std::vector<uint64_t> as;
+    if (has_color) {
+        ce.assign(in->color_endpoints, in->color_endpoints + nce);
+        cs.assign(in->color_selectors_linear, in->color_selectors_linear + ncs);
+        ice.assign(in->color_endpoint_idx, in->color_endpoint_idx + nblocks);
+        ics.assign(in->color_selector_idx, in->color_selector_idx + nblocks);
+    }
+    if (n_alpha) {
+        ae.assign(in->alpha_endpoints, in->alpha_endpoints + nae);
+        as.assign(in->alpha_selectors_linear, in->alpha_selectors_linear + nas);
+        ia[0].assign(in->alpha0_endpoint_idx, in->alpha0_endpoint_idx + nblocks);
+        ia[1].assign(in->alpha0_selector_idx, in->alpha0_selector_idx + nblocks);
+        if (n_alpha == 2) {
+            ia[2].assign(in->alpha1_endpoint_idx, in->alpha1_endpoint_idx + nblocks);
+            ia[3].assign(in->alpha1_selector_idx, in->alpha1_selector_idx + nblocks);
+        }
+    }
+    if (in->order_palettes) {
+        // bits of an endpoint order: its packed palette plus the level streams' index deltas under it
+        auto delta_bits = [&](const std::vector<uint16_t>& remap, uint32_t n,
+                              std::initializer_list<const std::vector<uint32_t>*> chans, size_t palette_bytes) {
+            std::vector<uint32_t> hist(n, 0);
+            for (uint32_t l = 0; l < in->levels; l++)
+                for (const std::vector<uint32_t>* ch : chans) {
+                    uint32_t prev = 0;
+                    for (uint32_t b = first[l]; b < first[l + 1]; b++) {
+                        const uint32_t idx = remap[(*ch)[b]];
+                        if (!refs[b]) hist[(idx + n - prev) % n]++;
+                        prev = idx;
+                    }
+                }
+            PrefixCode pc = build_code(hist, 16);
+            uint64_t bits = (uint64_t)palette_bytes * 8;
+            for (uint32_t sy = 0; sy < n; sy++) bits += (uint64_t)hist[sy] * pc.len[sy];
+            BitWriter bw;
+            transmit_model(bw, pc);
+            return bits + bw.buf.size() * 8 + (uint64_t)bw.n;
+        };
+        // transition counts between consecutive blocks' endpoint indices (blocks with deltas only)
+        auto transitions = [&](uint32_t n, std::initializer_list<const std::vector<uint32_t>*> chans,
+                               std::vector<uint32_t>& hist) {
+            hist.assign((size_t)n * n, 0);
+            std::vector<uint32_t> sum(n, 0);
+            for (const std::vector<uint32_t>* ch : chans) {
+                uint32_t prev = 0;
+                for (uint32_t b = 0; b < nblocks; b++) {
+                    const uint32_t i = (*ch)[b];
+                    if (!refs[b] && i != prev) {
+                        hist[(size_t)i * n + prev]++;
+                        hist[(size_t)prev * n + i]++;
+                        sum[i]++;
+                        sum[prev]++;
+                    }
+                    prev = i;
+                }
+            }
+            uint32_t selected = 0, best = 0;
+            for (uint32_t i = 0; i < n; i++)
+                if (best < sum[i]) {
+                    best = sum[i];
+                    selected = i;
+                }
+            return selected;
+        };
+        auto apply = [&](const std::vector<uint16_t>& remap, std::vector<uint32_t>* idx) {
+            for (uint32_t& v : *idx) v = remap[v];
+        };
+        const float weights[3] = {0.0f, 1.0f / 6.0f, 0.5f};
+        if (has_color) {
+            std::vector<ColorEp> un(nce);
+            for (uint32_t i = 0; i < nce; i++)
+                for (int h = 0; h < 2; h++) {
+                    const uint32_t c = (ce[i] >> (16 * h)) & 0xFFFF;
+                    int* o = h ? un[i].hi : un[i].lo;
+                    const int r = (c >> 11) & 31, g = (c >> 5) & 63, b = c & 31;
+                    o[0] = (r << 3) | (r >> 2);
+                    o[1] = (g << 2) | (g >> 4);
+                    o[2] = (b << 3) | (b >> 2);
+                }
+            std::vector<uint32_t> hist;
+            const uint32_t sel = transitions(nce, {&ice}, hist);
+            const ColorEp zero = {{0, 0, 0}, {0, 0, 0}};
+            std::vector<std::vector<uint16_t>> trial;
+            trial.push_back(chain_order(nce, [&](uint32_t i, uint32_t j) { return ep_dist(un[i], j == nce ? zero : un[j]); },
+                                        nce));
+            for (float w : weights) trial.push_back(remap_color_chain(un, hist, nce, sel, w));
+            uint64_t best_bits = ~0ull;
+            std::vector<uint16_t> best;
+            for (auto& r : trial) {
+                std::vector<uint32_t> pal(nce);
+                for (uint32_t i = 0; i < nce; i++) pal[r[i]] = ce[i];
+                const uint64_t bits = delta_bits(r, nce, {&ice}, pack_color_endpoints(pal).size());
+                if (bits < best_bits) {
+                    best_bits = bits;
+                    best = r;
+                }
+            }
+            std::vector<uint32_t> pal(nce);
+            for (uint32_t i = 0; i < nce; i++) pal[best[i]] = ce[i];
+            ce.swap(pal);
+            apply(best, &ice);
+            const std::vector<uint16_t> rs = order_color_selectors(cs);
+            std::vector<uint32_t> spal(ncs);
+            for (uint32_t i = 0; i < ncs; i++) spal[rs[i]] = cs[i];
+            cs.swap(spal);
+            apply(rs, &ics);
+        }
+        if (n_alpha) {
+            std::vector<uint32_t> hist;
+            uint32_t sel;
+            if (n_alpha == 2) sel = transitions(nae, {&ia[0], &ia[2]}, hist);
+            else sel = transitions(nae, {&ia[0]}, hist);
+            std::vector<std::vector<uint16_t>> trial;
+            trial.push_back(chain_order(nae, [&](uint32_t i, uint32_t j) {
+                const int l0 = (int)(ae[i] & 255), h0 = (int)((ae[i] >> 8) & 255);
+                const int l1 = j == nae ? 0 : (int)(ae[j] & 255), h1 = j == nae ? 0 : (int)((ae[j] >> 8) & 255);
+                return (uint32_t)((l0 - l1) * (l0 - l1) + (h0 - h1) * (h0 - h1));
+            }, nae));
+            for (float w : weights) trial.push_back(remap_alpha_chain(ae, hist, nae, sel, w));
+            uint64_t best_bits = ~0ull;
+            std::vector<uint16_t> best;
+            for (auto& r : trial) {
+                std::vector<uint32_t> pal(nae);
+                for (uint32_t i = 0; i < nae; i++) pal[r[i]] = ae[i];
+                const size_t pb = pack_alpha_endpoints(pal).size();
+                const uint64_t bits = n_alpha == 2 ? delta_bits(r, nae, {&ia[0], &ia[2]}, pb)
+                                                   : delta_bits(r, nae, {&ia[0]}, pb);
+                if (bits < best_bits) {
+                    best_bits = bits;
+                    best = r;
+                }
+            }
+            std::vector<uint32_t> pal(nae);
+            for (uint32_t i = 0; i < nae; i++) pal[best[i]] = ae[i];
+            ae.swap(pal);
+            apply(best, &ia[0]);
+            if (n_alpha == 2) apply(best, &ia[2]);
+            const std::vector<uint16_t> rs = order_alpha_selectors(as);
+            std::vector<uint64_t> spal(nas);
+            for (uint32_t i = 0; i < nas; i++) spal[rs[i]] = as[i];
+            as.swap(spal);
+            apply(rs, &ia[1]);
+            if (n_alpha == 2) apply(rs, &ia[3]);
+        }
+    }
+
     struct Level { std::vector<uint8_t> bytes; };
     std::vector<Level> lv(in->levels);
     PrefixCode cref, cce, ccs, cae, cas;
@@ -423,9 +769,7 @@ extern "C" int crn2_pack(const crn2_pack_input* in, uint8_t** out, uint32_t* out
                         uint32_t g = refs[b] | (refs[b + W] << 2) | (refs[b + 1] << 4) | (refs[b + W + 1] << 6);
                         if (pass) put_sym(bw, cref, g); else fref[g]++;
                     }
-                    uint32_t cur[3] = {has_color ? in->color_endpoint_idx[b] : 0,
-                                       n_alpha ? in->alpha0_endpoint_idx[b] : 0,
-                                       n_alpha == 2 ? in->alpha1_endpoint_idx[b] : 0};
+                    uint32_t cur[3] = {has_color ? ice[b] : 0, n_alpha ? ia[0][b] : 0, n_alpha == 2 ? ia[2][b] : 0};
                     if (refs[b] == 0) {
                         if (has_color) {
                             uint32_t d = (cur[0] + nce - run[0]) % nce;
@@ -440,15 +784,15 @@ extern "C" int crn2_pack(const crn2_pack_input* in, uint8_t** out, uint32_t* out
                     run[1] = cur[1];
                     run[2] = cur[2];
                     if (has_color) {
-                        uint32_t s = in->color_selector_idx[b];
+                        uint32_t s = ics[b];
                         if (pass) put_sym(bw, ccs, s); else fcs[s]++;
                     }
                     if (n_alpha >= 1) {
-                        uint32_t s = in->alpha0_selector_idx[b];
+                        uint32_t s = ia[1][b];
                         if (pass) put_sym(bw, cas, s); else fas[s]++;
                     }
                     if (n_alpha == 2) {
-                        uint32_t s = in->alpha1_selector_idx[b];
+                        uint32_t s = ia[3][b];
                         if (pass) put_sym(bw, cas, s); else fas[s]++;
                     }
                 }
@@ -473,12 +817,12 @@ extern "C" int crn2_pack(const crn2_pack_input* in, uint8_t** out, uint32_t* out
 
     std::vector<uint8_t> p_ce, p_cs, p_ae, p_as;
     if (has_color) {
-        p_ce = pack_color_endpoints(std::vector<uint32_t>(in->color_endpoints, in->color_endpoints + nce));
-        p_cs = pack_color_selectors(std::vector<uint32_t>(in->color_selectors_linear, in->color_selectors_linear + ncs));
+        p_ce = pack_color_endpoints(ce);
+        p_cs = pack_color_selectors(cs);
     }
     if (n_alpha) {
-        p_ae = pack_alpha_endpoints(std::vector<uint32_t>(in->alpha_endpoints, in->alpha_endpoints + nae));
-        p_as = pack_alpha_selectors(std::vector<uint64_t>(in->alpha_selectors_linear, in->alpha_selectors_linear + nas));
+        p_ae = pack_alpha_endpoints(ae);
+        p_as = pack_alpha_selectors(as);
     }
 
     const uint32_t hsize = 74 + 4 * (in->levels - 1);

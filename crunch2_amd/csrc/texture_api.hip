@@ -315,6 +315,7 @@ extern "C" int crn2_compress_crn(const uint8_t* const* levels, uint32_t width, u
     in.format = format;
     in.userdata0 = userdata0;
     in.userdata1 = userdata1;
+    in.order_palettes = 1;
     if (has_color) {
         in.color_endpoints = ce.data();
         in.n_color_endpoints = counts[0];

@@ -64,10 +64,13 @@ def compress(faces_levels, fmt, palette_sizes):
 
 
 def pack(width, height, levels, faces, fmt, color_endpoints=None, color_selectors=None, alpha_endpoints=None,
-         alpha_selectors=None, ce_idx=None, cs_idx=None, a0e_idx=None, a0s_idx=None, a1e_idx=None, a1s_idx=None):
+         alpha_selectors=None, ce_idx=None, cs_idx=None, a0e_idx=None, a0s_idx=None, a1e_idx=None, a1s_idx=None,
+         order_palettes=False):
     """Writes a .crn from explicit palettes and per-block indices (blocks of the
     even-padded grid of every level, faces stacked, rows in raster order).
-    Colour selectors are linear 2-bit words, alpha selectors linear 3-bit words."""
+    Colour selectors are linear 2-bit words, alpha selectors linear 3-bit words.
+    With order_palettes the palettes are reordered as crnlib's compressor
+    orders them before packing (the decoded texture is the same)."""
     keep = []
 
     def ptr(a, dt):
@@ -89,6 +92,7 @@ def pack(width, height, levels, faces, fmt, color_endpoints=None, color_selector
     p.alpha0_selector_idx, _ = ptr(a0s_idx, np.uint32)
     p.alpha1_endpoint_idx, _ = ptr(a1e_idx, np.uint32)
     p.alpha1_selector_idx, _ = ptr(a1s_idx, np.uint32)
+    p.order_palettes = int(bool(order_palettes))
     out = C.c_void_p()
     size = C.c_uint32()
     rc = lib().crn2_pack(C.byref(p), C.byref(out), C.byref(size))
@@ -109,18 +113,62 @@ def padded_grid(width, height, levels):
     return out
 
 
-def synth_crn(size, seed, fmt, palette_sizes=None, faces=1, kind=None):
-    """Seeded texture (full mip chain) compressed to .crn in `fmt`."""
+def compress_hc(faces_levels, fmt, palette_sizes, perceptual=True, tile_params=(2.0, 2.0, 3.0)):
+    """faces_levels[f][l] = RGBA8 array of level l of face f; returns .crn
+    bytes from the GPU dxt_hc (crn2_compress_crn: the clustered path of
+    crn_compress) and the project's CRN writer.  Needs a GPU."""
+    faces = len(faces_levels)
+    nl = len(faces_levels[0])
+    h, w = faces_levels[0][0].shape[:2]
+    arr = (C.c_void_p * (faces * nl))()
+    keep = []
+    for f in range(faces):
+        for l in range(nl):
+            a = np.ascontiguousarray(faces_levels[f][l], dtype=np.uint8)
+            keep.append(a)
+            arr[f * nl + l] = a.ctypes.data
+    cb = (C.c_uint32 * 4)(*[max(8, int(v)) for v in palette_sizes])
+    tp = (C.c_float * 3)(*tile_params)
+    out = C.c_void_p()
+    size = C.c_uint32()
+    check(lib().crn2_compress_crn(arr, w, h, nl, faces, fmt, int(bool(perceptual)), cb, tp, 0, 0, C.byref(out),
+                                  C.byref(size)), "crn2_compress_crn")
+    data = C.string_at(out, size.value)
+    lib().crn2_free(out)
+    return data
+
+
+def q128_palette_sizes(width, height, fmt):
+    """The palette sizes crnlib's compressor derives for quality 128
+    (crn_comp::quantize_images, crnlib/crn_comp.cpp:541-577)."""
+    entries = min(max(((width + 3) // 4) * ((height + 3) // 4), 8), 8192)
+    q = np.float32(128.0 / 255.0)
+    cmul = np.float32(0.75) if fmt == FMT_DXT5 else np.float32(1.0)
+
+    def size(lo, e):
+        qq = np.float32(q ** np.float32(e))
+        v = np.float32(0.5) + (np.float32(lo) + (np.float32(entries) - np.float32(lo)) * qq)
+        return int(min(max(int(v), 8), 8192))
+    return (size(64, np.float32(1.8) * cmul), size(96, np.float32(1.65) * cmul), size(24, np.float32(2.1)),
+            size(48, np.float32(1.65)))
+
+
+def synth_crn(size, seed, fmt, palette_sizes=None, faces=1, kind=None, writer="simple"):
+    """Seeded texture (full mip chain) compressed to .crn in `fmt`: by the
+    simple block quantizer (writer "simple", CPU) or by the GPU dxt_hc
+    (writer "hc", palette sizes of crnlib's quality 128 unless given)."""
     if kind is None:
         kind = {FMT_DXT1: RGB, FMT_DXT5: RGBA, FMT_DXT5A: RGBA, FMT_DXN_XY: NORMAL, FMT_DXN_YX: NORMAL}[fmt]
-    if palette_sizes is None:
-        palette_sizes = {FMT_DXT1: Q128_DXT1, FMT_DXT5: Q128_DXT5}.get(fmt, Q128_ALPHA)
     w, h = size if isinstance(size, tuple) else (size, size)
     fl = [mip_chain(synth(w, h, seed + 7919 * f, kind)) for f in range(faces)]
+    if writer == "hc":
+        return compress_hc(fl, fmt, palette_sizes or q128_palette_sizes(w, h, fmt))
+    if palette_sizes is None:
+        palette_sizes = {FMT_DXT1: Q128_DXT1, FMT_DXT5: Q128_DXT5}.get(fmt, Q128_ALPHA)
     return compress(fl, fmt, palette_sizes)
 
 
-def mixed_set(n=512, seed=2024, workers=None):
+def mixed_set(n=512, seed=2024, workers=None, writer="simple"):
     """The serving-style texture set: n seeded textures with sides
     log-uniform in [64, 2048] and full mip chains, 60 % RGB -> DXT1, 25 % RGBA
     -> DXT5, 15 % normal maps -> DXN_XY, each packed to .crn by the project's
@@ -135,7 +183,15 @@ def mixed_set(n=512, seed=2024, workers=None):
         u = rng.random()
         fmt = FMT_DXT1 if u < 0.60 else (FMT_DXT5 if u < 0.85 else FMT_DXN_XY)
         jobs.append((fmt, side, 5000 + i))
-    workers = workers or min(16, os.cpu_count() or 1)
-    with ThreadPoolExecutor(max_workers=workers) as pool:
-        data = list(pool.map(lambda j: synth_crn((j[1], j[1]), j[2], j[0]), jobs))
+    if writer == "hc":
+        # mip chains on CPU threads, the GPU dxt_hc and writer on the calling thread in order
+        kinds = {FMT_DXT1: RGB, FMT_DXT5: RGBA, FMT_DXN_XY: NORMAL}
+        workers = workers or min(16, os.cpu_count() or 1)
+        with ThreadPoolExecutor(max_workers=workers) as pool:
+            chains = pool.map(lambda j: mip_chain(synth(j[1], j[1], j[2], kinds[j[0]])), jobs)
+            data = [compress_hc([ch], j[0], q128_palette_sizes(j[1], j[1], j[0])) for j, ch in zip(jobs, chains)]
+    else:
+        workers = workers or min(16, os.cpu_count() or 1)
+        with ThreadPoolExecutor(max_workers=workers) as pool:
+            data = list(pool.map(lambda j: synth_crn((j[1], j[1]), j[2], j[0]), jobs))
     return [(j[0], j[1], d) for j, d in zip(jobs, data)]
