@@ -44,6 +44,11 @@ class PackInput(C.Structure):
         ("alpha1_endpoint_idx", C.c_void_p), ("alpha1_selector_idx", C.c_void_p), ("order_palettes", C.c_uint32)]
 
 
+class EncodeItem(C.Structure):
+    _fields_ = [("rgba", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32),
+                ("blocks", C.c_void_p)]
+
+
 # name -> (restype, argtypes); every function declared in include/crn2.h
 SIGNATURES = {
     "crn2_validate_file": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -89,6 +94,13 @@ SIGNATURES = {
     "crn2_encode_image_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
                                            C.c_void_p, C.c_void_p]),
     "crn2_compress_blocks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_void_p]),
+    "crn2_encode_set_create": (C.c_int, [C.POINTER(EncodeItem), C.c_uint32, C.c_int, C.c_int,
+                                         C.POINTER(C.c_void_p)]),
+    "crn2_encode_set_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "crn2_encode_set_run_graph": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "crn2_encode_set_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "crn2_encode_set_destroy": (C.c_int, [C.c_void_p]),
     "crn2_decode_dxt_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                          C.c_uint32, C.c_void_p]),
     "crn2_dxt_error_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,

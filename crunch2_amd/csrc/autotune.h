@@ -17,7 +17,10 @@
 
 namespace crn2 {
 
-enum TuneKernel : int { TUNE_HC_SEL = 0, TUNE_DXT1 = 1, TUNE_DXT5A = 2, TUNE_MLP_INFER = 3, TUNE_COUNT = 4 };
+enum TuneKernel : int {
+    TUNE_HC_SEL = 0, TUNE_DXT1 = 1, TUNE_DXT5A = 2, TUNE_MLP_INFER = 3, TUNE_DXT1_SET = 4, TUNE_DXT5A_SET = 5,
+    TUNE_COUNT = 6
+};
 
 struct Autotuner {
     std::mutex mu;

@@ -27,6 +27,7 @@
 
 #include "autotune.h"
 #include "crn2_internal.h"
+#include "encode_set.h"
 
 namespace {
 
@@ -1088,6 +1089,29 @@ __global__ void __launch_bounds__(256, 2) dxt1_image_kernel(const uint32_t* __re
     if (err) err[b] = o.error;
 }
 
+// Texture-set encode: colour elements of many level images in one flat
+// order, each thread walking it with a grid stride (a persistent grid when
+// the launch is smaller than the work).
+__global__ void __launch_bounds__(256, 2) dxt1_set_kernel(const crn2::SetJob* __restrict__ jobs, uint32_t njobs,
+                                                         uint64_t begin, uint64_t end, int quality)
+{
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = begin + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < end; i += step) {
+        const crn2::SetJob j = jobs[crn2::set_job_of(jobs, njobs, i)];
+        const uint32_t b = (uint32_t)(i - j.first);
+        const uint32_t bx = b % j.blocks_x, by = b / j.blocks_x;
+        uint32_t p[16];
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const uint32_t sy = min(by * 4 + y, j.height - 1);
+#pragma unroll
+            for (int x = 0; x < 4; x++) p[4 * y + x] = j.rgba[(size_t)sy * j.width + min(bx * 4 + x, j.width - 1)];
+        }
+        const Dxt1Out o = encode_block(p, quality, (int)j.flags);
+        j.out[(size_t)b * j.stride] = (uint64_t)o.lo_hi | ((uint64_t)o.selectors << 32);
+    }
+}
+
 }  // namespace
 
 extern "C" int crn2_dxt1_init(void)
@@ -1178,5 +1202,17 @@ extern "C" int crn2_encode_dxt1_image_device(const void* rgba, uint32_t width, u
                        static_cast<hipStream_t>(stream), static_cast<const uint32_t*>(rgba), width, height,
                        (uint32_t)bx, (uint32_t)n, quality, flags, static_cast<uint64_t*>(blocks), block_stride_u64,
                        static_cast<uint64_t*>(errors));
+    return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
+}
+
+extern "C" int crn2_dxt1_set_launch(const crn2::SetJob* jobs, uint32_t njobs, uint64_t begin, uint64_t end,
+                                    int quality, uint32_t grid, uint32_t tb, hipStream_t stream)
+{
+    if (!jobs || !njobs || !grid || (tb != 64 && tb != 128 && tb != 256) || quality < 3 || quality > 4)
+        return CRN2_EINVAL;
+    if (begin >= end) return CRN2_OK;
+    const int rc = dxt1_tables_ready();
+    if (rc) return rc;
+    hipLaunchKernelGGL(dxt1_set_kernel, dim3(grid), dim3(tb), 0, stream, jobs, njobs, begin, end, quality);
     return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
 }

@@ -24,6 +24,7 @@
 
 #include "autotune.h"
 #include "crn2_internal.h"
+#include "encode_set.h"
 
 namespace {
 
@@ -215,6 +216,27 @@ __global__ void __launch_bounds__(256) dxt5a_image_kernel(const uint8_t* __restr
     if (err) err[b] = r.err;
 }
 
+// Texture-set encode, alpha elements (DXT5 alpha, DXN channels, DXT5A) of
+// many level images in one flat order, grid-stride as dxt1_set_kernel.
+__global__ void __launch_bounds__(256) dxt5a_set_kernel(const crn2::SetJob* __restrict__ jobs, uint32_t njobs,
+                                                        uint64_t begin, uint64_t end, int quality)
+{
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = begin + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < end; i += step) {
+        const crn2::SetJob j = jobs[crn2::set_job_of(jobs, njobs, i)];
+        const uint32_t b = (uint32_t)(i - j.first);
+        const uint32_t bx = b % j.blocks_x, by = b / j.blocks_x;
+        uint8_t v[16];
+        for (int y = 0; y < 4; y++)
+            for (int x = 0; x < 4; x++) {
+                const uint32_t sx = min(bx * 4 + x, j.width - 1), sy = min(by * 4 + y, j.height - 1);
+                v[y * 4 + x] = (uint8_t)(j.rgba[(size_t)sy * j.width + sx] >> (8 * j.comp));
+            }
+        const E2Result r = dxt5a_search(v, quality, (int)j.flags);
+        j.out[(size_t)b * j.stride] = (uint64_t)r.first | ((uint64_t)r.second << 8) | (r.sel48 << 16);
+    }
+}
+
 }  // namespace
 
 extern "C" int crn2_dxt5a_blocks_device(const void* values, uint32_t nblocks, int quality, int use_both,
@@ -247,5 +269,15 @@ extern "C" int crn2_encode_dxt5a_image_device(const void* rgba, uint32_t width, 
     hipLaunchKernelGGL(dxt5a_image_kernel, dim3(nb), dim3(tb), 0, static_cast<hipStream_t>(stream),
                        static_cast<const uint8_t*>(rgba), width, height, comp, bx, by, quality, use_both,
                        static_cast<uint64_t*>(blocks), block_stride_u64, static_cast<uint64_t*>(errors));
+    return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
+}
+
+extern "C" int crn2_dxt5a_set_launch(const crn2::SetJob* jobs, uint32_t njobs, uint64_t begin, uint64_t end,
+                                     int quality, uint32_t grid, uint32_t tb, hipStream_t stream)
+{
+    if (!jobs || !njobs || !grid || (tb != 64 && tb != 128 && tb != 256) || quality < 0 || quality > 4)
+        return CRN2_EINVAL;
+    if (begin >= end) return CRN2_OK;
+    hipLaunchKernelGGL(dxt5a_set_kernel, dim3(grid), dim3(tb), 0, stream, jobs, njobs, begin, end, quality);
     return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
 }

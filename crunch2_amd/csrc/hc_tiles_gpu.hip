@@ -14,7 +14,8 @@
 //
 // Three kernels, all exact restatements with the reference's operation order
 // (the library is built with -ffp-contract=off):
-//   tile_error_kernel   one thread per (chunk, tile): colour and alpha errors;
+//   tile_error_kernel   one thread per (chunk, tile): colour errors;
+//   tile_alpha_kernel   one wave per (chunk, tile, alpha channel): alpha errors;
 //   tile_choose_kernel  one thread per chunk: PSNR, derating, best covering,
 //                       block encodings, tile indices, kept tiles' sources;
 //   tile_vec_kernel     one thread per kept tile: palettized endpoints.
@@ -282,74 +283,16 @@ __device__ uint32_t tile_color_error(const uint32_t (&px)[N])
     return err;
 }
 
-// dxt5_endpoint_optimizer at normal quality, 8-value blocks only: the best
-// error over every pair of distinct values (first-appearance order), per
-// value the nearest palette entry (first of the smallest, stop at zero), with
-// the reference's early exits (crn_dxt5a.cpp:53-103, 198-262).
-template <int N>
-__device__ uint32_t tile_alpha_error(const uint32_t (&px)[N], uint32_t comp)
-{
-    uint8_t uv[N];
-    uint8_t uw[N];
-    uint32_t nu = 0;
-    for (int i = 0; i < N; i++) {
-        const uint32_t v = (px[i] >> (8 * comp)) & 255u;
-        uint32_t k = 0;
-        while (k < nu && uv[k] != v) k++;
-        if (k == nu) {
-            uv[nu] = (uint8_t)v;
-            uw[nu] = 0;
-            nu++;
-        }
-        uw[k]++;
-    }
-    if (nu == 1) return 0;
-    uint64_t best = ~0ull;
-    for (uint32_t i = 0; i + 1 < nu; i++)
-        for (uint32_t j = i + 1; j < nu; j++) {
-            const uint32_t l = uv[i], h = uv[j];
-            uint32_t pal[8];
-            pal[0] = l;
-            pal[1] = h;
-            pal[2] = (l * 6 + h) / 7;
-            pal[3] = (l * 5 + h * 2) / 7;
-            pal[4] = (l * 4 + h * 3) / 7;
-            pal[5] = (l * 3 + h * 4) / 7;
-            pal[6] = (l * 2 + h * 5) / 7;
-            pal[7] = (l + h * 6) / 7;
-            uint64_t err = 0;
-            for (uint32_t u = 0; u < nu; u++) {
-                const int val = uv[u], w = uw[u];
-                uint32_t b = 0xFFFFFFFFu;
-                for (int s = 0; s < 8; s++) {
-                    int e = val - (int)pal[s];
-                    e = e * e * w;
-                    if ((uint32_t)e < b) {
-                        b = (uint32_t)e;
-                        if (!b) break;
-                    }
-                }
-                err += b;
-                if (err > best) break;
-            }
-            if (err < best) {
-                best = err;
-                if (!err) return 0;
-            }
-        }
-    return (uint32_t)best;
-}
-
 template <int N>
 __device__ void tile_errors(const TileArgs& a, const Chunk& c, uint32_t t, uint32_t* out)
 {
     uint32_t px[N];
 #pragma unroll
     for (int i = 0; i < N; i++) px[i] = tile_texel(a.blocks, c.b, c.width, t, (uint32_t)i);
-    if (a.has_color) out[t] = tile_color_error<N>(px);
-    for (int k = 0; k < a.num_alpha; k++) out[9 * (1 + k) + t] = tile_alpha_error<N>(px, a.alpha_comp[k]);
+    out[t] = tile_color_error<N>(px);
 }
 
+// Colour errors, one thread per (chunk, tile).
 __global__ void __launch_bounds__(256) tile_error_kernel(TileArgs a)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -360,6 +303,93 @@ __global__ void __launch_bounds__(256) tile_error_kernel(TileArgs a)
     if (t < 4) tile_errors<16>(a, c, t, out);
     else if (t < 8) tile_errors<32>(a, c, t, out);
     else tile_errors<64>(a, c, t, out);
+}
+
+// Alpha errors, one wave per (chunk, tile, alpha channel): the error of
+// dxt5_endpoint_optimizer at normal quality, 8-value blocks only
+// (crn_dxt5a.cpp:53-103, 198-262): the best error over every pair of distinct
+// values, per value the nearest palette entry.  The reference walks the pairs
+// in first-appearance order with early exits; its result is the smallest
+// exact error over the unordered pairs of distinct values (the 8-value
+// palette of (l, h) is the palette of (h, l), and the early exits only skip
+// pairs that cannot lower the minimum), so here the distinct values come from
+// a 256-bin histogram in LDS, the lanes split the pairs, and a wave minimum
+// finishes.
+constexpr uint32_t kAlphaWaves = 4;
+__global__ void __launch_bounds__(64 * kAlphaWaves) tile_alpha_kernel(TileArgs a)
+{
+    __shared__ uint32_t hist[kAlphaWaves][256];
+    __shared__ uint8_t uv[kAlphaWaves][64];
+    __shared__ uint8_t uw[kAlphaWaves][64];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t job = blockIdx.x * kAlphaWaves + wave;
+    const uint32_t njobs = a.nchunks * 9 * (uint32_t)a.num_alpha;
+    if (job >= njobs) return;   // whole waves leave together
+    const uint32_t k = job % (uint32_t)a.num_alpha, ct = job / (uint32_t)a.num_alpha;
+    const uint32_t ci = ct / 9, t = ct % 9;
+    const Chunk c = a.chunks[ci];
+    const uint32_t n = t < 4 ? 16 : (t < 8 ? 32 : 64);
+    uint32_t* h = hist[wave];
+    for (uint32_t v = lane; v < 256; v += 64) h[v] = 0;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane < n) atomicAdd(&h[(tile_texel(a.blocks, c.b, c.width, t, lane) >> (8 * a.alpha_comp[k])) & 255u], 1u);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    // compact the occupied bins (any order: the result does not depend on it)
+    uint32_t nu = 0;
+    for (uint32_t v0 = 0; v0 < 256; v0 += 64) {
+        const uint32_t cnt = h[v0 + lane];
+        const uint64_t occ = __ballot(cnt != 0);
+        if (cnt) {
+            const uint32_t r = nu + (uint32_t)__popcll(occ & ((1ull << lane) - 1ull));
+            uv[wave][r] = (uint8_t)(v0 + lane);
+            uw[wave][r] = (uint8_t)cnt;
+        }
+        nu += (uint32_t)__popcll(occ);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    uint64_t best = ~0ull;
+    if (nu > 1) {
+        const uint32_t npairs = nu * (nu - 1) / 2;
+        for (uint32_t p = lane; p < npairs; p += 64) {
+            // pair p in row-major order of the strict upper triangle
+            uint32_t i = 0, left = p;
+            while (left >= nu - 1 - i) {
+                left -= nu - 1 - i;
+                i++;
+            }
+            const uint32_t j = i + 1 + left;
+            const uint32_t l = uv[wave][i], hv = uv[wave][j];
+            uint32_t pal[8];
+            pal[0] = l;
+            pal[1] = hv;
+            pal[2] = (l * 6 + hv) / 7;
+            pal[3] = (l * 5 + hv * 2) / 7;
+            pal[4] = (l * 4 + hv * 3) / 7;
+            pal[5] = (l * 3 + hv * 4) / 7;
+            pal[6] = (l * 2 + hv * 5) / 7;
+            pal[7] = (l + hv * 6) / 7;
+            uint64_t err = 0;
+            for (uint32_t u = 0; u < nu && err < best; u++) {
+                const int val = uv[wave][u], w = uw[wave][u];
+                uint32_t b = 0xFFFFFFFFu;
+#pragma unroll
+                for (int s = 0; s < 8; s++) {
+                    const int e = val - (int)pal[s];
+                    b = min(b, (uint32_t)(e * e * w));
+                }
+                err += b;
+            }
+            best = min(best, err);
+        }
+    } else {
+        best = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) best = min(best, (uint64_t)__shfl_xor(best, o, 64));
+    if (lane == 0) a.terr[(size_t)ci * 27 + 9 * (1 + k) + t] = (uint32_t)best;
 }
 
 __global__ void __launch_bounds__(256) tile_choose_kernel(TileArgs a)
@@ -764,7 +794,12 @@ extern "C" int crn2_hc_tiles_device(const void* blocks, uint32_t num_blocks, uin
         hipMemsetAsync(tile_vec, 0, (size_t)num_blocks * 10 * sizeof(float), st) != hipSuccess))
         rc = CRN2_EHIP;
     if (rc == CRN2_OK) {
-        hipLaunchKernelGGL(tile_error_kernel, dim3((nch * 9 + 255) / 256), dim3(256), 0, st, a);
+        if (has_color) hipLaunchKernelGGL(tile_error_kernel, dim3((nch * 9 + 255) / 256), dim3(256), 0, st, a);
+        if (num_alpha) {
+            const uint32_t jobs = nch * 9 * num_alpha;
+            hipLaunchKernelGGL(tile_alpha_kernel, dim3((jobs + kAlphaWaves - 1) / kAlphaWaves), dim3(64 * kAlphaWaves),
+                               0, st, a);
+        }
         hipLaunchKernelGGL(tile_choose_kernel, dim3((nch + 255) / 256), dim3(256), 0, st, a);
         hipLaunchKernelGGL(tile_vec_kernel, dim3((num_blocks + 63) / 64), dim3(64), 0, st, a);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = CRN2_EHIP;

@@ -17,9 +17,12 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <new>
 #include <vector>
 
+#include "autotune.h"
 #include "crn2_internal.h"
+#include "encode_set.h"
 
 namespace {
 
@@ -337,4 +340,197 @@ extern "C" int crn2_compress_crn(const uint8_t* const* levels, uint32_t width, u
         }
     }
     return crn2_pack(&in, out_file, out_size);
+}
+
+// ------------------------------------------------------ texture-set encode
+// A whole set of level images (any sizes, DXT1 / DXT5 / DXT5A / DXN formats)
+// encoded in two launches: every colour element of the set in one flat order
+// (dxt1_set_kernel), then every alpha element (dxt5a_set_kernel), as the
+// reference's task pool hands out blocks of one image (crn_dxt_image.cpp:
+// 480-485) but across images.  Launch shapes (threads per workgroup x
+// workgroups, identical results) come from the autotuner, timed on a sample
+// of the set's own elements the first time a plan runs with tuning on.
+struct crn2_encode_set {
+    uint32_t magic;
+    crn2::SetJob* d_jobs[2];   // colour, alpha
+    uint32_t njobs[2];
+    uint64_t total[2];
+    int quality;
+    int shape[2];              // index into kSetShapes; -1 = not chosen yet
+    hipGraphExec_t graph;
+};
+
+namespace {
+constexpr uint32_t kSetMagic = 0x5E7E7C0Du;
+struct SetShape { uint32_t tb, wg_per_cu; };   // wg_per_cu 0: one thread per element
+constexpr SetShape kSetShapes[] = {{256, 0}, {128, 0}, {64, 0}, {256, 8}, {128, 16}, {64, 32}};
+constexpr int kNumSetShapes = sizeof(kSetShapes) / sizeof(kSetShapes[0]);
+constexpr uint64_t kTuneSample = 1u << 17;   // elements each candidate shape is timed on
+
+int set_launch(const crn2_encode_set* s, int t, int shape, uint64_t begin, uint64_t end, hipStream_t st)
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    const SetShape sh = kSetShapes[shape];
+    const uint64_t n = end - begin, full = (n + sh.tb - 1) / sh.tb;
+    const uint64_t cap = sh.wg_per_cu ? (uint64_t)cus * sh.wg_per_cu : full;
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(full, std::min<uint64_t>(cap, 1u << 30)));
+    return t == 0 ? crn2_dxt1_set_launch(s->d_jobs[0], s->njobs[0], begin, end, s->quality, grid, sh.tb, st)
+                  : crn2_dxt5a_set_launch(s->d_jobs[1], s->njobs[1], begin, end, s->quality, grid, sh.tb, st);
+}
+
+int set_choose_shapes(crn2_encode_set* s, hipStream_t st)
+{
+    for (int t = 0; t < 2; t++) {
+        if (s->shape[t] >= 0 || !s->total[t]) continue;
+        const uint64_t end = std::min<uint64_t>(s->total[t], kTuneSample);
+        // the tuner keys its choice by kernel and the sample's size class
+        s->shape[t] = crn2::tuner().pick(t == 0 ? crn2::TUNE_DXT1_SET : crn2::TUNE_DXT5A_SET, end, kNumSetShapes, 0,
+                                         st, [&](int sh) {
+                                             return set_launch(s, t, sh, 0, end, st) == CRN2_OK ? hipSuccess
+                                                                                                  : hipErrorUnknown;
+                                         });
+    }
+    return CRN2_OK;
+}
+
+int set_run(crn2_encode_set* s, hipStream_t st)
+{
+    for (int t = 0; t < 2; t++)
+        if (s->total[t]) {
+            const int rc = set_launch(s, t, s->shape[t] < 0 ? 0 : s->shape[t], 0, s->total[t], st);
+            if (rc) return rc;
+        }
+    return CRN2_OK;
+}
+}  // namespace
+
+extern "C" int crn2_encode_set_create(const crn2_encode_item* items, uint32_t n, int quality, int flags,
+                                      crn2_encode_set** out)
+{
+    if (!out) return CRN2_EINVAL;
+    *out = nullptr;
+    if (!items || !n || quality < 3 || quality > 4 || (flags & ~7)) return CRN2_EINVAL;
+    const bool perceptual = flags & 1, both = (flags >> 1) & 1, gray = (flags >> 2) & 1;
+    std::vector<crn2::SetJob> jobs[2];
+    uint64_t total[2] = {0, 0};
+    auto add = [&](int t, const crn2_encode_item& it, uint64_t* o, uint32_t stride, uint32_t fl, uint32_t comp) {
+        crn2::SetJob j;
+        j.rgba = static_cast<const uint32_t*>(it.rgba);
+        j.out = o;
+        j.first = total[t];
+        j.width = it.width;
+        j.height = it.height;
+        j.blocks_x = (it.width + 3) / 4;
+        j.stride = stride;
+        j.flags = fl;
+        j.comp = comp;
+        jobs[t].push_back(j);
+        total[t] += (uint64_t)j.blocks_x * ((it.height + 3) / 4);
+    };
+    const uint32_t cflags = (perceptual ? 1u : 0u) | (gray ? 4u : 0u);
+    for (uint32_t i = 0; i < n; i++) {
+        const crn2_encode_item& it = items[i];
+        if (!it.rgba || !it.blocks || !it.width || !it.height || (reinterpret_cast<uintptr_t>(it.rgba) & 3) ||
+            (reinterpret_cast<uintptr_t>(it.blocks) & 7))
+            return CRN2_EINVAL;
+        uint64_t* o = static_cast<uint64_t*>(it.blocks);
+        switch (it.format) {
+            case CRN2_FMT_DXT1: add(0, it, o, 1, cflags | (both ? 2u : 0u), 0); break;
+            case CRN2_FMT_DXT5:
+                add(1, it, o, 2, both, 3);
+                add(0, it, o + 1, 2, cflags, 0);
+                break;
+            case CRN2_FMT_DXT5A: add(1, it, o, 1, both, 3); break;
+            case CRN2_FMT_DXN_XY:
+            case CRN2_FMT_DXN_YX: {
+                const uint32_t c0 = it.format == CRN2_FMT_DXN_XY ? 0 : 1;
+                add(1, it, o, 2, both, c0);
+                add(1, it, o + 1, 2, both, 1 - c0);
+                break;
+            }
+            default: return CRN2_EUNSUPPORTED;
+        }
+    }
+    crn2_encode_set* s = new (std::nothrow) crn2_encode_set;
+    if (!s) return CRN2_ENOMEM;
+    s->magic = kSetMagic;
+    s->quality = quality;
+    s->graph = nullptr;
+    for (int t = 0; t < 2; t++) {
+        s->d_jobs[t] = nullptr;
+        s->njobs[t] = (uint32_t)jobs[t].size();
+        s->total[t] = total[t];
+        s->shape[t] = -1;
+        if (!jobs[t].empty()) {
+            const size_t bytes = jobs[t].size() * sizeof(crn2::SetJob);
+            if (hipMalloc(reinterpret_cast<void**>(&s->d_jobs[t]), bytes) != hipSuccess ||
+                hipMemcpy(s->d_jobs[t], jobs[t].data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+                crn2_encode_set_destroy(s);
+                return CRN2_ENOMEM;
+            }
+        }
+    }
+    *out = s;
+    return CRN2_OK;
+}
+
+extern "C" int crn2_encode_set_run(crn2_encode_set* s, void* stream)
+{
+    if (!s || s->magic != kSetMagic) return CRN2_EINVAL;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int rc = set_choose_shapes(s, st);
+    return rc ? rc : set_run(s, st);
+}
+
+extern "C" int crn2_encode_set_run_graph(crn2_encode_set* s, void* stream)
+{
+    if (!s || s->magic != kSetMagic) return CRN2_EINVAL;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (!s->graph) {
+        int rc = set_choose_shapes(s, st);
+        if (rc) return rc;
+        if (hipStreamSynchronize(st) != hipSuccess) return CRN2_EHIP;
+        hipStream_t cs = nullptr;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return CRN2_EHIP;
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = CRN2_EHIP;
+        if (!rc) rc = set_run(s, cs);
+        if (hipStreamEndCapture(cs, &g) != hipSuccess && !rc) rc = CRN2_EHIP;
+        if (!rc && hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0) != hipSuccess) {
+            s->graph = nullptr;
+            rc = CRN2_EHIP;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cs);
+        if (rc) return rc;
+    }
+    return hipGraphLaunch(s->graph, st) == hipSuccess ? CRN2_OK : CRN2_EHIP;
+}
+
+extern "C" int crn2_encode_set_info(const crn2_encode_set* s, uint64_t* colour_elements, uint64_t* alpha_elements,
+                                    int* colour_shape, int* alpha_shape)
+{
+    if (!s || s->magic != kSetMagic) return CRN2_EINVAL;
+    if (colour_elements) *colour_elements = s->total[0];
+    if (alpha_elements) *alpha_elements = s->total[1];
+    if (colour_shape) *colour_shape = s->shape[0];
+    if (alpha_shape) *alpha_shape = s->shape[1];
+    return CRN2_OK;
+}
+
+extern "C" int crn2_encode_set_destroy(crn2_encode_set* s)
+{
+    if (!s || s->magic != kSetMagic) return CRN2_EINVAL;
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    for (int t = 0; t < 2; t++)
+        if (s->d_jobs[t]) (void)hipFree(s->d_jobs[t]);
+    s->magic = 0;
+    delete s;
+    return CRN2_OK;
 }
