@@ -1216,3 +1216,5 @@ extern "C" int crn2_dxt1_set_launch(const crn2::SetJob* jobs, uint32_t njobs, ui
     hipLaunchKernelGGL(dxt1_set_kernel, dim3(grid), dim3(tb), 0, stream, jobs, njobs, begin, end, quality);
     return hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
 }
+
+extern "C" int crn2_dxt1_set_prepare(void) { return dxt1_tables_ready(); }

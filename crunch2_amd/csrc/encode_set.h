@@ -36,6 +36,9 @@ __device__ inline uint32_t set_job_of(const SetJob* __restrict__ jobs, uint32_t 
 }  // namespace crn2
 
 extern "C" {
+// Per-device set-up the colour set kernel needs (the single-colour tables);
+// done before any stream capture, since it copies to device memory.
+int crn2_dxt1_set_prepare(void);
 // Queue the set kernels over flat elements [begin, end): grid workgroups of tb
 // threads, each thread taking elements begin + tid, + grid * tb, ... (internal).
 int crn2_dxt1_set_launch(const crn2::SetJob* jobs, uint32_t njobs, uint64_t begin, uint64_t end, int quality,

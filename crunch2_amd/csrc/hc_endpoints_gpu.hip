@@ -9,7 +9,7 @@
 //   2. hc_ep_blocks_kernel, one thread per block: the cluster's four colours
 //      (rounded middle colours when the search chose alternate rounding), the
 //      block's weight and its 16 nearest-colour selectors;
-//   3. hc_ep_refine_kernel, one thread per cluster: the refiner -- double
+//   3. hc_ep_refine_kernel, one wave per cluster: the refiner -- double
 //      least-squares sums in texel order (so that they round as the
 //      reference's do), then up to 8 rounds over the +-1 neighbours of either
 //      endpoint scored through per-selector histograms with the selectors
@@ -201,21 +201,27 @@ __global__ void __launch_bounds__(64) hc_ep_refine_kernel(const uint32_t* __rest
                                                           uint32_t* __restrict__ color_values,
                                                           uint32_t* __restrict__ not_encoded)
 {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wave per cluster, as hc_alpha_refine_kernel: the serial texel-order
+    // sums run on every lane, the texels arrive 64 at a time
+    const uint32_t c = blockIdx.x, lane = threadIdx.x;
     if (c >= nclusters) return;
     const uint32_t o = ofs[c], n = ofs[c + 1] - o;
-    endpoints[c] = 0;
-    for (int i = 0; i < 4; i++) color_values[4 * c + i] = 0;
+    if (lane == 0) {
+        endpoints[c] = 0;
+        for (int i = 0; i < 4; i++) color_values[4 * c + i] = 0;
+    }
     if (!n) return;
     const ClusterRes r = res[c];
     if (r.flags & 256u) {
-        atomicAdd(not_encoded, 1u);
+        if (lane == 0) atomicAdd(not_encoded, 1u);
         return;
     }
     int cv[4][3];
     cluster_colors(r, cv, false);
-    for (int i = 0; i < 4; i++)
-        color_values[4 * c + i] = (uint32_t)cv[i][0] | (uint32_t)cv[i][1] << 8 | (uint32_t)cv[i][2] << 16 | 0xFF000000u;
+    if (lane == 0)
+        for (int i = 0; i < 4; i++)
+            color_values[4 * c + i] =
+                (uint32_t)cv[i][0] | (uint32_t)cv[i][1] << 8 | (uint32_t)cv[i][2] << 16 | 0xFF000000u;
     // dxt_endpoint_refiner::refine (crn_dxt_endpoint_refiner.cpp:36-144)
     const uint32_t* px = px_all + o;
     const uint8_t* sel = sel_all + o;
@@ -224,9 +230,13 @@ __global__ void __launch_bounds__(64) hc_ep_refine_kernel(const uint32_t* __rest
     uint64_t hist[4] = {0, 0, 0, 0}, D2[4][3], DD[4][3];
     for (int s = 0; s < 4; s++)
         for (int k = 0; k < 3; k++) D2[s][k] = DD[s][k] = 0;
-    for (uint32_t i = 0; i < n; i++) {
-        const uint32_t p = px[i];
-        const uint32_t s = sel[i] & 3u;
+    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+      const uint32_t mine = i0 + lane < n ? (px[i0 + lane] & 0x00FFFFFFu) | ((uint32_t)(sel[i0 + lane] & 3u) << 24) : 0u;
+      const uint32_t m = min(64u, n - i0);
+      for (uint32_t t = 0; t < m; t++) {
+        const uint32_t p = (uint32_t)__builtin_amdgcn_readlane((int)mine, (int)t);
+        const uint32_t i = i0 + t;
+        const uint32_t s = p >> 24;
         const int pc[3] = {(int)(p & 0xFFu), (int)((p >> 8) & 0xFFu), (int)((p >> 16) & 0xFFu)};
         const double k = (double)((float)k_to_linear[s] * 1.0f / 3.0f);
         const double alpha = 1.0f - k;
@@ -247,6 +257,7 @@ __global__ void __launch_bounds__(64) hc_ep_refine_kernel(const uint32_t* __rest
             D2[s][q] += (uint64_t)(pc[q] * 2);
             DD[s][q] += (uint64_t)(pc[q] * pc[q]);
         }
+      }
     }
     double a[3], b[3];
     if (beta2_sum == 0.0f) {
@@ -280,7 +291,7 @@ __global__ void __launch_bounds__(64) hc_ep_refine_kernel(const uint32_t* __rest
     uint32_t lo = 0, hi = 0;
     uint64_t err;
     optimize_dxt1(hist, D2, DD, perceptual, l, h, lo, hi, err);
-    endpoints[c] = err < r.error ? (lo | hi << 16) : r.lo_hi;
+    if (lane == 0) endpoints[c] = err < r.error ? (lo | hi << 16) : r.lo_hi;
 }
 
 // ------------------------------------------------------------ alpha (H.8)
@@ -296,7 +307,7 @@ __global__ void __launch_bounds__(64) hc_ep_refine_kernel(const uint32_t* __rest
 //     search; canonical order with the eight-value invert table;
 //   hc_alpha_blocks_kernel, one thread per block: values in linear order,
 //     encoding weights from the endpoint spread, 3-bit nearest-value selectors;
-//   hc_alpha_refine_kernel, one thread per cluster: the DXT5 refiner
+//   hc_alpha_refine_kernel, one wave per cluster: the DXT5 refiner
 //     (least squares on the alpha values in texel order, then the canonical
 //     23 x 23 box scored through per-selector histograms).
 struct AlphaRes {
@@ -506,40 +517,67 @@ __global__ void __launch_bounds__(64) hc_alpha_refine_kernel(const uint8_t* __re
                                                              uint32_t* __restrict__ alpha_values,
                                                              uint32_t* __restrict__ refined_values)
 {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wave per cluster: every lane runs the same (serial, texel-order)
+    // computation; the texels arrive 64 at a time by one coalesced load and
+    // are broadcast lane by lane, so no step waits on a lone global load
+    const uint32_t c = blockIdx.x, lane = threadIdx.x;
     if (c >= nclusters) return;
     const uint32_t o = ofs[c], n = ofs[c + 1] - o;
-    endpoints[c] = 0;
-    alpha_values[2 * c] = alpha_values[2 * c + 1] = 0;
-    refined_values[2 * c] = refined_values[2 * c + 1] = 0;
+    if (lane == 0) {
+        endpoints[c] = 0;
+        alpha_values[2 * c] = alpha_values[2 * c + 1] = 0;
+        refined_values[2 * c] = refined_values[2 * c + 1] = 0;
+    }
     if (!n) return;
     const AlphaRes r = res[c];
     uint32_t bv[8], av[8];
     alpha_values_of(r.first, r.second, bv);
     for (int i = 0; i < 8; i++) av[i] = bv[k5_from_linear[i]];
-    alpha_values[2 * c] = av[0] | av[1] << 8 | av[2] << 16 | av[3] << 24;
-    alpha_values[2 * c + 1] = av[4] | av[5] << 8 | av[6] << 16 | av[7] << 24;
+    if (lane == 0) {
+        alpha_values[2 * c] = av[0] | av[1] << 8 | av[2] << 16 | av[3] << 24;
+        alpha_values[2 * c + 1] = av[4] | av[5] << 8 | av[6] << 16 | av[7] << 24;
+    }
     // dxt_endpoint_refiner::refine, DXT5 selectors (crn_dxt_endpoint_refiner.cpp:36-144)
     const uint8_t* vals = vals_all + o;
     const uint8_t* sel = sel_all + o;
-    double alpha2_sum = 0.0f, beta2_sum = 0.0f, alphabeta_sum = 0.0f, ax = 0, bx = 0, first = 0;
+    // Every product the sums take depends only on the selector s and the
+    // value a: alpha = 1 - k(s), beta = k(s), x = a / 255.  The rounded
+    // products are tabulated once (LDS), so each texel adds five table
+    // entries to the five running sums in texel order -- the same sums, bit
+    // for bit, as multiplying and adding per texel.
+    __shared__ double t_ax[8][256], t_bx[8][256], t_ab[3][8];
+    for (uint32_t e = lane; e < 8 * 256; e += 64) {
+        const uint32_t s = e >> 8, a = e & 255u;
+        const double k = (double)((float)k5_to_linear[s] * 1.0f / 7.0f);
+        const double alpha = 1.0f - k, beta = k, x = (double)((float)a / 255.0f);
+        t_ax[s][a] = alpha * x;
+        t_bx[s][a] = beta * x;
+        if (a == 0) {
+            t_ab[0][s] = alpha * alpha;
+            t_ab[1][s] = beta * beta;
+            t_ab[2][s] = alpha * beta;
+        }
+    }
+    __syncthreads();
+    double alpha2_sum = 0.0f, beta2_sum = 0.0f, alphabeta_sum = 0.0f, ax = 0, bx = 0;
+    const double first = (double)((float)vals[0] / 255.0f);
     uint64_t hist[8], D2[8], DD[8];
     for (int s = 0; s < 8; s++) hist[s] = D2[s] = DD[s] = 0;
-    for (uint32_t i = 0; i < n; i++) {
-        const uint32_t s = sel[i] & 7u, a = vals[i];
-        const double k = (double)((float)k5_to_linear[s] * 1.0f / 7.0f);
-        const double alpha = 1.0f - k;
-        const double beta = k;
-        const double x = (double)((float)a / 255.0f);
-        if (!i) first = x;
-        alpha2_sum += alpha * alpha;
-        beta2_sum += beta * beta;
-        alphabeta_sum += alpha * beta;
-        ax += alpha * x;
-        bx += beta * x;
+    for (uint32_t i0 = 0; i0 < n; i0 += 64) {
+      const uint32_t mine = i0 + lane < n ? ((uint32_t)sel[i0 + lane] & 7u) | ((uint32_t)vals[i0 + lane] << 8) : 0u;
+      const uint32_t m = min(64u, n - i0);
+      for (uint32_t t = 0; t < m; t++) {
+        const uint32_t sv = (uint32_t)__builtin_amdgcn_readlane((int)mine, (int)t);
+        const uint32_t s = sv & 7u, a = sv >> 8;
+        alpha2_sum += t_ab[0][s];
+        beta2_sum += t_ab[1][s];
+        alphabeta_sum += t_ab[2][s];
+        ax += t_ax[s][a];
+        bx += t_bx[s][a];
         hist[s]++;
         D2[s] += (uint64_t)(a * 2);
         DD[s] += (uint64_t)(a * a);
+      }
     }
     double a, b;
     if (beta2_sum == 0.0f) {
@@ -596,9 +634,11 @@ __global__ void __launch_bounds__(64) hc_alpha_refine_kernel(const uint8_t* __re
         alpha_values_of(f, sc, bv);
         for (int i = 0; i < 8; i++) av[i] = bv[k5_from_linear[i]];
     }
-    refined_values[2 * c] = av[0] | av[1] << 8 | av[2] << 16 | av[3] << 24;
-    refined_values[2 * c + 1] = av[4] | av[5] << 8 | av[6] << 16 | av[7] << 24;
-    endpoints[c] = f | sc << 8 | (refined ? 1u : 0u) << 16;
+    if (lane == 0) {
+        refined_values[2 * c] = av[0] | av[1] << 8 | av[2] << 16 | av[3] << 24;
+        refined_values[2 * c + 1] = av[4] | av[5] << 8 | av[6] << 16 | av[7] << 24;
+        endpoints[c] = f | sc << 8 | (refined ? 1u : 0u) << 16;
+    }
 }
 
 inline uint32_t grid1(uint64_t n, uint32_t tb) { return (uint32_t)((n + tb - 1) / tb); }
@@ -635,7 +675,7 @@ extern "C" int crn2_hc_color_endpoints_device(const void* cluster_px, const uint
         if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
     }
     if (rc == CRN2_OK) {
-        hipLaunchKernelGGL(hc_ep_refine_kernel, dim3(grid1(nclusters, 64)), dim3(64), 0, s,
+        hipLaunchKernelGGL(hc_ep_refine_kernel, dim3(nclusters), dim3(64), 0, s,
                            static_cast<const uint32_t*>(cluster_px), cluster_ofs, nclusters, res, sel, perceptual != 0,
                            endpoints, static_cast<uint32_t*>(color_values), not_encoded);
         if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
@@ -675,7 +715,7 @@ extern "C" int crn2_hc_alpha_endpoints_device(const void* cluster_vals, const ui
         if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
     }
     if (rc == CRN2_OK) {
-        hipLaunchKernelGGL(hc_alpha_refine_kernel, dim3(grid1(nclusters, 64)), dim3(64), 0, s,
+        hipLaunchKernelGGL(hc_alpha_refine_kernel, dim3(nclusters), dim3(64), 0, s,
                            static_cast<const uint8_t*>(cluster_vals), cluster_ofs, nclusters, res, sel, endpoints,
                            static_cast<uint32_t*>(alpha_values), static_cast<uint32_t*>(refined_values));
         if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;

@@ -357,6 +357,7 @@ struct crn2_encode_set {
     uint64_t total[2];
     int quality;
     int shape[2];              // index into kSetShapes; -1 = not chosen yet
+    int cus;                   // compute units of the device the plan was made on
     hipGraphExec_t graph;
 };
 
@@ -369,13 +370,7 @@ constexpr uint64_t kTuneSample = 1u << 17;   // elements each candidate shape is
 
 int set_launch(const crn2_encode_set* s, int t, int shape, uint64_t begin, uint64_t end, hipStream_t st)
 {
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-    }
+    const int cus = s->cus;
     const SetShape sh = kSetShapes[shape];
     const uint64_t n = end - begin, full = (n + sh.tb - 1) / sh.tb;
     const uint64_t cap = sh.wg_per_cu ? (uint64_t)cus * sh.wg_per_cu : full;
@@ -386,6 +381,10 @@ int set_launch(const crn2_encode_set* s, int t, int shape, uint64_t begin, uint6
 
 int set_choose_shapes(crn2_encode_set* s, hipStream_t st)
 {
+    if (s->total[0]) {
+        const int rc = crn2_dxt1_set_prepare();
+        if (rc) return rc;
+    }
     for (int t = 0; t < 2; t++) {
         if (s->shape[t] >= 0 || !s->total[t]) continue;
         const uint64_t end = std::min<uint64_t>(s->total[t], kTuneSample);
@@ -462,6 +461,10 @@ extern "C" int crn2_encode_set_create(const crn2_encode_item* items, uint32_t n,
     s->magic = kSetMagic;
     s->quality = quality;
     s->graph = nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || s->cus <= 0)
+        s->cus = 256;
     for (int t = 0; t < 2; t++) {
         s->d_jobs[t] = nullptr;
         s->njobs[t] = (uint32_t)jobs[t].size();

@@ -208,6 +208,34 @@ int crn2_encode_image_device(const void* rgba, uint32_t width, uint32_t height, 
 int crn2_compress_blocks(const uint32_t* pixels, uint32_t nblocks, uint32_t format, int quality, int flags,
                          void* out);
 
+/* A texture set encoded in two launches (no reference counterpart for the
+ * batch; the per-block work is that of crn2_encode_image_device): every
+ * item is one level image in device memory (rgba: width x height texels, 4-byte
+ * aligned) and the device buffer its DXTn blocks go to (8-byte aligned,
+ * minimal pitch, the element layout of crn2_encode_image_device); formats
+ * DXT1, DXT5, DXT5A, DXN_XY, DXN_YX.  All colour elements of the set run in
+ * one launch, all alpha elements in a second, each thread taking elements
+ * with a grid stride.  quality 3 or 4, flags as crn2_encode_image_device.  With
+ * the autotuner on (crn2_autotune_mode(1)), the first run times six launch
+ * shapes (64 / 128 / 256 threads per workgroup, one thread per element or a
+ * persistent grid of 8 x 256 threads per CU) on the first 131,072 elements of
+ * each launch and keeps the fastest.  crn2_encode_set_run_graph captures the
+ * two launches into a HIP graph on its first call and replays it afterwards
+ * (the items' buffers must stay where they were). */
+typedef struct crn2_encode_item {
+    const void* rgba;
+    uint32_t width, height, format;
+    void* blocks;
+} crn2_encode_item;
+typedef struct crn2_encode_set crn2_encode_set;
+int crn2_encode_set_create(const crn2_encode_item* items, uint32_t n, int quality, int flags, crn2_encode_set** out);
+int crn2_encode_set_run(crn2_encode_set* set, void* stream);
+int crn2_encode_set_run_graph(crn2_encode_set* set, void* stream);
+/* Elements per launch and the chosen launch shapes (-1: default, not tuned). */
+int crn2_encode_set_info(const crn2_encode_set* set, uint64_t* colour_elements, uint64_t* alpha_elements,
+                         int* colour_shape, int* alpha_shape);
+int crn2_encode_set_destroy(crn2_encode_set* set);
+
 /* ------------------------------------------------------------ DXTn decode */
 /* crn_decompress_block (inc/crnlib.h:690, crnlib/crnlib.cpp:420-543) for a
  * whole level on the device: blocks (rows of (width+3)/4 blocks,
