@@ -288,20 +288,24 @@ const char* crn_get_dxt_quality_string(crn_dxt_quality q)
 
 namespace {
 
-// crn2 encoder flags of a crn_comp_params: bit0 perceptual, bit1 both block types, bit2 grayscale sampling
+// crn2 encoder flags of a crn_comp_params: bit0 perceptual, bit1 both block
+// types, bit2 grayscale sampling, bit3 DXT1A for transparency with the alpha
+// threshold in bits 8..15, bit4 transparent indices for black (DXT1 only)
 int encoder_flags(const crn_comp_params& p)
 {
-    return (p.get_flag(cCRNCompFlagPerceptual) ? 1 : 0) | (p.get_flag(cCRNCompFlagUseBothBlockTypes) ? 2 : 0) |
-           (p.get_flag(cCRNCompFlagGrayscaleSampling) ? 4 : 0);
+    int f = (p.get_flag(cCRNCompFlagPerceptual) ? 1 : 0) | (p.get_flag(cCRNCompFlagUseBothBlockTypes) ? 2 : 0) |
+            (p.get_flag(cCRNCompFlagGrayscaleSampling) ? 4 : 0);
+    if (crnd::crnd_get_fundamental_dxt_format(p.m_format) == cCRNFmtDXT1) {
+        if (p.get_flag(cCRNCompFlagDXT1AForTransparency)) f |= 8 | (int)((p.m_dxt1a_alpha_threshold & 255) << 8);
+        if (p.get_flag(cCRNCompFlagUseTransparentIndicesForBlack)) f |= 16;
+    }
+    return f;
 }
 
 // the parameter combinations the GPU path covers (see include/cxx/crnlib.h)
 bool gpu_supported(const crn_comp_params& p)
 {
-    if (!is_dxt(p.m_format)) return false;
-    if (p.get_flag(cCRNCompFlagDXT1AForTransparency) || p.get_flag(cCRNCompFlagUseTransparentIndicesForBlack))
-        return false;
-    return true;
+    return is_dxt(p.m_format) && p.m_dxt1a_alpha_threshold <= 255;
 }
 
 // palette sizes of crn_comp::quantize_images (crnlib/crn_comp.cpp:527-577)
@@ -433,8 +437,6 @@ crn_block_compressor_context_t crn_create_block_compressor(const crn_comp_params
 {
     const crn_format f = crnd::crnd_get_fundamental_dxt_format(params.m_format);
     if (!is_dxt(f) || !gpu_supported(params)) return nullptr;
-    if (f != cCRNFmtDXT5A && f != cCRNFmtDXN_XY && f != cCRNFmtDXN_YX && params.m_dxt_quality < cCRNDXTQualityBetter)
-        return nullptr;
     BlockCompressor* c = new (std::nothrow) BlockCompressor;
     if (!c) return nullptr;
     c->format = (uint32_t)f;
@@ -475,8 +477,7 @@ static_assert(sizeof(Dxt1Rec) == 24, "record");
 
 bool dxt1_block_path(const dxt1_endpoint_optimizer::params& p)
 {
-    return p.m_num_pixels == 16 && p.m_quality >= cCRNDXTQualityBetter && p.m_quality <= cCRNDXTQualityUber &&
-           !p.m_pixels_have_alpha && !p.m_use_transparent_indices_for_black;
+    return p.m_num_pixels == 16 && p.m_quality <= cCRNDXTQualityUber && p.m_dxt1a_alpha_threshold <= 255;
 }
 
 bool dxt1_cluster_path(const dxt1_endpoint_optimizer::params& p)
@@ -489,7 +490,8 @@ bool dxt1_cluster_path(const dxt1_endpoint_optimizer::params& p)
 int dxt1_flags(const dxt1_endpoint_optimizer::params& p)
 {
     return (p.m_perceptual ? 1 : 0) | (p.m_use_alpha_blocks ? 2 : 0) | (p.m_grayscale_sampling ? 4 : 0) |
-           (p.m_force_alpha_blocks ? 8 : 0);
+           (p.m_force_alpha_blocks ? 8 : 0) | (p.m_pixels_have_alpha ? 16 : 0) |
+           (p.m_use_transparent_indices_for_black ? 32 : 0) | (int)((p.m_dxt1a_alpha_threshold & 255) << 8);
 }
 
 void dxt1_fill(const Dxt1Rec& o, bool block, dxt1_endpoint_optimizer::results& r)

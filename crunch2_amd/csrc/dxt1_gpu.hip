@@ -1,8 +1,11 @@
 // dxt1_gpu.hip -- DXT1 endpoint search (crnlib's dxt1_endpoint_optimizer,
 // crnlib/crn_dxt1.cpp) for 16-texel blocks on gfx950, one lane per block.
 //
-// The search trajectory of crn_dxt1.cpp is reproduced step for step at
-// cCRNDXTQualityBetter/Uber: unique colours in first-appearance order,
+// The search trajectory of crn_dxt1.cpp is reproduced step for step at every
+// quality (superfast .. uber; probe tables and passes :716-746, the
+// projection evaluator of the qualities below Better :1594-1757), with DXT1A
+// transparency (:2116, :1380, :300-343) and the transparent-black trial of
+// try_alpha_as_black_optimization (:2001-2079): unique colours in first-appearance order,
 // perceptual weights (:1074-1108), PCA (:192-256), projection bounds and ray
 // clamp (:1135-1175), k-means seeds with fast_random jitter (:1181-1308),
 // probe sweeps along the axis, 26-neighbour and +-2 lattice moves, least-
@@ -13,9 +16,12 @@
 //
 // Two steps of the reference are result-invisible and are left out:
 //   * the error-plane bound of evaluate_solution (:1316-1322) is a rigorous
-//     lower bound on a candidate's error (every palette entry lies between
-//     the endpoints on each channel), so a pruned candidate would have been
-//     rejected by the full evaluation anyway;
+//     lower bound on a candidate's error under the RGB metrics (every palette
+//     entry lies between the endpoints on each channel), so a pruned
+//     candidate would have been rejected by the full evaluation anyway.
+//     Under grayscale sampling the error is a luma error, which the bound
+//     does not bound: there it prunes candidates the evaluation would keep,
+//     so it is evaluated (per unique colour, from the colours' planes);
 //   * the tried-solutions set (:1324-1330) only skips re-evaluations, and a
 //     re-evaluated candidate never replaces the best (strict <).
 // State lives in registers: 16-entry arrays are only indexed from fully
@@ -33,6 +39,8 @@ namespace {
 
 __device__ __constant__ int16_t k_uber_probe[15] = {0, 1, 2, 3, 5, 7, 9, 10, 13, 15, 19, 27, 43, 59, 91};
 __device__ __constant__ int16_t k_better_probe[10] = {0, 1, 2, 3, 5, 9, 15, 19, 27, 43};
+__device__ __constant__ int16_t k_normal_probe[5] = {0, 1, 3, 5, 7};
+__device__ __constant__ int16_t k_fast_probe[4] = {0, 1, 2, 3};
 __device__ __constant__ int8_t k_adj[26][3] = {
     {-1, -1, -1}, {0, -1, -1}, {1, -1, -1}, {-1, 0, -1}, {0, 0, -1}, {1, 0, -1}, {-1, 1, -1}, {0, 1, -1},
     {1, 1, -1}, {-1, -1, 0}, {0, -1, 0}, {1, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {-1, 1, 0}, {0, 1, 0},
@@ -52,11 +60,12 @@ struct Sol {
 
 struct Block {
     // parameters
-    int quality, perceptual, use_alpha, grayscale, force_alpha, hc;
+    int quality, perceptual, use_alpha, grayscale, force_alpha, have_alpha, hc;
     // unique colours
     uint32_t nu, total_w;
     int r[16], g[16], b[16];
     uint32_t w[16];
+    uint8_t pr[16], pg[16], pb[16]; // nearest 5/6/5-bit planes (grayscale bound)
     float nr[16], ng[16], nb[16];   // mean-subtracted normalised colours
     float mean[3], axis[3];
     Sol best, trial;
@@ -115,7 +124,7 @@ __device__ bool eval_uber(Block& s, uint32_t lo, uint32_t hi, int alt)
     t.err = s.best.err;
     t.alpha = 0;
     int first = 0, last = 1;
-    if (s.force_alpha) first = 1;
+    if (s.have_alpha || s.force_alpha) first = 1;
     else if (!s.use_alpha) last = 0;
     int c0r, c0g, c0b, c1r, c1g, c1b;
     unpack565(lo, 1, c0r, c0g, c0b);
@@ -154,6 +163,95 @@ __device__ bool eval_uber(Block& s, uint32_t lo, uint32_t hi, int alt)
             t.alt = alt;
         }
     }
+    t.enforce = !t.alpha && t.lo == t.hi;
+    if (t.enforce) {
+        int v;
+        if ((t.lo & 31) != 31) { t.lo++; v = 1; }
+        else { t.hi--; v = 0; }
+        uint32_t all = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) all |= (uint32_t)v << (2 * k);
+        t.sel = all;
+        t.enforced = v;
+    }
+    if (t.err < s.best.err) {
+        s.best = t;
+        s.best.nsel = (int)s.nu;
+        return true;
+    }
+    return false;
+}
+
+// evaluate_solution_fast (crn_dxt1.cpp:1594-1757), the evaluator below
+// cCRNDXTQualityBetter: each colour takes the palette entry whose projection
+// onto the endpoint axis (green weighted 24 under the perceptual metric) is
+// nearest, by midpoint thresholds, then one distance.  No candidate with the
+// alternate rounding reaches it, so the trial's rounding flag stays 0.
+__device__ bool eval_fast(Block& s, uint32_t lo, uint32_t hi)
+{
+    Sol& t = s.trial;
+    t.lo = lo;
+    t.hi = hi;
+    t.nsel = (int)s.nu;
+    t.err = s.best.err;
+    t.alpha = 0;
+    int first = 0, last = 1;
+    if (s.have_alpha || s.force_alpha) first = 1;
+    else if (!s.use_alpha) last = 0;
+    int c0r, c0g, c0b, c1r, c1g, c1b;
+    unpack565(lo, 1, c0r, c0g, c0b);
+    unpack565(hi, 1, c1r, c1g, c1b);
+    int vr = c1r - c0r, vg = c1g - c0g, vb = c1b - c0b;
+    if (s.perceptual) {
+        vr *= 8;
+        vg *= 24;
+    }
+    const int s0 = c0r * vr + c0g * vg + c0b * vb, s1 = c1r * vr + c1g * vg + c1b * vb;
+    const int dr = vr * 2, dg = vg * 2, db = vb * 2;
+    for (int type = first; type <= last; type++) {
+        int pr[4], pg[4], pb[4];
+        pr[0] = c0r; pg[0] = c0g; pb[0] = c0b;
+        pr[1] = c1r; pg[1] = c1g; pb[1] = c1b;
+        int t0, t1, t2;   // thresholds of the projection (twice the stop midpoints)
+        if (!type) {
+            pr[2] = (c0r * 2 + c1r) / 3; pg[2] = (c0g * 2 + c1g) / 3; pb[2] = (c0b * 2 + c1b) / 3;
+            pr[3] = (c1r * 2 + c0r) / 3; pg[3] = (c1g * 2 + c0g) / 3; pb[3] = (c1b * 2 + c0b) / 3;
+            const int s2 = pr[2] * vr + pg[2] * vg + pb[2] * vb, s3 = pr[3] * vr + pg[3] * vg + pb[3] * vb;
+            t0 = s2 + s0;      // below: entry 0, else 2
+            t1 = s3 + s2;      // half point
+            t2 = s1 + s3;      // below: entry 3, else 1
+        } else {
+            pr[2] = (c0r + c1r) >> 1; pg[2] = (c0g + c1g) >> 1; pb[2] = (c0b + c1b) >> 1;
+            pr[3] = pg[3] = pb[3] = 0;
+            const int s2 = pr[2] * vr + pg[2] * vg + pb[2] * vb;
+            t0 = s0 + s2;
+            t1 = s2 + s1;
+            t2 = 0;
+        }
+        uint64_t terr = 0;
+        uint32_t tsel = 0;
+        bool over = false;
+#pragma unroll
+        for (int k = 15; k >= 0; k--) {
+            if (over || k >= (int)s.nu) continue;
+            const int dot = s.r[k] * dr + s.g[k] * dg + s.b[k] * db;
+            uint32_t bi;
+            if (!type) bi = dot >= t1 ? (dot < t2 ? 3u : 1u) : (dot < t0 ? 0u : 2u);
+            else bi = dot < t0 ? 0u : (dot < t1 ? 2u : 1u);
+            const int qr = bi == 0 ? pr[0] : bi == 1 ? pr[1] : bi == 2 ? pr[2] : pr[3];
+            const int qg = bi == 0 ? pg[0] : bi == 1 ? pg[1] : bi == 2 ? pg[2] : pg[3];
+            const int qb = bi == 0 ? pb[0] : bi == 1 ? pb[1] : bi == 2 ? pb[2] : pb[3];
+            terr += (uint64_t)cd(s, k, qr, qg, qb) * s.w[k];
+            if (terr >= t.err) { over = true; continue; }
+            tsel |= bi << (2 * k);
+        }
+        if (!over && terr < t.err) {
+            t.err = terr;
+            t.alpha = type != 0;
+            t.sel = tsel;
+        }
+    }
+    t.alt = 0;
     t.enforce = !t.alpha && t.lo == t.hi;
     if (t.enforce) {
         int v;
@@ -235,9 +333,43 @@ __device__ void compute_selectors(Block& s)
     b.sel = sel;
 }
 
+// the error planes' bound of evaluate_solution (crn_dxt1.cpp:1316-1322,
+// planes :2134-2203): per channel, the colours whose plane is at or below the
+// lower endpoint's measured against it, those above the higher endpoint's
+// against that (the endpoints taken per channel in the order of the test
+// c0 < c1); only consulted under grayscale sampling
+__device__ uint64_t plane_bound(const Block& s, uint32_t lo, uint32_t hi)
+{
+    int ar, ag, ab, br, bg, bb;
+    unpack565(lo, 0, ar, ag, ab);
+    unpack565(hi, 0, br, bg, bb);
+    const int Ar = (ar << 3) | (ar >> 2), Ag = (ag << 2) | (ag >> 4), Ab = (ab << 3) | (ab >> 2);
+    const int Br = (br << 3) | (br >> 2), Bg = (bg << 2) | (bg >> 4), Bb = (bb << 3) | (bb >> 2);
+    uint64_t e = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        if (k >= (int)s.nu) continue;
+        const uint64_t w = s.w[k];
+        int d;
+        // red: a.r < b.r ? low(a) + high(b) : high(a) + low(b)
+        if (ar < br) d = s.pr[k] <= ar ? s.r[k] - Ar : (s.pr[k] > br ? s.r[k] - Br : 0);
+        else d = s.pr[k] > ar ? s.r[k] - Ar : (s.pr[k] <= br ? s.r[k] - Br : 0);
+        e += w * (uint64_t)(d * d);
+        if (ag < bg) d = s.pg[k] <= ag ? s.g[k] - Ag : (s.pg[k] > bg ? s.g[k] - Bg : 0);
+        else d = s.pg[k] > ag ? s.g[k] - Ag : (s.pg[k] <= bg ? s.g[k] - Bg : 0);
+        e += w * (uint64_t)(d * d);
+        if (ab < bb) d = s.pb[k] <= ab ? s.b[k] - Ab : (s.pb[k] > bb ? s.b[k] - Bb : 0);
+        else d = s.pb[k] > ab ? s.b[k] - Ab : (s.pb[k] <= bb ? s.b[k] - Bb : 0);
+        e += w * (uint64_t)(d * d);
+    }
+    return e;
+}
+
 __device__ inline bool evaluate(Block& s, uint32_t lo, uint32_t hi, int alt)
 {
-    return s.hc ? eval_hc(s, lo, hi, alt) : eval_uber(s, lo, hi, alt);
+    if (s.grayscale && plane_bound(s, lo, hi) >= s.best.err) return false;
+    if (s.hc) return eval_hc(s, lo, hi, alt);
+    return s.quality >= 3 ? eval_uber(s, lo, hi, alt) : eval_fast(s, lo, hi);
 }
 
 __device__ inline void canon(uint32_t& lo, uint32_t& hi)
@@ -701,11 +833,21 @@ __device__ void optimize_endpoints(Block& s, float* lo, float* hi)
     float olo[3] = {lo[0], lo[1], lo[2]}, ohi[3] = {hi[0], hi[1], hi[2]};
     const int16_t* tab = k_uber_probe;
     uint32_t range = 15, passes = 4;
-    const float dpt = .015625f;
+    float dpt = .015625f;
     if (s.quality == 3) {
         tab = k_better_probe;
         range = 10;
         passes = 2;
+    } else if (s.quality == 2) {
+        tab = k_normal_probe;
+        range = 5;
+        passes = 2;
+        dpt = .027063293f;
+    } else if (s.quality < 2) {
+        tab = k_fast_probe;
+        range = 4;
+        passes = s.quality == 0 ? 1 : 2;
+        dpt = .027063293f;
     }
     if (s.quality >= 3) try_median4(s, olo, ohi);
     float ax[2][3];
@@ -737,17 +879,21 @@ __device__ void optimize_endpoints(Block& s, float* lo, float* hi)
         sweep(tab, range, hi, ax, phigh, nh);
         for (uint32_t i = 0; i < nl; i++)
             for (uint32_t j = 0; j < nh; j++) eval_canon(s, plow[i], phigh[j]);
-        lattice(s, 0);
-        lattice(s, 1);
+        if (s.quality >= 2) {
+            lattice(s, 0);
+            lattice(s, 1);
+        }
         if (!s.best.err || (pass && s.best.err == prev)) break;
         if (s.quality >= 4) refine(s, 1);
     }
-    if (s.best.err) {
-        bool solid = false;
-        if (sel_all_equal(s.best)) solid = try_average_solid(s);
-        if (!solid && s.quality == 4) optimize_comps(s);
+    if (s.quality >= 2) {
+        if (s.best.err && !s.have_alpha) {
+            bool solid = false;
+            if (sel_all_equal(s.best)) solid = try_average_solid(s);
+            if (!solid && s.quality == 4) optimize_comps(s);
+        }
+        if (s.quality == 4 && s.best.err) try_combinatorial(s);
     }
-    if (s.quality == 4 && s.best.err) try_combinatorial(s);
 }
 
 __device__ void compute_vectors(Block& s, const float* pw, float* meanw, float (*nw)[3])
@@ -967,7 +1113,10 @@ struct Dxt1Out {
     uint32_t pad;
 };
 
-__device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags)
+// compute_internal + return_solution (crn_dxt1.cpp:2081-2232, 263-365).
+// DXT1A (flags bit 4): texels with alpha below the threshold (flags bits
+// 8..15) leave the colour set and take selector 3.
+__device__ Dxt1Out encode_core(const uint32_t* px, int quality, int flags)
 {
     Block s;
     s.quality = quality;
@@ -975,15 +1124,21 @@ __device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags)
     s.use_alpha = (flags >> 1) & 1;
     s.grayscale = (flags >> 2) & 1;
     s.force_alpha = (flags >> 3) & 1;
-    s.hc = quality == 4 && !s.force_alpha && !s.use_alpha && !s.grayscale;
+    s.have_alpha = (flags >> 4) & 1;
+    const uint32_t thresh = ((uint32_t)flags >> 8) & 255u;
+    s.hc = quality == 4 && !s.have_alpha && !s.force_alpha && !s.use_alpha && !s.grayscale;
     s.perceptual = perceptual_req && !s.grayscale;
     s.nu = 0;
-    s.total_w = 16;
+    s.total_w = 0;
     uint32_t keys[16];
-    uint32_t map = 0;    // 4 bits per texel: unique index
-    uint64_t map64 = 0;
+    uint32_t tmask = 0;  // transparent texels
+    uint64_t map64 = 0;  // 4 bits per texel: unique index
 #pragma unroll
     for (int i = 0; i < 16; i++) {
+        if (s.have_alpha && (px[i] >> 24) < thresh) {
+            tmask |= 1u << i;
+            continue;
+        }
         const uint32_t key = px[i] | 0xFF000000u;
         int k = -1;
 #pragma unroll
@@ -995,9 +1150,13 @@ __device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags)
             for (int u = 0; u < 16; u++)
                 if (u == k) {
                     keys[u] = key;
-                    s.r[u] = (int)(key & 0xFF);
-                    s.g[u] = (int)((key >> 8) & 0xFF);
-                    s.b[u] = (int)((key >> 16) & 0xFF);
+                    const uint32_t R = key & 0xFF, G = (key >> 8) & 0xFF, B = (key >> 16) & 0xFF;
+                    s.r[u] = (int)R;
+                    s.g[u] = (int)G;
+                    s.b[u] = (int)B;
+                    s.pr[u] = (uint8_t)((R >> 3) + (((R & 7) > (R >> 5)) ? 1 : 0));
+                    s.pg[u] = (uint8_t)((G >> 2) + (((G & 3) > (G >> 6)) ? 1 : 0));
+                    s.pb[u] = (uint8_t)((B >> 3) + (((B & 7) > (B >> 5)) ? 1 : 0));
                     s.w[u] = 1;
                 }
             s.nu++;
@@ -1006,9 +1165,19 @@ __device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags)
             for (int u = 0; u < 16; u++)
                 if (u == k) s.w[u]++;
         }
+        s.total_w++;
         map64 |= (uint64_t)k << (4 * i);
     }
-    (void)map;
+    Dxt1Out o;
+    o.pad = 0;
+    if (!s.nu) {
+        // no opaque texel: transparent block (the reference leaves m_error unset; 0 here)
+        o.error = 0;
+        o.lo_hi = 0;
+        o.selectors = 0xFFFFFFFFu;
+        o.flags = 1;
+        return o;
+    }
     s.best.lo = s.best.hi = 0;
     s.best.nsel = 0;
     s.best.err = ~0ull;
@@ -1016,7 +1185,7 @@ __device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags)
     s.best.sel = 0;
     s.best.alt = s.best.enforce = s.best.enforced = 0;
     s.trial = s.best;
-    if (s.nu == 1) {
+    if (s.nu == 1 && !tmask) {
         const uint32_t r = (uint32_t)s.r[0], g = (uint32_t)s.g[0], b = (uint32_t)s.b[0];
         evaluate(s, (g_om5[r][0] << 11) | (g_om6[g][0] << 5) | g_om5[b][0], (g_om5[r][1] << 11) | (g_om6[g][1] << 5) | g_om5[b][1], 0);
         if (s.use_alpha && s.best.err)
@@ -1027,7 +1196,6 @@ __device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags)
     }
     compute_selectors(s);
     const bool inv = s.best.alpha ? (s.best.lo > s.best.hi) : (s.best.lo < s.best.hi);
-    Dxt1Out o;
     o.error = s.best.err;
     const uint32_t lo = inv ? s.best.hi : s.best.lo, hi = inv ? s.best.lo : s.best.hi;
     o.lo_hi = lo | (hi << 16);
@@ -1037,13 +1205,87 @@ __device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags)
         const uint32_t u = (uint32_t)((map64 >> (4 * i)) & 15u);
         uint32_t v = (s.best.sel >> (2 * u)) & 3u;
         if (inv) v = s.best.alpha ? (v < 2 ? v ^ 1u : v) : (v ^ 1u);
+        if ((tmask >> i) & 1u) v = 3u;
         sel |= v << (2 * i);
     }
     o.selectors = sel;
     o.flags = (uint32_t)s.best.alpha | ((uint32_t)inv << 1) | ((uint32_t)s.best.alt << 2) |
               ((uint32_t)s.best.enforce << 3) | ((uint32_t)s.best.enforced << 4);
-    o.pad = 0;
     return o;
+}
+
+__device__ inline uint32_t metric(int perceptual, int grayscale, uint32_t p, int r, int g, int b)
+{
+    const int pr = (int)(p & 255u), pg = (int)((p >> 8) & 255u), pb = (int)((p >> 16) & 255u);
+    const int dr = pr - r, dg = pg - g, db = pb - b;
+    if (perceptual) return (uint32_t)(dr * dr * 8 + dg * dg * 25 + db * db);
+    if (grayscale) {
+        const int y0 = (pr * 19595 + pg * 38470 + pb * 7471 + 32768) >> 16;
+        const int y1 = (r * 19595 + g * 38470 + b * 7471 + 32768) >> 16;
+        return (uint32_t)((y0 - y1) * (y0 - y1));
+    }
+    return (uint32_t)(dr * dr + dg * dg + db * db);
+}
+
+// dxt1_endpoint_optimizer::compute (crn_dxt1.cpp:2234-2246).  flags bit 6
+// applies dxt_image's DXT1A rule first (crn_dxt_image.cpp:1439-1450: the
+// block has alpha when a texel's alpha is below the threshold); bit 5 with
+// alpha blocks allowed and no transparent texel runs
+// try_alpha_as_black_optimization (:2001-2079): texels whose r, g and b are
+// all <= 4 are made transparent, the block is searched again as DXT1A, and
+// that 3-colour block replaces the first result if its error, counting those
+// texels against black, is lower.  One copy of the search serves both runs.
+__device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags)
+{
+    if ((flags >> 6) & 1) {
+        const uint32_t thresh = ((uint32_t)flags >> 8) & 255u;
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < 16; i++) any |= (px[i] >> 24) < thresh;
+        flags = (flags & ~(64 | 16)) | (any ? 16 : 0);
+    }
+    uint32_t p[16];
+    uint32_t dark = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        p[i] = px[i];
+        if ((p[i] & 255u) <= 4 && ((p[i] >> 8) & 255u) <= 4 && ((p[i] >> 16) & 255u) <= 4) dark |= 1u << i;
+    }
+    const bool black = ((flags >> 1) & 1) && ((flags >> 5) & 1) && !((flags >> 4) & 1) && dark && dark != 0xFFFFu;
+    Dxt1Out first;
+    for (int pass = 0;; pass++) {
+        const Dxt1Out o = encode_core(p, quality, pass ? (flags | 16) : flags);
+        if (!pass) {
+            first = o;
+            if (!black) return first;
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                if ((dark >> i) & 1u) p[i] &= 0x00FFFFFFu;
+            continue;
+        }
+        // get_block_colors3 of the trial's endpoints, entry 3 black
+        int c0r, c0g, c0b, c1r, c1g, c1b;
+        unpack565(o.lo_hi & 0xFFFFu, 1, c0r, c0g, c0b);
+        unpack565(o.lo_hi >> 16, 1, c1r, c1g, c1b);
+        const int c2r = (c0r + c1r) >> 1, c2g = (c0g + c1g) >> 1, c2b = (c0b + c1b) >> 1;
+        const int gray = (flags >> 2) & 1, perc = (flags & 1) && !gray;
+        uint64_t err = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint32_t v = (o.selectors >> (2 * i)) & 3u;
+            const int r = v == 0 ? c0r : v == 1 ? c1r : v == 2 ? c2r : 0;
+            const int g = v == 0 ? c0g : v == 1 ? c1g : v == 2 ? c2g : 0;
+            const int b = v == 0 ? c0b : v == 1 ? c1b : v == 2 ? c2b : 0;
+            err += metric(perc, gray, p[i], r, g, b);
+        }
+        if (err < first.error) {
+            first.error = err;
+            first.lo_hi = o.lo_hi;
+            first.selectors = o.selectors;
+            first.flags |= 1u;
+        }
+        return first;
+    }
 }
 
 __global__ void __launch_bounds__(256, 2) dxt1_blocks_kernel(const uint32_t* __restrict__ px, uint32_t nblocks, int quality,
@@ -1170,7 +1412,7 @@ int dxt1_tables_ready()
 extern "C" int crn2_dxt1_blocks_device(const void* rgba, uint32_t nblocks, int quality, int flags, void* out,
                                        void* stream)
 {
-    if (!rgba || !out || quality < 3 || quality > 4 || (flags & ~15)) return CRN2_EINVAL;
+    if (!rgba || !out || quality < 0 || quality > 4 || (flags & ~0xFF7F)) return CRN2_EINVAL;
     if (reinterpret_cast<uintptr_t>(rgba) & 15) return CRN2_EINVAL;
     if (!nblocks) return CRN2_OK;
     const int rc = dxt1_tables_ready();
@@ -1190,7 +1432,8 @@ extern "C" int crn2_dxt1_blocks_device(const void* rgba, uint32_t nblocks, int q
 extern "C" int crn2_encode_dxt1_image_device(const void* rgba, uint32_t width, uint32_t height, int quality, int flags,
                                              void* blocks, uint32_t block_stride_u64, void* errors, void* stream)
 {
-    if (!rgba || !blocks || !width || !height || !block_stride_u64 || quality < 3 || quality > 4 || (flags & ~15))
+    if (!rgba || !blocks || !width || !height || !block_stride_u64 || quality < 0 || quality > 4 ||
+        (flags & ~0xFF7F))
         return CRN2_EINVAL;
     if (reinterpret_cast<uintptr_t>(rgba) & 3) return CRN2_EINVAL;
     const uint64_t bx = (width + 3) / 4, by = (height + 3) / 4, n = bx * by;
@@ -1208,7 +1451,7 @@ extern "C" int crn2_encode_dxt1_image_device(const void* rgba, uint32_t width, u
 extern "C" int crn2_dxt1_set_launch(const crn2::SetJob* jobs, uint32_t njobs, uint64_t begin, uint64_t end,
                                     int quality, uint32_t grid, uint32_t tb, hipStream_t stream)
 {
-    if (!jobs || !njobs || !grid || (tb != 64 && tb != 128 && tb != 256) || quality < 3 || quality > 4)
+    if (!jobs || !njobs || !grid || (tb != 64 && tb != 128 && tb != 256) || quality < 0 || quality > 4)
         return CRN2_EINVAL;
     if (begin >= end) return CRN2_OK;
     const int rc = dxt1_tables_ready();

@@ -51,22 +51,25 @@ __global__ void __launch_bounds__(256) dxt3_alpha_kernel(const uint32_t* __restr
 extern "C" int crn2_encode_image_device(const void* rgba, uint32_t width, uint32_t height, uint32_t format,
                                         int quality, int flags, void* blocks, void* stream)
 {
-    if (!rgba || !blocks || !width || !height || quality < 0 || quality > 4 || (flags & ~7)) return CRN2_EINVAL;
+    if (!rgba || !blocks || !width || !height || quality < 0 || quality > 4 || (flags & ~0xFF1F)) return CRN2_EINVAL;
     if (reinterpret_cast<uintptr_t>(rgba) & 3) return CRN2_EINVAL;
     const uint64_t bx = (width + 3) / 4, n = bx * ((height + 3) / 4);
     if (n > 0xFFFFFFFFull) return CRN2_EINVAL;
     const bool perceptual = flags & 1, both = (flags >> 1) & 1, gray = (flags >> 2) & 1;
+    // DXT1 only: DXT1A for texels with alpha below the threshold in bits
+    // 8..15, and transparent indices for black texels
+    const bool dxt1a = (flags >> 3) & 1, black = (flags >> 4) & 1;
+    const int thresh = (flags >> 8) & 255;
     hipStream_t st = static_cast<hipStream_t>(stream);
     uint64_t* out = static_cast<uint64_t*>(blocks);
-    // colour elements: the better and uber searches (the fast evaluator of lower qualities is not built)
-    const bool has_color = format <= CRN2_FMT_DXT5_AGBR;
-    if (has_color && quality < 3) return CRN2_EUNSUPPORTED;
     const int cq = quality;
     const int cflags = (perceptual ? 1 : 0) | (gray ? 4 : 0);
     switch (format) {
         case CRN2_FMT_DXT1:
-            return crn2_encode_dxt1_image_device(rgba, width, height, cq, cflags | (both ? 2 : 0), out, 1, nullptr,
-                                                 stream);
+            return crn2_encode_dxt1_image_device(rgba, width, height, cq,
+                                                 cflags | (both ? 2 : 0) | (dxt1a ? (64 | (thresh << 8)) : 0) |
+                                                     (black ? 32 : 0),
+                                                 out, 1, nullptr, stream);
         case CRN2_FMT_DXT3: {
             const uint32_t tb = 256;
             hipLaunchKernelGGL(dxt3_alpha_kernel, dim3((uint32_t)((n + tb - 1) / tb)), dim3(tb), 0, st,
@@ -414,8 +417,10 @@ extern "C" int crn2_encode_set_create(const crn2_encode_item* items, uint32_t n,
 {
     if (!out) return CRN2_EINVAL;
     *out = nullptr;
-    if (!items || !n || quality < 3 || quality > 4 || (flags & ~7)) return CRN2_EINVAL;
+    if (!items || !n || quality < 0 || quality > 4 || (flags & ~0xFF1F)) return CRN2_EINVAL;
     const bool perceptual = flags & 1, both = (flags >> 1) & 1, gray = (flags >> 2) & 1;
+    const uint32_t dxt1_extra = (((flags >> 3) & 1) ? (64u | (((uint32_t)flags >> 8 & 255u) << 8)) : 0u) |
+                                (((flags >> 4) & 1) ? 32u : 0u);
     std::vector<crn2::SetJob> jobs[2];
     uint64_t total[2] = {0, 0};
     auto add = [&](int t, const crn2_encode_item& it, uint64_t* o, uint32_t stride, uint32_t fl, uint32_t comp) {
@@ -440,7 +445,7 @@ extern "C" int crn2_encode_set_create(const crn2_encode_item* items, uint32_t n,
             return CRN2_EINVAL;
         uint64_t* o = static_cast<uint64_t*>(it.blocks);
         switch (it.format) {
-            case CRN2_FMT_DXT1: add(0, it, o, 1, cflags | (both ? 2u : 0u), 0); break;
+            case CRN2_FMT_DXT1: add(0, it, o, 1, cflags | (both ? 2u : 0u) | dxt1_extra, 0); break;
             case CRN2_FMT_DXT5:
                 add(1, it, o, 2, both, 3);
                 add(0, it, o + 1, 2, cflags, 0);

@@ -8,7 +8,17 @@ import torch
 from ._lib import lib, check
 
 PERCEPTUAL, ALPHA_BLOCKS, GRAYSCALE, FORCE_ALPHA = 1, 2, 4, 8
-UBER, BETTER = 4, 3
+# DXT1A: texels with alpha below the threshold (threshold(t), bits 8..15) are
+# transparent; BLACK_AS_TRANSPARENT runs the transparent-black trial;
+# DXT1A_PER_BLOCK sets HAVE_ALPHA for the blocks that have such a texel
+HAVE_ALPHA, BLACK_AS_TRANSPARENT, DXT1A_PER_BLOCK = 16, 32, 64
+UBER, BETTER, NORMAL, FAST, SUPERFAST = 4, 3, 2, 1, 0
+
+
+def threshold(t=128):
+    return (int(t) & 255) << 8
+
+
 OUT_DTYPE = np.dtype([("error", "<u8"), ("lo_hi", "<u4"), ("selectors", "<u4"), ("flags", "<u4"), ("pad", "<u4")])
 
 

@@ -172,9 +172,15 @@ int crn2_encode_dxt5a_image_device(const void* rgba, uint32_t width, uint32_t he
 /* crnlib's dxt1_endpoint_optimizer::compute (crnlib/crn_dxt1.h:223,
  * crnlib/crn_dxt1.cpp:2234-2246) for many 16-texel blocks at once, on the
  * device, endpoint caching off.  rgba: 16 RGBA8 texels per block (64 bytes,
- * 16-byte aligned); quality 3 (better) or 4 (uber); flags: bit0 perceptual,
- * bit1 use_alpha_blocks (3-colour blocks), bit2 grayscale_sampling, bit3
- * force_alpha_blocks.  out: 24 bytes per block = {u64 results::m_error,
+ * 16-byte aligned); quality 0 (superfast) .. 4 (uber), crn_dxt_quality;
+ * flags: bit0 perceptual, bit1 use_alpha_blocks (3-colour blocks), bit2
+ * grayscale_sampling, bit3 force_alpha_blocks, bit4 pixels_have_alpha (DXT1A:
+ * texels with alpha below the threshold in bits 8..15 are transparent,
+ * selector 3), bit5 use_transparent_indices_for_black
+ * (try_alpha_as_black_optimization, crn_dxt1.cpp:2001-2079), bit6 sets bit4
+ * per block when a texel's alpha is below the threshold (dxt_image's DXT1A
+ * rule, crn_dxt_image.cpp:1439-1450); a block without opaque texels gives
+ * error 0, endpoints 0 and all selectors 3.  out: 24 bytes per block = {u64 results::m_error,
  * u32 m_low_color | m_high_color << 16, u32 selectors (2-bit DXT1 code of
  * texel i at bit 2i), u32 m_alpha_block | m_reordered << 1 |
  * m_alternate_rounding << 2 | m_enforce_selector << 3 | m_enforced_selector << 4,
@@ -195,9 +201,13 @@ int crn2_encode_dxt1_image_device(const void* rgba, uint32_t width, uint32_t hei
  * unclustered dxt_image path: the element layout of dxt_image::init_internal
  * (crnlib/crn_dxt_image.cpp:120-200) filled by the GPU searches; the output is
  * the level's DXTn blocks in raster order at minimal pitch.  quality 0..4
- * (crn_dxt_quality; colour formats need 3 or 4: CRN2_EUNSUPPORTED below);
- * flags: bit0 perceptual, bit1 use both block types (3-colour DXT1 blocks for
- * DXT1 only, 6-value alpha blocks), bit2 grayscale sampling. */
+ * (crn_dxt_quality); flags: bit0 perceptual, bit1 use both block types
+ * (3-colour DXT1 blocks for DXT1 only, 6-value alpha blocks), bit2 grayscale
+ * sampling, bit3 DXT1A for transparency (DXT1: blocks with a texel of alpha
+ * below the threshold in bits 8..15 -- crn_comp_params::m_dxt1a_alpha_threshold,
+ * 128 by default -- are 3-colour blocks with those texels transparent;
+ * cCRNCompFlagDXT1AForTransparency), bit4 transparent indices for black
+ * (DXT1; cCRNCompFlagUseTransparentIndicesForBlack). */
 int crn2_encode_image_device(const void* rgba, uint32_t width, uint32_t height, uint32_t format, int quality,
                              int flags, void* blocks, void* stream);
 
@@ -215,7 +225,7 @@ int crn2_compress_blocks(const uint32_t* pixels, uint32_t nblocks, uint32_t form
  * minimal pitch, the element layout of crn2_encode_image_device); formats
  * DXT1, DXT5, DXT5A, DXN_XY, DXN_YX.  All colour elements of the set run in
  * one launch, all alpha elements in a second, each thread taking elements
- * with a grid stride.  quality 3 or 4, flags as crn2_encode_image_device.  With
+ * with a grid stride.  quality 0..4, flags as crn2_encode_image_device.  With
  * the autotuner on (crn2_autotune_mode(1)), the first run times six launch
  * shapes (64 / 128 / 256 threads per workgroup, one thread per element or a
  * persistent grid of 8 x 256 threads per CU) on the first 131,072 elements of

@@ -8,13 +8,13 @@
  * members are its CPU working storage), so any number may share a GPU.
  *
  * compute() covers:
- *   dxt1: 16 texels at better / uber quality (the per-block kernel: both
- *         block types, perceptual or uniform, grayscale sampling, forced
- *         3-colour blocks), and any number of texels on the path dxt_hc uses
- *         (uber, no 3-colour blocks, no alpha, no grayscale: the cluster
- *         kernel).  Texels with alpha below the threshold
- *         (m_pixels_have_alpha), m_use_transparent_indices_for_black and
- *         lower qualities return false.
+ *   dxt1: 16 texels at any quality (the per-block kernel: both block
+ *         types, perceptual or uniform, grayscale sampling, forced 3-colour
+ *         blocks, DXT1A transparency with m_pixels_have_alpha and
+ *         m_dxt1a_alpha_threshold, m_use_transparent_indices_for_black), and
+ *         any number of texels on the path dxt_hc uses (uber, no 3-colour
+ *         blocks, no alpha, no grayscale: the cluster kernel).  Other
+ *         combinations return false.
  *   dxt5: 16 values, any quality.
  * m_endpoint_caching is ignored: results equal the reference's with caching
  * off.  compute_batch runs n independent calls in one launch.

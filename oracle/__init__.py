@@ -187,22 +187,34 @@ class Dxt1Result(C.Structure):
 
 
 DXT1_PERCEPTUAL, DXT1_ALPHA_BLOCKS, DXT1_GRAYSCALE, DXT1_FORCE_ALPHA = 1, 2, 4, 8
+# DXT1A: texels with alpha below the threshold (bits 8..15) are transparent
+DXT1_HAVE_ALPHA, DXT1_BLACK_AS_TRANSPARENT = 16, 32
 
 
-def dxt1_blocks(rgba, quality=4, flags=DXT1_PERCEPTUAL | DXT1_ALPHA_BLOCKS):
+def dxt1_threshold(t):
+    return (int(t) & 255) << 8
+
+
+def dxt1_blocks(rgba, quality=4, flags=DXT1_PERCEPTUAL | DXT1_ALPHA_BLOCKS, count=False):
     """dxt1_endpoint_optimizer results for blocks of 16 RGBA8 texels
     ([n,16,4] uint8): (selectors [n,16], error, low, high, alpha_block,
-    reordered, alternate_rounding) as numpy arrays."""
+    reordered, alternate_rounding) as numpy arrays; with count=True also the
+    number of colour distances the evaluators computed."""
     px = np.ascontiguousarray(rgba, np.uint8).reshape(-1, 16, 4)
     n = px.shape[0]
     sel = np.zeros((n, 16), np.uint8)
     res = (Dxt1Result * n)()
     L = lib()
-    L.orc_dxt1_blocks.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
-    assert L.orc_dxt1_blocks(_p(px), n, quality, flags, _p(sel), res)
-    f = lambda k, dt: np.array([getattr(r, k) for r in res], dt)
-    return (sel, f("error", np.uint64), f("low", np.uint16), f("high", np.uint16), f("alpha_block", np.uint8),
-            f("reordered", np.uint8), f("alternate_rounding", np.uint8))
+    nd = C.c_uint64()
+    L.orc_dxt1_blocks_count.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]
+    assert L.orc_dxt1_blocks_count(_p(px), n, quality, flags, _p(sel), res, C.byref(nd))
+    raw = np.frombuffer(res, np.uint8).reshape(n, C.sizeof(Dxt1Result))
+    f = lambda k, dt: raw[:, getattr(Dxt1Result, k).offset:getattr(Dxt1Result, k).offset + np.dtype(dt).itemsize] \
+        .copy().view(dt).reshape(n)
+    out = (sel, f("error", np.uint64), f("low", np.uint16), f("high", np.uint16), f("alpha_block", np.uint8),
+           f("reordered", np.uint8), f("alternate_rounding", np.uint8))
+    return out + (nd.value,) if count else out
 
 
 def dxt1_cluster(rgba, quality=4, flags=DXT1_PERCEPTUAL):

@@ -23,12 +23,21 @@
  *   optimize_endpoint_comps ...... :369-486
  *   try_combinatorial_encoding ... :1871-1997
  *   return_solution .............. :263-365
+ *   evaluate_solution_fast ....... :1594-1757 (qualities below Better:
+ *                                   selectors by projection onto the endpoint
+ *                                   axis, green weight 24)
+ *   optimize_endpoints per quality  :716-746 (probe tables, passes, spacing),
+ *                                   :905/:1030 (lattice and solid steps from
+ *                                   Normal up)
+ *   DXT1A transparency ........... :2116 (pixels below the alpha threshold
+ *                                   leave the colour set), :1380/:1604 (3-colour
+ *                                   blocks only), :300-343 (selector 3 for them)
+ *   try_alpha_as_black_optimization :2001-2079, called from compute :2234-2246
  * Not restated: endpoint caching (parity is defined with
- * cCRNCompFlagDisableEndpointCaching), the "fast" evaluator used below
- * cCRNDXTQualityBetter, try_alpha_as_black_optimization and DXT1A alpha
- * thresholds (pixels_have_alpha); the solution sort of optimize_endpoints
+ * cCRNCompFlagDisableEndpointCaching); the solution sort of optimize_endpoints
  * (:798-816) only orders the early exits of the hc evaluators, whose result
- * it cannot change, so it is left out.
+ * it cannot change, so it is left out.  A block with no opaque texel leaves
+ * the reference's m_error unassigned (:2207-2212); here it is 0.
  *
  * Parity pins (tests/test_dxt1_oracle.py): single-colour blocks reach the
  * optimum of the exhaustive single-colour table search; reported error equals
@@ -58,6 +67,7 @@ typedef struct {
 
 typedef struct {
     int quality, perceptual_req, use_alpha_blocks, grayscale, force_alpha_blocks;
+    int have_alpha, alpha_thresh;
     /* state */
     int evaluate_hc, perceptual;
     ucolor* uc;
@@ -71,6 +81,7 @@ typedef struct {
     /* hash sets: unique colours, tried solutions */
     uint32_t* ckeys; uint32_t* cvals; uint32_t cmask;
     uint32_t* skeys; uint8_t* sused; uint32_t smask, scount;
+    uint64_t ndist;   /* colour distances evaluated (work count) */
 } e1;
 
 /* ---------------------------------------------------------------- tables */
@@ -221,7 +232,7 @@ static int eval_uber(e1* s, coords c, int alt)
     t->err = s->best.err;
     t->alpha_block = 0;
     int first = 0, last = 1;
-    if (s->force_alpha_blocks) first = 1;
+    if (s->have_alpha || s->force_alpha_blocks) first = 1;
     else if (!s->use_alpha_blocks) last = 0;
     c4 col[4];
     col[0] = unpack565(c.lo, 1);
@@ -245,6 +256,7 @@ static int eval_uber(e1* s, coords c, int alt)
         }
         for (int i = (int)s->nu - 1; i >= 0; i--) {
             const c4 px = s->uc[i].c;
+            s->ndist += (uint64_t)ncol;
             uint32_t be = cdist(s, s->perceptual, px, col[0]), bi = 0;
             for (int k = 1; k < ncol; k++) {
                 uint32_t e = cdist(s, s->perceptual, px, col[k]);
@@ -287,6 +299,7 @@ static int eval_hc(e1* s, coords c, int alt)
     uint64_t err = 0;
     for (uint32_t i = 0; i < s->nu; i++) {
         const c4 px = s->uc[i].c;
+        s->ndist += 4;
         uint32_t e0 = cdist(s, s->perceptual, px, c0), e1v = cdist(s, s->perceptual, px, c1);
         uint32_t e2 = cdist(s, s->perceptual, px, c2), e3 = cdist(s, s->perceptual, px, c3);
         uint32_t e01 = e0 < e1v ? e0 : e1v, e23 = e2 < e3 ? e2 : e3;
@@ -304,6 +317,95 @@ static int eval_hc(e1* s, coords c, int alt)
         else { s->best.c.hi--; s->best.enforced = 0; }
     }
     return 1;
+}
+
+/* evaluate_solution_fast (crn_dxt1.cpp:1594-1757): each colour takes the
+   palette entry its projection onto the endpoint axis falls nearest to
+   (green weighted 24 under the perceptual metric), then one distance. */
+static int eval_fast(e1* s, coords c, int alt)
+{
+    solution* t = &s->trial;
+    t->c = c;
+    t->nsel = (int)s->nu;
+    t->err = s->best.err;
+    t->alpha_block = 0;
+    int first = 0, last = 1;
+    if (s->have_alpha || s->force_alpha_blocks) first = 1;
+    else if (!s->use_alpha_blocks) last = 0;
+    c4 col[4];
+    col[0] = unpack565(c.lo, 1);
+    col[1] = unpack565(c.hi, 1);
+    int vr = col[1].r - col[0].r, vg = col[1].g - col[0].g, vb = col[1].b - col[0].b;
+    if (s->perceptual) {
+        vr *= 8;
+        vg *= 24;
+    }
+    int stops[4];
+    stops[0] = col[0].r * vr + col[0].g * vg + col[0].b * vb;
+    stops[1] = col[1].r * vr + col[1].g * vg + col[1].b * vb;
+    const int dr = vr * 2, dg = vg * 2, db = vb * 2;
+    for (int type = first; type <= last; type++) {
+        uint64_t terr = 0;
+        if (!type) {
+            col[2].r = (uint8_t)((col[0].r * 2 + col[1].r + alt) / 3);
+            col[2].g = (uint8_t)((col[0].g * 2 + col[1].g + alt) / 3);
+            col[2].b = (uint8_t)((col[0].b * 2 + col[1].b + alt) / 3);
+            col[3].r = (uint8_t)((col[1].r * 2 + col[0].r + alt) / 3);
+            col[3].g = (uint8_t)((col[1].g * 2 + col[0].g + alt) / 3);
+            col[3].b = (uint8_t)((col[1].b * 2 + col[0].b + alt) / 3);
+            stops[2] = col[2].r * vr + col[2].g * vg + col[2].b * vb;
+            stops[3] = col[3].r * vr + col[3].g * vg + col[3].b * vb;
+            const int p31 = stops[1] + stops[3], p32 = stops[3] + stops[2], p20 = stops[2] + stops[0];
+            for (int i = (int)s->nu - 1; i >= 0; i--) {
+                const c4 px = s->uc[i].c;
+                const int dot = px.r * dr + px.g * dg + px.b * db;
+                uint8_t bi;
+                if (dot >= p32) bi = dot < p31 ? 3 : 1;
+                else bi = dot < p20 ? 0 : 2;
+                s->ndist++;
+                terr += (uint64_t)cdist(s, s->perceptual, px, col[bi]) * s->uc[i].w;
+                if (terr >= t->err) break;
+                s->trial_sel[i] = bi;
+            }
+        } else {
+            col[2].r = (uint8_t)((col[0].r + col[1].r + alt) >> 1);
+            col[2].g = (uint8_t)((col[0].g + col[1].g + alt) >> 1);
+            col[2].b = (uint8_t)((col[0].b + col[1].b + alt) >> 1);
+            stops[2] = col[2].r * vr + col[2].g * vg + col[2].b * vb;
+            const int p02 = stops[0] + stops[2], p21 = stops[2] + stops[1];
+            for (int i = (int)s->nu - 1; i >= 0; i--) {
+                const c4 px = s->uc[i].c;
+                const int dot = px.r * dr + px.g * dg + px.b * db;
+                const uint8_t bi = dot < p02 ? 0 : dot < p21 ? 2 : 1;
+                s->ndist++;
+                terr += (uint64_t)cdist(s, s->perceptual, px, col[bi]) * s->uc[i].w;
+                if (terr >= t->err) break;
+                s->trial_sel[i] = bi;
+            }
+        }
+        if (terr < t->err) {
+            t->err = terr;
+            t->alpha_block = type != 0;
+            memcpy(t->sel, s->trial_sel, s->nu);
+        }
+    }
+    /* the fast evaluator leaves the trial's rounding flag as it was: no
+       alternate-rounding candidate is ever evaluated below Better */
+    t->alt_round = 0;
+    t->enforce = !t->alpha_block && t->c.lo == t->c.hi;
+    if (t->enforce) {
+        int sv;
+        if ((t->c.lo & 31) != 31) { t->c.lo++; sv = 1; }
+        else { t->c.hi--; sv = 0; }
+        memset(t->sel, sv, s->nu);
+        t->enforced = sv;
+    }
+    if (t->err < s->best.err) {
+        sol_copy(&s->best, t, s->nu);
+        s->best.nsel = (int)s->nu;
+        return 1;
+    }
+    return 0;
 }
 
 static void compute_selectors(e1* s)
@@ -339,7 +441,8 @@ static int evaluate(e1* s, coords c, int alt)
     if (re + ge + be >= s->best.err) return 0;
     if (!alt && !tried_insert(s, (uint32_t)c.lo | ((uint32_t)c.hi << 16))) return 0;
     if (s->evaluate_hc) return eval_hc(s, c, alt);
-    return eval_uber(s, c, alt);
+    if (s->quality >= 3) return eval_uber(s, c, alt);
+    return eval_fast(s, c, alt);
 }
 
 /* ------------------------------------------------------------- vectors */
@@ -823,6 +926,8 @@ static void try_combinatorial(e1* s)
 /* ------------------------------------------------------ optimize endpoints */
 static const int16_t k_uber_probe[15] = {0, 1, 2, 3, 5, 7, 9, 10, 13, 15, 19, 27, 43, 59, 91};
 static const int16_t k_better_probe[10] = {0, 1, 2, 3, 5, 9, 15, 19, 27, 43};
+static const int16_t k_normal_probe[5] = {0, 1, 3, 5, 7};
+static const int16_t k_fast_probe[4] = {0, 1, 2, 3};
 static const int8_t k_adj[26][3] = {
     {-1, -1, -1}, {0, -1, -1}, {1, -1, -1}, {-1, 0, -1}, {0, 0, -1}, {1, 0, -1}, {-1, 1, -1}, {0, 1, -1},
     {1, 1, -1}, {-1, -1, 0}, {0, -1, 0}, {1, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {-1, 1, 0}, {0, 1, 0},
@@ -896,6 +1001,16 @@ static void optimize_endpoints(e1* s, v3 lo, v3 hi)
         tab = k_better_probe;
         range = 10;
         passes = 2;
+    } else if (s->quality == 2) {
+        tab = k_normal_probe;
+        range = 5;
+        passes = 2;
+        dist = .027063293f;
+    } else if (s->quality < 2) {
+        tab = k_fast_probe;
+        range = 4;
+        passes = s->quality == 0 ? 1 : 2;
+        dist = .027063293f;
     }
     if (s->quality >= 3) try_median4(s, olo, ohi);
     v3 ax[2];
@@ -928,17 +1043,21 @@ static void optimize_endpoints(e1* s, v3 lo, v3 hi)
                 coords c = {plow[i], phigh[j]};
                 evaluate(s, canon(c), 0);
             }
-        lattice(s, 0);
-        lattice(s, 1);
+        if (s->quality >= 2) {
+            lattice(s, 0);
+            lattice(s, 1);
+        }
         if (!s->best.err || (pass && s->best.err == prev)) break;
         if (s->quality >= 4) refine(s, 1);
     }
-    if (s->best.err) {
-        int solid = 0;
-        if (sel_all_equal(&s->best)) solid = try_average_solid(s);
-        if (!solid && s->quality == 4) optimize_comps(s);
+    if (s->quality >= 2) {
+        if (s->best.err && !s->have_alpha) {
+            int solid = 0;
+            if (sel_all_equal(&s->best)) solid = try_average_solid(s);
+            if (!solid && s->quality == 4) optimize_comps(s);
+        }
+        if (s->quality == 4 && s->best.err) try_combinatorial(s);
     }
-    if (s->quality == 4 && s->best.err) try_combinatorial(s);
 }
 
 static void handle_multicolor(e1* s)
@@ -1036,12 +1155,10 @@ typedef struct {
     uint8_t alpha_block, reordered, alternate_rounding, enforce_selector, enforced_selector;
 } orc_dxt1_result;
 
-/* quality 3 (better) or 4 (uber); flags bit0 perceptual, bit1 use_alpha_blocks,
- * bit2 grayscale_sampling, bit3 force_alpha_blocks.  px: RGBA8 texels. */
-int orc_dxt1_compute(const uint8_t* px, uint32_t n, int quality, int flags, uint8_t* selectors, orc_dxt1_result* res)
+/* compute_internal (crn_dxt1.cpp:2081-2232) and return_solution */
+static void compute_internal(const uint8_t* px, uint32_t n, int quality, int flags, uint8_t* selectors,
+                             orc_dxt1_result* res, uint64_t* ndist)
 {
-    if (!px || !n || quality < 3 || quality > 4) return 0;
-    init_tables();
     e1 S;
     memset(&S, 0, sizeof(S));
     e1* s = &S;
@@ -1050,7 +1167,9 @@ int orc_dxt1_compute(const uint8_t* px, uint32_t n, int quality, int flags, uint
     s->use_alpha_blocks = (flags >> 1) & 1;
     s->grayscale = (flags >> 2) & 1;
     s->force_alpha_blocks = (flags >> 3) & 1;
-    s->evaluate_hc = quality == 4 && !s->force_alpha_blocks && !s->use_alpha_blocks && !s->grayscale;
+    s->have_alpha = (flags >> 4) & 1;
+    s->alpha_thresh = (flags >> 8) & 255;
+    s->evaluate_hc = quality == 4 && !s->have_alpha && !s->force_alpha_blocks && !s->use_alpha_blocks && !s->grayscale;
     s->perceptual = s->perceptual_req && !s->grayscale;
     uint32_t cap = 16;
     while (cap < 2 * n) cap <<= 1;
@@ -1069,6 +1188,7 @@ int orc_dxt1_compute(const uint8_t* px, uint32_t n, int quality, int flags, uint
     s->sused = (uint8_t*)calloc(scap, 1);
     sol_clear(&s->best);
     for (uint32_t i = 0; i < n; i++) {
+        if (s->have_alpha && px[4 * i + 3] < s->alpha_thresh) continue;
         const uint32_t key = ((uint32_t)px[4 * i] | ((uint32_t)px[4 * i + 1] << 8) | ((uint32_t)px[4 * i + 2] << 16)) | 0xFF000000u;
         int k = lookup_color(s, key);
         if (k < 0) {
@@ -1128,7 +1248,16 @@ int orc_dxt1_compute(const uint8_t* px, uint32_t n, int quality, int flags, uint
         s->glo[c] = gp[c][2] + C * C * gp[c][0] - 2 * C * gp[c][1];
         s->ghi[c] = gp[63][2] + C * C * gp[63][0] - 2 * C * gp[63][1] - s->glo[c];
     }
-    if (s->nu == 1) {
+    const int transparent = s->total_w != n;
+    if (!s->nu) {
+        memset(res, 0, sizeof(*res));
+        res->alpha_block = 1;
+        memset(selectors, 3, n);
+        free(s->ckeys); free(s->cvals); free(s->uc); free(s->norm); free(s->normw);
+        free(s->best.sel); free(s->trial.sel); free(s->trial_sel); free(s->skeys); free(s->sused);
+        return;
+    }
+    if (s->nu == 1 && !transparent) {
         const uint32_t r = s->uc[0].c.r, g = s->uc[0].c.g, b = s->uc[0].c.b;
         coords c = {(uint16_t)((om5[r][0] << 11) | (om6[g][0] << 5) | om5[b][0]),
                     (uint16_t)((om5[r][1] << 11) | (om6[g][1] << 5) | om5[b][1])};
@@ -1154,14 +1283,98 @@ int orc_dxt1_compute(const uint8_t* px, uint32_t n, int quality, int flags, uint
     res->high = inv ? s->best.c.lo : s->best.c.hi;
     const uint8_t* tab = inv ? (s->best.alpha_block ? k_inv_alpha : k_inv_color) : k_inv_null;
     for (uint32_t i = 0; i < n; i++) {
+        if (s->have_alpha && px[4 * i + 3] < s->alpha_thresh) {
+            selectors[i] = 3;
+            continue;
+        }
         const uint32_t key = ((uint32_t)px[4 * i] | ((uint32_t)px[4 * i + 1] << 8) | ((uint32_t)px[4 * i + 2] << 16)) | 0xFF000000u;
         selectors[i] = tab[s->best.sel[lookup_color(s, key)]];
     }
     res->alpha_block = (uint8_t)s->best.alpha_block;
     res->error = s->best.err;
+    if (ndist) *ndist += s->ndist;
     free(s->ckeys); free(s->cvals); free(s->uc); free(s->norm); free(s->normw);
     free(s->best.sel); free(s->trial.sel); free(s->trial_sel); free(s->skeys); free(s->sused);
+}
+
+static uint32_t metric(int perceptual, int grayscale, const uint8_t* a, c4 b)
+{
+    const int dr = (int)a[0] - (int)b.r, dg = (int)a[1] - (int)b.g, db = (int)a[2] - (int)b.b;
+    if (perceptual) return (uint32_t)(dr * dr * 8 + dg * dg * 25 + db * db);
+    if (grayscale) {
+        const int y0 = ((int)a[0] * 19595 + (int)a[1] * 38470 + (int)a[2] * 7471 + 32768) >> 16;
+        const int y1 = ((int)b.r * 19595 + (int)b.g * 38470 + (int)b.b * 7471 + 32768) >> 16;
+        return (uint32_t)((y0 - y1) * (y0 - y1));
+    }
+    return (uint32_t)(dr * dr + dg * dg + db * db);
+}
+
+/* try_alpha_as_black_optimization (crn_dxt1.cpp:2001-2079): texels whose
+   r, g and b are all <= 4 are re-encoded as DXT1A transparent texels (palette
+   entry 3, black) when the block has some but not only such colours; the
+   3-colour result replaces the first one if its error, now counting those
+   texels against black, is lower. */
+static void alpha_as_black(const uint8_t* px, uint32_t n, int quality, int flags, uint8_t* selectors,
+                           orc_dxt1_result* res, uint64_t* ndist)
+{
+    uint32_t nu = 0, ndark = 0;
+    uint32_t* seen = (uint32_t*)malloc(4 * (size_t)n);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t key = (uint32_t)px[4 * i] | ((uint32_t)px[4 * i + 1] << 8) | ((uint32_t)px[4 * i + 2] << 16);
+        uint32_t k = 0;
+        while (k < nu && seen[k] != key) k++;
+        if (k < nu) continue;
+        seen[nu++] = key;
+        if (px[4 * i] <= 4 && px[4 * i + 1] <= 4 && px[4 * i + 2] <= 4) ndark++;
+    }
+    free(seen);
+    if (!ndark || ndark == nu) return;
+    uint8_t* tp = (uint8_t*)malloc(4 * (size_t)n);
+    uint8_t* tsel = (uint8_t*)malloc(n);
+    memcpy(tp, px, 4 * (size_t)n);
+    for (uint32_t i = 0; i < n; i++)
+        if (tp[4 * i] <= 4 && tp[4 * i + 1] <= 4 && tp[4 * i + 2] <= 4) tp[4 * i + 3] = 0;
+    orc_dxt1_result tr;
+    compute_internal(tp, n, quality, flags | 16, tsel, &tr, ndist);
+    c4 pal[4];
+    pal[0] = unpack565(tr.low, 1);
+    pal[1] = unpack565(tr.high, 1);
+    pal[2].r = (uint8_t)((pal[0].r + pal[1].r) >> 1);
+    pal[2].g = (uint8_t)((pal[0].g + pal[1].g) >> 1);
+    pal[2].b = (uint8_t)((pal[0].b + pal[1].b) >> 1);
+    pal[3].r = pal[3].g = pal[3].b = 0;
+    const int gray = (flags >> 2) & 1, perceptual = (flags & 1) && !gray;
+    uint64_t err = 0;
+    for (uint32_t i = 0; i < n; i++) err += metric(perceptual, gray, tp + 4 * i, pal[tsel[i]]);
+    if (err < res->error) {
+        res->error = err;
+        res->low = tr.low;
+        res->high = tr.high;
+        memcpy(selectors, tsel, n);
+        res->alpha_block = 1;
+    }
+    free(tp);
+    free(tsel);
+}
+
+/* quality 0 (superfast) .. 4 (uber); flags bit0 perceptual, bit1
+ * use_alpha_blocks, bit2 grayscale_sampling, bit3 force_alpha_blocks, bit4
+ * pixels_have_alpha (DXT1A: texels with alpha below bits 8..15 are
+ * transparent), bit5 use_transparent_indices_for_black.  px: RGBA8 texels. */
+int orc_dxt1_compute_count(const uint8_t* px, uint32_t n, int quality, int flags, uint8_t* selectors,
+                           orc_dxt1_result* res, uint64_t* ndist)
+{
+    if (!px || !n || quality < 0 || quality > 4 || (flags & ~0xFF3F)) return 0;
+    init_tables();
+    compute_internal(px, n, quality, flags, selectors, res, ndist);
+    if (((flags >> 1) & 1) && ((flags >> 5) & 1) && !((flags >> 4) & 1))
+        alpha_as_black(px, n, quality, flags, selectors, res, ndist);
     return 1;
+}
+
+int orc_dxt1_compute(const uint8_t* px, uint32_t n, int quality, int flags, uint8_t* selectors, orc_dxt1_result* res)
+{
+    return orc_dxt1_compute_count(px, n, quality, flags, selectors, res, 0);
 }
 
 /* blocks of 16 RGBA8 texels; out_sel 16 bytes per block; out_res per block */
@@ -1170,6 +1383,19 @@ int orc_dxt1_blocks(const uint8_t* px, uint32_t nblocks, int quality, int flags,
 {
     for (uint32_t b = 0; b < nblocks; b++)
         if (!orc_dxt1_compute(px + 64 * (size_t)b, 16, quality, flags, out_sel + 16 * (size_t)b, &out_res[b]))
+            return 0;
+    return 1;
+}
+
+/* the same, also returning the number of colour distances the evaluators
+   computed (early exits included): the encode's algorithmic work count */
+int orc_dxt1_blocks_count(const uint8_t* px, uint32_t nblocks, int quality, int flags, uint8_t* out_sel,
+                          orc_dxt1_result* out_res, uint64_t* ndist)
+{
+    *ndist = 0;
+    for (uint32_t b = 0; b < nblocks; b++)
+        if (!orc_dxt1_compute_count(px + 64 * (size_t)b, 16, quality, flags, out_sel + 16 * (size_t)b, &out_res[b],
+                                    ndist))
             return 0;
     return 1;
 }

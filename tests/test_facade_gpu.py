@@ -227,8 +227,34 @@ def test_dxt1_optimizer_class(gpu):
     osel, ores = oracle.dxt1_cluster(px, 4, oracle.DXT1_PERCEPTUAL)
     assert (r.error, r.low, r.high) == (ores.error, ores.low, ores.high)
     assert np.array_equal(sel, osel)
-    # DXT1A transparency is not built: refused, not approximated
-    p = D1Params(0, blocks[0].ctypes.data, 16, 128, 4, True, True, True, False, True, False, False)
+    # DXT1A transparency at normal quality, and the transparent-black trial
+    ab = blocks.copy()
+    ab[::3, ::4, 3] = 40
+    ref_a = oracle.dxt1_blocks(ab, 2, oracle.DXT1_PERCEPTUAL | oracle.DXT1_ALPHA_BLOCKS | oracle.DXT1_HAVE_ALPHA |
+                               oracle.dxt1_threshold(128))
+    for i in range(0, len(ab), 7):
+        p = D1Params(0, ab[i].ctypes.data, 16, 128, 2, True, True, True, False, True, False, False)
+        sel = np.zeros(16, np.uint8)
+        r = D1Results()
+        r.selectors = sel.ctypes.data
+        assert compute(obj, C.byref(p), C.byref(r))
+        assert (r.error, r.low, r.high, r.alpha_block) == (ref_a[1][i], ref_a[2][i], ref_a[3][i], ref_a[4][i]), i
+        assert np.array_equal(sel, ref_a[0][i])
+    # 1-D lattice of dark texels: the transparent-black trial
+    bb = blocks.copy()
+    bb[:, ::2, :3] = 2
+    ref_b = oracle.dxt1_blocks(bb, 4, oracle.DXT1_PERCEPTUAL | oracle.DXT1_ALPHA_BLOCKS |
+                               oracle.DXT1_BLACK_AS_TRANSPARENT | oracle.dxt1_threshold(128))
+    for i in range(0, len(bb), 9):
+        p = D1Params(0, bb[i].ctypes.data, 16, 128, 4, False, True, True, False, True, True, False)
+        sel = np.zeros(16, np.uint8)
+        r = D1Results()
+        r.selectors = sel.ctypes.data
+        assert compute(obj, C.byref(p), C.byref(r))
+        assert (r.error, r.low, r.high, r.alpha_block) == (ref_b[1][i], ref_b[2][i], ref_b[3][i], ref_b[4][i]), i
+        assert np.array_equal(sel, ref_b[0][i])
+    # a cluster of more than 16 texels with alpha: refused, not approximated
+    p = D1Params(0, px.ctypes.data, len(px), 128, 4, True, True, True, False, True, False, False)
     assert not compute(obj, C.byref(p), C.byref(D1Results()))
 
 
