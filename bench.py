@@ -1,11 +1,14 @@
 """Benchmark: CRN->DXTn transcode of a 4096x4096 RGBA8 texture with its full
-13-level mip chain, in DXT1 and DXT5 (one step = both files, all levels),
-inputs resident in HBM, on one MI355X.
+13-level mip chain, in DXT1 and DXT5 (one step = both files, all levels, in
+128 independent copies), inputs resident in HBM, on one MI355X.  Beside it,
+the encode workloads: plain DXT1 of a 256^2 texture, the learned vs plain
+DXT1 encode of an 8192^2 texture, dxt_hc of a 2048^2 texture and the
+512-texture serving set (encode and transcode).
 
-Output: one JSON line (see the keys below).  The CPU baseline is the
-project's C restatement of the transcoder (oracle/) timed on this host.
+Output: one JSON line (see the keys below).  The CPU baselines are the
+project's C restatements (oracle/) timed on this host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
 import ctypes as C
@@ -18,10 +21,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes one copy of the 4096^2 DXT1 + DXT5 pair moves through the
-# pipeline (per-kernel FETCH_SIZE / WRITE_SIZE, scaled by a device copy of
-# known size): 1.035 GB, 26.6x the algorithmic bytes, mostly the per-bit tables.
-TRAFFIC_PER_COPY = 1.035e9
+# HBM bytes the walk kernel moves per copy of the 4096^2 DXT1 + DXT5 pair
+# (its FETCH_SIZE + WRITE_SIZE, scaled by a device copy of known size), for
+# the default inputs (GPU dxt_hc + the project's writer); None until measured.
+WALK_TRAFFIC_PER_COPY = None
 STAGES = ("crnd_tables_kernel", "crnd_walk_kernel", "crnd_sym_kernel<even>", "crnd_sym_kernel<odd>",
           "crnd_index_{chunk,carry}_kernel", "crnd_index_final_kernel (+ block write)")
 
@@ -52,51 +55,164 @@ def psnr(sse, npix):
     return 99.0 if mse == 0 else 20.0 * math.log10(255.0 / math.sqrt(mse))
 
 
-def encode_leg(size, frac):
+VALU_PEAK_TOPS = 39.3      # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz (no packed int32 ops)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+MFMA_INT8_PEAK_TOPS = 5000.0
+OPS_PER_DISTANCE = 9      # 3 differences, 3 squares, 2 weight multiplies and adds folded, compare-select
+
+
+def gpu_ms(fn, reps):
+    """Average duration of fn's launches on torch's current stream (HIP events)."""
+    import torch
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def cpu_threads():
+    return min(16, os.cpu_count() or 1)
+
+
+def oracle_dxt1_rate(blocks, quality, flags, seconds):
+    """The oracle's DXT1 search on host threads (one slice of blocks per
+    thread, ctypes calls outside the GIL), repeated for about `seconds`."""
+    import numpy as np
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    th = cpu_threads()
+    parts = np.array_split(blocks, th)
+    done, t = 0, 0.0
+    with ThreadPoolExecutor(max_workers=th) as pool:
+        while t < seconds:
+            s0 = time.perf_counter()
+            list(pool.map(lambda b: oracle.dxt1_blocks(b, quality, flags), parts))
+            t += time.perf_counter() - s0
+            done += len(blocks)
+    return done * 16 / t / 1e6, th, t
+
+
+def plain_leg(size, cpu_seconds):
+    """Plain DXT1 block-by-block compression of one size x size RGB texture,
+    uber, perceptual, both block types: 8-byte blocks + per-block error from
+    one image launch; every block checked against the oracle; VALU roofline
+    from the oracle's count of colour distances."""
+    import numpy as np
+    import torch
+    import oracle
+    from crunch2_amd import dxt1, texture
+    from crunch2_amd._lib import lib
+    img = texture.synth(size, size, 1, texture.RGB)
+    blocks = dxt1.image_blocks(img)
+    n = len(blocks)
+    flags = dxt1.PERCEPTUAL | dxt1.ALPHA_BLOCKS
+    src = torch.from_numpy(np.ascontiguousarray(img)).cuda()
+    out = torch.empty(n, dtype=torch.int64, device="cuda")
+    err = torch.empty(n, dtype=torch.int64, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+
+    def run():
+        assert lib().crn2_encode_dxt1_image_device(src.data_ptr(), size, size, 4, flags, out.data_ptr(), 1,
+                                                   err.data_ptr(), C.c_void_p(st)) == 0
+    run()
+    torch.cuda.synchronize()
+    ms = gpu_ms(run, 20)
+    ref = oracle.dxt1_blocks(blocks, 4, flags, count=True)
+    w = out.cpu().numpy().view(np.uint64)
+    rsel = (ref[0].astype(np.uint64) << (2 * np.arange(16, dtype=np.uint64))).sum(1)
+    exp = ref[2].astype(np.uint64) | (ref[3].astype(np.uint64) << 16) | (rsel << 32)
+    exact = float((w == exp).mean())
+    assert exact == 1.0, "plain DXT1 differs from the oracle"
+    assert np.array_equal(err.cpu().numpy().view(np.uint64), ref[1])
+    ops = ref[7] * OPS_PER_DISTANCE
+    cpu, th, ct = oracle_dxt1_rate(blocks, 4, flags, cpu_seconds)
+    achieved = ops / (ms * 1e-3) / 1e12
+    return {"workload": "plain DXT1 of one %dx%d RGB texture (%d blocks), uber, perceptual, both block types, "
+                        "8-byte blocks + per-block error" % (size, size, n),
+            "mtexels_s": round(n * 16 / ms * 1e-3, 2), "ms": round(ms, 4), "bit_exact_rate": exact,
+            "roofline": {"bound": "valu", "kernel": "dxt1_image_kernel", "achieved": round(achieved, 4),
+                         "peak": VALU_PEAK_TOPS, "unit": "Tops/s (int32 lane-ops)",
+                         "frac": round(achieved / VALU_PEAK_TOPS, 6), "traffic": None,
+                         "ops_per_launch": int(ops), "colour_distances": int(ref[7])},
+            "cpu_baseline": {"value": round(cpu, 3), "unit": "Mtexels/s", "cores": th, "kind": "port",
+                             "sample": "the oracle's search over the same %d blocks on %d threads (%.1fs)"
+                                       % (n, th, ct)}}
+
+
+def encode_leg(size, frac, cpu_seconds):
     """DXT1 encode of one size x size RGB texture: the plain GPU search against
     the learned proposal (+ refinement, + the plain search for the `frac` of
     blocks with the lowest confidence)."""
     import numpy as np
     import torch
+    import oracle
     from crunch2_amd import dxt1, learned, texture
     from crunch2_amd.train_learned import DEFAULT_OUT
     if not os.path.exists(DEFAULT_OUT):
         return {"skipped": "no trained parameters at " + DEFAULT_OUT}
     theta = torch.from_numpy(np.load(DEFAULT_OUT)).cuda()
     img = texture.synth(size, size, 4242, texture.RGB)
-    blocks = torch.from_numpy(dxt1.image_blocks(img)).cuda()
+    nb = dxt1.image_blocks(img)
+    blocks = torch.from_numpy(nb).cuda()
     n = blocks.shape[0]
     texels = n * 16
     # warm both paths on a slice
     dxt1.encode_blocks_device(blocks[:4096].contiguous(), dxt1.UBER, learned.LABEL_FLAGS)
     learned.encode(theta, blocks[:4096].contiguous(), fallback_frac=frac)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    raw = dxt1.encode_blocks_device(blocks, dxt1.UBER, learned.LABEL_FLAGS)
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
+    raw = torch.empty((n, 24), dtype=torch.uint8, device="cuda")
+    t_plain = gpu_ms(lambda: dxt1.encode_blocks_device(blocks, dxt1.UBER, learned.LABEL_FLAGS, out=raw), 1) * 1e-3
     pend = raw.view(torch.int32).reshape(-1, 6)[:, 2].contiguous()
     perr, psel = learned.block_errors(blocks, pend)
     t0 = time.perf_counter()
     ends, sels, errs, nfb = learned.encode(theta, blocks, fallback_frac=frac)
     torch.cuda.synchronize()
     t_learn = time.perf_counter() - t0
+    mlp_ms = gpu_ms(lambda: learned.infer(theta, blocks), 3)
     sse_p = int(decode_sse(blocks, pend, psel).sum())
     sse_l = int(decode_sse(blocks, ends, sels).sum())
     ps, pl = psnr(sse_p, texels), psnr(sse_l, texels)
+    # algorithmic work of the plain search from the oracle on a sample of the blocks
+    idx = np.random.default_rng(0).choice(n, 4096, replace=False)
+    smp = nb[idx]
+    ref = oracle.dxt1_blocks(smp, 4, oracle.DXT1_PERCEPTUAL, count=True)
+    got = raw[torch.from_numpy(idx).cuda()].cpu().numpy().view(np.uint32).reshape(-1, 6)
+    assert np.array_equal(got[:, 2] & 0xFFFF, ref[2]) and np.array_equal(got[:, 2] >> 16, ref[3]), \
+        "plain search differs from the oracle on the sample"
+    ops = ref[7] / 4096 * n * OPS_PER_DISTANCE
+    valu = ops / t_plain / 1e12
+    mlp_flops = 2.0 * n * (57 * 128 + 128 * 128 + 128 * 7)
+    mlp_tf = mlp_flops / (mlp_ms * 1e-3) / 1e12
+    cpu, th, ct = oracle_dxt1_rate(smp, 4, oracle.DXT1_PERCEPTUAL, cpu_seconds)
     return {"workload": "DXT1 encode of one %dx%d RGB texture (%d blocks), uber, perceptual, four-colour blocks"
                         % (size, size, n),
             "plain_mtexels_s": round(texels / t_plain / 1e6, 1), "learned_mtexels_s": round(texels / t_learn / 1e6, 1),
             "speedup": round(t_plain / t_learn, 3), "psnr_plain_db": round(ps, 4), "psnr_learned_db": round(pl, 4),
             "delta_db": round(pl - ps, 4), "blocks_worse_than_plain": int((errs > perr).sum()),
-            "fallback_blocks": nfb, "fallback_frac": frac}
+            "fallback_blocks": nfb, "fallback_frac": frac,
+            "roofline": {"bound": "valu", "kernel": "dxt1_blocks_kernel (plain search)", "achieved": round(valu, 4),
+                         "peak": VALU_PEAK_TOPS, "unit": "Tops/s (int32 lane-ops)",
+                         "frac": round(valu / VALU_PEAK_TOPS, 6), "traffic": None,
+                         "ops_per_launch": ops,
+                         "ops_from": "the oracle's colour-distance count on 4,096 sampled blocks, scaled to %d" % n},
+            "mlp_roofline": {"bound": "mfma", "kernel": "mlp_infer_kernel (bf16 MFMA, fp32 accumulation)",
+                             "achieved": round(mlp_tf, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": round(mlp_tf / MFMA_BF16_PEAK_TFLOPS, 5), "ms": round(mlp_ms, 3),
+                             "flops_per_launch": mlp_flops},
+            "cpu_baseline": {"value": round(cpu, 3), "unit": "Mtexels/s", "cores": th, "kind": "port",
+                             "sample": "the oracle's plain search over 4,096 sampled blocks of the texture on %d "
+                                       "threads (%.1fs)" % (th, ct)}}
 
 
-def hc_leg(size, cpu_size):
-    """dxt_hc of one size x size RGB texture (SURVEY config 3: 4096 colour
-    endpoints, 8192 colour selectors, hierarchical, no mips) on the GPU, the
-    colour selector assignment on INT8 matrix cores timed alone, and the
-    oracle's single-thread composition on a cpu_size^2 sample."""
+def hc_leg(size, cpu):
+    """dxt_hc of one size x size RGB texture (4096 colour endpoints, 8192
+    colour selectors, hierarchical, no mips) on the GPU, the colour selector
+    assignment on INT8 matrix cores timed alone, and (cpu) the oracle's
+    single-thread composition of the same texture, whose output is also
+    compared with the GPU's."""
     import numpy as np
     import torch
     from crunch2_amd import dxt1, hc, texture
@@ -107,7 +223,7 @@ def hc_leg(size, cpu_size):
     sizes = (4096, 8192, 3072, 3072)
     tb = torch.from_numpy(blocks).cuda()
     hc.compress(tb, levels, codebook_sizes=sizes)       # warm-up
-    reps, t0 = 3, None
+    reps = 3
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -138,33 +254,135 @@ def hc_leg(size, cpu_size):
     cols[:, :, :3] = lin.to(torch.uint8)
     ei = out[0][:, 0].to(torch.int32).contiguous()
     hc.color_selectors(tb, ei, cols, used_sel)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(5):
-        hc.color_selectors(tb, ei, cols, used_sel)
-    ev1.record()
-    torch.cuda.synchronize()
-    sel_ms = ev0.elapsed_time(ev1) / 5
+    sel_ms = gpu_ms(lambda: hc.color_selectors(tb, ei, cols, used_sel), 5)
     ops = 2.0 * n * m * 64 * 4
+    tops = ops / (sel_ms * 1e-3) / 1e12
     line = {"workload": "dxt_hc of one %dx%d RGB texture (%d blocks), 4096 colour endpoints / 8192 colour "
                         "selectors, perceptual, no mips" % (size, size, n),
             "mtexels_s": round(n * 16 / t / 1e6, 2), "ms": round(t * 1e3, 2),
             "color_endpoints": int(ce.shape[0]), "color_selectors": int(m),
             "psnr_db": round(psnr(sse, n * 16), 3),
-            "selector_assignment": {"ms": round(sel_ms, 3), "int8_tops": round(ops / (sel_ms * 1e-3) / 1e12, 1),
-                                    "peak_tops": 5000.0, "frac": round(ops / (sel_ms * 1e-3) / 5.0e15, 4),
-                                    "ops": ops}}
-    if cpu_size:
+            "roofline": {"bound": "mfma", "kernel": "colour selector assignment (v_mfma_i32_32x32x32_i8, blocks x "
+                                                    "codewords x 64, exact int32)",
+                         "achieved": round(tops, 1), "peak": MFMA_INT8_PEAK_TOPS, "unit": "TOP/s",
+                         "frac": round(tops / MFMA_INT8_PEAK_TOPS, 4), "ms": round(sel_ms, 3), "ops": ops,
+                         "traffic": None}}
+    if cpu:
         import oracle
-        cimg = texture.synth(cpu_size, cpu_size, 11, texture.RGB)
-        cb = dxt1.image_blocks(cimg)
         t0 = time.perf_counter()
-        oracle.hc_compress(cb, [(0, cb.shape[0], cpu_size // 4, 1.0)], codebook_sizes=sizes)
+        ref = oracle.hc_compress(blocks, levels, codebook_sizes=sizes)
         ct = time.perf_counter() - t0
-        line["cpu_port"] = {"mtexels_s": round(cb.shape[0] * 16 / ct / 1e6, 3), "cores": 1,
-                            "sample": "the oracle's dxt_hc composition on a %dx%d texture of the same generator "
-                                      "(%.1fs)" % (cpu_size, cpu_size, ct)}
+        same = (np.array_equal(out[0].cpu().numpy().view(np.uint16), ref[0]) and
+                np.array_equal(out[1].cpu().numpy().view(np.uint16), ref[1]) and
+                np.array_equal(out[2].cpu().numpy().view(np.uint32), ref[2]) and
+                np.array_equal(out[3].cpu().numpy().view(np.uint32), ref[3]))
+        assert same, "dxt_hc differs from the oracle"
+        line["bit_exact"] = same
+        line["cpu_baseline"] = {"value": round(n * 16 / ct / 1e6, 3), "unit": "Mtexels/s", "cores": 1,
+                                "kind": "port",
+                                "sample": "the oracle's dxt_hc composition of the same %dx%d texture on one thread "
+                                          "(%.1fs); its output equals the GPU's" % (size, size, ct)}
     return line
+
+
+def mixed_leg(count, cpu_seconds):
+    """The serving-style set: `count` textures, sides log-uniform in [64, 2048]
+    with full mip chains, 60 % RGB -> DXT1, 25 % RGBA -> DXT5, 15 % normal
+    maps -> DXN_XY.  Encode: every level image of the set in one graph replay
+    of the texture-set encoder (uber, perceptual, both block types).
+    Transcode: the set's .crn files (simple-quantizer writer, to keep the leg
+    short) in one batch plan."""
+    import numpy as np
+    import torch
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    from crunch2_amd import crnd, texture
+    from crunch2_amd._lib import FMT_DXN_XY, FMT_DXT1, FMT_DXT5, EncodeItem, lib
+    rng = np.random.default_rng(2024)
+    jobs = []
+    for i in range(count):
+        side = int(round(2 ** rng.uniform(6, 11)))
+        u = rng.random()
+        jobs.append((FMT_DXT1 if u < 0.60 else (FMT_DXT5 if u < 0.85 else FMT_DXN_XY), side, 5000 + i))
+    kinds = {FMT_DXT1: texture.RGB, FMT_DXT5: texture.RGBA, FMT_DXN_XY: texture.NORMAL}
+    t0 = time.time()
+    with ThreadPoolExecutor(max_workers=cpu_threads()) as pool:
+        chains = list(pool.map(lambda j: texture.mip_chain(texture.synth(j[1], j[1], j[2], kinds[j[0]])), jobs))
+    log("mixed set: %d mip chains in %.1fs" % (len(chains), time.time() - t0))
+    items, keep, texels, out_bytes = [], [], 0, 0
+    for (fmt, side, _), chain in zip(jobs, chains):
+        for lv in chain:
+            h, w = lv.shape[:2]
+            src = torch.from_numpy(np.ascontiguousarray(lv).reshape(-1)).cuda()
+            nbytes = ((w + 3) // 4) * ((h + 3) // 4) * (8 if fmt == FMT_DXT1 else 16)
+            dst = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+            keep.append((src, dst))
+            items.append(EncodeItem(src.data_ptr(), w, h, fmt, dst.data_ptr()))
+            texels += ((w + 3) // 4) * ((h + 3) // 4) * 16
+            out_bytes += nbytes
+    arr = (EncodeItem * len(items))(*items)
+    es = C.c_void_p()
+    assert lib().crn2_encode_set_create(arr, len(items), 4, 1 | 2, C.byref(es)) == 0
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert lib().crn2_encode_set_run_graph(es, st) == 0      # capture + first replay
+    torch.cuda.synchronize()
+    enc_ms = gpu_ms(lambda: lib().crn2_encode_set_run_graph(es, st), 2)
+    assert lib().crn2_encode_set_destroy(es) == 0
+    # transcode of the set's files
+    t0 = time.time()
+    with ThreadPoolExecutor(max_workers=cpu_threads()) as pool:
+        files = list(pool.map(lambda j: texture.synth_crn((j[1], j[1]), j[2], j[0]), jobs))
+    log("mixed set: %d .crn files in %.1fs" % (len(files), time.time() - t0))
+    ctxs, outs = [], []
+    tex_t, crn_bytes, dxt_bytes = 0, 0, 0
+    for d in files:
+        ctx = crnd.unpack_begin(d)
+        assert ctx is not None
+        inf = crnd.get_texture_info(d)
+        nb = crnd.levels_size(ctx, 0, inf["levels"])
+        outs.append(torch.empty(nb, dtype=torch.uint8, device="cuda"))
+        ctxs.append(ctx)
+        crn_bytes += len(d)
+        dxt_bytes += nb
+        for l in range(inf["levels"]):
+            li = crnd.get_level_info(d, l)
+            tex_t += li["blocks_x"] * li["blocks_y"] * 16
+    k = len(ctxs)
+    plan = C.c_void_p()
+    assert lib().crn2_batch_create((C.c_void_p * k)(*[c.handle for c in ctxs]), k,
+                                   (C.c_void_p * k)(*[o.data_ptr() for o in outs]),
+                                   (C.c_uint64 * k)(*[o.numel() for o in outs]), C.byref(plan)) == 0
+    assert lib().crn2_batch_run(plan, st) == 0
+    torch.cuda.synchronize()
+    for i in (0, k // 2, k - 1):
+        ref = oracle.unpack_all(files[i])
+        exp = np.concatenate([ref[l][0].reshape(-1) for l in sorted(ref)])
+        assert np.array_equal(outs[i].cpu().numpy(), exp), "mixed-set transcode differs from the oracle"
+    tr_ms = gpu_ms(lambda: lib().crn2_batch_run(plan, st), 3)
+    assert lib().crn2_batch_status(plan, st) == 0
+    lib().crn2_batch_destroy(plan)
+    for c in ctxs:
+        crnd.unpack_end(c)
+    # CPU baselines: the oracle's DXT1 search on a sample of the set's DXT1 blocks
+    from crunch2_amd import dxt1
+    dx = [c[0] for j, c in zip(jobs, chains) if j[0] == FMT_DXT1][:8]
+    smp = np.concatenate([dxt1.image_blocks(x) for x in dx])[:4096]
+    cpu_enc, th, ct = oracle_dxt1_rate(smp, 4, oracle.DXT1_PERCEPTUAL | oracle.DXT1_ALPHA_BLOCKS, cpu_seconds)
+    gbs = (crn_bytes + dxt_bytes) / (tr_ms * 1e-3) / 1e9
+    return {"workload": "serving-style set of %d textures (sides 64..2048 log-uniform, full mips; %d DXT1 / %d DXT5 / "
+                        "%d DXN_XY), %d level images" % (count, sum(j[0] == FMT_DXT1 for j in jobs),
+                                                         sum(j[0] == FMT_DXT5 for j in jobs),
+                                                         sum(j[0] == FMT_DXN_XY for j in jobs), len(items)),
+            "encode": {"mtexels_s": round(texels / (enc_ms * 1e-3) / 1e6, 1), "ms": round(enc_ms, 2),
+                       "texels": texels, "dxtn_bytes": out_bytes,
+                       "how": "one HIP graph replay of the texture-set encoder (uber, perceptual, both block types)",
+                       "cpu_baseline": {"value": round(cpu_enc, 3), "unit": "Mtexels/s", "cores": th, "kind": "port",
+                                        "sample": "the oracle's DXT1 search on 4,096 blocks of the set's largest "
+                                                  "DXT1 images on %d threads (%.1fs)" % (th, ct)}},
+            "transcode": {"mtexels_s": round(tex_t / (tr_ms * 1e-3) / 1e6, 1), "ms": round(tr_ms, 3),
+                          "gb_s": round(gbs, 2), "hbm_frac": round(gbs / HBM_PEAK_GBS, 5),
+                          "crn_bytes": crn_bytes, "dxtn_bytes": dxt_bytes,
+                          "inputs": "the simple block quantizer's .crn files at quality-128 palette sizes"}}
 
 
 def main():
@@ -177,8 +395,12 @@ def main():
                     help="independent copies of every file per step (separate contexts and outputs), so that one "
                          "launch sequence moves hundreds of MB")
     ap.add_argument("--mixed", type=int, default=0,
-                    help="instead of the 4096^2 pair, a mixed set of this many textures (sides log-uniform in "
-                         "[64, 2048], full mips; 60%% DXT1 RGB, 25%% DXT5 RGBA, 15%% DXN normal maps)")
+                    help="instead of the 4096^2 pair, transcode a mixed set of this many textures (sides "
+                         "log-uniform in [64, 2048], full mips; 60%% DXT1 RGB, 25%% DXT5 RGBA, 15%% DXN normal maps)")
+    ap.add_argument("--mixed-set", type=int, default=512,
+                    help="textures of the serving-set leg (encode + transcode); 0 skips it")
+    ap.add_argument("--plain-size", type=int, default=256,
+                    help="side of the RGB texture of the plain DXT1 leg; 0 skips it")
     ap.add_argument("--streams", type=int, default=4, help="plans (groups of copies) per step, one stream each")
     ap.add_argument("--writer", choices=("hc", "simple"), default="hc",
                     help="how the input .crn files are made: the GPU dxt_hc with the project's CRN writer, or the "
@@ -189,8 +411,9 @@ def main():
                     help="side of the RGB texture of the DXT1 encode leg (learned vs plain); 0 skips it")
     ap.add_argument("--hc-size", type=int, default=2048,
                     help="side of the RGB texture of the dxt_hc leg; 0 skips it")
-    ap.add_argument("--hc-cpu-size", type=int, default=512,
-                    help="side of the texture the oracle's dxt_hc composition is timed on; 0 skips it")
+    ap.add_argument("--hc-cpu", type=int, default=1,
+                    help="1: time the oracle's dxt_hc composition on the same texture (about 15 s); 0 skips it")
+    ap.add_argument("--encode-cpu-seconds", type=float, default=5.0, help="CPU baseline sample of each encode leg")
     ap.add_argument("--fallback-frac", type=float, default=0.08,
                     help="fraction of blocks (lowest confidence first) that also run the plain search in the encode leg")
     args = ap.parse_args()
@@ -226,6 +449,13 @@ def main():
                 files.append(texture.synth_crn(args.size, 1 + b, fmt, palette_sizes=pal, writer=args.writer))
         log("inputs built in %.1fs: %s bytes" % (time.time() - t0, [len(f) for f in files]))
 
+    # crnd_unpack_begin of every file on the host (header, models, palettes):
+    # the part of the example2 rate (begin + levels) outside the GPU pipeline
+    b0 = time.perf_counter()
+    for _ in range(3):
+        for d in files:
+            crnd.unpack_end(crnd.unpack_begin(d))
+    begin_ms = (time.perf_counter() - b0) * 1e3 / 3
     ctxs, outs = [], []
     texels = 0         # of one copy of the files
     crn_bytes = 0
@@ -324,7 +554,13 @@ def main():
     mtex = args.copies * texels / (ms * 1e-3) / 1e6
     alg_bytes = args.copies * (crn_bytes + out_bytes)
     # algorithmic bytes of one step (all copies): compressed in + DXTn out
-    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    step_gbs = alg_bytes / (ms * 1e-3) / 1e9
+    # dominant kernel: the walk (the longest stage); one launch covers the
+    # first plan's copies, timed alone with HIP events on its stream
+    walk_ms = prof["crnd_walk_kernel"]
+    plan_bytes = per[0] * (crn_bytes + out_bytes)
+    achieved = plan_bytes / (walk_ms * 1e-3) / 1e9
+    default_inputs = not args.mixed and args.size == 4096 and args.batch == 1 and args.writer == "hc"
 
     # ---- CPU baseline: the C restatement on the host cores, one context per
     # level of every file (the reference transcoder is single-threaded per
@@ -388,14 +624,21 @@ def main():
                                 "copies per step" % (args.size, args.size, args.batch, args.copies)),
                    "copies": args.copies, "streams": groups, "one_copy_ms": round(one_ms, 3),
                    "one_copy_mtexels_s": round(texels / (one_ms * 1e-3) / 1e6, 1),
+                   "begin_ms_per_copy": round(begin_ms, 3),
+                   "one_copy_with_begin_mtexels_s": round(texels / ((one_ms + begin_ms) * 1e-3) / 1e6, 1),
+                   "step_with_begin_mtexels_s": round(args.copies * texels /
+                                                      ((ms + args.copies * begin_ms) * 1e-3) / 1e6, 1),
                    "crn_bytes": args.copies * crn_bytes,
                    "dxtn_bytes": args.copies * out_bytes, "texels": args.copies * texels},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": "crnd_walk_kernel (one launch per plan of %d copies)" % per[0],
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6),
-                     "traffic": None if (args.mixed or args.size != 4096 or args.batch != 1 or args.writer != "simple")
-                     else TRAFFIC_PER_COPY * args.copies,
-                     "kernel": "transcode pipeline (tables, walk, symbols, indices, write; %d plans of %s "
-                               "copies on %d streams)" % (groups, "/".join(str(x) for x in per), groups),
+                     "traffic": (WALK_TRAFFIC_PER_COPY * per[0]) if (default_inputs and WALK_TRAFFIC_PER_COPY)
+                     else None,
+                     "algorithmic_bytes_per_launch": plan_bytes, "launch_ms": round(walk_ms, 4),
+                     "step": {"achieved": round(step_gbs, 2), "frac": round(step_gbs / HBM_PEAK_GBS, 6),
+                              "what": "algorithmic bytes of the whole step / step time (%d plans of %s copies on "
+                                      "%d streams)" % (groups, "/".join(str(x) for x in per), groups)},
                      "stage_ms_one_plan": {k: round(v, 4) for k, v in prof.items()}},
         "cpu_baseline": {"value": round(cpu_mtex, 1), "unit": "Mtexels/s", "cores": threads, "kind": "port",
                          "sample": "%d full passes over the same %d files, %d level jobs on %d threads (%.1fs of "
@@ -404,10 +647,18 @@ def main():
     }
     for pl in plans:
         assert lib().crn2_batch_status(pl, C.c_void_p(stream.cuda_stream)) == 0
+    if args.plain_size:
+        line["plain_dxt1"] = plain_leg(args.plain_size, args.encode_cpu_seconds)
+        log("plain DXT1 leg done")
     if args.encode_size:
-        line["encode"] = encode_leg(args.encode_size, args.fallback_frac)
+        line["encode"] = encode_leg(args.encode_size, args.fallback_frac, args.encode_cpu_seconds)
+        log("learned encode leg done")
     if args.hc_size:
-        line["dxt_hc"] = hc_leg(args.hc_size, args.hc_cpu_size)
+        line["dxt_hc"] = hc_leg(args.hc_size, args.hc_cpu)
+        log("dxt_hc leg done")
+    if args.mixed_set:
+        line["mixed_set"] = mixed_leg(args.mixed_set, args.encode_cpu_seconds)
+        log("serving-set leg done")
     print(json.dumps(line))
     for pl in plans:
         lib().crn2_batch_destroy(pl)
