@@ -294,6 +294,12 @@ def hc_selector_refine(details, bits):
     return out
 
 
+def tree_position_order(on):
+    """Sum in the reference's position order (True) or the GPU's chunked
+    order (False, the default) in every later tree_codebook / hc_compress."""
+    lib().orc_tree_set_position_order(int(bool(on)))
+
+
 def tree_codebook(vectors, weights, max_splits):
     """tree_clusterizer::generate_codebook (crn_tree_clusterizer.h:89-171) in
     the fixed chunked summation order.  vectors: (n, dim) float32, dim <= 16;

@@ -28,7 +28,9 @@
  *     stably (left first) into the two children.
  *
  * Two deliberate, documented differences:
- *   - Summation order.  The reference sums float and double quantities over a
+ *   - Summation order (orc_tree_set_position_order(1) restores the reference's;
+ *     tests/test_tree_order_cpu.py measures the difference on dxt_hc inputs).
+ *     The reference sums float and double quantities over a
  *     node's vectors in position order.  Here the node's positions are cut
  *     into chunks of 4096; within a chunk lane t (of 256) sums positions
  *     chunk + t + 256 j in order, the 256 lane sums are combined by a fixed
@@ -103,6 +105,13 @@ typedef void (*tc_acc_fn)(const tc_ctx* x, uint32_t vi, const void* prm, float* 
 
 #define TC_MAXF 136
 
+/* 1: sum in the reference's position order instead (every quantity one
+ * accumulator from zero, positions b, b + 1, ..., e - 1): the variant that
+ * measures what the chunked order changes (orc_tree_set_position_order). */
+static int tc_position_order = 0;
+
+void orc_tree_set_position_order(int on) { tc_position_order = on ? 1 : 0; }
+
 static void tc_pass(const tc_ctx* x, uint32_t b, uint32_t e, tc_acc_fn acc, const void* prm, uint32_t nf,
                     uint32_t nd, uint32_t nu, float* tf, double* td, uint64_t* tu)
 {
@@ -111,6 +120,10 @@ static void tc_pass(const tc_ctx* x, uint32_t b, uint32_t e, tc_acc_fn acc, cons
     for (uint32_t k = 0; k < nf; k++) tf[k] = 0.0f;
     for (uint32_t k = 0; k < nd; k++) td[k] = 0.0;
     for (uint32_t k = 0; k < nu; k++) tu[k] = 0;
+    if (tc_position_order) {
+        for (uint32_t i = b; i < e; i++) acc(x, x->perm[i], prm, tf, td, tu);
+        return;
+    }
     for (uint32_t c0 = b; c0 < e; c0 += TC_CHUNK) {
         const uint32_t c1 = e - c0 < TC_CHUNK ? e : c0 + TC_CHUNK;
         /* Lanes at or past `live` hold no position: their sums stay +0.0, and
