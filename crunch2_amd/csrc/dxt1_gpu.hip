@@ -1,31 +1,36 @@
 // dxt1_gpu.hip -- DXT1 endpoint search (crnlib's dxt1_endpoint_optimizer,
-// crnlib/crn_dxt1.cpp) for 16-texel blocks on gfx950, one lane per block.
+// crnlib/crn_dxt1.cpp) for 16-texel blocks on gfx950, one wavefront per block.
 //
 // The search trajectory of crn_dxt1.cpp is reproduced step for step at every
 // quality (superfast .. uber; probe tables and passes :716-746, the
 // projection evaluator of the qualities below Better :1594-1757), with DXT1A
 // transparency (:2116, :1380, :300-343) and the transparent-black trial of
-// try_alpha_as_black_optimization (:2001-2079): unique colours in first-appearance order,
-// perceptual weights (:1074-1108), PCA (:192-256), projection bounds and ray
-// clamp (:1135-1175), k-means seeds with fast_random jitter (:1181-1308),
-// probe sweeps along the axis, 26-neighbour and +-2 lattice moves, least-
-// squares refinement (:525-698), solid-block tries (:93-153), per-channel
-// exhaustive search (:369-486) and combinatorial recovery (:1886-1997).
-// The float/double types and the order of every float expression follow the
-// reference; the file is compiled with -ffp-contract=off.
+// try_alpha_as_black_optimization (:2001-2079): unique colours in
+// first-appearance order, perceptual weights (:1074-1108), PCA (:192-256),
+// projection bounds and ray clamp (:1135-1175), k-means seeds with
+// fast_random jitter (:1181-1308), probe sweeps along the axis, 26-neighbour
+// and +-2 lattice moves, least-squares refinement (:525-698), solid-block
+// tries (:93-153), per-channel exhaustive search (:369-486) and combinatorial
+// recovery (:1886-1997).  The float/double types and the order of every float
+// expression follow the reference; the file is compiled with -ffp-contract=off.
 //
-// Two steps of the reference are result-invisible and are left out:
-//   * the error-plane bound of evaluate_solution (:1316-1322) is a rigorous
-//     lower bound on a candidate's error under the RGB metrics (every palette
-//     entry lies between the endpoints on each channel), so a pruned
-//     candidate would have been rejected by the full evaluation anyway.
-//     Under grayscale sampling the error is a luma error, which the bound
-//     does not bound: there it prunes candidates the evaluation would keep,
-//     so it is evaluated (per unique colour, from the colours' planes);
-//   * the tried-solutions set (:1324-1330) only skips re-evaluations, and a
-//     re-evaluated candidate never replaces the best (strict <).
-// State lives in registers: 16-entry arrays are only indexed from fully
-// unrolled loops, selectors are 2-bit fields of one 32-bit word.
+// Work split.  Colour k of the block lives on lane k.  The search's decisions
+// are the same on all lanes; the sums over colours that round (float means,
+// covariance, k-means centres) run over the colours in order, the exact
+// integer ones in any order.  Candidate endpoint pairs are evaluated one per
+// lane, 64 at a time, and taken in the reference's order: evaluate_solution
+// (:1312-1340) keeps a candidate when the error planes' bound of the pair and
+// then its error are below the best error at its turn, so a lane's candidate
+// is taken when both are below the best error left by the candidates before
+// it (the bound is not a strict lower bound of the error -- a texel on the
+// lower endpoint's plane may lie above it -- so it is applied exactly, not
+// only as a prune).  Lists that do not depend on the search state (sweeps,
+// refinement boxes, pairs) are first cut to the candidates whose bound is
+// below the best error when they are generated and gathered 64 at a time;
+// the lattice moves, which read the current best endpoints, are generated
+// again after each candidate taken.  The tried-solutions set (:1324-1330)
+// only skips re-evaluations of candidates that can no longer be taken and is
+// left out.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -50,26 +55,39 @@ __device__ __constant__ int8_t k_adj[26][3] = {
 // ryg single-colour tables, filled by crn2_dxt1_init (built on the host)
 __device__ uint8_t g_om5[256][2], g_om6[256][2], g_om5_3[256][2], g_om6_3[256][2];
 
-struct Sol {
-    uint32_t lo, hi;
-    uint64_t err;
-    uint32_t sel;     // 2 bits per unique colour
-    int nsel;         // 0: empty selector vector
-    int alpha, alt, enforce, enforced;
-};
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+// LDS words per wave: error planes (r lo/hi 32+32, g 64+64, b 32+32), the
+// pending candidate list (128 packed coordinates) and their bounds (128)
+constexpr uint32_t kPlaneWords = 256;
+constexpr uint32_t kLdsWords = kPlaneWords + 128 + 128;
 
-struct Block {
-    // parameters
-    int quality, perceptual, use_alpha, grayscale, force_alpha, have_alpha, hc;
-    // unique colours
-    uint32_t nu, total_w;
-    int r[16], g[16], b[16];
-    uint32_t w[16];
-    uint8_t pr[16], pg[16], pb[16]; // nearest 5/6/5-bit planes (grayscale bound)
-    float nr[16], ng[16], nb[16];   // mean-subtracted normalised colours
-    float mean[3], axis[3];
-    Sol best, trial;
-};
+__device__ inline int lane_id() { return (int)__lane_id(); }
+__device__ inline int rdl(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ inline uint32_t rdlu(uint32_t v, int k) { return (uint32_t)__builtin_amdgcn_readlane((int)v, k); }
+__device__ inline float rdlf(float v, int k) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k)); }
+__device__ inline double rdld(double v, int k)
+{
+    const uint64_t u = __double_as_longlong(v);
+    const uint32_t lo = rdlu((uint32_t)u, k), hi = rdlu((uint32_t)(u >> 32), k);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ inline uint64_t ballot(bool p) { return __ballot(p); }
+__device__ inline uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
+__device__ inline uint32_t wave_min(uint32_t v)
+{
+    for (int o = 32; o; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+__device__ inline float wave_minf(float v)
+{
+    for (int o = 32; o; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline float wave_maxf(float v)
+{
+    for (int o = 32; o; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
 
 __device__ inline void unpack565(uint32_t p, int scaled, int& r, int& g, int& b)
 {
@@ -98,10 +116,25 @@ __device__ inline uint32_t pack565(int r, int g, int b, int scaled)
     return B | (G << 5) | (R << 11);
 }
 
-__device__ inline uint32_t dist(const Block& s, int dr, int dg, int db, int ar, int ag, int ab, int br, int bg, int bb)
+__device__ inline int clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
+
+__device__ inline void canon(uint32_t& lo, uint32_t& hi)
 {
-    if (s.perceptual) return (uint32_t)(dr * dr * 8 + dg * dg * 25 + db * db);
-    if (s.grayscale) {
+    if (lo < hi) {
+        const uint32_t t = lo;
+        lo = hi;
+        hi = t;
+    }
+}
+
+// color_distance (crn_color.h:728-745) and the grayscale variant of
+// dxt1_endpoint_optimizer::color_distance (crn_dxt1.cpp:1342-1368):
+// texel (ar, ag, ab) against palette entry (br, bg, bb)
+__device__ inline uint32_t dist(int perceptual, int grayscale, int ar, int ag, int ab, int br, int bg, int bb)
+{
+    const int dr = ar - br, dg = ag - bg, db = ab - bb;
+    if (perceptual) return (uint32_t)(dr * dr * 8 + dg * dg * 25 + db * db);
+    if (grayscale) {
         const int y0 = (ar * 19595 + ag * 38470 + ab * 7471 + 32768) >> 16;
         const int y1 = (br * 19595 + bg * 38470 + bb * 7471 + 32768) >> 16;
         return (uint32_t)((y0 - y1) * (y0 - y1));
@@ -109,209 +142,362 @@ __device__ inline uint32_t dist(const Block& s, int dr, int dg, int db, int ar, 
     return (uint32_t)(dr * dr + dg * dg + db * db);
 }
 
-__device__ inline uint32_t cd(const Block& s, int i, int pr, int pg, int pb)
+struct Best {
+    uint32_t lo, hi, err;
+    int alpha, alt, enforce, enforced;
+    bool sel_valid;     // the selector vector holds values (m_selectors non-empty)
+};
+
+// One block, held by one wavefront.  Parameters, the best solution and the
+// search's float state are the same on every lane; colour k of the block's
+// unique colours (first-appearance order) lives on lane k, as does its
+// selector in the best solution.  Candidate endpoints are evaluated one per
+// lane; their acceptance follows the reference's sequential order.
+struct Wb {
+    int quality, perceptual, use_alpha, grayscale, force_alpha, have_alpha, hc;
+    uint32_t nu, total_w;
+    int cr, cg, cb;          // lane k: colour k (0 past nu)
+    uint32_t cw;             // lane k: its weight (0 past nu)
+    float nr, ng, nb;        // lane k: normalised colour minus the mean
+    float mean[3], axis[3];
+    Best best;
+    uint32_t bsel;           // lane k: selector of colour k in the best solution
+    uint32_t* planes;        // LDS
+    uint32_t* pend;          // LDS: pending candidates (lo | hi << 16)
+    uint32_t* pbound;        // LDS: their plane bounds
+};
+
+__device__ inline int colour_r(const Wb& s, int k) { return rdl(s.cr, k); }
+__device__ inline int colour_g(const Wb& s, int k) { return rdl(s.cg, k); }
+__device__ inline int colour_b(const Wb& s, int k) { return rdl(s.cb, k); }
+__device__ inline uint32_t colour_w(const Wb& s, int k) { return rdlu(s.cw, k); }
+
+// Error planes of compute_internal (crn_dxt1.cpp:2134-2203): for every 5/6-bit
+// value c of a channel, lo[c] = the weighted error of the colours whose
+// nearest plane is <= c against c's expanded value, hi[c] = of the others
+// (x8 red, x25 green under the perceptual metric).  Every entry is below
+// 16 * 255^2 * 25 < 2^32.
+__device__ void build_planes(Wb& s)
 {
-    return dist(s, s.r[i] - pr, s.g[i] - pg, s.b[i] - pb, s.r[i], s.g[i], s.b[i], pr, pg, pb);
+    for (int e = lane_id(); e < 128; e += 64) {
+        const int ch = e < 32 ? 0 : (e < 96 ? 1 : 2);
+        const uint32_t c = (uint32_t)(ch == 0 ? e : (ch == 1 ? e - 32 : e - 96));
+        const uint32_t C = ch == 1 ? ((c << 2) | (c >> 4)) : ((c << 3) | (c >> 2));
+        const uint32_t scale = s.perceptual ? (ch == 0 ? 8u : (ch == 1 ? 25u : 1u)) : 1u;
+        uint32_t lo = 0, hi = 0;
+        for (int k = 0; k < (int)s.nu; k++) {
+            const uint32_t V = (uint32_t)(ch == 0 ? colour_r(s, k) : (ch == 1 ? colour_g(s, k) : colour_b(s, k)));
+            const uint32_t p = ch == 1 ? (V >> 2) + (((V & 3) > (V >> 6)) ? 1u : 0u)
+                                       : (V >> 3) + (((V & 7) > (V >> 5)) ? 1u : 0u);
+            const int d = (int)V - (int)C;
+            const uint32_t t = colour_w(s, k) * (uint32_t)(d * d) * scale;
+            if (p <= c) lo += t;
+            else hi += t;
+        }
+        const uint32_t base = ch == 0 ? 0u : (ch == 1 ? 64u : 192u), n = ch == 1 ? 64u : 32u;
+        s.planes[base + c] = lo;
+        s.planes[base + n + c] = hi;
+    }
+    __builtin_amdgcn_wave_barrier();
 }
 
-// evaluate_solution_uber (crn_dxt1.cpp:1370-1592)
-__device__ bool eval_uber(Block& s, uint32_t lo, uint32_t hi, int alt)
+// the planes' bound of evaluate_solution (crn_dxt1.cpp:1316-1322)
+__device__ inline uint32_t plane_bound(const Wb& s, uint32_t lo, uint32_t hi)
 {
-    Sol& t = s.trial;
-    t.lo = lo;
-    t.hi = hi;
-    t.nsel = (int)s.nu;
-    t.err = s.best.err;
-    t.alpha = 0;
-    int first = 0, last = 1;
-    if (s.have_alpha || s.force_alpha) first = 1;
-    else if (!s.use_alpha) last = 0;
-    int c0r, c0g, c0b, c1r, c1g, c1b;
-    unpack565(lo, 1, c0r, c0g, c0b);
-    unpack565(hi, 1, c1r, c1g, c1b);
-    for (int type = first; type <= last; type++) {
-        int c2r, c2g, c2b, c3r = 0, c3g = 0, c3b = 0;
-        if (!type) {
-            c2r = (c0r * 2 + c1r + alt) / 3; c2g = (c0g * 2 + c1g + alt) / 3; c2b = (c0b * 2 + c1b + alt) / 3;
-            c3r = (c1r * 2 + c0r + alt) / 3; c3g = (c1g * 2 + c0g + alt) / 3; c3b = (c1b * 2 + c0b + alt) / 3;
-        } else {
-            c2r = (c0r + c1r + alt) >> 1; c2g = (c0g + c1g + alt) >> 1; c2b = (c0b + c1b + alt) >> 1;
-        }
-        uint64_t terr = 0;
-        uint32_t tsel = 0;
-        bool over = false;
-#pragma unroll
-        for (int k = 15; k >= 0; k--) {
-            if (over || k >= (int)s.nu) continue;
-            uint32_t be = cd(s, k, c0r, c0g, c0b), bi = 0;
-            uint32_t e = cd(s, k, c1r, c1g, c1b);
-            if (e < be) { be = e; bi = 1; }
-            e = cd(s, k, c2r, c2g, c2b);
-            if (e < be) { be = e; bi = 2; }
-            if (!type) {
-                e = cd(s, k, c3r, c3g, c3b);
-                if (e < be) { be = e; bi = 3; }
-            }
-            terr += (uint64_t)be * s.w[k];
-            if (terr >= t.err) { over = true; continue; }
-            tsel |= bi << (2 * k);
-        }
-        if (!over && terr < t.err) {
-            t.err = terr;
-            t.alpha = type != 0;
-            t.sel = tsel;
-            t.alt = alt;
-        }
-    }
-    t.enforce = !t.alpha && t.lo == t.hi;
-    if (t.enforce) {
-        int v;
-        if ((t.lo & 31) != 31) { t.lo++; v = 1; }
-        else { t.hi--; v = 0; }
-        uint32_t all = 0;
-#pragma unroll
-        for (int k = 0; k < 16; k++) all |= (uint32_t)v << (2 * k);
-        t.sel = all;
-        t.enforced = v;
-    }
-    if (t.err < s.best.err) {
-        s.best = t;
-        s.best.nsel = (int)s.nu;
-        return true;
-    }
-    return false;
+    int ar, ag, ab, br, bg, bb;
+    unpack565(lo, 0, ar, ag, ab);
+    unpack565(hi, 0, br, bg, bb);
+    const uint32_t* P = s.planes;
+    const uint32_t re = ar < br ? P[ar] + P[32 + br] : P[32 + ar] + P[br];
+    const uint32_t ge = ag < bg ? P[64 + ag] + P[128 + bg] : P[128 + ag] + P[64 + bg];
+    const uint32_t be = ab < bb ? P[192 + ab] + P[224 + bb] : P[224 + ab] + P[192 + bb];
+    return re + ge + be;
 }
 
-// evaluate_solution_fast (crn_dxt1.cpp:1594-1757), the evaluator below
-// cCRNDXTQualityBetter: each colour takes the palette entry whose projection
-// onto the endpoint axis (green weighted 24 under the perceptual metric) is
-// nearest, by midpoint thresholds, then one distance.  No candidate with the
-// alternate rounding reaches it, so the trial's rounding flag stays 0.
-__device__ bool eval_fast(Block& s, uint32_t lo, uint32_t hi)
-{
-    Sol& t = s.trial;
-    t.lo = lo;
-    t.hi = hi;
-    t.nsel = (int)s.nu;
-    t.err = s.best.err;
-    t.alpha = 0;
-    int first = 0, last = 1;
-    if (s.have_alpha || s.force_alpha) first = 1;
-    else if (!s.use_alpha) last = 0;
-    int c0r, c0g, c0b, c1r, c1g, c1b;
-    unpack565(lo, 1, c0r, c0g, c0b);
-    unpack565(hi, 1, c1r, c1g, c1b);
-    int vr = c1r - c0r, vg = c1g - c0g, vb = c1b - c0b;
-    if (s.perceptual) {
-        vr *= 8;
-        vg *= 24;
-    }
-    const int s0 = c0r * vr + c0g * vg + c0b * vb, s1 = c1r * vr + c1g * vg + c1b * vb;
-    const int dr = vr * 2, dg = vg * 2, db = vb * 2;
-    for (int type = first; type <= last; type++) {
-        int pr[4], pg[4], pb[4];
-        pr[0] = c0r; pg[0] = c0g; pb[0] = c0b;
-        pr[1] = c1r; pg[1] = c1g; pb[1] = c1b;
-        int t0, t1, t2;   // thresholds of the projection (twice the stop midpoints)
-        if (!type) {
-            pr[2] = (c0r * 2 + c1r) / 3; pg[2] = (c0g * 2 + c1g) / 3; pb[2] = (c0b * 2 + c1b) / 3;
-            pr[3] = (c1r * 2 + c0r) / 3; pg[3] = (c1g * 2 + c0g) / 3; pb[3] = (c1b * 2 + c0b) / 3;
-            const int s2 = pr[2] * vr + pg[2] * vg + pb[2] * vb, s3 = pr[3] * vr + pg[3] * vg + pb[3] * vb;
-            t0 = s2 + s0;      // below: entry 0, else 2
-            t1 = s3 + s2;      // half point
-            t2 = s1 + s3;      // below: entry 3, else 1
-        } else {
-            pr[2] = (c0r + c1r) >> 1; pg[2] = (c0g + c1g) >> 1; pb[2] = (c0b + c1b) >> 1;
-            pr[3] = pg[3] = pb[3] = 0;
-            const int s2 = pr[2] * vr + pg[2] * vg + pb[2] * vb;
-            t0 = s0 + s2;
-            t1 = s2 + s1;
-            t2 = 0;
-        }
-        uint64_t terr = 0;
-        uint32_t tsel = 0;
-        bool over = false;
-#pragma unroll
-        for (int k = 15; k >= 0; k--) {
-            if (over || k >= (int)s.nu) continue;
-            const int dot = s.r[k] * dr + s.g[k] * dg + s.b[k] * db;
-            uint32_t bi;
-            if (!type) bi = dot >= t1 ? (dot < t2 ? 3u : 1u) : (dot < t0 ? 0u : 2u);
-            else bi = dot < t0 ? 0u : (dot < t1 ? 2u : 1u);
-            const int qr = bi == 0 ? pr[0] : bi == 1 ? pr[1] : bi == 2 ? pr[2] : pr[3];
-            const int qg = bi == 0 ? pg[0] : bi == 1 ? pg[1] : bi == 2 ? pg[2] : pg[3];
-            const int qb = bi == 0 ? pb[0] : bi == 1 ? pb[1] : bi == 2 ? pb[2] : pb[3];
-            terr += (uint64_t)cd(s, k, qr, qg, qb) * s.w[k];
-            if (terr >= t.err) { over = true; continue; }
-            tsel |= bi << (2 * k);
-        }
-        if (!over && terr < t.err) {
-            t.err = terr;
-            t.alpha = type != 0;
-            t.sel = tsel;
-        }
-    }
-    t.alt = 0;
-    t.enforce = !t.alpha && t.lo == t.hi;
-    if (t.enforce) {
-        int v;
-        if ((t.lo & 31) != 31) { t.lo++; v = 1; }
-        else { t.hi--; v = 0; }
-        uint32_t all = 0;
-#pragma unroll
-        for (int k = 0; k < 16; k++) all |= (uint32_t)v << (2 * k);
-        t.sel = all;
-        t.enforced = v;
-    }
-    if (t.err < s.best.err) {
-        s.best = t;
-        s.best.nsel = (int)s.nu;
-        return true;
-    }
-    return false;
-}
-
-// evaluate_solution_hc_* (crn_dxt1.cpp:1759-1835)
-__device__ bool eval_hc(Block& s, uint32_t lo, uint32_t hi, int alt)
+// Error of candidate (lo, hi, alt) on this lane under the block's evaluator,
+// and the block type it would take: evaluate_solution_hc_*
+// (crn_dxt1.cpp:1759-1835), _uber (:1370-1592: 4-colour, then a 3-colour
+// block only if strictly better) or _fast (:1594-1757, below Better).  Each
+// evaluator's early exits only decide whether the candidate is kept, which
+// the caller decides from the full error.
+__device__ uint32_t cand_err(const Wb& s, uint32_t lo, uint32_t hi, int alt, int& type)
 {
     int c0r, c0g, c0b, c1r, c1g, c1b;
     unpack565(lo, 1, c0r, c0g, c0b);
     unpack565(hi, 1, c1r, c1g, c1b);
     const int c2r = (c0r * 2 + c1r + alt) / 3, c2g = (c0g * 2 + c1g + alt) / 3, c2b = (c0b * 2 + c1b + alt) / 3;
     const int c3r = (c1r * 2 + c0r + alt) / 3, c3g = (c1g * 2 + c0g + alt) / 3, c3b = (c1b * 2 + c0b + alt) / 3;
-    uint64_t err = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (k >= (int)s.nu) continue;
-        const uint32_t e0 = cd(s, k, c0r, c0g, c0b), e1 = cd(s, k, c1r, c1g, c1b);
-        const uint32_t e2 = cd(s, k, c2r, c2g, c2b), e3 = cd(s, k, c3r, c3g, c3b);
-        const uint32_t e01 = min(e0, e1), e23 = min(e2, e3);
-        err += (uint64_t)min(e01, e23) * s.w[k];
+    const int h2r = (c0r + c1r + alt) >> 1, h2g = (c0g + c1g + alt) >> 1, h2b = (c0b + c1b + alt) >> 1;
+    const int P = s.perceptual, G = s.grayscale;
+    type = 0;
+    if (s.hc) {
+        uint32_t e = 0;
+        for (int k = 0; k < (int)s.nu; k++) {
+            const int r = colour_r(s, k), g = colour_g(s, k), b = colour_b(s, k);
+            const uint32_t e0 = dist(P, G, r, g, b, c0r, c0g, c0b), e1 = dist(P, G, r, g, b, c1r, c1g, c1b);
+            const uint32_t e2 = dist(P, G, r, g, b, c2r, c2g, c2b), e3 = dist(P, G, r, g, b, c3r, c3g, c3b);
+            e += min(min(e0, e1), min(e2, e3)) * colour_w(s, k);
+        }
+        return e;
     }
-    if (err >= s.best.err) return false;
-    s.best.lo = lo;
-    s.best.hi = hi;
-    s.best.err = err;
-    s.best.alpha = 0;
-    s.best.alt = alt;
-    s.best.enforce = lo == hi;
-    if (s.best.enforce) {
-        if ((s.best.lo & 31) != 31) { s.best.lo++; s.best.enforced = 1; }
-        else { s.best.hi--; s.best.enforced = 0; }
+    int first = 0, last = 1;
+    if (s.have_alpha || s.force_alpha) first = 1;
+    else if (!s.use_alpha) last = 0;
+    uint32_t e4 = 0, e3 = 0;
+    if (s.quality >= 3) {
+        for (int k = 0; k < (int)s.nu; k++) {
+            const int r = colour_r(s, k), g = colour_g(s, k), b = colour_b(s, k);
+            const uint32_t w = colour_w(s, k);
+            const uint32_t d0 = dist(P, G, r, g, b, c0r, c0g, c0b), d1 = dist(P, G, r, g, b, c1r, c1g, c1b);
+            if (first == 0) {
+                const uint32_t d2 = dist(P, G, r, g, b, c2r, c2g, c2b), d3 = dist(P, G, r, g, b, c3r, c3g, c3b);
+                e4 += min(min(d0, d1), min(d2, d3)) * w;
+            }
+            if (last == 1) e3 += min(min(d0, d1), dist(P, G, r, g, b, h2r, h2g, h2b)) * w;
+        }
+    } else {
+        int vr = c1r - c0r, vg = c1g - c0g, vb = c1b - c0b;
+        if (P) {
+            vr *= 8;
+            vg *= 24;
+        }
+        const int s0 = c0r * vr + c0g * vg + c0b * vb, s1 = c1r * vr + c1g * vg + c1b * vb;
+        const int s2 = c2r * vr + c2g * vg + c2b * vb, s3 = c3r * vr + c3g * vg + c3b * vb;
+        const int h2 = h2r * vr + h2g * vg + h2b * vb;
+        const int dr = vr * 2, dg = vg * 2, db = vb * 2;
+        for (int k = 0; k < (int)s.nu; k++) {
+            const int r = colour_r(s, k), g = colour_g(s, k), b = colour_b(s, k);
+            const uint32_t w = colour_w(s, k);
+            const int dot = r * dr + g * dg + b * db;
+            if (first == 0) {
+                int pr, pg, pb;
+                if (dot >= s3 + s2) {
+                    const bool three = dot < s1 + s3;
+                    pr = three ? c3r : c1r; pg = three ? c3g : c1g; pb = three ? c3b : c1b;
+                } else {
+                    const bool zero = dot < s2 + s0;
+                    pr = zero ? c0r : c2r; pg = zero ? c0g : c2g; pb = zero ? c0b : c2b;
+                }
+                e4 += dist(P, G, r, g, b, pr, pg, pb) * w;
+            }
+            if (last == 1) {
+                int pr, pg, pb;
+                if (dot < s0 + h2) { pr = c0r; pg = c0g; pb = c0b; }
+                else if (dot < h2 + s1) { pr = h2r; pg = h2g; pb = h2b; }
+                else { pr = c1r; pg = c1g; pb = c1b; }
+                e3 += dist(P, G, r, g, b, pr, pg, pb) * w;
+            }
+        }
     }
-    return true;
+    if (first == 1) {
+        type = 1;
+        return e3;
+    }
+    if (last == 1 && e3 < e4) {
+        type = 1;
+        return e3;
+    }
+    return e4;
 }
 
-__device__ void compute_selectors(Block& s)
+// Lane k's selector of colour k under (lo, hi, alt, type) by the evaluator's
+// rule: the first minimum of the palette (uber), the projection thresholds
+// (fast).
+__device__ uint32_t colour_selector(const Wb& s, uint32_t lo, uint32_t hi, int alt, int type)
+{
+    int c0r, c0g, c0b, c1r, c1g, c1b;
+    unpack565(lo, 1, c0r, c0g, c0b);
+    unpack565(hi, 1, c1r, c1g, c1b);
+    int pr[4], pg[4], pb[4];
+    pr[0] = c0r; pg[0] = c0g; pb[0] = c0b;
+    pr[1] = c1r; pg[1] = c1g; pb[1] = c1b;
+    if (!type) {
+        pr[2] = (c0r * 2 + c1r + alt) / 3; pg[2] = (c0g * 2 + c1g + alt) / 3; pb[2] = (c0b * 2 + c1b + alt) / 3;
+        pr[3] = (c1r * 2 + c0r + alt) / 3; pg[3] = (c1g * 2 + c0g + alt) / 3; pb[3] = (c1b * 2 + c0b + alt) / 3;
+    } else {
+        pr[2] = (c0r + c1r + alt) >> 1; pg[2] = (c0g + c1g + alt) >> 1; pb[2] = (c0b + c1b + alt) >> 1;
+        pr[3] = pg[3] = pb[3] = 0;
+    }
+    const int ncol = type ? 3 : 4;
+    if (s.quality >= 3) {
+        uint32_t be = dist(s.perceptual, s.grayscale, s.cr, s.cg, s.cb, pr[0], pg[0], pb[0]), bi = 0;
+        for (int j = 1; j < 4; j++) {
+            if (j >= ncol) break;
+            const uint32_t e = dist(s.perceptual, s.grayscale, s.cr, s.cg, s.cb, pr[j], pg[j], pb[j]);
+            if (e < be) {
+                be = e;
+                bi = (uint32_t)j;
+            }
+        }
+        return bi;
+    }
+    int vr = c1r - c0r, vg = c1g - c0g, vb = c1b - c0b;
+    if (s.perceptual) {
+        vr *= 8;
+        vg *= 24;
+    }
+    int st[4];
+    for (int j = 0; j < 4; j++) st[j] = pr[j] * vr + pg[j] * vg + pb[j] * vb;
+    const int dot = s.cr * vr * 2 + s.cg * vg * 2 + s.cb * vb * 2;
+    if (!type) return dot >= st[3] + st[2] ? (dot < st[1] + st[3] ? 3u : 1u) : (dot < st[2] + st[0] ? 0u : 2u);
+    return dot < st[0] + st[2] ? 0u : (dot < st[2] + st[1] ? 2u : 1u);
+}
+
+// The winner of an evaluate_solution call becomes the best solution.
+__device__ void accept(Wb& s, uint32_t lo, uint32_t hi, int alt, int type, uint32_t err)
+{
+    Best& b = s.best;
+    b.lo = lo;
+    b.hi = hi;
+    b.err = err;
+    if (s.hc) {
+        // evaluate_solution_hc_*: selectors are computed later (compute_selectors)
+        b.alpha = 0;
+        b.alt = alt;
+        b.enforce = lo == hi;
+        if (b.enforce) {
+            if ((b.lo & 31) != 31) { b.lo++; b.enforced = 1; }
+            else { b.hi--; b.enforced = 0; }
+        }
+        return;
+    }
+    b.alpha = type;
+    b.alt = s.quality >= 3 ? alt : 0;
+    s.bsel = colour_selector(s, lo, hi, b.alt, type);
+    b.sel_valid = true;
+    b.enforce = !b.alpha && lo == hi;
+    if (b.enforce) {
+        int v;
+        if ((b.lo & 31) != 31) { b.lo++; v = 1; }
+        else { b.hi--; v = 0; }
+        s.bsel = (uint32_t)v;
+        b.enforced = v;
+    }
+}
+
+// The reference's sequential evaluate_solution calls over the candidates on
+// the valid lanes, in lane order: a candidate is taken when its plane bound
+// and its error are both below the current best error.  stop_first returns
+// after the first one taken (its lane, or -1).
+__device__ int take(Wb& s, bool valid, uint32_t lo, uint32_t hi, int alt, uint32_t bnd, uint32_t err, int type,
+                    bool stop_first)
+{
+    int last = -1;
+    uint64_t m = ballot(valid && bnd < s.best.err && err < s.best.err);
+    while (m) {
+        const int l = __ffsll((unsigned long long)m) - 1;
+        accept(s, rdlu(lo, l), rdlu(hi, l), rdl(alt, l), rdl(type, l), rdlu(err, l));
+        last = l;
+        if (stop_first) break;
+        m = ballot(valid && lane_id() > l && bnd < s.best.err && err < s.best.err);
+    }
+    return last;
+}
+
+// Evaluates the pending candidates (the first cnt entries) in order.
+__device__ bool flush(Wb& s, int alt, uint32_t cnt)
+{
+    const bool v = (uint32_t)lane_id() < cnt;
+    const uint32_t pk = v ? s.pend[lane_id()] : 0u, bnd = v ? s.pbound[lane_id()] : kNone;
+    const uint32_t lo = pk & 0xFFFFu, hi = pk >> 16;
+    int type = 0;
+    const uint32_t e = cand_err(s, lo, hi, alt, type);
+    return take(s, v, lo, hi, alt, bnd, v ? e : kNone, type, false) >= 0;
+}
+
+// n candidates that do not depend on the search state, candidate i =
+// gen(i, lo, hi) (false: skipped), evaluated in order with rounding flag alt.
+// gen runs on every lane (lanes past n with index 0), so that the shuffles in
+// it read active lanes.
+// Candidates whose bound already reaches the best error are dropped before
+// the full evaluation (the bound is checked again, in order, when taken);
+// the rest are gathered 64 at a time.
+template <class Gen>
+__device__ bool eval_list(Wb& s, uint32_t n, int alt, Gen gen)
+{
+    bool any = false;
+    uint32_t cnt = 0;
+    for (uint32_t base = 0; base < n; base += 64) {
+        const uint32_t i = base + (uint32_t)lane_id();
+        uint32_t lo = 0, hi = 0;
+        const bool made = gen(i < n ? i : 0u, lo, hi);
+        const bool ok = i < n && made;
+        const uint32_t bnd = ok ? plane_bound(s, lo, hi) : kNone;
+        const bool alive = ok && bnd < s.best.err;
+        const uint64_t am = ballot(alive);
+        if (!am) continue;
+        const uint32_t pos = cnt + (uint32_t)__popcll(am & lanes_below());
+        if (alive) {
+            s.pend[pos] = lo | (hi << 16);
+            s.pbound[pos] = bnd;
+        }
+        cnt += (uint32_t)__popcll(am);
+        __builtin_amdgcn_wave_barrier();
+        if (cnt >= 64) {
+            any |= flush(s, alt, 64);
+            const uint32_t rest = cnt - 64;
+            uint32_t a = 0, b = 0;
+            if ((uint32_t)lane_id() < rest) {
+                a = s.pend[64 + lane_id()];
+                b = s.pbound[64 + lane_id()];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if ((uint32_t)lane_id() < rest) {
+                s.pend[lane_id()] = a;
+                s.pbound[lane_id()] = b;
+            }
+            __builtin_amdgcn_wave_barrier();
+            cnt = rest;
+        }
+    }
+    if (cnt) any |= flush(s, alt, cnt);
+    return any;
+}
+
+// n <= 64 candidates that depend on the best solution (lattice moves):
+// candidate i = gen(i, lo, hi) is generated from the state at its turn, so
+// after a candidate is taken the ones after it are generated again.
+template <class Gen>
+__device__ bool eval_chain(Wb& s, uint32_t n, Gen gen)
+{
+    bool any = false;
+    uint32_t start = 0;
+    while (start < n) {
+        const uint32_t i = (uint32_t)lane_id();
+        uint32_t lo = 0, hi = 0;
+        const bool made = gen(i < n ? i : 0u, lo, hi);
+        const bool ok = i >= start && i < n && made;
+        const uint32_t bnd = ok ? plane_bound(s, lo, hi) : kNone;
+        if (!ballot(ok && bnd < s.best.err)) break;
+        int type = 0;
+        const uint32_t e = cand_err(s, lo, hi, 0, type);
+        const int l = take(s, ok, lo, hi, 0, bnd, ok ? e : kNone, type, true);
+        if (l < 0) break;
+        any = true;
+        start = (uint32_t)l + 1;
+    }
+    return any;
+}
+
+// one candidate (the reference's single evaluate_solution call)
+__device__ bool eval_one(Wb& s, uint32_t lo, uint32_t hi, int alt)
+{
+    return eval_list(s, 1, alt, [&](uint32_t, uint32_t& a, uint32_t& b) {
+        a = lo;
+        b = hi;
+        return true;
+    });
+}
+
+// compute_selectors (crn_dxt1.cpp:1837-1869): the hc evaluator's selectors
+// of the best solution
+__device__ void compute_selectors(Wb& s)
 {
     if (!s.hc) return;
-    Sol& b = s.best;
-    b.nsel = (int)s.nu;
+    Best& b = s.best;
+    b.sel_valid = true;
     if (b.enforce) {
-        uint32_t all = 0;
-#pragma unroll
-        for (int k = 0; k < 16; k++) all |= (uint32_t)b.enforced << (2 * k);
-        b.sel = all;
+        s.bsel = (uint32_t)b.enforced;
         return;
     }
     int c0r, c0g, c0b, c1r, c1g, c1b;
@@ -320,84 +506,30 @@ __device__ void compute_selectors(Block& s)
     const int a = b.alt;
     const int c2r = (c0r * 2 + c1r + a) / 3, c2g = (c0g * 2 + c1g + a) / 3, c2b = (c0b * 2 + c1b + a) / 3;
     const int c3r = (c1r * 2 + c0r + a) / 3, c3g = (c1g * 2 + c0g + a) / 3, c3b = (c1b * 2 + c0b + a) / 3;
-    uint32_t sel = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (k >= (int)s.nu) continue;
-        const uint32_t e0 = cd(s, k, c0r, c0g, c0b), e1 = cd(s, k, c1r, c1g, c1b);
-        const uint32_t e2 = cd(s, k, c2r, c2g, c2b), e3 = cd(s, k, c3r, c3g, c3b);
-        const uint32_t e01 = min(e0, e1), e23 = min(e2, e3);
-        const uint32_t v = e01 <= e23 ? (e01 == e0 ? 0u : 1u) : (e23 == e2 ? 2u : 3u);
-        sel |= v << (2 * k);
-    }
-    b.sel = sel;
+    const int P = s.perceptual, G = s.grayscale;
+    const uint32_t e0 = dist(P, G, s.cr, s.cg, s.cb, c0r, c0g, c0b), e1 = dist(P, G, s.cr, s.cg, s.cb, c1r, c1g, c1b);
+    const uint32_t e2 = dist(P, G, s.cr, s.cg, s.cb, c2r, c2g, c2b), e3 = dist(P, G, s.cr, s.cg, s.cb, c3r, c3g, c3b);
+    const uint32_t e01 = min(e0, e1), e23 = min(e2, e3);
+    s.bsel = e01 <= e23 ? (e01 == e0 ? 0u : 1u) : (e23 == e2 ? 2u : 3u);
 }
 
-// the error planes' bound of evaluate_solution (crn_dxt1.cpp:1316-1322,
-// planes :2134-2203): per channel, the colours whose plane is at or below the
-// lower endpoint's measured against it, those above the higher endpoint's
-// against that (the endpoints taken per channel in the order of the test
-// c0 < c1); only consulted under grayscale sampling
-__device__ uint64_t plane_bound(const Block& s, uint32_t lo, uint32_t hi)
+__device__ bool sel_all_equal(const Wb& s)
 {
-    int ar, ag, ab, br, bg, bb;
-    unpack565(lo, 0, ar, ag, ab);
-    unpack565(hi, 0, br, bg, bb);
-    const int Ar = (ar << 3) | (ar >> 2), Ag = (ag << 2) | (ag >> 4), Ab = (ab << 3) | (ab >> 2);
-    const int Br = (br << 3) | (br >> 2), Bg = (bg << 2) | (bg >> 4), Bb = (bb << 3) | (bb >> 2);
-    uint64_t e = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (k >= (int)s.nu) continue;
-        const uint64_t w = s.w[k];
-        int d;
-        // red: a.r < b.r ? low(a) + high(b) : high(a) + low(b)
-        if (ar < br) d = s.pr[k] <= ar ? s.r[k] - Ar : (s.pr[k] > br ? s.r[k] - Br : 0);
-        else d = s.pr[k] > ar ? s.r[k] - Ar : (s.pr[k] <= br ? s.r[k] - Br : 0);
-        e += w * (uint64_t)(d * d);
-        if (ag < bg) d = s.pg[k] <= ag ? s.g[k] - Ag : (s.pg[k] > bg ? s.g[k] - Bg : 0);
-        else d = s.pg[k] > ag ? s.g[k] - Ag : (s.pg[k] <= bg ? s.g[k] - Bg : 0);
-        e += w * (uint64_t)(d * d);
-        if (ab < bb) d = s.pb[k] <= ab ? s.b[k] - Ab : (s.pb[k] > bb ? s.b[k] - Bb : 0);
-        else d = s.pb[k] > ab ? s.b[k] - Ab : (s.pb[k] <= bb ? s.b[k] - Bb : 0);
-        e += w * (uint64_t)(d * d);
-    }
-    return e;
+    if (!s.best.sel_valid) return false;
+    const uint32_t f = rdlu(s.bsel, 0);
+    return !ballot((uint32_t)lane_id() < s.nu && s.bsel != f);
 }
 
-__device__ inline bool evaluate(Block& s, uint32_t lo, uint32_t hi, int alt)
-{
-    if (s.grayscale && plane_bound(s, lo, hi) >= s.best.err) return false;
-    if (s.hc) return eval_hc(s, lo, hi, alt);
-    return s.quality >= 3 ? eval_uber(s, lo, hi, alt) : eval_fast(s, lo, hi);
-}
-
-__device__ inline void canon(uint32_t& lo, uint32_t& hi)
-{
-    if (lo < hi) {
-        const uint32_t t = lo; lo = hi; hi = t;
-    }
-}
-
-__device__ inline bool eval_canon(Block& s, uint32_t lo, uint32_t hi)
-{
-    canon(lo, hi);
-    return evaluate(s, lo, hi, 0);
-}
-
-__device__ inline int clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
-
-// refine_solution (crn_dxt1.cpp:525-698), levels 0 and 1
-__device__ bool refine(Block& s, int level)
+// refine_solution (crn_dxt1.cpp:525-698), levels 0 and 1.  Every sum is of
+// integers below 2^53, exact as the reference's doubles.
+__device__ bool refine(Wb& s, int level)
 {
     compute_selectors(s);
-    double a0 = 0, a1 = 0, a2 = 0, t1r = 0, t1g = 0, t1b = 0, t2r = 0, t2g = 0, t2b = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (k >= (int)s.nu) continue;
-        const double w = s.w[k];
-        const double r = s.r[k] * w, g = s.g[k] * w, b = s.b[k] * w;
-        const int step = (int)((s.best.sel >> (2 * k)) & 3u) ^ 1;
+    int64_t a0 = 0, a1 = 0, a2 = 0, t1r = 0, t1g = 0, t1b = 0, t2r = 0, t2g = 0, t2b = 0;
+    for (int k = 0; k < (int)s.nu; k++) {
+        const int64_t w = colour_w(s, k);
+        const int64_t r = colour_r(s, k) * w, g = colour_g(s, k) * w, b = colour_b(s, k) * w;
+        const int step = (int)rdlu(s.bsel, k) ^ 1;
         const int w1 = step == 0 ? 3 : step == 1 ? 0 : step == 2 ? 2 : 1;
         const int q0 = step >= 2 ? 2 : 0;
         const int q1 = step == 0 ? 0 : step == 1 ? 9 : step == 2 ? 1 : 4;
@@ -408,40 +540,39 @@ __device__ bool refine(Block& s, int level)
         t1r += w1 * r; t1g += w1 * g; t1b += w1 * b;
         t2r += r; t2g += g; t2b += b;
     }
-    t2r = 3 * t2r - t1r;
-    t2g = 3 * t2g - t1g;
-    t2b = 3 * t2b - t1b;
-    const double xx = a2, yy = a1, xy = a0;
+    const double T1r = (double)t1r, T1g = (double)t1g, T1b = (double)t1b;
+    const double T2r = 3 * (double)t2r - T1r, T2g = 3 * (double)t2g - T1g, T2b = 3 * (double)t2b - T1b;
+    const double xx = (double)a2, yy = (double)a1, xy = (double)a0;
     const double t = xx * yy - xy * xy;
     if (!yy || !xx || fabs(t) < (double).0000125f) return false;
     const double frb = (double)(3.0f * 31.0f / 255.0f) / t;
     const double fg = frb * (double)(63.0f / 31.0f);
     int e0[3], e1[3];
-    e0[0] = clampi((int)((t1r * yy - t2r * xy) * frb + (double)0.5f), 31);
-    e0[1] = clampi((int)((t1g * yy - t2g * xy) * fg + (double)0.5f), 63);
-    e0[2] = clampi((int)((t1b * yy - t2b * xy) * frb + (double)0.5f), 31);
-    e1[0] = clampi((int)((t2r * xx - t1r * xy) * frb + (double)0.5f), 31);
-    e1[1] = clampi((int)((t2g * xx - t1g * xy) * fg + (double)0.5f), 63);
-    e1[2] = clampi((int)((t2b * xx - t1b * xy) * frb + (double)0.5f), 31);
-    bool improved = false;
+    e0[0] = clampi((int)((T1r * yy - T2r * xy) * frb + (double)0.5f), 31);
+    e0[1] = clampi((int)((T1g * yy - T2g * xy) * fg + (double)0.5f), 63);
+    e0[2] = clampi((int)((T1b * yy - T2b * xy) * frb + (double)0.5f), 31);
+    e1[0] = clampi((int)((T2r * xx - T1r * xy) * frb + (double)0.5f), 31);
+    e1[1] = clampi((int)((T2g * xx - T1g * xy) * fg + (double)0.5f), 63);
+    e1[2] = clampi((int)((T2b * xx - T1b * xy) * frb + (double)0.5f), 31);
     if (level == 0) {
         uint32_t lo = (uint32_t)((e1[0] << 11) | (e1[1] << 5) | e1[2]);
         uint32_t hi = (uint32_t)((e0[0] << 11) | (e0[1] << 5) | e0[2]);
-        improved |= eval_canon(s, lo, hi);
-    } else {
-        for (int i = 0; i < 2; i++)
-            for (int rr = -1; rr <= 1; rr++)
-                for (int gr = -1; gr <= 1; gr++)
-                    for (int br = -1; br <= 1; br++) {
-                        int c0[3] = {e0[0], e0[1], e0[2]}, c1[3] = {e1[0], e1[1], e1[2]};
-                        int* c = i ? c1 : c0;
-                        c[0] = clampi(c[0] + rr, 31);
-                        c[1] = clampi(c[1] + gr, 63);
-                        c[2] = clampi(c[2] + br, 31);
-                        improved |= eval_canon(s, pack565(c0[0], c0[1], c0[2], 0), pack565(c1[0], c1[1], c1[2], 0));
-                    }
+        canon(lo, hi);
+        return eval_one(s, lo, hi, 0);
     }
-    return improved;
+    // 2 x 27 candidates: one endpoint moved by -1..+1 per channel (i, r, g, b order)
+    return eval_list(s, 54, 0, [&](uint32_t idx, uint32_t& lo, uint32_t& hi) {
+        const int i = (int)idx / 27, rr = ((int)idx % 27) / 9 - 1, gr = ((int)idx % 9) / 3 - 1, br = (int)idx % 3 - 1;
+        int c0[3] = {e0[0], e0[1], e0[2]}, c1[3] = {e1[0], e1[1], e1[2]};
+        int* c = i ? c1 : c0;
+        c[0] = clampi(c[0] + rr, 31);
+        c[1] = clampi(c[1] + gr, 63);
+        c[2] = clampi(c[2] + br, 31);
+        lo = pack565(c0[0], c0[1], c0[2], 0);
+        hi = pack565(c1[0], c1[1], c1[2], 0);
+        canon(lo, hi);
+        return true;
+    });
 }
 
 // fast_random (crn_rand.cpp:310-405)
@@ -463,16 +594,17 @@ struct FastRandom {
     }
 };
 
-__device__ bool try_median4(Block& s, const float* lo, const float* hi)
+// try_median4 (crn_dxt1.cpp:1181-1308).  Each colour's nearest centre is
+// found on its lane; the float sums run over the colours in order.
+__device__ bool try_median4(Wb& s, const float* lo, const float* hi)
 {
     float m[4][3];
     if (s.nu <= 4) {
         for (int i = 0; i < 4; i++) {
             const int k = min(i, (int)s.nu - 1);
-            // k is at most 3: select through constant indices
-            m[i][0] = k == 0 ? s.nr[0] : k == 1 ? s.nr[1] : k == 2 ? s.nr[2] : s.nr[3];
-            m[i][1] = k == 0 ? s.ng[0] : k == 1 ? s.ng[1] : k == 2 ? s.ng[2] : s.ng[3];
-            m[i][2] = k == 0 ? s.nb[0] : k == 1 ? s.nb[1] : k == 2 ? s.nb[2] : s.nb[3];
+            m[i][0] = rdlf(s.nr, k);
+            m[i][1] = rdlf(s.ng, k);
+            m[i][2] = rdlf(s.nb, k);
         }
     } else {
         for (int c = 0; c < 3; c++) {
@@ -486,36 +618,37 @@ __device__ bool try_median4(Block& s, const float* lo, const float* hi)
         FastRandom rm;
         uint32_t rover = 0;
         float prev = 1.0e+37f;
+        const float wf = (float)s.cw;
+        const float pr = s.nr * wf, pg = s.ng * wf, pb = s.nb * wf;
         for (uint32_t it = 0; it < 8; it++) {
+            // this lane's colour: nearest centre (first minimum) and its distance
+            float bd = 0;
+            {
+                float d = m[0][0] - s.nr; bd += d * d;
+                d = m[0][1] - s.ng; bd += d * d;
+                d = m[0][2] - s.nb; bd += d * d;
+            }
+            int bi = 0;
+            for (int j = 1; j < 4; j++) {
+                float dd = 0;
+                float d = m[j][0] - s.nr; dd += d * d;
+                d = m[j][1] - s.ng; dd += d * d;
+                d = m[j][2] - s.nb; dd += d * d;
+                if (dd < bd) { bd = dd; bi = j; }
+            }
             float nm[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
             float nw[4] = {0, 0, 0, 0};
             float total = 0;
-#pragma unroll
-            for (int k = 0; k < 16; k++) {
-                if (k >= (int)s.nu) continue;
-                const float v0 = s.nr[k], v1 = s.ng[k], v2 = s.nb[k];
-                float bd = 0;
-                {
-                    float d = m[0][0] - v0; bd += d * d;
-                    d = m[0][1] - v1; bd += d * d;
-                    d = m[0][2] - v2; bd += d * d;
-                }
-                int bi = 0;
-                for (int j = 1; j < 4; j++) {
-                    float dd = 0;
-                    float d = m[j][0] - v0; dd += d * d;
-                    d = m[j][1] - v1; dd += d * d;
-                    d = m[j][2] - v2; dd += d * d;
-                    if (dd < bd) { bd = dd; bi = j; }
-                }
-                total += bd;
-                const float w = (float)s.w[k];
-                for (int j = 0; j < 4; j++)
-                    if (j == bi) {
-                        nm[j][0] += v0 * w;
-                        nm[j][1] += v1 * w;
-                        nm[j][2] += v2 * w;
-                        nw[j] += w;
+            for (int k = 0; k < (int)s.nu; k++) {
+                total += rdlf(bd, k);
+                const int j = rdl(bi, k);
+                const float a = rdlf(pr, k), b = rdlf(pg, k), c = rdlf(pb, k), w = rdlf(wf, k);
+                for (int q = 0; q < 4; q++)
+                    if (q == j) {
+                        nm[q][0] += a;
+                        nm[q][1] += b;
+                        nm[q][2] += c;
+                        nw[q] += w;
                     }
             }
             uint32_t hidx = 0;
@@ -549,67 +682,74 @@ __device__ bool try_median4(Block& s, const float* lo, const float* hi)
             }
         }
     }
-    bool improved = false;
+    uint32_t cl[6], ch[6];
+    int q = 0;
     for (int i = 0; i < 3; i++)
-        for (int j = i + 1; j < 4; j++) {
+        for (int j = i + 1; j < 4; j++, q++) {
             float a[3], b[3];
             for (int c = 0; c < 3; c++) {
                 a[c] = m[i][c] + s.mean[c];
                 b[c] = m[j][c] + s.mean[c];
             }
-            const uint32_t lo2 = pack565((int)floorf(.5f + a[0] * 31.0f), (int)floorf(.5f + a[1] * 63.0f),
-                                         (int)floorf(.5f + a[2] * 31.0f), 0);
-            const uint32_t hi2 = pack565((int)floorf(.5f + b[0] * 31.0f), (int)floorf(.5f + b[1] * 63.0f),
-                                         (int)floorf(.5f + b[2] * 31.0f), 0);
-            improved |= eval_canon(s, lo2, hi2);
+            cl[q] = pack565((int)floorf(.5f + a[0] * 31.0f), (int)floorf(.5f + a[1] * 63.0f),
+                            (int)floorf(.5f + a[2] * 31.0f), 0);
+            ch[q] = pack565((int)floorf(.5f + b[0] * 31.0f), (int)floorf(.5f + b[1] * 63.0f),
+                            (int)floorf(.5f + b[2] * 31.0f), 0);
         }
+    bool improved = eval_list(s, 6, 0, [&](uint32_t i, uint32_t& lo2, uint32_t& hi2) {
+        lo2 = i == 0 ? cl[0] : i == 1 ? cl[1] : i == 2 ? cl[2] : i == 3 ? cl[3] : i == 4 ? cl[4] : cl[5];
+        hi2 = i == 0 ? ch[0] : i == 1 ? ch[1] : i == 2 ? ch[2] : i == 3 ? ch[3] : i == 4 ? ch[4] : ch[5];
+        canon(lo2, hi2);
+        return true;
+    });
     improved |= refine(s, s.quality == 4 ? 1 : 0);
     return improved;
 }
 
-__device__ bool try_average_solid(Block& s)
+// try_average_block_as_solid (crn_dxt1.cpp:93-153).  A candidate after the
+// error reaches 0 is never taken, so the reference's "&& error" conditions
+// need no test.
+__device__ bool try_average_solid(Wb& s)
 {
-    uint64_t tr = 0, tg = 0, tb = 0;
-    uint32_t tw = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (k >= (int)s.nu) continue;
-        tw += s.w[k];
-        tr += (uint64_t)s.r[k] * s.w[k];
-        tg += (uint64_t)s.g[k] * s.w[k];
-        tb += (uint64_t)s.b[k] * s.w[k];
+    uint32_t tr = 0, tg = 0, tb = 0, tw = 0;
+    for (int k = 0; k < (int)s.nu; k++) {
+        const uint32_t w = colour_w(s, k);
+        tw += w;
+        tr += (uint32_t)colour_r(s, k) * w;
+        tg += (uint32_t)colour_g(s, k) * w;
+        tb += (uint32_t)colour_b(s, k) * w;
     }
     const uint32_t half = tw >> 1;
-    const uint32_t ar = (uint32_t)((tr + half) / tw), ag = (uint32_t)((tg + half) / tw), ab = (uint32_t)((tb + half) / tw);
-    bool improved = evaluate(s, (g_om5[ar][0] << 11) | (g_om6[ag][0] << 5) | g_om5[ab][0],
-                             (g_om5[ar][1] << 11) | (g_om6[ag][1] << 5) | g_om5[ab][1], 0);
-    if (s.use_alpha && s.best.err)
-        improved |= evaluate(s, (g_om5_3[ar][0] << 11) | (g_om6_3[ag][0] << 5) | g_om5_3[ab][0],
-                             (g_om5_3[ar][1] << 11) | (g_om6_3[ag][1] << 5) | g_om5_3[ab][1], 0);
-    if (s.quality == 4) {
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-            if (k >= (int)s.nu) continue;
-            const uint32_t r = (uint32_t)s.r[k], g = (uint32_t)s.g[k], b = (uint32_t)s.b[k];
-            if (r == ar && g == ag && b == ab) continue;
-            improved |= evaluate(s, (g_om5[r][0] << 11) | (g_om6[g][0] << 5) | g_om5[b][0],
-                                 (g_om5[r][1] << 11) | (g_om6[g][1] << 5) | g_om5[b][1], 0);
-            if (s.use_alpha && s.best.err)
-                improved |= evaluate(s, (g_om5_3[r][0] << 11) | (g_om6_3[g][0] << 5) | g_om5_3[b][0],
-                                     (g_om5_3[r][1] << 11) | (g_om6_3[g][1] << 5) | g_om5_3[b][1], 0);
+    const uint32_t ar = (tr + half) / tw, ag = (tg + half) / tw, ab = (tb + half) / tw;
+    const uint32_t per = s.use_alpha ? 2u : 1u;
+    const uint32_t slots = s.quality == 4 ? 1u + s.nu : 1u;
+    return eval_list(s, slots * per, 0, [&](uint32_t idx, uint32_t& lo, uint32_t& hi) {
+        const uint32_t slot = idx / per, three = idx % per;
+        const int from = slot ? (int)slot - 1 : 0;
+        const uint32_t kr = (uint32_t)__shfl(s.cr, from), kg = (uint32_t)__shfl(s.cg, from),
+                       kb = (uint32_t)__shfl(s.cb, from);
+        uint32_t r = ar, g = ag, b = ab;
+        if (slot) {
+            if (kr == ar && kg == ag && kb == ab) return false;
+            r = kr;
+            g = kg;
+            b = kb;
         }
-    }
-    return improved;
+        if (three) {
+            lo = ((uint32_t)g_om5_3[r][0] << 11) | ((uint32_t)g_om6_3[g][0] << 5) | g_om5_3[b][0];
+            hi = ((uint32_t)g_om5_3[r][1] << 11) | ((uint32_t)g_om6_3[g][1] << 5) | g_om5_3[b][1];
+        } else {
+            lo = ((uint32_t)g_om5[r][0] << 11) | ((uint32_t)g_om6[g][0] << 5) | g_om5[b][0];
+            hi = ((uint32_t)g_om5[r][1] << 11) | ((uint32_t)g_om6[g][1] << 5) | g_om5[b][1];
+        }
+        return true;
+    });
 }
 
-// compute_endpoint_component_errors + optimize_endpoint_comps (crn_dxt1.cpp:369-486).
-// For 16 texels every table entry and every partial sum is below
-// 16 * 255^2 * 4 < 2^32, so 32-bit arithmetic (wrapping in the intermediate
-// W*p*p - WP2*p + WPP steps like the reference's 64-bit version) is exact.
-// Per-channel error of one selector class as a function of the class's value
-// p: W p^2 - WP2 p + WPP (exact in 32 bits: at most 16 texels x 255^2), the
-// reference's compute_endpoint_component_errors tables (crn_dxt1.cpp:369-418)
-// evaluated where they are read instead of stored.
+// compute_endpoint_component_errors + optimize_endpoint_comps
+// (crn_dxt1.cpp:369-486).  Per selector class the channel error of value p
+// is W p^2 - WP2 p + WPP, below 2^32 for 16 texels (32-bit arithmetic that
+// wraps in the intermediate steps, like the reference's 64-bit one, is exact).
 struct QErr {
     uint32_t W[4], WP2[4], WPP[4];
     __device__ inline uint32_t at(int sl, uint32_t p) const { return W[sl] * p * p - WP2[sl] * p + WPP[sl]; }
@@ -620,15 +760,13 @@ __device__ inline uint32_t expand_comp(int comp, uint32_t c)
     return comp == 1 ? (((c << 2) | (c >> 4)) & 0xFF) : (((c << 3) | (c >> 2)) & 0xFF);
 }
 
-__device__ void comp_errors(const Block& s, int comp, QErr& q, uint32_t* brem)
+__device__ void comp_errors(const Wb& s, int comp, QErr& q, uint32_t* brem)
 {
     for (int k = 0; k < 4; k++) q.W[k] = q.WP2[k] = q.WPP[k] = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (k >= (int)s.nu) continue;
-        const uint32_t p = (uint32_t)(comp == 0 ? s.r[k] : comp == 1 ? s.g[k] : s.b[k]);
-        const uint32_t w = s.w[k];
-        const uint32_t sl = (s.best.sel >> (2 * k)) & 3u;
+    for (int k = 0; k < (int)s.nu; k++) {
+        const uint32_t p = (uint32_t)(comp == 0 ? colour_r(s, k) : comp == 1 ? colour_g(s, k) : colour_b(s, k));
+        const uint32_t w = colour_w(s, k);
+        const uint32_t sl = rdlu(s.bsel, k);
         for (uint32_t c = 0; c < 4; c++)
             if (c == sl) {
                 q.W[c] += w;
@@ -637,26 +775,17 @@ __device__ void comp_errors(const Block& s, int comp, QErr& q, uint32_t* brem)
             }
     }
     const uint32_t lim = comp == 1 ? 64 : 32;
-    for (int sl = 0; sl < 2; sl++) {
-        uint32_t best = q.WPP[sl];
-        for (uint32_t c = 1; c < lim; c++) {
-            const uint32_t e = q.at(sl, expand_comp(comp, c));
-            if (e < best) best = e;
-        }
-        brem[sl] = best;
-    }
+    const uint32_t l = (uint32_t)lane_id();
+    for (int sl = 0; sl < 2; sl++) brem[sl] = wave_min(l < lim ? q.at(sl, expand_comp(comp, l)) : kNone);
     for (int sl = 2; sl < 4; sl++) {
-        uint32_t best = q.WPP[sl];
-        for (uint32_t p = 1; p < 256; p++) {
-            const uint32_t e = q.at(sl, p);
-            if (e < best) best = e;
-        }
-        brem[sl] = best;
+        uint32_t v = q.at(sl, l);
+        for (uint32_t p = l + 64; p < 256; p += 64) v = min(v, q.at(sl, p));
+        brem[sl] = wave_min(v);
     }
     for (int sl = 3; sl; sl--) brem[sl - 1] += brem[sl];
 }
 
-__device__ void optimize_comps(Block& s)
+__device__ void optimize_comps(Wb& s)
 {
     compute_selectors(s);
     if (s.best.alpha || !s.best.err) return;
@@ -681,22 +810,34 @@ __device__ void optimize_comps(Block& s)
             if (e0 + brem[1] >= best) continue;
             if (comp == 0) low[0] = (int)c0; else if (comp == 1) low[1] = (int)c0; else low[2] = (int)c0;
             const uint32_t plow = pack565(low[0], low[1], low[2], 0);
-            for (uint32_t c1 = 0; c1 < lim; c1++) {
-                p1 = expand_comp(comp, c1);
-                uint32_t e = e0 + q.at(1, p1);
-                if (e + brem[2] >= best) continue;
-                e += q.at(2, (p0 * 2 + p1) / 3);
-                if (e + brem[3] >= best) continue;
-                e += q.at(3, (p0 + p1 * 2) / 3);
-                if (e >= best) continue;
-                if (comp == 0) high[0] = (int)c1; else if (comp == 1) high[1] = (int)c1; else high[2] = (int)c1;
-                if (!evaluate(s, plow, pack565(high[0], high[1], high[2], 0), 0)) continue;
+            // the c1 loop: every c1 that passes the bounds is a candidate, in order
+            uint32_t start = 0;
+            while (start < lim) {
+                const uint32_t c1 = (uint32_t)lane_id();
+                const uint32_t lp1 = expand_comp(comp, c1);
+                uint32_t e = e0 + q.at(1, lp1);
+                bool pass = c1 >= start && c1 < lim && !(e + brem[2] >= best);
+                e += q.at(2, (p0 * 2 + lp1) / 3);
+                pass = pass && !(e + brem[3] >= best);
+                e += q.at(3, (p0 + lp1 * 2) / 3);
+                pass = pass && !(e >= best);
+                if (!ballot(pass)) break;
+                int hh[3] = {high[0], high[1], high[2]};
+                hh[comp] = (int)c1;
+                const uint32_t phigh = pack565(hh[0], hh[1], hh[2], 0);
+                const uint32_t bnd = pass ? plane_bound(s, plow, phigh) : kNone;
+                int type = 0;
+                const uint32_t ce = cand_err(s, plow, phigh, 0, type);
+                const int l = take(s, pass, plow, phigh, 0, bnd, pass ? ce : kNone, type, true);
+                if (l < 0) break;
                 if (!s.best.err) return;
                 compute_selectors(s);
                 comp_errors(s, comp, q, brem);
+                p1 = expand_comp(comp, (uint32_t)l);
                 best = q.at(0, p0) + q.at(1, p1) + q.at(2, (p0 * 2 + p1) / 3) + q.at(3, (p0 + p1 * 2) / 3);
                 e0 = q.at(0, p0);
                 if (e0 + brem[1] >= best) break;
+                start = (uint32_t)l + 1;
             }
         }
     }
@@ -711,26 +852,29 @@ __device__ inline void lerp_color(int ar, int ag, int ab, int br, int bg, int bb
     b = clampi((int)(rd + ((float)ab + ((float)bb - (float)ab) * f)), 255);
 }
 
-__device__ void try_combinatorial(Block& s)
+// try_combinatorial_encoding (crn_dxt1.cpp:1886-1997): the packed colours on
+// lanes 0..np-1, every pair i < j in order, then again with the alternate
+// rounding where only an exact solution counts.
+__device__ void try_combinatorial(Wb& s)
 {
     if (s.nu < 2 || s.nu > 4) return;
-    uint32_t packed[64];
+    uint32_t pk = 0;
     uint32_t np = 0;
     auto add = [&](int r, int g, int b) {
         const uint32_t p = pack565(r, g, b, 1);
-        for (uint32_t k = 0; k < np; k++)
-            if (packed[k] == p) return;
-        packed[np++] = p;
+        if (ballot((uint32_t)lane_id() < np && pk == p)) return;
+        if ((uint32_t)lane_id() == np) pk = p;
+        np++;
     };
     for (int k = 0; k < 4; k++)
-        if (k < (int)s.nu) add(s.r[k], s.g[k], s.b[k]);
+        if (k < (int)s.nu) add(colour_r(s, k), colour_g(s, k), colour_b(s, k));
     if (s.nu == 2) {
         const float fs[10] = {2.0f, 3.0f, .5f, 1.5f, -1.0f, 2.0f, -.5f, .5f, -2.0f, -1.0f};
         for (int k = 0; k < 2; k++)
             for (int q = 0; q < 2; q++) {
                 const int a = q, b2 = q ^ 1;
-                const int ar = a ? s.r[1] : s.r[0], ag = a ? s.g[1] : s.g[0], ab = a ? s.b[1] : s.b[0];
-                const int br = b2 ? s.r[1] : s.r[0], bg = b2 ? s.g[1] : s.g[0], bb = b2 ? s.b[1] : s.b[0];
+                const int ar = colour_r(s, a), ag = colour_g(s, a), ab = colour_b(s, a);
+                const int br = colour_r(s, b2), bg = colour_g(s, b2), bb = colour_b(s, b2);
                 for (int f = 0; f < 10; f++) {
                     int r, g, b;
                     lerp_color(ar, ag, ab, br, bg, bb, fs[f], k, r, g, b);
@@ -742,12 +886,8 @@ __device__ void try_combinatorial(Block& s)
         for (int i = 0; i <= 2; i++)
             for (int j = 0; j <= 2; j++) {
                 if (i == j) continue;
-                const int ar = i == 0 ? s.r[0] : i == 1 ? s.r[1] : s.r[2];
-                const int ag = i == 0 ? s.g[0] : i == 1 ? s.g[1] : s.g[2];
-                const int ab = i == 0 ? s.b[0] : i == 1 ? s.b[1] : s.b[2];
-                const int br = j == 0 ? s.r[0] : j == 1 ? s.r[1] : s.r[2];
-                const int bg = j == 0 ? s.g[0] : j == 1 ? s.g[1] : s.g[2];
-                const int bb = j == 0 ? s.b[0] : j == 1 ? s.b[1] : s.b[2];
+                const int ar = colour_r(s, i), ag = colour_g(s, i), ab = colour_b(s, i);
+                const int br = colour_r(s, j), bg = colour_g(s, j), bb = colour_b(s, j);
                 for (int f = 0; f < 4; f++) {
                     int r, g, b;
                     lerp_color(ar, ag, ab, br, bg, bb, fs[f], 1, r, g, b);
@@ -755,20 +895,32 @@ __device__ void try_combinatorial(Block& s)
                 }
             }
     }
-    for (uint32_t i = 0; s.best.err && i + 1 < np; i++)
-        for (uint32_t j = i + 1; s.best.err && j < np; j++) evaluate(s, packed[i], packed[j], 0);
-    const uint64_t err = s.best.err;
+    const uint32_t npairs = np * (np - 1) / 2;
+    auto pair = [&](uint32_t t, uint32_t& lo, uint32_t& hi) {
+        uint32_t i = 0, row = np - 1;
+        while (t >= row) {
+            t -= row;
+            i++;
+            row--;
+        }
+        lo = (uint32_t)__shfl((int)pk, (int)i);
+        hi = (uint32_t)__shfl((int)pk, (int)(i + 1 + t));
+        return true;
+    };
+    eval_list(s, npairs, 0, pair);
+    const uint32_t err = s.best.err;
     if (err) s.best.err = 1;
-    for (uint32_t i = 0; s.best.err && i + 1 < np; i++)
-        for (uint32_t j = i + 1; s.best.err && j < np; j++) evaluate(s, packed[i], packed[j], 1);
+    eval_list(s, npairs, 1, pair);
     if (s.best.err) s.best.err = err;
 }
 
-__device__ void sweep(const int16_t* tab, uint32_t range, const float* base, const float (*ax)[3], uint32_t* out,
-                      uint32_t& n)
+// the probe sweep along the axis (crn_dxt1.cpp:840-892): the packed probes on
+// lanes 0..n-1
+__device__ uint32_t sweep(const int16_t* tab, uint32_t range, const float* base, const float (*ax)[3], uint32_t& n)
 {
     int prev[2] = {-1, -1};
     n = 0;
+    uint32_t list = 0;
     const float i0 = base[0] + .5f, i1 = base[1] + .5f, i2 = base[2] + .5f;
     for (uint32_t i = 0; i < range; i++) {
         const int ls = i ? 0 : 1;
@@ -780,55 +932,49 @@ __device__ void sweep(const int16_t* tab, uint32_t range, const float* base, con
             const int r = clampi((int)floorf(pr), 31), g = clampi((int)floorf(pg), 63), b = clampi((int)floorf(pb), 31);
             const int packed = b | (g << 5) | (r << 11);
             if (packed != prev[sg]) {
-                out[n++] = (uint32_t)packed;
+                if ((uint32_t)lane_id() == n) list = (uint32_t)packed;
+                n++;
                 prev[sg] = packed;
             }
         }
     }
+    return list;
 }
 
-__device__ void lattice(Block& s, int which)
+// the lattice moves of optimize_endpoints (crn_dxt1.cpp:905-1016) around the
+// low (which = 0) or high endpoint
+__device__ void lattice(Wb& s, int which)
 {
     int lr, lg, lb;
     unpack565(which ? s.best.hi : s.best.lo, 0, lr, lg, lb);
-    for (int i = 0; i < 26; i++) {
-        const int r = lr + k_adj[i][0];
-        if (r < 0 || r > 31) continue;
-        const int g = lg + k_adj[i][1];
-        if (g < 0 || g > 63) continue;
-        const int b = lb + k_adj[i][2];
-        if (b < 0 || b > 31) continue;
+    eval_chain(s, 26, [&](uint32_t i, uint32_t& lo, uint32_t& hi) {
+        const int r = lr + k_adj[i][0], g = lg + k_adj[i][1], b = lb + k_adj[i][2];
+        if (r < 0 || r > 31 || g < 0 || g > 63 || b < 0 || b > 31) return false;
         const uint32_t p = pack565(r, g, b, 0);
-        if (which) eval_canon(s, s.best.lo, p);
-        else eval_canon(s, p, s.best.hi);
-    }
+        lo = which ? s.best.lo : p;
+        hi = which ? p : s.best.hi;
+        canon(lo, hi);
+        return true;
+    });
     if (s.quality == 4) {
         unpack565(which ? s.best.hi : s.best.lo, 0, lr, lg, lb);
-        for (int a = 0; a < 3; a++) {
-            const int limit = a == 1 ? 63 : 31;
-            for (int sg = -2; sg <= 2; sg += 4) {
-                int c[3] = {lr, lg, lb};
-                const int q = c[a] + sg;
-                if (q < 0 || q > limit) continue;
-                c[a] = q;
-                const uint32_t p = pack565(c[0], c[1], c[2], 0);
-                if (which) eval_canon(s, s.best.lo, p);
-                else eval_canon(s, p, s.best.hi);
-            }
-        }
+        eval_chain(s, 6, [&](uint32_t i, uint32_t& lo, uint32_t& hi) {
+            const int a = (int)i >> 1, sg = (i & 1) ? 2 : -2;
+            int c[3] = {lr, lg, lb};
+            const int q = c[a] + sg;
+            if (q < 0 || q > (a == 1 ? 63 : 31)) return false;
+            c[a] = q;
+            const uint32_t p = pack565(c[0], c[1], c[2], 0);
+            lo = which ? s.best.lo : p;
+            hi = which ? p : s.best.hi;
+            canon(lo, hi);
+            return true;
+        });
     }
 }
 
-__device__ bool sel_all_equal(const Sol& b)
-{
-    if (!b.nsel) return false;
-    const uint32_t f = b.sel & 3u;
-    for (int k = 1; k < 16; k++)
-        if (k < b.nsel && ((b.sel >> (2 * k)) & 3u) != f) return false;
-    return true;
-}
-
-__device__ void optimize_endpoints(Block& s, float* lo, float* hi)
+// optimize_endpoints (crn_dxt1.cpp:703-1067)
+__device__ void optimize_endpoints(Wb& s, float* lo, float* hi)
 {
     float olo[3] = {lo[0], lo[1], lo[2]}, ohi[3] = {hi[0], hi[1], hi[2]};
     const int16_t* tab = k_uber_probe;
@@ -863,7 +1009,6 @@ __device__ void optimize_endpoints(Block& s, float* lo, float* hi)
         const float b = hi[c] * lim[c];
         hi[c] = b < 0.0f ? 0.0f : (b > lim[c] ? lim[c] : b);
     }
-    uint32_t plow[31], phigh[31];
     for (uint32_t pass = 0; pass < passes; pass++) {
         if (pass) {
             int r, g, b;
@@ -872,13 +1017,17 @@ __device__ void optimize_endpoints(Block& s, float* lo, float* hi)
             unpack565(s.best.hi, 0, r, g, b);
             hi[0] = (float)r; hi[1] = (float)g; hi[2] = (float)b;
         }
-        const uint64_t prev = s.best.err;
+        const uint32_t prev = s.best.err;
         if (!prev) break;
         uint32_t nl = 0, nh = 0;
-        sweep(tab, range, lo, ax, plow, nl);
-        sweep(tab, range, hi, ax, phigh, nh);
-        for (uint32_t i = 0; i < nl; i++)
-            for (uint32_t j = 0; j < nh; j++) eval_canon(s, plow[i], phigh[j]);
+        const uint32_t plow = sweep(tab, range, lo, ax, nl);
+        const uint32_t phigh = sweep(tab, range, hi, ax, nh);
+        eval_list(s, nl * nh, 0, [&](uint32_t k, uint32_t& a, uint32_t& b) {
+            a = (uint32_t)__shfl((int)plow, (int)(k / nh));
+            b = (uint32_t)__shfl((int)phigh, (int)(k % nh));
+            canon(a, b);
+            return true;
+        });
         if (s.quality >= 2) {
             lattice(s, 0);
             lattice(s, 1);
@@ -889,29 +1038,28 @@ __device__ void optimize_endpoints(Block& s, float* lo, float* hi)
     if (s.quality >= 2) {
         if (s.best.err && !s.have_alpha) {
             bool solid = false;
-            if (sel_all_equal(s.best)) solid = try_average_solid(s);
+            if (sel_all_equal(s)) solid = try_average_solid(s);
             if (!solid && s.quality == 4) optimize_comps(s);
         }
         if (s.quality == 4 && s.best.err) try_combinatorial(s);
     }
 }
 
-__device__ void compute_vectors(Block& s, const float* pw, float* meanw, float (*nw)[3])
+// compute_vectors (crn_dxt1.cpp:155-189): normalised colours (this lane's),
+// their weighted means summed in colour order; returns the weighted vector
+// of this lane's colour minus the weighted mean
+__device__ void compute_vectors(Wb& s, const float* pw, float& nwr, float& nwg, float& nwb)
 {
+    const float w = (float)s.cw;
+    const float n0 = (float)s.cr * 1.0f / 255.0f, n1 = (float)s.cg * 1.0f / 255.0f, n2 = (float)s.cb * 1.0f / 255.0f;
+    nwr = pw[0] * n0;
+    nwg = pw[1] * n1;
+    nwb = pw[2] * n2;
+    const float a0 = n0 * w, a1 = n1 * w, a2 = n2 * w, b0 = nwr * w, b1 = nwg * w, b2 = nwb * w;
     float m[3] = {0, 0, 0}, mw[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (k >= (int)s.nu) continue;
-        const float w = (float)s.w[k];
-        const float n0 = s.r[k] * 1.0f / 255.0f, n1 = s.g[k] * 1.0f / 255.0f, n2 = s.b[k] * 1.0f / 255.0f;
-        s.nr[k] = n0;
-        s.ng[k] = n1;
-        s.nb[k] = n2;
-        nw[k][0] = pw[0] * n0;
-        nw[k][1] = pw[1] * n1;
-        nw[k][2] = pw[2] * n2;
-        m[0] += n0 * w; m[1] += n1 * w; m[2] += n2 * w;
-        mw[0] += nw[k][0] * w; mw[1] += nw[k][1] * w; mw[2] += nw[k][2] * w;
+    for (int k = 0; k < (int)s.nu; k++) {
+        m[0] += rdlf(a0, k); m[1] += rdlf(a1, k); m[2] += rdlf(a2, k);
+        mw[0] += rdlf(b0, k); mw[1] += rdlf(b1, k); mw[2] += rdlf(b2, k);
     }
     if (s.total_w) {
         const float inv = 1.0f / (float)s.total_w;
@@ -920,56 +1068,48 @@ __device__ void compute_vectors(Block& s, const float* pw, float* meanw, float (
             mw[c] *= inv;
         }
     }
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (k >= (int)s.nu) continue;
-        s.nr[k] -= m[0];
-        s.ng[k] -= m[1];
-        s.nb[k] -= m[2];
-        nw[k][0] -= mw[0];
-        nw[k][1] -= mw[1];
-        nw[k][2] -= mw[2];
-    }
-    for (int c = 0; c < 3; c++) {
-        s.mean[c] = m[c];
-        meanw[c] = mw[c];
-    }
+    s.nr = n0 - m[0];
+    s.ng = n1 - m[1];
+    s.nb = n2 - m[2];
+    nwr -= mw[0];
+    nwg -= mw[1];
+    nwb -= mw[2];
+    for (int c = 0; c < 3; c++) s.mean[c] = m[c];
 }
 
-__device__ void compute_pca(Block& s, const float (*nc)[3], float* axis)
+// compute_pca (crn_dxt1.cpp:192-256) of the lanes' vectors (r, g, b)
+__device__ void compute_pca(const Wb& s, float r, float g, float b, float* axis)
 {
-    double cov[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (k >= (int)s.nu) continue;
-        const float r = nc[k][0], g = nc[k][1], b = nc[k][2];
-        if (s.w[k] > 1) {
-            const double w = s.w[k];
-            cov[0] += (double)(r * r) * w; cov[1] += (double)(r * g) * w; cov[2] += (double)(r * b) * w;
-            cov[3] += (double)(g * g) * w; cov[4] += (double)(g * b) * w; cov[5] += (double)(b * b) * w;
-        } else {
-            cov[0] += (double)(r * r); cov[1] += (double)(r * g); cov[2] += (double)(r * b);
-            cov[3] += (double)(g * g); cov[4] += (double)(g * b); cov[5] += (double)(b * b);
-        }
+    double t[6];
+    if (s.cw > 1) {
+        const double w = s.cw;
+        t[0] = (double)(r * r) * w; t[1] = (double)(r * g) * w; t[2] = (double)(r * b) * w;
+        t[3] = (double)(g * g) * w; t[4] = (double)(g * b) * w; t[5] = (double)(b * b) * w;
+    } else {
+        t[0] = (double)(r * r); t[1] = (double)(r * g); t[2] = (double)(r * b);
+        t[3] = (double)(g * g); t[4] = (double)(g * b); t[5] = (double)(b * b);
     }
+    double cov[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < (int)s.nu; k++)
+        for (int q = 0; q < 6; q++) cov[q] += rdld(t[q], k);
     double vr = (double).9f, vg = 1.0, vb = (double).7f;
     for (int it = 0; it < 8; it++) {
-        double r = vr * cov[0] + vg * cov[1] + vb * cov[2];
-        double g = vr * cov[1] + vg * cov[3] + vb * cov[4];
-        double b = vr * cov[2] + vg * cov[4] + vb * cov[5];
-        double m = fabs(r);
-        if (fabs(g) > m) m = fabs(g);
-        if (fabs(b) > m) m = fabs(b);
+        double x = vr * cov[0] + vg * cov[1] + vb * cov[2];
+        double y = vr * cov[1] + vg * cov[3] + vb * cov[4];
+        double z = vr * cov[2] + vg * cov[4] + vb * cov[5];
+        double m = fabs(x);
+        if (fabs(y) > m) m = fabs(y);
+        if (fabs(z) > m) m = fabs(z);
         if (m > 1e-10) {
             m = 1.0 / m;
-            r *= m;
-            g *= m;
-            b *= m;
+            x *= m;
+            y *= m;
+            z *= m;
         }
-        const double d = (vr - r) * (vr - r) + (vg - g) * (vg - g) + (vb - b) * (vb - b);
-        vr = r;
-        vg = g;
-        vb = b;
+        const double d = (vr - x) * (vr - x) + (vg - y) * (vg - y) + (vb - z) * (vb - z);
+        vr = x;
+        vg = y;
+        vb = z;
         if (it > 2 && d < 1e-8) break;
     }
     double len = vr * vr + vg * vg + vb * vb;
@@ -985,6 +1125,7 @@ __device__ void compute_pca(Block& s, const float (*nc)[3], float* axis)
     axis[2] = (float)(vb * len);
 }
 
+// ray_aabb against the unit cube (crn_intersect.h:44-120)
 __device__ bool ray_unit_box(float* coord, const float* org, const float* dir)
 {
     int quad[3];
@@ -1016,21 +1157,21 @@ __device__ bool ray_unit_box(float* coord, const float* org, const float* dir)
     return true;
 }
 
-__device__ void handle_multicolor(Block& s)
+// handle_multicolor_block (crn_dxt1.cpp:1069-1178)
+__device__ void handle_multicolor(Wb& s)
 {
     uint32_t npass = 1;
     float pw[3] = {1.0f, 1.0f, 1.0f};
     if (s.perceptual) {
+        const int l = (s.cr + s.cg + s.cb + 1) / 3;
+        const float lf = (float)l;
+        const float scale = (float)s.cw / (lf > 1.0f ? lf : 1.0f);
+        const float tr = scale * (float)s.cr, tb = scale * (float)s.cb, tl = (float)l;
         float red = 0, blue = 0, al = 0;
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-            if (k >= (int)s.nu) continue;
-            const int l = (s.r[k] + s.g[k] + s.b[k] + 1) / 3;
-            const float lf = (float)l;
-            const float scale = (float)s.w[k] / (lf > 1.0f ? lf : 1.0f);
-            red += scale * (float)s.r[k];
-            blue += scale * (float)s.b[k];
-            al += (float)l;
+        for (int k = 0; k < (int)s.nu; k++) {
+            red += rdlf(tr, k);
+            blue += rdlf(tb, k);
+            al += rdlf(tl, k);
         }
         red /= (float)s.total_w;
         blue /= (float)s.total_w;
@@ -1049,11 +1190,10 @@ __device__ void handle_multicolor(Block& s)
             for (int c = 0; c < 3; c++) pw[c] = lum[c] + (pw[c] - lum[c]) * p;
         }
     }
-    float nw[16][3];
-    float meanw[3];
     for (uint32_t pass = 0; pass < npass; pass++) {
-        compute_vectors(s, pw, meanw, nw);
-        compute_pca(s, nw, s.axis);
+        float nwr, nwg, nwb;
+        compute_vectors(s, pw, nwr, nwg, nwb);
+        compute_pca(s, nwr, nwg, nwb, s.axis);
         s.axis[0] /= pw[0];
         s.axis[1] /= pw[1];
         s.axis[2] /= pw[2];
@@ -1072,16 +1212,11 @@ __device__ void handle_multicolor(Block& s)
             else break;
         }
     }
-    float l = 1e+9f, h = -1e+9f;
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (k >= (int)s.nu) continue;
-        float d = s.nr[k] * s.axis[0];
-        d += s.ng[k] * s.axis[1];
-        d += s.nb[k] * s.axis[2];
-        l = d < l ? d : l;
-        h = h < d ? d : h;
-    }
+    float d = s.nr * s.axis[0];
+    d += s.ng * s.axis[1];
+    d += s.nb * s.axis[2];
+    const bool mine = (uint32_t)lane_id() < s.nu;
+    const float l = wave_minf(mine ? d : 1e+9f), h = wave_maxf(mine ? d : -1e+9f);
     float lo[3], hi[3];
     for (int c = 0; c < 3; c++) {
         lo[c] = s.mean[c] + l * s.axis[c];
@@ -1113,12 +1248,23 @@ struct Dxt1Out {
     uint32_t pad;
 };
 
-// compute_internal + return_solution (crn_dxt1.cpp:2081-2232, 263-365).
-// DXT1A (flags bit 4): texels with alpha below the threshold (flags bits
-// 8..15) leave the colour set and take selector 3.
-__device__ Dxt1Out encode_core(const uint32_t* px, int quality, int flags)
+// the 2-bit values v of lanes 0..15 as one selector word (texel i at bit 2i)
+__device__ inline uint32_t pack_selectors(uint32_t v)
 {
-    Block s;
+    const uint32_t b0 = (uint32_t)ballot(((v & 1u) != 0) && lane_id() < 16) & 0xFFFFu;
+    const uint32_t b1 = (uint32_t)ballot(((v & 2u) != 0) && lane_id() < 16) & 0xFFFFu;
+    uint32_t w = 0;
+    for (int i = 0; i < 16; i++) w |= (((b0 >> i) & 1u) << (2 * i)) | (((b1 >> i) & 1u) << (2 * i + 1));
+    return w;
+}
+
+// compute_internal + return_solution (crn_dxt1.cpp:2081-2232, 263-365) for
+// the block whose texel i is px on lane i (i < 16).  DXT1A (flags bit 4):
+// texels with alpha below the threshold (flags bits 8..15) leave the colour
+// set and take selector 3.
+__device__ Dxt1Out encode_core(uint32_t px, int quality, int flags, uint32_t* lds)
+{
+    Wb s;
     s.quality = quality;
     const int perceptual_req = flags & 1;
     s.use_alpha = (flags >> 1) & 1;
@@ -1128,46 +1274,47 @@ __device__ Dxt1Out encode_core(const uint32_t* px, int quality, int flags)
     const uint32_t thresh = ((uint32_t)flags >> 8) & 255u;
     s.hc = quality == 4 && !s.have_alpha && !s.force_alpha && !s.use_alpha && !s.grayscale;
     s.perceptual = perceptual_req && !s.grayscale;
-    s.nu = 0;
-    s.total_w = 0;
-    uint32_t keys[16];
-    uint32_t tmask = 0;  // transparent texels
-    uint64_t map64 = 0;  // 4 bits per texel: unique index
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        if (s.have_alpha && (px[i] >> 24) < thresh) {
-            tmask |= 1u << i;
-            continue;
-        }
-        const uint32_t key = px[i] | 0xFF000000u;
-        int k = -1;
-#pragma unroll
-        for (int u = 0; u < 16; u++)
-            if (k < 0 && u < (int)s.nu && keys[u] == key) k = u;
-        if (k < 0) {
-            k = (int)s.nu;
-#pragma unroll
-            for (int u = 0; u < 16; u++)
-                if (u == k) {
-                    keys[u] = key;
-                    const uint32_t R = key & 0xFF, G = (key >> 8) & 0xFF, B = (key >> 16) & 0xFF;
-                    s.r[u] = (int)R;
-                    s.g[u] = (int)G;
-                    s.b[u] = (int)B;
-                    s.pr[u] = (uint8_t)((R >> 3) + (((R & 7) > (R >> 5)) ? 1 : 0));
-                    s.pg[u] = (uint8_t)((G >> 2) + (((G & 3) > (G >> 6)) ? 1 : 0));
-                    s.pb[u] = (uint8_t)((B >> 3) + (((B & 7) > (B >> 5)) ? 1 : 0));
-                    s.w[u] = 1;
-                }
-            s.nu++;
-        } else {
-#pragma unroll
-            for (int u = 0; u < 16; u++)
-                if (u == k) s.w[u]++;
-        }
-        s.total_w++;
-        map64 |= (uint64_t)k << (4 * i);
+    s.planes = lds;
+    s.pend = lds + kPlaneWords;
+    s.pbound = lds + kPlaneWords + 128;
+    const int me = lane_id();
+    const bool texel = me < 16;
+    const bool opaque = texel && !(s.have_alpha && (px >> 24) < thresh);
+    const uint32_t key = px | 0xFF000000u;
+    const uint32_t omask = (uint32_t)ballot(opaque) & 0xFFFFu;
+    // first appearance of each colour among the opaque texels
+    int firstj = -1;
+    for (int j = 0; j < 16; j++) {
+        const uint32_t kj = rdlu(key, j);
+        if (firstj < 0 && j < me && ((omask >> j) & 1u) && kj == key) firstj = j;
     }
+    const bool first = opaque && firstj < 0;
+    const uint32_t fmask = (uint32_t)ballot(first) & 0xFFFFu;
+    s.nu = (uint32_t)__popc(fmask);
+    s.total_w = (uint32_t)__popc(omask);
+    const uint32_t tidx = (uint32_t)__popc(fmask & ((1u << (first ? me : (firstj < 0 ? 0 : firstj))) - 1u));
+    // colour k on lane k: the k-th first appearance, weight = its texels
+    int src = 0;
+    {
+        int cnt = 0;
+        for (int j = 0; j < 16; j++)
+            if ((fmask >> j) & 1u) {
+                if (cnt == me) src = j;
+                cnt++;
+            }
+    }
+    const uint32_t ckey = (uint32_t)__shfl((int)key, src);
+    const bool colour = (uint32_t)me < s.nu;
+    s.cr = colour ? (int)(ckey & 0xFF) : 0;
+    s.cg = colour ? (int)((ckey >> 8) & 0xFF) : 0;
+    s.cb = colour ? (int)((ckey >> 16) & 0xFF) : 0;
+    uint32_t w = 0;
+    for (int j = 0; j < 16; j++) {
+        const uint32_t tj = rdlu(tidx, j);
+        if (((omask >> j) & 1u) && tj == (uint32_t)me) w++;
+    }
+    s.cw = colour ? w : 0u;
+    s.nr = s.ng = s.nb = 0.0f;
     Dxt1Out o;
     o.pad = 0;
     if (!s.nu) {
@@ -1178,19 +1325,26 @@ __device__ Dxt1Out encode_core(const uint32_t* px, int quality, int flags)
         o.flags = 1;
         return o;
     }
+    build_planes(s);
     s.best.lo = s.best.hi = 0;
-    s.best.nsel = 0;
-    s.best.err = ~0ull;
-    s.best.alpha = 0;
-    s.best.sel = 0;
-    s.best.alt = s.best.enforce = s.best.enforced = 0;
-    s.trial = s.best;
-    if (s.nu == 1 && !tmask) {
-        const uint32_t r = (uint32_t)s.r[0], g = (uint32_t)s.g[0], b = (uint32_t)s.b[0];
-        evaluate(s, (g_om5[r][0] << 11) | (g_om6[g][0] << 5) | g_om5[b][0], (g_om5[r][1] << 11) | (g_om6[g][1] << 5) | g_om5[b][1], 0);
-        if (s.use_alpha && s.best.err)
-            evaluate(s, (g_om5_3[r][0] << 11) | (g_om6_3[g][0] << 5) | g_om5_3[b][0],
-                     (g_om5_3[r][1] << 11) | (g_om6_3[g][1] << 5) | g_om5_3[b][1], 0);
+    s.best.err = kNone;
+    s.best.alpha = s.best.alt = s.best.enforce = s.best.enforced = 0;
+    s.best.sel_valid = false;
+    s.bsel = 0;
+    s.mean[0] = s.mean[1] = s.mean[2] = 0.0f;
+    s.axis[0] = s.axis[1] = s.axis[2] = 0.0f;
+    if (s.nu == 1 && omask == 0xFFFFu) {
+        const uint32_t r = (uint32_t)colour_r(s, 0), g = (uint32_t)colour_g(s, 0), b = (uint32_t)colour_b(s, 0);
+        eval_list(s, s.use_alpha ? 2u : 1u, 0, [&](uint32_t i, uint32_t& lo, uint32_t& hi) {
+            if (i) {
+                lo = ((uint32_t)g_om5_3[r][0] << 11) | ((uint32_t)g_om6_3[g][0] << 5) | g_om5_3[b][0];
+                hi = ((uint32_t)g_om5_3[r][1] << 11) | ((uint32_t)g_om6_3[g][1] << 5) | g_om5_3[b][1];
+            } else {
+                lo = ((uint32_t)g_om5[r][0] << 11) | ((uint32_t)g_om6[g][0] << 5) | g_om5[b][0];
+                hi = ((uint32_t)g_om5[r][1] << 11) | ((uint32_t)g_om6[g][1] << 5) | g_om5[b][1];
+            }
+            return true;
+        });
     } else {
         handle_multicolor(s);
     }
@@ -1199,16 +1353,10 @@ __device__ Dxt1Out encode_core(const uint32_t* px, int quality, int flags)
     o.error = s.best.err;
     const uint32_t lo = inv ? s.best.hi : s.best.lo, hi = inv ? s.best.lo : s.best.hi;
     o.lo_hi = lo | (hi << 16);
-    uint32_t sel = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const uint32_t u = (uint32_t)((map64 >> (4 * i)) & 15u);
-        uint32_t v = (s.best.sel >> (2 * u)) & 3u;
-        if (inv) v = s.best.alpha ? (v < 2 ? v ^ 1u : v) : (v ^ 1u);
-        if ((tmask >> i) & 1u) v = 3u;
-        sel |= v << (2 * i);
-    }
-    o.selectors = sel;
+    uint32_t v = (uint32_t)__shfl((int)s.bsel, (int)tidx);
+    if (inv) v = s.best.alpha ? (v < 2 ? v ^ 1u : v) : (v ^ 1u);
+    if (!opaque) v = 3u;
+    o.selectors = pack_selectors(v);
     o.flags = (uint32_t)s.best.alpha | ((uint32_t)inv << 1) | ((uint32_t)s.best.alt << 2) |
               ((uint32_t)s.best.enforce << 3) | ((uint32_t)s.best.enforced << 4);
     return o;
@@ -1216,15 +1364,7 @@ __device__ Dxt1Out encode_core(const uint32_t* px, int quality, int flags)
 
 __device__ inline uint32_t metric(int perceptual, int grayscale, uint32_t p, int r, int g, int b)
 {
-    const int pr = (int)(p & 255u), pg = (int)((p >> 8) & 255u), pb = (int)((p >> 16) & 255u);
-    const int dr = pr - r, dg = pg - g, db = pb - b;
-    if (perceptual) return (uint32_t)(dr * dr * 8 + dg * dg * 25 + db * db);
-    if (grayscale) {
-        const int y0 = (pr * 19595 + pg * 38470 + pb * 7471 + 32768) >> 16;
-        const int y1 = (r * 19595 + g * 38470 + b * 7471 + 32768) >> 16;
-        return (uint32_t)((y0 - y1) * (y0 - y1));
-    }
-    return (uint32_t)(dr * dr + dg * dg + db * db);
+    return dist(perceptual, grayscale, (int)(p & 255u), (int)((p >> 8) & 255u), (int)((p >> 16) & 255u), r, g, b);
 }
 
 // dxt1_endpoint_optimizer::compute (crn_dxt1.cpp:2234-2246).  flags bit 6
@@ -1234,76 +1374,55 @@ __device__ inline uint32_t metric(int perceptual, int grayscale, uint32_t p, int
 // try_alpha_as_black_optimization (:2001-2079): texels whose r, g and b are
 // all <= 4 are made transparent, the block is searched again as DXT1A, and
 // that 3-colour block replaces the first result if its error, counting those
-// texels against black, is lower.  One copy of the search serves both runs.
-__device__ Dxt1Out encode_block(const uint32_t* px, int quality, int flags)
+// texels against black, is lower.
+__device__ Dxt1Out encode_block(uint32_t px, int quality, int flags, uint32_t* lds)
 {
+    const bool texel = lane_id() < 16;
     if ((flags >> 6) & 1) {
         const uint32_t thresh = ((uint32_t)flags >> 8) & 255u;
-        bool any = false;
-#pragma unroll
-        for (int i = 0; i < 16; i++) any |= (px[i] >> 24) < thresh;
+        const bool any = ballot(texel && (px >> 24) < thresh) != 0;
         flags = (flags & ~(64 | 16)) | (any ? 16 : 0);
     }
-    uint32_t p[16];
-    uint32_t dark = 0;
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        p[i] = px[i];
-        if ((p[i] & 255u) <= 4 && ((p[i] >> 8) & 255u) <= 4 && ((p[i] >> 16) & 255u) <= 4) dark |= 1u << i;
-    }
+    const bool isdark = texel && (px & 255u) <= 4 && ((px >> 8) & 255u) <= 4 && ((px >> 16) & 255u) <= 4;
+    const uint32_t dark = (uint32_t)ballot(isdark) & 0xFFFFu;
     const bool black = ((flags >> 1) & 1) && ((flags >> 5) & 1) && !((flags >> 4) & 1) && dark && dark != 0xFFFFu;
-    Dxt1Out first;
-    for (int pass = 0;; pass++) {
-        const Dxt1Out o = encode_core(p, quality, pass ? (flags | 16) : flags);
-        if (!pass) {
-            first = o;
-            if (!black) return first;
-#pragma unroll
-            for (int i = 0; i < 16; i++)
-                if ((dark >> i) & 1u) p[i] &= 0x00FFFFFFu;
-            continue;
-        }
-        // get_block_colors3 of the trial's endpoints, entry 3 black
-        int c0r, c0g, c0b, c1r, c1g, c1b;
-        unpack565(o.lo_hi & 0xFFFFu, 1, c0r, c0g, c0b);
-        unpack565(o.lo_hi >> 16, 1, c1r, c1g, c1b);
-        const int c2r = (c0r + c1r) >> 1, c2g = (c0g + c1g) >> 1, c2b = (c0b + c1b) >> 1;
-        const int gray = (flags >> 2) & 1, perc = (flags & 1) && !gray;
-        uint64_t err = 0;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint32_t v = (o.selectors >> (2 * i)) & 3u;
-            const int r = v == 0 ? c0r : v == 1 ? c1r : v == 2 ? c2r : 0;
-            const int g = v == 0 ? c0g : v == 1 ? c1g : v == 2 ? c2g : 0;
-            const int b = v == 0 ? c0b : v == 1 ? c1b : v == 2 ? c2b : 0;
-            err += metric(perc, gray, p[i], r, g, b);
-        }
-        if (err < first.error) {
-            first.error = err;
-            first.lo_hi = o.lo_hi;
-            first.selectors = o.selectors;
-            first.flags |= 1u;
-        }
-        return first;
+    Dxt1Out first = encode_core(px, quality, flags, lds);
+    if (!black) return first;
+    const uint32_t p2 = isdark ? (px & 0x00FFFFFFu) : px;
+    const Dxt1Out o = encode_core(p2, quality, flags | 16, lds);
+    // get_block_colors3 of the trial's endpoints, entry 3 black
+    int c0r, c0g, c0b, c1r, c1g, c1b;
+    unpack565(o.lo_hi & 0xFFFFu, 1, c0r, c0g, c0b);
+    unpack565(o.lo_hi >> 16, 1, c1r, c1g, c1b);
+    const int c2r = (c0r + c1r) >> 1, c2g = (c0g + c1g) >> 1, c2b = (c0b + c1b) >> 1;
+    const int gray = (flags >> 2) & 1, perc = (flags & 1) && !gray;
+    const uint32_t v = (o.selectors >> (2 * (lane_id() & 15))) & 3u;
+    const int r = v == 0 ? c0r : v == 1 ? c1r : v == 2 ? c2r : 0;
+    const int g = v == 0 ? c0g : v == 1 ? c1g : v == 2 ? c2g : 0;
+    const int b = v == 0 ? c0b : v == 1 ? c1b : v == 2 ? c2b : 0;
+    uint32_t e = texel ? metric(perc, gray, p2, r, g, b) : 0u;
+    for (int off = 8; off; off >>= 1) e += (uint32_t)__shfl_xor((int)e, off, 16);
+    const uint64_t err = rdlu(e, 0);
+    if (err < first.error) {
+        first.error = err;
+        first.lo_hi = o.lo_hi;
+        first.selectors = o.selectors;
+        first.flags |= 1u;
     }
+    return first;
 }
 
-__global__ void __launch_bounds__(256, 2) dxt1_blocks_kernel(const uint32_t* __restrict__ px, uint32_t nblocks, int quality,
-                                                         int flags, Dxt1Out* __restrict__ out)
+constexpr uint32_t kWaves = 4;   // waves (blocks) per workgroup of the fixed-shape kernels
+
+__global__ void __launch_bounds__(256) dxt1_blocks_kernel(const uint32_t* __restrict__ px, uint32_t nblocks, int quality,
+                                                          int flags, Dxt1Out* __restrict__ out)
 {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t lds[kWaves][kLdsWords];
+    const uint32_t w = threadIdx.x >> 6, b = blockIdx.x * (blockDim.x >> 6) + w;
     if (b >= nblocks) return;
-    uint32_t p[16];
-    const uint4* src = reinterpret_cast<const uint4*>(px + 16 * (size_t)b);
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const uint4 v = src[q];
-        p[4 * q] = v.x;
-        p[4 * q + 1] = v.y;
-        p[4 * q + 2] = v.z;
-        p[4 * q + 3] = v.w;
-    }
-    out[b] = encode_block(p, quality, flags);
+    const uint32_t p = lane_id() < 16 ? px[16 * (size_t)b + lane_id()] : 0u;
+    const Dxt1Out o = encode_block(p, quality, flags, lds[w]);
+    if (lane_id() == 0) out[b] = o;
 }
 
 // Whole image, as dxt_image::init_task and set_block_pixels do for cColorDXT1
@@ -1311,46 +1430,42 @@ __global__ void __launch_bounds__(256, 2) dxt1_blocks_kernel(const uint32_t* __r
 // texels with edge clamp, the search, then the 8-byte dxt1_block (low colour,
 // high colour, 2-bit selectors row by row, texel (x, y) at bit 2 (x + 4 y) of
 // the last four bytes) at blocks[i * stride]; errors (optional) = m_error.
-__global__ void __launch_bounds__(256, 2) dxt1_image_kernel(const uint32_t* __restrict__ rgba, uint32_t width,
-                                                           uint32_t height, uint32_t blocks_x, uint32_t nblocks,
-                                                           int quality, int flags, uint64_t* __restrict__ out,
-                                                           uint32_t stride, uint64_t* __restrict__ err)
+__global__ void __launch_bounds__(256) dxt1_image_kernel(const uint32_t* __restrict__ rgba, uint32_t width,
+                                                         uint32_t height, uint32_t blocks_x, uint32_t nblocks,
+                                                         int quality, int flags, uint64_t* __restrict__ out,
+                                                         uint32_t stride, uint64_t* __restrict__ err)
 {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t lds[kWaves][kLdsWords];
+    const uint32_t w = threadIdx.x >> 6, b = blockIdx.x * (blockDim.x >> 6) + w;
     if (b >= nblocks) return;
     const uint32_t bx = b % blocks_x, by = b / blocks_x;
-    uint32_t p[16];
-#pragma unroll
-    for (int y = 0; y < 4; y++) {
-        const uint32_t sy = min(by * 4 + y, height - 1);
-#pragma unroll
-        for (int x = 0; x < 4; x++) p[4 * y + x] = rgba[(size_t)sy * width + min(bx * 4 + x, width - 1)];
+    const uint32_t l = (uint32_t)lane_id() & 15u;
+    const uint32_t p = rgba[(size_t)min(by * 4 + (l >> 2), height - 1) * width + min(bx * 4 + (l & 3), width - 1)];
+    const Dxt1Out o = encode_block(p, quality, flags, lds[w]);
+    if (lane_id() == 0) {
+        out[(size_t)b * stride] = (uint64_t)o.lo_hi | ((uint64_t)o.selectors << 32);
+        if (err) err[b] = o.error;
     }
-    const Dxt1Out o = encode_block(p, quality, flags);
-    out[(size_t)b * stride] = (uint64_t)o.lo_hi | ((uint64_t)o.selectors << 32);
-    if (err) err[b] = o.error;
 }
 
 // Texture-set encode: colour elements of many level images in one flat
-// order, each thread walking it with a grid stride (a persistent grid when
-// the launch is smaller than the work).
-__global__ void __launch_bounds__(256, 2) dxt1_set_kernel(const crn2::SetJob* __restrict__ jobs, uint32_t njobs,
-                                                         uint64_t begin, uint64_t end, int quality)
+// order, each wave walking it with a grid stride (a persistent grid when the
+// launch is smaller than the work).
+__global__ void __launch_bounds__(256) dxt1_set_kernel(const crn2::SetJob* __restrict__ jobs, uint32_t njobs,
+                                                       uint64_t begin, uint64_t end, int quality)
 {
-    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = begin + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < end; i += step) {
+    __shared__ uint32_t lds[kWaves][kLdsWords];
+    const uint32_t w = threadIdx.x >> 6, wpg = blockDim.x >> 6;
+    const uint64_t step = (uint64_t)gridDim.x * wpg;
+    for (uint64_t i = begin + (uint64_t)blockIdx.x * wpg + w; i < end; i += step) {
         const crn2::SetJob j = jobs[crn2::set_job_of(jobs, njobs, i)];
         const uint32_t b = (uint32_t)(i - j.first);
         const uint32_t bx = b % j.blocks_x, by = b / j.blocks_x;
-        uint32_t p[16];
-#pragma unroll
-        for (int y = 0; y < 4; y++) {
-            const uint32_t sy = min(by * 4 + y, j.height - 1);
-#pragma unroll
-            for (int x = 0; x < 4; x++) p[4 * y + x] = j.rgba[(size_t)sy * j.width + min(bx * 4 + x, j.width - 1)];
-        }
-        const Dxt1Out o = encode_block(p, quality, (int)j.flags);
-        j.out[(size_t)b * j.stride] = (uint64_t)o.lo_hi | ((uint64_t)o.selectors << 32);
+        const uint32_t l = (uint32_t)lane_id() & 15u;
+        const uint32_t p =
+            j.rgba[(size_t)min(by * 4 + (l >> 2), j.height - 1) * j.width + min(bx * 4 + (l & 3), j.width - 1)];
+        const Dxt1Out o = encode_block(p, quality, (int)j.flags, lds[w]);
+        if (lane_id() == 0) j.out[(size_t)b * j.stride] = (uint64_t)o.lo_hi | ((uint64_t)o.selectors << 32);
     }
 }
 
@@ -1418,9 +1533,9 @@ extern "C" int crn2_dxt1_blocks_device(const void* rgba, uint32_t nblocks, int q
     const int rc = dxt1_tables_ready();
     if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    // launch shapes (threads per workgroup, one block per lane; identical results): 64, 128, 256
+    // launch shapes (one block per wavefront; 1, 2 or 4 wavefronts per workgroup; identical results)
     auto launch = [&](int shape) -> hipError_t {
-        const uint32_t tb = 64u << shape, nb = (nblocks + tb - 1) / tb;
+        const uint32_t tb = 64u << shape, per = tb / 64, nb = (nblocks + per - 1) / per;
         hipLaunchKernelGGL(dxt1_blocks_kernel, dim3(nb), dim3(tb), 0, s, static_cast<const uint32_t*>(rgba), nblocks,
                            quality, flags, static_cast<Dxt1Out*>(out));
         return hipGetLastError();
@@ -1440,8 +1555,8 @@ extern "C" int crn2_encode_dxt1_image_device(const void* rgba, uint32_t width, u
     if (n > 0xFFFFFFFFull) return CRN2_EINVAL;
     const int rc = dxt1_tables_ready();
     if (rc) return rc;
-    const uint32_t tb = 256;
-    hipLaunchKernelGGL(dxt1_image_kernel, dim3((uint32_t)((n + tb - 1) / tb)), dim3(tb), 0,
+    const uint32_t tb = 64 * kWaves;
+    hipLaunchKernelGGL(dxt1_image_kernel, dim3((uint32_t)((n + kWaves - 1) / kWaves)), dim3(tb), 0,
                        static_cast<hipStream_t>(stream), static_cast<const uint32_t*>(rgba), width, height,
                        (uint32_t)bx, (uint32_t)n, quality, flags, static_cast<uint64_t*>(blocks), block_stride_u64,
                        static_cast<uint64_t*>(errors));

@@ -375,7 +375,9 @@ int set_launch(const crn2_encode_set* s, int t, int shape, uint64_t begin, uint6
 {
     const int cus = s->cus;
     const SetShape sh = kSetShapes[shape];
-    const uint64_t n = end - begin, full = (n + sh.tb - 1) / sh.tb;
+    // colour elements take one wavefront each, alpha elements one lane
+    const uint64_t per = t == 0 ? sh.tb / 64 : sh.tb;
+    const uint64_t n = end - begin, full = (n + per - 1) / per;
     const uint64_t cap = sh.wg_per_cu ? (uint64_t)cus * sh.wg_per_cu : full;
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(full, std::min<uint64_t>(cap, 1u << 30)));
     return t == 0 ? crn2_dxt1_set_launch(s->d_jobs[0], s->njobs[0], begin, end, s->quality, grid, sh.tb, st)
