@@ -158,6 +158,7 @@ struct Wb {
     uint32_t nu, total_w;
     int cr, cg, cb;          // lane k: colour k (0 past nu)
     uint32_t cw;             // lane k: its weight (0 past nu)
+    uint32_t cpk;            // lane k: r | g << 8 | b << 16 | weight << 24 (0 past nu)
     float nr, ng, nb;        // lane k: normalised colour minus the mean
     float mean[3], axis[3];
     Best best;
@@ -214,13 +215,22 @@ __device__ inline uint32_t plane_bound(const Wb& s, uint32_t lo, uint32_t hi)
     return re + ge + be;
 }
 
-// Error of candidate (lo, hi, alt) on this lane under the block's evaluator,
-// and the block type it would take: evaluate_solution_hc_*
-// (crn_dxt1.cpp:1759-1835), _uber (:1370-1592: 4-colour, then a 3-colour
-// block only if strictly better) or _fast (:1594-1757, below Better).  Each
-// evaluator's early exits only decide whether the candidate is kept, which
-// the caller decides from the full error.
-__device__ uint32_t cand_err(const Wb& s, uint32_t lo, uint32_t hi, int alt, int& type)
+// lanes per candidate for a batch of m candidates: the colours of a
+// candidate are split over its lanes when the batch leaves lanes free
+__device__ inline uint32_t lanes_per(uint32_t m)
+{
+    return m <= 4 ? 16u : (m <= 8 ? 8u : (m <= 16 ? 4u : (m <= 32 ? 2u : 1u)));
+}
+
+// Error of candidate (lo, hi, alt) under the block's evaluator, and the block
+// type it would take: evaluate_solution_hc_* (crn_dxt1.cpp:1759-1835), _uber
+// (:1370-1592: 4-colour, then a 3-colour block only if strictly better) or
+// _fast (:1594-1757, below Better).  The candidate is held by g aligned lanes
+// (g a power of two <= 16), each summing the colours k = its index mod g; the
+// integer sums are then added over the g lanes.  Each evaluator's early exits
+// only decide whether the candidate is kept, which the caller decides from
+// the full error.
+__device__ uint32_t cand_err(const Wb& s, uint32_t lo, uint32_t hi, int alt, int& type, uint32_t g)
 {
     int c0r, c0g, c0b, c1r, c1g, c1b;
     unpack565(lo, 1, c0r, c0g, c0b);
@@ -229,31 +239,28 @@ __device__ uint32_t cand_err(const Wb& s, uint32_t lo, uint32_t hi, int alt, int
     const int c3r = (c1r * 2 + c0r + alt) / 3, c3g = (c1g * 2 + c0g + alt) / 3, c3b = (c1b * 2 + c0b + alt) / 3;
     const int h2r = (c0r + c1r + alt) >> 1, h2g = (c0g + c1g + alt) >> 1, h2b = (c0b + c1b + alt) >> 1;
     const int P = s.perceptual, G = s.grayscale;
+    const uint32_t sub = (uint32_t)lane_id() & (g - 1), kend = ((s.nu + g - 1) / g) * g;
     type = 0;
-    if (s.hc) {
-        uint32_t e = 0;
-        for (int k = 0; k < (int)s.nu; k++) {
-            const int r = colour_r(s, k), g = colour_g(s, k), b = colour_b(s, k);
-            const uint32_t e0 = dist(P, G, r, g, b, c0r, c0g, c0b), e1 = dist(P, G, r, g, b, c1r, c1g, c1b);
-            const uint32_t e2 = dist(P, G, r, g, b, c2r, c2g, c2b), e3 = dist(P, G, r, g, b, c3r, c3g, c3b);
-            e += min(min(e0, e1), min(e2, e3)) * colour_w(s, k);
-        }
-        return e;
-    }
     int first = 0, last = 1;
-    if (s.have_alpha || s.force_alpha) first = 1;
-    else if (!s.use_alpha) last = 0;
+    if (s.hc) {
+        first = last = 0;
+    } else if (s.have_alpha || s.force_alpha) {
+        first = 1;
+    } else if (!s.use_alpha) {
+        last = 0;
+    }
     uint32_t e4 = 0, e3 = 0;
-    if (s.quality >= 3) {
-        for (int k = 0; k < (int)s.nu; k++) {
-            const int r = colour_r(s, k), g = colour_g(s, k), b = colour_b(s, k);
-            const uint32_t w = colour_w(s, k);
-            const uint32_t d0 = dist(P, G, r, g, b, c0r, c0g, c0b), d1 = dist(P, G, r, g, b, c1r, c1g, c1b);
+    if (s.hc || s.quality >= 3) {
+        for (uint32_t k = sub; k < kend; k += g) {
+            const uint32_t pk = (uint32_t)__shfl((int)s.cpk, (int)k);
+            const int r = (int)(pk & 255u), gg = (int)((pk >> 8) & 255u), b = (int)((pk >> 16) & 255u);
+            const uint32_t w = pk >> 24;
+            const uint32_t d0 = dist(P, G, r, gg, b, c0r, c0g, c0b), d1 = dist(P, G, r, gg, b, c1r, c1g, c1b);
             if (first == 0) {
-                const uint32_t d2 = dist(P, G, r, g, b, c2r, c2g, c2b), d3 = dist(P, G, r, g, b, c3r, c3g, c3b);
+                const uint32_t d2 = dist(P, G, r, gg, b, c2r, c2g, c2b), d3 = dist(P, G, r, gg, b, c3r, c3g, c3b);
                 e4 += min(min(d0, d1), min(d2, d3)) * w;
             }
-            if (last == 1) e3 += min(min(d0, d1), dist(P, G, r, g, b, h2r, h2g, h2b)) * w;
+            if (last == 1) e3 += min(min(d0, d1), dist(P, G, r, gg, b, h2r, h2g, h2b)) * w;
         }
     } else {
         int vr = c1r - c0r, vg = c1g - c0g, vb = c1b - c0b;
@@ -265,10 +272,11 @@ __device__ uint32_t cand_err(const Wb& s, uint32_t lo, uint32_t hi, int alt, int
         const int s2 = c2r * vr + c2g * vg + c2b * vb, s3 = c3r * vr + c3g * vg + c3b * vb;
         const int h2 = h2r * vr + h2g * vg + h2b * vb;
         const int dr = vr * 2, dg = vg * 2, db = vb * 2;
-        for (int k = 0; k < (int)s.nu; k++) {
-            const int r = colour_r(s, k), g = colour_g(s, k), b = colour_b(s, k);
-            const uint32_t w = colour_w(s, k);
-            const int dot = r * dr + g * dg + b * db;
+        for (uint32_t k = sub; k < kend; k += g) {
+            const uint32_t pk = (uint32_t)__shfl((int)s.cpk, (int)k);
+            const int r = (int)(pk & 255u), gg = (int)((pk >> 8) & 255u), b = (int)((pk >> 16) & 255u);
+            const uint32_t w = pk >> 24;
+            const int dot = r * dr + gg * dg + b * db;
             if (first == 0) {
                 int pr, pg, pb;
                 if (dot >= s3 + s2) {
@@ -278,16 +286,20 @@ __device__ uint32_t cand_err(const Wb& s, uint32_t lo, uint32_t hi, int alt, int
                     const bool zero = dot < s2 + s0;
                     pr = zero ? c0r : c2r; pg = zero ? c0g : c2g; pb = zero ? c0b : c2b;
                 }
-                e4 += dist(P, G, r, g, b, pr, pg, pb) * w;
+                e4 += dist(P, G, r, gg, b, pr, pg, pb) * w;
             }
             if (last == 1) {
                 int pr, pg, pb;
                 if (dot < s0 + h2) { pr = c0r; pg = c0g; pb = c0b; }
                 else if (dot < h2 + s1) { pr = h2r; pg = h2g; pb = h2b; }
                 else { pr = c1r; pg = c1g; pb = c1b; }
-                e3 += dist(P, G, r, g, b, pr, pg, pb) * w;
+                e3 += dist(P, G, r, gg, b, pr, pg, pb) * w;
             }
         }
+    }
+    for (uint32_t o = 1; o < g; o <<= 1) {
+        e4 += (uint32_t)__shfl_xor((int)e4, (int)o);
+        e3 += (uint32_t)__shfl_xor((int)e3, (int)o);
     }
     if (first == 1) {
         type = 1;
@@ -397,12 +409,14 @@ __device__ int take(Wb& s, bool valid, uint32_t lo, uint32_t hi, int alt, uint32
 // Evaluates the pending candidates (the first cnt entries) in order.
 __device__ bool flush(Wb& s, int alt, uint32_t cnt)
 {
-    const bool v = (uint32_t)lane_id() < cnt;
-    const uint32_t pk = v ? s.pend[lane_id()] : 0u, bnd = v ? s.pbound[lane_id()] : kNone;
+    const uint32_t g = lanes_per(cnt), c = (uint32_t)lane_id() / g;
+    const bool v = c < cnt;
+    const uint32_t pk = v ? s.pend[c] : 0u, bnd = v ? s.pbound[c] : kNone;
     const uint32_t lo = pk & 0xFFFFu, hi = pk >> 16;
     int type = 0;
-    const uint32_t e = cand_err(s, lo, hi, alt, type);
-    return take(s, v, lo, hi, alt, bnd, v ? e : kNone, type, false) >= 0;
+    const uint32_t e = cand_err(s, lo, hi, alt, type, g);
+    const bool rep = v && ((uint32_t)lane_id() & (g - 1)) == 0;
+    return take(s, rep, lo, hi, alt, bnd, rep ? e : kNone, type, false) >= 0;
 }
 
 // n candidates that do not depend on the search state, candidate i =
@@ -463,18 +477,19 @@ __device__ bool eval_chain(Wb& s, uint32_t n, Gen gen)
     bool any = false;
     uint32_t start = 0;
     while (start < n) {
-        const uint32_t i = (uint32_t)lane_id();
+        const uint32_t g = lanes_per(n - start), i = start + (uint32_t)lane_id() / g;
         uint32_t lo = 0, hi = 0;
         const bool made = gen(i < n ? i : 0u, lo, hi);
-        const bool ok = i >= start && i < n && made;
+        const bool ok = i < n && made;
         const uint32_t bnd = ok ? plane_bound(s, lo, hi) : kNone;
-        if (!ballot(ok && bnd < s.best.err)) break;
+        const bool rep = ok && ((uint32_t)lane_id() & (g - 1)) == 0;
+        if (!ballot(rep && bnd < s.best.err)) break;
         int type = 0;
-        const uint32_t e = cand_err(s, lo, hi, 0, type);
-        const int l = take(s, ok, lo, hi, 0, bnd, ok ? e : kNone, type, true);
+        const uint32_t e = cand_err(s, lo, hi, 0, type, g);
+        const int l = take(s, rep, lo, hi, 0, bnd, rep ? e : kNone, type, true);
         if (l < 0) break;
         any = true;
-        start = (uint32_t)l + 1;
+        start += (uint32_t)l / g + 1;
     }
     return any;
 }
@@ -563,13 +578,10 @@ __device__ bool refine(Wb& s, int level)
     // 2 x 27 candidates: one endpoint moved by -1..+1 per channel (i, r, g, b order)
     return eval_list(s, 54, 0, [&](uint32_t idx, uint32_t& lo, uint32_t& hi) {
         const int i = (int)idx / 27, rr = ((int)idx % 27) / 9 - 1, gr = ((int)idx % 9) / 3 - 1, br = (int)idx % 3 - 1;
-        int c0[3] = {e0[0], e0[1], e0[2]}, c1[3] = {e1[0], e1[1], e1[2]};
-        int* c = i ? c1 : c0;
-        c[0] = clampi(c[0] + rr, 31);
-        c[1] = clampi(c[1] + gr, 63);
-        c[2] = clampi(c[2] + br, 31);
-        lo = pack565(c0[0], c0[1], c0[2], 0);
-        hi = pack565(c1[0], c1[1], c1[2], 0);
+        const int a0 = i ? 0 : rr, a1 = i ? 0 : gr, a2 = i ? 0 : br;
+        const int b0 = i ? rr : 0, b1 = i ? gr : 0, b2 = i ? br : 0;
+        lo = pack565(clampi(e0[0] + a0, 31), clampi(e0[1] + a1, 63), clampi(e0[2] + a2, 31), 0);
+        hi = pack565(clampi(e1[0] + b0, 31), clampi(e1[1] + b1, 63), clampi(e1[2] + b2, 31), 0);
         canon(lo, hi);
         return true;
     });
@@ -785,6 +797,10 @@ __device__ void comp_errors(const Wb& s, int comp, QErr& q, uint32_t* brem)
     for (int sl = 3; sl; sl--) brem[sl - 1] += brem[sl];
 }
 
+// The candidates of one channel are the pairs (c0, c1) in order, each
+// evaluated when the error-table bounds pass with the state at its turn
+// (:437-470); after a candidate is taken the tables and bounds change, so the
+// pairs after it are filtered again.  Passing pairs are gathered 64 at a time.
 __device__ void optimize_comps(Wb& s)
 {
     compute_selectors(s);
@@ -795,50 +811,68 @@ __device__ void optimize_comps(Wb& s)
     uint32_t brem[4];
     QErr q;
     for (int comp = 0; comp < 3; comp++) {
-        uint32_t p0 = (uint32_t)(comp == 0 ? slr : comp == 1 ? slg : slb);
-        uint32_t p1 = (uint32_t)(comp == 0 ? shr : comp == 1 ? shg : shb);
-        int low[3], high[3];
-        unpack565(s.best.lo, 0, low[0], low[1], low[2]);
-        unpack565(s.best.hi, 0, high[0], high[1], high[2]);
+        const uint32_t p0 = (uint32_t)(comp == 0 ? slr : comp == 1 ? slg : slb);
+        const uint32_t p1 = (uint32_t)(comp == 0 ? shr : comp == 1 ? shg : shb);
+        int lr, lg, lb, hr, hg, hb;
+        unpack565(s.best.lo, 0, lr, lg, lb);
+        unpack565(s.best.hi, 0, hr, hg, hb);
         comp_errors(s, comp, q, brem);
         uint32_t best = q.at(0, p0) + q.at(1, p1) + q.at(2, (p0 * 2 + p1) / 3) + q.at(3, (p0 + p1 * 2) / 3);
         if (brem[0] >= best) continue;
-        const uint32_t lim = comp == 1 ? 64 : 32;
-        for (uint32_t c0 = 0; c0 < lim; c0++) {
-            p0 = expand_comp(comp, c0);
-            uint32_t e0 = q.at(0, p0);
-            if (e0 + brem[1] >= best) continue;
-            if (comp == 0) low[0] = (int)c0; else if (comp == 1) low[1] = (int)c0; else low[2] = (int)c0;
-            const uint32_t plow = pack565(low[0], low[1], low[2], 0);
-            // the c1 loop: every c1 that passes the bounds is a candidate, in order
-            uint32_t start = 0;
-            while (start < lim) {
-                const uint32_t c1 = (uint32_t)lane_id();
-                const uint32_t lp1 = expand_comp(comp, c1);
-                uint32_t e = e0 + q.at(1, lp1);
-                bool pass = c1 >= start && c1 < lim && !(e + brem[2] >= best);
-                e += q.at(2, (p0 * 2 + lp1) / 3);
+        const uint32_t lim = comp == 1 ? 64 : 32, shift = comp == 1 ? 6 : 5, total = lim * lim;
+        uint32_t pos = 0;      // first pair not yet decided
+        uint32_t scan = 0;     // first pair not yet filtered
+        uint32_t cnt = 0;      // pending pairs (indices) in s.pend
+        while (pos < total) {
+            // filter pairs from `scan` until 64 pass or the pairs run out
+            while (cnt < 64 && scan < total) {
+                const uint32_t idx = scan + (uint32_t)lane_id();
+                const uint32_t c0 = idx >> shift, c1 = idx & (lim - 1);
+                const uint32_t q0 = expand_comp(comp, c0), q1 = expand_comp(comp, c1);
+                const uint32_t e0 = q.at(0, q0);
+                uint32_t e = e0 + q.at(1, q1);
+                bool pass = idx < total && !(e0 + brem[1] >= best) && !(e + brem[2] >= best);
+                e += q.at(2, (q0 * 2 + q1) / 3);
                 pass = pass && !(e + brem[3] >= best);
-                e += q.at(3, (p0 + lp1 * 2) / 3);
+                e += q.at(3, (q0 + q1 * 2) / 3);
                 pass = pass && !(e >= best);
-                if (!ballot(pass)) break;
-                int hh[3] = {high[0], high[1], high[2]};
-                hh[comp] = (int)c1;
-                const uint32_t phigh = pack565(hh[0], hh[1], hh[2], 0);
-                const uint32_t bnd = pass ? plane_bound(s, plow, phigh) : kNone;
-                int type = 0;
-                const uint32_t ce = cand_err(s, plow, phigh, 0, type);
-                const int l = take(s, pass, plow, phigh, 0, bnd, pass ? ce : kNone, type, true);
-                if (l < 0) break;
-                if (!s.best.err) return;
-                compute_selectors(s);
-                comp_errors(s, comp, q, brem);
-                p1 = expand_comp(comp, (uint32_t)l);
-                best = q.at(0, p0) + q.at(1, p1) + q.at(2, (p0 * 2 + p1) / 3) + q.at(3, (p0 + p1 * 2) / 3);
-                e0 = q.at(0, p0);
-                if (e0 + brem[1] >= best) break;
-                start = (uint32_t)l + 1;
+                const uint64_t pm = ballot(pass);
+                if (pass) s.pend[cnt + (uint32_t)__popcll(pm & lanes_below())] = idx;
+                cnt += (uint32_t)__popcll(pm);
+                scan += 64;
+                __builtin_amdgcn_wave_barrier();
             }
+            if (!cnt) break;
+            const uint32_t take_n = min(cnt, 64u), g = lanes_per(take_n), cl = (uint32_t)lane_id() / g;
+            const bool v = cl < take_n;
+            const uint32_t idx = v ? s.pend[cl] : 0u;
+            const uint32_t c0 = idx >> shift, c1 = idx & (lim - 1);
+            const uint32_t plow = pack565(comp == 0 ? (int)c0 : lr, comp == 1 ? (int)c0 : lg, comp == 2 ? (int)c0 : lb, 0);
+            const uint32_t phigh = pack565(comp == 0 ? (int)c1 : hr, comp == 1 ? (int)c1 : hg, comp == 2 ? (int)c1 : hb, 0);
+            const uint32_t bnd = v ? plane_bound(s, plow, phigh) : kNone;
+            int type = 0;
+            const uint32_t ce = cand_err(s, plow, phigh, 0, type, g);
+            const bool rep = v && ((uint32_t)lane_id() & (g - 1)) == 0;
+            const int l = take(s, rep, plow, phigh, 0, bnd, rep ? ce : kNone, type, true);
+            if (l < 0) {
+                // none of these was taken: keep the rest of the pending pairs
+                const uint32_t rest = cnt - take_n;
+                const uint32_t moved = (uint32_t)lane_id() < rest ? s.pend[64 + lane_id()] : 0u;
+                __builtin_amdgcn_wave_barrier();
+                if ((uint32_t)lane_id() < rest) s.pend[lane_id()] = moved;
+                __builtin_amdgcn_wave_barrier();
+                cnt = rest;
+                pos = rest ? rdlu(idx, (int)((take_n - 1) * g)) + 1 : scan;
+                continue;
+            }
+            if (!s.best.err) return;
+            compute_selectors(s);
+            comp_errors(s, comp, q, brem);
+            const uint32_t wi = rdlu(idx, l), w0 = expand_comp(comp, wi >> shift), w1 = expand_comp(comp, wi & (lim - 1));
+            best = q.at(0, w0) + q.at(1, w1) + q.at(2, (w0 * 2 + w1) / 3) + q.at(3, (w0 + w1 * 2) / 3);
+            // the pairs after the one taken, filtered again with the new tables
+            pos = scan = wi + 1;
+            cnt = 0;
         }
     }
 }
@@ -960,11 +994,9 @@ __device__ void lattice(Wb& s, int which)
         unpack565(which ? s.best.hi : s.best.lo, 0, lr, lg, lb);
         eval_chain(s, 6, [&](uint32_t i, uint32_t& lo, uint32_t& hi) {
             const int a = (int)i >> 1, sg = (i & 1) ? 2 : -2;
-            int c[3] = {lr, lg, lb};
-            const int q = c[a] + sg;
-            if (q < 0 || q > (a == 1 ? 63 : 31)) return false;
-            c[a] = q;
-            const uint32_t p = pack565(c[0], c[1], c[2], 0);
+            const int r = lr + (a == 0 ? sg : 0), g = lg + (a == 1 ? sg : 0), b = lb + (a == 2 ? sg : 0);
+            if (r < 0 || r > 31 || g < 0 || g > 63 || b < 0 || b > 31) return false;
+            const uint32_t p = pack565(r, g, b, 0);
             lo = which ? s.best.lo : p;
             hi = which ? p : s.best.hi;
             canon(lo, hi);
@@ -1314,6 +1346,7 @@ __device__ Dxt1Out encode_core(uint32_t px, int quality, int flags, uint32_t* ld
         if (((omask >> j) & 1u) && tj == (uint32_t)me) w++;
     }
     s.cw = colour ? w : 0u;
+    s.cpk = colour ? (ckey & 0x00FFFFFFu) | (w << 24) : 0u;
     s.nr = s.ng = s.nb = 0.0f;
     Dxt1Out o;
     o.pad = 0;
