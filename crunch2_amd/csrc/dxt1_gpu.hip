@@ -168,17 +168,17 @@ struct Wb {
     uint32_t* pbound;        // LDS: their plane bounds
 };
 
-__device__ inline int colour_r(const Wb& s, int k) { return rdl(s.cr, k); }
-__device__ inline int colour_g(const Wb& s, int k) { return rdl(s.cg, k); }
-__device__ inline int colour_b(const Wb& s, int k) { return rdl(s.cb, k); }
-__device__ inline uint32_t colour_w(const Wb& s, int k) { return rdlu(s.cw, k); }
+__device__ __forceinline__ int colour_r(const Wb& s, int k) { return rdl(s.cr, k); }
+__device__ __forceinline__ int colour_g(const Wb& s, int k) { return rdl(s.cg, k); }
+__device__ __forceinline__ int colour_b(const Wb& s, int k) { return rdl(s.cb, k); }
+__device__ __forceinline__ uint32_t colour_w(const Wb& s, int k) { return rdlu(s.cw, k); }
 
 // Error planes of compute_internal (crn_dxt1.cpp:2134-2203): for every 5/6-bit
 // value c of a channel, lo[c] = the weighted error of the colours whose
 // nearest plane is <= c against c's expanded value, hi[c] = of the others
 // (x8 red, x25 green under the perceptual metric).  Every entry is below
 // 16 * 255^2 * 25 < 2^32.
-__device__ void build_planes(Wb& s)
+__device__ __forceinline__ void build_planes(Wb& s)
 {
     for (int e = lane_id(); e < 128; e += 64) {
         const int ch = e < 32 ? 0 : (e < 96 ? 1 : 2);
@@ -203,7 +203,7 @@ __device__ void build_planes(Wb& s)
 }
 
 // the planes' bound of evaluate_solution (crn_dxt1.cpp:1316-1322)
-__device__ inline uint32_t plane_bound(const Wb& s, uint32_t lo, uint32_t hi)
+__device__ __forceinline__ uint32_t plane_bound(const Wb& s, uint32_t lo, uint32_t hi)
 {
     int ar, ag, ab, br, bg, bb;
     unpack565(lo, 0, ar, ag, ab);
@@ -230,7 +230,7 @@ __device__ inline uint32_t lanes_per(uint32_t m)
 // integer sums are then added over the g lanes.  Each evaluator's early exits
 // only decide whether the candidate is kept, which the caller decides from
 // the full error.
-__device__ uint32_t cand_err(const Wb& s, uint32_t lo, uint32_t hi, int alt, int& type, uint32_t g)
+__device__ __forceinline__ uint32_t cand_err(const Wb& s, uint32_t lo, uint32_t hi, int alt, int& type, uint32_t g)
 {
     int c0r, c0g, c0b, c1r, c1g, c1b;
     unpack565(lo, 1, c0r, c0g, c0b);
@@ -315,7 +315,7 @@ __device__ uint32_t cand_err(const Wb& s, uint32_t lo, uint32_t hi, int alt, int
 // Lane k's selector of colour k under (lo, hi, alt, type) by the evaluator's
 // rule: the first minimum of the palette (uber), the projection thresholds
 // (fast).
-__device__ uint32_t colour_selector(const Wb& s, uint32_t lo, uint32_t hi, int alt, int type)
+__device__ __forceinline__ uint32_t colour_selector(const Wb& s, uint32_t lo, uint32_t hi, int alt, int type)
 {
     int c0r, c0g, c0b, c1r, c1g, c1b;
     unpack565(lo, 1, c0r, c0g, c0b);
@@ -356,7 +356,7 @@ __device__ uint32_t colour_selector(const Wb& s, uint32_t lo, uint32_t hi, int a
 }
 
 // The winner of an evaluate_solution call becomes the best solution.
-__device__ void accept(Wb& s, uint32_t lo, uint32_t hi, int alt, int type, uint32_t err)
+__device__ __forceinline__ void accept(Wb& s, uint32_t lo, uint32_t hi, int alt, int type, uint32_t err)
 {
     Best& b = s.best;
     b.lo = lo;
@@ -391,7 +391,7 @@ __device__ void accept(Wb& s, uint32_t lo, uint32_t hi, int alt, int type, uint3
 // the valid lanes, in lane order: a candidate is taken when its plane bound
 // and its error are both below the current best error.  stop_first returns
 // after the first one taken (its lane, or -1).
-__device__ int take(Wb& s, bool valid, uint32_t lo, uint32_t hi, int alt, uint32_t bnd, uint32_t err, int type,
+__device__ __forceinline__ int take(Wb& s, bool valid, uint32_t lo, uint32_t hi, int alt, uint32_t bnd, uint32_t err, int type,
                     bool stop_first)
 {
     int last = -1;
@@ -407,7 +407,7 @@ __device__ int take(Wb& s, bool valid, uint32_t lo, uint32_t hi, int alt, uint32
 }
 
 // Evaluates the pending candidates (the first cnt entries) in order.
-__device__ bool flush(Wb& s, int alt, uint32_t cnt)
+__device__ __forceinline__ bool flush(Wb& s, int alt, uint32_t cnt)
 {
     const uint32_t g = lanes_per(cnt), c = (uint32_t)lane_id() / g;
     const bool v = c < cnt;
@@ -427,7 +427,7 @@ __device__ bool flush(Wb& s, int alt, uint32_t cnt)
 // the full evaluation (the bound is checked again, in order, when taken);
 // the rest are gathered 64 at a time.
 template <class Gen>
-__device__ bool eval_list(Wb& s, uint32_t n, int alt, Gen gen)
+__device__ __forceinline__ bool eval_list(Wb& s, uint32_t n, int alt, Gen gen)
 {
     bool any = false;
     uint32_t cnt = 0;
@@ -472,7 +472,7 @@ __device__ bool eval_list(Wb& s, uint32_t n, int alt, Gen gen)
 // candidate i = gen(i, lo, hi) is generated from the state at its turn, so
 // after a candidate is taken the ones after it are generated again.
 template <class Gen>
-__device__ bool eval_chain(Wb& s, uint32_t n, Gen gen)
+__device__ __forceinline__ bool eval_chain(Wb& s, uint32_t n, Gen gen)
 {
     bool any = false;
     uint32_t start = 0;
@@ -495,7 +495,7 @@ __device__ bool eval_chain(Wb& s, uint32_t n, Gen gen)
 }
 
 // one candidate (the reference's single evaluate_solution call)
-__device__ bool eval_one(Wb& s, uint32_t lo, uint32_t hi, int alt)
+__device__ __forceinline__ bool eval_one(Wb& s, uint32_t lo, uint32_t hi, int alt)
 {
     return eval_list(s, 1, alt, [&](uint32_t, uint32_t& a, uint32_t& b) {
         a = lo;
@@ -506,7 +506,7 @@ __device__ bool eval_one(Wb& s, uint32_t lo, uint32_t hi, int alt)
 
 // compute_selectors (crn_dxt1.cpp:1837-1869): the hc evaluator's selectors
 // of the best solution
-__device__ void compute_selectors(Wb& s)
+__device__ __forceinline__ void compute_selectors(Wb& s)
 {
     if (!s.hc) return;
     Best& b = s.best;
@@ -528,7 +528,7 @@ __device__ void compute_selectors(Wb& s)
     s.bsel = e01 <= e23 ? (e01 == e0 ? 0u : 1u) : (e23 == e2 ? 2u : 3u);
 }
 
-__device__ bool sel_all_equal(const Wb& s)
+__device__ __forceinline__ bool sel_all_equal(const Wb& s)
 {
     if (!s.best.sel_valid) return false;
     const uint32_t f = rdlu(s.bsel, 0);
@@ -537,7 +537,7 @@ __device__ bool sel_all_equal(const Wb& s)
 
 // refine_solution (crn_dxt1.cpp:525-698), levels 0 and 1.  Every sum is of
 // integers below 2^53, exact as the reference's doubles.
-__device__ bool refine(Wb& s, int level)
+__device__ __forceinline__ bool refine(Wb& s, int level)
 {
     compute_selectors(s);
     int64_t a0 = 0, a1 = 0, a2 = 0, t1r = 0, t1g = 0, t1b = 0, t2r = 0, t2g = 0, t2b = 0;
@@ -608,7 +608,7 @@ struct FastRandom {
 
 // try_median4 (crn_dxt1.cpp:1181-1308).  Each colour's nearest centre is
 // found on its lane; the float sums run over the colours in order.
-__device__ bool try_median4(Wb& s, const float* lo, const float* hi)
+__device__ __forceinline__ bool try_median4(Wb& s, const float* lo, const float* hi)
 {
     float m[4][3];
     if (s.nu <= 4) {
@@ -721,7 +721,7 @@ __device__ bool try_median4(Wb& s, const float* lo, const float* hi)
 // try_average_block_as_solid (crn_dxt1.cpp:93-153).  A candidate after the
 // error reaches 0 is never taken, so the reference's "&& error" conditions
 // need no test.
-__device__ bool try_average_solid(Wb& s)
+__device__ __forceinline__ bool try_average_solid(Wb& s)
 {
     uint32_t tr = 0, tg = 0, tb = 0, tw = 0;
     for (int k = 0; k < (int)s.nu; k++) {
@@ -772,7 +772,7 @@ __device__ inline uint32_t expand_comp(int comp, uint32_t c)
     return comp == 1 ? (((c << 2) | (c >> 4)) & 0xFF) : (((c << 3) | (c >> 2)) & 0xFF);
 }
 
-__device__ void comp_errors(const Wb& s, int comp, QErr& q, uint32_t* brem)
+__device__ __forceinline__ void comp_errors(const Wb& s, int comp, QErr& q, uint32_t* brem)
 {
     for (int k = 0; k < 4; k++) q.W[k] = q.WP2[k] = q.WPP[k] = 0;
     for (int k = 0; k < (int)s.nu; k++) {
@@ -801,7 +801,7 @@ __device__ void comp_errors(const Wb& s, int comp, QErr& q, uint32_t* brem)
 // evaluated when the error-table bounds pass with the state at its turn
 // (:437-470); after a candidate is taken the tables and bounds change, so the
 // pairs after it are filtered again.  Passing pairs are gathered 64 at a time.
-__device__ void optimize_comps(Wb& s)
+__device__ __forceinline__ void optimize_comps(Wb& s)
 {
     compute_selectors(s);
     if (s.best.alpha || !s.best.err) return;
@@ -889,7 +889,7 @@ __device__ inline void lerp_color(int ar, int ag, int ab, int br, int bg, int bb
 // try_combinatorial_encoding (crn_dxt1.cpp:1886-1997): the packed colours on
 // lanes 0..np-1, every pair i < j in order, then again with the alternate
 // rounding where only an exact solution counts.
-__device__ void try_combinatorial(Wb& s)
+__device__ __forceinline__ void try_combinatorial(Wb& s)
 {
     if (s.nu < 2 || s.nu > 4) return;
     uint32_t pk = 0;
@@ -950,7 +950,7 @@ __device__ void try_combinatorial(Wb& s)
 
 // the probe sweep along the axis (crn_dxt1.cpp:840-892): the packed probes on
 // lanes 0..n-1
-__device__ uint32_t sweep(const int16_t* tab, uint32_t range, const float* base, const float (*ax)[3], uint32_t& n)
+__device__ __forceinline__ uint32_t sweep(const int16_t* tab, uint32_t range, const float* base, const float (*ax)[3], uint32_t& n)
 {
     int prev[2] = {-1, -1};
     n = 0;
@@ -977,7 +977,7 @@ __device__ uint32_t sweep(const int16_t* tab, uint32_t range, const float* base,
 
 // the lattice moves of optimize_endpoints (crn_dxt1.cpp:905-1016) around the
 // low (which = 0) or high endpoint
-__device__ void lattice(Wb& s, int which)
+__device__ __forceinline__ void lattice(Wb& s, int which)
 {
     int lr, lg, lb;
     unpack565(which ? s.best.hi : s.best.lo, 0, lr, lg, lb);
@@ -1006,7 +1006,7 @@ __device__ void lattice(Wb& s, int which)
 }
 
 // optimize_endpoints (crn_dxt1.cpp:703-1067)
-__device__ void optimize_endpoints(Wb& s, float* lo, float* hi)
+__device__ __forceinline__ void optimize_endpoints(Wb& s, float* lo, float* hi)
 {
     float olo[3] = {lo[0], lo[1], lo[2]}, ohi[3] = {hi[0], hi[1], hi[2]};
     const int16_t* tab = k_uber_probe;
@@ -1080,7 +1080,7 @@ __device__ void optimize_endpoints(Wb& s, float* lo, float* hi)
 // compute_vectors (crn_dxt1.cpp:155-189): normalised colours (this lane's),
 // their weighted means summed in colour order; returns the weighted vector
 // of this lane's colour minus the weighted mean
-__device__ void compute_vectors(Wb& s, const float* pw, float& nwr, float& nwg, float& nwb)
+__device__ __forceinline__ void compute_vectors(Wb& s, const float* pw, float& nwr, float& nwg, float& nwb)
 {
     const float w = (float)s.cw;
     const float n0 = (float)s.cr * 1.0f / 255.0f, n1 = (float)s.cg * 1.0f / 255.0f, n2 = (float)s.cb * 1.0f / 255.0f;
@@ -1110,7 +1110,7 @@ __device__ void compute_vectors(Wb& s, const float* pw, float& nwr, float& nwg, 
 }
 
 // compute_pca (crn_dxt1.cpp:192-256) of the lanes' vectors (r, g, b)
-__device__ void compute_pca(const Wb& s, float r, float g, float b, float* axis)
+__device__ __forceinline__ void compute_pca(const Wb& s, float r, float g, float b, float* axis)
 {
     double t[6];
     if (s.cw > 1) {
@@ -1158,7 +1158,7 @@ __device__ void compute_pca(const Wb& s, float r, float g, float b, float* axis)
 }
 
 // ray_aabb against the unit cube (crn_intersect.h:44-120)
-__device__ bool ray_unit_box(float* coord, const float* org, const float* dir)
+__device__ __forceinline__ bool ray_unit_box(float* coord, const float* org, const float* dir)
 {
     int quad[3];
     float cand[3], maxt[3];
@@ -1190,7 +1190,7 @@ __device__ bool ray_unit_box(float* coord, const float* org, const float* dir)
 }
 
 // handle_multicolor_block (crn_dxt1.cpp:1069-1178)
-__device__ void handle_multicolor(Wb& s)
+__device__ __forceinline__ void handle_multicolor(Wb& s)
 {
     uint32_t npass = 1;
     float pw[3] = {1.0f, 1.0f, 1.0f};
