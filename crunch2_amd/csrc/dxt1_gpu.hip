@@ -250,9 +250,13 @@ __device__ __forceinline__ uint32_t cand_err(const Wb& s, uint32_t lo, uint32_t 
         last = 0;
     }
     uint32_t e4 = 0, e3 = 0;
+    // colour k: from a scalar register when every lane reads the same one
+    // (g == 1), else from lane k
+    auto colour = [&](uint32_t k) {
+        return g == 1 ? rdlu(s.cpk, (int)k) : (uint32_t)__shfl((int)s.cpk, (int)k);
+    };
     if (s.hc || s.quality >= 3) {
-        for (uint32_t k = sub; k < kend; k += g) {
-            const uint32_t pk = (uint32_t)__shfl((int)s.cpk, (int)k);
+        auto body = [&](uint32_t pk) {
             const int r = (int)(pk & 255u), gg = (int)((pk >> 8) & 255u), b = (int)((pk >> 16) & 255u);
             const uint32_t w = pk >> 24;
             const uint32_t d0 = dist(P, G, r, gg, b, c0r, c0g, c0b), d1 = dist(P, G, r, gg, b, c1r, c1g, c1b);
@@ -261,6 +265,11 @@ __device__ __forceinline__ uint32_t cand_err(const Wb& s, uint32_t lo, uint32_t 
                 e4 += min(min(d0, d1), min(d2, d3)) * w;
             }
             if (last == 1) e3 += min(min(d0, d1), dist(P, G, r, gg, b, h2r, h2g, h2b)) * w;
+        };
+        if (g == 1) {
+            for (uint32_t k = 0; k < s.nu; k++) body(rdlu(s.cpk, (int)k));
+        } else {
+            for (uint32_t k = sub; k < kend; k += g) body(colour(k));
         }
     } else {
         int vr = c1r - c0r, vg = c1g - c0g, vb = c1b - c0b;
@@ -273,7 +282,7 @@ __device__ __forceinline__ uint32_t cand_err(const Wb& s, uint32_t lo, uint32_t 
         const int h2 = h2r * vr + h2g * vg + h2b * vb;
         const int dr = vr * 2, dg = vg * 2, db = vb * 2;
         for (uint32_t k = sub; k < kend; k += g) {
-            const uint32_t pk = (uint32_t)__shfl((int)s.cpk, (int)k);
+            const uint32_t pk = colour(k);
             const int r = (int)(pk & 255u), gg = (int)((pk >> 8) & 255u), b = (int)((pk >> 16) & 255u);
             const uint32_t w = pk >> 24;
             const int dot = r * dr + gg * dg + b * db;
@@ -1294,7 +1303,7 @@ __device__ inline uint32_t pack_selectors(uint32_t v)
 // the block whose texel i is px on lane i (i < 16).  DXT1A (flags bit 4):
 // texels with alpha below the threshold (flags bits 8..15) leave the colour
 // set and take selector 3.
-__device__ Dxt1Out encode_core(uint32_t px, int quality, int flags, uint32_t* lds)
+__device__ __forceinline__ Dxt1Out encode_core(uint32_t px, int quality, int flags, uint32_t* lds)
 {
     Wb s;
     s.quality = quality;
@@ -1408,7 +1417,7 @@ __device__ inline uint32_t metric(int perceptual, int grayscale, uint32_t p, int
 // all <= 4 are made transparent, the block is searched again as DXT1A, and
 // that 3-colour block replaces the first result if its error, counting those
 // texels against black, is lower.
-__device__ Dxt1Out encode_block(uint32_t px, int quality, int flags, uint32_t* lds)
+__device__ __forceinline__ Dxt1Out encode_block(uint32_t px, int quality, int flags, uint32_t* lds)
 {
     const bool texel = lane_id() < 16;
     if ((flags >> 6) & 1) {
