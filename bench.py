@@ -385,6 +385,77 @@ def mixed_leg(count, cpu_seconds):
                           "inputs": "the simple block quantizer's .crn files at quality-128 palette sizes"}}
 
 
+FP32_PEAK_TFLOPS = 157.3   # MI355X dense fp32 (packed FMA); one multiply and one add per
+                           # contributor, unfused and unpacked, reach a quarter of it
+
+
+def mip_leg(size, cpu_seconds):
+    """Mip generation of one size x size RGBA texture with crn_mipmap_params'
+    defaults (Kaiser, gamma filtering, blurriness 0.9, clamp, alpha valid):
+    levels 1..n-1, each resampled from level 0, on the GPU; the whole chain
+    checked against the oracle (computed level by level on host threads, which
+    is also the CPU baseline); fp32 roofline of the dominant launch (pass 1)
+    from its multiply + add per contributor."""
+    import numpy as np
+    import torch
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    from crunch2_amd import mips, texture
+    img = texture.synth(size, size, 23, texture.RGBA)
+    p = mips.params()
+    plan = mips.Plan(size, size, p)
+    src = torch.from_numpy(img).cuda()
+    out = torch.empty(plan.chain_bytes, dtype=torch.uint8, device="cuda")
+    for _ in range(2):
+        plan.run(src, out)
+    torch.cuda.synchronize()
+    ms = gpu_ms(lambda: plan.run(src, out), 10)
+    plan.profile(True)
+    for _ in range(5):
+        plan.run(src, out)
+    stage, runs = plan.profile(False)
+    stage = [t / runs for t in stage]
+    got = plan.split(out.cpu().numpy())
+    # algorithmic multiply-adds of each pass (contributors x lines x components)
+    nc = 4
+    mac1 = mac2 = 0
+    for l, (w, h) in enumerate(plan.sizes()[1:], 1):
+        xo = len(mips.contributors(size, w)[1])
+        yo = len(mips.contributors(size, h)[1])
+        if plan.y_first[l]:
+            mac1 += nc * size * yo
+            mac2 += nc * h * xo
+        else:
+            mac1 += nc * size * xo
+            mac2 += nc * w * yo
+    achieved = 2 * mac1 / (stage[1] * 1e-3) / 1e12
+    th = cpu_threads()
+    sizes = plan.sizes()[1:]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=th) as pool:
+        ref = list(pool.map(lambda wh: oracle.mip_resample(img, wh[0], wh[1])[0], sizes))
+    cpu_s = time.perf_counter() - t0
+    exact = all(np.array_equal(a, b) for a, b in zip(got[1:], ref)) and np.array_equal(got[0], img)
+    assert exact, "GPU mip chain differs from the oracle"
+    plan.close()
+    return {"workload": "mip chain of one %dx%d RGBA8 texture (%d levels): Kaiser, sRGB gamma filtering, "
+                        "blurriness 0.9, clamp, 4 components, every level resampled from level 0"
+                        % (size, size, plan.levels),
+            "level0_mtexels_s": round(size * size / (ms * 1e-3) / 1e6, 1), "ms": round(ms, 4),
+            "bit_exact": exact,
+            "stage_ms": {"mip_planes_kernel": round(stage[0], 4), "mip_pass1_kernel": round(stage[1], 4),
+                         "mip_pass2_kernel": round(stage[2], 4)},
+            "roofline": {"bound": "valu", "kernel": "mip_pass1_kernel", "achieved": round(achieved, 3),
+                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s (fp32)",
+                         "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None,
+                         "flops_per_launch": 2 * mac1, "pass2_flops": 2 * mac2,
+                         "ceiling_unfused_unpacked": round(FP32_PEAK_TFLOPS / 4, 2)},
+            "cpu_baseline": {"value": round(size * size / cpu_s / 1e6, 3), "unit": "level-0 Mtexels/s",
+                             "cores": th, "kind": "port",
+                             "sample": "the oracle's resample of all %d generated levels of the same image, one "
+                                       "level per thread on %d threads (%.1fs)" % (len(sizes), th, cpu_s)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -414,6 +485,7 @@ def main():
     ap.add_argument("--hc-cpu", type=int, default=1,
                     help="1: time the oracle's dxt_hc composition on the same texture (about 15 s); 0 skips it")
     ap.add_argument("--encode-cpu-seconds", type=float, default=5.0, help="CPU baseline sample of each encode leg")
+    ap.add_argument("--mip-size", type=int, default=8192, help="side of the mip generation leg (0 = skip)")
     ap.add_argument("--fallback-frac", type=float, default=0.08,
                     help="fraction of blocks (lowest confidence first) that also run the plain search in the encode leg")
     args = ap.parse_args()
@@ -659,6 +731,9 @@ def main():
     if args.mixed_set:
         line["mixed_set"] = mixed_leg(args.mixed_set, args.encode_cpu_seconds)
         log("serving-set leg done")
+    if args.mip_size:
+        line["mips"] = mip_leg(args.mip_size, args.encode_cpu_seconds)
+        log("mip generation leg done")
     print(json.dumps(line))
     for pl in plans:
         lib().crn2_batch_destroy(pl)

@@ -49,6 +49,11 @@ class EncodeItem(C.Structure):
                 ("blocks", C.c_void_p)]
 
 
+class MipParams(C.Structure):
+    _fields_ = [("filter", C.c_uint32), ("flags", C.c_uint32), ("filter_scale", C.c_float),
+                ("source_gamma", C.c_float), ("max_levels", C.c_uint32), ("min_mip_size", C.c_uint32)]
+
+
 # name -> (restype, argtypes); every function declared in include/crn2.h
 SIGNATURES = {
     "crn2_validate_file": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -177,6 +182,18 @@ SIGNATURES = {
                                      C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_void_p),
                                      C.POINTER(C.c_uint32)]),
     "crn2_free": (None, [C.c_void_p]),
+    "crn2_mip_level_count": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "crn2_mip_plan_create": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(MipParams), C.POINTER(C.c_void_p)]),
+    "crn2_mip_plan_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.c_void_p,
+                                     C.c_void_p]),
+    "crn2_mip_plan_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "crn2_mip_plan_destroy": (C.c_int, [C.c_void_p]),
+    "crn2_mip_plan_profile": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "crn2_generate_mipmaps_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(MipParams), C.c_void_p,
+                                               C.c_uint64, C.c_void_p]),
+    "crn2_mip_contributors": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_float, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_uint32]),
+    "crn2_mip_gamma_tables": (C.c_int, [C.c_float, C.c_void_p, C.c_void_p]),
 }
 
 # internal (not in include/): the block packer behind crn2_compress_rgba

@@ -18,12 +18,12 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
 
 ORACLE_SOURCES = ["crnd_oracle.c", "dxt5a_oracle.c", "dxt1_oracle.c", "hc_oracle.c", "hc_endpoints.c",
-                  "tree_clusterizer.c", "hc_tiles.c"]
-HOST_SOURCES = ["crn_pack.cpp", "synth.cpp", "crn_api.cpp", "crnd_host.cpp", "crn_files.cpp", "crnlib_facade.cpp"]
+                  "tree_clusterizer.c", "hc_tiles.c", "mip_oracle.c"]
+HOST_SOURCES = ["crn_pack.cpp", "synth.cpp", "crn_api.cpp", "crnd_host.cpp", "crn_files.cpp", "crnlib_facade.cpp", "mip_host.cpp"]
 HIP_SOURCES = ["crnd_gpu.hip", "dxt5a_gpu.hip", "dxt1_gpu.hip", "hc_gpu.hip", "mlp_gpu.hip", "dxt1c_gpu.hip",
                "hc_endpoints_gpu.hip", "tree_gpu.hip",
                "hc_codebook_gpu.hip", "hc_clusters_gpu.hip", "hc_tiles_gpu.hip",
-               "hc_compress_gpu.hip", "dxt_decode_gpu.hip", "texture_api.hip"]
+               "hc_compress_gpu.hip", "dxt_decode_gpu.hip", "texture_api.hip", "mip_gpu.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_*_f32 generation; -ffp-contract=off: float

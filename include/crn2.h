@@ -599,6 +599,65 @@ int crn2_write_dds(uint32_t width, uint32_t height, uint32_t levels, uint32_t fa
  * every level and face transcoded on the GPU, then written as a DDS file. */
 int crn2_crn_to_dds(const void* crn, uint32_t size, uint8_t** out, uint32_t* out_size);
 
+/* ------------------------------------------------------- mip generation */
+/* mipmapped_texture::generate_mipmaps (crnlib/crn_mipmapped_texture.cpp:
+ * 2140-2220): level l (max(1, w >> l) x max(1, h >> l)) resampled from level 0
+ * by image_utils::resample (crnlib/crn_image_utils.cpp:478-865) with crnlib's
+ * Resampler (crnlib/crn_resampler.cpp) -- same levels, bytes for bytes.
+ * filter: the crn_mip_filter values (inc/crnlib.h:437-449). */
+#define CRN2_MIP_BOX 0
+#define CRN2_MIP_TENT 1
+#define CRN2_MIP_LANCZOS4 2
+#define CRN2_MIP_MITCHELL 3
+#define CRN2_MIP_KAISER 4
+#define CRN2_MIP_SRGB 1        /* gamma filtering: R, G, B through the source-gamma tables */
+#define CRN2_MIP_WRAP 2        /* tiled: contributors wrap around the edges (else clamp) */
+#define CRN2_MIP_ALPHA 4       /* alpha is valid and resampled (else 255 in levels >= 1) */
+#define CRN2_MIP_RENORMALIZE 8 /* image_utils::renorm_normal_map on every generated level */
+#define CRN2_MIP_THREADED 16   /* the multithreaded resampler's order (X pass then Y pass on
+                                  every level), which crnlib uses with helper threads;
+                                  else the single-thread Resampler's per-level choice */
+typedef struct crn2_mip_params {
+    uint32_t filter;       /* CRN2_MIP_BOX .. CRN2_MIP_KAISER */
+    uint32_t flags;        /* CRN2_MIP_* flags */
+    float filter_scale;    /* crn_mipmap_params::m_blurriness (0.9) */
+    float source_gamma;    /* resample_params::m_source_gamma (2.2: generate_mipmaps keeps it) */
+    uint32_t max_levels;   /* total levels, level 0 included; 0 = no limit */
+    uint32_t min_mip_size; /* halve while either side is above this (1) */
+} crn2_mip_params;
+/* Level count of generate_mipmaps (crn_mipmapped_texture.cpp:2145-2157). */
+uint32_t crn2_mip_level_count(uint32_t width, uint32_t height, uint32_t min_mip_size, uint32_t max_levels);
+/* A reusable plan for one level-0 size: contributor lists and gamma tables on
+ * the device, scratch for the two passes.  Sides up to 16,384
+ * (CRNLIB_RESAMPLER_MAX_DIMENSION). */
+typedef struct crn2_mip_plan crn2_mip_plan;
+int crn2_mip_plan_create(uint32_t width, uint32_t height, const crn2_mip_params* params, crn2_mip_plan** out);
+/* levels, bytes of the whole chain (RGBA8, levels 0..n-1 back to back), the
+ * byte offset of each level (n entries) and 1 where the level ran Y then X
+ * (n entries, 0 for level 0); any pointer may be NULL. */
+int crn2_mip_plan_info(const crn2_mip_plan* plan, uint32_t* levels, uint64_t* chain_bytes, uint64_t* level_offsets,
+                       uint8_t* y_first);
+/* rgba (device, RGBA8 width x height, tightly packed) -> out (device,
+ * chain_bytes): level 0 copied, levels 1..n-1 generated, all on `stream`.
+ * Allocates nothing; calls on one plan must not overlap. */
+int crn2_mip_plan_run(crn2_mip_plan* plan, const void* rgba, void* out, uint64_t out_size, void* stream);
+int crn2_mip_plan_destroy(crn2_mip_plan* plan);
+/* Stage timing of a plan's runs with HIP events on the run stream (each run
+ * then waits for its events): on != 0 switches it on (totals reset); ms[0..2]
+ * = total ms of the level-0 planes, pass 1 and pass 2 since then. */
+int crn2_mip_plan_profile(crn2_mip_plan* plan, int on, double* ms, uint64_t* runs);
+/* One-shot form: plan, run on `stream`, wait, free. */
+int crn2_generate_mipmaps_device(const void* rgba, uint32_t width, uint32_t height, const crn2_mip_params* params,
+                                 void* out, uint64_t out_size, void* stream);
+/* Host copy of the contributor list of one axis (Resampler::make_clist,
+ * crn_resampler.cpp:117-416): offs[dst + 1] and up to cap (pixel, weight)
+ * pairs; returns the pair count, or a negative CRN2_E* code. */
+int crn2_mip_contributors(uint32_t src, uint32_t dst, int wrap, uint32_t filter, float filter_scale, uint32_t* offs,
+                          uint32_t* pix, float* weight, uint32_t cap);
+/* The 256-entry sRGB-to-linear and 8,192-entry linear-to-sRGB tables of
+ * resample_single_thread (crn_image_utils.cpp:514-545). */
+int crn2_mip_gamma_tables(float gamma, float* srgb_to_linear, uint8_t* linear_to_srgb);
+
 /* ---------------------------------------------- writer and test textures */
 /* CRC16 of crn_decomp.h:2048-2069. */
 uint16_t crn2_crc16(const void* p, uint32_t len, uint16_t crc);

@@ -677,3 +677,83 @@ def dxt5a_values(values, quality=4, use_both=False):
     assert L.orc_dxt5a_compute(_p(v), v.shape[0], quality, int(bool(use_both)), _p(sel), C.byref(err), C.byref(f),
                                C.byref(s), C.byref(t), C.byref(r))
     return sel, err.value, f.value, s.value, t.value, r.value
+
+
+MIP_SRGB, MIP_WRAP, MIP_ALPHA, MIP_RENORMALIZE, MIP_THREADED = 1, 2, 4, 8, 16
+
+
+def mip_filter(filter, t):
+    """crnlib's resample filter `filter` (0 box, 1 tent, 2 lanczos4, 3
+    mitchell, 4 kaiser) at float t."""
+    L = lib()
+    L.orc_mip_filter.restype = C.c_float
+    L.orc_mip_filter.argtypes = [C.c_int, C.c_float]
+    return L.orc_mip_filter(filter, float(t))
+
+
+def mip_contributors(src, dst, filter=4, filter_scale=0.9, wrap=False):
+    """Resampler::make_clist: (offs [dst + 1], pix, weight)."""
+    L = lib()
+    L.orc_mip_contributors.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_int]
+    offs = np.zeros(dst + 1, np.uint32)
+    n = L.orc_mip_contributors(src, dst, int(wrap), filter, filter_scale, _p(offs), None, None, 0)
+    assert n >= 0
+    pix = np.zeros(max(n, 1), np.uint32)
+    w = np.zeros(max(n, 1), np.float32)
+    assert L.orc_mip_contributors(src, dst, int(wrap), filter, filter_scale, _p(offs), _p(pix), _p(w), n) == n
+    return offs, pix[:n], w[:n]
+
+
+def mip_tables(gamma=2.2):
+    L = lib()
+    L.orc_mip_tables.argtypes = [C.c_float, C.c_void_p, C.c_void_p]
+    s2l = np.zeros(256, np.float32)
+    l2s = np.zeros(8192, np.uint8)
+    L.orc_mip_tables(gamma, _p(s2l), _p(l2s))
+    return s2l, l2s
+
+
+def mip_level_count(w, h, min_size=1, max_levels=0):
+    L = lib()
+    L.orc_mip_level_count.restype = C.c_uint32
+    L.orc_mip_level_count.argtypes = [C.c_uint32] * 4
+    return L.orc_mip_level_count(w, h, min_size, max_levels)
+
+
+def mip_generate(img, filter=4, filter_scale=0.9, gamma=2.2, flags=MIP_SRGB | MIP_ALPHA, max_levels=0,
+                 min_size=1):
+    """generate_mipmaps from RGBA8 level 0 [h, w, 4]: (levels as [h, w, 4]
+    arrays, orders: 1 where the level ran Y then X)."""
+    L = lib()
+    L.orc_mip_generate.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_int, C.c_float, C.c_float, C.c_int,
+                                   C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    n = mip_level_count(w, h, min_size, max_levels)
+    sizes = [(max(1, w >> l), max(1, h >> l)) for l in range(n)]
+    out = np.zeros(sum(a * b * 4 for a, b in sizes), np.uint8)
+    nl = C.c_uint32()
+    orders = np.zeros(n, np.int32)
+    assert L.orc_mip_generate(_p(img), w, h, filter, filter_scale, gamma, flags, max_levels, min_size, _p(out),
+                              C.byref(nl), _p(orders)) == 0
+    assert nl.value == n
+    levels, o = [], 0
+    for a, b in sizes:
+        levels.append(out[o:o + a * b * 4].reshape(b, a, 4))
+        o += a * b * 4
+    return levels, orders
+
+
+def mip_resample(img, dst_w, dst_h, filter=4, filter_scale=0.9, gamma=2.2, flags=MIP_SRGB | MIP_ALPHA):
+    """One resample() call: (image [dst_h, dst_w, 4], 1 if Y ran before X)."""
+    L = lib()
+    L.orc_mip_resample.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                   C.c_int, C.c_void_p, C.c_void_p]
+    img = np.ascontiguousarray(img, np.uint8)
+    h, w = img.shape[:2]
+    out = np.zeros((dst_h, dst_w, 4), np.uint8)
+    order = C.c_int()
+    assert L.orc_mip_resample(_p(img), w, h, dst_w, dst_h, filter, filter_scale, gamma, flags, _p(out),
+                              C.byref(order)) == 0
+    return out, order.value
