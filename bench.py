@@ -385,6 +385,9 @@ def mixed_leg(count, cpu_seconds):
                           "inputs": "the simple block quantizer's .crn files at quality-128 palette sizes"}}
 
 
+# HBM bytes mip_pass1_kernel moves per 8192^2 chain (FETCH_SIZE + WRITE_SIZE,
+# scaled by a device copy of known size); 0 until measured
+MIP_PASS1_TRAFFIC = 0
 FP32_PEAK_TFLOPS = 157.3   # MI355X dense fp32 (packed FMA); one multiply and one add per
                            # contributor, unfused and unpacked, reach a quarter of it
 
@@ -416,19 +419,25 @@ def mip_leg(size, cpu_seconds):
     stage, runs = plan.profile(False)
     stage = [t / runs for t in stage]
     got = plan.split(out.cpu().numpy())
-    # algorithmic multiply-adds of each pass (contributors x lines x components)
+    # algorithmic multiply-adds and HBM bytes of each pass: pass 1 reads the
+    # level-0 byte planes once per level (4 components) and writes the float
+    # intermediate; pass 2 reads the intermediate and writes the RGBA8 level
     nc = 4
-    mac1 = mac2 = 0
+    mac1 = mac2 = bytes1 = bytes2 = 0
     for l, (w, h) in enumerate(plan.sizes()[1:], 1):
         xo = len(mips.contributors(size, w)[1])
         yo = len(mips.contributors(size, h)[1])
+        d = h if plan.y_first[l] else w
+        bytes1 += nc * size * size + 4 * nc * size * d
+        bytes2 += 4 * nc * size * d + 4 * w * h
         if plan.y_first[l]:
             mac1 += nc * size * yo
             mac2 += nc * h * xo
         else:
             mac1 += nc * size * xo
             mac2 += nc * w * yo
-    achieved = 2 * mac1 / (stage[1] * 1e-3) / 1e12
+    achieved = bytes1 / (stage[1] * 1e-3) / 1e9
+    tflops = 2 * mac1 / (stage[1] * 1e-3) / 1e12
     th = cpu_threads()
     sizes = plan.sizes()[1:]
     t0 = time.perf_counter()
@@ -445,11 +454,15 @@ def mip_leg(size, cpu_seconds):
             "bit_exact": exact,
             "stage_ms": {"mip_planes_kernel": round(stage[0], 4), "mip_pass1_kernel": round(stage[1], 4),
                          "mip_pass2_kernel": round(stage[2], 4)},
-            "roofline": {"bound": "valu", "kernel": "mip_pass1_kernel", "achieved": round(achieved, 3),
-                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s (fp32)",
-                         "frac": round(achieved / FP32_PEAK_TFLOPS, 5), "traffic": None,
-                         "flops_per_launch": 2 * mac1, "pass2_flops": 2 * mac2,
-                         "ceiling_unfused_unpacked": round(FP32_PEAK_TFLOPS / 4, 2)},
+            "roofline": {"bound": "hbm", "kernel": "mip_pass1_kernel", "achieved": round(achieved, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                         "traffic": MIP_PASS1_TRAFFIC if (size == 8192 and MIP_PASS1_TRAFFIC) else None,
+                         "algorithmic_bytes_per_launch": bytes1, "launch_ms": round(stage[1], 4),
+                         "flops_per_launch": 2 * mac1, "achieved_tflops": round(tflops, 3),
+                         "valu_frac": round(tflops / FP32_PEAK_TFLOPS, 5),
+                         "valu_ceiling_unfused_unpacked": round(FP32_PEAK_TFLOPS / 4, 2),
+                         "pass2": {"algorithmic_bytes": bytes2, "flops": 2 * mac2, "launch_ms": round(stage[2], 4),
+                                   "achieved": round(bytes2 / (stage[2] * 1e-3) / 1e9, 1)}},
             "cpu_baseline": {"value": round(size * size / cpu_s / 1e6, 3), "unit": "level-0 Mtexels/s",
                              "cores": th, "kind": "port",
                              "sample": "the oracle's resample of all %d generated levels of the same image, one "

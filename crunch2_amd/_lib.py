@@ -191,6 +191,8 @@ SIGNATURES = {
     "crn2_mip_plan_profile": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_uint64)]),
     "crn2_generate_mipmaps_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(MipParams), C.c_void_p,
                                                C.c_uint64, C.c_void_p]),
+    "crn2_generate_mipmaps": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(MipParams), C.c_void_p,
+                                        C.c_uint64]),
     "crn2_mip_contributors": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_float, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_uint32]),
     "crn2_mip_gamma_tables": (C.c_int, [C.c_float, C.c_void_p, C.c_void_p]),

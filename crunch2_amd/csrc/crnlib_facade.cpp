@@ -396,22 +396,67 @@ void* crn_compress(const crn_comp_params& comp_params, const crn_mipmap_params& 
     if (pActual_quality_level) *pActual_quality_level = 0;
     if (pActual_bitrate) *pActual_bitrate = 0.0f;
     if (!comp_params.check() || !mip_params.check() || !gpu_supported(comp_params)) return nullptr;
-    // the given levels are used as they are; generating levels (Kaiser and the
-    // other filters of crn_resampler) and scaling or cropping are not built
+    // scaling, cropping and the renormalized top level are not built
     if (mip_params.m_scale_mode != cCRNSMDisabled || mip_params.m_clamp_scale || mip_params.m_window_right ||
-        mip_params.m_window_bottom)
+        mip_params.m_window_bottom || (mip_params.m_renormalize && mip_params.m_rtopmip))
         return nullptr;
     crn_uint32 levels = comp_params.m_levels;
+    bool generate = false;
     switch (mip_params.m_mode) {
         case cCRNMipModeNoMips: levels = 1; break;
         case cCRNMipModeUseSourceMips: break;
-        case cCRNMipModeUseSourceOrGenerateMips:
-            if (levels == 1 && (comp_params.m_width > 1 || comp_params.m_height > 1)) return nullptr;
-            break;
+        case cCRNMipModeUseSourceOrGenerateMips: generate = levels == 1; break;
+        case cCRNMipModeGenerateMips: generate = true; break;
         default: return nullptr;
     }
-    levels = std::min<crn_uint32>(levels, std::max<crn_uint32>(1, mip_params.m_max_levels));
-    return compress_levels(comp_params, levels, compressed_size, pActual_quality_level, pActual_bitrate);
+    if (!generate) {
+        levels = std::min<crn_uint32>(levels, std::max<crn_uint32>(1, mip_params.m_max_levels));
+        return compress_levels(comp_params, levels, compressed_size, pActual_quality_level, pActual_bitrate);
+    }
+    // mipmapped_texture::generate_mipmaps from level 0 of every face on the GPU
+    // (crn_texture_comp.cpp:546-572): alpha is resampled when any given texel
+    // is below 255 (crn_texture_comp.cpp:298-323); the resampler's source gamma
+    // stays 2.2 whatever m_gamma says; crnlib's multithreaded resampler (X then
+    // Y on every level) runs when helper threads are asked for
+    bool alpha = false;
+    for (crn_uint32 f = 0; f < comp_params.m_faces && !alpha; f++)
+        for (crn_uint32 l = 0; l < comp_params.m_levels && !alpha; l++) {
+            const crn_uint32* px = comp_params.m_pImages[f][l];
+            if (!px) return nullptr;
+            const size_t n = (size_t)std::max<crn_uint32>(1, comp_params.m_width >> l) *
+                             std::max<crn_uint32>(1, comp_params.m_height >> l);
+            for (size_t i = 0; i < n && !alpha; i++) alpha = (px[i] >> 24) < 255;
+        }
+    crn2_mip_params mp;
+    mp.filter = (uint32_t)mip_params.m_filter;
+    mp.flags = (mip_params.m_gamma_filtering ? CRN2_MIP_SRGB : 0) | (mip_params.m_tiled ? CRN2_MIP_WRAP : 0) |
+               (alpha ? CRN2_MIP_ALPHA : 0) | (mip_params.m_renormalize ? CRN2_MIP_RENORMALIZE : 0) |
+               (comp_params.m_num_helper_threads > 0 ? CRN2_MIP_THREADED : 0);
+    mp.filter_scale = mip_params.m_blurriness;
+    mp.source_gamma = 2.2f;
+    mp.max_levels = mip_params.m_max_levels;
+    mp.min_mip_size = mip_params.m_min_mip_size;
+    levels = crn2_mip_level_count(comp_params.m_width, comp_params.m_height, mp.min_mip_size, mp.max_levels);
+    if (!levels || levels > cCRNMaxLevels) return nullptr;
+    uint64_t chain = 0;
+    for (crn_uint32 l = 0; l < levels; l++)
+        chain += 4ull * std::max<crn_uint32>(1, comp_params.m_width >> l) *
+                 std::max<crn_uint32>(1, comp_params.m_height >> l);
+    std::vector<std::vector<uint8_t>> store(comp_params.m_faces, std::vector<uint8_t>(chain));
+    crn_comp_params q(comp_params);
+    q.m_levels = levels;
+    for (crn_uint32 f = 0; f < comp_params.m_faces; f++) {
+        if (crn2_generate_mipmaps(comp_params.m_pImages[f][0], comp_params.m_width, comp_params.m_height, &mp,
+                                  store[f].data(), chain) != CRN2_OK)
+            return nullptr;
+        uint64_t ofs = 0;
+        for (crn_uint32 l = 0; l < levels; l++) {
+            q.m_pImages[f][l] = reinterpret_cast<const crn_uint32*>(store[f].data() + ofs);
+            ofs += 4ull * std::max<crn_uint32>(1, comp_params.m_width >> l) *
+                   std::max<crn_uint32>(1, comp_params.m_height >> l);
+        }
+    }
+    return compress_levels(q, levels, compressed_size, pActual_quality_level, pActual_bitrate);
 }
 
 void* crn_decompress_crn_to_dds(const void* pCRN_file_data, crn_uint32& file_size)

@@ -6,18 +6,23 @@
 // Every level is resampled from level 0 in two passes.  A pass is a set of
 // sequential dot products: output o of line k is
 //     s = 0;  s += in[k][p] * w  for (p, w) in list(o), in list order,
-// (the second pass starts from the first product, which differs from 0 + it
-// only in the sign of a zero) with separate float multiplies and adds
-// (-ffp-contract=off), so every sum rounds exactly as the reference's.
+// (the reference's Y pass starts from the first product, which differs from
+// 0 + it only in the sign of a zero) with separate float multiplies and adds
+// (-ffp-contract=off), so every sum rounds exactly as the reference's.  A
+// list is stored as runs of contributors: a run of one texel repeated (the
+// clamped edge of a wide filter: most of the contributors of the deepest
+// levels) loads it once; a run of consecutive texels walks them.
 // The order of the passes is the reference's choice per level: X then Y
 // (always, for the multithreaded resampler) or, where the single-thread
 // Resampler delays X, Y then X.  Y then X is run as X then Y on the
 // transposed image -- the same sums in the same order.
 //
-// Layout in HBM (floats, one plane per component c):
-//   level-0 planes ... lx[c][x][y] (column-major: the lanes of a wave are
-//                      64 consecutive rows, so every contributor column is one
-//                      coalesced 256-byte load) and/or ly[c][y][x];
+// Layout in HBM (one plane per component c):
+//   level-0 planes ... bytes, lx[c][x][y] (column-major: the lanes of a wave
+//                      are 64 consecutive rows, so every contributor column is
+//                      one coalesced 64-byte load) and/or ly[c][y][x]; every
+//                      level re-reads them, so bytes rather than floats (the
+//                      sRGB table sits in LDS) cut the dominant traffic 4x;
 //   pass-1 output .... inter[c][line][o], o along the first pass's axis;
 //   pass 2 ........... lanes = 64 consecutive o, one wave per component, the
 //                      contributor list of the block's output line in scalar
@@ -40,12 +45,18 @@
 
 namespace {
 
-constexpr int kOutPerLane = 4;  // pass-1 outputs per lane (neighbours share columns)
+// pass-1 outputs per lane: a lane's outputs are contiguous in its line of
+// the intermediate, so 16 of them leave as four 16-byte stores (lanes are
+// lines, a line apart); a level whose outputs have long lists takes one per
+// lane (shorter serial chains, more waves)
+constexpr uint32_t kOutPerLane = 16;
+constexpr uint32_t kLongList = 256;
 
 struct Pass1Level {
     uint32_t unit0;   // first wave of this level
     uint32_t nlb;     // blocks of 64 lines
-    uint32_t runs;    // runs of kOutPerLane outputs
+    uint32_t per;     // outputs per lane (1 or kOutPerLane)
+    uint32_t runs;    // groups of `per` outputs
     uint32_t lines;   // lines (rows for X first, columns for Y first)
     uint32_t d;       // outputs per line
     uint32_t s;       // source samples per line
@@ -66,11 +77,16 @@ struct Pass2Level {
     uint64_t out;  // texel offset of this level in the chain
 };
 
+// contributors pix[k] .. pix[k] + (n - 1) * step with weights wt[w .. w + n)
+struct Run {
+    uint32_t pix, n, step, w;
+};
+
 struct Tables {
     const Pass1Level* p1;
     const Pass2Level* p2;
-    const uint32_t* offs;
-    const uint32_t* pix;
+    const uint32_t* sofs;  // runs of output o of a list: sofs[list + o] .. sofs[list + o + 1]
+    const Run* runs;
     const float* wt;
     const float* s2l;
     const uint8_t* l2s;
@@ -79,22 +95,74 @@ struct Tables {
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ __forceinline__ float unif(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 
+// Source samples: float (the intermediate), or level-0 bytes converted as
+// resample_single_thread does (crn_image_utils.cpp:575-590): colour through
+// srgb_to_linear when sRGB (the 256 floats staged in LDS), alpha and linear
+// colour v * (1/255).
+enum Src { kFloat, kLinear8, kTable8 };
+
+template <Src S>
+__device__ __forceinline__ float sample(const void* base, uint32_t i, const float* tab)
+{
+    if constexpr (S == kFloat) {
+        return static_cast<const float*>(base)[i];
+    } else {
+        const uint32_t v = static_cast<const uint8_t*>(base)[i];
+        if constexpr (S == kTable8) return tab[v];
+        else return __fmul_rn((float)v, 1.0f / 255.0f);
+    }
+}
+
+// s + src(p * stride + lane) * w over the runs [rb, re) of one output, in
+// order.  Lists, runs and weights are wave-uniform (scalar loads); each
+// contributor of a consecutive run is one coalesced load of 64 samples, 16 in
+// flight.  A batch past the end of a run repeats its last texel with weight
+// 0: s + v * 0 == s (up to the sign of a zero sum), so the sums are the
+// reference's.  The weights of a batch are read as 16 consecutive floats
+// (the table buffer continues past the last list).
+template <Src S>
+__device__ __forceinline__ float dot_runs(const Tables& t, uint32_t rb, uint32_t re, const void* base,
+                                          uint32_t lane, uint32_t stride, float s, const float* tab)
+{
+    for (uint32_t g = rb; g < re; g++) {
+        const Run r = t.runs[g];
+        const uint32_t n = uni(r.n), p = uni(r.pix);
+        const float* wp = t.wt + uni(r.w);
+        if (uni(r.step) == 0) {
+            const float v = sample<S>(base, p * stride + lane, tab);
+            uint32_t k = 0;
+            for (; k + 16 <= n; k += 16) {
+#pragma unroll
+                for (int i = 0; i < 16; i++) s = __fadd_rn(s, __fmul_rn(v, unif(wp[k + i])));
+            }
+            for (; k < n; k++) s = __fadd_rn(s, __fmul_rn(v, unif(wp[k])));
+        } else {
+            for (uint32_t k = 0; k < n; k += 16) {
+                float v[16];
+#pragma unroll
+                for (int i = 0; i < 16; i++) v[i] = sample<S>(base, (p + min(k + i, n - 1)) * stride + lane, tab);
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const float w = unif(wp[k + i]);
+                    s = __fadd_rn(s, __fmul_rn(v[i], k + i < n ? w : 0.0f));
+                }
+            }
+        }
+    }
+    return s;
+}
+
 // ------------------------------------------------------------ level-0 planes
 
-// RGBA8 -> float planes: colour through srgb_to_linear when sRGB, alpha (and
-// colour otherwise) v * (1/255) (crn_image_utils.cpp:575-590).  32x32 tiles;
-// the column-major planes are written from the transposed tile.
+// RGBA8 -> one byte plane per component, column-major (lx) and/or row-major
+// (ly); 32x32 tiles, the column-major planes written from the transposed tile.
 __global__ void __launch_bounds__(256) mip_planes_kernel(const uint32_t* __restrict__ rgba, uint32_t w, uint32_t h,
-                                                         float* __restrict__ lx, float* __restrict__ ly,
-                                                         const float* __restrict__ s2l, uint32_t ncomp, int srgb)
+                                                         uint8_t* __restrict__ lx, uint8_t* __restrict__ ly,
+                                                         uint32_t ncomp)
 {
     __shared__ uint32_t tile[32][33];
     const uint32_t tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     const uint32_t x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
-    auto conv = [&](uint32_t p, uint32_t c) {
-        const uint32_t v = (p >> (8 * c)) & 255;
-        return (srgb && c < 3) ? s2l[v] : (float)v * (1.0f / 255.0f);
-    };
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const uint32_t y = y0 + ty + 8 * i, x = x0 + tx;
@@ -102,7 +170,7 @@ __global__ void __launch_bounds__(256) mip_planes_kernel(const uint32_t* __restr
         const uint32_t p = in ? rgba[(size_t)y * w + x] : 0;
         tile[ty + 8 * i][tx] = p;
         if (ly && in)
-            for (uint32_t c = 0; c < ncomp; c++) ly[((size_t)c * h + y) * w + x] = conv(p, c);
+            for (uint32_t c = 0; c < ncomp; c++) ly[((size_t)c * h + y) * w + x] = (uint8_t)(p >> (8 * c));
     }
     if (!lx) return;
     __syncthreads();
@@ -111,20 +179,23 @@ __global__ void __launch_bounds__(256) mip_planes_kernel(const uint32_t* __restr
         const uint32_t x = x0 + ty + 8 * i, y = y0 + tx;
         if (x < w && y < h) {
             const uint32_t p = tile[tx][ty + 8 * i];
-            for (uint32_t c = 0; c < ncomp; c++) lx[((size_t)c * w + x) * h + y] = conv(p, c);
+            for (uint32_t c = 0; c < ncomp; c++) lx[((size_t)c * w + x) * h + y] = (uint8_t)(p >> (8 * c));
         }
     }
 }
 
 // ------------------------------------------------------------------- pass 1
 
-// One wave per (level, component, 64 lines, kOutPerLane outputs); lanes are
-// lines, the contributor list is wave-uniform (scalar loads), each
-// contributor one coalesced load of 64 consecutive floats.
-__global__ void __launch_bounds__(256) mip_pass1_kernel(Tables t, const float* __restrict__ lx,
-                                                        const float* __restrict__ ly, float* __restrict__ inter,
-                                                        uint32_t nlevels, uint32_t units)
+// One wave per (level, component, 64 lines, 1 or kOutPerLane outputs); lanes
+// are lines.  Results wait in LDS (each lane's own) and leave as rows.
+__global__ void __launch_bounds__(256) mip_pass1_kernel(Tables t, const uint8_t* __restrict__ lx,
+                                                        const uint8_t* __restrict__ ly, float* __restrict__ inter,
+                                                        uint32_t nlevels, uint32_t units, uint32_t ncomp, int srgb)
 {
+    __shared__ float stage[4][kOutPerLane][64];
+    __shared__ float tab[256];
+    tab[threadIdx.x] = t.s2l[threadIdx.x];
+    __syncthreads();
     const uint32_t u = uni(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (u >= units) return;
     uint32_t l = 0;
@@ -137,27 +208,26 @@ __global__ void __launch_bounds__(256) mip_pass1_kernel(Tables t, const float* _
     const uint32_t line = lb * 64 + (threadIdx.x & 63);
     const bool valid = line < L.lines;
     const uint32_t rl = valid ? line : L.lines - 1;
-    const float* in = (L.layout ? ly : lx) + (size_t)c * L.s * L.lines + rl;
-    float* out = inter + L.inter + ((size_t)c * L.lines + rl) * L.d;
-    const size_t stride = L.lines;
-    for (int r = 0; r < kOutPerLane; r++) {
-        const uint32_t o = run * kOutPerLane + r;
-        if (o >= L.d) break;
-        const uint32_t k1 = uni(t.offs[L.list + o + 1]);
-        uint32_t k = uni(t.offs[L.list + o]);
-        float s = 0.0f;
-        for (; k + 8 <= k1; k += 8) {
-            float v[8], w[8];
+    const uint8_t* base = (L.layout ? ly : lx) + (size_t)c * L.s * L.lines;
+    float* out = inter + L.inter + ((size_t)c * L.lines + rl) * L.d + run * L.per;
+    const uint32_t nout = min(L.per, L.d - run * L.per);
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const bool table = srgb && c < 3;
+    for (uint32_t r = 0; r < nout; r++) {
+        const uint32_t o = run * L.per + r;
+        const uint32_t rb = uni(t.sofs[L.list + o]), re = uni(t.sofs[L.list + o + 1]);
+        stage[w][r][lane] = table ? dot_runs<kTable8>(t, rb, re, base, rl, L.lines, 0.0f, tab)
+                                  : dot_runs<kLinear8>(t, rb, re, base, rl, L.lines, 0.0f, tab);
+    }
+    if (!valid) return;
+    // each lane reads back only what it wrote
+    if (nout == kOutPerLane && !(reinterpret_cast<uintptr_t>(out) & 15)) {
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
-                v[i] = in[(size_t)uni(t.pix[k + i]) * stride];
-                w[i] = unif(t.wt[k + i]);
-            }
-#pragma unroll
-            for (int i = 0; i < 8; i++) s = __fadd_rn(s, __fmul_rn(v[i], w[i]));
-        }
-        for (; k < k1; k++) s = __fadd_rn(s, __fmul_rn(in[(size_t)uni(t.pix[k]) * stride], unif(t.wt[k])));
-        if (valid) out[o] = s;
+        for (uint32_t q = 0; q < kOutPerLane; q += 4)
+            *reinterpret_cast<float4*>(out + q) =
+                make_float4(stage[w][q][lane], stage[w][q + 1][lane], stage[w][q + 2][lane], stage[w][q + 3][lane]);
+    } else {
+        for (uint32_t r = 0; r < nout; r++) out[r] = stage[w][r][lane];
     }
 }
 
@@ -222,22 +292,9 @@ __global__ void __launch_bounds__(256) mip_pass2_kernel(Tables t, const float* _
     const uint32_t rx = valid ? x : L.d - 1;
     uint32_t val = 255;
     if (c < ncomp) {
-        const float* in = inter + L.inter + (size_t)c * L.lines * L.d + rx;
-        const size_t stride = L.d;
-        const uint32_t k0 = uni(t.offs[L.list + o2]), k1 = uni(t.offs[L.list + o2 + 1]);
-        float s = __fmul_rn(in[(size_t)uni(t.pix[k0]) * stride], unif(t.wt[k0]));
-        uint32_t k = k0 + 1;
-        for (; k + 8 <= k1; k += 8) {
-            float v[8], w[8];
-#pragma unroll
-            for (int i = 0; i < 8; i++) {
-                v[i] = in[(size_t)uni(t.pix[k + i]) * stride];
-                w[i] = unif(t.wt[k + i]);
-            }
-#pragma unroll
-            for (int i = 0; i < 8; i++) s = __fadd_rn(s, __fmul_rn(v[i], w[i]));
-        }
-        for (; k < k1; k++) s = __fadd_rn(s, __fmul_rn(in[(size_t)uni(t.pix[k]) * stride], unif(t.wt[k])));
+        const float* base = inter + L.inter + (size_t)c * L.lines * L.d;
+        float s = dot_runs<kFloat>(t, uni(t.sofs[L.list + o2]), uni(t.sofs[L.list + o2 + 1]), base, rx, L.d, 0.0f,
+                                   nullptr);
         s = clamp01(s);
         if (!srgb || c == 3) {
             const int v = (int)__fadd_rn(__fmul_rn(255.0f, s), .5f);
@@ -269,9 +326,10 @@ struct crn2_mip_plan {
     uint64_t chain_bytes;
     // device
     void* tables = nullptr;
-    float* scratch = nullptr;
+    void* scratch = nullptr;
     Tables t{};
-    float *lx = nullptr, *ly = nullptr, *inter = nullptr;
+    uint8_t *lx = nullptr, *ly = nullptr;  // level-0 byte planes
+    float* inter = nullptr;
     uint32_t units1 = 0, blocks2 = 0;
     // stage timing (HIP events on the run stream): planes, pass 1, pass 2
     bool prof = false;
@@ -363,16 +421,32 @@ int crn2_mip_plan_create(uint32_t width, uint32_t height, const crn2_mip_params*
     // host image of the device tables
     std::vector<Pass1Level> p1(nl);
     std::vector<Pass2Level> p2(nl);
-    std::vector<uint32_t> offs, pix;
+    std::vector<uint32_t> sofs;
+    std::vector<Run> runs;
     std::vector<float> wt;
     bool need_lx = false, need_ly = false;
     uint64_t inter_floats = 0;
     uint32_t units = 0, blocks = 0;
+    // a list as runs: one texel repeated (step 0) or consecutive texels (step 1)
     auto add_list = [&](const crn2mip::List& L) {
-        const uint32_t at = (uint32_t)offs.size(), base = (uint32_t)pix.size();
-        for (uint32_t o : L.offs) offs.push_back(base + o);
-        pix.insert(pix.end(), L.pix.begin(), L.pix.end());
+        const uint32_t at = (uint32_t)sofs.size(), base = (uint32_t)wt.size();
         wt.insert(wt.end(), L.wt.begin(), L.wt.end());
+        for (size_t o = 0; o + 1 < L.offs.size(); o++) {
+            sofs.push_back((uint32_t)runs.size());
+            for (uint32_t k = L.offs[o], end = L.offs[o + 1]; k < end;) {
+                const uint32_t p = L.pix[k];
+                uint32_t n = 1, step = 1;
+                if (k + 1 < end && L.pix[k + 1] == p) {
+                    step = 0;
+                    while (k + n < end && L.pix[k + n] == p) n++;
+                } else {
+                    while (k + n < end && L.pix[k + n] == p + n) n++;
+                }
+                runs.push_back(Run{p, n, step, base + k});
+                k += n;
+            }
+        }
+        sofs.push_back((uint32_t)runs.size());
         return at;
     };
     // deepest level first in both launches
@@ -389,7 +463,8 @@ int crn2_mip_plan_create(uint32_t width, uint32_t height, const crn2_mip_params*
         a.layout = v.xfirst ? 0 : 1;
         (v.xfirst ? need_lx : need_ly) = true;
         a.nlb = (a.lines + 63) / 64;
-        a.runs = (a.d + kOutPerLane - 1) / kOutPerLane;
+        a.per = first.ops() / a.d >= kLongList ? 1 : kOutPerLane;
+        a.runs = (a.d + a.per - 1) / a.per;
         a.unit0 = units;
         units += nc * a.nlb * a.runs;
         a.list = add_list(first);
@@ -418,12 +493,12 @@ int crn2_mip_plan_create(uint32_t width, uint32_t height, const crn2_mip_params*
     at = align_up(at + nl * sizeof(Pass1Level), 256);
     const size_t o_p2 = at;
     at = align_up(at + nl * sizeof(Pass2Level), 256);
-    const size_t o_offs = at;
-    at = align_up(at + offs.size() * 4, 256);
-    const size_t o_pix = at;
-    at = align_up(at + pix.size() * 4, 256);
+    const size_t o_sofs = at;
+    at = align_up(at + sofs.size() * 4, 256);
+    const size_t o_runs = at;
+    at = align_up(at + runs.size() * sizeof(Run), 256);
     const size_t o_wt = at;
-    at = align_up(at + wt.size() * 4, 256);
+    at = align_up(at + wt.size() * 4 + 64, 256);  // batches read up to 15 weights past a list
     const size_t o_s2l = at;
     at = align_up(at + sizeof(s2l), 256);
     const size_t o_l2s = at;
@@ -431,15 +506,15 @@ int crn2_mip_plan_create(uint32_t width, uint32_t height, const crn2_mip_params*
     std::vector<uint8_t> host(at, 0);
     std::memcpy(host.data() + o_p1, p1.data(), nl * sizeof(Pass1Level));
     std::memcpy(host.data() + o_p2, p2.data(), nl * sizeof(Pass2Level));
-    std::memcpy(host.data() + o_offs, offs.data(), offs.size() * 4);
-    std::memcpy(host.data() + o_pix, pix.data(), pix.size() * 4);
+    std::memcpy(host.data() + o_sofs, sofs.data(), sofs.size() * 4);
+    std::memcpy(host.data() + o_runs, runs.data(), runs.size() * sizeof(Run));
     std::memcpy(host.data() + o_wt, wt.data(), wt.size() * 4);
     std::memcpy(host.data() + o_s2l, s2l, sizeof(s2l));
     std::memcpy(host.data() + o_l2s, l2s.data(), l2s.size());
-    const uint64_t plane = (uint64_t)nc * width * height;
-    const uint64_t scratch_floats = (need_lx ? plane : 0) + (need_ly ? plane : 0) + inter_floats;
+    const uint64_t plane = align_up((uint64_t)nc * width * height, 256);  // bytes
+    const uint64_t scratch_bytes = (need_lx ? plane : 0) + (need_ly ? plane : 0) + inter_floats * sizeof(float);
     int rc = CRN2_OK;
-    if (!hip_ok(hipMalloc(&p->tables, at)) || !hip_ok(hipMalloc(&p->scratch, scratch_floats * sizeof(float))))
+    if (!hip_ok(hipMalloc(&p->tables, at)) || !hip_ok(hipMalloc(&p->scratch, scratch_bytes)))
         rc = CRN2_ENOMEM;
     else if (!hip_ok(hipMemcpy(p->tables, host.data(), at, hipMemcpyHostToDevice)))
         rc = CRN2_EHIP;
@@ -450,15 +525,15 @@ int crn2_mip_plan_create(uint32_t width, uint32_t height, const crn2_mip_params*
     const uint8_t* d = static_cast<const uint8_t*>(p->tables);
     p->t.p1 = reinterpret_cast<const Pass1Level*>(d + o_p1);
     p->t.p2 = reinterpret_cast<const Pass2Level*>(d + o_p2);
-    p->t.offs = reinterpret_cast<const uint32_t*>(d + o_offs);
-    p->t.pix = reinterpret_cast<const uint32_t*>(d + o_pix);
+    p->t.sofs = reinterpret_cast<const uint32_t*>(d + o_sofs);
+    p->t.runs = reinterpret_cast<const Run*>(d + o_runs);
     p->t.wt = reinterpret_cast<const float*>(d + o_wt);
     p->t.s2l = reinterpret_cast<const float*>(d + o_s2l);
     p->t.l2s = d + o_l2s;
-    float* f = p->scratch;
+    uint8_t* f = static_cast<uint8_t*>(p->scratch);
     if (need_lx) { p->lx = f; f += plane; }
     if (need_ly) { p->ly = f; f += plane; }
-    p->inter = f;
+    p->inter = reinterpret_cast<float*>(f);
     *out = p;
     return CRN2_OK;
 }
@@ -485,11 +560,10 @@ int crn2_mip_plan_run(crn2_mip_plan* p, const void* rgba, void* out, uint64_t ou
     const int srgb = (p->params.flags & CRN2_MIP_SRGB) != 0;
     if (p->prof) (void)hipEventRecord(p->ev[0], st);
     hipLaunchKernelGGL(mip_planes_kernel, dim3((p->width + 31) / 32, (p->height + 31) / 32), dim3(256), 0, st,
-                       static_cast<const uint32_t*>(rgba), p->width, p->height, p->lx, p->ly, p->t.s2l, p->ncomp,
-                       srgb);
+                       static_cast<const uint32_t*>(rgba), p->width, p->height, p->lx, p->ly, p->ncomp);
     if (p->prof) (void)hipEventRecord(p->ev[1], st);
     hipLaunchKernelGGL(mip_pass1_kernel, dim3((p->units1 + 3) / 4), dim3(256), 0, st, p->t, p->lx, p->ly, p->inter,
-                       p->levels - 1, p->units1);
+                       p->levels - 1, p->units1, p->ncomp, srgb);
     if (p->prof) (void)hipEventRecord(p->ev[2], st);
     hipLaunchKernelGGL(mip_pass2_kernel, dim3(p->blocks2), dim3(256), 0, st, p->t, p->inter,
                        static_cast<uint32_t*>(out), p->levels - 1, p->ncomp, srgb,
@@ -541,6 +615,29 @@ int crn2_generate_mipmaps_device(const void* rgba, uint32_t width, uint32_t heig
     if (rc != CRN2_OK) return rc;
     rc = crn2_mip_plan_run(p, rgba, out, out_size, stream);
     if (hipStreamSynchronize(static_cast<hipStream_t>(stream)) != hipSuccess && rc == CRN2_OK) rc = CRN2_EHIP;
+    plan_free(p);
+    return rc;
+}
+
+int crn2_generate_mipmaps(const void* rgba, uint32_t width, uint32_t height, const crn2_mip_params* params,
+                          void* out, uint64_t out_size)
+{
+    if (!rgba || !out) return CRN2_EINVAL;
+    crn2_mip_plan* p = nullptr;
+    int rc = crn2_mip_plan_create(width, height, params, &p);
+    if (rc != CRN2_OK) return rc;
+    if (out_size < p->chain_bytes) {
+        plan_free(p);
+        return CRN2_EINVAL;
+    }
+    const uint64_t l0 = (uint64_t)width * height * 4;
+    void *d_in = nullptr, *d_out = nullptr;
+    if (!hip_ok(hipMalloc(&d_in, l0)) || !hip_ok(hipMalloc(&d_out, p->chain_bytes))) rc = CRN2_ENOMEM;
+    if (rc == CRN2_OK && !hip_ok(hipMemcpy(d_in, rgba, l0, hipMemcpyHostToDevice))) rc = CRN2_EHIP;
+    if (rc == CRN2_OK) rc = crn2_mip_plan_run(p, d_in, d_out, p->chain_bytes, nullptr);
+    if (rc == CRN2_OK && !hip_ok(hipMemcpy(out, d_out, p->chain_bytes, hipMemcpyDeviceToHost))) rc = CRN2_EHIP;
+    if (d_in) (void)hipFree(d_in);
+    if (d_out) (void)hipFree(d_out);
     plan_free(p);
     return rc;
 }

@@ -649,6 +649,9 @@ int crn2_mip_plan_profile(crn2_mip_plan* plan, int on, double* ms, uint64_t* run
 /* One-shot form: plan, run on `stream`, wait, free. */
 int crn2_generate_mipmaps_device(const void* rgba, uint32_t width, uint32_t height, const crn2_mip_params* params,
                                  void* out, uint64_t out_size, void* stream);
+/* Host-memory form (the path of crn_compress): rgba and out in host memory. */
+int crn2_generate_mipmaps(const void* rgba, uint32_t width, uint32_t height, const crn2_mip_params* params,
+                          void* out, uint64_t out_size);
 /* Host copy of the contributor list of one axis (Resampler::make_clist,
  * crn_resampler.cpp:117-416): offs[dst + 1] and up to cap (pixel, weight)
  * pairs; returns the pair count, or a negative CRN2_E* code. */

@@ -12,9 +12,12 @@
  *       searches of every level (crn2_encode_image_device); CRN output = the
  *       GPU dxt_hc (crn2_hc_compress_device) and the project's CRN writer
  *       (valid .crn, transcodes bit-exactly, but not byte-identical to the
- *       reference's writer).  Clustered DDS (quality < 255 or a target
- *       bitrate with DDS output), ETC formats, DXT3 in CRN and mipmap
- *       generation other than "use the source levels" return NULL.
+ *       reference's writer).  With crn_mipmap_params, levels are generated
+ *       on the GPU as mipmapped_texture::generate_mipmaps makes them
+ *       (crn2_generate_mipmaps, bit-exact) when the mode asks for it.
+ *       Clustered DDS (quality < 255 or a target bitrate with DDS output),
+ *       ETC formats, DXT3 in CRN, scaling, cropping and a renormalized top
+ *       level return NULL.
  *   crn_decompress_crn_to_dds ........ GPU transcode of every level and face
  *       into a DDS file laid out as mipmapped_texture::write_dds writes it
  *       (crnlib/crn_mipmapped_texture.cpp:921-1140).

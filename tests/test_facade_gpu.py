@@ -286,3 +286,53 @@ def test_dxt5_optimizer_class(gpu):
         assert r.error == err_ref[i], i
         assert (r.first, r.second) == (first[i], second[i])
         assert np.array_equal(sel, sel_ref[i])
+
+
+class MipmapParams(C.Structure):
+    # include/cxx/crnlib.h's crn_mipmap_params: 21 four-byte members
+    _fields_ = [("size_of_obj", C.c_uint32), ("mode", C.c_uint32), ("filter", C.c_uint32),
+                ("gamma_filtering", C.c_uint32), ("gamma", C.c_float), ("blurriness", C.c_float),
+                ("max_levels", C.c_uint32), ("min_mip_size", C.c_uint32), ("renormalize", C.c_uint32),
+                ("rtopmip", C.c_uint32), ("tiled", C.c_uint32), ("scale_mode", C.c_uint32), ("scale_x", C.c_float),
+                ("scale_y", C.c_float), ("window", C.c_uint32 * 4), ("clamp_scale", C.c_uint32),
+                ("clamp_width", C.c_uint32), ("clamp_height", C.c_uint32)]
+
+
+def mip_params(**kw):
+    m = MipmapParams()
+    m.size_of_obj = C.sizeof(MipmapParams)
+    m.mode, m.filter, m.gamma_filtering, m.gamma, m.blurriness = 0, 4, 1, 2.2, 0.9
+    m.max_levels, m.min_mip_size, m.scale_x, m.scale_y = 16, 1, 1.0, 1.0
+    for k, v in kw.items():
+        setattr(m, k, v)
+    return m
+
+
+@pytest.mark.parametrize("case", [
+    dict(mode=2),                                              # generate, crnlib defaults (Kaiser, sRGB)
+    dict(mode=0, filter=0, blurriness=1.0, tiled=1),           # source-or-generate, one level given
+    dict(mode=2, filter=2, gamma_filtering=0, max_levels=4, min_mip_size=2),
+    dict(mode=2, filter=3, helper_threads=2, gamma=1.75),      # threaded order; gamma is not used
+])
+def test_crn_compress_generates_mips_like_generate_mipmaps(gpu, case):
+    case = dict(case)
+    threads = case.pop("helper_threads", 0)
+    img = texture.synth(96, 64, 7, texture.RGBA)
+    m = mip_params(**case)
+    p = comp_params(96, 64, 1, 0, 1)
+    p.num_helper_threads = threads
+    keep = np.ascontiguousarray(img)
+    p.images[0][0] = keep.ctypes.data
+    f = fn("_Z12crn_compressRK15crn_comp_paramsRK17crn_mipmap_paramsRjPjPf", C.c_void_p,
+           [C.POINTER(CompParams), C.POINTER(MipmapParams), C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p])
+    size = C.c_uint32(0)
+    out = f(C.byref(p), C.byref(m), C.byref(size), None, None)
+    assert out
+    raw = C.string_at(out, size.value)
+    fn("_Z14crn_free_blockPv", None, [C.c_void_p])(out)
+    flags = (oracle.MIP_SRGB if m.gamma_filtering else 0) | (oracle.MIP_WRAP if m.tiled else 0) | \
+        oracle.MIP_ALPHA | (oracle.MIP_THREADED if threads else 0)
+    levels, _ = oracle.mip_generate(img, m.filter, m.blurriness, 2.2, flags, m.max_levels, m.min_mip_size)
+    p2 = comp_params(96, 64, len(levels), 0, 1)
+    exp = compress(p2, levels)
+    assert raw == exp
