@@ -385,9 +385,9 @@ def mixed_leg(count, cpu_seconds):
                           "inputs": "the simple block quantizer's .crn files at quality-128 palette sizes"}}
 
 
-# HBM bytes mip_pass1_kernel moves per 8192^2 chain (FETCH_SIZE + WRITE_SIZE,
-# scaled by a device copy of known size); 0 until measured
-MIP_PASS1_TRAFFIC = 0
+# HBM bytes mip_pass1_kernel moves per 8192^2 chain (FETCH_SIZE x 2 +
+# WRITE_SIZE x 1, the factors a device copy of known size reads back)
+MIP_PASS1_TRAFFIC = 17.834e9
 FP32_PEAK_TFLOPS = 157.3   # MI355X dense fp32 (packed FMA); one multiply and one add per
                            # contributor, unfused and unpacked, reach a quarter of it
 
