@@ -357,6 +357,7 @@ __device__ inline uint32_t dxt1_error(const int (*px)[3], uint32_t c0, uint32_t 
         }
     }
     uint32_t err = 0, sel = 0;
+#pragma unroll
     for (int i = 0; i < 16; i++) {
         uint32_t best = 0xFFFFFFFFu, bs = 0;
         for (int s = 0; s < 4; s++) {
@@ -394,6 +395,7 @@ __device__ inline uint32_t dxt1_error_below(const int (*px)[3], uint32_t c0, uin
         }
     }
     uint32_t err = 0;
+#pragma unroll
     for (int i = 0; i < 16; i++) {
         uint32_t best = 0xFFFFFFFFu;
         for (int s = 0; s < 4; s++) {
@@ -448,7 +450,9 @@ __device__ inline bool descend(const int (*px)[3], int (*e)[3], uint32_t& best, 
                     }
                 }
         if (bs < 0) return true;
-        e[bs][bk] += bd;
+        for (int s = 0; s < 2; s++)      // static indices: e stays in registers
+            for (int k = 0; k < 3; k++)
+                if (s == bs && k == bk) e[s][k] += bd;
         best = cand;
         steps++;
     }
@@ -463,6 +467,7 @@ __device__ inline bool lsq_endpoints(const int (*px)[3], uint32_t lo, uint32_t h
     dxt1_error(px, lo, hi, &sel);
     const int u_of[4] = {0, 3, 1, 2};   // 3 w by selector
     int a11 = 0, a12 = 0, a22 = 0, b1[3] = {0, 0, 0}, b2[3] = {0, 0, 0};
+#pragma unroll
     for (int i = 0; i < 16; i++) {
         const int u = u_of[(sel >> (2 * i)) & 3u], v = 3 - u;
         a11 += v * v;
@@ -551,10 +556,12 @@ __device__ inline bool box_step(const int (*px)[3], int (*e)[3], uint32_t& best)
 __device__ inline void pca_start(const int (*px)[3], int (*e)[3])
 {
     float m[3] = {0, 0, 0};
+#pragma unroll
     for (int i = 0; i < 16; i++)
         for (int k = 0; k < 3; k++) m[k] += (float)px[i][k];
     for (int k = 0; k < 3; k++) m[k] *= 1.0f / 16.0f;
     float c[6] = {0, 0, 0, 0, 0, 0};   // xx xy xz yy yz zz
+#pragma unroll
     for (int i = 0; i < 16; i++) {
         const float x = px[i][0] - m[0], y = px[i][1] - m[1], z = px[i][2] - m[2];
         c[0] += x * x;
@@ -577,6 +584,7 @@ __device__ inline void pca_start(const int (*px)[3], int (*e)[3])
     }
     const float n2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
     float tmin = 0.0f, tmax = 0.0f;
+#pragma unroll
     for (int i = 0; i < 16; i++) {
         const float t = ((px[i][0] - m[0]) * v[0] + (px[i][1] - m[1]) * v[1] + (px[i][2] - m[2]) * v[2]) / n2;
         tmin = fminf(tmin, t);
@@ -587,6 +595,62 @@ __device__ inline void pca_start(const int (*px)[3], int (*e)[3])
         e[0][k] = (int)fminf(fmaxf(rintf((m[k] + tmax * v[k]) * lim[k] / 255.0f), 0.0f), lim[k]);
         e[1][k] = (int)fminf(fmaxf(rintf((m[k] + tmin * v[k]) * lim[k] / 255.0f), 0.0f), lim[k]);
     }
+}
+
+// Sorts the 16 texels by falling error under endpoints e (a fixed sorting
+// network: the texels stay in registers); nibble i of *perm becomes the
+// original index of texel i.
+__device__ inline void order_texels(int (*px)[3], const int (*e)[3], uint64_t* perm)
+{
+    *perm = 0xFEDCBA9876543210ull;
+    uint32_t lo, hi;
+    {
+        const uint32_t ca = pack565(e[0]), cb = pack565(e[1]);
+        lo = ca > cb ? ca : cb;
+        hi = ca > cb ? cb : ca;
+    }
+    int p[4][3];
+    expand565(lo, p[0]);
+    expand565(hi, p[1]);
+    const bool four = lo > hi;
+    for (int k = 0; k < 3; k++) {
+        p[2][k] = four ? (2 * p[0][k] + p[1][k]) / 3 : (p[0][k] + p[1][k]) / 2;
+        p[3][k] = four ? (p[0][k] + 2 * p[1][k]) / 3 : 0;
+    }
+    uint32_t key[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        uint32_t best = 0xFFFFFFFFu;
+        for (int s = 0; s < 4; s++) {
+            const int dr = px[i][0] - p[s][0], dg = px[i][1] - p[s][1], db = px[i][2] - p[s][2];
+            best = min(best, (uint32_t)(8 * dr * dr + 25 * dg * dg + db * db));
+        }
+        key[i] = best;
+    }
+    // Batcher's odd-even merge sort network for 16 elements, descending
+#pragma unroll
+    for (int pp = 1; pp < 16; pp <<= 1)
+#pragma unroll
+        for (int k = pp; k >= 1; k >>= 1)
+#pragma unroll
+            for (int j = k % pp; j + k < 16; j += 2 * k)
+#pragma unroll
+                for (int i = 0; i < k; i++)
+                    if (i + j + k < 16 && (i + j) / (2 * pp) == (i + j + k) / (2 * pp)) {
+                        const int a = i + j, b = i + j + k;
+                        if (key[a] < key[b]) {
+                            const uint32_t tk = key[a];
+                            key[a] = key[b];
+                            key[b] = tk;
+                            for (int c = 0; c < 3; c++) {
+                                const int tv = px[a][c];
+                                px[a][c] = px[b][c];
+                                px[b][c] = tv;
+                            }
+                            const uint64_t na = (*perm >> (4 * a)) & 15u, nb = (*perm >> (4 * b)) & 15u;
+                            *perm ^= ((na ^ nb) << (4 * a)) | ((na ^ nb) << (4 * b));
+                        }
+                    }
 }
 
 // Start points, each refined, the lowest final error kept (earlier start on
@@ -604,6 +668,7 @@ __global__ void __launch_bounds__(256) refine_kernel(const uint8_t* __restrict__
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n) return;
     int px[16][3];
+#pragma unroll
     for (int i = 0; i < 16; i++)
         for (int k = 0; k < 3; k++) px[i][k] = rgba[(size_t)b * 64 + 4 * i + k];
     const int lim[3] = {31, 63, 31};
@@ -629,22 +694,25 @@ __global__ void __launch_bounds__(256) refine_kernel(const uint8_t* __restrict__
         consider(e);
     }
     if (starts & 2u) {
-        int bi = 0, di = 0, bl = -1, dl = 1 << 30;
+        // brightest and darkest texels (first extreme), kept by value so that
+        // the texels are only ever indexed statically
+        int bl = -1, dl = 1 << 30, bc[3] = {0, 0, 0}, dc[3] = {0, 0, 0};
+#pragma unroll
         for (int i = 0; i < 16; i++) {
             const int l = 2 * px[i][0] + 4 * px[i][1] + px[i][2];
             if (l > bl) {
                 bl = l;
-                bi = i;
+                for (int k = 0; k < 3; k++) bc[k] = px[i][k];
             }
             if (l < dl) {
                 dl = l;
-                di = i;
+                for (int k = 0; k < 3; k++) dc[k] = px[i][k];
             }
         }
         int e[2][3];
         for (int k = 0; k < 3; k++) {
-            e[0][k] = (int)rintf((float)px[bi][k] * (float)lim[k] / 255.0f);
-            e[1][k] = (int)rintf((float)px[di][k] * (float)lim[k] / 255.0f);
+            e[0][k] = (int)rintf((float)bc[k] * (float)lim[k] / 255.0f);
+            e[1][k] = (int)rintf((float)dc[k] * (float)lim[k] / 255.0f);
         }
         consider(e);
     }
@@ -652,6 +720,15 @@ __global__ void __launch_bounds__(256) refine_kernel(const uint8_t* __restrict__
         int e[2][3];
         pca_start(px, e);
         consider(e);
+    }
+    uint64_t perm = 0;   // 0: texels in their own order
+    if ((starts & 16u) && best) {
+        // The lattice scans 728 candidates whose sums stop at the current
+        // best: visit the texels in order of falling error under the kept
+        // endpoints so that losing candidates stop early.  Only the order of
+        // the integer sums changes (selectors are mapped back below), so
+        // every result is the same.
+        order_texels(px, keep, &perm);
     }
     if (starts & 16u) {
         // polish the best result on the both-ends lattice, each improvement
@@ -666,6 +743,12 @@ __global__ void __launch_bounds__(256) refine_kernel(const uint8_t* __restrict__
     uint32_t lo, hi, sel;
     pair_error(px, keep[0], keep[1], lo, hi);
     best = dxt1_error(px, lo, hi, &sel);
+    if (perm) {
+        uint32_t os = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) os |= ((sel >> (2 * i)) & 3u) << (2 * ((perm >> (4 * i)) & 15u));
+        sel = os;
+    }
     out[(size_t)b * 4 + 0] = lo | (hi << 16);
     out[(size_t)b * 4 + 1] = sel;
     out[(size_t)b * 4 + 2] = best;
