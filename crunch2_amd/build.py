@@ -17,7 +17,7 @@ LIB = os.path.join(PKG, "libcrn2.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
 
-ORACLE_SOURCES = ["crnd_oracle.c", "dxt5a_oracle.c", "dxt1_oracle.c", "hc_oracle.c", "hc_endpoints.c",
+ORACLE_SOURCES = ["crnd_oracle.c", "dxt5a_oracle.c", "dxt1_oracle.c", "hc_oracle.c", "hc_endpoints.c", "qdxt_oracle.c",
                   "tree_clusterizer.c", "hc_tiles.c", "mip_oracle.c"]
 HOST_SOURCES = ["crn_pack.cpp", "synth.cpp", "crn_api.cpp", "crnd_host.cpp", "crn_files.cpp", "crnlib_facade.cpp", "mip_host.cpp"]
 HIP_SOURCES = ["crnd_gpu.hip", "dxt5a_gpu.hip", "dxt1_gpu.hip", "hc_gpu.hip", "mlp_gpu.hip", "dxt1c_gpu.hip",

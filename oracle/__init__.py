@@ -757,3 +757,174 @@ def mip_resample(img, dst_w, dst_h, filter=4, filter_scale=0.9, gamma=2.2, flags
     assert L.orc_mip_resample(_p(img), w, h, dst_w, dst_h, filter, filter_scale, gamma, flags, _p(out),
                               C.byref(order)) == 0
     return out, order.value
+
+
+# ------------------------------------------------------------------ qdxt1
+class QdxtParams(C.Structure):
+    _fields_ = [("quality_level", C.c_uint32), ("dxt_quality", C.c_int), ("flags", C.c_int),
+                ("alpha_threshold", C.c_uint32), ("pow_mul", C.c_float)]
+
+
+class QdxtStats(C.Structure):
+    _fields_ = [("codebook_size", C.c_uint32), ("max_selector_clusters", C.c_uint32),
+                ("endpoint_clusters", C.c_uint32), ("selector_clusters", C.c_uint32),
+                ("max_endpoint_clusters", C.c_uint32), ("max_selector_target", C.c_uint32)]
+
+
+QDXT_PERCEPTUAL, QDXT_ALPHA_BLOCKS, QDXT_HIERARCHICAL = 1, 2, 4
+
+
+def _vq_lib():
+    L = lib()
+    L.orc_vq_create.restype = C.c_void_p
+    L.orc_vq_create.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
+    for f in ("orc_vq_codebook_size", "orc_vq_num_nodes", "orc_vq_pops"):
+        getattr(L, f).restype = C.c_uint32
+        getattr(L, f).argtypes = [C.c_void_p]
+    L.orc_vq_node.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    L.orc_vq_node_axis.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+    L.orc_vq_retrieve.restype = C.c_uint32
+    L.orc_vq_retrieve.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+    L.orc_vq_destroy.argtypes = [C.c_void_p]
+    L.orc_vq_threaded_clusters.restype = C.c_uint32
+    L.orc_vq_threaded_clusters.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                           C.c_void_p, C.c_void_p]
+    L.orc_vq_pca_division.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                      C.POINTER(C.c_uint32), C.c_void_p, C.POINTER(C.c_uint32), C.c_void_p]
+    L.orc_qdxt1_train.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]
+    L.orc_qdxt1_distinct_selectors.restype = C.c_uint32
+    L.orc_qdxt1_distinct_selectors.argtypes = [C.c_void_p, C.c_uint32]
+    L.orc_qdxt1.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.POINTER(QdxtParams), C.c_void_p,
+                            C.c_void_p, C.POINTER(QdxtStats)]
+    L.orc_representative_colors.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+    return L
+
+
+def _clusters(idx, ofs, nc):
+    return [idx[ofs[k]:ofs[k + 1]].copy() for k in range(nc)]
+
+
+class VQ:
+    """clusterizer<vec<d,float>> grown to max_size leaves (crn_clusterizer.h)."""
+
+    def __init__(self, vecs, weights, max_size, quick=False):
+        self.L = _vq_lib()
+        self.vecs = np.ascontiguousarray(vecs, np.float32)
+        self.w = np.ascontiguousarray(weights, np.uint32)
+        self.n, self.d = self.vecs.shape
+        self.h = self.L.orc_vq_create(_p(self.vecs), _p(self.w), self.n, self.d, max_size, int(quick))
+        assert self.h
+
+    @property
+    def codebook_size(self):
+        return self.L.orc_vq_codebook_size(self.h)
+
+    @property
+    def pops(self):
+        return self.L.orc_vq_pops(self.h)
+
+    def nodes(self):
+        """[(left, right, number, unsplittable, vectors, variance, centroid)] in node order."""
+        out = []
+        for i in range(self.L.orc_vq_num_nodes(self.h)):
+            info = np.zeros(5, np.int32)
+            var = C.c_float()
+            cen = np.zeros(self.d, np.float32)
+            self.L.orc_vq_node(self.h, i, _p(info), C.byref(var), _p(cen), None)
+            vec = np.zeros(max(1, int(info[4])), np.uint32)
+            self.L.orc_vq_node(self.h, i, _p(info), C.byref(var), None, _p(vec))
+            out.append((int(info[0]), int(info[1]), int(info[2]), int(info[3]), vec[:int(info[4])], var.value, cen))
+        return out
+
+    def axis(self, i):
+        a = np.zeros(self.d, np.float32)
+        self.L.orc_vq_node_axis(self.h, i, _p(a))
+        return a
+
+    def retrieve(self, max_clusters):
+        idx = np.zeros(self.n, np.uint32)
+        ofs = np.zeros(self.n + 1, np.uint32)
+        nc = self.L.orc_vq_retrieve(self.h, max_clusters, _p(idx), _p(ofs))
+        return _clusters(idx, ofs, nc)
+
+    def close(self):
+        if self.h:
+            self.L.orc_vq_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def vq_threaded_clusters(vecs, weights, max_clusters):
+    """threaded_clusterizer::create_clusters: a list of index arrays."""
+    L = _vq_lib()
+    vecs = np.ascontiguousarray(vecs, np.float32)
+    w = np.ascontiguousarray(weights, np.uint32)
+    n, d = vecs.shape
+    idx = np.zeros(n, np.uint32)
+    ofs = np.zeros(n + 1, np.uint32)
+    nc = L.orc_vq_threaded_clusters(_p(vecs), _p(w), n, d, max_clusters, _p(idx), _p(ofs))
+    assert nc
+    return _clusters(idx, ofs, nc)
+
+
+def vq_pca_division(vecs, weights, ind):
+    """compute_pca + compute_division of the listed vectors: (left, right, axis)."""
+    L = _vq_lib()
+    vecs = np.ascontiguousarray(vecs, np.float32)
+    w = np.ascontiguousarray(weights, np.uint32)
+    ind = np.ascontiguousarray(ind, np.uint32)
+    d = vecs.shape[1]
+    left = np.zeros(len(ind), np.uint32)
+    right = np.zeros(len(ind), np.uint32)
+    nl, nr = C.c_uint32(), C.c_uint32()
+    axis = np.zeros(d, np.float32)
+    L.orc_vq_pca_division(_p(vecs), _p(w), d, _p(ind), len(ind), _p(left), C.byref(nl), _p(right), C.byref(nr),
+                          _p(axis))
+    return left[:nl.value], right[:nr.value], axis
+
+
+def representative_colors(px):
+    """dxt_fast::find_representative_colors of n RGBA8 texels: (lo rgb, hi rgb)."""
+    L = _vq_lib()
+    px = np.ascontiguousarray(px, np.uint8).reshape(-1, 4)
+    lo, hi = np.zeros(3, np.uint8), np.zeros(3, np.uint8)
+    L.orc_representative_colors(_p(px), len(px), _p(lo), _p(hi))
+    return lo, hi
+
+
+def qdxt1_train(blocks, level_desc, hierarchical=True):
+    """qdxt1::init's endpoint training vectors: (vecs [n, 6], weights [n], encodings [n])."""
+    L = _vq_lib()
+    blocks = np.ascontiguousarray(blocks, np.uint8).reshape(-1, 64)
+    n = len(blocks)
+    ld = np.ascontiguousarray(level_desc, np.uint32).reshape(-1)
+    vecs = np.zeros((n, 6), np.float32)
+    w = np.zeros(n, np.uint32)
+    enc = np.zeros(n, np.uint8)
+    assert L.orc_qdxt1_train(_p(blocks), n, len(ld) // 3, _p(ld), int(hierarchical), _p(vecs), _p(w), _p(enc))
+    return vecs, w, enc
+
+
+def qdxt1_distinct_selectors(blocks):
+    L = _vq_lib()
+    blocks = np.ascontiguousarray(blocks, np.uint8).reshape(-1, 64)
+    return L.orc_qdxt1_distinct_selectors(_p(blocks), len(blocks))
+
+
+def qdxt1(blocks, level_desc, quality_level=128, dxt_quality=4,
+          flags=QDXT_PERCEPTUAL | QDXT_ALPHA_BLOCKS | QDXT_HIERARCHICAL, alpha_threshold=128, pow_mul=1.0):
+    """qdxt1 init + pack: (blocks u64 [n], blocks after the endpoint pass, stats dict)."""
+    L = _vq_lib()
+    blocks = np.ascontiguousarray(blocks, np.uint8).reshape(-1, 64)
+    n = len(blocks)
+    ld = np.ascontiguousarray(level_desc, np.uint32).reshape(-1)
+    out = np.zeros(n, np.uint64)
+    mid = np.zeros(n, np.uint64)
+    p = QdxtParams(quality_level, dxt_quality, flags, alpha_threshold, pow_mul)
+    st = QdxtStats()
+    assert L.orc_qdxt1(_p(blocks), n, len(ld) // 3, _p(ld), C.byref(p), _p(out), _p(mid), C.byref(st))
+    stats = {f: getattr(st, f) for f, _ in QdxtStats._fields_}
+    return out, mid, stats

@@ -1,0 +1,271 @@
+"""Pins of the qdxt oracle (oracle/qdxt_oracle.c) against what the mathematics
+and independent numpy formulations fix: representative colours of one- and
+two-colour tiles, the tile choice recomputed in image space, the
+clusterizer's sums, variances, partitions, split order and PCA axes, the
+threaded divisions, quality 255 against the per-block endpoint search, and
+the selector optimisation against a brute-force search."""
+import numpy as np
+import pytest
+
+import oracle
+from crunch2_amd import texture
+from tests.qdxt_cases import level_blocks, texture_blocks
+
+LINEAR = np.array([0, 3, 1, 2], np.float32)
+
+
+def test_representative_colors_closed_forms():
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        n = int(rng.choice([16, 32, 64]))
+        a, b = rng.integers(0, 256, 3), rng.integers(0, 256, 3)
+        k = int(rng.integers(1, n))
+        px = np.zeros((n, 4), np.uint8)
+        px[:k, :3], px[k:, :3] = a, b
+        px = px[rng.permutation(n)]
+        lo, hi = oracle.representative_colors(px)
+        if (a == b).all():
+            assert (lo == a).all() and (hi == a).all()
+            continue
+        ea, eb = int((a * a).sum()), int((b * b).sum())
+        exp = (b, a) if ea > eb else (a, b)
+        assert {tuple(lo), tuple(hi)} == {tuple(a), tuple(b)}
+        assert tuple(lo) == tuple(exp[0]) or ea == eb
+    px = np.full((16, 4), 77, np.uint8)
+    lo, hi = oracle.representative_colors(px)
+    assert (lo == 77).all() and (hi == 77).all()
+
+
+def _palette(lo, hi):
+    def unpack(v):
+        r, g, b = (v >> 11) & 31, (v >> 5) & 63, v & 31
+        return np.array([(r << 3) | (r >> 2), (g << 2) | (g >> 4), (b << 3) | (b >> 2)], np.int64)
+    c0, c1 = unpack(int(lo)), unpack(int(hi))
+    if lo > hi:
+        return np.stack([c0, c1, (2 * c0 + c1) // 3, (2 * c1 + c0) // 3])
+    return np.stack([c0, c1, (c0 + c1) >> 1, np.zeros(3, np.int64)])
+
+
+LAYOUTS = [(0, 0, 8, 8), (0, 0, 8, 4), (0, 4, 8, 4), (0, 0, 4, 8), (4, 0, 4, 8), (0, 0, 4, 4), (4, 0, 4, 4),
+           (0, 4, 4, 4), (4, 4, 4, 4)]
+ENCODINGS = [[0], [1, 2], [3, 4], [1, 7, 8], [2, 5, 6], [3, 6, 8], [4, 5, 7], [5, 6, 7, 8]]
+
+
+@pytest.mark.parametrize("size,height", [(64, 64), (40, 24)])
+def test_tile_choice_in_image_space(size, height):
+    img = texture.synth(size, height, 9, texture.RGB)
+    levels = texture.mip_chain(img)[:3]
+    blocks, desc = level_blocks(levels)
+    vecs, w, enc = oracle.qdxt1_train(blocks, desc)
+    for lvl, im in enumerate(levels):
+        first, bw, bh = desc[3 * lvl:3 * lvl + 3]
+        ys = np.minimum(np.arange(bh * 4), im.shape[0] - 1)
+        xs = np.minimum(np.arange(bw * 4), im.shape[1] - 1)
+        pad = im[ys][:, xs]
+        der = 1.5 if lvl == 0 else max(0.25, 1.5 / 3.1 ** lvl)
+        for cy in range((bh + 1) // 2):
+            for cx in range((bw + 1) // 2):
+                yy = np.minimum(np.arange(cy * 8, cy * 8 + 8), bh * 4 - 1)
+                xx = np.minimum(np.arange(cx * 8, cx * 8 + 8), bw * 4 - 1)
+                chunk = pad[yy][:, xx]
+                errs = []
+                for x0, y0, lw, lh in LAYOUTS:
+                    t = chunk[y0:y0 + lh, x0:x0 + lw].reshape(-1, 4)
+                    lo, hi, sel = oracle.dxt_fast_color(t)
+                    pal = _palette(lo, hi)
+                    errs.append(int(((t[:, :3].astype(np.int64) - pal[sel]) ** 2).sum()))
+                best, best_e = -1.0, 0
+                for e, tiles in enumerate(ENCODINGS):
+                    ms = sum(errs[t] for t in tiles) / 192.0
+                    ps = 999999.0 if not ms else min(500.0, max(0.0, 20 * np.log10(255.0 / np.sqrt(ms))))
+                    ps -= der * (len(tiles) - 1) / 3.0
+                    if ps > best + 1e-9:
+                        best, best_e = ps, e
+                for t in ENCODINGS[best_e]:
+                    x0, y0, lw, lh = LAYOUTS[t]
+                    lo, hi = oracle.representative_colors(chunk[y0:y0 + lh, x0:x0 + lw].reshape(-1, 4))
+                    wt = min(8, max(1, int(((lo.astype(int) - hi) ** 2).sum()) // 5000))
+                    for by in range(lh // 4):
+                        for bx in range(lw // 4):
+                            gy, gx = cy * 2 + by + y0 // 4, cx * 2 + bx + x0 // 4
+                            if gy >= bh or gx >= bw:
+                                continue
+                            b = first + gx + gy * bw
+                            assert enc[b] == best_e, (lvl, cy, cx)
+                            assert (vecs[b] == np.concatenate([lo, hi])).all() and w[b] == wt
+
+
+def _weighted(v, w):
+    w = w.astype(np.float64)
+    mean = (v * w[:, None]).sum(0) / w.sum()
+    return mean, float((w[:, None] * (v - mean) ** 2).sum())
+
+
+@pytest.mark.parametrize("d,n,seed", [(6, 3000, 1), (16, 2000, 2), (2, 1500, 3)])
+def test_clusterizer_structure(d, n, seed):
+    rng = np.random.default_rng(seed)
+    if d == 16:
+        v = rng.integers(0, 4, (n, d)).astype(np.float32)
+        w = rng.integers(1, 2049, n).astype(np.uint32)
+    else:
+        centres = rng.integers(0, 256, (12, d))
+        v = np.clip(centres[rng.integers(0, 12, n)] + rng.normal(0, 9, (n, d)), 0, 255).round().astype(np.float32)
+        w = rng.integers(1, 9, n).astype(np.uint32)
+    q = oracle.VQ(v, w, 300)
+    nodes = q.nodes()
+    mean, var = _weighted(v.astype(np.float64), w)
+    assert np.allclose(nodes[0][6], mean, rtol=1e-5, atol=1e-3)
+    assert abs(nodes[0][5] - var) <= 4e-7 * float((w[:, None] * v.astype(np.float64) ** 2).sum()) + 1e-2
+    splits = []
+    for i, (l, r, num, uns, vec, var_i, cen) in enumerate(nodes):
+        assert (np.diff(vec.astype(np.int64)) > 0).all()
+        m, vv = _weighted(v[vec].astype(np.float64), w[vec])
+        # the reference's ttsum - |sum|^2 / weight cancels in float: error
+        # relative to ttsum
+        tt = float((w[vec].astype(np.float64)[:, None] * v[vec].astype(np.float64) ** 2).sum())
+        assert abs(var_i - vv) <= 4e-7 * tt + 1e-2
+        if l >= 0:
+            both = np.sort(np.concatenate([nodes[l][4], nodes[r][4]]))
+            assert (both == vec).all() and len(nodes[l][4]) and len(nodes[r][4])
+            assert np.allclose(nodes[l][6], _weighted(v[nodes[l][4]].astype(np.float64), w[nodes[l][4]])[0],
+                               rtol=1e-4, atol=1e-2)
+            splits.append((num, var_i))
+    splits.sort()
+    assert [s[0] for s in splits] == list(range(len(splits)))
+    vs = np.array([s[1] for s in splits])
+    assert (vs[1:] <= vs[:-1] * (1 + 1e-5) + 1e-3).all()
+    assert q.codebook_size == len(splits) + 1
+    for k in (1, 2, 7, 50, q.codebook_size, q.codebook_size + 10):
+        cl = q.retrieve(k)
+        assert len(cl) == min(k, q.codebook_size)
+        assert (np.sort(np.concatenate(cl)) == np.arange(n)).all()
+    # PCA axis of nodes with a clear dominant direction
+    checked = 0
+    for i, (l, r, num, uns, vec, var_i, cen) in enumerate(nodes):
+        if l < 0 or len(vec) < 50:
+            continue
+        x = v[vec].astype(np.float64)
+        ww = w[vec].astype(np.float64)
+        c = x - (x * ww[:, None]).sum(0) / ww.sum()
+        cov = (c * ww[:, None]).T @ c / ww.sum()
+        ev, evec = np.linalg.eigh(cov)
+        if ev[-1] < 6 * ev[-2]:
+            continue
+        assert abs(float(q.axis(i) @ evec[:, -1])) > 0.98, i
+        checked += 1
+    assert checked or d == 16
+    q.close()
+
+
+def test_clusterizer_degenerate():
+    v = np.full((10, 6), 40, np.float32)
+    w = np.ones(10, np.uint32)
+    q = oracle.VQ(v, w, 100)
+    nodes = q.nodes()
+    assert len(nodes) == 1 and nodes[0][3] == 1 and q.codebook_size == 1 and q.pops == 1
+    q.close()
+    q = oracle.VQ(v[:1], w[:1], 100)
+    assert q.codebook_size == 1 and [len(c) for c in q.retrieve(5)] == [1]
+    q.close()
+
+
+def test_threaded_divisions_follow_the_dominant_axis():
+    rng = np.random.default_rng(5)
+    n = 4000
+    v = rng.integers(0, 4, (n, 16)).astype(np.float32)
+    v[: n // 2, :8] = np.minimum(3, v[: n // 2, :8] + 2)
+    w = rng.integers(1, 2049, n).astype(np.uint32)
+    left, right, axis = oracle.vq_pca_division(v, w, np.arange(n))
+    x, ww = v.astype(np.float64), w.astype(np.float64)
+    mean = (x * ww[:, None]).sum(0) / ww.sum()
+    c = x - mean
+    cov = (c * ww[:, None]).T @ c / ww.sum()
+    ev, evec = np.linalg.eigh(cov)
+    assert abs(float(axis @ evec[:, -1])) > 0.99
+    t = c @ axis
+    assert set(left.tolist()) == set(np.nonzero(t < 0)[0].tolist()) or \
+        np.abs(t[np.setxor1d(left, np.nonzero(t < 0)[0])]).max() < 1e-3
+    cl = oracle.vq_threaded_clusters(v, w, 400)
+    assert (np.sort(np.concatenate(cl)) == np.arange(n)).all()
+    # every cluster lies inside one of the four partitions of the two-level division
+    l0, r0, _ = oracle.vq_pca_division(v, w, np.arange(n))
+    parts = []
+    for side in (l0, r0):
+        a, b, _ = oracle.vq_pca_division(v, w, side)
+        parts += [set(a.tolist()), set(b.tolist())]
+    for c_ in cl:
+        assert sum(set(c_.tolist()) <= p for p in parts) == 1
+    small = oracle.vq_threaded_clusters(v, w, 100)
+    assert len(small) == 100
+
+
+def test_quality_255_is_the_block_search():
+    blocks, desc = texture_blocks(64, 4)
+    flags = oracle.QDXT_PERCEPTUAL | oracle.QDXT_ALPHA_BLOCKS | oracle.QDXT_HIERARCHICAL
+    out, mid, st = oracle.qdxt1(blocks, desc, 255, flags=flags)
+    sel, err, lo, hi = oracle.dxt1_blocks(blocks.reshape(-1, 16, 4), 4,
+                                          oracle.DXT1_PERCEPTUAL | oracle.DXT1_ALPHA_BLOCKS)[:4]
+    exp = lo.astype(np.uint64) | (hi.astype(np.uint64) << np.uint64(16))
+    mask = np.zeros(len(blocks), np.uint64)
+    for t in range(16):
+        mask |= sel[:, t].astype(np.uint64) << np.uint64(32 + 2 * t)
+    assert (out == (exp | mask)).all()
+    assert st["endpoint_clusters"] == len(blocks)
+
+
+def _sel_vectors(mid, perceptual):
+    n = len(mid)
+    sv = np.zeros((n, 16), np.float32)
+    sw = np.zeros(n, np.uint32)
+    for b in range(n):
+        blk = int(mid[b])
+        sv[b] = LINEAR[[(blk >> (32 + 2 * t)) & 3 for t in range(16)]]
+        pal = _palette(blk & 0xFFFF, (blk >> 16) & 0xFFFF)
+        d = pal[0] - pal[1]
+        dist = int((d * d * ([8, 25, 1] if perceptual else [1, 1, 1])).sum())
+        sw[b] = min(2048, max(1, dist // 2000))
+    return sv, sw
+
+
+@pytest.mark.parametrize("flags,q,kind", [(7, 128, texture.RGB), (5, 100, texture.RGB), (3, 160, texture.RGBA)])
+def test_selector_optimisation_brute_force(flags, q, kind):
+    blocks, desc = texture_blocks(64, 6, kind)
+    out, mid, st = oracle.qdxt1(blocks, desc, q, flags=flags)
+    perceptual = flags & 1
+    sv, sw = _sel_vectors(mid, perceptual)
+    clusters = oracle.vq_threaded_clusters(sv, sw, st["max_selector_target"])
+    assert len(clusters) == st["selector_clusters"]
+    exp = mid.copy()
+    thresh = 128 if flags & 2 else 0
+    wts = np.array([8, 25, 1] if perceptual else [1, 1, 1])
+    px = blocks.reshape(-1, 16, 4).astype(np.int64)
+    for c in clusters:
+        if len(c) <= 1:
+            continue
+        lo = (mid[c] & np.uint64(0xFFFF)).astype(np.int64)
+        hi = ((mid[c] >> np.uint64(16)) & np.uint64(0xFFFF)).astype(np.int64)
+        cats = [c[lo > hi], c[(lo <= hi) & ~((px[c, :, 3] < thresh).any(1) if thresh else np.zeros(len(c), bool))]]
+        for typ, members in enumerate(cats):
+            if len(members) <= 1:
+                continue
+            pals = np.stack([_palette(int(mid[b]) & 0xFFFF, (int(mid[b]) >> 16) & 0xFFFF) for b in members])
+            err = ((px[members, :, None, :3] - pals[:, None, :, :]) ** 2 * wts).sum(-1).sum(0)  # [16, 4]
+            best = np.argmin(err[:, :3 if typ else 4], axis=1)
+            word = sum(int(best[t]) << (2 * t) for t in range(16))
+            for b in members:
+                exp[b] = (exp[b] & np.uint64(0xFFFFFFFF)) | np.uint64(word << 32)
+    assert (out == exp).all()
+    assert (out & np.uint64(0xFFFFFFFF) == mid & np.uint64(0xFFFFFFFF)).all()
+
+
+def test_cluster_counts_follow_the_quality():
+    blocks, desc = texture_blocks(64, 8)
+    prev = 0
+    for q in (0, 40, 128, 200, 254):
+        out, mid, st = oracle.qdxt1(blocks, desc, q)
+        mec = min(st["codebook_size"], max(96, int(st["codebook_size"] * (q / 255.0) ** 1.8)))
+        assert abs(st["max_endpoint_clusters"] - mec) <= 1
+        assert st["endpoint_clusters"] == min(st["max_endpoint_clusters"], st["codebook_size"])
+        assert st["endpoint_clusters"] >= prev
+        prev = st["endpoint_clusters"]
+    assert st["max_selector_clusters"] == oracle.qdxt1_distinct_selectors(blocks) + 128
