@@ -54,6 +54,11 @@ class MipParams(C.Structure):
                 ("source_gamma", C.c_float), ("max_levels", C.c_uint32), ("min_mip_size", C.c_uint32)]
 
 
+class Qdxt1Params(C.Structure):
+    _fields_ = [("quality_level", C.c_uint32), ("dxt_quality", C.c_int), ("flags", C.c_int),
+                ("dxt1a_alpha_threshold", C.c_uint32)]
+
+
 # name -> (restype, argtypes); every function declared in include/crn2.h
 SIGNATURES = {
     "crn2_validate_file": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -196,6 +201,17 @@ SIGNATURES = {
     "crn2_mip_contributors": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.c_uint32, C.c_float, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_uint32]),
     "crn2_mip_gamma_tables": (C.c_int, [C.c_float, C.c_void_p, C.c_void_p]),
+    "crn2_qdxt1_init": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int, C.POINTER(C.c_void_p),
+                                  C.c_void_p]),
+    "crn2_qdxt1_pack": (C.c_int, [C.c_void_p, C.POINTER(Qdxt1Params), C.c_float, C.c_void_p, C.c_void_p]),
+    "crn2_qdxt1_info": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint32)] * 4),
+    "crn2_qdxt1_training": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "crn2_qdxt1_destroy": (C.c_int, [C.c_void_p]),
+    "crn2_vq_clusters_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                          C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                          C.c_void_p]),
+    "crn2_vq_threaded_clusters_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                   C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.c_void_p]),
 }
 
 # internal (not in include/): the block packer behind crn2_compress_rgba

@@ -23,7 +23,7 @@ HOST_SOURCES = ["crn_pack.cpp", "synth.cpp", "crn_api.cpp", "crnd_host.cpp", "cr
 HIP_SOURCES = ["crnd_gpu.hip", "dxt5a_gpu.hip", "dxt1_gpu.hip", "hc_gpu.hip", "mlp_gpu.hip", "dxt1c_gpu.hip",
                "hc_endpoints_gpu.hip", "tree_gpu.hip",
                "hc_codebook_gpu.hip", "hc_clusters_gpu.hip", "hc_tiles_gpu.hip",
-               "hc_compress_gpu.hip", "dxt_decode_gpu.hip", "texture_api.hip", "mip_gpu.hip"]
+               "hc_compress_gpu.hip", "dxt_decode_gpu.hip", "texture_api.hip", "mip_gpu.hip", "qdxt_gpu.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -packed-fp32-ops: no v_pk_*_f32 generation; -ffp-contract=off: float

@@ -1,0 +1,1770 @@
+// qdxt_gpu.hip -- crnlib's clustered DXT1 compressor (qdxt1,
+// crnlib/crn_qdxt1.cpp) and the vector quantizers it runs
+// (clusterizer, crnlib/crn_clusterizer.h; threaded_clusterizer,
+// crnlib/crn_threaded_clusterizer.h) on gfx950.
+//
+// Pipeline (one qdxt1 object = one init, any number of packs):
+//   init   training vectors: per 2x2-block chunk one thread per tile layout
+//          (dxt_fast compression + error), one thread per chunk for the
+//          encoding choice and the chosen tiles' representative colours;
+//          the 6-D endpoint tree grown to 65,535 leaves; the distinct
+//          dxt_fast selector words counted in a device hash set.
+//   pack   endpoint clusters cut from the tree on the host (depth first);
+//          their texels gathered and searched one wave per cluster by the
+//          cluster endpoint search (dxt1c_gpu.hip) or, at quality 255,
+//          block by block (dxt1_gpu.hip); selector vectors per block; the
+//          threaded selector clusters (three PCA divisions, four trees);
+//          the selector optimisation one wave per selector cluster, lanes =
+//          (texel, selector value).
+//
+// The tree.  The reference grows it best first: a max-heap on node
+// variance pops the worst node, which is split (PCA start, up to eight
+// rounds of 2-means) and its children pushed.  A split depends only on the
+// node's own vectors, so the device splits nodes in rounds, one wave per
+// node, and the host replays the heap (the reference's insert / down-heap
+// on the same float variances) as far as the splits computed so far allow;
+// when it pops a node not yet split it stops, and the next round splits
+// that node together with every heap node that could still be popped
+// within the remaining leaf budget (those with fewer than that many heap
+// nodes of higher variance).  Node vectors live in a pool of index segments:
+// a split writes its children (left part then right part, each in the
+// parent's order) to a fresh segment, so every node's list stays in
+// ascending vector order, which is the order of the reference's lists.
+//
+// Sums.  The reference accumulates float sums over a node's vectors in list
+// order (centroids, the covariance, the 2-means centres).  The split kernel
+// keeps that order: a batch of 64 vectors is staged in LDS by all lanes and
+// lane k adds the batch's terms of component k (or covariance entry k) one
+// after another.  Integer sums (weights, sums of squares: exact in double)
+// are wave reductions.  Per-vector work (projections, distances, the side
+// of each vector) is one vector per lane.  The file is compiled with
+// -ffp-contract=off so that no multiply-add is fused, as in the reference's
+// x86-64 build.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "crn2_internal.h"
+#include "dxt_fast_dev.h"
+
+namespace {
+
+using namespace dxtf;
+
+__constant__ uint8_t q_om5[256][2], q_om6[256][2];
+
+// g_chunk_tile_layouts (crnlib/crn_dxt_hc_common.cpp:47-63): x, y, width, height
+__constant__ uint8_t c_layout[9][4] = {{0, 0, 8, 8}, {0, 0, 8, 4}, {0, 4, 8, 4}, {0, 0, 4, 8}, {4, 0, 4, 8},
+                                       {0, 0, 4, 4}, {4, 0, 4, 4}, {0, 4, 4, 4}, {4, 4, 4, 4}};
+// g_chunk_encodings (:29-45): tile count, then the layout of each tile
+__constant__ uint8_t c_enc[8][5] = {{1, 0}, {2, 1, 2}, {2, 3, 4}, {3, 1, 7, 8}, {3, 2, 5, 6},
+                                    {3, 3, 6, 8}, {3, 4, 5, 7}, {4, 5, 6, 7, 8}};
+__constant__ float c_linear[4] = {0.0f, 3.0f, 1.0f, 2.0f};   // g_dxt1_to_linear
+
+__device__ inline uint32_t lane() { return threadIdx.x & 63u; }
+
+// ------------------------------------------------------------ init: training
+struct TChunk {
+    uint32_t first, bw, bh, cx, cy;
+    float der;    // the level's colour PSNR derating
+};
+
+// Chunk texel (x, y), clamped to the level's block area (crn_qdxt1.cpp:127-147).
+__device__ inline uint32_t chunk_texel(const uint32_t* __restrict__ blocks, const TChunk& c, uint32_t x, uint32_t y)
+{
+    const uint32_t lw = c.bw * 4, lh = c.bh * 4;
+    uint32_t px = c.cx * 8 + x, py = c.cy * 8 + y;
+    px = px < lw - 1 ? px : lw - 1;
+    py = py < lh - 1 ? py : lh - 1;
+    return blocks[(size_t)(c.first + (py >> 2) * c.bw + (px >> 2)) * 16 + (py & 3) * 4 + (px & 3)];
+}
+
+// dxt1_block::get_block_colors: 4 colours when low > high, else 3 + black
+__device__ inline void block_colors(uint32_t lo, uint32_t hi, int (&col)[4][3])
+{
+    unpack565(lo, col[0]);
+    unpack565(hi, col[1]);
+    for (int k = 0; k < 3; k++) {
+        if (lo > hi) {
+            col[2][k] = (col[0][k] * 2 + col[1][k]) / 3;
+            col[3][k] = (col[1][k] * 2 + col[0][k]) / 3;
+        } else {
+            col[2][k] = (col[0][k] + col[1][k]) >> 1;
+            col[3][k] = 0;
+        }
+    }
+}
+
+// One tile layout: dxt_fast compression and the RGB squared error against
+// the resulting block colours (crn_qdxt1.cpp:172-197), texels in raster
+// order within the tile.
+template <int N>
+__device__ uint32_t layout_error(const uint32_t* __restrict__ blocks, const TChunk& c, uint32_t l)
+{
+    const uint32_t xo = c_layout[l][0], yo = c_layout[l][1], w = c_layout[l][2];
+    uint32_t px[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) px[i] = chunk_texel(blocks, c, xo + (uint32_t)i % w, yo + (uint32_t)i / w);
+    uint8_t sel[N];
+    uint32_t lo, hi;
+    compress_color<N>(px, lo, hi, sel, q_om5, q_om6);
+    int col[4][3];
+    block_colors(lo, hi, col);
+    uint32_t err = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < 3; k++) {
+            const int d = ch(px[i], k) - col[sel[i]][k];
+            err += (uint32_t)(d * d);
+        }
+    return err;
+}
+
+__global__ void __launch_bounds__(256) qdxt_layout_kernel(const uint32_t* __restrict__ blocks,
+                                                          const TChunk* __restrict__ chunks, uint32_t nchunks,
+                                                          uint32_t* __restrict__ lerr)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nchunks * 9) return;
+    const TChunk c = chunks[i / 9];
+    const uint32_t l = i % 9;
+    lerr[i] = l == 0 ? layout_error<64>(blocks, c, l) : (l < 5 ? layout_error<32>(blocks, c, l)
+                                                               : layout_error<16>(blocks, c, l));
+}
+
+// dxt_fast::find_representative_colors (crn_dxt_fast.cpp:855-997) over n
+// texels given by texel(i); colours packed r | g << 8 | b << 16.
+template <class F>
+__device__ void rep_colors(uint32_t n, F texel, uint32_t& lo_out, uint32_t& hi_out)
+{
+    uint64_t s[3] = {0, 0, 0};
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t p = texel(i);
+        for (int k = 0; k < 3; k++) s[k] += (uint32_t)ch(p, k);
+    }
+    int ave[3];
+    for (int k = 0; k < 3; k++) ave[k] = (int)((s[k] + (n / 2)) / n);
+    int fd = -1;
+    uint32_t fp = texel(0);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t p = texel(i);
+        const int r = ch(p, 0) - ave[0], g = ch(p, 1) - ave[1], b = ch(p, 2) - ave[2];
+        const int d = r * r + g * g + b * b;
+        if (d > fd) {
+            fd = d;
+            fp = p;
+        }
+    }
+    int l[3] = {ch(fp, 0), ch(fp, 1), ch(fp, 2)};
+    int od = -1;
+    uint32_t op = texel(0);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t p = texel(i);
+        const int r = ch(p, 0) - l[0], g = ch(p, 1) - l[1], b = ch(p, 2) - l[2];
+        const int d = r * r + g * g + b * b;
+        if (d > od) {
+            od = d;
+            op = p;
+        }
+    }
+    int h[3] = {ch(op, 0), ch(op, 1), ch(op, 2)};
+    for (int k = 0; k < 3; k++) {
+        l[k] = (l[k] + ave[k]) >> 1;
+        h[k] = (h[k] + ave[k]) >> 1;
+    }
+    for (int iter = 0; iter < 4; iter++) {
+        if (l[0] == h[0] && l[1] == h[1] && l[2] == h[2]) break;
+        uint64_t nc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        uint32_t wt[2] = {0, 0};
+        int vr = h[0] - l[0], vg = h[1] - l[1], vb = h[2] - l[2];
+        const int mid = (vr * l[0] + vg * l[1] + vb * l[2]) + (vr * h[0] + vg * h[1] + vb * h[2]);
+        vr *= 2;
+        vg *= 2;
+        vb *= 2;
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t p = texel(i);
+            const int dot = ch(p, 0) * vr + ch(p, 1) * vg + ch(p, 2) * vb;
+            const uint32_t m = dot > mid ? 1u : 0u;
+            for (int k = 0; k < 3; k++) nc[m][k] += (uint32_t)ch(p, k);
+            wt[m]++;
+        }
+        if (!wt[0] || !wt[1]) break;
+        int n8[2][3];
+        for (int j = 0; j < 2; j++)
+            for (int k = 0; k < 3; k++) n8[j][k] = (int)(uint8_t)((nc[j][k] + (wt[j] / 2)) / wt[j]);
+        if (n8[0][0] == l[0] && n8[0][1] == l[1] && n8[0][2] == l[2] && n8[1][0] == h[0] && n8[1][1] == h[1] &&
+            n8[1][2] == h[2])
+            break;
+        for (int k = 0; k < 3; k++) {
+            l[k] = n8[0][k];
+            h[k] = n8[1][k];
+        }
+    }
+    const uint32_t e0 = (uint32_t)(l[0] * l[0] + l[1] * l[1] + l[2] * l[2]);
+    const uint32_t e1 = (uint32_t)(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    const uint32_t lp = (uint32_t)l[0] | (uint32_t)l[1] << 8 | (uint32_t)l[2] << 16;
+    const uint32_t hp = (uint32_t)h[0] | (uint32_t)h[1] << 8 | (uint32_t)h[2] << 16;
+    lo_out = e0 > e1 ? hp : lp;
+    hi_out = e0 > e1 ? lp : hp;
+}
+
+__device__ inline void put_training(float* __restrict__ vecs, uint32_t* __restrict__ weights, uint32_t b,
+                                    uint32_t lo, uint32_t hi)
+{
+    int dist = 0;
+    for (int k = 0; k < 3; k++) {
+        vecs[6 * (size_t)b + k] = (float)ch(lo, k);
+        vecs[6 * (size_t)b + 3 + k] = (float)ch(hi, k);
+        const int d = ch(lo, k) - ch(hi, k);
+        dist += d * d;
+    }
+    const uint32_t w = (uint32_t)dist / 5000;
+    weights[b] = w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
+// Encoding choice of one chunk (crn_qdxt1.cpp:199-247) and the training
+// vectors of its blocks from the chosen tiles (:249-340).
+__global__ void __launch_bounds__(256) qdxt_choose_kernel(const uint32_t* __restrict__ blocks,
+                                                          const TChunk* __restrict__ chunks, uint32_t nchunks,
+                                                          const uint32_t* __restrict__ lerr,
+                                                          float* __restrict__ vecs, uint32_t* __restrict__ weights,
+                                                          uint8_t* __restrict__ enc)
+{
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= nchunks) return;
+    const TChunk c = chunks[ci];
+    double best = -1.0f;
+    uint32_t be = 0;
+    for (uint32_t e = 0; e < 8; e++) {
+        double total = 0;
+        for (uint32_t t = 0; t < c_enc[e][0]; t++) total += (double)lerr[ci * 9 + c_enc[e][1 + t]];
+        const double ms = total * (1.0f / (64.0f * 3.0f));
+        const double rms = sqrt(ms);
+        double psnr = 999999.0f;
+        if (ms) {
+            psnr = log10(255.0f / rms) * 20.0f;
+            psnr = psnr < 0.0 ? 0.0 : (psnr > 500.0 ? 500.0 : psnr);
+        }
+        const float cder = 0.0f + (c.der - 0.0f) * ((float)(c_enc[e][0] - 1) / 3.0f);
+        psnr = psnr - cder;
+        if (psnr > best) {
+            best = psnr;
+            be = e;
+        }
+    }
+    for (uint32_t t = 0; t < c_enc[be][0]; t++) {
+        const uint32_t l = c_enc[be][1 + t];
+        const uint32_t xo = c_layout[l][0], yo = c_layout[l][1], w = c_layout[l][2], h = c_layout[l][3];
+        uint32_t lo, hi;
+        rep_colors(w * h, [&](uint32_t i) { return chunk_texel(blocks, c, xo + i % w, yo + i / w); }, lo, hi);
+        for (uint32_t y = 0; y < (h >> 2); y++) {
+            const uint32_t by = c.cy * 2 + y + (yo >> 2);
+            if (by >= c.bh) continue;
+            for (uint32_t x = 0; x < (w >> 2); x++) {
+                const uint32_t bx = c.cx * 2 + x + (xo >> 2);
+                if (bx >= c.bw) break;
+                const uint32_t b = c.first + bx + by * c.bw;
+                put_training(vecs, weights, b, lo, hi);
+                if (enc) enc[b] = (uint8_t)be;
+            }
+        }
+    }
+}
+
+// Non-hierarchical training vectors: one block's 16 texels (crn_qdxt1.cpp:368-399).
+__global__ void __launch_bounds__(256) qdxt_block_train_kernel(const uint32_t* __restrict__ blocks, uint32_t n,
+                                                               float* __restrict__ vecs,
+                                                               uint32_t* __restrict__ weights,
+                                                               uint8_t* __restrict__ enc)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const uint32_t* p = blocks + (size_t)b * 16;
+    uint32_t lo, hi;
+    rep_colors(16, [&](uint32_t i) { return p[i]; }, lo, hi);
+    put_training(vecs, weights, b, lo, hi);
+    if (enc) enc[b] = 255;
+}
+
+__device__ inline uint32_t hash32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+// The distinct selector words of dxt_fast::compress_color_block per block
+// (crn_qdxt1.cpp:409-426), counted in an open-addressing set (key + 1, 0 empty).
+__global__ void __launch_bounds__(256) qdxt_selword_kernel(const uint32_t* __restrict__ blocks, uint32_t n,
+                                                           unsigned long long* __restrict__ table, uint32_t mask,
+                                                           uint32_t* __restrict__ count)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    uint32_t px[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) px[i] = blocks[(size_t)b * 16 + i];
+    uint8_t sel[16];
+    uint32_t lo, hi;
+    compress_color<16>(px, lo, hi, sel, q_om5, q_om6);
+    uint32_t word = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) word |= (uint32_t)sel[i] << (2 * i);
+    const unsigned long long key = (unsigned long long)word + 1ull;
+    uint32_t h = hash32(word) & mask;
+    for (;;) {
+        const unsigned long long prev = atomicCAS(&table[h], 0ull, key);
+        if (prev == 0ull) {
+            atomicAdd(count, 1u);
+            break;
+        }
+        if (prev == key) break;
+        h = (h + 1) & mask;
+    }
+}
+
+// --------------------------------------------------------------- the trees
+struct VqSplit {
+    uint32_t node;        // device slot of the node's centroid and weight
+    uint32_t seg, n;      // its vectors: pool[seg, seg + n)
+    uint32_t child_seg;   // the children's vectors go to pool[child_seg, child_seg + n)
+    uint32_t left, right; // device slots for the children
+};
+
+struct VqResult {
+    uint32_t split;       // 0: unsplittable
+    uint32_t nl, nr;
+    float lvar, rvar;
+};
+
+struct VqArgs {
+    const float* v;                 // vectors, D floats each
+    const uint32_t* w;              // weights
+    uint32_t* pool;                 // index segments
+    float* cen;                     // per slot: D floats
+    unsigned long long* tw;         // per slot
+};
+
+template <int D>
+struct VqLds {
+    static constexpr int kE = D * (D + 1) / 2;
+    float a[64][D + 1];   // staged batch: vector - centroid, or products
+    float b[64][D + 1];   // staged batch: (vector - centroid) * weight
+    uint32_t side[64];
+    float cov[D][D];
+    float axis[D], prev[D], x[D];
+    double red[64];
+};
+
+__device__ inline double wave_sum_d(double v)
+{
+    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v)
+{
+    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ inline double wave_max_d(double v)
+{
+    for (int o = 32; o; o >>= 1) {
+        const double t = __shfl_xor(v, o);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+
+template <int D>
+__device__ inline float vdot(const float (&a)[D], const float (&b)[D])
+{
+    float r = a[0] * b[0];
+#pragma unroll
+    for (int i = 1; i < D; i++) r += a[i] * b[i];
+    return r;
+}
+
+template <int D>
+__device__ inline float vsqd(const float (&a)[D], const float (&b)[D])
+{
+    float s = 0;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        const float t = a[i] - b[i];
+        s += t * t;
+    }
+    return s;
+}
+
+template <int D>
+__device__ inline void load_vec(const VqArgs& A, uint32_t vi, float (&v)[D])
+{
+#pragma unroll
+    for (int c = 0; c < D; c++) v[c] = A.v[(size_t)vi * D + c];
+}
+
+// Sequential float sums over a list in order: sum over the vectors j with
+// side(j) == s of v_j * (float)w_j, for s = 0 and 1, per component.  Lane
+// k < 2D holds (side k / D, component k % D).  gen(j, v, w, side) gives the
+// j-th vector of the list and its side (2: neither).
+template <int D, class Gen>
+__device__ float seq_side_sums(VqLds<D>& L, uint32_t n, Gen gen)
+{
+    float acc = 0.0f;
+    const uint32_t me = lane();
+    const uint32_t my_side = me / D, my_comp = me % D;
+    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+        const uint32_t j = j0 + me;
+        if (j < n) {
+            float v[D];
+            uint32_t w, s;
+            gen(j, v, w, s);
+#pragma unroll
+            for (int c = 0; c < D; c++) L.a[me][c] = v[c] * (float)w;
+            L.side[me] = s;
+        }
+        __syncthreads();
+        const uint32_t cnt = n - j0 < 64 ? n - j0 : 64;
+        if (me < 2 * D)
+            for (uint32_t k = 0; k < cnt; k++)
+                if (L.side[k] == my_side) acc += L.a[k][my_comp];
+        __syncthreads();
+    }
+    return acc;
+}
+
+// The D sums of side s as a vector on every lane.
+template <int D>
+__device__ inline void gather_sums(float acc, uint32_t s, float (&out)[D])
+{
+#pragma unroll
+    for (int c = 0; c < D; c++) out[c] = __shfl(acc, (int)(s * D + c));
+}
+
+// Weighted covariance about cen, entries in list order (crn_clusterizer.h:
+// 494-509, crn_threaded_clusterizer.h:266-279); then scaled as mode asks,
+// mirrored into L.cov.  mode 0: float one_over (clusterizer), 1: double
+// one_over, result rounded to float (threaded clusterizer).
+template <int D, class Get>
+__device__ void covariance(VqLds<D>& L, uint32_t n, const float (&cen)[D], Get get, float oo_f, double oo_d,
+                           int mode)
+{
+    constexpr int kE = VqLds<D>::kE;
+    const uint32_t me = lane();
+    int ex[3], ey[3];
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        int e = (int)me + 64 * r, x = 0;
+        if (e < kE) {
+            int row = D;
+            while (e >= row) {
+                e -= row;
+                row--;
+                x++;
+            }
+            ex[r] = x;
+            ey[r] = x + e;
+        } else {
+            ex[r] = ey[r] = -1;
+        }
+    }
+    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+        const uint32_t j = j0 + me;
+        if (j < n) {
+            float v[D];
+            uint32_t w;
+            get(j, v, w);
+            const float wf = (float)w;
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+                const float d = v[c] - cen[c];
+                L.a[me][c] = d;
+                L.b[me][c] = d * wf;
+            }
+        }
+        __syncthreads();
+        const uint32_t cnt = n - j0 < 64 ? n - j0 : 64;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            if (ex[r] >= 0)
+                for (uint32_t k = 0; k < cnt; k++) acc[r] = acc[r] + L.a[k][ex[r]] * L.b[k][ey[r]];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        if (ex[r] >= 0) {
+            const float c = mode ? (float)(acc[r] * oo_d) : acc[r] * oo_f;
+            L.cov[ex[r]][ey[r]] = c;
+            L.cov[ey[r]][ex[r]] = c;
+        }
+    __syncthreads();
+}
+
+// Power iteration from lerp(.75, 1.25) and normalisation (crn_clusterizer.h:
+// 523-566, crn_threaded_clusterizer.h:288-339; vec::norm is the sum of
+// squares, vec::normalize crn_vec.h:674-690).  Result on every lane.
+template <int D>
+__device__ void pca_axis(VqLds<D>& L, float (&axis)[D])
+{
+    const uint32_t me = lane();
+    if (me < D) {
+        const float t = (float)me * (1.0f / (float)(D - 1));
+        L.axis[me] = .75f + (1.25f - .75f) * t;
+        L.prev[me] = L.axis[me];
+    }
+    __syncthreads();
+    for (int iter = 0; iter < 10; iter++) {
+        double sum = 0;
+        if (me < D)
+            for (int j = 0; j < D; j++) sum += L.axis[j] * L.cov[me][j];
+        const double max_sum = wave_max_d(me < D ? fabs(sum) : 0.0);
+        float x = (float)sum;
+        if (max_sum != 0.0f) x *= (float)(1.0f / max_sum);
+        __syncthreads();
+        if (me < D) L.x[me] = L.prev[me] - x;   // delta
+        __syncthreads();
+        float nrm = L.x[0] * L.x[0];
+        for (int i = 1; i < D; i++) nrm += L.x[i] * L.x[i];
+        __syncthreads();
+        if (me < D) {
+            L.prev[me] = L.axis[me];
+            L.axis[me] = x;
+        }
+        __syncthreads();
+        if (nrm < .0025f) break;
+    }
+#pragma unroll
+    for (int i = 0; i < D; i++) axis[i] = L.axis[i];
+    double nn = axis[0] * axis[0];
+    for (int i = 1; i < D; i++) nn += axis[i] * axis[i];
+    if (nn != 0) {
+        const float s = (float)(1.0f / sqrt(nn));
+#pragma unroll
+        for (int i = 0; i < D; i++) axis[i] *= s;
+    }
+    __syncthreads();
+}
+
+// First vector of the list (in order) at the largest distance from ref
+// (compute_split_estimate, crn_clusterizer.h:445-481).
+template <int D>
+__device__ void furthest(const VqArgs& A, const uint32_t* __restrict__ list, uint32_t n, const float (&ref)[D],
+                         float (&out)[D])
+{
+    const uint32_t me = lane();
+    float bd = -1.0f;
+    uint32_t bj = 0xFFFFFFFFu;
+    for (uint32_t j = me; j < n; j += 64) {
+        float v[D];
+        load_vec<D>(A, list[j], v);
+        const float d = vsqd<D>(v, ref);
+        if (d > bd) {
+            bd = d;
+            bj = j;
+        }
+    }
+    for (int o = 32; o; o >>= 1) {
+        const float od = __shfl_xor(bd, o);
+        const uint32_t oj = (uint32_t)__shfl_xor((int)bj, o);
+        if (od > bd || (od == bd && oj < bj)) {
+            bd = od;
+            bj = oj;
+        }
+    }
+    if (bj == 0xFFFFFFFFu) {
+#pragma unroll
+        for (int c = 0; c < D; c++) out[c] = 0.0f;
+    } else {
+        load_vec<D>(A, list[bj], out);
+    }
+}
+
+// split_node (crn_clusterizer.h:722-871) of one node per wave.
+template <int D>
+__global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* __restrict__ splits,
+                                                      VqResult* __restrict__ results)
+{
+    __shared__ VqLds<D> L;
+    const VqSplit sp = splits[blockIdx.x];
+    const uint32_t* list = A.pool + sp.seg;
+    const uint32_t n = sp.n, me = lane();
+    float cen[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) cen[c] = A.cen[(size_t)sp.node * D + c];
+    const unsigned long long ptw = A.tw[sp.node];
+    float lc[D], rc[D];
+    if (n == 2) {
+        load_vec<D>(A, list[0], lc);
+        load_vec<D>(A, list[1], rc);
+    } else {
+        // compute_split_pca (:483-601)
+        covariance<D>(L, n, cen,
+                      [&](uint32_t j, float (&v)[D], uint32_t& w) {
+                          load_vec<D>(A, list[j], v);
+                          w = A.w[list[j]];
+                      },
+                      1.0f / (float)ptw, 0.0, 0);
+        float axis[D];
+        pca_axis<D>(L, axis);
+        double lwp = 0, rwp = 0;
+        for (uint32_t j = me; j < n; j += 64) {
+            float v[D], d[D];
+            load_vec<D>(A, list[j], v);
+#pragma unroll
+            for (int c = 0; c < D; c++) d[c] = v[c] - cen[c];
+            const double t = vdot<D>(d, axis);
+            if (t < 0.0f) lwp += (float)A.w[list[j]];
+            else rwp += (float)A.w[list[j]];
+        }
+        lwp = wave_sum_d(lwp);
+        rwp = wave_sum_d(rwp);
+        const float acc = seq_side_sums<D>(L, n, [&](uint32_t j, float (&v)[D], uint32_t& w, uint32_t& s) {
+            load_vec<D>(A, list[j], v);
+            w = A.w[list[j]];
+            float d[D];
+#pragma unroll
+            for (int c = 0; c < D; c++) d[c] = v[c] - cen[c];
+            const double t = vdot<D>(d, axis);
+            s = t < 0.0f ? 0u : 1u;
+        });
+        if (lwp > 0.0f && rwp > 0.0f) {
+            float sl[D], sr[D];
+            gather_sums<D>(acc, 0, sl);
+            gather_sums<D>(acc, 1, sr);
+            const float fl = (float)(1.0f / lwp), fr = (float)(1.0f / rwp);
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+                lc[c] = sl[c] * fl;
+                rc[c] = sr[c] * fr;
+            }
+        } else {
+            float fur[D], opp[D];
+            furthest<D>(A, list, n, cen, fur);
+            furthest<D>(A, list, n, fur, opp);
+#pragma unroll
+            for (int c = 0; c < D; c++) {
+                lc[c] = (fur[c] + cen[c]) * .5f;
+                rc[c] = (opp[c] + cen[c]) * .5f;
+            }
+        }
+    }
+    // 2-means (:746-832)
+    float plc[D], prc[D];
+    unsigned long long lw = 0, rw = 0;
+    float prev_total = 1e+10f, lvar = 0.0f, rvar = 0.0f;
+    uint32_t nl = 0;
+    bool uns = false;
+    for (int loop = 0; loop < 8; loop++) {
+#pragma unroll
+        for (int c = 0; c < D; c++) {
+            plc[c] = lc[c];
+            prc[c] = rc[c];
+        }
+        double lt = 0, rt = 0;
+        unsigned long long lwl = 0, rwl = 0;
+        uint32_t nll = 0;
+        for (uint32_t j = me; j < n; j += 64) {
+            float v[D];
+            load_vec<D>(A, list[j], v);
+            const uint32_t w = A.w[list[j]];
+            const double ld = vsqd<D>(plc, v), rd = vsqd<D>(prc, v);
+            const double tt = vdot<D>(v, v) * (float)w;
+            if (ld < rd) {
+                lwl += w;
+                lt += tt;
+                nll++;
+            } else {
+                rwl += w;
+                rt += tt;
+            }
+        }
+        lw = wave_sum_u64(lwl);
+        rw = wave_sum_u64(rwl);
+        lt = wave_sum_d(lt);
+        rt = wave_sum_d(rt);
+        nl = (uint32_t)wave_sum_u64(nll);
+        if (!lw || !rw) {
+            uns = true;
+            break;
+        }
+        const float acc = seq_side_sums<D>(L, n, [&](uint32_t j, float (&v)[D], uint32_t& w, uint32_t& s) {
+            load_vec<D>(A, list[j], v);
+            w = A.w[list[j]];
+            const double ld = vsqd<D>(plc, v), rd = vsqd<D>(prc, v);
+            s = ld < rd ? 0u : 1u;
+        });
+        float sl[D], sr[D];
+        gather_sums<D>(acc, 0, sl);
+        gather_sums<D>(acc, 1, sr);
+        lvar = (float)(lt - (vdot<D>(sl, sl) / (float)lw));
+        rvar = (float)(rt - (vdot<D>(sr, sr) / (float)rw));
+        const float fl = 1.0f / (float)lw, fr = 1.0f / (float)rw;
+#pragma unroll
+        for (int c = 0; c < D; c++) {
+            lc[c] = sl[c] * fl;
+            rc[c] = sr[c] * fr;
+        }
+        const float total = lvar + rvar;
+        if (total < .00001f) break;
+        if (((prev_total - total) / total) < .00125f) break;
+        prev_total = total;
+    }
+    VqResult r;
+    if (uns) {
+        r.split = 0;
+        r.nl = r.nr = 0;
+        r.lvar = r.rvar = 0.0f;
+        if (me == 0) results[blockIdx.x] = r;
+        return;
+    }
+    // the final round's sides (from the centres before its update), stable
+    uint32_t* out = A.pool + sp.child_seg;
+    uint32_t lpos = 0, rpos = nl;
+    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+        const uint32_t j = j0 + me;
+        bool left = false;
+        uint32_t vi = 0;
+        if (j < n) {
+            float v[D];
+            vi = list[j];
+            load_vec<D>(A, vi, v);
+            left = (double)vsqd<D>(plc, v) < (double)vsqd<D>(prc, v);
+        }
+        const unsigned long long lm = __ballot(j < n && left), rm = __ballot(j < n && !left);
+        const unsigned long long below = me ? (~0ull >> (64 - me)) : 0ull;
+        if (j < n) {
+            if (left) out[lpos + (uint32_t)__popcll(lm & below)] = vi;
+            else out[rpos + (uint32_t)__popcll(rm & below)] = vi;
+        }
+        lpos += (uint32_t)__popcll(lm);
+        rpos += (uint32_t)__popcll(rm);
+    }
+    if (me < D) {
+        A.cen[(size_t)sp.left * D + me] = lc[me];
+        A.cen[(size_t)sp.right * D + me] = rc[me];
+    }
+    if (me == 0) {
+        A.tw[sp.left] = lw;
+        A.tw[sp.right] = rw;
+        r.split = 1;
+        r.nl = nl;
+        r.nr = n - nl;
+        r.lvar = lvar;
+        r.rvar = rvar;
+        results[blockIdx.x] = r;
+    }
+}
+
+// The root of a tree (crn_clusterizer.h:77-96): centroid, weight, variance
+// over the list in order.  One wave.
+template <int D>
+__global__ void __launch_bounds__(64) vq_root_kernel(VqArgs A, uint32_t seg, uint32_t n, uint32_t slot,
+                                                     float* __restrict__ var_out)
+{
+    __shared__ VqLds<D> L;
+    const uint32_t* list = A.pool + seg;
+    const uint32_t me = lane();
+    unsigned long long tw = 0;
+    double tt = 0;
+    for (uint32_t j = me; j < n; j += 64) {
+        float v[D];
+        load_vec<D>(A, list[j], v);
+        const uint32_t w = A.w[list[j]];
+        tw += w;
+        tt += vdot<D>(v, v) * (float)w;
+    }
+    tw = wave_sum_u64(tw);
+    tt = wave_sum_d(tt);
+    const float acc = seq_side_sums<D>(L, n, [&](uint32_t j, float (&v)[D], uint32_t& w, uint32_t& s) {
+        load_vec<D>(A, list[j], v);
+        w = A.w[list[j]];
+        s = 0;
+    });
+    float sum[D];
+    gather_sums<D>(acc, 0, sum);
+    const float var = (float)(tt - (vdot<D>(sum, sum) / (float)tw));
+    const float f = 1.0f / (float)tw;
+    if (me < D) A.cen[(size_t)slot * D + me] = sum[me] * f;
+    if (me == 0) {
+        A.tw[slot] = tw;
+        *var_out = var;
+    }
+}
+
+// threaded_clusterizer::compute_split (crn_threaded_clusterizer.h:237-379):
+// PCA of the listed vectors and the division by the sign of the projection,
+// left part then right part to pool[out_seg, out_seg + n).  One wave per job.
+struct DivJob {
+    uint32_t seg, n, out_seg;
+};
+
+template <int D>
+__global__ void __launch_bounds__(64) vq_division_kernel(VqArgs A, const DivJob* __restrict__ jobs,
+                                                         uint32_t* __restrict__ nleft)
+{
+    __shared__ VqLds<D> L;
+    const DivJob jb = jobs[blockIdx.x];
+    const uint32_t* list = A.pool + jb.seg;
+    const uint32_t n = jb.n, me = lane();
+    double tw = 0;
+    for (uint32_t j = me; j < n; j += 64) tw += A.w[list[j]];
+    tw = wave_sum_d(tw);
+    const float acc = seq_side_sums<D>(L, n, [&](uint32_t j, float (&v)[D], uint32_t& w, uint32_t& s) {
+        load_vec<D>(A, list[j], v);
+        w = A.w[list[j]];
+        s = 0;
+    });
+    float cen[D], axis[D];
+    gather_sums<D>(acc, 0, cen);
+    if (tw == 0.0f) {
+#pragma unroll
+        for (int c = 0; c < D; c++) axis[c] = 0.0f;
+    } else {
+        const double oo = 1.0f / tw;
+#pragma unroll
+        for (int c = 0; c < D; c++) cen[c] = (float)(cen[c] * oo);
+        covariance<D>(L, n, cen,
+                      [&](uint32_t j, float (&v)[D], uint32_t& w) {
+                          load_vec<D>(A, list[j], v);
+                          w = A.w[list[j]];
+                      },
+                      0.0f, oo, 1);
+        pca_axis<D>(L, axis);
+    }
+    uint32_t nl = 0;
+    for (uint32_t j = me; j < n; j += 64) {
+        float v[D], d[D];
+        load_vec<D>(A, list[j], v);
+#pragma unroll
+        for (int c = 0; c < D; c++) d[c] = v[c] - cen[c];
+        nl += vdot<D>(d, axis) < 0.0f ? 1u : 0u;
+    }
+    nl = (uint32_t)wave_sum_u64(nl);
+    uint32_t* out = A.pool + jb.out_seg;
+    uint32_t lpos = 0, rpos = nl;
+    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+        const uint32_t j = j0 + me;
+        bool left = false;
+        uint32_t vi = 0;
+        if (j < n) {
+            float v[D], d[D];
+            vi = list[j];
+            load_vec<D>(A, vi, v);
+#pragma unroll
+            for (int c = 0; c < D; c++) d[c] = v[c] - cen[c];
+            left = vdot<D>(d, axis) < 0.0f;
+        }
+        const unsigned long long lm = __ballot(j < n && left), rm = __ballot(j < n && !left);
+        const unsigned long long below = me ? (~0ull >> (64 - me)) : 0ull;
+        if (j < n) {
+            if (left) out[lpos + (uint32_t)__popcll(lm & below)] = vi;
+            else out[rpos + (uint32_t)__popcll(rm & below)] = vi;
+        }
+        lpos += (uint32_t)__popcll(lm);
+        rpos += (uint32_t)__popcll(rm);
+    }
+    if (me == 0) nleft[blockIdx.x] = nl;
+}
+
+// Clusters as contiguous lists: cluster k's segment copied to idx[ofs[k], ...).
+struct SegCopy {
+    uint32_t seg, n, dst;
+};
+
+__global__ void __launch_bounds__(256) seg_copy_kernel(const uint32_t* __restrict__ pool,
+                                                       const SegCopy* __restrict__ segs, uint32_t* __restrict__ idx)
+{
+    const SegCopy s = segs[blockIdx.x];
+    for (uint32_t j = threadIdx.x; j < s.n; j += blockDim.x) idx[s.dst + j] = pool[s.seg + j];
+}
+
+// ------------------------------------------------------------ pack kernels
+// Texels of the listed blocks, 16 per block, in list order.
+__global__ void __launch_bounds__(256) gather_texels_kernel(const uint32_t* __restrict__ blocks,
+                                                            const uint32_t* __restrict__ idx, uint32_t n,
+                                                            uint32_t* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * 16) return;
+    out[i] = blocks[(size_t)idx[i >> 4] * 16 + (i & 15)];
+}
+
+struct Dxt1Rec {   // crn2_dxt1_blocks_device / crn2_dxt1_clusters_device records
+    unsigned long long error;
+    uint32_t lo_hi, selectors_or_unique, flags, pad;
+};
+
+// Per block of cluster c: the cluster's endpoints and the block's 16
+// selectors from the per-texel selectors (crn_qdxt1.cpp:633-650).
+__global__ void __launch_bounds__(256) cluster_blocks_kernel(const uint32_t* __restrict__ idx,
+                                                             const uint32_t* __restrict__ block_cluster,
+                                                             uint32_t n, const Dxt1Rec* __restrict__ rec,
+                                                             const uint8_t* __restrict__ sel,
+                                                             unsigned long long* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t word = 0;
+    for (int t = 0; t < 16; t++) word |= (uint32_t)(sel[(size_t)i * 16 + t] & 3u) << (2 * t);
+    out[idx[i]] = (unsigned long long)rec[block_cluster[i]].lo_hi | ((unsigned long long)word << 32);
+}
+
+__global__ void __launch_bounds__(256) single_blocks_kernel(const Dxt1Rec* __restrict__ rec, uint32_t n,
+                                                            unsigned long long* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = (unsigned long long)rec[i].lo_hi | ((unsigned long long)rec[i].selectors_or_unique << 32);
+}
+
+__device__ inline uint32_t cdist(int perceptual, int r0, int g0, int b0, int r1, int g1, int b1)
+{
+    const int dr = r0 - r1, dg = g0 - g1, db = b0 - b1;
+    return perceptual ? (uint32_t)(dr * dr * 8 + dg * dg * 25 + db * db) : (uint32_t)(dr * dr + dg * dg + db * db);
+}
+
+// Selector vectors and weights (crn_qdxt1.cpp:843-874).
+__global__ void __launch_bounds__(256) selector_vec_kernel(const unsigned long long* __restrict__ blk, uint32_t n,
+                                                           int perceptual, float* __restrict__ v,
+                                                           uint32_t* __restrict__ w)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const unsigned long long x = blk[b];
+    for (int t = 0; t < 16; t++) v[(size_t)b * 16 + t] = c_linear[(x >> (32 + 2 * t)) & 3];
+    int c0[3], c1[3];
+    unpack565((uint32_t)(x & 0xFFFF), c0);
+    unpack565((uint32_t)((x >> 16) & 0xFFFF), c1);
+    const uint32_t d = cdist(perceptual, c0[0], c0[1], c0[2], c1[0], c1[1], c1[2]) / 2000;
+    w[b] = d < 1 ? 1 : (d > 2048 ? 2048 : d);
+}
+
+// optimize_selectors_task (crn_qdxt1.cpp:714-840), one wave per selector
+// cluster: lane = texel * 4 + selector value, summing that value's error over
+// the group's blocks (exact integer sums).
+__global__ void __launch_bounds__(64) selector_opt_kernel(const uint32_t* __restrict__ blocks,
+                                                          const uint32_t* __restrict__ idx,
+                                                          const uint32_t* __restrict__ ofs,
+                                                          unsigned long long* __restrict__ blk, int perceptual,
+                                                          uint32_t thresh)
+{
+    const uint32_t c = blockIdx.x, me = lane();
+    const uint32_t b0 = ofs[c], m = ofs[c + 1] - b0;
+    if (m <= 1) return;
+    const uint32_t t = me >> 2, s = me & 3;
+    unsigned long long err[2] = {0, 0};
+    uint32_t cnt[2] = {0, 0};
+    for (uint32_t j = 0; j < m; j++) {
+        const uint32_t b = idx[b0 + j];
+        const unsigned long long x = blk[b];
+        const uint32_t lo = (uint32_t)(x & 0xFFFF), hi = (uint32_t)((x >> 16) & 0xFFFF);
+        int typ = 0;
+        if (lo <= hi) {
+            typ = 1;
+            if (thresh > 0) {
+                bool ha = false;
+                for (int k = 0; k < 16; k++) ha |= (blocks[(size_t)b * 16 + k] >> 24) < thresh;
+                if (ha) continue;
+            }
+        }
+        int col[4][3];
+        block_colors(lo, hi, col);
+        const uint32_t p = blocks[(size_t)b * 16 + t];
+        err[typ] += cdist(perceptual, ch(p, 0), ch(p, 1), ch(p, 2), col[s][0], col[s][1], col[s][2]);
+        cnt[typ]++;
+    }
+    for (int typ = 0; typ < 2; typ++) {
+        if (cnt[typ] <= 1) continue;
+        // the first value of least error among the allowed ones (s ascending)
+        const uint32_t base = me & ~3u;
+        uint32_t best_s = 0;
+        unsigned long long best_e = 0xFFFFFFFFFFull;
+        for (uint32_t k = 0; k < (typ ? 3u : 4u); k++) {
+            const unsigned long long e = __shfl(err[typ], (int)(base + k));
+            if (e < best_e) {
+                best_e = e;
+                best_s = k;
+            }
+        }
+        uint32_t word = 0;
+        for (uint32_t tt = 0; tt < 16; tt++) word |= (uint32_t)__shfl((int)best_s, (int)(tt * 4)) << (2 * tt);
+        for (uint32_t j = me; j < m; j += 64) {
+            const uint32_t b = idx[b0 + j];
+            const unsigned long long x = blk[b];
+            const uint32_t lo = (uint32_t)(x & 0xFFFF), hi = (uint32_t)((x >> 16) & 0xFFFF);
+            bool mine = typ == 0 ? lo > hi : lo <= hi;
+            if (mine && typ == 1 && thresh > 0)
+                for (int k = 0; k < 16; k++) mine &= (blocks[(size_t)b * 16 + k] >> 24) >= thresh;
+            if (mine) blk[b] = (x & 0xFFFFFFFFull) | ((unsigned long long)word << 32);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- host side
+std::mutex g_q_mu;
+bool g_q_tables[64];
+
+int q_tables(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_q_mu);
+    if (dev < 0 || dev >= 64) return CRN2_EINVAL;
+    if (g_q_tables[dev]) return CRN2_OK;
+    uint8_t t5[256][2], t6[256][2];
+    build_omatch4(t5, t6);
+    if (hipMemcpyToSymbol(HIP_SYMBOL(q_om5), t5, 512) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(q_om6), t6, 512) != hipSuccess)
+        return CRN2_EHIP;
+    g_q_tables[dev] = true;
+    return CRN2_OK;
+}
+
+uint32_t grid(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+// Device buffers that grow as the trees do.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int grow(size_t need, hipStream_t st, bool keep)
+    {
+        if (need <= bytes) return CRN2_OK;
+        size_t nb = bytes ? bytes : 4096;
+        while (nb < need) nb *= 2;
+        void* q = nullptr;
+        if (hipMalloc(&q, nb) != hipSuccess) return CRN2_ENOMEM;
+        if (keep && bytes && hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+            (void)hipFree(q);
+            return CRN2_EHIP;
+        }
+        if (p) {
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(p);
+        }
+        p = q;
+        bytes = nb;
+        return CRN2_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+struct HNode {
+    uint32_t slot, seg, n;
+    float var;
+    int left = -1, right = -1, cbi = -1;
+    bool uns = false;
+    int state = 0;   // 0 not split yet, 1 split result known, 2 split queued
+    VqResult res;
+    uint32_t child_seg = 0, lslot = 0, rslot = 0;
+};
+
+// One best-first tree replayed on the host (crn_clusterizer.h:65-176).
+struct Tree {
+    std::vector<HNode> nodes;
+    std::vector<uint32_t> heap;
+    uint32_t heap_size = 0, total_leaves = 1, max_size = 0, leaves = 1;
+    int pending = -1, split_index = 0;
+    bool done = false;
+
+    void insert_heap(uint32_t node)
+    {
+        const float var = nodes[node].var;
+        uint32_t pos = ++heap_size;
+        if (heap_size >= heap.size()) heap.resize(heap_size + 1);
+        for (;;) {
+            const uint32_t parent = pos >> 1;
+            if (!parent) break;
+            if (nodes[heap[parent]].var > var) break;
+            heap[pos] = heap[parent];
+            pos = parent;
+        }
+        heap[pos] = node;
+    }
+    void down_heap(uint32_t pos)
+    {
+        uint32_t child;
+        const uint32_t orig = heap[pos];
+        const float ov = nodes[orig].var;
+        while ((child = pos << 1) <= heap_size) {
+            if (child < heap_size && nodes[heap[child]].var < nodes[heap[child + 1]].var) child++;
+            if (ov > nodes[heap[child]].var) break;
+            heap[pos] = heap[child];
+            pos = child;
+        }
+        heap[pos] = orig;
+    }
+    // Replays as far as the known splits allow; returns false when the
+    // popped node still needs its split.
+    bool advance()
+    {
+        for (;;) {
+            if (pending >= 0) {
+                HNode& p = nodes[(uint32_t)pending];
+                if (p.n == 1) {
+                    // split_node returns at once for a single vector
+                } else {
+                    if (p.state != 1) return false;
+                    if (!p.res.split) {
+                        p.uns = true;
+                    } else {
+                        HNode l, r;
+                        l.slot = p.lslot;
+                        l.seg = p.child_seg;
+                        l.n = p.res.nl;
+                        l.var = p.res.lvar;
+                        r.slot = p.rslot;
+                        r.seg = p.child_seg + p.res.nl;
+                        r.n = p.res.nr;
+                        r.var = p.res.rvar;
+                        const uint32_t li = (uint32_t)nodes.size();
+                        nodes.push_back(l);
+                        nodes.push_back(r);
+                        HNode& pp = nodes[(uint32_t)pending];
+                        pp.left = (int)li;
+                        pp.right = (int)li + 1;
+                        pp.cbi = split_index++;
+                        leaves++;
+                        if (nodes[li].n > 1 && nodes[li].var > 0.0f) insert_heap(li);
+                        if (nodes[li + 1].n > 1 && nodes[li + 1].var > 0.0f) insert_heap(li + 1);
+                    }
+                }
+                pending = -1;
+                total_leaves++;
+            }
+            if (!(total_leaves < max_size && heap_size)) {
+                done = true;
+                return true;
+            }
+            const uint32_t worst = heap[1];
+            heap[1] = heap[heap_size];
+            heap_size--;
+            if (heap_size) down_heap(1);
+            pending = (int)worst;
+        }
+    }
+    // The popped node and the heap nodes that may be popped within the
+    // remaining budget, not yet split.
+    void candidates(std::vector<uint32_t>& out) const
+    {
+        out.clear();
+        if (pending >= 0 && nodes[(uint32_t)pending].state == 0) out.push_back((uint32_t)pending);
+        const uint32_t budget = max_size > total_leaves ? max_size - total_leaves : 0;
+        std::vector<std::pair<float, uint32_t>> h;
+        h.reserve(heap_size);
+        for (uint32_t i = 1; i <= heap_size; i++) h.push_back({nodes[heap[i]].var, heap[i]});
+        std::sort(h.begin(), h.end(), [](const std::pair<float, uint32_t>& a, const std::pair<float, uint32_t>& b) {
+            return a.first > b.first;
+        });
+        if (!budget || h.empty()) return;
+        const float edge = h[std::min<size_t>(budget, h.size()) - 1].first;
+        for (size_t i = 0; i < h.size(); i++) {
+            if (i >= budget && h[i].first < edge) break;
+            if (nodes[h[i].second].state == 0) out.push_back(h[i].second);
+        }
+    }
+    // retrieve_clusters (crn_clusterizer.h:321-352): (segment, count) per cluster
+    void retrieve(uint32_t max_clusters, std::vector<std::pair<uint32_t, uint32_t>>& out) const
+    {
+        out.clear();
+        std::vector<uint32_t> stack;
+        uint32_t cur = 0;
+        for (;;) {
+            const HNode& nd = nodes[cur];
+            if (nd.left < 0 || (nd.cbi + 2) > (int)max_clusters) {
+                out.push_back({nd.seg, nd.n});
+                if (stack.empty()) break;
+                cur = stack.back();
+                stack.pop_back();
+                continue;
+            }
+            cur = (uint32_t)nd.left;
+            stack.push_back((uint32_t)nd.right);
+        }
+    }
+};
+
+// The tree machinery of one dimension: vectors, pool, slots.
+struct Forest {
+    uint32_t d = 0;
+    const float* v = nullptr;
+    const uint32_t* w = nullptr;
+    DevBuf pool, cen, tw, jobs, res, varbuf;
+    uint32_t pool_used = 0, slots = 0;
+    hipStream_t st = nullptr;
+
+    void release()
+    {
+        pool.release();
+        cen.release();
+        tw.release();
+        jobs.release();
+        res.release();
+        varbuf.release();
+    }
+    VqArgs args()
+    {
+        VqArgs a;
+        a.v = v;
+        a.w = w;
+        a.pool = (uint32_t*)pool.p;
+        a.cen = (float*)cen.p;
+        a.tw = (unsigned long long*)tw.p;
+        return a;
+    }
+    int reserve(uint32_t more_pool, uint32_t more_slots)
+    {
+        int rc = pool.grow(((size_t)pool_used + more_pool) * 4, st, true);
+        if (rc == CRN2_OK) rc = cen.grow(((size_t)slots + more_slots) * d * 4, st, true);
+        if (rc == CRN2_OK) rc = tw.grow(((size_t)slots + more_slots) * 8, st, true);
+        return rc;
+    }
+    // A new tree over pool[seg, seg + n): root computed on the device.
+    int root(Tree& t, uint32_t seg, uint32_t n, uint32_t max_size)
+    {
+        int rc = reserve(0, 1);
+        if (rc == CRN2_OK) rc = varbuf.grow(4, st, false);
+        if (rc != CRN2_OK) return rc;
+        const uint32_t slot = slots++;
+        VqArgs a = args();
+        float* vb = (float*)varbuf.p;
+        if (d == 2) hipLaunchKernelGGL(vq_root_kernel<2>, dim3(1), dim3(64), 0, st, a, seg, n, slot, vb);
+        else if (d == 6) hipLaunchKernelGGL(vq_root_kernel<6>, dim3(1), dim3(64), 0, st, a, seg, n, slot, vb);
+        else hipLaunchKernelGGL(vq_root_kernel<16>, dim3(1), dim3(64), 0, st, a, seg, n, slot, vb);
+        float var = 0;
+        if (hipMemcpyAsync(&var, vb, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return CRN2_EHIP;
+        t = Tree();
+        HNode r;
+        r.slot = slot;
+        r.seg = seg;
+        r.n = n;
+        r.var = var;
+        t.nodes.push_back(r);
+        t.max_size = max_size;
+        t.heap.assign((size_t)max_size + 2, 0);
+        t.heap[1] = 0;
+        t.heap_size = 1;
+        return CRN2_OK;
+    }
+    // Grows every tree to its end, one device round per step.
+    int grow(std::vector<Tree*>& trees)
+    {
+        std::vector<VqSplit> sp;
+        std::vector<std::pair<Tree*, uint32_t>> who;
+        std::vector<uint32_t> cand;
+        std::vector<VqResult> rs;
+        for (;;) {
+            sp.clear();
+            who.clear();
+            for (Tree* t : trees) {
+                if (t->done) continue;
+                if (t->advance()) continue;
+                t->candidates(cand);
+                for (uint32_t ni : cand) {
+                    HNode& nd = t->nodes[ni];
+                    if (nd.n < 2) continue;
+                    VqSplit s;
+                    s.node = nd.slot;
+                    s.seg = nd.seg;
+                    s.n = nd.n;
+                    sp.push_back(s);
+                    who.push_back({t, ni});
+                }
+            }
+            if (sp.empty()) {
+                bool all = true;
+                for (Tree* t : trees) all = all && t->done;
+                if (all) return CRN2_OK;
+                return CRN2_EINVAL;   // a pending node without a split to run: cannot happen
+            }
+            uint64_t need = 0;
+            for (const VqSplit& s : sp) need += s.n;
+            if (need > 0xFFFFFFFFull - pool_used) return CRN2_ENOMEM;
+            int rc = reserve((uint32_t)need, 2 * (uint32_t)sp.size());
+            if (rc == CRN2_OK) rc = jobs.grow(sp.size() * sizeof(VqSplit), st, false);
+            if (rc == CRN2_OK) rc = res.grow(sp.size() * sizeof(VqResult), st, false);
+            if (rc != CRN2_OK) return rc;
+            for (size_t i = 0; i < sp.size(); i++) {
+                sp[i].child_seg = pool_used;
+                pool_used += sp[i].n;
+                sp[i].left = slots++;
+                sp[i].right = slots++;
+                HNode& nd = who[i].first->nodes[who[i].second];
+                nd.child_seg = sp[i].child_seg;
+                nd.lslot = sp[i].left;
+                nd.rslot = sp[i].right;
+                nd.state = 2;
+            }
+            if (hipMemcpyAsync(jobs.p, sp.data(), sp.size() * sizeof(VqSplit), hipMemcpyHostToDevice, st) !=
+                hipSuccess)
+                return CRN2_EHIP;
+            VqArgs a = args();
+            const dim3 g((uint32_t)sp.size());
+            const VqSplit* js = (const VqSplit*)jobs.p;
+            VqResult* rr = (VqResult*)res.p;
+            if (d == 2) hipLaunchKernelGGL(vq_split_kernel<2>, g, dim3(64), 0, st, a, js, rr);
+            else if (d == 6) hipLaunchKernelGGL(vq_split_kernel<6>, g, dim3(64), 0, st, a, js, rr);
+            else hipLaunchKernelGGL(vq_split_kernel<16>, g, dim3(64), 0, st, a, js, rr);
+            rs.resize(sp.size());
+            if (hipMemcpyAsync(rs.data(), res.p, sp.size() * sizeof(VqResult), hipMemcpyDeviceToHost, st) !=
+                    hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                return CRN2_EHIP;
+            for (size_t i = 0; i < sp.size(); i++) {
+                HNode& nd = who[i].first->nodes[who[i].second];
+                nd.res = rs[i];
+                nd.state = 1;
+            }
+        }
+    }
+    // Threaded division jobs (one launch): left part count per job.
+    int divide(const std::vector<DivJob>& js, std::vector<uint32_t>& nl)
+    {
+        int rc = jobs.grow(js.size() * sizeof(DivJob), st, false);
+        if (rc == CRN2_OK) rc = res.grow(js.size() * 4, st, false);
+        if (rc != CRN2_OK) return rc;
+        if (hipMemcpyAsync(jobs.p, js.data(), js.size() * sizeof(DivJob), hipMemcpyHostToDevice, st) != hipSuccess)
+            return CRN2_EHIP;
+        VqArgs a = args();
+        const DivJob* jp = (const DivJob*)jobs.p;
+        uint32_t* o = (uint32_t*)res.p;
+        const dim3 g((uint32_t)js.size());
+        if (d == 2) hipLaunchKernelGGL(vq_division_kernel<2>, g, dim3(64), 0, st, a, jp, o);
+        else if (d == 6) hipLaunchKernelGGL(vq_division_kernel<6>, g, dim3(64), 0, st, a, jp, o);
+        else hipLaunchKernelGGL(vq_division_kernel<16>, g, dim3(64), 0, st, a, jp, o);
+        nl.resize(js.size());
+        if (hipMemcpyAsync(nl.data(), res.p, js.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return CRN2_EHIP;
+        return CRN2_OK;
+    }
+    // Identity list 0..n-1 at a fresh pool segment.
+    int identity(uint32_t n, uint32_t& seg)
+    {
+        int rc = reserve(n, 0);
+        if (rc != CRN2_OK) return rc;
+        seg = pool_used;
+        pool_used += n;
+        std::vector<uint32_t> id(n);
+        for (uint32_t i = 0; i < n; i++) id[i] = i;
+        if (hipMemcpyAsync((uint32_t*)pool.p + seg, id.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) !=
+                hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return CRN2_EHIP;
+        return CRN2_OK;
+    }
+    // Clusters to contiguous device lists: idx (n entries) and host offsets.
+    int lists(const std::vector<std::pair<uint32_t, uint32_t>>& cl, uint32_t* idx, std::vector<uint32_t>& ofs)
+    {
+        std::vector<SegCopy> sc(cl.size());
+        ofs.assign(cl.size() + 1, 0);
+        for (size_t k = 0; k < cl.size(); k++) {
+            sc[k].seg = cl[k].first;
+            sc[k].n = cl[k].second;
+            sc[k].dst = ofs[k];
+            ofs[k + 1] = ofs[k] + cl[k].second;
+        }
+        if (cl.empty()) return CRN2_OK;
+        int rc = jobs.grow(sc.size() * sizeof(SegCopy), st, false);
+        if (rc != CRN2_OK) return rc;
+        if (hipMemcpyAsync(jobs.p, sc.data(), sc.size() * sizeof(SegCopy), hipMemcpyHostToDevice, st) != hipSuccess)
+            return CRN2_EHIP;
+        hipLaunchKernelGGL(seg_copy_kernel, dim3((uint32_t)sc.size()), dim3(256), 0, st, (const uint32_t*)pool.p,
+                           (const SegCopy*)jobs.p, idx);
+        return hipStreamSynchronize(st) == hipSuccess ? CRN2_OK : CRN2_EHIP;
+    }
+};
+
+// threaded_clusterizer::create_clusters (crn_threaded_clusterizer.h:71-181)
+// over vectors 0..n-1: clusters as pool segments, partitions in order.
+int threaded_clusters(Forest& F, uint32_t n, uint32_t max_clusters, std::vector<std::pair<uint32_t, uint32_t>>& cl)
+{
+    cl.clear();
+    uint32_t seg = 0;
+    int rc = F.identity(n, seg);
+    if (rc != CRN2_OK) return rc;
+    if (max_clusters < 128) {
+        Tree t;
+        rc = F.root(t, seg, n, max_clusters);
+        if (rc != CRN2_OK) return rc;
+        std::vector<Tree*> ts = {&t};
+        rc = F.grow(ts);
+        if (rc != CRN2_OK) return rc;
+        t.retrieve(t.leaves, cl);
+        return CRN2_OK;
+    }
+    rc = F.reserve(2 * n, 0);
+    if (rc != CRN2_OK) return rc;
+    const uint32_t s1 = F.pool_used;
+    F.pool_used += n;
+    std::vector<uint32_t> nl;
+    rc = F.divide({{seg, n, s1}}, nl);
+    if (rc != CRN2_OK) return rc;
+    const uint32_t s2 = F.pool_used;
+    F.pool_used += n;
+    const uint32_t n0 = nl[0], n1 = n - nl[0];
+    rc = F.divide({{s1, n0, s2}, {s1 + n0, n1, s2 + n0}}, nl);
+    if (rc != CRN2_OK) return rc;
+    const uint32_t ps[4] = {s2, s2 + nl[0], s2 + n0, s2 + n0 + nl[1]};
+    const uint32_t pn[4] = {nl[0], n0 - nl[0], nl[1], n1 - nl[1]};
+    uint32_t parts = 0;
+    for (int k = 0; k < 4; k++) parts += pn[k] != 0;
+    const uint32_t target = (max_clusters + (parts / 2)) / parts;
+    std::vector<Tree> trees(4);
+    std::vector<Tree*> tp;
+    for (int k = 0; k < 4; k++) {
+        if (!pn[k]) continue;
+        rc = F.root(trees[k], ps[k], pn[k], target);
+        if (rc != CRN2_OK) return rc;
+        tp.push_back(&trees[k]);
+    }
+    rc = F.grow(tp);
+    if (rc != CRN2_OK) return rc;
+    std::vector<std::pair<uint32_t, uint32_t>> part;
+    for (int k = 0; k < 4; k++) {
+        if (!pn[k]) continue;
+        trees[k].retrieve(trees[k].leaves, part);
+        cl.insert(cl.end(), part.begin(), part.end());
+    }
+    return CRN2_OK;
+}
+
+}  // namespace
+
+struct crn2_qdxt1 {
+    uint32_t n = 0;
+    const uint32_t* blocks = nullptr;
+    hipStream_t st = nullptr;
+    Forest F;            // 6-D endpoint tree
+    Tree tree;
+    uint32_t max_sel = 0;
+    float* vecs = nullptr;
+    uint32_t* weights = nullptr;
+    uint8_t* enc = nullptr;
+    uint32_t endpoint_clusters = 0, selector_clusters = 0;
+};
+
+extern "C" int crn2_qdxt1_init(const void* blocks, uint32_t nblocks, uint32_t nlevels, const uint32_t* level_desc,
+                               int hierarchical, crn2_qdxt1** out, void* stream)
+{
+    if (!blocks || !nblocks || !out || (nlevels && !level_desc)) return CRN2_EINVAL;
+    *out = nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return CRN2_EHIP;
+    int rc = q_tables(dev);
+    if (rc != CRN2_OK) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // chunks of every level (crn_qdxt1.cpp:106-125)
+    std::vector<TChunk> chunks;
+    const bool hier = hierarchical && nlevels;
+    if (hier) {
+        for (uint32_t l = 0; l < nlevels; l++) {
+            const uint32_t first = level_desc[3 * l], bw = level_desc[3 * l + 1], bh = level_desc[3 * l + 2];
+            if (!bw || !bh || (uint64_t)first + (uint64_t)bw * bh > nblocks) return CRN2_EINVAL;
+            float der = 1.5f;
+            if (l && der > .25f) {
+                const float t = der / powf(3.1f, (float)l);
+                der = .25f > t ? .25f : t;
+            }
+            for (uint32_t cy = 0; cy < (bh + 1) / 2; cy++)
+                for (uint32_t cx = 0; cx < (bw + 1) / 2; cx++) chunks.push_back({first, bw, bh, cx, cy, der});
+        }
+    }
+    crn2_qdxt1* q = new crn2_qdxt1();
+    q->n = nblocks;
+    q->blocks = static_cast<const uint32_t*>(blocks);
+    q->st = st;
+    q->F.d = 6;
+    q->F.st = st;
+    TChunk* dch = nullptr;
+    uint32_t* lerr = nullptr;
+    unsigned long long* table = nullptr;
+    uint32_t* count = nullptr;
+    uint32_t tsize = 1;
+    while (tsize < 2 * nblocks) tsize <<= 1;
+    auto fail = [&](int code) {
+        if (dch) (void)hipFree(dch);
+        if (lerr) (void)hipFree(lerr);
+        if (table) (void)hipFree(table);
+        if (count) (void)hipFree(count);
+        crn2_qdxt1_destroy(q);
+        return code;
+    };
+    if (hipMalloc((void**)&q->vecs, (size_t)nblocks * 6 * 4) != hipSuccess ||
+        hipMalloc((void**)&q->weights, (size_t)nblocks * 4) != hipSuccess ||
+        hipMalloc((void**)&q->enc, nblocks) != hipSuccess)
+        return fail(CRN2_ENOMEM);
+    if (hier) {
+        const uint32_t nc = (uint32_t)chunks.size();
+        if (hipMalloc((void**)&dch, nc * sizeof(TChunk)) != hipSuccess ||
+            hipMalloc((void**)&lerr, (size_t)nc * 9 * 4) != hipSuccess)
+            return fail(CRN2_ENOMEM);
+        if (hipMemcpyAsync(dch, chunks.data(), nc * sizeof(TChunk), hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(CRN2_EHIP);
+        hipLaunchKernelGGL(qdxt_layout_kernel, dim3(grid((uint64_t)nc * 9, 256)), dim3(256), 0, st, q->blocks, dch,
+                           nc, lerr);
+        hipLaunchKernelGGL(qdxt_choose_kernel, dim3(grid(nc, 256)), dim3(256), 0, st, q->blocks, dch, nc, lerr,
+                           q->vecs, q->weights, q->enc);
+    } else {
+        hipLaunchKernelGGL(qdxt_block_train_kernel, dim3(grid(nblocks, 256)), dim3(256), 0, st, q->blocks, nblocks,
+                           q->vecs, q->weights, q->enc);
+    }
+    // distinct dxt_fast selector words
+    if (hipMalloc((void**)&table, (size_t)tsize * 8) != hipSuccess || hipMalloc((void**)&count, 4) != hipSuccess)
+        return fail(CRN2_ENOMEM);
+    if (hipMemsetAsync(table, 0, (size_t)tsize * 8, st) != hipSuccess || hipMemsetAsync(count, 0, 4, st) != hipSuccess)
+        return fail(CRN2_EHIP);
+    hipLaunchKernelGGL(qdxt_selword_kernel, dim3(grid(nblocks, 256)), dim3(256), 0, st, q->blocks, nblocks, table,
+                       tsize - 1, count);
+    uint32_t distinct = 0;
+    if (hipMemcpyAsync(&distinct, count, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(CRN2_EHIP);
+    q->max_sel = distinct + 128;
+    // the endpoint tree, grown to 65,535 leaves (crn_qdxt1.cpp:402-407)
+    q->F.v = q->vecs;
+    q->F.w = q->weights;
+    uint32_t seg = 0;
+    rc = q->F.identity(nblocks, seg);
+    if (rc == CRN2_OK) rc = q->F.root(q->tree, seg, nblocks, 65535);
+    std::vector<Tree*> ts = {&q->tree};
+    if (rc == CRN2_OK) rc = q->F.grow(ts);
+    if (rc != CRN2_OK) return fail(rc);
+    (void)hipFree(dch);
+    (void)hipFree(lerr);
+    (void)hipFree(table);
+    (void)hipFree(count);
+    dch = nullptr;
+    lerr = nullptr;
+    table = nullptr;
+    count = nullptr;
+    *out = q;
+    return CRN2_OK;
+}
+
+extern "C" int crn2_qdxt1_training(const crn2_qdxt1* q, float* vecs, uint32_t* weights, uint8_t* encodings)
+{
+    if (!q) return CRN2_EINVAL;
+    if ((vecs && hipMemcpy(vecs, q->vecs, (size_t)q->n * 24, hipMemcpyDeviceToHost) != hipSuccess) ||
+        (weights && hipMemcpy(weights, q->weights, (size_t)q->n * 4, hipMemcpyDeviceToHost) != hipSuccess) ||
+        (encodings && hipMemcpy(encodings, q->enc, q->n, hipMemcpyDeviceToHost) != hipSuccess))
+        return CRN2_EHIP;
+    return CRN2_OK;
+}
+
+extern "C" int crn2_qdxt1_info(const crn2_qdxt1* q, uint32_t* codebook_size, uint32_t* max_selector_clusters,
+                               uint32_t* endpoint_clusters, uint32_t* selector_clusters)
+{
+    if (!q) return CRN2_EINVAL;
+    if (codebook_size) *codebook_size = q->tree.leaves;
+    if (max_selector_clusters) *max_selector_clusters = q->max_sel;
+    if (endpoint_clusters) *endpoint_clusters = q->endpoint_clusters;
+    if (selector_clusters) *selector_clusters = q->selector_clusters;
+    return CRN2_OK;
+}
+
+extern "C" int crn2_qdxt1_pack(crn2_qdxt1* q, const crn2_qdxt1_params* p, float quality_power_mul,
+                               void* out_blocks, void* stream)
+{
+    if (!q || !p || !out_blocks || p->quality_level > 255 || p->dxt_quality < 0 || p->dxt_quality > 4 ||
+        (p->flags & ~7))
+        return CRN2_EINVAL;
+    const int perceptual = p->flags & 1, use_alpha = (p->flags >> 1) & 1;
+    const uint32_t thresh = use_alpha ? (p->dxt1a_alpha_threshold > 255 ? 255 : p->dxt1a_alpha_threshold) : 0;
+    const float quality = p->quality_level / (float)255;
+    // multi-block clusters go through the cluster endpoint search, which
+    // covers the uber search without 3-colour blocks (the path of DXT5's
+    // colour part and of DXT1 without both block types)
+    if (quality < 1.0f && (use_alpha || p->dxt_quality != 4)) return CRN2_EUNSUPPORTED;
+    if (quality >= 1.0f && p->dxt_quality == 0) return CRN2_EUNSUPPORTED;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    q->F.st = st;
+    const float eq = powf(quality, 1.8f * quality_power_mul), sq = powf(quality, 1.65f * quality_power_mul);
+    const uint32_t cbs = q->tree.leaves;
+    uint32_t mec = (uint32_t)(cbs * eq);
+    mec = mec < 96 ? 96 : (mec > cbs ? cbs : mec);
+    uint32_t msc = (uint32_t)(q->max_sel * sq);
+    msc = msc < 128 ? 128 : (msc > q->max_sel ? q->max_sel : msc);
+    unsigned long long* out = static_cast<unsigned long long*>(out_blocks);
+    const uint32_t n = q->n;
+    int rc = CRN2_OK;
+    if (quality >= 1.0f) {
+        // every block alone: the block search with dxt_image's DXT1A rule
+        Dxt1Rec* rec = nullptr;
+        if (hipMallocAsync((void**)&rec, (size_t)n * sizeof(Dxt1Rec), st) != hipSuccess) return CRN2_ENOMEM;
+        const int fl = perceptual | (use_alpha << 1) | (thresh ? 64 : 0) | (int)(thresh << 8);
+        rc = crn2_dxt1_blocks_device(q->blocks, n, p->dxt_quality, fl, rec, st);
+        if (rc == CRN2_OK)
+            hipLaunchKernelGGL(single_blocks_kernel, dim3(grid(n, 256)), dim3(256), 0, st, rec, n, out);
+        (void)hipFreeAsync(rec, st);
+        q->endpoint_clusters = n;
+        q->selector_clusters = 0;
+        if (rc == CRN2_OK && hipStreamSynchronize(st) != hipSuccess) rc = CRN2_EHIP;
+        return rc;
+    }
+    std::vector<std::pair<uint32_t, uint32_t>> cl;
+    q->tree.retrieve(mec, cl);
+    const uint32_t nc = (uint32_t)cl.size();
+    q->endpoint_clusters = nc;
+    uint32_t *idx = nullptr, *px = nullptr, *tofs = nullptr, *bcl = nullptr, *sofs_d = nullptr;
+    Dxt1Rec* rec = nullptr;
+    uint8_t* sel = nullptr;
+    float* sv = nullptr;
+    uint32_t* sw = nullptr;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(st);
+        for (void* x : {(void*)idx, (void*)px, (void*)tofs, (void*)bcl, (void*)rec, (void*)sel, (void*)sv, (void*)sw,
+                        (void*)sofs_d})
+            if (x) (void)hipFree(x);
+    };
+    if (hipMalloc((void**)&idx, (size_t)n * 4) != hipSuccess || hipMalloc((void**)&px, (size_t)n * 64) != hipSuccess ||
+        hipMalloc((void**)&tofs, ((size_t)nc + 1) * 4) != hipSuccess ||
+        hipMalloc((void**)&bcl, (size_t)n * 4) != hipSuccess ||
+        hipMalloc((void**)&rec, (size_t)nc * sizeof(Dxt1Rec)) != hipSuccess ||
+        hipMalloc((void**)&sel, (size_t)n * 16) != hipSuccess) {
+        cleanup();
+        return CRN2_ENOMEM;
+    }
+    std::vector<uint32_t> ofs;
+    rc = q->F.lists(cl, idx, ofs);
+    if (rc != CRN2_OK) {
+        cleanup();
+        return rc;
+    }
+    std::vector<uint32_t> texofs(nc + 1), blockcl(n);
+    for (uint32_t k = 0; k <= nc; k++) texofs[k] = ofs[k] * 16;
+    for (uint32_t k = 0; k < nc; k++)
+        for (uint32_t j = ofs[k]; j < ofs[k + 1]; j++) blockcl[j] = k;
+    if (hipMemcpyAsync(tofs, texofs.data(), ((size_t)nc + 1) * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(bcl, blockcl.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess) {
+        cleanup();
+        return CRN2_EHIP;
+    }
+    hipLaunchKernelGGL(gather_texels_kernel, dim3(grid((uint64_t)n * 16, 256)), dim3(256), 0, st, q->blocks, idx, n,
+                       px);
+    rc = crn2_dxt1_clusters_device(px, tofs, nc, perceptual, rec, sel, st);
+    if (rc != CRN2_OK) {
+        cleanup();
+        return rc;
+    }
+    hipLaunchKernelGGL(cluster_blocks_kernel, dim3(grid(n, 256)), dim3(256), 0, st, idx, bcl, n, rec, sel, out);
+    // selector clusters (crn_qdxt1.cpp:843-874, 1006-1020)
+    if (hipMalloc((void**)&sv, (size_t)n * 64) != hipSuccess || hipMalloc((void**)&sw, (size_t)n * 4) != hipSuccess) {
+        cleanup();
+        return CRN2_ENOMEM;
+    }
+    hipLaunchKernelGGL(selector_vec_kernel, dim3(grid(n, 256)), dim3(256), 0, st, out, n, perceptual, sv, sw);
+    Forest S;
+    S.d = 16;
+    S.v = sv;
+    S.w = sw;
+    S.st = st;
+    std::vector<std::pair<uint32_t, uint32_t>> scl;
+    rc = threaded_clusters(S, n, msc, scl);
+    if (rc == CRN2_OK) {
+        q->selector_clusters = (uint32_t)scl.size();
+        std::vector<uint32_t> sofs;
+        rc = S.lists(scl, idx, sofs);
+        if (rc == CRN2_OK && hipMalloc((void**)&sofs_d, sofs.size() * 4) != hipSuccess) rc = CRN2_ENOMEM;
+        if (rc == CRN2_OK &&
+            hipMemcpyAsync(sofs_d, sofs.data(), sofs.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+            rc = CRN2_EHIP;
+        if (rc == CRN2_OK && !scl.empty())
+            hipLaunchKernelGGL(selector_opt_kernel, dim3((uint32_t)scl.size()), dim3(64), 0, st, q->blocks, idx,
+                               sofs_d, out, perceptual, thresh);
+    }
+    if (rc == CRN2_OK && hipStreamSynchronize(st) != hipSuccess) rc = CRN2_EHIP;
+    S.release();
+    cleanup();
+    return rc;
+}
+
+extern "C" int crn2_qdxt1_destroy(crn2_qdxt1* q)
+{
+    if (!q) return CRN2_OK;
+    if (q->st) (void)hipStreamSynchronize(q->st);
+    q->F.release();
+    if (q->vecs) (void)hipFree(q->vecs);
+    if (q->weights) (void)hipFree(q->weights);
+    if (q->enc) (void)hipFree(q->enc);
+    delete q;
+    return CRN2_OK;
+}
+
+// The clusterizer on its own, for any vectors (dimension 2, 6 or 16) in
+// device memory: grown to max_size leaves and cut at max_clusters (0: every
+// leaf).  idx / ofs (host) receive the clusters.
+extern "C" int crn2_vq_clusters_device(const float* vecs, const uint32_t* weights, uint32_t n, uint32_t dim,
+                                       uint32_t max_size, uint32_t max_clusters, uint32_t* idx, uint32_t* ofs,
+                                       uint32_t* nclusters, uint32_t* codebook_size, void* stream)
+{
+    if (!vecs || !weights || !n || (dim != 2 && dim != 6 && dim != 16) || !idx || !ofs || !nclusters) return CRN2_EINVAL;
+    Forest F;
+    F.d = dim;
+    F.v = vecs;
+    F.w = weights;
+    F.st = static_cast<hipStream_t>(stream);
+    Tree t;
+    uint32_t seg = 0;
+    int rc = F.identity(n, seg);
+    if (rc == CRN2_OK) rc = F.root(t, seg, n, max_size);
+    std::vector<Tree*> ts = {&t};
+    if (rc == CRN2_OK) rc = F.grow(ts);
+    std::vector<std::pair<uint32_t, uint32_t>> cl;
+    uint32_t* didx = nullptr;
+    std::vector<uint32_t> o;
+    if (rc == CRN2_OK) {
+        t.retrieve(max_clusters ? max_clusters : t.leaves, cl);
+        if (hipMalloc((void**)&didx, (size_t)n * 4) != hipSuccess) rc = CRN2_ENOMEM;
+    }
+    if (rc == CRN2_OK) rc = F.lists(cl, didx, o);
+    if (rc == CRN2_OK && hipMemcpy(idx, didx, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = CRN2_EHIP;
+    if (rc == CRN2_OK) {
+        memcpy(ofs, o.data(), o.size() * 4);
+        *nclusters = (uint32_t)cl.size();
+        if (codebook_size) *codebook_size = t.leaves;
+    }
+    if (didx) (void)hipFree(didx);
+    F.release();
+    return rc;
+}
+
+// threaded_clusterizer::create_clusters for vectors in device memory.
+extern "C" int crn2_vq_threaded_clusters_device(const float* vecs, const uint32_t* weights, uint32_t n,
+                                                uint32_t dim, uint32_t max_clusters, uint32_t* idx, uint32_t* ofs,
+                                                uint32_t* nclusters, void* stream)
+{
+    if (!vecs || !weights || !n || (dim != 2 && dim != 6 && dim != 16) || !idx || !ofs || !nclusters) return CRN2_EINVAL;
+    Forest F;
+    F.d = dim;
+    F.v = vecs;
+    F.w = weights;
+    F.st = static_cast<hipStream_t>(stream);
+    std::vector<std::pair<uint32_t, uint32_t>> cl;
+    int rc = threaded_clusters(F, n, max_clusters, cl);
+    uint32_t* didx = nullptr;
+    std::vector<uint32_t> o;
+    if (rc == CRN2_OK && hipMalloc((void**)&didx, (size_t)n * 4) != hipSuccess) rc = CRN2_ENOMEM;
+    if (rc == CRN2_OK) rc = F.lists(cl, didx, o);
+    if (rc == CRN2_OK && hipMemcpy(idx, didx, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = CRN2_EHIP;
+    if (rc == CRN2_OK) {
+        memcpy(ofs, o.data(), o.size() * 4);
+        *nclusters = (uint32_t)cl.size();
+    }
+    if (didx) (void)hipFree(didx);
+    F.release();
+    return rc;
+}

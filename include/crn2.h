@@ -661,6 +661,59 @@ int crn2_mip_contributors(uint32_t src, uint32_t dst, int wrap, uint32_t filter,
  * resample_single_thread (crn_image_utils.cpp:514-545). */
 int crn2_mip_gamma_tables(float gamma, float* srgb_to_linear, uint8_t* linear_to_srgb);
 
+/* ------------------------------------------- clustered DXT1 (qdxt1) */
+/* crnlib's qdxt1 (crnlib/crn_qdxt1.h:36-229, crn_qdxt1.cpp): DXT1 blocks
+ * whose endpoints and selectors are shared across clusters of similar
+ * blocks, the DDS path of crn_compress below the top quality.  blocks
+ * (device): the dxt_pixel_blocks of every level, 16 RGBA8 texels each (r
+ * first), levels back to back; level_desc (host): first block, width and
+ * height in blocks per level (qdxt1_params::m_mip_desc).  init
+ * (qdxt1::init, :80-430) computes the endpoint training vectors, grows the
+ * endpoint tree to 65,535 leaves and counts the distinct dxt_fast selector
+ * words; pack (qdxt1::pack, :876-1023) writes one 8-byte dxt1_block per
+ * block (low colour, high colour, selector bytes) to out_blocks (device).
+ * Endpoint caching is off in the endpoint search.  Supported: quality 255
+ * at any dxt quality but superfast; below 255, the uber search without
+ * 3-colour blocks (flags bit1 clear: DXT1 without both block types, and the
+ * colour part of DXT5) -- CRN2_EUNSUPPORTED otherwise. */
+#define CRN2_QDXT_PERCEPTUAL 1
+#define CRN2_QDXT_ALPHA_BLOCKS 2   /* qdxt1_params::m_use_alpha_blocks */
+#define CRN2_QDXT_HIERARCHICAL 4   /* qdxt1_params::m_hierarchical */
+typedef struct crn2_qdxt1_params {
+    uint32_t quality_level;          /* 0..255 (cCRNMaxQualityLevel) */
+    int dxt_quality;                 /* crn_dxt_quality */
+    int flags;                       /* CRN2_QDXT_* */
+    uint32_t dxt1a_alpha_threshold;  /* texels below are transparent (with alpha blocks) */
+} crn2_qdxt1_params;
+typedef struct crn2_qdxt1 crn2_qdxt1;
+int crn2_qdxt1_init(const void* blocks, uint32_t nblocks, uint32_t nlevels, const uint32_t* level_desc,
+                    int hierarchical, crn2_qdxt1** out, void* stream);
+/* quality_power_mul: qdxt_pack's codebook-curve factor (1; 0.75 for DXT5's
+ * colour, 1.5 for DXT5_CCxY). */
+int crn2_qdxt1_pack(crn2_qdxt1* q, const crn2_qdxt1_params* params, float quality_power_mul, void* out_blocks,
+                    void* stream);
+/* Endpoint codebook size (tree leaves), selector budget (distinct dxt_fast
+ * selector words + 128), and the cluster counts of the last pack. */
+int crn2_qdxt1_info(const crn2_qdxt1* q, uint32_t* codebook_size, uint32_t* max_selector_clusters,
+                    uint32_t* endpoint_clusters, uint32_t* selector_clusters);
+/* Host copies of init's training vectors (6 floats per block), weights and
+ * per-block tile encodings (255: not hierarchical); any pointer may be NULL. */
+int crn2_qdxt1_training(const crn2_qdxt1* q, float* vecs, uint32_t* weights, uint8_t* encodings);
+int crn2_qdxt1_destroy(crn2_qdxt1* q);
+/* clusterizer<vec<dim,float>> (crnlib/crn_clusterizer.h) over n vectors in
+ * device memory (dim 2, 6 or 16; weights u32): grown best first to max_size
+ * leaves, then retrieve_clusters(max_clusters; 0 = every leaf).  idx (host,
+ * n) and ofs (host, n + 1) receive the clusters' vector lists (ascending
+ * within a cluster, clusters in depth-first order). */
+int crn2_vq_clusters_device(const float* vecs, const uint32_t* weights, uint32_t n, uint32_t dim, uint32_t max_size,
+                            uint32_t max_clusters, uint32_t* idx, uint32_t* ofs, uint32_t* nclusters,
+                            uint32_t* codebook_size, void* stream);
+/* threaded_clusterizer::create_clusters (crnlib/crn_threaded_clusterizer.h:
+ * 71-181) for vectors in device memory; results as crn2_vq_clusters_device. */
+int crn2_vq_threaded_clusters_device(const float* vecs, const uint32_t* weights, uint32_t n, uint32_t dim,
+                                     uint32_t max_clusters, uint32_t* idx, uint32_t* ofs, uint32_t* nclusters,
+                                     void* stream);
+
 /* ---------------------------------------------- writer and test textures */
 /* CRC16 of crn_decomp.h:2048-2069. */
 uint16_t crn2_crc16(const void* p, uint32_t len, uint16_t crc);

@@ -1,0 +1,121 @@
+"""GPU clustered DXT1 (crn2_qdxt1_*) and its vector quantizers (crn2_vq_*)
+against the oracle's restatement of crnlib's qdxt1 / clusterizer /
+threaded_clusterizer, bit for bit: the clusters of random 2-, 6- and 16-D
+vectors at several leaf budgets and cuts, the threaded selector clusters,
+the training vectors (hierarchical tiles over mip chains, odd level sizes,
+and per block), and whole packs at several qualities, codebook-curve factors
+and textures, up to a 512 x 512 mip chain."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from crunch2_amd import qdxt, texture
+from crunch2_amd._lib import CRN2_EUNSUPPORTED, lib
+from tests.qdxt_cases import level_blocks, texture_blocks
+
+pytestmark = pytest.mark.gpu
+
+P, A, H = qdxt.PERCEPTUAL, qdxt.ALPHA_BLOCKS, qdxt.HIERARCHICAL
+
+
+def _same_clusters(got, exp, what):
+    assert len(got) == len(exp), (what, len(got), len(exp))
+    for k, (a, b) in enumerate(zip(got, exp)):
+        assert np.array_equal(a, b), (what, k, a[:8], b[:8])
+
+
+def _vectors(d, n, seed):
+    rng = np.random.default_rng(seed)
+    if d == 16:
+        v = rng.integers(0, 4, (n, d)).astype(np.float32)
+        v[: n // 3, :6] = 0
+        w = rng.integers(1, 2049, n).astype(np.uint32)
+    else:
+        centres = rng.integers(0, 256, (24, d))
+        v = np.clip(centres[rng.integers(0, 24, n)] + rng.normal(0, 7, (n, d)), 0, 255).round().astype(np.float32)
+        v[: n // 10] = v[0]
+        w = rng.integers(1, 9, n).astype(np.uint32)
+    return v, w
+
+
+@pytest.mark.parametrize("d,n,max_size", [(6, 5000, 1000), (6, 3000, 65535), (16, 4000, 700), (2, 2000, 300),
+                                          (6, 1, 10), (6, 2, 10)])
+def test_vq_clusters_match_oracle(gpu, d, n, max_size):
+    v, w = _vectors(d, n, d * 7 + n)
+    q = oracle.VQ(v, w, max_size)
+    for cut in (0, 1, 7, 96, max_size // 3):
+        got, cbs = qdxt.vq_clusters(v, w, max_size, cut)
+        assert cbs == q.codebook_size
+        _same_clusters(got, q.retrieve(cut if cut else q.codebook_size), (d, n, max_size, cut))
+    q.close()
+
+
+@pytest.mark.parametrize("d,n,maxc", [(16, 6000, 500), (16, 3000, 100), (6, 2500, 300)])
+def test_vq_threaded_clusters_match_oracle(gpu, d, n, maxc):
+    v, w = _vectors(d, n, 3 * n + maxc)
+    _same_clusters(qdxt.vq_threaded_clusters(v, w, maxc), oracle.vq_threaded_clusters(v, w, maxc), (d, n, maxc))
+
+
+@pytest.mark.parametrize("size,height,kind,hier", [(128, 128, texture.RGB, True), (72, 40, texture.RGBA, True),
+                                                   (64, 64, texture.NORMAL, False)])
+def test_training_vectors_match_oracle(gpu, size, height, kind, hier):
+    blocks, desc = level_blocks(texture.mip_chain(texture.synth(size, height, 5, kind)))
+    ev, ew, ee = oracle.qdxt1_train(blocks, desc, hier)
+    q = qdxt.Qdxt1(blocks, desc, hier)
+    gv, gw, ge = q.training()
+    assert np.array_equal(gv, ev) and np.array_equal(gw, ew) and np.array_equal(ge, ee)
+    assert q.info()["max_selector_clusters"] == oracle.qdxt1_distinct_selectors(blocks) + 128
+    q.close()
+
+
+CASES = [
+    (128, 11, texture.RGB, 128, P | H, 1.0),
+    (128, 12, texture.RGB, 40, P | H, 1.0),
+    (128, 13, texture.RGB, 220, H, 1.0),
+    (96, 14, texture.NORMAL, 128, H, 1.5),
+    (128, 15, texture.RGBA, 90, P | H, 0.75),
+    (128, 16, texture.RGB, 128, P, 1.0),
+    (128, 17, texture.RGBA, 255, P | A | H, 1.0),
+    (64, 18, texture.RGB, 255, H, 1.0),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=["%d-s%d-k%d-q%d-f%d-m%g" % c for c in CASES])
+def test_pack_matches_oracle(gpu, case):
+    size, seed, kind, ql, flags, mul = case
+    blocks, desc = texture_blocks(size, seed, kind)
+    exp, _, st = oracle.qdxt1(blocks, desc, ql, 4, flags, 128, mul)
+    q = qdxt.Qdxt1(blocks, desc, bool(flags & H))
+    got = q.pack(qdxt.params(ql, 4, flags, 128), mul).cpu().numpy().view(np.uint64)
+    info = q.info()
+    assert info["codebook_size"] == st["codebook_size"]
+    assert info["max_selector_clusters"] == st["max_selector_clusters"]
+    assert info["endpoint_clusters"] == st["endpoint_clusters"]
+    if ql < 255:
+        assert info["selector_clusters"] == st["selector_clusters"]
+    bad = np.nonzero(got != exp)[0]
+    assert not len(bad), (len(bad), bad[:5], got[bad[:3]], exp[bad[:3]])
+    q.close()
+
+
+def test_pack_512_chain_and_repacks(gpu):
+    blocks, desc = texture_blocks(512, 21)
+    q = qdxt.Qdxt1(blocks, desc)
+    for ql in (128, 64):
+        exp, _, st = oracle.qdxt1(blocks, desc, ql, 4, P | H, 128, 1.0)
+        got = q.pack(qdxt.params(ql, 4, P | H)).cpu().numpy().view(np.uint64)
+        assert np.array_equal(got, exp), ql
+    q.close()
+
+
+def test_unsupported_configurations(gpu):
+    blocks, desc = texture_blocks(32, 3)
+    q = qdxt.Qdxt1(blocks, desc)
+    out = torch.zeros(len(blocks), dtype=torch.int64, device="cuda")
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for p in (qdxt.params(128, 4, P | A | H), qdxt.params(128, 3, P | H), qdxt.params(255, 0, P | H)):
+        assert lib().crn2_qdxt1_pack(q.h, C.byref(p), 1.0, C.c_void_p(out.data_ptr()), s) == CRN2_EUNSUPPORTED
+    q.close()
