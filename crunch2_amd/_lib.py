@@ -59,6 +59,10 @@ class Qdxt1Params(C.Structure):
                 ("dxt1a_alpha_threshold", C.c_uint32)]
 
 
+class Qdxt5Params(C.Structure):
+    _fields_ = [("quality_level", C.c_uint32), ("dxt_quality", C.c_int), ("flags", C.c_int)]
+
+
 # name -> (restype, argtypes); every function declared in include/crn2.h
 SIGNATURES = {
     "crn2_validate_file": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -207,6 +211,12 @@ SIGNATURES = {
     "crn2_qdxt1_info": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint32)] * 4),
     "crn2_qdxt1_training": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "crn2_qdxt1_destroy": (C.c_int, [C.c_void_p]),
+    "crn2_qdxt5_init": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int, C.c_uint32,
+                                  C.POINTER(C.c_void_p), C.c_void_p]),
+    "crn2_qdxt5_pack": (C.c_int, [C.c_void_p, C.POINTER(Qdxt5Params), C.c_void_p, C.c_void_p]),
+    "crn2_qdxt5_info": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint32)] * 4),
+    "crn2_qdxt5_training": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "crn2_qdxt5_destroy": (C.c_int, [C.c_void_p]),
     "crn2_vq_clusters_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                           C.c_void_p]),

@@ -1009,6 +1009,444 @@ __global__ void __launch_bounds__(64) selector_opt_kernel(const uint32_t* __rest
     }
 }
 
+// ------------------------------------------------------------------- qdxt5
+// dxt5_block::get_block_values (crn_dxt.cpp:395-455)
+__device__ inline void alpha_values(uint32_t l, uint32_t h, uint32_t (&v)[8])
+{
+    v[0] = l;
+    v[1] = h;
+    if (l > h) {
+        for (uint32_t j = 2; j < 8; j++) v[j] = (l * (8 - j) + h * (j - 1)) / 7;
+    } else {
+        for (uint32_t j = 2; j < 6; j++) v[j] = (l * (6 - j) + h * (j - 1)) / 5;
+        v[6] = 0;
+        v[7] = 255;
+    }
+}
+
+// dxt_fast::compress_alpha_block (crn_dxt_fast.cpp:788-826) of N values
+template <int N>
+__device__ inline void fast_alpha(const uint32_t (&a)[N], uint32_t& low8, uint32_t& high8, uint8_t (&sel)[N])
+{
+    int mn = (int)a[0], mx = (int)a[0];
+#pragma unroll
+    for (int i = 1; i < N; i++) {
+        mn = (int)a[i] < mn ? (int)a[i] : mn;
+        mx = (int)a[i] > mx ? (int)a[i] : mx;
+    }
+    low8 = (uint32_t)mx;
+    high8 = (uint32_t)mn;
+    const int dist = mx - mn, bias = mn * 7 - (dist >> 1), dist4 = dist * 4, dist2 = dist * 2;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        int v = (int)a[i] * 7 - bias, ind, t;
+        t = (dist4 - v) >> 31;
+        ind = t & 4;
+        v -= dist4 & t;
+        t = (dist2 - v) >> 31;
+        ind += t & 2;
+        v -= dist2 & t;
+        t = (dist - v) >> 31;
+        ind += t & 1;
+        ind = -ind & 7;
+        ind ^= (2 > ind);
+        sel[i] = (uint8_t)ind;
+    }
+}
+
+template <int N>
+__device__ uint32_t alpha_layout_error(const uint32_t* __restrict__ blocks, const TChunk& c, uint32_t l, uint32_t comp)
+{
+    const uint32_t xo = c_layout[l][0], yo = c_layout[l][1], w = c_layout[l][2];
+    uint32_t a[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) a[i] = (uint32_t)ch(chunk_texel(blocks, c, xo + (uint32_t)i % w, yo + (uint32_t)i / w),
+                                                   (int)comp);
+    uint8_t sel[N];
+    uint32_t lo, hi, v[8];
+    fast_alpha<N>(a, lo, hi, sel);
+    alpha_values(lo, hi, v);
+    uint32_t err = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int d = (int)a[i] - (int)v[sel[i]];
+        err += (uint32_t)(d * d);
+    }
+    return err;
+}
+
+__global__ void __launch_bounds__(256) qdxt5_layout_kernel(const uint32_t* __restrict__ blocks,
+                                                           const TChunk* __restrict__ chunks, uint32_t nchunks,
+                                                           uint32_t comp, uint32_t* __restrict__ lerr)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nchunks * 9) return;
+    const TChunk c = chunks[i / 9];
+    const uint32_t l = i % 9;
+    lerr[i] = l == 0 ? alpha_layout_error<64>(blocks, c, l, comp)
+                     : (l < 5 ? alpha_layout_error<32>(blocks, c, l, comp) : alpha_layout_error<16>(blocks, c, l, comp));
+}
+
+__device__ inline void put_alpha_training(float* __restrict__ vecs, uint32_t* __restrict__ weights, uint32_t b,
+                                          uint32_t lo, uint32_t hi)
+{
+    const int l0 = ch(lo, 0), h0 = ch(hi, 0);
+    vecs[2 * (size_t)b] = (float)l0;
+    vecs[2 * (size_t)b + 1] = (float)h0;
+    const uint32_t w = (uint32_t)((l0 - h0) * (l0 - h0)) / 8;
+    weights[b] = w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
+// qdxt5::init's encoding choice (crn_qdxt5.cpp:196-246) and training vectors
+// (:248-318): tiles of grey texels (the channel in r, g and b).
+__global__ void __launch_bounds__(256) qdxt5_choose_kernel(const uint32_t* __restrict__ blocks,
+                                                           const TChunk* __restrict__ chunks, uint32_t nchunks,
+                                                           uint32_t comp, const uint32_t* __restrict__ lerr,
+                                                           float* __restrict__ vecs, uint32_t* __restrict__ weights,
+                                                           uint8_t* __restrict__ enc)
+{
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= nchunks) return;
+    const TChunk c = chunks[ci];
+    double best = -1.0f;
+    uint32_t be = 0;
+    for (uint32_t e = 0; e < 8; e++) {
+        double total = 0;
+        for (uint32_t t = 0; t < c_enc[e][0]; t++) total += (double)lerr[ci * 9 + c_enc[e][1 + t]];
+        const double ms = total * (1.0f / 64.0f);
+        const double rms = sqrt(ms);
+        double psnr = 999999.0f;
+        if (ms) {
+            psnr = log10(255.0f / rms) * 20.0f;
+            psnr = psnr < 0.0 ? 0.0 : (psnr > 500.0 ? 500.0 : psnr);
+        }
+        const float ader = 0.0f + (c.der - 0.0f) * ((float)(c_enc[e][0] - 1) / 3.0f);
+        psnr = psnr - ader;
+        if (psnr > best) {
+            best = psnr;
+            be = e;
+        }
+    }
+    for (uint32_t t = 0; t < c_enc[be][0]; t++) {
+        const uint32_t l = c_enc[be][1 + t];
+        const uint32_t xo = c_layout[l][0], yo = c_layout[l][1], w = c_layout[l][2], h = c_layout[l][3];
+        uint32_t lo, hi;
+        rep_colors(w * h,
+                   [&](uint32_t i) {
+                       const uint32_t a = (uint32_t)ch(chunk_texel(blocks, c, xo + i % w, yo + i / w), (int)comp);
+                       return a | a << 8 | a << 16 | 0xFF000000u;
+                   },
+                   lo, hi);
+        for (uint32_t y = 0; y < (h >> 2); y++) {
+            const uint32_t by = c.cy * 2 + y + (yo >> 2);
+            if (by >= c.bh) continue;
+            for (uint32_t x = 0; x < (w >> 2); x++) {
+                const uint32_t bx = c.cx * 2 + x + (xo >> 2);
+                if (bx >= c.bw) break;
+                const uint32_t b = c.first + bx + by * c.bw;
+                put_alpha_training(vecs, weights, b, lo, hi);
+                if (enc) enc[b] = (uint8_t)be;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) qdxt5_block_train_kernel(const uint32_t* __restrict__ blocks, uint32_t n,
+                                                                uint32_t comp, float* __restrict__ vecs,
+                                                                uint32_t* __restrict__ weights,
+                                                                uint8_t* __restrict__ enc)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const uint32_t* p = blocks + (size_t)b * 16;
+    uint32_t lo, hi;
+    rep_colors(16,
+               [&](uint32_t i) {
+                   const uint32_t a = (uint32_t)ch(p[i], (int)comp);
+                   return a | a << 8 | a << 16 | 0xFF000000u;
+               },
+               lo, hi);
+    put_alpha_training(vecs, weights, b, lo, hi);
+    if (enc) enc[b] = 255;
+}
+
+__global__ void __launch_bounds__(256) qdxt5_selword_kernel(const uint32_t* __restrict__ blocks, uint32_t n,
+                                                            uint32_t comp, unsigned long long* __restrict__ table,
+                                                            uint32_t mask, uint32_t* __restrict__ count)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    uint32_t a[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) a[i] = (uint32_t)ch(blocks[(size_t)b * 16 + i], (int)comp);
+    uint8_t sel[16];
+    uint32_t lo, hi;
+    fast_alpha<16>(a, lo, hi, sel);
+    unsigned long long word = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) word |= (unsigned long long)sel[i] << (3 * i);
+    const unsigned long long key = word + 1ull;
+    uint32_t h = hash32((uint32_t)word ^ hash32((uint32_t)(word >> 32))) & mask;
+    for (;;) {
+        const unsigned long long prev = atomicCAS(&table[h], 0ull, key);
+        if (prev == 0ull) {
+            atomicAdd(count, 1u);
+            break;
+        }
+        if (prev == key) break;
+        h = (h + 1) & mask;
+    }
+}
+
+// Channel values of the listed blocks, 16 per block, in list order.
+__global__ void __launch_bounds__(256) gather_values_kernel(const uint32_t* __restrict__ blocks,
+                                                            const uint32_t* __restrict__ idx, uint32_t n, uint32_t comp,
+                                                            uint8_t* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * 16) return;
+    out[i] = (uint8_t)ch(blocks[(size_t)idx[i >> 4] * 16 + (i & 15)], (int)comp);
+}
+
+__constant__ uint8_t c_six_invert[8] = {1, 0, 5, 4, 3, 2, 6, 7};
+__constant__ uint8_t c_eight_invert[8] = {1, 0, 7, 6, 5, 4, 3, 2};
+
+struct E2Lds {
+    uint32_t count[256], first[256], uval[256], uw[256], rank[256];
+    uint8_t trial[256], best[256];
+};
+
+// dxt5_endpoint_optimizer::compute (crnlib/crn_dxt5a.cpp:40-262) over one
+// cluster's values, one wave: distinct values in first-appearance order with
+// their counts, every distinct pair, then (better and uber, error left) the
+// +-8 / +-16 box that follows the best; each candidate scored by all lanes
+// (first minimum per value, exact wave sum) and kept on a strict improvement;
+// both palettes when use_both; the canonical order with the invert tables.
+__global__ void __launch_bounds__(64) qdxt5_search_kernel(const uint8_t* __restrict__ vals_all,
+                                                          const uint32_t* __restrict__ ofs, int quality, int use_both,
+                                                          uint32_t* __restrict__ res, uint8_t* __restrict__ sel_all)
+{
+    __shared__ E2Lds L;
+    const uint32_t c = blockIdx.x, me = lane();
+    const uint32_t o = ofs[c], n = ofs[c + 1] - o;
+    if (!n) return;
+    const uint8_t* vals = vals_all + o;
+    for (uint32_t v = me; v < 256; v += 64) {
+        L.count[v] = 0;
+        L.first[v] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    for (uint32_t i = me; i < n; i += 64) {
+        atomicAdd(&L.count[vals[i]], 1u);
+        atomicMin(&L.first[vals[i]], i);
+    }
+    __syncthreads();
+    for (uint32_t v = me; v < 256; v += 64) {
+        if (!L.count[v]) continue;
+        const uint32_t f = L.first[v];
+        uint32_t r = 0;
+        for (uint32_t u = 0; u < 256; u++) r += (L.count[u] && L.first[u] < f) ? 1u : 0u;
+        L.rank[v] = r;
+        L.uval[r] = v;
+        L.uw[r] = L.count[v];
+    }
+    __syncthreads();
+    uint32_t nu = 0;
+    for (uint32_t v = 0; v < 256; v++) nu += L.count[v] ? 1u : 0u;
+    if (nu == 1) {
+        if (me == 0) res[c] = L.uval[0] | L.uval[0] << 8;
+        for (uint32_t i = me; i < n; i += 64) sel_all[o + i] = 0;
+        return;
+    }
+    unsigned long long best_err = ~0ull;
+    uint32_t bf = 0, bs = 0, bt = 0;
+    bool done = false;
+    auto evaluate = [&](uint32_t lo, uint32_t hi) {
+        for (uint32_t type = 0; type < (use_both ? 2u : 1u) && !done; type++) {
+            uint32_t pal[8];
+            pal[0] = lo;
+            pal[1] = hi;
+            if (!type) {
+                for (uint32_t j = 2; j < 8; j++) pal[j] = (lo * (8 - j) + hi * (j - 1)) / 7;
+            } else {
+                for (uint32_t j = 2; j < 6; j++) pal[j] = (lo * (6 - j) + hi * (j - 1)) / 5;
+                pal[6] = 0;
+                pal[7] = 255;
+            }
+            unsigned long long part = 0;
+            for (uint32_t i = me; i < nu; i += 64) {
+                const int val = (int)L.uval[i], w = (int)L.uw[i];
+                uint32_t best = 0xFFFFFFFFu, bsel = 0;
+                for (uint32_t j = 0; j < 8; j++) {
+                    int e = val - (int)pal[j];
+                    e = e * e * w;
+                    if ((uint32_t)e < best) {
+                        best = (uint32_t)e;
+                        bsel = j;
+                        if (!best) break;
+                    }
+                }
+                L.trial[i] = (uint8_t)bsel;
+                part += best;
+            }
+            const unsigned long long err = wave_sum_u64(part);
+            if (err < best_err) {
+                best_err = err;
+                bf = lo;
+                bs = hi;
+                bt = type;
+                for (uint32_t i = me; i < nu; i += 64) L.best[i] = L.trial[i];
+                if (!err) done = true;
+            }
+        }
+    };
+    for (uint32_t i = 0; i + 1 < nu && !done; i++)
+        for (uint32_t j = i + 1; j < nu && !done; j++) evaluate(L.uval[i], L.uval[j]);
+    if (quality >= 3 && best_err && !done) {
+        const int probe = quality == 4 ? 16 : 8;
+        for (int dl = -probe; dl <= probe && !done; dl++) {
+            const int l = (int)bf + dl;
+            if (l < 0) continue;
+            if (l > 255) break;
+            for (int dh = -probe; dh <= probe && !done; dh++) {
+                const int h = (int)bs + dh;
+                if (h < 0) continue;
+                if (h > 255) break;
+                evaluate((uint32_t)l, (uint32_t)h);
+            }
+        }
+    }
+    uint32_t f = bf, sc = bs;
+    int inv = 0;   // 0 none, 1 six, 2 eight
+    if (f != sc) {
+        if (bt) {
+            if (f > sc) {
+                f = bs;
+                sc = bf;
+                inv = 1;
+            }
+        } else if (f <= sc) {
+            f = bs;
+            sc = bf;
+            inv = 2;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = me; i < n; i += 64) {
+        uint32_t s = bf == bs ? 0u : L.best[L.rank[vals[i]]];
+        if (inv == 1) s = c_six_invert[s];
+        else if (inv == 2) s = c_eight_invert[s];
+        sel_all[o + i] = (uint8_t)s;
+    }
+    if (me == 0) res[c] = f | sc << 8;
+}
+
+__global__ void __launch_bounds__(256) alpha_blocks_kernel(const uint32_t* __restrict__ idx,
+                                                           const uint32_t* __restrict__ block_cluster, uint32_t n,
+                                                           const uint32_t* __restrict__ res,
+                                                           const uint8_t* __restrict__ sel,
+                                                           unsigned long long* __restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long word = 0;
+    for (int t = 0; t < 16; t++) word |= (unsigned long long)(sel[(size_t)i * 16 + t] & 7u) << (3 * t);
+    out[idx[i]] = (unsigned long long)(res[block_cluster[i]] & 0xFFFFu) | (word << 16);
+}
+
+__constant__ float c_d5_linear[8] = {0, 7, 1, 2, 3, 4, 5, 6};
+__constant__ float c_d5_linear6[8] = {0, 5, 1, 2, 3, 4, 0, 0};
+
+// Selector vectors of DXT5 alpha blocks (crn_qdxt5.cpp:708-766): kind[b] =
+// 0 / 1 for 8- / 6-value blocks, 2 for a 6-value block using the absolute
+// values 0 and 255 (left out).
+__global__ void __launch_bounds__(256) alpha_selector_vec_kernel(const unsigned long long* __restrict__ blk, uint32_t n,
+                                                                 float* __restrict__ v, uint32_t* __restrict__ w,
+                                                                 uint8_t* __restrict__ kind)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const unsigned long long x = blk[b];
+    const uint32_t lo = (uint32_t)(x & 255), hi = (uint32_t)((x >> 8) & 255);
+    const bool a6 = lo <= hi;
+    bool absolute = false;
+    for (int t = 0; t < 16; t++) {
+        const uint32_t s = (uint32_t)((x >> (16 + 3 * t)) & 7);
+        float f;
+        if (a6) {
+            if (s >= 6) {
+                absolute = true;
+                f = 0.0f;
+            } else {
+                f = c_d5_linear6[s];
+            }
+        } else {
+            f = c_d5_linear[s];
+        }
+        v[(size_t)b * 16 + t] = f;
+    }
+    const uint32_t d = (uint32_t)(((int)lo - (int)hi) * ((int)lo - (int)hi)) / 8;
+    w[b] = d < 1 ? 1 : (d > 2048 ? 2048 : d);
+    kind[b] = absolute ? 2 : (a6 ? 1 : 0);
+}
+
+// optimize_selectors_task of qdxt5 (crn_qdxt5.cpp:578-700), one wave per
+// selector cluster: (texel, value) pair p = lane + 64 r (texel p >> 3, value
+// p & 7) summing the squared error over the group's blocks.
+__global__ void __launch_bounds__(64) alpha_selector_opt_kernel(const uint32_t* __restrict__ blocks,
+                                                                const uint32_t* __restrict__ idx,
+                                                                const uint32_t* __restrict__ ofs,
+                                                                unsigned long long* __restrict__ blk, uint32_t comp)
+{
+    const uint32_t c = blockIdx.x, me = lane();
+    const uint32_t b0 = ofs[c], m = ofs[c + 1] - b0;
+    if (m <= 1) return;
+    unsigned long long err[2][2] = {{0, 0}, {0, 0}};   // [type][r]
+    uint32_t cnt[2] = {0, 0};
+    for (uint32_t j = 0; j < m; j++) {
+        const uint32_t b = idx[b0 + j];
+        const unsigned long long x = blk[b];
+        const uint32_t lo = (uint32_t)(x & 255), hi = (uint32_t)((x >> 8) & 255);
+        const int typ = lo <= hi ? 1 : 0;
+        uint32_t v[8];
+        alpha_values(lo, hi, v);
+        for (int r = 0; r < 2; r++) {
+            const uint32_t p = me + 64 * r, t = p >> 3, s = p & 7;
+            const int e = ch(blocks[(size_t)b * 16 + t], (int)comp) - (int)v[s];
+            err[typ][r] += (unsigned long long)(e * e);
+        }
+        cnt[typ]++;
+    }
+    for (int typ = 0; typ < 2; typ++) {
+        if (cnt[typ] <= 1) continue;
+        uint32_t best_s[2];
+        for (int r = 0; r < 2; r++) {
+            const uint32_t base = me & ~7u;
+            unsigned long long be = 0xFFFFFFFFFFull;
+            best_s[r] = 0;
+            for (uint32_t k = 0; k < 8; k++) {
+                const unsigned long long e = __shfl(err[typ][r], (int)(base + k));
+                if (e < be) {
+                    be = e;
+                    best_s[r] = k;
+                }
+            }
+        }
+        unsigned long long word = 0;
+        for (uint32_t t = 0; t < 16; t++) {
+            const uint32_t r = t >> 3, src = (t & 7) * 8;
+            const uint32_t s = (uint32_t)__shfl((int)best_s[r], (int)src);
+            word |= (unsigned long long)s << (3 * t);
+        }
+        for (uint32_t j = me; j < m; j += 64) {
+            const uint32_t b = idx[b0 + j];
+            const unsigned long long x = blk[b];
+            const bool a6 = (uint32_t)(x & 255) <= (uint32_t)((x >> 8) & 255);
+            if ((int)a6 == typ) blk[b] = (x & 0xFFFFull) | (word << 16);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- host side
 std::mutex g_q_mu;
 bool g_q_tables[64];
@@ -1351,20 +1789,26 @@ struct Forest {
             return CRN2_EHIP;
         return CRN2_OK;
     }
-    // Identity list 0..n-1 at a fresh pool segment.
-    int identity(uint32_t n, uint32_t& seg)
+    // A host list of vector indices at a fresh pool segment.
+    int upload(const std::vector<uint32_t>& list, uint32_t& seg)
     {
+        const uint32_t n = (uint32_t)list.size();
         int rc = reserve(n, 0);
         if (rc != CRN2_OK) return rc;
         seg = pool_used;
         pool_used += n;
-        std::vector<uint32_t> id(n);
-        for (uint32_t i = 0; i < n; i++) id[i] = i;
-        if (hipMemcpyAsync((uint32_t*)pool.p + seg, id.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) !=
-                hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
+        if (n && (hipMemcpyAsync((uint32_t*)pool.p + seg, list.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) !=
+                      hipSuccess ||
+                  hipStreamSynchronize(st) != hipSuccess))
             return CRN2_EHIP;
         return CRN2_OK;
+    }
+    // Identity list 0..n-1 at a fresh pool segment.
+    int identity(uint32_t n, uint32_t& seg)
+    {
+        std::vector<uint32_t> id(n);
+        for (uint32_t i = 0; i < n; i++) id[i] = i;
+        return upload(id, seg);
     }
     // Clusters to contiguous device lists: idx (n entries) and host offsets.
     int lists(const std::vector<std::pair<uint32_t, uint32_t>>& cl, uint32_t* idx, std::vector<uint32_t>& ofs)
@@ -1389,13 +1833,13 @@ struct Forest {
 };
 
 // threaded_clusterizer::create_clusters (crn_threaded_clusterizer.h:71-181)
-// over vectors 0..n-1: clusters as pool segments, partitions in order.
-int threaded_clusters(Forest& F, uint32_t n, uint32_t max_clusters, std::vector<std::pair<uint32_t, uint32_t>>& cl)
+// over the vectors listed (ascending) at pool[seg, seg + n): clusters as pool
+// segments, partitions in order.
+int threaded_clusters_seg(Forest& F, uint32_t seg, uint32_t n, uint32_t max_clusters,
+                          std::vector<std::pair<uint32_t, uint32_t>>& cl)
 {
     cl.clear();
-    uint32_t seg = 0;
-    int rc = F.identity(n, seg);
-    if (rc != CRN2_OK) return rc;
+    int rc = CRN2_OK;
     if (max_clusters < 128) {
         Tree t;
         rc = F.root(t, seg, n, max_clusters);
@@ -1440,6 +1884,13 @@ int threaded_clusters(Forest& F, uint32_t n, uint32_t max_clusters, std::vector<
         cl.insert(cl.end(), part.begin(), part.end());
     }
     return CRN2_OK;
+}
+
+int threaded_clusters(Forest& F, uint32_t n, uint32_t max_clusters, std::vector<std::pair<uint32_t, uint32_t>>& cl)
+{
+    uint32_t seg = 0;
+    const int rc = F.identity(n, seg);
+    return rc != CRN2_OK ? rc : threaded_clusters_seg(F, seg, n, max_clusters, cl);
 }
 
 }  // namespace
@@ -1767,4 +2218,267 @@ extern "C" int crn2_vq_threaded_clusters_device(const float* vecs, const uint32_
     if (didx) (void)hipFree(didx);
     F.release();
     return rc;
+}
+
+// ------------------------------------------------------------------- qdxt5
+struct crn2_qdxt5 {
+    uint32_t n = 0, comp = 3;
+    const uint32_t* blocks = nullptr;
+    hipStream_t st = nullptr;
+    Forest F;            // 2-D endpoint tree
+    Tree tree;
+    uint32_t max_sel = 0;
+    float* vecs = nullptr;
+    uint32_t* weights = nullptr;
+    uint8_t* enc = nullptr;
+    uint32_t endpoint_clusters = 0, selector_clusters = 0;
+};
+
+extern "C" int crn2_qdxt5_init(const void* blocks, uint32_t nblocks, uint32_t nlevels, const uint32_t* level_desc,
+                               int hierarchical, uint32_t comp_index, crn2_qdxt5** out, void* stream)
+{
+    if (!blocks || !nblocks || !out || (nlevels && !level_desc) || comp_index > 3) return CRN2_EINVAL;
+    *out = nullptr;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::vector<TChunk> chunks;
+    const bool hier = hierarchical && nlevels;
+    if (hier) {
+        for (uint32_t l = 0; l < nlevels; l++) {
+            const uint32_t first = level_desc[3 * l], bw = level_desc[3 * l + 1], bh = level_desc[3 * l + 2];
+            if (!bw || !bh || (uint64_t)first + (uint64_t)bw * bh > nblocks) return CRN2_EINVAL;
+            // the alpha PSNR derating (crn_qdxt5.cpp:223-229)
+            float der = 2.4f;
+            if (l && der > .25f) {
+                const float t = der / powf(3.0f, (float)l);
+                der = .25f > t ? .25f : t;
+            }
+            for (uint32_t cy = 0; cy < (bh + 1) / 2; cy++)
+                for (uint32_t cx = 0; cx < (bw + 1) / 2; cx++) chunks.push_back({first, bw, bh, cx, cy, der});
+        }
+    }
+    crn2_qdxt5* q = new crn2_qdxt5();
+    q->n = nblocks;
+    q->comp = comp_index;
+    q->blocks = static_cast<const uint32_t*>(blocks);
+    q->st = st;
+    q->F.d = 2;
+    q->F.st = st;
+    TChunk* dch = nullptr;
+    uint32_t* lerr = nullptr;
+    unsigned long long* table = nullptr;
+    uint32_t* count = nullptr;
+    uint32_t tsize = 1;
+    while (tsize < 2 * nblocks) tsize <<= 1;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(st);
+        if (dch) (void)hipFree(dch);
+        if (lerr) (void)hipFree(lerr);
+        if (table) (void)hipFree(table);
+        if (count) (void)hipFree(count);
+    };
+    auto fail = [&](int code) {
+        release();
+        crn2_qdxt5_destroy(q);
+        return code;
+    };
+    if (hipMalloc((void**)&q->vecs, (size_t)nblocks * 2 * 4) != hipSuccess ||
+        hipMalloc((void**)&q->weights, (size_t)nblocks * 4) != hipSuccess ||
+        hipMalloc((void**)&q->enc, nblocks) != hipSuccess)
+        return fail(CRN2_ENOMEM);
+    if (hier) {
+        const uint32_t nc = (uint32_t)chunks.size();
+        if (hipMalloc((void**)&dch, nc * sizeof(TChunk)) != hipSuccess ||
+            hipMalloc((void**)&lerr, (size_t)nc * 9 * 4) != hipSuccess)
+            return fail(CRN2_ENOMEM);
+        if (hipMemcpyAsync(dch, chunks.data(), nc * sizeof(TChunk), hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(CRN2_EHIP);
+        hipLaunchKernelGGL(qdxt5_layout_kernel, dim3(grid((uint64_t)nc * 9, 256)), dim3(256), 0, st, q->blocks, dch,
+                           nc, comp_index, lerr);
+        hipLaunchKernelGGL(qdxt5_choose_kernel, dim3(grid(nc, 256)), dim3(256), 0, st, q->blocks, dch, nc,
+                           comp_index, lerr, q->vecs, q->weights, q->enc);
+    } else {
+        hipLaunchKernelGGL(qdxt5_block_train_kernel, dim3(grid(nblocks, 256)), dim3(256), 0, st, q->blocks, nblocks,
+                           comp_index, q->vecs, q->weights, q->enc);
+    }
+    if (hipMalloc((void**)&table, (size_t)tsize * 8) != hipSuccess || hipMalloc((void**)&count, 4) != hipSuccess)
+        return fail(CRN2_ENOMEM);
+    if (hipMemsetAsync(table, 0, (size_t)tsize * 8, st) != hipSuccess || hipMemsetAsync(count, 0, 4, st) != hipSuccess)
+        return fail(CRN2_EHIP);
+    hipLaunchKernelGGL(qdxt5_selword_kernel, dim3(grid(nblocks, 256)), dim3(256), 0, st, q->blocks, nblocks,
+                       comp_index, table, tsize - 1, count);
+    uint32_t distinct = 0;
+    if (hipMemcpyAsync(&distinct, count, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return fail(CRN2_EHIP);
+    q->max_sel = distinct + 128;
+    q->F.v = q->vecs;
+    q->F.w = q->weights;
+    uint32_t seg = 0;
+    int rc = q->F.identity(nblocks, seg);
+    if (rc == CRN2_OK) rc = q->F.root(q->tree, seg, nblocks, 65535);
+    std::vector<Tree*> ts = {&q->tree};
+    if (rc == CRN2_OK) rc = q->F.grow(ts);
+    if (rc != CRN2_OK) return fail(rc);
+    release();
+    *out = q;
+    return CRN2_OK;
+}
+
+extern "C" int crn2_qdxt5_training(const crn2_qdxt5* q, float* vecs, uint32_t* weights, uint8_t* encodings)
+{
+    if (!q) return CRN2_EINVAL;
+    if ((vecs && hipMemcpy(vecs, q->vecs, (size_t)q->n * 8, hipMemcpyDeviceToHost) != hipSuccess) ||
+        (weights && hipMemcpy(weights, q->weights, (size_t)q->n * 4, hipMemcpyDeviceToHost) != hipSuccess) ||
+        (encodings && hipMemcpy(encodings, q->enc, q->n, hipMemcpyDeviceToHost) != hipSuccess))
+        return CRN2_EHIP;
+    return CRN2_OK;
+}
+
+extern "C" int crn2_qdxt5_info(const crn2_qdxt5* q, uint32_t* codebook_size, uint32_t* max_selector_clusters,
+                               uint32_t* endpoint_clusters, uint32_t* selector_clusters)
+{
+    if (!q) return CRN2_EINVAL;
+    if (codebook_size) *codebook_size = q->tree.leaves;
+    if (max_selector_clusters) *max_selector_clusters = q->max_sel;
+    if (endpoint_clusters) *endpoint_clusters = q->endpoint_clusters;
+    if (selector_clusters) *selector_clusters = q->selector_clusters;
+    return CRN2_OK;
+}
+
+extern "C" int crn2_qdxt5_pack(crn2_qdxt5* q, const crn2_qdxt5_params* p, void* out_blocks, void* stream)
+{
+    if (!q || !p || !out_blocks || p->quality_level > 255 || p->dxt_quality < 0 || p->dxt_quality > 4 ||
+        (p->flags & ~1))
+        return CRN2_EINVAL;
+    if (p->dxt_quality == 0) return CRN2_EUNSUPPORTED;
+    const int use_both = p->flags & 1;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    q->F.st = st;
+    const float quality = p->quality_level / (float)255;
+    const float eq = powf(quality, 2.1f), sq = powf(quality, 1.65f);
+    const uint32_t cbs = q->tree.leaves;
+    uint32_t mec = (uint32_t)(cbs * eq);
+    mec = mec < 16 ? 16 : (mec > cbs ? cbs : mec);
+    uint32_t msc = (uint32_t)(q->max_sel * sq);
+    msc = msc < 32 ? 32 : (msc > q->max_sel ? q->max_sel : msc);
+    unsigned long long* out = static_cast<unsigned long long*>(out_blocks);
+    const uint32_t n = q->n;
+    std::vector<std::pair<uint32_t, uint32_t>> cl;
+    uint32_t seg0 = 0;
+    int rc = CRN2_OK;
+    if (quality >= 1.0f) {
+        // every block alone, in block order
+        rc = q->F.identity(n, seg0);
+        if (rc != CRN2_OK) return rc;
+        cl.resize(n);
+        for (uint32_t b = 0; b < n; b++) cl[b] = {seg0 + b, 1};
+    } else {
+        q->tree.retrieve(mec, cl);
+    }
+    const uint32_t nc = (uint32_t)cl.size();
+    q->endpoint_clusters = nc;
+    q->selector_clusters = 0;
+    uint32_t *idx = nullptr, *vofs = nullptr, *bcl = nullptr, *res = nullptr, *sw = nullptr, *sofs_d = nullptr;
+    uint8_t *vals = nullptr, *sel = nullptr, *kind = nullptr;
+    float* sv = nullptr;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(st);
+        for (void* x : {(void*)idx, (void*)vofs, (void*)bcl, (void*)res, (void*)sw, (void*)sofs_d, (void*)vals,
+                        (void*)sel, (void*)kind, (void*)sv})
+            if (x) (void)hipFree(x);
+    };
+    if (hipMalloc((void**)&idx, (size_t)n * 4) != hipSuccess ||
+        hipMalloc((void**)&vofs, ((size_t)nc + 1) * 4) != hipSuccess ||
+        hipMalloc((void**)&bcl, (size_t)n * 4) != hipSuccess || hipMalloc((void**)&res, (size_t)nc * 4) != hipSuccess ||
+        hipMalloc((void**)&vals, (size_t)n * 16) != hipSuccess || hipMalloc((void**)&sel, (size_t)n * 16) != hipSuccess) {
+        cleanup();
+        return CRN2_ENOMEM;
+    }
+    std::vector<uint32_t> ofs;
+    rc = q->F.lists(cl, idx, ofs);
+    if (rc != CRN2_OK) {
+        cleanup();
+        return rc;
+    }
+    std::vector<uint32_t> vofs_h(nc + 1), blockcl(n);
+    for (uint32_t k = 0; k <= nc; k++) vofs_h[k] = ofs[k] * 16;
+    for (uint32_t k = 0; k < nc; k++)
+        for (uint32_t j = ofs[k]; j < ofs[k + 1]; j++) blockcl[j] = k;
+    if (hipMemcpyAsync(vofs, vofs_h.data(), ((size_t)nc + 1) * 4, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(bcl, blockcl.data(), (size_t)n * 4, hipMemcpyHostToDevice, st) != hipSuccess) {
+        cleanup();
+        return CRN2_EHIP;
+    }
+    hipLaunchKernelGGL(gather_values_kernel, dim3(grid((uint64_t)n * 16, 256)), dim3(256), 0, st, q->blocks, idx, n,
+                       q->comp, vals);
+    hipLaunchKernelGGL(qdxt5_search_kernel, dim3(nc), dim3(64), 0, st, vals, vofs, p->dxt_quality, use_both, res, sel);
+    hipLaunchKernelGGL(alpha_blocks_kernel, dim3(grid(n, 256)), dim3(256), 0, st, idx, bcl, n, res, sel, out);
+    if (quality >= 1.0f) {
+        rc = hipStreamSynchronize(st) == hipSuccess ? CRN2_OK : CRN2_EHIP;
+        cleanup();
+        return rc;
+    }
+    // selector clusters per block type (crn_qdxt5.cpp:700-841)
+    if (hipMalloc((void**)&sv, (size_t)n * 64) != hipSuccess || hipMalloc((void**)&sw, (size_t)n * 4) != hipSuccess ||
+        hipMalloc((void**)&kind, n) != hipSuccess) {
+        cleanup();
+        return CRN2_ENOMEM;
+    }
+    hipLaunchKernelGGL(alpha_selector_vec_kernel, dim3(grid(n, 256)), dim3(256), 0, st, out, n, sv, sw, kind);
+    std::vector<uint8_t> kh(n);
+    if (hipMemcpyAsync(kh.data(), kind, n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        cleanup();
+        return CRN2_EHIP;
+    }
+    Forest S;
+    S.d = 16;
+    S.v = sv;
+    S.w = sw;
+    S.st = st;
+    std::vector<std::pair<uint32_t, uint32_t>> scl, part;
+    for (uint32_t type = 0; type < 2 && rc == CRN2_OK; type++) {
+        std::vector<uint32_t> members;
+        for (uint32_t b = 0; b < n; b++)
+            if (kh[b] == type) members.push_back(b);
+        const uint32_t cnt = (uint32_t)members.size();
+        if (!cnt) continue;
+        if ((cnt / (float)n) < .01f) continue;
+        uint32_t mc = (uint32_t)(((uint64_t)cnt * msc + (n - 1)) / n);
+        mc = mc < 64 ? 64 : mc;
+        mc = mc < cnt ? mc : cnt;
+        if (mc >= cnt) continue;
+        uint32_t seg = 0;
+        rc = S.upload(members, seg);
+        if (rc == CRN2_OK) rc = threaded_clusters_seg(S, seg, cnt, mc, part);
+        if (rc == CRN2_OK) scl.insert(scl.end(), part.begin(), part.end());
+    }
+    if (rc == CRN2_OK && !scl.empty()) {
+        q->selector_clusters = (uint32_t)scl.size();
+        std::vector<uint32_t> sofs;
+        rc = S.lists(scl, idx, sofs);
+        if (rc == CRN2_OK && hipMalloc((void**)&sofs_d, sofs.size() * 4) != hipSuccess) rc = CRN2_ENOMEM;
+        if (rc == CRN2_OK &&
+            hipMemcpyAsync(sofs_d, sofs.data(), sofs.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
+            rc = CRN2_EHIP;
+        if (rc == CRN2_OK)
+            hipLaunchKernelGGL(alpha_selector_opt_kernel, dim3((uint32_t)scl.size()), dim3(64), 0, st, q->blocks,
+                               idx, sofs_d, out, q->comp);
+    }
+    if (rc == CRN2_OK && hipStreamSynchronize(st) != hipSuccess) rc = CRN2_EHIP;
+    S.release();
+    cleanup();
+    return rc;
+}
+
+extern "C" int crn2_qdxt5_destroy(crn2_qdxt5* q)
+{
+    if (!q) return CRN2_OK;
+    if (q->st) (void)hipStreamSynchronize(q->st);
+    q->F.release();
+    if (q->vecs) (void)hipFree(q->vecs);
+    if (q->weights) (void)hipFree(q->weights);
+    if (q->enc) (void)hipFree(q->enc);
+    delete q;
+    return CRN2_OK;
 }

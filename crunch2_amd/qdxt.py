@@ -6,7 +6,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import Qdxt1Params, check, lib
+from ._lib import Qdxt1Params, Qdxt5Params, check, lib
 
 PERCEPTUAL, ALPHA_BLOCKS, HIERARCHICAL = 1, 2, 4
 UBER = 4
@@ -63,6 +63,63 @@ class Qdxt1:
     def close(self):
         if self.h:
             lib().crn2_qdxt1_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+BOTH_BLOCK_TYPES = 1
+
+
+def params5(quality_level=128, dxt_quality=UBER, flags=BOTH_BLOCK_TYPES):
+    return Qdxt5Params(quality_level, dxt_quality, flags)
+
+
+class Qdxt5:
+    """One qdxt5 object over one channel (comp 3: alpha; 0 / 1: DXN X / Y)."""
+
+    def __init__(self, blocks, level_desc, comp=3, hierarchical=True):
+        if isinstance(blocks, torch.Tensor):
+            self.blocks = blocks.contiguous()
+        else:
+            self.blocks = torch.from_numpy(np.ascontiguousarray(blocks, np.uint8).reshape(-1, 64)).cuda()
+        self.n = self.blocks.shape[0]
+        self.desc = np.ascontiguousarray(level_desc, np.uint32).reshape(-1)
+        h = C.c_void_p()
+        check(lib().crn2_qdxt5_init(C.c_void_p(self.blocks.data_ptr()), self.n, len(self.desc) // 3,
+                                    self.desc.ctypes.data_as(C.c_void_p), int(hierarchical), comp, C.byref(h),
+                                    _stream()), "crn2_qdxt5_init")
+        self.h = h
+
+    def pack(self, p=None, out=None):
+        """The DXT5 alpha blocks (uint64 device tensor [n]: first | second << 8 | selectors << 16)."""
+        p = p if p is not None else params5()
+        if out is None:
+            out = torch.zeros(self.n, dtype=torch.int64, device=self.blocks.device)
+        check(lib().crn2_qdxt5_pack(self.h, C.byref(p), C.c_void_p(out.data_ptr()), _stream()), "crn2_qdxt5_pack")
+        return out
+
+    def info(self):
+        v = [C.c_uint32() for _ in range(4)]
+        check(lib().crn2_qdxt5_info(self.h, *[C.byref(x) for x in v]), "crn2_qdxt5_info")
+        return dict(zip(("codebook_size", "max_selector_clusters", "endpoint_clusters", "selector_clusters"),
+                        [x.value for x in v]))
+
+    def training(self):
+        vecs = np.zeros((self.n, 2), np.float32)
+        w = np.zeros(self.n, np.uint32)
+        enc = np.zeros(self.n, np.uint8)
+        check(lib().crn2_qdxt5_training(self.h, vecs.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p),
+                                        enc.ctypes.data_as(C.c_void_p)), "crn2_qdxt5_training")
+        return vecs, w, enc
+
+    def close(self):
+        if self.h:
+            lib().crn2_qdxt5_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -700,6 +700,31 @@ int crn2_qdxt1_info(const crn2_qdxt1* q, uint32_t* codebook_size, uint32_t* max_
  * per-block tile encodings (255: not hierarchical); any pointer may be NULL. */
 int crn2_qdxt1_training(const crn2_qdxt1* q, float* vecs, uint32_t* weights, uint8_t* encodings);
 int crn2_qdxt1_destroy(crn2_qdxt1* q);
+/* crnlib's qdxt5 (crnlib/crn_qdxt5.h:36-228, crn_qdxt5.cpp): clustered DXT5
+ * alpha blocks of one channel (comp_index: 3 for DXT5's alpha, 0 / 1 for
+ * DXN's X and Y), the alpha half of the DDS path below the top quality.
+ * blocks / level_desc as crn2_qdxt1_init; init (qdxt5::init, :80-428): 2-D
+ * endpoint training vectors, the endpoint tree to 65,535 leaves, the
+ * distinct dxt_fast alpha selector words; pack (qdxt5::pack, :843-935)
+ * writes one 8-byte DXT5 alpha block per block (first, second, 48 bits of
+ * 3-bit selectors, texel 0 lowest) to out_blocks (device).  dxt_quality
+ * fast .. uber (superfast: CRN2_EUNSUPPORTED). */
+#define CRN2_QDXT5_BOTH_BLOCK_TYPES 1   /* qdxt5_params::m_use_both_block_types */
+typedef struct crn2_qdxt5_params {
+    uint32_t quality_level;   /* 0..255 */
+    int dxt_quality;          /* crn_dxt_quality */
+    int flags;                /* CRN2_QDXT5_BOTH_BLOCK_TYPES */
+} crn2_qdxt5_params;
+typedef struct crn2_qdxt5 crn2_qdxt5;
+int crn2_qdxt5_init(const void* blocks, uint32_t nblocks, uint32_t nlevels, const uint32_t* level_desc,
+                    int hierarchical, uint32_t comp_index, crn2_qdxt5** out, void* stream);
+int crn2_qdxt5_pack(crn2_qdxt5* q, const crn2_qdxt5_params* params, void* out_blocks, void* stream);
+int crn2_qdxt5_info(const crn2_qdxt5* q, uint32_t* codebook_size, uint32_t* max_selector_clusters,
+                    uint32_t* endpoint_clusters, uint32_t* selector_clusters);
+/* Host copies of init's training vectors (2 floats per block), weights and
+ * tile encodings; any pointer may be NULL. */
+int crn2_qdxt5_training(const crn2_qdxt5* q, float* vecs, uint32_t* weights, uint8_t* encodings);
+int crn2_qdxt5_destroy(crn2_qdxt5* q);
 /* clusterizer<vec<dim,float>> (crnlib/crn_clusterizer.h) over n vectors in
  * device memory (dim 2, 6 or 16; weights u32): grown best first to max_size
  * leaves, then retrieve_clusters(max_clusters; 0 = every leaf).  idx (host,

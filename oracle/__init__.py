@@ -928,3 +928,49 @@ def qdxt1(blocks, level_desc, quality_level=128, dxt_quality=4,
     assert L.orc_qdxt1(_p(blocks), n, len(ld) // 3, _p(ld), C.byref(p), _p(out), _p(mid), C.byref(st))
     stats = {f: getattr(st, f) for f, _ in QdxtStats._fields_}
     return out, mid, stats
+
+
+class Qdxt5Params(C.Structure):
+    _fields_ = [("quality_level", C.c_uint32), ("dxt_quality", C.c_int), ("flags", C.c_int), ("comp", C.c_uint32)]
+
+
+QDXT5_BOTH = 1
+
+
+def qdxt5_train(blocks, level_desc, comp=3, hierarchical=True):
+    """qdxt5::init's training vectors: (vecs [n, 2], weights [n], encodings [n])."""
+    L = _vq_lib()
+    L.orc_qdxt5_train.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_int, C.c_uint32, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]
+    blocks = np.ascontiguousarray(blocks, np.uint8).reshape(-1, 64)
+    n = len(blocks)
+    ld = np.ascontiguousarray(level_desc, np.uint32).reshape(-1)
+    vecs = np.zeros((n, 2), np.float32)
+    w = np.zeros(n, np.uint32)
+    enc = np.zeros(n, np.uint8)
+    assert L.orc_qdxt5_train(_p(blocks), n, len(ld) // 3, _p(ld), int(hierarchical), comp, _p(vecs), _p(w), _p(enc))
+    return vecs, w, enc
+
+
+def qdxt5_distinct_selectors(blocks, comp=3):
+    L = _vq_lib()
+    L.orc_qdxt5_distinct_selectors.restype = C.c_uint32
+    L.orc_qdxt5_distinct_selectors.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+    blocks = np.ascontiguousarray(blocks, np.uint8).reshape(-1, 64)
+    return L.orc_qdxt5_distinct_selectors(_p(blocks), len(blocks), comp)
+
+
+def qdxt5(blocks, level_desc, quality_level=128, dxt_quality=4, flags=QDXT5_BOTH | QDXT_HIERARCHICAL, comp=3):
+    """qdxt5 init + pack: (alpha blocks u64 [n], blocks after the endpoint pass, stats dict)."""
+    L = _vq_lib()
+    L.orc_qdxt5.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.POINTER(Qdxt5Params), C.c_void_p,
+                            C.c_void_p, C.POINTER(QdxtStats)]
+    blocks = np.ascontiguousarray(blocks, np.uint8).reshape(-1, 64)
+    n = len(blocks)
+    ld = np.ascontiguousarray(level_desc, np.uint32).reshape(-1)
+    out = np.zeros(n, np.uint64)
+    mid = np.zeros(n, np.uint64)
+    p = Qdxt5Params(quality_level, dxt_quality, flags, comp)
+    st = QdxtStats()
+    assert L.orc_qdxt5(_p(blocks), n, len(ld) // 3, _p(ld), C.byref(p), _p(out), _p(mid), C.byref(st))
+    return out, mid, {f: getattr(st, f) for f, _ in QdxtStats._fields_}

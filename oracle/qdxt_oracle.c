@@ -1028,3 +1028,374 @@ int orc_qdxt1(const uint8_t* blocks, uint32_t nblocks, uint32_t nlevels, const u
     free(ofs);
     return 1;
 }
+
+/* ------------------------------------------------------------------ qdxt5 */
+int orc_dxt5a_compute(const uint8_t* values, uint32_t n, int quality, int use_both, uint8_t* selectors,
+                      uint64_t* error, uint8_t* first, uint8_t* second, uint8_t* block_type, uint8_t* reordered);
+
+static const uint8_t k_dxt5_to_linear[8] = {0, 7, 1, 2, 3, 4, 5, 6};
+static const uint8_t k_dxt5_alpha6_to_linear[8] = {0, 5, 1, 2, 3, 4, 0, 0};
+
+/* dxt5_block::get_block_values (crn_dxt.cpp:395-455) */
+static void block_values(uint32_t l, uint32_t h, uint32_t* v)
+{
+    v[0] = l;
+    v[1] = h;
+    if (l > h) {
+        for (uint32_t j = 2; j < 8; j++) v[j] = (l * (8 - j) + h * (j - 1)) / 7;
+    } else {
+        for (uint32_t j = 2; j < 6; j++) v[j] = (l * (6 - j) + h * (j - 1)) / 5;
+        v[6] = 0;
+        v[7] = 255;
+    }
+}
+
+/* dxt_fast::compress_alpha_block (crn_dxt_fast.cpp:788-826) of channel comp */
+static void fast_alpha(uint32_t n, const uint8_t* px, uint32_t comp, uint32_t* low8, uint32_t* high8, uint8_t* sel)
+{
+    int mn = px[comp], mx = px[comp];
+    for (uint32_t i = 1; i < n; i++) {
+        const int v = px[4 * i + comp];
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    *low8 = (uint32_t)mx;
+    *high8 = (uint32_t)mn;
+    const int dist = mx - mn, bias = mn * 7 - (dist >> 1), dist4 = dist * 4, dist2 = dist * 2;
+    for (uint32_t i = 0; i < n; i++) {
+        int a = px[4 * i + comp] * 7 - bias, ind, t;
+        t = (dist4 - a) >> 31;
+        ind = t & 4;
+        a -= dist4 & t;
+        t = (dist2 - a) >> 31;
+        ind += t & 2;
+        a -= dist2 & t;
+        t = (dist - a) >> 31;
+        ind += t & 1;
+        ind = -ind & 7;
+        ind ^= (2 > ind);
+        sel[i] = (uint8_t)ind;
+    }
+}
+
+/* qdxt5::init's endpoint training vectors (crn_qdxt5.cpp:80-400): 2 floats per block */
+int orc_qdxt5_train(const uint8_t* blocks, uint32_t nblocks, uint32_t nlevels, const uint32_t* level_desc,
+                    int hierarchical, uint32_t comp, float* vecs, uint32_t* weights, uint8_t* encodings)
+{
+    if (comp > 3) return 0;
+    if (hierarchical && nlevels) {
+        for (uint32_t level = 0; level < nlevels; level++) {
+            const uint32_t first = level_desc[3 * level], bw = level_desc[3 * level + 1],
+                           bh = level_desc[3 * level + 2];
+            if ((uint64_t)first + (uint64_t)bw * bh > nblocks) return 0;
+            const uint32_t ncx = (bw + 1) / 2, ncy = (bh + 1) / 2, lw = bw * 4, lh = bh * 4;
+            for (uint32_t cy = 0; cy < ncy; cy++)
+                for (uint32_t cx = 0; cx < ncx; cx++) {
+                    uint8_t cp[64 * 4];
+                    for (uint32_t y = 0; y < 8; y++) {
+                        const uint32_t py = cy * 8 + y < lh - 1 ? cy * 8 + y : lh - 1;
+                        for (uint32_t x = 0; x < 8; x++) {
+                            const uint32_t pxx = cx * 8 + x < lw - 1 ? cx * 8 + x : lw - 1;
+                            const uint32_t b = first + (py >> 2) * bw + (pxx >> 2);
+                            memcpy(cp + 4 * (x + y * 8), blocks + 64 * (size_t)b + 4 * ((py & 3) * 4 + (pxx & 3)), 4);
+                        }
+                    }
+                    uint64_t lerr[9];
+                    for (int l = 0; l < 9; l++) {
+                        const uint32_t w = k_layouts[l][2], h = k_layouts[l][3];
+                        const uint32_t xo = k_layouts[l][0], yo = k_layouts[l][1];
+                        uint8_t lp[64 * 4], sel[64];
+                        for (uint32_t y = 0; y < h; y++)
+                            for (uint32_t x = 0; x < w; x++)
+                                memcpy(lp + 4 * (x + y * w), cp + 4 * ((xo + x) + (yo + y) * 8), 4);
+                        const uint32_t n = w * h;
+                        uint32_t lo, hi, vals[8];
+                        fast_alpha(n, lp, comp, &lo, &hi, sel);
+                        block_values(lo, hi, vals);
+                        uint64_t err = 0;
+                        for (uint32_t i = 0; i < n; i++) {
+                            const int dd = (int)lp[4 * i + comp] - (int)vals[sel[i]];
+                            err += (uint64_t)(dd * dd);
+                        }
+                        lerr[l] = err;
+                    }
+                    double best_psnr = -1.0f;
+                    uint32_t best_e = 0;
+                    for (uint32_t e = 0; e < 8; e++) {
+                        double total = 0;
+                        for (uint32_t t = 0; t < k_encodings[e][0]; t++) total += (double)lerr[k_encodings[e][1 + t]];
+                        const double ms = total * (1.0f / 64.0f);
+                        const double rms = sqrt(ms);
+                        double psnr = 999999.0f;
+                        if (ms) {
+                            psnr = log10(255.0f / rms) * 20.0f;
+                            psnr = psnr < 0.0 ? 0.0 : (psnr > 500.0 ? 500.0 : psnr);
+                        }
+                        float der = 2.4f;
+                        if (level && der > .25f) {
+                            const float t = der / powf(3.0f, (float)level);
+                            der = .25f > t ? .25f : t;
+                        }
+                        const float ader = 0.0f + (der - 0.0f) * ((float)(k_encodings[e][0] - 1) / 3.0f);
+                        psnr = psnr - ader;
+                        if (psnr > best_psnr) {
+                            best_psnr = psnr;
+                            best_e = e;
+                        }
+                    }
+                    for (uint32_t t = 0; t < k_encodings[best_e][0]; t++) {
+                        const uint32_t l = k_encodings[best_e][1 + t];
+                        const uint32_t xo = k_layouts[l][0], yo = k_layouts[l][1], w = k_layouts[l][2],
+                                       h = k_layouts[l][3];
+                        uint8_t tp[64 * 4], lo[3], hi[3];
+                        for (uint32_t y = 0; y < h; y++)
+                            for (uint32_t x = 0; x < w; x++) {
+                                const uint8_t a = cp[4 * ((xo + x) + (yo + y) * 8) + comp];
+                                uint8_t* d = tp + 4 * (x + y * w);
+                                d[0] = d[1] = d[2] = a;
+                                d[3] = 255;
+                            }
+                        representative_colors(w * h, tp, lo, hi);
+                        const int dist = ((int)lo[0] - (int)hi[0]) * ((int)lo[0] - (int)hi[0]);
+                        uint32_t wt = (uint32_t)dist / 8;
+                        wt = wt < 1 ? 1 : (wt > 8 ? 8 : wt);
+                        for (uint32_t y = 0; y < (h >> 2); y++) {
+                            const uint32_t by = cy * 2 + y + (yo >> 2);
+                            if (by >= bh) continue;
+                            for (uint32_t x = 0; x < (w >> 2); x++) {
+                                const uint32_t bx = cx * 2 + x + (xo >> 2);
+                                if (bx >= bw) break;
+                                const uint32_t b = first + bx + by * bw;
+                                vecs[2 * (size_t)b] = lo[0];
+                                vecs[2 * (size_t)b + 1] = hi[0];
+                                weights[b] = wt;
+                                if (encodings) encodings[b] = (uint8_t)best_e;
+                            }
+                        }
+                    }
+                }
+        }
+    } else {
+        for (uint32_t b = 0; b < nblocks; b++) {
+            uint8_t tp[16 * 4], lo[3], hi[3];
+            for (int i = 0; i < 16; i++) {
+                tp[4 * i] = tp[4 * i + 1] = tp[4 * i + 2] = blocks[64 * (size_t)b + 4 * i + comp];
+                tp[4 * i + 3] = 255;
+            }
+            representative_colors(16, tp, lo, hi);
+            const int dist = ((int)lo[0] - (int)hi[0]) * ((int)lo[0] - (int)hi[0]);
+            uint32_t wt = (uint32_t)dist / 8;
+            vecs[2 * (size_t)b] = lo[0];
+            vecs[2 * (size_t)b + 1] = hi[0];
+            weights[b] = wt < 1 ? 1 : (wt > 8 ? 8 : wt);
+            if (encodings) encodings[b] = 255;
+        }
+    }
+    return 1;
+}
+
+static int cmp_u64(const void* a, const void* b)
+{
+    const uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+/* distinct dxt_fast alpha selector words (crn_qdxt5.cpp:409-422) */
+uint32_t orc_qdxt5_distinct_selectors(const uint8_t* blocks, uint32_t nblocks, uint32_t comp)
+{
+    uint64_t* s = (uint64_t*)malloc(sizeof(uint64_t) * (nblocks ? nblocks : 1));
+    for (uint32_t b = 0; b < nblocks; b++) {
+        uint8_t sel[16];
+        uint32_t lo, hi;
+        fast_alpha(16, blocks + 64 * (size_t)b, comp, &lo, &hi, sel);
+        uint64_t m = 0;
+        for (int i = 0; i < 16; i++) m |= (uint64_t)sel[i] << (3 * i);
+        s[b] = m;
+    }
+    qsort(s, nblocks, sizeof(uint64_t), cmp_u64);
+    uint32_t u = 0;
+    for (uint32_t b = 0; b < nblocks; b++) u += !b || s[b] != s[b - 1];
+    free(s);
+    return u;
+}
+
+typedef struct {
+    uint32_t quality_level;
+    int dxt_quality;
+    int flags;       /* bit0 use_both_block_types, bit2 hierarchical */
+    uint32_t comp;   /* channel */
+} orc_qdxt5_params;
+
+/* qdxt5 init + pack (crn_qdxt5.cpp): out: one 8-byte DXT5 alpha block per
+   block (first | second << 8 | 3-bit selectors << 16, texel 0 lowest);
+   mid: after the endpoint pass. */
+int orc_qdxt5(const uint8_t* blocks, uint32_t nblocks, uint32_t nlevels, const uint32_t* level_desc,
+              const orc_qdxt5_params* p, uint64_t* out, uint64_t* mid, orc_qdxt1_stats* st)
+{
+    if (!nblocks || p->quality_level > 255 || p->dxt_quality < 1 || p->dxt_quality > 4 || p->comp > 3) return 0;
+    const int use_both = p->flags & 1, hier = (p->flags >> 2) & 1;
+    float* vecs = (float*)malloc(sizeof(float) * 2 * (size_t)nblocks);
+    uint32_t* w = (uint32_t*)malloc(sizeof(uint32_t) * nblocks);
+    if (!orc_qdxt5_train(blocks, nblocks, nlevels, level_desc, hier, p->comp, vecs, w, 0)) {
+        free(vecs);
+        free(w);
+        return 0;
+    }
+    orc_vq* q = orc_vq_create(vecs, w, nblocks, 2, 65535, 0);
+    const uint32_t max_sel = orc_qdxt5_distinct_selectors(blocks, nblocks, p->comp) + 128;
+    const float quality = p->quality_level / (float)255;
+    const float eq = powf(quality, 2.1f), sq = powf(quality, 1.65f);
+    const uint32_t cbs = q->codebook_size;
+    uint32_t mec = (uint32_t)(cbs * eq);
+    mec = mec < 16 ? 16 : (mec > cbs ? cbs : mec);
+    uint32_t msc = (uint32_t)(max_sel * sq);
+    msc = msc < 32 ? 32 : (msc > max_sel ? max_sel : msc);
+    uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * nblocks);
+    uint32_t* ofs = (uint32_t*)malloc(sizeof(uint32_t) * (nblocks + 1));
+    uint32_t nc;
+    if (quality >= 1.0f) {
+        for (uint32_t i = 0; i < nblocks; i++) {
+            idx[i] = i;
+            ofs[i] = i;
+        }
+        ofs[nblocks] = nblocks;
+        nc = nblocks;
+    } else {
+        nc = orc_vq_retrieve(q, mec, idx, ofs);
+    }
+    orc_vq_destroy(q);
+    free(vecs);
+    free(w);
+    uint8_t* vals = (uint8_t*)malloc(16 * (size_t)nblocks);
+    uint8_t* sel = (uint8_t*)malloc(16 * (size_t)nblocks);
+    for (uint32_t c = 0; c < nc; c++) {
+        const uint32_t m = ofs[c + 1] - ofs[c];
+        for (uint32_t j = 0; j < m; j++)
+            for (int t = 0; t < 16; t++) vals[16 * j + t] = blocks[64 * (size_t)idx[ofs[c] + j] + 4 * t + p->comp];
+        uint64_t err;
+        uint8_t f, s, bt, ro;
+        orc_dxt5a_compute(vals, 16 * m, p->dxt_quality, use_both, sel, &err, &f, &s, &bt, &ro);
+        for (uint32_t j = 0; j < m; j++) {
+            uint64_t word = 0;
+            for (int t = 0; t < 16; t++) word |= (uint64_t)sel[16 * j + t] << (3 * t);
+            out[idx[ofs[c] + j]] = (uint64_t)f | ((uint64_t)s << 8) | (word << 16);
+        }
+    }
+    free(vals);
+    free(sel);
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->codebook_size = cbs;
+        st->max_selector_clusters = max_sel;
+        st->endpoint_clusters = nc;
+        st->max_endpoint_clusters = mec;
+        st->max_selector_target = msc;
+    }
+    if (mid) memcpy(mid, out, 8 * (size_t)nblocks);
+    if (quality >= 1.0f) {
+        free(idx);
+        free(ofs);
+        return 1;
+    }
+    /* selector clusters per block type (crn_qdxt5.cpp:700-841) */
+    float* sv = (float*)malloc(sizeof(float) * 16 * (size_t)nblocks);
+    uint32_t* sw = (uint32_t*)malloc(sizeof(uint32_t) * nblocks);
+    uint32_t* remap = (uint32_t*)malloc(sizeof(uint32_t) * nblocks);
+    uint32_t* cidx = (uint32_t*)malloc(sizeof(uint32_t) * nblocks);
+    uint32_t* cofs = (uint32_t*)malloc(sizeof(uint32_t) * (nblocks + 1));
+    uint32_t ns = 0, pos = 0;
+    ofs[0] = 0;
+    for (uint32_t type = 0; type < 2; type++) {
+        uint32_t cnt = 0;
+        for (uint32_t b = 0; b < nblocks; b++) {
+            const uint32_t lo = (uint32_t)(out[b] & 255), hi = (uint32_t)((out[b] >> 8) & 255);
+            const uint32_t a6 = lo <= hi;
+            if (a6 != type) continue;
+            int absolute = 0;
+            float v[16];
+            for (int t = 0; t < 16; t++) {
+                const uint32_t s = (uint32_t)((out[b] >> (16 + 3 * t)) & 7);
+                if (a6) {
+                    if (s >= 6) {
+                        absolute = 1;
+                        v[t] = 0.0f;
+                    } else {
+                        v[t] = k_dxt5_alpha6_to_linear[s];
+                    }
+                } else {
+                    v[t] = k_dxt5_to_linear[s];
+                }
+            }
+            if (absolute) continue;
+            memcpy(sv + 16 * (size_t)cnt, v, sizeof(v));
+            const int d = ((int)lo - (int)hi) * ((int)lo - (int)hi);
+            uint32_t wt = (uint32_t)d / 8;
+            sw[cnt] = wt < 1 ? 1 : (wt > 2048 ? 2048 : wt);
+            remap[cnt] = b;
+            cnt++;
+        }
+        if (!cnt) continue;
+        if ((cnt / (float)nblocks) < .01f) continue;
+        uint32_t mc = (uint32_t)(((uint64_t)cnt * msc + (nblocks - 1)) / nblocks);
+        mc = mc < 64 ? 64 : mc;
+        mc = mc < cnt ? mc : cnt;
+        if (mc >= cnt) continue;
+        const uint32_t k = orc_vq_threaded_clusters(sv, sw, cnt, 16, mc, cidx, cofs);
+        for (uint32_t c = 0; c < k; c++) {
+            for (uint32_t j = cofs[c]; j < cofs[c + 1]; j++) idx[pos++] = remap[cidx[j]];
+            ofs[++ns] = pos;
+        }
+    }
+    free(sv);
+    free(sw);
+    free(remap);
+    free(cidx);
+    free(cofs);
+    if (st) st->selector_clusters = ns;
+    /* optimize_selectors_task (:578-700) */
+    uint32_t* cat[2];
+    cat[0] = (uint32_t*)malloc(sizeof(uint32_t) * nblocks);
+    cat[1] = (uint32_t*)malloc(sizeof(uint32_t) * nblocks);
+    for (uint32_t c = 0; c < ns; c++) {
+        const uint32_t m = ofs[c + 1] - ofs[c];
+        if (m <= 1) continue;
+        uint32_t nct[2] = {0, 0};
+        for (uint32_t j = 0; j < m; j++) {
+            const uint32_t b = idx[ofs[c] + j];
+            const uint32_t a6 = (uint32_t)(out[b] & 255) <= (uint32_t)((out[b] >> 8) & 255);
+            cat[a6][nct[a6]++] = b;
+        }
+        for (int type = 0; type < 2; type++) {
+            if (nct[type] <= 1) continue;
+            uint64_t word = 0;
+            for (int t = 0; t < 16; t++) {
+                uint32_t best_s = 0;
+                uint64_t best_e = 0xFFFFFFFFFFULL;
+                for (uint32_t s = 0; s < 8; s++) {
+                    uint64_t total = 0;
+                    for (uint32_t j = 0; j < nct[type]; j++) {
+                        const uint32_t b = cat[type][j];
+                        uint32_t v[8];
+                        block_values((uint32_t)(out[b] & 255), (uint32_t)((out[b] >> 8) & 255), v);
+                        const int e = (int)blocks[64 * (size_t)b + 4 * t + p->comp] - (int)v[s];
+                        total += (uint64_t)(e * e);
+                    }
+                    if (total < best_e) {
+                        best_e = total;
+                        best_s = s;
+                    }
+                }
+                word |= (uint64_t)best_s << (3 * t);
+            }
+            for (uint32_t j = 0; j < nct[type]; j++) {
+                const uint32_t b = cat[type][j];
+                out[b] = (out[b] & 0xFFFFull) | (word << 16);
+            }
+        }
+    }
+    free(cat[0]);
+    free(cat[1]);
+    free(idx);
+    free(ofs);
+    return 1;
+}

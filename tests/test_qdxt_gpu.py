@@ -119,3 +119,41 @@ def test_unsupported_configurations(gpu):
     for p in (qdxt.params(128, 4, P | A | H), qdxt.params(128, 3, P | H), qdxt.params(255, 0, P | H)):
         assert lib().crn2_qdxt1_pack(q.h, C.byref(p), 1.0, C.c_void_p(out.data_ptr()), s) == CRN2_EUNSUPPORTED
     q.close()
+
+
+# ------------------------------------------------------------------ qdxt5
+@pytest.mark.parametrize("size,kind,comp,hier", [(96, texture.RGBA, 3, True), (64, texture.NORMAL, 1, False)])
+def test_qdxt5_training_matches_oracle(gpu, size, kind, comp, hier):
+    blocks, desc = texture_blocks(size, 8, kind)
+    ev, ew, ee = oracle.qdxt5_train(blocks, desc, comp, hier)
+    q = qdxt.Qdxt5(blocks, desc, comp, hier)
+    gv, gw, ge = q.training()
+    assert np.array_equal(gv, ev) and np.array_equal(gw, ew) and np.array_equal(ge, ee)
+    assert q.info()["max_selector_clusters"] == oracle.qdxt5_distinct_selectors(blocks, comp) + 128
+    q.close()
+
+
+CASES5 = [
+    (128, 31, texture.RGBA, 3, 128, 4, 1),
+    (128, 32, texture.RGBA, 3, 60, 4, 0),
+    (128, 33, texture.NORMAL, 0, 200, 3, 1),
+    (96, 34, texture.NORMAL, 1, 128, 2, 1),
+    (64, 35, texture.RGBA, 3, 255, 4, 1),
+]
+
+
+@pytest.mark.parametrize("case", CASES5, ids=["%d-s%d-k%d-c%d-q%d-d%d-b%d" % c for c in CASES5])
+def test_qdxt5_pack_matches_oracle(gpu, case):
+    size, seed, kind, comp, ql, dq, both = case
+    blocks, desc = texture_blocks(size, seed, kind)
+    exp, _, st = oracle.qdxt5(blocks, desc, ql, dq, both | oracle.QDXT_HIERARCHICAL, comp)
+    q = qdxt.Qdxt5(blocks, desc, comp)
+    got = q.pack(qdxt.params5(ql, dq, both)).cpu().numpy().view(np.uint64)
+    info = q.info()
+    assert info["codebook_size"] == st["codebook_size"]
+    assert info["endpoint_clusters"] == st["endpoint_clusters"]
+    if ql < 255:
+        assert info["selector_clusters"] == st["selector_clusters"]
+    bad = np.nonzero(got != exp)[0]
+    assert not len(bad), (len(bad), bad[:5], got[bad[:3]], exp[bad[:3]])
+    q.close()

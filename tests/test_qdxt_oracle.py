@@ -269,3 +269,113 @@ def test_cluster_counts_follow_the_quality():
         assert st["endpoint_clusters"] >= prev
         prev = st["endpoint_clusters"]
     assert st["max_selector_clusters"] == oracle.qdxt1_distinct_selectors(blocks) + 128
+
+
+# ------------------------------------------------------------------ qdxt5
+D5_LINEAR = np.array([0, 7, 1, 2, 3, 4, 5, 6], np.float32)
+D5_LINEAR6 = np.array([0, 5, 1, 2, 3, 4, 0, 0], np.float32)
+
+
+def _values(lo, hi):
+    lo, hi = int(lo), int(hi)
+    if lo > hi:
+        return np.array([lo, hi] + [(lo * (8 - j) + hi * (j - 1)) // 7 for j in range(2, 8)], np.int64)
+    return np.array([lo, hi] + [(lo * (6 - j) + hi * (j - 1)) // 5 for j in range(2, 6)] + [0, 255], np.int64)
+
+
+def test_qdxt5_quality_255_is_the_block_search():
+    blocks, desc = texture_blocks(64, 4, texture.RGBA)
+    for flags, use_both in ((5, True), (4, False)):
+        out, mid, st = oracle.qdxt5(blocks, desc, 255, 4, flags, 3)
+        vals = blocks.reshape(-1, 16, 4)[:, :, 3]
+        sel, err, first, second = oracle.dxt5a_blocks(vals, 4, use_both)[:4]
+        word = np.zeros(len(blocks), np.uint64)
+        for t in range(16):
+            word |= sel[:, t].astype(np.uint64) << np.uint64(3 * t)
+        assert (out == (first.astype(np.uint64) | (second.astype(np.uint64) << np.uint64(8)) |
+                        (word << np.uint64(16)))).all()
+
+
+@pytest.mark.parametrize("q,flags,comp", [(128, 5, 3), (60, 4, 3), (180, 5, 1)])
+def test_qdxt5_selector_optimisation_brute_force(q, flags, comp):
+    blocks, desc = texture_blocks(64, 7, texture.RGBA if comp == 3 else texture.NORMAL)
+    out, mid, st = oracle.qdxt5(blocks, desc, q, 4, flags, comp)
+    n = len(blocks)
+    sv = np.zeros((n, 16), np.float32)
+    sw = np.zeros(n, np.uint32)
+    a6 = np.zeros(n, bool)
+    absolute = np.zeros(n, bool)
+    for b in range(n):
+        x = int(mid[b])
+        lo, hi = x & 255, (x >> 8) & 255
+        s = np.array([(x >> (16 + 3 * t)) & 7 for t in range(16)])
+        a6[b] = lo <= hi
+        absolute[b] = a6[b] and (s >= 6).any()
+        sv[b] = np.where(a6[b], D5_LINEAR6[s], D5_LINEAR[s])
+        sw[b] = min(2048, max(1, (lo - hi) ** 2 // 8))
+    clusters = []
+    for typ in (0, 1):
+        members = np.nonzero((a6 == bool(typ)) & ~absolute)[0]
+        if not len(members) or len(members) / float(n) < .01:
+            continue
+        mc = min(max(64, (len(members) * st["max_selector_target"] + n - 1) // n), len(members))
+        if mc >= len(members):
+            continue
+        clusters += [members[c] for c in oracle.vq_threaded_clusters(sv[members], sw[members], mc)]
+    assert len(clusters) == st["selector_clusters"]
+    exp = mid.copy()
+    px = blocks.reshape(-1, 16, 4)[:, :, comp].astype(np.int64)
+    for c in clusters:
+        if len(c) <= 1:
+            continue
+        lo = (mid[c] & np.uint64(255)).astype(np.int64)
+        hi = ((mid[c] >> np.uint64(8)) & np.uint64(255)).astype(np.int64)
+        for members in (c[lo > hi], c[lo <= hi]):
+            if len(members) <= 1:
+                continue
+            vals = np.stack([_values(mid[b] & np.uint64(255), (mid[b] >> np.uint64(8)) & np.uint64(255))
+                             for b in members])
+            err = ((px[members][:, :, None] - vals[:, None, :]) ** 2).sum(0)   # [16, 8]
+            best = np.argmin(err, axis=1)
+            word = sum(int(best[t]) << (3 * t) for t in range(16))
+            for b in members:
+                exp[b] = (exp[b] & np.uint64(0xFFFF)) | np.uint64(word << 16)
+    assert (out == exp).all()
+
+
+def test_qdxt5_training_tile_choice():
+    img = texture.synth(48, 40, 9, texture.RGBA)
+    levels = texture.mip_chain(img)[:3]
+    blocks, desc = level_blocks(levels)
+    vecs, w, enc = oracle.qdxt5_train(blocks, desc, 3)
+    for lvl, im in enumerate(levels):
+        first, bw, bh = desc[3 * lvl:3 * lvl + 3]
+        ys = np.minimum(np.arange(bh * 4), im.shape[0] - 1)
+        xs = np.minimum(np.arange(bw * 4), im.shape[1] - 1)
+        pad = im[ys][:, xs, 3].astype(np.int64)
+        der = 2.4 if lvl == 0 else max(0.25, 2.4 / 3.0 ** lvl)
+        for cy in range((bh + 1) // 2):
+            for cx in range((bw + 1) // 2):
+                yy = np.minimum(np.arange(cy * 8, cy * 8 + 8), bh * 4 - 1)
+                xx = np.minimum(np.arange(cx * 8, cx * 8 + 8), bw * 4 - 1)
+                chunk = pad[yy][:, xx]
+                errs = []
+                for x0, y0, lw, lh in LAYOUTS:
+                    t = chunk[y0:y0 + lh, x0:x0 + lw].reshape(-1)
+                    # min / max endpoints: the nearest of the 8 (or 6) values
+                    # reached by dxt_fast's selector arithmetic
+                    lo, hi = t.max(), t.min()
+                    vals = _values(lo, hi)
+                    if lo == hi:
+                        errs.append(0)
+                        continue
+                    errs.append(int(((t[:, None] - vals[None, :]) ** 2).min(1).sum()))
+                best, best_e = -1.0, 0
+                for e, tiles in enumerate(ENCODINGS):
+                    ms = sum(errs[t] for t in tiles) / 64.0
+                    ps = 999999.0 if not ms else min(500.0, max(0.0, 20 * np.log10(255.0 / np.sqrt(ms))))
+                    ps -= der * (len(tiles) - 1) / 3.0
+                    if ps > best + 1e-9:
+                        best, best_e = ps, e
+                b0 = first + min(cx * 2, bw - 1) + min(cy * 2, bh - 1) * bw
+                assert enc[b0] == best_e, (lvl, cy, cx, errs)
