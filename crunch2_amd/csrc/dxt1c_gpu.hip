@@ -2,11 +2,14 @@
 // wave per cluster (the dxt_hc endpoint-cluster case).
 //
 // Reference: dxt1_endpoint_optimizer::compute (crnlib/crn_dxt1.cpp:2081-2246)
-// on the path dxt_hc uses for its endpoint clusters
-// (determine_color_endpoint_codebook_task, crn_dxt_hc.cpp:663-754): uber
-// quality, no 3-colour blocks, so the hc evaluators
-// (evaluate_solution_hc_*, compute_selectors_hc, crn_dxt1.cpp:1759-1869) judge
-// every candidate.  Unique colours (first-appearance order), the error planes,
+// at uber quality.  Without 3-colour blocks (the path dxt_hc uses for its
+// endpoint clusters, determine_color_endpoint_codebook_task,
+// crn_dxt_hc.cpp:663-754) the hc evaluators (evaluate_solution_hc_*,
+// compute_selectors_hc, crn_dxt1.cpp:1759-1869) judge every candidate; with
+// them (qdxt1's clusters, crn_qdxt1.cpp:616-626) evaluate_solution_uber
+// (:1370-1592) scores both block types, and the solid-block tries, the
+// one-colour shortcut and return_solution take their 3-colour branches
+// (:119, :143, :2220, :263-365).  Unique colours (first-appearance order), the error planes,
 // PCA, the probe sweeps, refine_solution, try_median4,
 // optimize_endpoint_comps and the special cases follow crn_dxt1.cpp as in the
 // per-block kernel (dxt1_gpu.hip).
@@ -42,7 +45,7 @@ __device__ __constant__ int8_t c_adj[26][3] = {
     {1, 1, -1}, {-1, -1, 0}, {0, -1, 0}, {1, -1, 0}, {-1, 0, 0}, {1, 0, 0}, {-1, 1, 0}, {0, 1, 0},
     {1, 1, 0}, {-1, -1, 1}, {0, -1, 1}, {1, -1, 1}, {-1, 0, 1}, {0, 0, 1}, {1, 0, 1}, {-1, 1, 1},
     {0, 1, 1}, {1, 1, 1}};
-__device__ uint8_t g_c_om5[256][2], g_c_om6[256][2];
+__device__ uint8_t g_c_om5[256][2], g_c_om6[256][2], g_c_om5_3[256][2], g_c_om6_3[256][2];
 
 template <uint32_t MaxU>
 struct LdsT {
@@ -79,10 +82,17 @@ struct LdsS {
     uint8_t pad[kSort > kSearch ? kSort - kSearch : 4];
 };
 
+// A colour store tagged with the evaluator: kUber -- evaluate_solution_uber
+// with both block types (3-colour blocks allowed); else the hc evaluators.
+template <class Base, bool U>
+struct LdsU : Base {
+    static constexpr bool kUber = U;
+};
+
 struct Best {
     uint32_t lo, hi;
     uint64_t err;
-    int alt, enforce, enforced;
+    int alt, enforce, enforced, alpha;
 };
 
 struct St {
@@ -176,6 +186,19 @@ __device__ inline Pal palette4(uint32_t lo, uint32_t hi, int alt)
     return p;
 }
 
+// get_block_colors3 with the alternate rounding (crn_dxt1.cpp:1432-1440)
+__device__ inline Pal palette3(uint32_t lo, uint32_t hi, int alt)
+{
+    Pal p;
+    unpack565(lo, 1, p.r[0], p.g[0], p.b[0]);
+    unpack565(hi, 1, p.r[1], p.g[1], p.b[1]);
+    p.r[2] = (p.r[0] + p.r[1] + alt) >> 1;
+    p.g[2] = (p.g[0] + p.g[1] + alt) >> 1;
+    p.b[2] = (p.b[0] + p.b[1] + alt) >> 1;
+    p.r[3] = p.g[3] = p.b[3] = 0;
+    return p;
+}
+
 // evaluate_solution_hc (crn_dxt1.cpp:1759-1835): sum over colours of the
 // weighted distance to the nearest of the four colours.
 template <class Lds>
@@ -211,6 +234,63 @@ __device__ bool eval_hc(St& s, const Lds& L, uint32_t lo, uint32_t hi, int alt)
     return true;
 }
 
+// evaluate_solution_uber (crn_dxt1.cpp:1370-1592), both block types: the
+// 4-colour error, then (alpha blocks allowed) the 3-colour error, each
+// kept when below the best so far (exact integer sums over the colours);
+// a 4-colour pair with equal endpoints enforces one selector.  The selectors
+// of the kept type are recomputed on demand (compute_selectors).
+template <class Lds>
+__device__ bool eval_uber(St& s, const Lds& L, uint32_t lo, uint32_t hi, int alt)
+{
+    const Pal p4 = palette4(lo, hi, alt), p3 = palette3(lo, hi, alt);
+    uint64_t e4 = 0, e3 = 0;
+    for (uint32_t i = lane_id(); i < s.nu; i += 64) {
+        const uint32_t px = L.col[i];
+        const uint32_t d0 = cdist(s.perceptual, px, p4.r[0], p4.g[0], p4.b[0]);
+        const uint32_t d1 = cdist(s.perceptual, px, p4.r[1], p4.g[1], p4.b[1]);
+        const uint32_t d2 = cdist(s.perceptual, px, p4.r[2], p4.g[2], p4.b[2]);
+        const uint32_t d3 = cdist(s.perceptual, px, p4.r[3], p4.g[3], p4.b[3]);
+        e4 += (uint64_t)min(min(d0, d1), min(d2, d3)) * L.w[i];
+        {
+            const uint32_t t2 = cdist(s.perceptual, px, p3.r[2], p3.g[2], p3.b[2]);
+            e3 += (uint64_t)min(min(d0, d1), t2) * L.w[i];
+        }
+    }
+    e4 = wave_sum(e4);
+    e3 = wave_sum(e3);
+    uint64_t terr = s.best.err;
+    int talpha = 0;
+    bool took = false;
+    if (e4 < terr) {
+        terr = e4;
+        took = true;
+    }
+    if (e3 < terr) {
+        terr = e3;
+        talpha = 1;
+        took = true;
+    }
+    if (!took || terr >= s.best.err) return false;
+    s.best.lo = lo;
+    s.best.hi = hi;
+    s.best.err = terr;
+    s.best.alpha = talpha;
+    s.best.alt = alt;
+    s.best.enforce = !talpha && lo == hi;
+    if (s.best.enforce) {
+        if ((s.best.lo & 31) != 31) {
+            s.best.lo++;
+            s.best.enforced = 1;
+        } else {
+            s.best.hi--;
+            s.best.enforced = 0;
+        }
+    }
+    s.sel_valid = false;
+    s.sel_nonempty = true;
+    return true;
+}
+
 // compute_selectors_hc (crn_dxt1.cpp:1837-1869)
 template <class Lds>
 __device__ void compute_selectors(St& s, Lds& L)
@@ -218,6 +298,16 @@ __device__ void compute_selectors(St& s, Lds& L)
     if (s.sel_valid) return;
     if (s.best.enforce) {
         for (uint32_t i = lane_id(); i < s.nu; i += 64) L.sel[i] = (uint8_t)s.best.enforced;
+    } else if (Lds::kUber && s.best.alpha) {
+        // the 3-colour palette's first nearest entry (evaluate_solution_uber)
+        const Pal p = palette3(s.best.lo, s.best.hi, s.best.alt);
+        for (uint32_t i = lane_id(); i < s.nu; i += 64) {
+            const uint32_t px = L.col[i];
+            const uint32_t e0 = cdist(s.perceptual, px, p.r[0], p.g[0], p.b[0]);
+            const uint32_t e1 = cdist(s.perceptual, px, p.r[1], p.g[1], p.b[1]);
+            const uint32_t e2 = cdist(s.perceptual, px, p.r[2], p.g[2], p.b[2]);
+            L.sel[i] = (uint8_t)(e1 < e0 ? (e2 < e1 ? 2 : 1) : (e2 < e0 ? 2 : 0));
+        }
     } else {
         const Pal p = palette4(s.best.lo, s.best.hi, s.best.alt);
         for (uint32_t i = lane_id(); i < s.nu; i += 64) {
@@ -247,7 +337,8 @@ __device__ bool evaluate(St& s, Lds& L, uint32_t lo, uint32_t hi, int alt)
     const uint64_t ge = ag < bg ? L.glo[ag] + L.ghi[bg] : L.ghi[ag] + L.glo[bg];
     const uint64_t be = ab < bb ? L.blo[ab] + L.bhi[bb] : L.bhi[ab] + L.blo[bb];
     if (re + ge + be >= s.best.err) return false;
-    return eval_hc(s, L, lo, hi, alt);
+    if constexpr (Lds::kUber) return eval_uber(s, L, lo, hi, alt);
+    else return eval_hc(s, L, lo, hi, alt);
 }
 
 template <class Lds>
@@ -592,12 +683,18 @@ __device__ bool try_average_solid(St& s, Lds& L)
     const uint32_t ar = (uint32_t)((tr + half) / tw), ag = (uint32_t)((tg + half) / tw), ab = (uint32_t)((tb + half) / tw);
     bool improved = evaluate(s, L, (g_c_om5[ar][0] << 11) | (g_c_om6[ag][0] << 5) | g_c_om5[ab][0],
                              (g_c_om5[ar][1] << 11) | (g_c_om6[ag][1] << 5) | g_c_om5[ab][1], 0);
+    if (Lds::kUber && s.best.err)
+        improved |= evaluate(s, L, (g_c_om5_3[ar][0] << 11) | (g_c_om6_3[ag][0] << 5) | g_c_om5_3[ab][0],
+                             (g_c_om5_3[ar][1] << 11) | (g_c_om6_3[ag][1] << 5) | g_c_om5_3[ab][1], 0);
     for (uint32_t i = 0; i < s.nu; i++) {
         const uint32_t c = L.col[i];
         const uint32_t r = c & 255u, g = (c >> 8) & 255u, b = (c >> 16) & 255u;
         if (r == ar && g == ag && b == ab) continue;
         improved |= evaluate(s, L, (g_c_om5[r][0] << 11) | (g_c_om6[g][0] << 5) | g_c_om5[b][0],
                              (g_c_om5[r][1] << 11) | (g_c_om6[g][1] << 5) | g_c_om5[b][1], 0);
+        if (Lds::kUber && s.best.err)
+            improved |= evaluate(s, L, (g_c_om5_3[r][0] << 11) | (g_c_om6_3[g][0] << 5) | g_c_om5_3[b][0],
+                                 (g_c_om5_3[r][1] << 11) | (g_c_om6_3[g][1] << 5) | g_c_om5_3[b][1], 0);
     }
     return improved;
 }
@@ -657,7 +754,7 @@ template <class Lds>
 __device__ void optimize_comps(St& s, Lds& L)
 {
     compute_selectors(s, L);
-    if (!s.best.err) return;
+    if (s.best.alpha || !s.best.err) return;
     int slo[3], shi[3];
     unpack565(s.best.lo, 1, slo[0], slo[1], slo[2]);
     unpack565(s.best.hi, 1, shi[0], shi[1], shi[2]);
@@ -858,6 +955,8 @@ __device__ void optimize_endpoints(St& s, Lds& L, float* lo, float* hi)
     }
     if (s.best.err) {
         bool solid = false;
+        // the uber evaluator keeps the best solution's selectors current
+        if (Lds::kUber && s.sel_nonempty) compute_selectors(s, L);
         if (sel_all_equal(s, L)) solid = try_average_solid(s, L);
         if (!solid) optimize_comps(s, L);
     }
@@ -1153,7 +1252,7 @@ __device__ void cluster_search(St& s, Lds& L, const uint32_t* __restrict__ px, u
     __syncthreads();
     s.best.lo = s.best.hi = 0;
     s.best.err = ~0ull;
-    s.best.alt = s.best.enforce = s.best.enforced = 0;
+    s.best.alt = s.best.enforce = s.best.enforced = s.best.alpha = 0;
     s.sel_valid = false;
     s.sel_nonempty = false;
     if (s.nu == 1) {
@@ -1161,18 +1260,23 @@ __device__ void cluster_search(St& s, Lds& L, const uint32_t* __restrict__ px, u
         const uint32_t r = c & 255u, g = (c >> 8) & 255u, b = (c >> 16) & 255u;
         evaluate(s, L, (g_c_om5[r][0] << 11) | (g_c_om6[g][0] << 5) | g_c_om5[b][0],
                  (g_c_om5[r][1] << 11) | (g_c_om6[g][1] << 5) | g_c_om5[b][1], 0);
+        if (Lds::kUber && s.best.err)
+            evaluate(s, L, (g_c_om5_3[r][0] << 11) | (g_c_om6_3[g][0] << 5) | g_c_om5_3[b][0],
+                     (g_c_om5_3[r][1] << 11) | (g_c_om6_3[g][1] << 5) | g_c_om5_3[b][1], 0);
     } else {
         handle_multicolor(s, L);
     }
     compute_selectors(s, L);
-    // return_solution (crn_dxt1.cpp:263-365): four-colour blocks keep low > high
-    const bool inv = s.best.lo < s.best.hi;
+    // return_solution (crn_dxt1.cpp:263-365): four-colour blocks keep low >
+    // high (selectors through {1, 0, 3, 2}), 3-colour blocks low <= high
+    // (through {1, 0, 2, 3})
+    const bool inv = s.best.alpha ? s.best.lo > s.best.hi : s.best.lo < s.best.hi;
     const uint32_t lo = inv ? s.best.hi : s.best.lo, hi = inv ? s.best.lo : s.best.hi;
     if (selectors) {
         for (uint32_t i = lane; i < n; i += 64) {
             const uint32_t key = (px[p0 + i] & 0x00FFFFFFu) | 0xFF000000u;
             uint32_t v = L.sel[unique_index(L, i, key)];
-            if (inv) v ^= 1u;
+            if (inv) v = s.best.alpha ? (v < 2 ? v ^ 1u : v) : (v ^ 1u);
             selectors[p0 + i] = (uint8_t)v;
         }
     }
@@ -1180,13 +1284,13 @@ __device__ void cluster_search(St& s, Lds& L, const uint32_t* __restrict__ px, u
         o.error = s.best.err;
         o.lo_hi = lo | (hi << 16);
         o.nunique = s.nu;
-        o.flags = (uint32_t)inv | ((uint32_t)s.best.alt << 2) | ((uint32_t)s.best.enforce << 3) |
-                  ((uint32_t)s.best.enforced << 4);
+        o.flags = (uint32_t)inv | ((uint32_t)s.best.alpha << 1) | ((uint32_t)s.best.alt << 2) |
+                  ((uint32_t)s.best.enforce << 3) | ((uint32_t)s.best.enforced << 4);
         out[cl] = o;
     }
 }
 
-template <uint32_t MaxU>
+template <uint32_t MaxU, bool Uber>
 __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __restrict__ px,
                                                           const uint32_t* __restrict__ ofs,
                                                           const uint32_t* __restrict__ list,
@@ -1194,7 +1298,7 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
                                                           ClusterOut* __restrict__ out,
                                                           uint8_t* __restrict__ selectors)
 {
-    typedef std::conditional_t<(MaxU < kMaxU), LdsS<MaxU>, LdsT<MaxU>> Lds;
+    typedef LdsU<std::conditional_t<(MaxU < kMaxU), LdsS<MaxU>, LdsT<MaxU>>, Uber> Lds;
     extern __shared__ __align__(16) uint8_t lds_raw[];
     Lds& L = *reinterpret_cast<Lds*>(lds_raw);
     if (blockIdx.x >= *count) return;
@@ -1210,12 +1314,12 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
     if constexpr (MaxU < kMaxU) {
         // this class holds at most MaxU texels (dxt1_cluster_classes_kernel)
         St s;
-        s.perceptual = perceptual;
+        s.perceptual = perceptual & 1;
         s.total_w = n;
         s.nu = unique_sorted(L, px, p0, n);
         for (uint32_t i = lane; i < 3 * 64 * 3; i += 64) (&L.pl[0][0][0])[i] = 0;   // over the sort buffers
         __syncthreads();
-        cluster_search(s, L, px, p0, n, perceptual, cl, out, selectors);
+        cluster_search(s, L, px, p0, n, perceptual & 1, cl, out, selectors);
         return;
     }
     else {
@@ -1225,7 +1329,7 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
     __builtin_amdgcn_wave_barrier();
     // unique colours in first-appearance order (one lane; the order fixes the float sums)
     St s;
-    s.perceptual = perceptual;
+    s.perceptual = perceptual & 1;
     s.nu = 0;
     s.total_w = n;
     bool overflow = false;
@@ -1257,7 +1361,7 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
         if (lane == 0) out[cl] = o;
         return;
     }
-    cluster_search(s, L, px, p0, n, perceptual, cl, out, selectors);
+    cluster_search(s, L, px, p0, n, perceptual & 1, cl, out, selectors);
     }
 }
 
@@ -1268,6 +1372,7 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
 // its colour with compare-and-swap and keeps the smallest texel index and
 // the count per slot, the first texel of each colour is marked, and a prefix
 // count over the marks gives each colour its rank.
+template <bool Uber>
 __global__ void __launch_bounds__(64) dxt1_cluster_big_kernel(const uint32_t* __restrict__ px,
                                                               const uint32_t* __restrict__ ofs, uint32_t nclusters,
                                                               int perceptual, ClusterOut* __restrict__ out,
@@ -1279,7 +1384,7 @@ __global__ void __launch_bounds__(64) dxt1_cluster_big_kernel(const uint32_t* __
     if (cl >= nclusters || !(out[cl].flags & kNotEncoded)) return;
     const uint32_t lane = lane_id();
     const uint32_t p0 = ofs[cl], p1 = ofs[cl + 1], n = p1 - p0, total = ofs[nclusters];
-    GStore L;
+    LdsU<GStore, Uber> L;
     L.kCapU = n;
     L.kCapH = 2 * n;
     // scratch layout, 37 bytes per texel of all clusters (`total`): u32
@@ -1348,10 +1453,10 @@ __global__ void __launch_bounds__(64) dxt1_cluster_big_kernel(const uint32_t* __
     __threadfence();
     __syncthreads();
     St s;
-    s.perceptual = perceptual;
+    s.perceptual = perceptual & 1;
     s.nu = base_rank;
     s.total_w = n;
-    cluster_search(s, L, px, p0, n, perceptual, cl, out, selectors);
+    cluster_search(s, L, px, p0, n, perceptual & 1, cl, out, selectors);
 }
 
 }  // namespace
@@ -1412,6 +1517,21 @@ uint32_t* g_noalloc_scratch[kMaxDevices];
 // Per-device set-up: the single-colour tables in device memory, the kernels'
 // LDS limits and the device address of `scratch`.  Idempotent; runs on the
 // device that is current.
+template <bool Uber>
+int set_lds()
+{
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<512, Uber>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsS<512>)) != hipSuccess ||
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<1024, Uber>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsS<1024>)) != hipSuccess ||
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<2048, Uber>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsS<2048>)) != hipSuccess ||
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<kMaxU, Uber>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<kMaxU>)) != hipSuccess)
+        return CRN2_EHIP;
+    return CRN2_OK;
+}
+
 int clusters_init(int* dev_out)
 {
     int dev = 0;
@@ -1419,20 +1539,22 @@ int clusters_init(int* dev_out)
     *dev_out = dev;
     std::lock_guard<std::mutex> lk(g_init_mu);
     if (g_init[dev]) return CRN2_OK;
-    // ryg single-colour tables (crn_ryg_dxt.cpp:58-99, 517-535), four-colour
-    static uint8_t t5[256][2], t6[256][2];
+    // ryg single-colour tables (crn_ryg_dxt.cpp:58-99, 517-535), four- and
+    // three-colour
+    static uint8_t t5[256][2], t6[256][2], t5_3[256][2], t6_3[256][2];
     static bool tables = false;
     if (!tables) {
-        for (int which = 0; which < 2; which++) {
-            const int size = which ? 64 : 32;
-            uint8_t (*t)[2] = which ? t6 : t5;
+        for (int which = 0; which < 4; which++) {
+            const int size = (which & 1) ? 64 : 32;
+            const bool three = which >= 2;
+            uint8_t (*t)[2] = which == 0 ? t5 : which == 1 ? t6 : which == 2 ? t5_3 : t6_3;
             for (int i = 0; i < 256; i++) {
                 int best = 256;
                 for (int mn = 0; mn < size; mn++)
                     for (int mx = 0; mx < size; mx++) {
                         const int me = size == 32 ? ((mn << 3) | (mn >> 2)) : ((mn << 2) | (mn >> 4));
                         const int xe = size == 32 ? ((mx << 3) | (mx >> 2)) : ((mx << 2) | (mx >> 4));
-                        int err = abs(((xe * 2 + me) / 3) - i);
+                        int err = three ? abs(((me + xe) >> 1) - i) : abs(((xe * 2 + me) / 3) - i);
                         err += (abs(xe - me) * 8) >> 8;
                         if (err < best) {
                             t[i][0] = (uint8_t)mx;
@@ -1446,20 +1568,28 @@ int clusters_init(int* dev_out)
     }
     if (hipMemcpyToSymbol(HIP_SYMBOL(g_c_om5), t5, 512) != hipSuccess) return CRN2_EHIP;
     if (hipMemcpyToSymbol(HIP_SYMBOL(g_c_om6), t6, 512) != hipSuccess) return CRN2_EHIP;
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<512>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsS<512>)) != hipSuccess ||
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<1024>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsS<1024>)) != hipSuccess ||
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<2048>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsS<2048>)) != hipSuccess ||
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&dxt1_cluster_kernel<kMaxU>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(LdsT<kMaxU>)) != hipSuccess)
-        return CRN2_EHIP;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_c_om5_3), t5_3, 512) != hipSuccess) return CRN2_EHIP;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_c_om6_3), t6_3, 512) != hipSuccess) return CRN2_EHIP;
+    if (set_lds<false>() != CRN2_OK || set_lds<true>() != CRN2_OK) return CRN2_EHIP;
     void* p = nullptr;
     if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CRN2_EHIP;
     g_noalloc_scratch[dev] = static_cast<uint32_t*>(p);
     g_init[dev] = true;
     return CRN2_OK;
+}
+
+template <bool Uber>
+void launch_classes(const uint32_t* px, const uint32_t* offsets, uint32_t nclusters, const uint32_t* list,
+                    const uint32_t* work, int perceptual, ClusterOut* o, uint8_t* selectors, const hipStream_t* cls)
+{
+    hipLaunchKernelGGL((dxt1_cluster_kernel<kMaxU, Uber>), dim3(nclusters), dim3(64), sizeof(LdsT<kMaxU>), cls[3], px,
+                       offsets, list + 3 * (size_t)nclusters, work + 3, perceptual, o, selectors);
+    hipLaunchKernelGGL((dxt1_cluster_kernel<2048, Uber>), dim3(nclusters), dim3(64), sizeof(LdsS<2048>), cls[2], px,
+                       offsets, list + 2 * (size_t)nclusters, work + 2, perceptual, o, selectors);
+    hipLaunchKernelGGL((dxt1_cluster_kernel<1024, Uber>), dim3(nclusters), dim3(64), sizeof(LdsS<1024>), cls[1], px,
+                       offsets, list + (size_t)nclusters, work + 1, perceptual, o, selectors);
+    hipLaunchKernelGGL((dxt1_cluster_kernel<512, Uber>), dim3(nclusters), dim3(64), sizeof(LdsS<512>), cls[0], px,
+                       offsets, list, work, perceptual, o, selectors);
 }
 
 // The search proper.  work: 4 class counters + 4 lists of nclusters entries
@@ -1494,14 +1624,10 @@ int clusters_run(const void* rgba, const uint32_t* offsets, uint32_t nclusters, 
         cls[2] = side[1];
         cls[1] = side[2];
     }
-    hipLaunchKernelGGL(dxt1_cluster_kernel<kMaxU>, dim3(nclusters), dim3(64), sizeof(LdsT<kMaxU>), cls[3], px,
-                       offsets, list + 3 * (size_t)nclusters, work + 3, flags & 1, o, selectors);
-    hipLaunchKernelGGL(dxt1_cluster_kernel<2048>, dim3(nclusters), dim3(64), sizeof(LdsS<2048>), cls[2], px,
-                       offsets, list + 2 * (size_t)nclusters, work + 2, flags & 1, o, selectors);
-    hipLaunchKernelGGL(dxt1_cluster_kernel<1024>, dim3(nclusters), dim3(64), sizeof(LdsS<1024>), cls[1], px,
-                       offsets, list + (size_t)nclusters, work + 1, flags & 1, o, selectors);
-    hipLaunchKernelGGL(dxt1_cluster_kernel<512>, dim3(nclusters), dim3(64), sizeof(LdsS<512>), cls[0], px, offsets,
-                       list, work, flags & 1, o, selectors);
+    if (flags & 2)
+        launch_classes<true>(px, offsets, nclusters, list, work, flags & 1, o, selectors, cls);
+    else
+        launch_classes<false>(px, offsets, nclusters, list, work, flags & 1, o, selectors, cls);
     if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
     if (fork)
         for (int i = 0; i < 3; i++)
@@ -1524,9 +1650,14 @@ int clusters_big_pass(const void* rgba, const uint32_t* offsets, uint32_t nclust
     if (total <= kMaxU) return CRN2_OK;      // no cluster can hold more colours than the LDS store
     uint8_t* big = nullptr;
     if (hipMallocAsync((void**)&big, 37 * (size_t)total, st) != hipSuccess) return CRN2_ENOMEM;
-    hipLaunchKernelGGL(dxt1_cluster_big_kernel, dim3(nclusters), dim3(64), 0, st,
-                       static_cast<const uint32_t*>(rgba), offsets, nclusters, flags & 1,
-                       static_cast<ClusterOut*>(out), selectors, big);
+    if (flags & 2)
+        hipLaunchKernelGGL(dxt1_cluster_big_kernel<true>, dim3(nclusters), dim3(64), 0, st,
+                           static_cast<const uint32_t*>(rgba), offsets, nclusters, flags & 1,
+                           static_cast<ClusterOut*>(out), selectors, big);
+    else
+        hipLaunchKernelGGL(dxt1_cluster_big_kernel<false>, dim3(nclusters), dim3(64), 0, st,
+                           static_cast<const uint32_t*>(rgba), offsets, nclusters, flags & 1,
+                           static_cast<ClusterOut*>(out), selectors, big);
     const int rc = hipGetLastError() == hipSuccess ? CRN2_OK : CRN2_EHIP;
     (void)hipFreeAsync(big, st);
     return rc;
@@ -1536,7 +1667,7 @@ int clusters_big_pass(const void* rgba, const uint32_t* offsets, uint32_t nclust
 extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags,
                                          void* out, uint8_t* selectors, void* stream)
 {
-    if (!rgba || !offsets || !out || (flags & ~1)) return CRN2_EINVAL;
+    if (!rgba || !offsets || !out || (flags & ~3)) return CRN2_EINVAL;
     if (!nclusters) return CRN2_OK;
     int dev = 0;
     int rc = clusters_init(&dev);
@@ -1554,7 +1685,7 @@ extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offse
 extern "C" int crn2_dxt1_clusters_noalloc_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters,
                                                  int flags, void* out, uint8_t* selectors, void* stream)
 {
-    if (!rgba || !offsets || !out || (flags & ~1) || nclusters > kNoAllocMaxClusters) return CRN2_EINVAL;
+    if (!rgba || !offsets || !out || (flags & ~3) || nclusters > kNoAllocMaxClusters) return CRN2_EINVAL;
     if (!nclusters) return CRN2_OK;
     int dev = 0;
     const int rc = clusters_init(&dev);

@@ -270,9 +270,11 @@ int crn2_decompress_blocks(const void* blocks, uint32_t nblocks, uint32_t format
  * dxt1_endpoint_optimizer::compute with uber quality and no 3-colour blocks;
  * crn_dxt1.cpp:1759-1869 evaluators), one wave per cluster.  rgba: all
  * clusters' texels (RGBA8 words, alpha ignored), cluster c owning texels
- * [offsets[c], offsets[c+1]); flags bit0 perceptual.  out: 24 bytes per
+ * [offsets[c], offsets[c+1]); flags bit0 perceptual, bit1 m_use_alpha_blocks
+ * (the uber evaluator with 3-colour blocks, crn_dxt1.cpp:1370-1592, as qdxt1
+ * runs it, crn_qdxt1.cpp:616-626).  out: 24 bytes per
  * cluster = {u64 m_error, u32 m_low_color | m_high_color << 16, u32 unique
- * colours, u32 m_reordered | m_alternate_rounding << 2 | m_enforce_selector
+ * colours, u32 m_reordered | m_alpha_block << 1 | m_alternate_rounding << 2 | m_enforce_selector
  * << 3 | m_enforced_selector << 4 | 256 when the cluster has more than 4096
  * unique colours (not encoded), u32 0}; selectors (optional): the 2-bit
  * DXT1 code of every texel.  Allocates stream-ordered scratch (not for graph
@@ -673,9 +675,9 @@ int crn2_mip_gamma_tables(float gamma, float* srgb_to_linear, uint8_t* linear_to
  * words; pack (qdxt1::pack, :876-1023) writes one 8-byte dxt1_block per
  * block (low colour, high colour, selector bytes) to out_blocks (device).
  * Endpoint caching is off in the endpoint search.  Supported: quality 255
- * at any dxt quality but superfast; below 255, the uber search without
- * 3-colour blocks (flags bit1 clear: DXT1 without both block types, and the
- * colour part of DXT5) -- CRN2_EUNSUPPORTED otherwise. */
+ * at any dxt quality but superfast; below 255, uber quality with or without
+ * 3-colour blocks, except DXT1A transparency (alpha blocks allowed and a
+ * texel below the alpha threshold) -- CRN2_EUNSUPPORTED otherwise. */
 #define CRN2_QDXT_PERCEPTUAL 1
 #define CRN2_QDXT_ALPHA_BLOCKS 2   /* qdxt1_params::m_use_alpha_blocks */
 #define CRN2_QDXT_HIERARCHICAL 4   /* qdxt1_params::m_hierarchical */

@@ -86,3 +86,38 @@ def test_clusters_beyond_the_lds_store(gpu, perceptual):
         assert int(r["error"][c]) == ores.error, c
         assert (int(r["lo_hi"][c] & 0xFFFF), int(r["lo_hi"][c] >> 16)) == (ores.low, ores.high), c
         assert np.array_equal(sel[ofs[c]:ofs[c + 1]], osel), c
+
+
+@pytest.mark.parametrize("perceptual", [True, False])
+def test_clusters_with_three_colour_blocks_match_oracle(gpu, perceptual):
+    # qdxt1's endpoint clusters: the uber evaluator with both block types
+    # (crn_dxt1.cpp:1370-1592), solid tries and return_solution with their
+    # 3-colour branches; every size class and the global-memory pass
+    rng = np.random.default_rng(21)
+    sizes = [1, 2, 3, 16, 17, 64, 200] + list(rng.integers(16, 2048, 30)) + [4096, 5000]
+    parts = clusters(40 + perceptual, sizes)
+    parts.append(np.repeat(parts[0][:1], 300, axis=0))
+    parts.append(rng.integers(0, 256, (6000, 4), dtype=np.uint8))
+    two = np.zeros((64, 4), np.uint8)
+    two[:32, :3], two[32:, :3] = (10, 200, 30), (200, 20, 240)
+    parts.append(two)
+    px = np.concatenate(parts)
+    ofs = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.int32)
+    print("clusters", len(parts), "texels", len(px), flush=True)
+    raw, sel = dxt1.encode_clusters_device(torch.from_numpy(px).cuda(), torch.from_numpy(ofs).cuda(), perceptual,
+                                           alpha_blocks=True)
+    torch.cuda.synchronize()
+    r = raw.cpu().numpy().view(dxt1.CLUSTER_DTYPE).reshape(-1)
+    sel = sel.cpu().numpy()
+    flags = (oracle.DXT1_PERCEPTUAL if perceptual else 0) | oracle.DXT1_ALPHA_BLOCKS
+    alpha_blocks = 0
+    for c, p in enumerate(parts):
+        osel, ores = oracle.dxt1_cluster(p, 4, flags)
+        lo, hi = int(r["lo_hi"][c] & 0xFFFF), int(r["lo_hi"][c] >> 16)
+        s = sel[ofs[c]:ofs[c + 1]]
+        assert int(r["error"][c]) == ores.error, (c, len(p))
+        assert (lo, hi) == (ores.low, ores.high), c
+        assert np.array_equal(s, osel), c
+        assert ((int(r["flags"][c]) >> 1) & 1) == ores.alpha_block, c
+        alpha_blocks += ores.alpha_block
+    print("3-colour clusters", alpha_blocks, "of", len(parts), flush=True)

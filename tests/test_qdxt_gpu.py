@@ -4,7 +4,8 @@ threaded_clusterizer, bit for bit: the clusters of random 2-, 6- and 16-D
 vectors at several leaf budgets and cuts, the threaded selector clusters,
 the training vectors (hierarchical tiles over mip chains, odd level sizes,
 and per block), and whole packs at several qualities, codebook-curve factors
-and textures, up to a 512 x 512 mip chain."""
+and textures, with and without 3-colour blocks, up to a 512 x 512 mip chain;
+and qdxt5 the same way."""
 import ctypes as C
 
 import numpy as np
@@ -80,6 +81,9 @@ CASES = [
     (128, 16, texture.RGB, 128, P, 1.0),
     (128, 17, texture.RGBA, 255, P | A | H, 1.0),
     (64, 18, texture.RGB, 255, H, 1.0),
+    (128, 19, texture.RGB, 128, P | A | H, 1.0),
+    (128, 20, texture.RGB, 60, A | H, 1.0),
+    (96, 22, texture.NORMAL, 200, P | A, 1.0),
 ]
 
 
@@ -112,12 +116,17 @@ def test_pack_512_chain_and_repacks(gpu):
 
 
 def test_unsupported_configurations(gpu):
-    blocks, desc = texture_blocks(32, 3)
+    # DXT1A transparency inside a cluster, qualities below uber under 255,
+    # superfast
+    blocks, desc = texture_blocks(32, 3, texture.RGBA)
+    assert (blocks.reshape(-1, 4)[:, 3] < 128).any()
     q = qdxt.Qdxt1(blocks, desc)
     out = torch.zeros(len(blocks), dtype=torch.int64, device="cuda")
     s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     for p in (qdxt.params(128, 4, P | A | H), qdxt.params(128, 3, P | H), qdxt.params(255, 0, P | H)):
         assert lib().crn2_qdxt1_pack(q.h, C.byref(p), 1.0, C.c_void_p(out.data_ptr()), s) == CRN2_EUNSUPPORTED
+    # without alpha blocks the threshold is off, and alpha below it is fine
+    q.pack(qdxt.params(128, 4, P | H))
     q.close()
 
 
