@@ -411,31 +411,78 @@ __device__ inline void load_vec(const VqArgs& A, uint32_t vi, float (&v)[D])
     for (int c = 0; c < D; c++) v[c] = A.v[(size_t)vi * D + c];
 }
 
-// Sequential float sums over a list in order: sum over the vectors j with
-// side(j) == s of v_j * (float)w_j, for s = 0 and 1, per component.  Lane
-// k < 2D holds (side k / D, component k % D).  gen(j, v, w, side) gives the
-// j-th vector of the list and its side (2: neither).
-template <int D, class Gen>
-__device__ float seq_side_sums(VqLds<D>& L, uint32_t n, Gen gen)
+// Per-lane partial statistics of one pass (reduced by the caller).
+struct PassStats {
+    unsigned long long w[2];   // weights per side
+    double tt[2];              // sums of w |v|^2 per side (integers: exact in any order)
+    uint32_t cnt[2];           // vectors per side
+};
+
+// One pass over a node's list in order, 64 vectors per batch: every lane
+// takes one vector, decides its side (side_of: 0 or 1) and stages v * w in
+// the array of its side (0 in the other); lane k < 2D then adds the batch's
+// terms of (side k / D, component k % D) one after another, so the float sums
+// keep the list order.  The next batch's vectors are loaded while the
+// current one is summed.  Returns the sum of lane k's (side, component).
+template <int D, class SideF>
+__device__ float side_pass(VqLds<D>& L, const VqArgs& A, const uint32_t* __restrict__ list, uint32_t n,
+                           SideF side_of, PassStats& st)
 {
-    float acc = 0.0f;
     const uint32_t me = lane();
-    const uint32_t my_side = me / D, my_comp = me % D;
+    float acc = 0.0f;
+    st.w[0] = st.w[1] = 0;
+    st.tt[0] = st.tt[1] = 0.0;
+    st.cnt[0] = st.cnt[1] = 0;
+    float v[D];
+    uint32_t w = 0;
+    if (me < n) {
+        const uint32_t vi = list[me];
+        load_vec<D>(A, vi, v);
+        w = A.w[vi];
+    }
+    const float* src = me < D ? &L.a[0][me] : &L.b[0][me < 2 * D ? me - D : 0];
     for (uint32_t j0 = 0; j0 < n; j0 += 64) {
-        const uint32_t j = j0 + me;
-        if (j < n) {
-            float v[D];
-            uint32_t w, s;
-            gen(j, v, w, s);
+        if (j0 + me < n) {
+            const uint32_t sd = side_of(v);
+            const float wf = (float)w;
+            const double t = vdot<D>(v, v) * wf;
 #pragma unroll
-            for (int c = 0; c < D; c++) L.a[me][c] = v[c] * (float)w;
-            L.side[me] = s;
+            for (int c = 0; c < D; c++) {
+                const float p = v[c] * wf;
+                L.a[me][c] = sd == 0 ? p : 0.0f;
+                L.b[me][c] = sd == 0 ? 0.0f : p;
+            }
+            if (sd == 0) {
+                st.w[0] += w;
+                st.tt[0] += t;
+                st.cnt[0]++;
+            } else {
+                st.w[1] += w;
+                st.tt[1] += t;
+                st.cnt[1]++;
+            }
+        }
+        const uint32_t jn = j0 + 64 + me;
+        if (jn < n) {
+            const uint32_t vi = list[jn];
+            load_vec<D>(A, vi, v);
+            w = A.w[vi];
         }
         __syncthreads();
         const uint32_t cnt = n - j0 < 64 ? n - j0 : 64;
-        if (me < 2 * D)
-            for (uint32_t k = 0; k < cnt; k++)
-                if (L.side[k] == my_side) acc += L.a[k][my_comp];
+        if (me < 2 * D) {
+            // adding 0.0f to a sum of non-negative terms leaves it unchanged
+            uint32_t k = 0;
+            for (; k + 4 <= cnt; k += 4) {
+                const float t0 = src[(k + 0) * (D + 1)], t1 = src[(k + 1) * (D + 1)];
+                const float t2 = src[(k + 2) * (D + 1)], t3 = src[(k + 3) * (D + 1)];
+                acc += t0;
+                acc += t1;
+                acc += t2;
+                acc += t3;
+            }
+            for (; k < cnt; k++) acc += src[k * (D + 1)];
+        }
         __syncthreads();
     }
     return acc;
@@ -452,10 +499,12 @@ __device__ inline void gather_sums(float acc, uint32_t s, float (&out)[D])
 // Weighted covariance about cen, entries in list order (crn_clusterizer.h:
 // 494-509, crn_threaded_clusterizer.h:266-279); then scaled as mode asks,
 // mirrored into L.cov.  mode 0: float one_over (clusterizer), 1: double
-// one_over, result rounded to float (threaded clusterizer).
-template <int D, class Get>
-__device__ void covariance(VqLds<D>& L, uint32_t n, const float (&cen)[D], Get get, float oo_f, double oo_d,
-                           int mode)
+// one_over, result rounded to float (threaded clusterizer).  Lane e holds
+// entries e, e + 64 and e + 128 of the upper triangle; batches of 64
+// vectors as in side_pass, the next one loaded during the sums.
+template <int D>
+__device__ void covariance(VqLds<D>& L, const VqArgs& A, const uint32_t* __restrict__ list, uint32_t n,
+                           const float (&cen)[D], float oo_f, double oo_d, int mode)
 {
     constexpr int kE = VqLds<D>::kE;
     const uint32_t me = lane();
@@ -477,12 +526,15 @@ __device__ void covariance(VqLds<D>& L, uint32_t n, const float (&cen)[D], Get g
             ex[r] = ey[r] = -1;
         }
     }
+    float v[D];
+    uint32_t w = 0;
+    if (me < n) {
+        const uint32_t vi = list[me];
+        load_vec<D>(A, vi, v);
+        w = A.w[vi];
+    }
     for (uint32_t j0 = 0; j0 < n; j0 += 64) {
-        const uint32_t j = j0 + me;
-        if (j < n) {
-            float v[D];
-            uint32_t w;
-            get(j, v, w);
+        if (j0 + me < n) {
             const float wf = (float)w;
 #pragma unroll
             for (int c = 0; c < D; c++) {
@@ -491,12 +543,30 @@ __device__ void covariance(VqLds<D>& L, uint32_t n, const float (&cen)[D], Get g
                 L.b[me][c] = d * wf;
             }
         }
+        const uint32_t jn = j0 + 64 + me;
+        if (jn < n) {
+            const uint32_t vi = list[jn];
+            load_vec<D>(A, vi, v);
+            w = A.w[vi];
+        }
         __syncthreads();
         const uint32_t cnt = n - j0 < 64 ? n - j0 : 64;
 #pragma unroll
         for (int r = 0; r < 3; r++)
-            if (ex[r] >= 0)
-                for (uint32_t k = 0; k < cnt; k++) acc[r] = acc[r] + L.a[k][ex[r]] * L.b[k][ey[r]];
+            if (ex[r] >= 0) {
+                const float* pa = &L.a[0][ex[r]];
+                const float* pb = &L.b[0][ey[r]];
+                float a = acc[r];
+                uint32_t k = 0;
+                for (; k + 2 <= cnt; k += 2) {
+                    const float x0 = pa[k * (D + 1)] * pb[k * (D + 1)];
+                    const float x1 = pa[(k + 1) * (D + 1)] * pb[(k + 1) * (D + 1)];
+                    a = a + x0;
+                    a = a + x1;
+                }
+                if (k < cnt) a = a + pa[k * (D + 1)] * pb[k * (D + 1)];
+                acc[r] = a;
+            }
         __syncthreads();
     }
 #pragma unroll
@@ -607,35 +677,18 @@ __global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* _
         load_vec<D>(A, list[1], rc);
     } else {
         // compute_split_pca (:483-601)
-        covariance<D>(L, n, cen,
-                      [&](uint32_t j, float (&v)[D], uint32_t& w) {
-                          load_vec<D>(A, list[j], v);
-                          w = A.w[list[j]];
-                      },
-                      1.0f / (float)ptw, 0.0, 0);
+        covariance<D>(L, A, list, n, cen, 1.0f / (float)ptw, 0.0, 0);
         float axis[D];
         pca_axis<D>(L, axis);
-        double lwp = 0, rwp = 0;
-        for (uint32_t j = me; j < n; j += 64) {
-            float v[D], d[D];
-            load_vec<D>(A, list[j], v);
-#pragma unroll
-            for (int c = 0; c < D; c++) d[c] = v[c] - cen[c];
-            const double t = vdot<D>(d, axis);
-            if (t < 0.0f) lwp += (float)A.w[list[j]];
-            else rwp += (float)A.w[list[j]];
-        }
-        lwp = wave_sum_d(lwp);
-        rwp = wave_sum_d(rwp);
-        const float acc = seq_side_sums<D>(L, n, [&](uint32_t j, float (&v)[D], uint32_t& w, uint32_t& s) {
-            load_vec<D>(A, list[j], v);
-            w = A.w[list[j]];
+        PassStats ps;
+        const float acc = side_pass<D>(L, A, list, n, [&](const float (&v)[D]) {
             float d[D];
 #pragma unroll
             for (int c = 0; c < D; c++) d[c] = v[c] - cen[c];
             const double t = vdot<D>(d, axis);
-            s = t < 0.0f ? 0u : 1u;
-        });
+            return t < 0.0f ? 0u : 1u;
+        }, ps);
+        const double lwp = (double)wave_sum_u64(ps.w[0]), rwp = (double)wave_sum_u64(ps.w[1]);
         if (lwp > 0.0f && rwp > 0.0f) {
             float sl[D], sr[D];
             gather_sums<D>(acc, 0, sl);
@@ -669,39 +722,19 @@ __global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* _
             plc[c] = lc[c];
             prc[c] = rc[c];
         }
-        double lt = 0, rt = 0;
-        unsigned long long lwl = 0, rwl = 0;
-        uint32_t nll = 0;
-        for (uint32_t j = me; j < n; j += 64) {
-            float v[D];
-            load_vec<D>(A, list[j], v);
-            const uint32_t w = A.w[list[j]];
+        PassStats ps;
+        const float acc = side_pass<D>(L, A, list, n, [&](const float (&v)[D]) {
             const double ld = vsqd<D>(plc, v), rd = vsqd<D>(prc, v);
-            const double tt = vdot<D>(v, v) * (float)w;
-            if (ld < rd) {
-                lwl += w;
-                lt += tt;
-                nll++;
-            } else {
-                rwl += w;
-                rt += tt;
-            }
-        }
-        lw = wave_sum_u64(lwl);
-        rw = wave_sum_u64(rwl);
-        lt = wave_sum_d(lt);
-        rt = wave_sum_d(rt);
-        nl = (uint32_t)wave_sum_u64(nll);
+            return ld < rd ? 0u : 1u;
+        }, ps);
+        lw = wave_sum_u64(ps.w[0]);
+        rw = wave_sum_u64(ps.w[1]);
+        const double lt = wave_sum_d(ps.tt[0]), rt = wave_sum_d(ps.tt[1]);
+        nl = (uint32_t)wave_sum_u64(ps.cnt[0]);
         if (!lw || !rw) {
             uns = true;
             break;
         }
-        const float acc = seq_side_sums<D>(L, n, [&](uint32_t j, float (&v)[D], uint32_t& w, uint32_t& s) {
-            load_vec<D>(A, list[j], v);
-            w = A.w[list[j]];
-            const double ld = vsqd<D>(plc, v), rd = vsqd<D>(prc, v);
-            s = ld < rd ? 0u : 1u;
-        });
         float sl[D], sr[D];
         gather_sums<D>(acc, 0, sl);
         gather_sums<D>(acc, 1, sr);
@@ -773,22 +806,10 @@ __global__ void __launch_bounds__(64) vq_root_kernel(VqArgs A, uint32_t seg, uin
     __shared__ VqLds<D> L;
     const uint32_t* list = A.pool + seg;
     const uint32_t me = lane();
-    unsigned long long tw = 0;
-    double tt = 0;
-    for (uint32_t j = me; j < n; j += 64) {
-        float v[D];
-        load_vec<D>(A, list[j], v);
-        const uint32_t w = A.w[list[j]];
-        tw += w;
-        tt += vdot<D>(v, v) * (float)w;
-    }
-    tw = wave_sum_u64(tw);
-    tt = wave_sum_d(tt);
-    const float acc = seq_side_sums<D>(L, n, [&](uint32_t j, float (&v)[D], uint32_t& w, uint32_t& s) {
-        load_vec<D>(A, list[j], v);
-        w = A.w[list[j]];
-        s = 0;
-    });
+    PassStats ps;
+    const float acc = side_pass<D>(L, A, list, n, [&](const float (&)[D]) { return 0u; }, ps);
+    const unsigned long long tw = wave_sum_u64(ps.w[0]);
+    const double tt = wave_sum_d(ps.tt[0]);
     float sum[D];
     gather_sums<D>(acc, 0, sum);
     const float var = (float)(tt - (vdot<D>(sum, sum) / (float)tw));
@@ -815,14 +836,9 @@ __global__ void __launch_bounds__(64) vq_division_kernel(VqArgs A, const DivJob*
     const DivJob jb = jobs[blockIdx.x];
     const uint32_t* list = A.pool + jb.seg;
     const uint32_t n = jb.n, me = lane();
-    double tw = 0;
-    for (uint32_t j = me; j < n; j += 64) tw += A.w[list[j]];
-    tw = wave_sum_d(tw);
-    const float acc = seq_side_sums<D>(L, n, [&](uint32_t j, float (&v)[D], uint32_t& w, uint32_t& s) {
-        load_vec<D>(A, list[j], v);
-        w = A.w[list[j]];
-        s = 0;
-    });
+    PassStats ps;
+    const float acc = side_pass<D>(L, A, list, n, [&](const float (&)[D]) { return 0u; }, ps);
+    const double tw = (double)wave_sum_u64(ps.w[0]);
     float cen[D], axis[D];
     gather_sums<D>(acc, 0, cen);
     if (tw == 0.0f) {
@@ -832,12 +848,7 @@ __global__ void __launch_bounds__(64) vq_division_kernel(VqArgs A, const DivJob*
         const double oo = 1.0f / tw;
 #pragma unroll
         for (int c = 0; c < D; c++) cen[c] = (float)(cen[c] * oo);
-        covariance<D>(L, n, cen,
-                      [&](uint32_t j, float (&v)[D], uint32_t& w) {
-                          load_vec<D>(A, list[j], v);
-                          w = A.w[list[j]];
-                      },
-                      0.0f, oo, 1);
+        covariance<D>(L, A, list, n, cen, 0.0f, oo, 1);
         pca_axis<D>(L, axis);
     }
     uint32_t nl = 0;
