@@ -469,6 +469,78 @@ def mip_leg(size, cpu_seconds):
                                        "level per thread on %d threads (%.1fs)" % (len(sizes), th, cpu_s)}}
 
 
+def qdxt_leg(size):
+    """Clustered DXT5 -- crnlib's DDS path below the top quality
+    (mipmapped_texture::qdxt_pack_init / qdxt_pack,
+    crnlib/crn_mipmapped_texture.cpp:2310-2560) -- of one size x size RGBA
+    texture with its box mip chain: qdxt1 for the colour (no 3-colour blocks,
+    codebook curve 0.75, as DXT5 asks) and qdxt5 for the alpha, quality 128,
+    uber; a step is init + pack of both on the GPU.  Checked bit for bit
+    against the oracle, whose single-thread run of the same step is the CPU
+    baseline.  Roofline of the dominant launch (the 16-D selector-tree
+    splits) from the vector bytes each of its passes reads."""
+    import numpy as np
+    import torch
+    import oracle
+    from crunch2_amd import qdxt, texture
+    blocks, desc = qdxt.level_blocks(texture.mip_chain(texture.synth(size, size, 77, texture.RGBA)))
+    n = len(blocks)
+    b = torch.from_numpy(blocks).cuda()
+    P, H = qdxt.PERCEPTUAL, qdxt.HIERARCHICAL
+
+    def step():
+        q1 = qdxt.Qdxt1(b, desc)
+        c = q1.pack(qdxt.params(128, 4, P | H), 0.75)
+        q5 = qdxt.Qdxt5(b, desc, 3)
+        a = q5.pack(qdxt.params5(128, 4, qdxt.BOTH_BLOCK_TYPES))
+        info = (q1.info(), q5.info())
+        q1.close()
+        q5.close()
+        return c, a, info
+
+    step()
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        c, a, info = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    qdxt.profile(True)
+    step()
+    torch.cuda.synchronize()
+    sms, launches, passes = qdxt.profile(False)
+    got_c = c.cpu().numpy().view(np.uint64)
+    got_a = a.cpu().numpy().view(np.uint64)
+    t0 = time.perf_counter()
+    exp_c, _, _ = oracle.qdxt1(blocks, desc, 128, 4, oracle.QDXT_PERCEPTUAL | oracle.QDXT_HIERARCHICAL, 128, 0.75)
+    exp_a, _, _ = oracle.qdxt5(blocks, desc, 128, 4, oracle.QDXT5_BOTH | oracle.QDXT_HIERARCHICAL, 3)
+    cpu_s = time.perf_counter() - t0
+    exact = bool(np.array_equal(got_c, exp_c) and np.array_equal(got_a, exp_a))
+    assert exact, "GPU clustered DXT5 differs from the oracle"
+    texels = 16 * n
+    # one vector pass of a 16-D split reads the vector (64 B), its weight and its index
+    k = 2
+    bytes16 = passes[k] * (16 * 4 + 4 + 4)
+    achieved = bytes16 / (sms[k] * 1e-3) / 1e9 if sms[k] else 0.0
+    return {"workload": "clustered DXT5 (qdxt1 colour + qdxt5 alpha) of one %dx%d RGBA8 texture with its %d-level "
+                        "mip chain (%d blocks), quality 128, uber, init + pack" % (size, size, len(desc) // 3, n),
+            "mtexels_s": round(texels / (ms * 1e-3) / 1e6, 2), "ms": round(ms, 2), "bit_exact": exact,
+            "clusters": {"colour_endpoint": info[0]["endpoint_clusters"], "colour_selector":
+                         info[0]["selector_clusters"], "alpha_endpoint": info[1]["endpoint_clusters"],
+                         "alpha_selector": info[1]["selector_clusters"]},
+            "tree_split_ms": {"2d": round(sms[0], 2), "6d": round(sms[1], 2), "16d": round(sms[2], 2)},
+            "roofline": {"bound": "hbm", "kernel": "vq_split_kernel<16>", "achieved": round(achieved, 2),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                         "traffic": None, "algorithmic_bytes_per_step": bytes16, "vector_passes": passes[k],
+                         "launches": launches[k], "launch_ms_per_step": round(sms[k], 2),
+                         "note": "the splits' float sums run in list order, one add per vector and component on a "
+                                 "serial chain: latency-bound, not HBM-bound"},
+            "cpu_baseline": {"value": round(texels / cpu_s / 1e6, 3), "unit": "Mtexels/s", "cores": 1,
+                             "kind": "port", "sample": "the oracle's qdxt1 + qdxt5 of the same texture, one thread "
+                                                       "(%.1fs)" % cpu_s}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -499,6 +571,7 @@ def main():
                     help="1: time the oracle's dxt_hc composition on the same texture (about 15 s); 0 skips it")
     ap.add_argument("--encode-cpu-seconds", type=float, default=5.0, help="CPU baseline sample of each encode leg")
     ap.add_argument("--mip-size", type=int, default=8192, help="side of the mip generation leg (0 = skip)")
+    ap.add_argument("--qdxt-size", type=int, default=2048, help="side of the clustered DXT5 leg (0 = skip)")
     ap.add_argument("--fallback-frac", type=float, default=0.08,
                     help="fraction of blocks (lowest confidence first) that also run the plain search in the encode leg")
     args = ap.parse_args()
@@ -746,6 +819,8 @@ def main():
         log("serving-set leg done")
     if args.mip_size:
         line["mips"] = mip_leg(args.mip_size, args.encode_cpu_seconds)
+    if args.qdxt_size:
+        line["qdxt"] = qdxt_leg(args.qdxt_size)
         log("mip generation leg done")
     print(json.dumps(line))
     for pl in plans:

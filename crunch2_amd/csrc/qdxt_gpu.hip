@@ -351,13 +351,32 @@ struct VqArgs {
     uint32_t* pool;                 // index segments
     float* cen;                     // per slot: D floats
     unsigned long long* tw;         // per slot
+    unsigned long long* passes;     // profiling: vector passes of the splits (NULL: off)
+    float exact_max;                // > 0: components are integers in [0, exact_max] (see side_pass)
 };
+
+// Float sums of non-negative integers are exact in any order while the total
+// stays within 2^24: a node whose weight times the largest component is at
+// most that sums its vectors by a wave reduction (same bits as the list
+// order); others keep the serial order.
+__device__ inline bool exact_sums(const VqArgs& A, unsigned long long total_weight)
+{
+    return A.exact_max > 0.0f && (double)total_weight * (double)A.exact_max <= 16777216.0;
+}
+
+__device__ inline float wave_sum_f(float v)
+{
+    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
 
 template <int D>
 struct VqLds {
     static constexpr int kE = D * (D + 1) / 2;
-    float a[64][D + 1];   // staged batch: vector - centroid, or products
-    float b[64][D + 1];   // staged batch: (vector - centroid) * weight
+    static constexpr int kR = D >= 16 ? 4 : 8;   // vectors per lane per batch
+    static constexpr int kRow = 64 * kR + 4;     // component rows (padded: rows start in different banks)
+    alignas(16) float a[D][kRow];   // staged batch, component-major: vector - centroid, or products
+    alignas(16) float b[D][kRow];   // staged batch: (vector - centroid) * weight
     uint32_t side[64];
     float cov[D][D];
     float axis[D], prev[D], x[D];
@@ -411,6 +430,37 @@ __device__ inline void load_vec(const VqArgs& A, uint32_t vi, float (&v)[D])
     for (int c = 0; c < D; c++) v[c] = A.v[(size_t)vi * D + c];
 }
 
+// acc + x[0] + x[1] + ... + x[cnt - 1], added one after another; x is 16-byte
+// aligned LDS.  Reads 16 values ahead (four 128-bit loads) while adding.
+__device__ inline float serial_sum(float acc, const float* x, uint32_t cnt)
+{
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    uint32_t k = 0;
+    if (cnt >= 16) {
+        float4 q0 = x4[0], q1 = x4[1], q2 = x4[2], q3 = x4[3];
+        for (; k + 16 <= cnt; k += 16) {
+            float4 n0 = q0, n1 = q1, n2 = q2, n3 = q3;
+            if (k + 32 <= cnt) {
+                const uint32_t g = (k + 16) >> 2;
+                n0 = x4[g];
+                n1 = x4[g + 1];
+                n2 = x4[g + 2];
+                n3 = x4[g + 3];
+            }
+            acc += q0.x; acc += q0.y; acc += q0.z; acc += q0.w;
+            acc += q1.x; acc += q1.y; acc += q1.z; acc += q1.w;
+            acc += q2.x; acc += q2.y; acc += q2.z; acc += q2.w;
+            acc += q3.x; acc += q3.y; acc += q3.z; acc += q3.w;
+            q0 = n0;
+            q1 = n1;
+            q2 = n2;
+            q3 = n3;
+        }
+    }
+    for (; k < cnt; k++) acc += x[k];
+    return acc;
+}
+
 // Per-lane partial statistics of one pass (reduced by the caller).
 struct PassStats {
     unsigned long long w[2];   // weights per side
@@ -426,31 +476,32 @@ struct PassStats {
 // current one is summed.  Returns the sum of lane k's (side, component).
 template <int D, class SideF>
 __device__ float side_pass(VqLds<D>& L, const VqArgs& A, const uint32_t* __restrict__ list, uint32_t n,
-                           SideF side_of, PassStats& st)
+                           SideF side_of, PassStats& st, bool exact)
 {
     const uint32_t me = lane();
     float acc = 0.0f;
     st.w[0] = st.w[1] = 0;
     st.tt[0] = st.tt[1] = 0.0;
     st.cnt[0] = st.cnt[1] = 0;
-    float v[D];
-    uint32_t w = 0;
-    if (me < n) {
-        const uint32_t vi = list[me];
-        load_vec<D>(A, vi, v);
-        w = A.w[vi];
-    }
-    const float* src = me < D ? &L.a[0][me] : &L.b[0][me < 2 * D ? me - D : 0];
-    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
-        if (j0 + me < n) {
+    if (exact) {
+        // integer terms, total within 2^24: per-lane sums, then a wave reduction
+        float pl[D], pr[D];
+#pragma unroll
+        for (int c = 0; c < D; c++) pl[c] = pr[c] = 0.0f;
+#pragma unroll 4
+        for (uint32_t j = me; j < n; j += 64) {
+            float v[D];
+            const uint32_t vi = list[j];
+            load_vec<D>(A, vi, v);
+            const uint32_t w = A.w[vi];
             const uint32_t sd = side_of(v);
             const float wf = (float)w;
             const double t = vdot<D>(v, v) * wf;
 #pragma unroll
             for (int c = 0; c < D; c++) {
                 const float p = v[c] * wf;
-                L.a[me][c] = sd == 0 ? p : 0.0f;
-                L.b[me][c] = sd == 0 ? 0.0f : p;
+                pl[c] += sd == 0 ? p : 0.0f;
+                pr[c] += sd == 0 ? 0.0f : p;
             }
             if (sd == 0) {
                 st.w[0] += w;
@@ -462,27 +513,68 @@ __device__ float side_pass(VqLds<D>& L, const VqArgs& A, const uint32_t* __restr
                 st.cnt[1]++;
             }
         }
-        const uint32_t jn = j0 + 64 + me;
-        if (jn < n) {
-            const uint32_t vi = list[jn];
-            load_vec<D>(A, vi, v);
-            w = A.w[vi];
+#pragma unroll
+        for (int c = 0; c < D; c++) {
+            const float a = wave_sum_f(pl[c]), b = wave_sum_f(pr[c]);
+            if (me == (uint32_t)c) acc = a;
+            if (me == (uint32_t)(D + c)) acc = b;
+        }
+        return acc;
+    }
+    constexpr int R = VqLds<D>::kR;
+    constexpr uint32_t kB = 64u * R;   // vectors per batch
+    float v[R][D];
+    uint32_t w[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        w[r] = 0;
+        const uint32_t j = (uint32_t)r * 64u + me;
+        if (j < n) {
+            const uint32_t vi = list[j];
+            load_vec<D>(A, vi, v[r]);
+            w[r] = A.w[vi];
+        }
+    }
+    const float* src = me < D ? &L.a[me][0] : &L.b[me < 2 * D ? me - D : 0][0];
+    for (uint32_t j0 = 0; j0 < n; j0 += kB) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t k = (uint32_t)r * 64u + me;
+            if (j0 + k < n) {
+                const uint32_t sd = side_of(v[r]);
+                const float wf = (float)w[r];
+                const double t = vdot<D>(v[r], v[r]) * wf;
+#pragma unroll
+                for (int c = 0; c < D; c++) {
+                    const float p = v[r][c] * wf;
+                    L.a[c][k] = sd == 0 ? p : 0.0f;
+                    L.b[c][k] = sd == 0 ? 0.0f : p;
+                }
+                if (sd == 0) {
+                    st.w[0] += w[r];
+                    st.tt[0] += t;
+                    st.cnt[0]++;
+                } else {
+                    st.w[1] += w[r];
+                    st.tt[1] += t;
+                    st.cnt[1]++;
+                }
+            }
+        }
+        // the next batch loads while this one is summed
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t jn = j0 + kB + (uint32_t)r * 64u + me;
+            if (jn < n) {
+                const uint32_t vi = list[jn];
+                load_vec<D>(A, vi, v[r]);
+                w[r] = A.w[vi];
+            }
         }
         __syncthreads();
-        const uint32_t cnt = n - j0 < 64 ? n - j0 : 64;
-        if (me < 2 * D) {
-            // adding 0.0f to a sum of non-negative terms leaves it unchanged
-            uint32_t k = 0;
-            for (; k + 4 <= cnt; k += 4) {
-                const float t0 = src[(k + 0) * (D + 1)], t1 = src[(k + 1) * (D + 1)];
-                const float t2 = src[(k + 2) * (D + 1)], t3 = src[(k + 3) * (D + 1)];
-                acc += t0;
-                acc += t1;
-                acc += t2;
-                acc += t3;
-            }
-            for (; k < cnt; k++) acc += src[k * (D + 1)];
-        }
+        const uint32_t cnt = n - j0 < kB ? n - j0 : kB;
+        // adding 0.0f to a sum of non-negative terms leaves it unchanged
+        if (me < 2 * D) acc = serial_sum(acc, src, cnt);
         __syncthreads();
     }
     return acc;
@@ -494,6 +586,38 @@ __device__ inline void gather_sums(float acc, uint32_t s, float (&out)[D])
 {
 #pragma unroll
     for (int c = 0; c < D; c++) out[c] = __shfl(acc, (int)(s * D + c));
+}
+
+// acc + x[0] * y[0] + x[1] * y[1] + ..., each product rounded, then added one
+// after another; 128-bit loads, 8 values ahead.
+__device__ inline float serial_dot(float acc, const float* x, const float* y, uint32_t cnt)
+{
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const float4* y4 = reinterpret_cast<const float4*>(y);
+    uint32_t k = 0;
+    if (cnt >= 8) {
+        float4 a0 = x4[0], a1 = x4[1], b0 = y4[0], b1 = y4[1];
+        for (; k + 8 <= cnt; k += 8) {
+            float4 na0 = a0, na1 = a1, nb0 = b0, nb1 = b1;
+            if (k + 16 <= cnt) {
+                const uint32_t g = (k + 8) >> 2;
+                na0 = x4[g];
+                na1 = x4[g + 1];
+                nb0 = y4[g];
+                nb1 = y4[g + 1];
+            }
+            const float p0 = a0.x * b0.x, p1 = a0.y * b0.y, p2 = a0.z * b0.z, p3 = a0.w * b0.w;
+            const float p4 = a1.x * b1.x, p5 = a1.y * b1.y, p6 = a1.z * b1.z, p7 = a1.w * b1.w;
+            acc = acc + p0; acc = acc + p1; acc = acc + p2; acc = acc + p3;
+            acc = acc + p4; acc = acc + p5; acc = acc + p6; acc = acc + p7;
+            a0 = na0;
+            a1 = na1;
+            b0 = nb0;
+            b1 = nb1;
+        }
+    }
+    for (; k < cnt; k++) acc = acc + x[k] * y[k];
+    return acc;
 }
 
 // Weighted covariance about cen, entries in list order (crn_clusterizer.h:
@@ -526,47 +650,48 @@ __device__ void covariance(VqLds<D>& L, const VqArgs& A, const uint32_t* __restr
             ex[r] = ey[r] = -1;
         }
     }
-    float v[D];
-    uint32_t w = 0;
-    if (me < n) {
-        const uint32_t vi = list[me];
-        load_vec<D>(A, vi, v);
-        w = A.w[vi];
-    }
-    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
-        if (j0 + me < n) {
-            const float wf = (float)w;
+    constexpr int R = VqLds<D>::kR;
+    constexpr uint32_t kB = 64u * R;
+    float v[R][D];
+    uint32_t w[R];
 #pragma unroll
-            for (int c = 0; c < D; c++) {
-                const float d = v[c] - cen[c];
-                L.a[me][c] = d;
-                L.b[me][c] = d * wf;
+    for (int r = 0; r < R; r++) {
+        w[r] = 0;
+        const uint32_t j = (uint32_t)r * 64u + me;
+        if (j < n) {
+            const uint32_t vi = list[j];
+            load_vec<D>(A, vi, v[r]);
+            w[r] = A.w[vi];
+        }
+    }
+    for (uint32_t j0 = 0; j0 < n; j0 += kB) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t k = (uint32_t)r * 64u + me;
+            if (j0 + k < n) {
+                const float wf = (float)w[r];
+#pragma unroll
+                for (int c = 0; c < D; c++) {
+                    const float d = v[r][c] - cen[c];
+                    L.a[c][k] = d;
+                    L.b[c][k] = d * wf;
+                }
             }
         }
-        const uint32_t jn = j0 + 64 + me;
-        if (jn < n) {
-            const uint32_t vi = list[jn];
-            load_vec<D>(A, vi, v);
-            w = A.w[vi];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t jn = j0 + kB + (uint32_t)r * 64u + me;
+            if (jn < n) {
+                const uint32_t vi = list[jn];
+                load_vec<D>(A, vi, v[r]);
+                w[r] = A.w[vi];
+            }
         }
         __syncthreads();
-        const uint32_t cnt = n - j0 < 64 ? n - j0 : 64;
+        const uint32_t cnt = n - j0 < kB ? n - j0 : kB;
 #pragma unroll
         for (int r = 0; r < 3; r++)
-            if (ex[r] >= 0) {
-                const float* pa = &L.a[0][ex[r]];
-                const float* pb = &L.b[0][ey[r]];
-                float a = acc[r];
-                uint32_t k = 0;
-                for (; k + 2 <= cnt; k += 2) {
-                    const float x0 = pa[k * (D + 1)] * pb[k * (D + 1)];
-                    const float x1 = pa[(k + 1) * (D + 1)] * pb[(k + 1) * (D + 1)];
-                    a = a + x0;
-                    a = a + x1;
-                }
-                if (k < cnt) a = a + pa[k * (D + 1)] * pb[k * (D + 1)];
-                acc[r] = a;
-            }
+            if (ex[r] >= 0) acc[r] = serial_dot(acc[r], &L.a[ex[r]][0], &L.b[ey[r]][0], cnt);
         __syncthreads();
     }
 #pragma unroll
@@ -671,6 +796,7 @@ __global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* _
 #pragma unroll
     for (int c = 0; c < D; c++) cen[c] = A.cen[(size_t)sp.node * D + c];
     const unsigned long long ptw = A.tw[sp.node];
+    const bool exact = exact_sums(A, ptw);
     float lc[D], rc[D];
     if (n == 2) {
         load_vec<D>(A, list[0], lc);
@@ -687,7 +813,7 @@ __global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* _
             for (int c = 0; c < D; c++) d[c] = v[c] - cen[c];
             const double t = vdot<D>(d, axis);
             return t < 0.0f ? 0u : 1u;
-        }, ps);
+        }, ps, exact);
         const double lwp = (double)wave_sum_u64(ps.w[0]), rwp = (double)wave_sum_u64(ps.w[1]);
         if (lwp > 0.0f && rwp > 0.0f) {
             float sl[D], sr[D];
@@ -710,6 +836,7 @@ __global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* _
             }
         }
     }
+    uint32_t npass = n == 2 ? 0u : 2u;
     // 2-means (:746-832)
     float plc[D], prc[D];
     unsigned long long lw = 0, rw = 0;
@@ -717,6 +844,7 @@ __global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* _
     uint32_t nl = 0;
     bool uns = false;
     for (int loop = 0; loop < 8; loop++) {
+        npass++;
 #pragma unroll
         for (int c = 0; c < D; c++) {
             plc[c] = lc[c];
@@ -726,7 +854,7 @@ __global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* _
         const float acc = side_pass<D>(L, A, list, n, [&](const float (&v)[D]) {
             const double ld = vsqd<D>(plc, v), rd = vsqd<D>(prc, v);
             return ld < rd ? 0u : 1u;
-        }, ps);
+        }, ps, exact);
         lw = wave_sum_u64(ps.w[0]);
         rw = wave_sum_u64(ps.w[1]);
         const double lt = wave_sum_d(ps.tt[0]), rt = wave_sum_d(ps.tt[1]);
@@ -756,7 +884,10 @@ __global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* _
         r.split = 0;
         r.nl = r.nr = 0;
         r.lvar = r.rvar = 0.0f;
-        if (me == 0) results[blockIdx.x] = r;
+        if (me == 0) {
+            results[blockIdx.x] = r;
+            if (A.passes) atomicAdd(A.passes, (unsigned long long)n * npass);
+        }
         return;
     }
     // the final round's sides (from the centres before its update), stable
@@ -786,6 +917,7 @@ __global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* _
         A.cen[(size_t)sp.right * D + me] = rc[me];
     }
     if (me == 0) {
+        if (A.passes) atomicAdd(A.passes, (unsigned long long)n * (npass + 1));
         A.tw[sp.left] = lw;
         A.tw[sp.right] = rw;
         r.split = 1;
@@ -806,8 +938,11 @@ __global__ void __launch_bounds__(64) vq_root_kernel(VqArgs A, uint32_t seg, uin
     __shared__ VqLds<D> L;
     const uint32_t* list = A.pool + seg;
     const uint32_t me = lane();
+    unsigned long long tw0 = 0;
+    for (uint32_t j = me; j < n; j += 64) tw0 += A.w[list[j]];
+    tw0 = wave_sum_u64(tw0);
     PassStats ps;
-    const float acc = side_pass<D>(L, A, list, n, [&](const float (&)[D]) { return 0u; }, ps);
+    const float acc = side_pass<D>(L, A, list, n, [&](const float (&)[D]) { return 0u; }, ps, exact_sums(A, tw0));
     const unsigned long long tw = wave_sum_u64(ps.w[0]);
     const double tt = wave_sum_d(ps.tt[0]);
     float sum[D];
@@ -836,8 +971,11 @@ __global__ void __launch_bounds__(64) vq_division_kernel(VqArgs A, const DivJob*
     const DivJob jb = jobs[blockIdx.x];
     const uint32_t* list = A.pool + jb.seg;
     const uint32_t n = jb.n, me = lane();
+    unsigned long long tw0 = 0;
+    for (uint32_t j = me; j < n; j += 64) tw0 += A.w[list[j]];
+    tw0 = wave_sum_u64(tw0);
     PassStats ps;
-    const float acc = side_pass<D>(L, A, list, n, [&](const float (&)[D]) { return 0u; }, ps);
+    const float acc = side_pass<D>(L, A, list, n, [&](const float (&)[D]) { return 0u; }, ps, exact_sums(A, tw0));
     const double tw = (double)wave_sum_u64(ps.w[0]);
     float cen[D], axis[D];
     gather_sums<D>(acc, 0, cen);
@@ -1491,6 +1629,20 @@ int q_tables(int dev)
 
 uint32_t grid(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
+// Stage timing of the tree splits with HIP events on the launch stream
+// (crn2_qdxt_profile): per dimension the summed launch times, launch count
+// and vector passes (one vector read by one pass of a split).
+struct QProf {
+    bool on = false;
+    double ms[3] = {0, 0, 0};
+    uint64_t launches[3] = {0, 0, 0};
+    unsigned long long* passes = nullptr;   // device counters, one per dimension
+};
+QProf g_qprof;
+std::mutex g_qprof_mu;
+
+int dim_slot(uint32_t d) { return d == 2 ? 0 : (d == 6 ? 1 : 2); }
+
 // Device buffers that grow as the trees do.
 struct DevBuf {
     void* p = nullptr;
@@ -1662,6 +1814,7 @@ struct Forest {
     uint32_t d = 0;
     const float* v = nullptr;
     const uint32_t* w = nullptr;
+    float exact_max = 0.0f;   // > 0: the vectors' components are integers in [0, exact_max]
     DevBuf pool, cen, tw, jobs, res, varbuf;
     uint32_t pool_used = 0, slots = 0;
     hipStream_t st = nullptr;
@@ -1683,6 +1836,8 @@ struct Forest {
         a.pool = (uint32_t*)pool.p;
         a.cen = (float*)cen.p;
         a.tw = (unsigned long long*)tw.p;
+        a.passes = g_qprof.on && g_qprof.passes ? g_qprof.passes + dim_slot(d) : nullptr;
+        a.exact_max = exact_max;
         return a;
     }
     int reserve(uint32_t more_pool, uint32_t more_slots)
@@ -1777,14 +1932,28 @@ struct Forest {
             const dim3 g((uint32_t)sp.size());
             const VqSplit* js = (const VqSplit*)jobs.p;
             VqResult* rr = (VqResult*)res.p;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            const bool prof = g_qprof.on && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+            if (prof) (void)hipEventRecord(e0, st);
             if (d == 2) hipLaunchKernelGGL(vq_split_kernel<2>, g, dim3(64), 0, st, a, js, rr);
             else if (d == 6) hipLaunchKernelGGL(vq_split_kernel<6>, g, dim3(64), 0, st, a, js, rr);
             else hipLaunchKernelGGL(vq_split_kernel<16>, g, dim3(64), 0, st, a, js, rr);
+            if (prof) (void)hipEventRecord(e1, st);
             rs.resize(sp.size());
             if (hipMemcpyAsync(rs.data(), res.p, sp.size() * sizeof(VqResult), hipMemcpyDeviceToHost, st) !=
                     hipSuccess ||
                 hipStreamSynchronize(st) != hipSuccess)
                 return CRN2_EHIP;
+            if (prof) {
+                float ms = 0;
+                if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+                    std::lock_guard<std::mutex> lk(g_qprof_mu);
+                    g_qprof.ms[dim_slot(d)] += ms;
+                    g_qprof.launches[dim_slot(d)]++;
+                }
+            }
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
             for (size_t i = 0; i < sp.size(); i++) {
                 HNode& nd = who[i].first->nodes[who[i].second];
                 nd.res = rs[i];
@@ -1964,6 +2133,7 @@ extern "C" int crn2_qdxt1_init(const void* blocks, uint32_t nblocks, uint32_t nl
     q->st = st;
     q->F.d = 6;
     q->F.st = st;
+    q->F.exact_max = 255.0f;   // colours
     TChunk* dch = nullptr;
     uint32_t* lerr = nullptr;
     unsigned long long* table = nullptr;
@@ -2161,6 +2331,7 @@ extern "C" int crn2_qdxt1_pack(crn2_qdxt1* q, const crn2_qdxt1_params* p, float 
     S.v = sv;
     S.w = sw;
     S.st = st;
+    S.exact_max = 3.0f;   // g_dxt1_to_linear values
     std::vector<std::pair<uint32_t, uint32_t>> scl;
     rc = threaded_clusters(S, n, msc, scl);
     if (rc == CRN2_OK) {
@@ -2301,6 +2472,7 @@ extern "C" int crn2_qdxt5_init(const void* blocks, uint32_t nblocks, uint32_t nl
     q->st = st;
     q->F.d = 2;
     q->F.st = st;
+    q->F.exact_max = 255.0f;   // alpha values
     TChunk* dch = nullptr;
     uint32_t* lerr = nullptr;
     unsigned long long* table = nullptr;
@@ -2474,6 +2646,7 @@ extern "C" int crn2_qdxt5_pack(crn2_qdxt5* q, const crn2_qdxt5_params* p, void* 
     S.v = sv;
     S.w = sw;
     S.st = st;
+    S.exact_max = 7.0f;   // g_dxt5_to_linear / g_dxt5_alpha6_to_linear values
     std::vector<std::pair<uint32_t, uint32_t>> scl, part;
     for (uint32_t type = 0; type < 2 && rc == CRN2_OK; type++) {
         std::vector<uint32_t> members;
@@ -2518,5 +2691,34 @@ extern "C" int crn2_qdxt5_destroy(crn2_qdxt5* q)
     if (q->weights) (void)hipFree(q->weights);
     if (q->enc) (void)hipFree(q->enc);
     delete q;
+    return CRN2_OK;
+}
+
+// Split timing: on != 0 resets and starts it; ms[3], launches[3] and
+// passes[3] (2-, 6- and 16-D trees) receive the totals since then.
+extern "C" int crn2_qdxt_profile(int on, double* ms, uint64_t* launches, uint64_t* passes)
+{
+    std::lock_guard<std::mutex> lk(g_qprof_mu);
+    if (!g_qprof.passes && hipMalloc((void**)&g_qprof.passes, 3 * sizeof(unsigned long long)) != hipSuccess) {
+        g_qprof.passes = nullptr;
+        return CRN2_ENOMEM;
+    }
+    if (on) {
+        for (int i = 0; i < 3; i++) {
+            g_qprof.ms[i] = 0;
+            g_qprof.launches[i] = 0;
+        }
+        if (hipMemset(g_qprof.passes, 0, 3 * sizeof(unsigned long long)) != hipSuccess) return CRN2_EHIP;
+        g_qprof.on = true;
+        return CRN2_OK;
+    }
+    g_qprof.on = false;
+    unsigned long long p[3] = {0, 0, 0};
+    if (hipMemcpy(p, g_qprof.passes, sizeof(p), hipMemcpyDeviceToHost) != hipSuccess) return CRN2_EHIP;
+    for (int i = 0; i < 3; i++) {
+        if (ms) ms[i] = g_qprof.ms[i];
+        if (launches) launches[i] = g_qprof.launches[i];
+        if (passes) passes[i] = p[i];
+    }
     return CRN2_OK;
 }

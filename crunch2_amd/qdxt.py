@@ -12,6 +12,35 @@ PERCEPTUAL, ALPHA_BLOCKS, HIERARCHICAL = 1, 2, 4
 UBER = 4
 
 
+def level_blocks(levels):
+    """The pixel blocks of a texture's levels as mipmapped_texture::qdxt_pack_init
+    lays them out (crnlib/crn_mipmapped_texture.cpp:2410-2469): every level's
+    4x4 blocks in raster order, texels past the edge clamped, levels one after
+    another; returns (uint8 [n, 64] blocks, uint32 level descriptors: first
+    block, width and height in blocks per level)."""
+    blocks, desc, first = [], [], 0
+    for img in levels:
+        h, w = img.shape[:2]
+        bw, bh = (w + 3) // 4, (h + 3) // 4
+        ys = np.minimum(np.arange(bh * 4), h - 1)
+        xs = np.minimum(np.arange(bw * 4), w - 1)
+        pad = img[ys][:, xs]
+        blocks.append(pad.reshape(bh, 4, bw, 4, 4).transpose(0, 2, 1, 3, 4).reshape(-1, 64))
+        desc += [first, bw, bh]
+        first += bw * bh
+    return np.ascontiguousarray(np.concatenate(blocks), np.uint8), np.array(desc, np.uint32)
+
+
+def profile(on):
+    """Split timing (crn2_qdxt_profile): start with on=True; on=False returns
+    (ms [3], launches [3], vector passes [3]) for the 2-, 6- and 16-D trees."""
+    ms = (C.c_double * 3)()
+    la = (C.c_uint64 * 3)()
+    pa = (C.c_uint64 * 3)()
+    check(lib().crn2_qdxt_profile(int(bool(on)), ms, la, pa), "crn2_qdxt_profile")
+    return list(ms), list(la), list(pa)
+
+
 def params(quality_level=128, dxt_quality=UBER, flags=PERCEPTUAL | HIERARCHICAL, alpha_threshold=128):
     return Qdxt1Params(quality_level, dxt_quality, flags, alpha_threshold)
 

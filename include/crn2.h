@@ -727,6 +727,13 @@ int crn2_qdxt5_info(const crn2_qdxt5* q, uint32_t* codebook_size, uint32_t* max_
  * tile encodings; any pointer may be NULL. */
 int crn2_qdxt5_training(const crn2_qdxt5* q, float* vecs, uint32_t* weights, uint8_t* encodings);
 int crn2_qdxt5_destroy(crn2_qdxt5* q);
+/* Stage timing of the clusterizer's tree splits (no reference counterpart):
+ * on != 0 resets and starts it (HIP events around every split launch, a
+ * device counter of vector passes); on == 0 stops it and writes per tree
+ * dimension (2, 6, 16) the summed launch ms, launches and vector passes (one
+ * vector read by one pass of a split: covariance, PCA sides, each 2-means
+ * round, the partition).  Any output pointer may be NULL. */
+int crn2_qdxt_profile(int on, double* ms, uint64_t* launches, uint64_t* passes);
 /* clusterizer<vec<dim,float>> (crnlib/crn_clusterizer.h) over n vectors in
  * device memory (dim 2, 6 or 16; weights u32): grown best first to max_size
  * leaves, then retrieve_clusters(max_clusters; 0 = every leaf).  idx (host,

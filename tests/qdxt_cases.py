@@ -1,25 +1,8 @@
 """Inputs for the qdxt tests: the pixel blocks of a texture's levels, laid
 out as mipmapped_texture::qdxt_pack_init does (crnlib/crn_mipmapped_texture.cpp:
-2410-2469): every level's 4x4 blocks in raster order, texels past the edge
-clamped, levels one after another, with (first block, width, height in
-blocks) per level."""
-import numpy as np
-
+2410-2469; crunch2_amd.qdxt.level_blocks)."""
 from crunch2_amd import texture
-
-
-def level_blocks(levels):
-    blocks, desc, first = [], [], 0
-    for img in levels:
-        h, w = img.shape[:2]
-        bw, bh = (w + 3) // 4, (h + 3) // 4
-        ys = np.minimum(np.arange(bh * 4), h - 1)
-        xs = np.minimum(np.arange(bw * 4), w - 1)
-        pad = img[ys][:, xs]
-        blocks.append(pad.reshape(bh, 4, bw, 4, 4).transpose(0, 2, 1, 3, 4).reshape(-1, 64))
-        desc += [first, bw, bh]
-        first += bw * bh
-    return np.ascontiguousarray(np.concatenate(blocks), np.uint8), np.array(desc, np.uint32)
+from crunch2_amd.qdxt import level_blocks
 
 
 def texture_blocks(size, seed, kind=texture.RGB, mips=True, height=None):
