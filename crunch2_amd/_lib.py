@@ -217,6 +217,9 @@ SIGNATURES = {
     "crn2_qdxt5_info": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_uint32)] * 4),
     "crn2_qdxt5_training": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "crn2_qdxt5_destroy": (C.c_int, [C.c_void_p]),
+    "crn2_compress_dds_clustered": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                              C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_uint32,
+                                              C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_uint32)]),
     "crn2_qdxt_profile": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "crn2_vq_clusters_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),

@@ -351,10 +351,20 @@ void* compress_levels(const crn_comp_params& p, crn_uint32 levels, crn_uint32& c
     uint32_t size = 0;
     int rc;
     if (p.m_file_type == cCRNFileTypeDDS) {
-        // regular DDS only: a quality below the maximum or a bitrate target asks for clustered DDS
-        if (p.m_quality_level < cCRNMaxQualityLevel || p.m_target_bitrate > 0.0f) return nullptr;
-        rc = crn2_compress_dds(lv.data(), p.m_width, p.m_height, levels, p.m_faces, (uint32_t)p.m_format,
-                               (int)p.m_dxt_quality, encoder_flags(p), &out, &size);
+        // a bitrate target (dds_comp's quality search, crn_dds_comp.cpp:279-370) is not covered
+        if (p.m_target_bitrate > 0.0f) return nullptr;
+        if (p.m_quality_level < cCRNMaxQualityLevel) {
+            // clustered DDS (crn_dds_comp.cpp:158-205)
+            const int f = (p.get_flag(cCRNCompFlagPerceptual) ? 1 : 0) |
+                          (p.get_flag(cCRNCompFlagUseBothBlockTypes) ? 2 : 0) |
+                          (p.get_flag(cCRNCompFlagHierarchical) ? 4 : 0);
+            rc = crn2_compress_dds_clustered(lv.data(), p.m_width, p.m_height, levels, p.m_faces,
+                                             (uint32_t)p.m_format, p.m_quality_level, (int)p.m_dxt_quality, f,
+                                             p.m_dxt1a_alpha_threshold, &out, &size);
+        } else {
+            rc = crn2_compress_dds(lv.data(), p.m_width, p.m_height, levels, p.m_faces, (uint32_t)p.m_format,
+                                   (int)p.m_dxt_quality, encoder_flags(p), &out, &size);
+        }
     } else {
         uint32_t cb[4];
         palette_sizes(p, cb);

@@ -2722,3 +2722,131 @@ extern "C" int crn2_qdxt_profile(int on, double* ms, uint64_t* launches, uint64_
     }
     return CRN2_OK;
 }
+
+// ------------------------------------------------- clustered DDS (crn_compress)
+// dds_comp::convert_to_dxt below the top quality (crnlib/crn_dds_comp.cpp:
+// 148-205) through mipmapped_texture::qdxt_pack_init / qdxt_pack
+// (crn_mipmapped_texture.cpp:2310-2560): the pixel blocks of every face and
+// level (faces outer, texels past the edge clamped) packed by qdxt1 (DXT1,
+// DXT5's colour) and qdxt5 (DXT5's alpha, DXT5A, the two DXN channels), the
+// elements interleaved as qdxt_pack does and written as a DDS file.
+extern "C" int crn2_compress_dds_clustered(const uint8_t* const* levels, uint32_t width, uint32_t height,
+                                           uint32_t nlevels, uint32_t faces, uint32_t format,
+                                           uint32_t quality_level, int dxt_quality, int flags,
+                                           uint32_t alpha_threshold, uint8_t** out, uint32_t* out_size)
+{
+    if (!out || !out_size) return CRN2_EINVAL;
+    *out = nullptr;
+    *out_size = 0;
+    if (!levels || !width || !height || !nlevels || nlevels > 16 || (faces != 1 && faces != 6) ||
+        quality_level > 255 || dxt_quality < 0 || dxt_quality > 4 || (flags & ~7))
+        return CRN2_EINVAL;
+    const bool dxt1 = format == CRN2_FMT_DXT1, dxt5 = format == CRN2_FMT_DXT5, dxt5a = format == CRN2_FMT_DXT5A;
+    const bool dxn = format == CRN2_FMT_DXN_XY || format == CRN2_FMT_DXN_YX;
+    if (!dxt1 && !dxt5 && !dxt5a && !dxn) return CRN2_EUNSUPPORTED;
+    const int perceptual = flags & 1, both = (flags >> 1) & 1, hier = (flags >> 2) & 1;
+    // blocks and level descriptors (qdxt_pack_init, :2410-2469)
+    std::vector<uint32_t> blocks;
+    std::vector<uint32_t> desc;
+    std::vector<uint32_t> lcount;
+    for (uint32_t f = 0; f < faces; f++)
+        for (uint32_t l = 0; l < nlevels; l++) {
+            const uint8_t* img = levels[f * nlevels + l];
+            if (!img) return CRN2_EINVAL;
+            const uint32_t w = std::max<uint32_t>(1, width >> l), h = std::max<uint32_t>(1, height >> l);
+            const uint32_t bw = (w + 3) / 4, bh = (h + 3) / 4;
+            desc.push_back((uint32_t)(blocks.size() / 16));
+            desc.push_back(bw);
+            desc.push_back(bh);
+            lcount.push_back(bw * bh);
+            for (uint32_t by = 0; by < bh; by++)
+                for (uint32_t bx = 0; bx < bw; bx++)
+                    for (uint32_t y = 0; y < 4; y++)
+                        for (uint32_t x = 0; x < 4; x++) {
+                            const uint32_t py = std::min(by * 4 + y, h - 1), px = std::min(bx * 4 + x, w - 1);
+                            uint32_t t;
+                            memcpy(&t, img + 4 * ((size_t)py * w + px), 4);
+                            blocks.push_back(t);
+                        }
+        }
+    const uint32_t n = (uint32_t)(blocks.size() / 16);
+    const uint32_t nl = (uint32_t)lcount.size();
+    hipStream_t st = nullptr;
+    void* dblocks = nullptr;
+    unsigned long long *c = nullptr, *a0 = nullptr, *a1 = nullptr;
+    auto release = [&]() {
+        for (void* x : {(void*)dblocks, (void*)c, (void*)a0, (void*)a1})
+            if (x) (void)hipFree(x);
+    };
+    if (hipMalloc(&dblocks, blocks.size() * 4) != hipSuccess ||
+        hipMemcpy(dblocks, blocks.data(), blocks.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        release();
+        return CRN2_ENOMEM;
+    }
+    int rc = CRN2_OK;
+    // colour (qdxt_pack, :2528-2544: codebook curve 0.75 for DXT5's colour)
+    if (dxt1 || dxt5) {
+        crn2_qdxt1* q = nullptr;
+        rc = hipMalloc((void**)&c, (size_t)n * 8) == hipSuccess ? CRN2_OK : CRN2_ENOMEM;
+        if (rc == CRN2_OK) rc = crn2_qdxt1_init(dblocks, n, nl, desc.data(), hier, &q, st);
+        if (rc == CRN2_OK) {
+            crn2_qdxt1_params p;
+            p.quality_level = quality_level;
+            p.dxt_quality = dxt_quality;
+            p.flags = perceptual | ((dxt1 && both) ? CRN2_QDXT_ALPHA_BLOCKS : 0) | (hier ? CRN2_QDXT_HIERARCHICAL : 0);
+            p.dxt1a_alpha_threshold = alpha_threshold;
+            rc = crn2_qdxt1_pack(q, &p, dxt5 ? 0.75f : 1.0f, c, st);
+        }
+        if (q) crn2_qdxt1_destroy(q);
+    }
+    // alpha channels (qdxt_pack_init :2318-2365)
+    const uint32_t comps[2] = {dxn ? (format == CRN2_FMT_DXN_XY ? 0u : 1u) : 3u,
+                               format == CRN2_FMT_DXN_XY ? 1u : 0u};
+    const int nalpha = dxn ? 2 : ((dxt5 || dxt5a) ? 1 : 0);
+    for (int k = 0; k < nalpha && rc == CRN2_OK; k++) {
+        unsigned long long*& dst = k ? a1 : a0;
+        crn2_qdxt5* q = nullptr;
+        rc = hipMalloc((void**)&dst, (size_t)n * 8) == hipSuccess ? CRN2_OK : CRN2_ENOMEM;
+        if (rc == CRN2_OK) rc = crn2_qdxt5_init(dblocks, n, nl, desc.data(), hier, comps[k], &q, st);
+        if (rc == CRN2_OK) {
+            crn2_qdxt5_params p;
+            p.quality_level = quality_level;
+            p.dxt_quality = dxt_quality;
+            p.flags = both ? CRN2_QDXT5_BOTH_BLOCK_TYPES : 0;
+            rc = crn2_qdxt5_pack(q, &p, dst, st);
+        }
+        if (q) crn2_qdxt5_destroy(q);
+    }
+    std::vector<unsigned long long> hc, ha0, ha1;
+    if (rc == CRN2_OK && c) {
+        hc.resize(n);
+        if (hipMemcpy(hc.data(), c, (size_t)n * 8, hipMemcpyDeviceToHost) != hipSuccess) rc = CRN2_EHIP;
+    }
+    if (rc == CRN2_OK && a0) {
+        ha0.resize(n);
+        if (hipMemcpy(ha0.data(), a0, (size_t)n * 8, hipMemcpyDeviceToHost) != hipSuccess) rc = CRN2_EHIP;
+    }
+    if (rc == CRN2_OK && a1) {
+        ha1.resize(n);
+        if (hipMemcpy(ha1.data(), a1, (size_t)n * 8, hipMemcpyDeviceToHost) != hipSuccess) rc = CRN2_EHIP;
+    }
+    release();
+    if (rc != CRN2_OK) return rc;
+    // elements per block as qdxt_pack interleaves them (:2536-2553)
+    const uint32_t per = (uint32_t)(c ? 1 : 0) + (uint32_t)nalpha;
+    std::vector<std::vector<unsigned long long>> data(nl);
+    std::vector<const void*> ptrs(nl);
+    for (uint32_t i = 0, b0 = 0; i < nl; i++) {
+        data[i].resize((size_t)lcount[i] * per);
+        for (uint32_t b = 0; b < lcount[i]; b++) {
+            unsigned long long* e = &data[i][(size_t)b * per];
+            uint32_t k = 0;
+            if (nalpha) e[k++] = ha0[b0 + b];
+            if (nalpha > 1) e[k++] = ha1[b0 + b];
+            if (c) e[k++] = hc[b0 + b];
+        }
+        b0 += lcount[i];
+        ptrs[i] = data[i].data();
+    }
+    return crn2_write_dds(width, height, nlevels, faces, format, ptrs.data(), out, out_size);
+}

@@ -592,6 +592,18 @@ int crn2_compress_crn(const uint8_t* const* levels, uint32_t width, uint32_t hei
  * crn2_encode_image, written by crn2_write_dds. */
 int crn2_compress_dds(const uint8_t* const* levels, uint32_t width, uint32_t height, uint32_t nlevels,
                       uint32_t faces, uint32_t format, int quality, int flags, uint8_t** out, uint32_t* out_size);
+/* The clustered DDS path of crn_compress below quality 255 (dds_comp::
+ * convert_to_dxt, crnlib/crn_dds_comp.cpp:148-205; qdxt_pack_init /
+ * qdxt_pack, crn_mipmapped_texture.cpp:2310-2560): every face's levels
+ * through crn2_qdxt1_* (DXT1, DXT5's colour with the 0.75 codebook curve)
+ * and crn2_qdxt5_* (DXT5's alpha, DXT5A, DXN's two channels), written by
+ * crn2_write_dds.  flags: bit0 perceptual, bit1 both block types, bit2
+ * hierarchical.  Formats DXT1, DXT5, DXT5A, DXN_XY, DXN_YX
+ * (CRN2_EUNSUPPORTED otherwise, and where crn2_qdxt1_pack is).  *out is
+ * freed with crn2_free. */
+int crn2_compress_dds_clustered(const uint8_t* const* levels, uint32_t width, uint32_t height, uint32_t nlevels,
+                                uint32_t faces, uint32_t format, uint32_t quality_level, int dxt_quality, int flags,
+                                uint32_t alpha_threshold, uint8_t** out, uint32_t* out_size);
 /* A DDS file as mipmapped_texture::write_dds writes a DXTn texture
  * (crnlib/crn_mipmapped_texture.cpp:921-1140): level_data[f * levels + l] =
  * that level's blocks at minimal pitch.  *out is freed with crn2_free. */

@@ -15,9 +15,11 @@
  *       reference's writer).  With crn_mipmap_params, levels are generated
  *       on the GPU as mipmapped_texture::generate_mipmaps makes them
  *       (crn2_generate_mipmaps, bit-exact) when the mode asks for it.
- *       Clustered DDS (quality < 255 or a target bitrate with DDS output),
- *       ETC formats, DXT3 in CRN, scaling, cropping and a renormalized top
- *       level return NULL.
+ *       DDS below quality 255 is the clustered path (qdxt1 / qdxt5 on the
+ *       GPU, crn2_compress_dds_clustered) for DXT1, DXT5, DXT5A and DXN at
+ *       uber quality (DXT1A transparency in a cluster excepted).  A target
+ *       bitrate with DDS output, ETC formats, DXT3 in CRN, scaling, cropping
+ *       and a renormalized top level return NULL.
  *   crn_decompress_crn_to_dds ........ GPU transcode of every level and face
  *       into a DDS file laid out as mipmapped_texture::write_dds writes it
  *       (crnlib/crn_mipmapped_texture.cpp:921-1140).

@@ -336,3 +336,31 @@ def test_crn_compress_generates_mips_like_generate_mipmaps(gpu, case):
     p2 = comp_params(96, 64, len(levels), 0, 1)
     exp = compress(p2, levels)
     assert raw == exp
+
+
+@pytest.mark.parametrize("fmt", [0, 2, 7])
+def test_crn_compress_clustered_dds_matches_the_oracle(gpu, fmt):
+    # DDS below quality 255: qdxt1 / qdxt5 (crn_dds_comp.cpp:158-205) with the
+    # elements interleaved as qdxt_pack does (crn_mipmapped_texture.cpp:2536-2553)
+    kind = texture.RGB if fmt == 0 else (texture.RGBA if fmt == 2 else texture.NORMAL)
+    img = texture.synth(96, 64, 13 + fmt, kind)
+    chain = texture.mip_chain(img, 4)
+    p = comp_params(96, 64, 4, fmt, 1)
+    p.quality_level = 100
+    raw = compress(p, chain)
+    assert raw is not None and raw[:4] == b"DDS "
+    from crunch2_amd.qdxt import level_blocks
+    blocks, desc = level_blocks(chain)
+    P, A, H = oracle.QDXT_PERCEPTUAL, oracle.QDXT_ALPHA_BLOCKS, oracle.QDXT_HIERARCHICAL
+    elems = []
+    if fmt == 0:
+        elems.append(oracle.qdxt1(blocks, desc, 100, 4, P | A | H, 128, 1.0)[0])
+    elif fmt == 2:
+        elems.append(oracle.qdxt5(blocks, desc, 100, 4, oracle.QDXT5_BOTH | H, 3)[0])
+        elems.append(oracle.qdxt1(blocks, desc, 100, 4, P | H, 128, 0.75)[0])
+    else:
+        elems.append(oracle.qdxt5(blocks, desc, 100, 4, oracle.QDXT5_BOTH | H, 0)[0])
+        elems.append(oracle.qdxt5(blocks, desc, 100, 4, oracle.QDXT5_BOTH | H, 1)[0])
+    exp = np.stack(elems, 1).reshape(-1)
+    got = np.frombuffer(raw[128:], np.uint64)
+    assert got.shape == exp.shape and np.array_equal(got, exp)
