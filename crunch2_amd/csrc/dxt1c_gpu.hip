@@ -97,6 +97,7 @@ struct Best {
 
 struct St {
     int perceptual;
+    int have_alpha;      // DXT1A: texels below the alpha threshold are transparent (3-colour blocks only)
     uint32_t nu, total_w;
     float mean[3], meanw[3], pw[3], axis[3];
     Best best;
@@ -261,7 +262,7 @@ __device__ bool eval_uber(St& s, const Lds& L, uint32_t lo, uint32_t hi, int alt
     uint64_t terr = s.best.err;
     int talpha = 0;
     bool took = false;
-    if (e4 < terr) {
+    if (!s.have_alpha && e4 < terr) {
         terr = e4;
         took = true;
     }
@@ -953,7 +954,7 @@ __device__ void optimize_endpoints(St& s, Lds& L, float* lo, float* hi)
         if (!s.best.err || (pass && s.best.err == prev)) break;
         refine(s, L, 1);
     }
-    if (s.best.err) {
+    if (s.best.err && !s.have_alpha) {
         bool solid = false;
         // the uber evaluator keeps the best solution's selectors current
         if (Lds::kUber && s.sel_nonempty) compute_selectors(s, L);
@@ -1107,17 +1108,27 @@ __device__ inline uint32_t unique_index(const LdsS<MaxU>& L, uint32_t i, uint32_
 // of equal colours marked in a bitmap, a colour's rank = the number of marked
 // texels before its first one.  One wave; the block is the wave.
 template <uint32_t MaxU>
-__device__ uint32_t unique_sorted(LdsS<MaxU>& L, const uint32_t* __restrict__ px, uint32_t p0, uint32_t n)
+__device__ uint32_t unique_sorted(LdsS<MaxU>& L, const uint32_t* __restrict__ px, uint32_t p0, uint32_t n,
+                              uint32_t thresh, uint32_t& opaque)
 {
     uint32_t* key = reinterpret_cast<uint32_t*>(L.sel);
     uint16_t* idx = reinterpret_cast<uint16_t*>(key + MaxU);
     const uint32_t lane = lane_id();
     uint32_t P = 64;
     while (P < n) P <<= 1;   // n <= MaxU, a power of two
+    uint32_t nop = 0;
     for (uint32_t i = lane; i < P; i += 64) {
-        key[i] = i < n ? (px[p0 + i] & 0x00FFFFFFu) : 0xFFFFFFFFu;
+        // transparent texels (DXT1A) sort after every colour, before the padding
+        const uint32_t p = i < n ? px[p0 + i] : 0u;
+        const bool tr = i < n && (p >> 24) < thresh;
+        key[i] = i < n ? (tr ? 0xFFFFFFFEu : (p & 0x00FFFFFFu)) : 0xFFFFFFFFu;
         idx[i] = (uint16_t)i;
+        nop += (i < n && !tr) ? 1u : 0u;
     }
+    for (int off = 32; off; off >>= 1) nop += (uint32_t)__shfl_xor((int)nop, off);
+    opaque = nop;
+    const uint32_t nsrc = n;
+    n = nop;
     for (uint32_t i = lane; i < MaxU / 32; i += 64) L.fbits[i] = 0;
     __syncthreads();
     for (uint32_t k = 2; k <= P; k <<= 1)
@@ -1140,7 +1151,7 @@ __device__ uint32_t unique_sorted(LdsS<MaxU>& L, const uint32_t* __restrict__ px
     for (uint32_t j = lane; j < n; j += 64)
         if (j == 0 || key[j] != key[j - 1]) atomicOr(&L.fbits[idx[j] >> 5], 1u << (idx[j] & 31u));
     __syncthreads();
-    const uint32_t nw = (n + 31) / 32;   // at most 64 words: one per lane
+    const uint32_t nw = (nsrc + 31) / 32;   // first-texel bits of all texels: at most 64 words, one per lane
     const uint32_t cnt = lane < nw ? (uint32_t)__popc(L.fbits[lane]) : 0u;
     uint32_t incl = cnt;
     for (uint32_t off = 1; off < 64; off <<= 1) {
@@ -1209,11 +1220,23 @@ __global__ void __launch_bounds__(256) dxt1_cluster_classes_kernel(const uint32_
 // return_solution, for either colour store.
 template <class Lds>
 __device__ void cluster_search(St& s, Lds& L, const uint32_t* __restrict__ px, uint32_t p0, uint32_t n,
-                               int perceptual, uint32_t cl, ClusterOut* __restrict__ out,
+                               int flags, uint32_t cl, ClusterOut* __restrict__ out,
                                uint8_t* __restrict__ selectors)
 {
     const uint32_t lane = lane_id();
     ClusterOut o{};
+    const uint32_t thresh = s.have_alpha ? ((uint32_t)flags >> 8) & 255u : 0u;
+    if (!s.nu) {
+        // no opaque texel (compute_internal, crn_dxt1.cpp:2207-2212): a 3-colour
+        // block of zero endpoints, every selector transparent (error 0 here)
+        if (selectors)
+            for (uint32_t i = lane; i < n; i += 64) selectors[p0 + i] = 3;
+        if (lane == 0) {
+            o.flags = 2;
+            out[cl] = o;
+        }
+        return;
+    }
     for (uint32_t i = lane; i < s.nu; i += 64) {
         const uint32_t c = L.col[i];
         const unsigned long long w = L.w[i];
@@ -1227,7 +1250,7 @@ __device__ void cluster_search(St& s, Lds& L, const uint32_t* __restrict__ px, u
     }
     __syncthreads();
     if (lane == 0) {
-        const uint64_t sc[3] = {perceptual ? 8u : 1u, perceptual ? 25u : 1u, 1u};
+        const uint64_t sc[3] = {s.perceptual ? 8u : 1u, s.perceptual ? 25u : 1u, 1u};
         for (int ch = 0; ch < 3; ch++) {
             const int lim = ch == 1 ? 64 : 32;
             for (int c = 0; c < lim; c++)
@@ -1255,7 +1278,7 @@ __device__ void cluster_search(St& s, Lds& L, const uint32_t* __restrict__ px, u
     s.best.alt = s.best.enforce = s.best.enforced = s.best.alpha = 0;
     s.sel_valid = false;
     s.sel_nonempty = false;
-    if (s.nu == 1) {
+    if (s.nu == 1 && !s.have_alpha) {
         const uint32_t c = L.col[0];
         const uint32_t r = c & 255u, g = (c >> 8) & 255u, b = (c >> 16) & 255u;
         evaluate(s, L, (g_c_om5[r][0] << 11) | (g_c_om6[g][0] << 5) | g_c_om5[b][0],
@@ -1274,6 +1297,10 @@ __device__ void cluster_search(St& s, Lds& L, const uint32_t* __restrict__ px, u
     const uint32_t lo = inv ? s.best.hi : s.best.lo, hi = inv ? s.best.lo : s.best.hi;
     if (selectors) {
         for (uint32_t i = lane; i < n; i += 64) {
+            if ((px[p0 + i] >> 24) < thresh) {
+                selectors[p0 + i] = 3;   // transparent (return_solution, crn_dxt1.cpp:300-343)
+                continue;
+            }
             const uint32_t key = (px[p0 + i] & 0x00FFFFFFu) | 0xFF000000u;
             uint32_t v = L.sel[unique_index(L, i, key)];
             if (inv) v = s.best.alpha ? (v < 2 ? v ^ 1u : v) : (v ^ 1u);
@@ -1306,6 +1333,8 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
     const uint32_t lane = lane_id();
     const uint32_t p0 = ofs[cl], p1 = ofs[cl + 1];
     const uint32_t n = p1 - p0;
+    // DXT1A (qdxt1 with 3-colour blocks): texels with alpha below this are transparent
+    const uint32_t thresh = Uber ? ((uint32_t)perceptual >> 8) & 255u : 0u;
     ClusterOut o{};
     if (!n) {
         if (lane == 0) out[cl] = o;
@@ -1315,11 +1344,13 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
         // this class holds at most MaxU texels (dxt1_cluster_classes_kernel)
         St s;
         s.perceptual = perceptual & 1;
-        s.total_w = n;
-        s.nu = unique_sorted(L, px, p0, n);
+        uint32_t opaque = n;
+        s.nu = unique_sorted(L, px, p0, n, thresh, opaque);
+        s.total_w = opaque;
+        s.have_alpha = opaque != n;
         for (uint32_t i = lane; i < 3 * 64 * 3; i += 64) (&L.pl[0][0][0])[i] = 0;   // over the sort buffers
         __syncthreads();
-        cluster_search(s, L, px, p0, n, perceptual & 1, cl, out, selectors);
+        cluster_search(s, L, px, p0, n, perceptual, cl, out, selectors);
         return;
     }
     else {
@@ -1331,10 +1362,12 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
     St s;
     s.perceptual = perceptual & 1;
     s.nu = 0;
-    s.total_w = n;
+    uint32_t opaque = 0;
     bool overflow = false;
     if (lane == 0) {
         for (uint32_t i = 0; i < n; i++) {
+            if ((px[p0 + i] >> 24) < thresh) continue;   // transparent (DXT1A)
+            opaque++;
             const uint32_t key = (px[p0 + i] & 0x00FFFFFFu) | 0xFF000000u;
             uint32_t h = hash_slot(L, key);
             while (L.hkey[h] && L.hkey[h] != key) h = hash_next(L, h);
@@ -1354,6 +1387,8 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
     }
     __builtin_amdgcn_wave_barrier();
     s.nu = (uint32_t)__shfl((int)s.nu, 0);
+    s.total_w = (uint32_t)__shfl((int)opaque, 0);
+    s.have_alpha = s.total_w != n;
     overflow = __shfl((int)overflow, 0) != 0;
     if (overflow) {
         // more distinct colours than this store holds: left to dxt1_cluster_big_kernel
@@ -1361,7 +1396,7 @@ __global__ void __launch_bounds__(64) dxt1_cluster_kernel(const uint32_t* __rest
         if (lane == 0) out[cl] = o;
         return;
     }
-    cluster_search(s, L, px, p0, n, perceptual & 1, cl, out, selectors);
+    cluster_search(s, L, px, p0, n, perceptual, cl, out, selectors);
     }
 }
 
@@ -1414,7 +1449,11 @@ __global__ void __launch_bounds__(64) dxt1_cluster_big_kernel(const uint32_t* __
     for (uint32_t i = lane; i < 3 * 64 * 3; i += 64) (&T.pl[0][0][0])[i] = 0;
     __threadfence();
     __syncthreads();
+    const uint32_t thresh = Uber ? ((uint32_t)perceptual >> 8) & 255u : 0u;
+    uint32_t opaque = 0;
     for (uint32_t i = lane; i < n; i += 64) {
+        if ((px[p0 + i] >> 24) < thresh) continue;   // transparent (DXT1A)
+        opaque++;
         const uint32_t key = (px[p0 + i] & 0x00FFFFFFu) | 0xFF000000u;
         uint32_t h = hash_slot(L, key);
         for (;;) {
@@ -1452,11 +1491,13 @@ __global__ void __launch_bounds__(64) dxt1_cluster_big_kernel(const uint32_t* __
     }
     __threadfence();
     __syncthreads();
+    for (int off = 32; off; off >>= 1) opaque += (uint32_t)__shfl_xor((int)opaque, off);
     St s;
     s.perceptual = perceptual & 1;
     s.nu = base_rank;
-    s.total_w = n;
-    cluster_search(s, L, px, p0, n, perceptual & 1, cl, out, selectors);
+    s.total_w = opaque;
+    s.have_alpha = opaque != n;
+    cluster_search(s, L, px, p0, n, perceptual, cl, out, selectors);
 }
 
 }  // namespace
@@ -1625,7 +1666,7 @@ int clusters_run(const void* rgba, const uint32_t* offsets, uint32_t nclusters, 
         cls[1] = side[2];
     }
     if (flags & 2)
-        launch_classes<true>(px, offsets, nclusters, list, work, flags & 1, o, selectors, cls);
+        launch_classes<true>(px, offsets, nclusters, list, work, flags & 0xFF01, o, selectors, cls);
     else
         launch_classes<false>(px, offsets, nclusters, list, work, flags & 1, o, selectors, cls);
     if (hipGetLastError() != hipSuccess) rc = CRN2_EHIP;
@@ -1652,7 +1693,7 @@ int clusters_big_pass(const void* rgba, const uint32_t* offsets, uint32_t nclust
     if (hipMallocAsync((void**)&big, 37 * (size_t)total, st) != hipSuccess) return CRN2_ENOMEM;
     if (flags & 2)
         hipLaunchKernelGGL(dxt1_cluster_big_kernel<true>, dim3(nclusters), dim3(64), 0, st,
-                           static_cast<const uint32_t*>(rgba), offsets, nclusters, flags & 1,
+                           static_cast<const uint32_t*>(rgba), offsets, nclusters, flags & 0xFF01,
                            static_cast<ClusterOut*>(out), selectors, big);
     else
         hipLaunchKernelGGL(dxt1_cluster_big_kernel<false>, dim3(nclusters), dim3(64), 0, st,
@@ -1667,7 +1708,7 @@ int clusters_big_pass(const void* rgba, const uint32_t* offsets, uint32_t nclust
 extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters, int flags,
                                          void* out, uint8_t* selectors, void* stream)
 {
-    if (!rgba || !offsets || !out || (flags & ~3)) return CRN2_EINVAL;
+    if (!rgba || !offsets || !out || (flags & ~0xFF03) || ((flags & 0xFF00) && !(flags & 2))) return CRN2_EINVAL;
     if (!nclusters) return CRN2_OK;
     int dev = 0;
     int rc = clusters_init(&dev);
@@ -1685,7 +1726,9 @@ extern "C" int crn2_dxt1_clusters_device(const void* rgba, const uint32_t* offse
 extern "C" int crn2_dxt1_clusters_noalloc_device(const void* rgba, const uint32_t* offsets, uint32_t nclusters,
                                                  int flags, void* out, uint8_t* selectors, void* stream)
 {
-    if (!rgba || !offsets || !out || (flags & ~3) || nclusters > kNoAllocMaxClusters) return CRN2_EINVAL;
+    if (!rgba || !offsets || !out || (flags & ~0xFF03) || ((flags & 0xFF00) && !(flags & 2)) ||
+        nclusters > kNoAllocMaxClusters)
+        return CRN2_EINVAL;
     if (!nclusters) return CRN2_OK;
     int dev = 0;
     const int rc = clusters_init(&dev);

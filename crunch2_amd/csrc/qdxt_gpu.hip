@@ -1037,19 +1037,6 @@ __global__ void __launch_bounds__(256) seg_copy_kernel(const uint32_t* __restric
 }
 
 // ------------------------------------------------------------ pack kernels
-__global__ void clear_flag_kernel(uint32_t* __restrict__ flag)
-{
-    if (threadIdx.x == 0) *flag = 0;
-}
-
-// Sets *flag when a texel's alpha is below thresh (qdxt1's has_alpha_pixels,
-// crn_qdxt1.cpp:593-603).
-__global__ void __launch_bounds__(256) alpha_below_kernel(const uint32_t* __restrict__ texels, uint32_t n,
-                                                          uint32_t thresh, uint32_t* __restrict__ flag)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && (texels[i] >> 24) < thresh) atomicOr(flag, 1u);
-}
 // Texels of the listed blocks, 16 per block, in list order.
 __global__ void __launch_bounds__(256) gather_texels_kernel(const uint32_t* __restrict__ blocks,
                                                             const uint32_t* __restrict__ idx, uint32_t n,
@@ -2232,24 +2219,11 @@ extern "C" int crn2_qdxt1_pack(crn2_qdxt1* q, const crn2_qdxt1_params* p, float 
     const float quality = p->quality_level / (float)255;
     // multi-block clusters go through the cluster endpoint search at uber
     // quality: the hc evaluators without 3-colour blocks (DXT5's colour part,
-    // DXT1 without both block types), the uber evaluator with them; DXT1A
-    // transparency in a cluster is not covered there
+    // DXT1 without both block types), the uber evaluator with them (and
+    // DXT1A transparency in a cluster, crn_qdxt1.cpp:590-626)
     if (quality < 1.0f && p->dxt_quality != 4) return CRN2_EUNSUPPORTED;
     if (quality >= 1.0f && p->dxt_quality == 0) return CRN2_EUNSUPPORTED;
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (quality < 1.0f && thresh) {
-        uint32_t* flag = nullptr;
-        uint32_t any = 0;
-        if (hipMallocAsync((void**)&flag, 4, st) != hipSuccess) return CRN2_ENOMEM;
-        hipLaunchKernelGGL(clear_flag_kernel, dim3(1), dim3(64), 0, st, flag);
-        hipLaunchKernelGGL(alpha_below_kernel, dim3(grid((uint64_t)q->n * 16, 256)), dim3(256), 0, st, q->blocks,
-                           q->n * 16, thresh, flag);
-        const bool ok = hipMemcpyAsync(&any, flag, 4, hipMemcpyDeviceToHost, st) == hipSuccess &&
-                        hipStreamSynchronize(st) == hipSuccess;
-        (void)hipFreeAsync(flag, st);
-        if (!ok) return CRN2_EHIP;
-        if (any) return CRN2_EUNSUPPORTED;
-    }
     q->F.st = st;
     const float eq = powf(quality, 1.8f * quality_power_mul), sq = powf(quality, 1.65f * quality_power_mul);
     const uint32_t cbs = q->tree.leaves;
@@ -2314,7 +2288,7 @@ extern "C" int crn2_qdxt1_pack(crn2_qdxt1* q, const crn2_qdxt1_params* p, float 
     }
     hipLaunchKernelGGL(gather_texels_kernel, dim3(grid((uint64_t)n * 16, 256)), dim3(256), 0, st, q->blocks, idx, n,
                        px);
-    rc = crn2_dxt1_clusters_device(px, tofs, nc, perceptual | (use_alpha << 1), rec, sel, st);
+    rc = crn2_dxt1_clusters_device(px, tofs, nc, perceptual | (use_alpha << 1) | (int)(thresh << 8), rec, sel, st);
     if (rc != CRN2_OK) {
         cleanup();
         return rc;

@@ -64,18 +64,19 @@ def image_blocks(img):
 CLUSTER_DTYPE = np.dtype([("error", "<u8"), ("lo_hi", "<u4"), ("nunique", "<u4"), ("flags", "<u4"), ("pad", "<u4")])
 
 
-def encode_clusters_device(rgba_px, offsets, perceptual=True, selectors=True, alpha_blocks=False):
+def encode_clusters_device(rgba_px, offsets, perceptual=True, selectors=True, alpha_blocks=False, alpha_threshold=0):
     """DXT1 endpoint search per pixel cluster at uber quality: the hc
     evaluators (dxt_hc endpoint clusters), or with alpha_blocks the uber
-    evaluator with 3-colour blocks (qdxt1's clusters).
+    evaluator with 3-colour blocks (qdxt1's clusters), where texels with alpha
+    below alpha_threshold are transparent (DXT1A, selector 3).
     rgba_px: uint8 CUDA tensor [n, 4]; offsets: int32 CUDA tensor [k + 1].
     Returns (uint8 [k, 24] packed results, uint8 [n] selectors or None)."""
     k = offsets.shape[0] - 1
     out = torch.empty((k, 24), dtype=torch.uint8, device=rgba_px.device)
     sel = torch.empty(rgba_px.shape[0], dtype=torch.uint8, device=rgba_px.device) if selectors else None
     s = torch.cuda.current_stream().cuda_stream
-    check(lib().crn2_dxt1_clusters_device(C.c_void_p(rgba_px.data_ptr()), C.c_void_p(offsets.data_ptr()), k,
-                                          int(bool(perceptual)) | (2 if alpha_blocks else 0),
+    flags = int(bool(perceptual)) | (2 | threshold(alpha_threshold) if alpha_blocks else 0)
+    check(lib().crn2_dxt1_clusters_device(C.c_void_p(rgba_px.data_ptr()), C.c_void_p(offsets.data_ptr()), k, flags,
                                           C.c_void_p(out.data_ptr()),
                                           C.c_void_p(sel.data_ptr() if sel is not None else 0), C.c_void_p(s)),
           "crn2_dxt1_clusters_device")

@@ -272,7 +272,9 @@ int crn2_decompress_blocks(const void* blocks, uint32_t nblocks, uint32_t format
  * clusters' texels (RGBA8 words, alpha ignored), cluster c owning texels
  * [offsets[c], offsets[c+1]); flags bit0 perceptual, bit1 m_use_alpha_blocks
  * (the uber evaluator with 3-colour blocks, crn_dxt1.cpp:1370-1592, as qdxt1
- * runs it, crn_qdxt1.cpp:616-626).  out: 24 bytes per
+ * runs it, crn_qdxt1.cpp:616-626), bits 8..15 with bit1: the DXT1A alpha
+ * threshold (a cluster with a texel below it searches its opaque texels'
+ * colours as 3-colour blocks, transparent texels on selector 3).  out: 24 bytes per
  * cluster = {u64 m_error, u32 m_low_color | m_high_color << 16, u32 unique
  * colours, u32 m_reordered | m_alpha_block << 1 | m_alternate_rounding << 2 | m_enforce_selector
  * << 3 | m_enforced_selector << 4 | 256 when the cluster has more than 4096
@@ -688,8 +690,8 @@ int crn2_mip_gamma_tables(float gamma, float* srgb_to_linear, uint8_t* linear_to
  * block (low colour, high colour, selector bytes) to out_blocks (device).
  * Endpoint caching is off in the endpoint search.  Supported: quality 255
  * at any dxt quality but superfast; below 255, uber quality with or without
- * 3-colour blocks, except DXT1A transparency (alpha blocks allowed and a
- * texel below the alpha threshold) -- CRN2_EUNSUPPORTED otherwise. */
+ * 3-colour blocks (DXT1A transparency included) -- CRN2_EUNSUPPORTED
+ * otherwise. */
 #define CRN2_QDXT_PERCEPTUAL 1
 #define CRN2_QDXT_ALPHA_BLOCKS 2   /* qdxt1_params::m_use_alpha_blocks */
 #define CRN2_QDXT_HIERARCHICAL 4   /* qdxt1_params::m_hierarchical */

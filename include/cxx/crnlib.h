@@ -17,7 +17,7 @@
  *       (crn2_generate_mipmaps, bit-exact) when the mode asks for it.
  *       DDS below quality 255 is the clustered path (qdxt1 / qdxt5 on the
  *       GPU, crn2_compress_dds_clustered) for DXT1, DXT5, DXT5A and DXN at
- *       uber quality (DXT1A transparency in a cluster excepted).  A target
+ *       uber quality.  A target
  *       bitrate with DDS output, ETC formats, DXT3 in CRN, scaling, cropping
  *       and a renormalized top level return NULL.
  *   crn_decompress_crn_to_dds ........ GPU transcode of every level and face

@@ -121,3 +121,39 @@ def test_clusters_with_three_colour_blocks_match_oracle(gpu, perceptual):
         assert ((int(r["flags"][c]) >> 1) & 1) == ores.alpha_block, c
         alpha_blocks += ores.alpha_block
     print("3-colour clusters", alpha_blocks, "of", len(parts), flush=True)
+
+
+def test_clusters_with_transparent_texels_match_oracle(gpu):
+    # DXT1A (qdxt1's clusters with a texel below the threshold): colours from
+    # the opaque texels only, 3-colour blocks, transparent texels on selector 3
+    rng = np.random.default_rng(33)
+    parts = clusters(50, [16, 48, 200, 700, 1500, 3000, 5000])
+    for k, p in enumerate(parts):
+        p = p.copy()
+        p[:, 3] = 255
+        p[rng.random(len(p)) < 0.3, 3] = rng.integers(0, 128)
+        parts[k] = p
+    allt = parts[0].copy()
+    allt[:, 3] = 0                                                              # no opaque texel
+    one = np.repeat(parts[1][:1], 64, axis=0)
+    one[::3, 3] = 5                                                             # one opaque colour
+    noisy = rng.integers(0, 256, (6000, 4), dtype=np.uint8)                     # global-memory pass
+    parts += [allt, one, noisy]
+    px = np.concatenate(parts)
+    ofs = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.int32)
+    t = torch.from_numpy(px).cuda()
+    o = torch.from_numpy(ofs).cuda()
+    out, sel = dxt1.encode_clusters_device(t, o, True, True, alpha_blocks=True, alpha_threshold=128)
+    torch.cuda.synchronize()
+    r = out.cpu().numpy().view(dxt1.CLUSTER_DTYPE).reshape(-1)
+    sel = sel.cpu().numpy()
+    for c, p in enumerate(parts):
+        have = bool((p[:, 3] < 128).any())
+        flags = oracle.DXT1_PERCEPTUAL | oracle.DXT1_ALPHA_BLOCKS | (oracle.DXT1_HAVE_ALPHA if have else 0) | \
+            oracle.dxt1_threshold(128)
+        osel, ores = oracle.dxt1_cluster(p, 4, flags)
+        lo, hi = int(r["lo_hi"][c] & 0xFFFF), int(r["lo_hi"][c] >> 16)
+        assert int(r["error"][c]) == ores.error, (c, len(p))
+        assert (lo, hi) == (ores.low, ores.high), c
+        assert np.array_equal(sel[ofs[c]:ofs[c + 1]], osel), c
+        assert ((int(r["flags"][c]) >> 1) & 1) == ores.alpha_block, c

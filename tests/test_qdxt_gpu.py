@@ -84,6 +84,8 @@ CASES = [
     (128, 19, texture.RGB, 128, P | A | H, 1.0),
     (128, 20, texture.RGB, 60, A | H, 1.0),
     (96, 22, texture.NORMAL, 200, P | A, 1.0),
+    (128, 23, texture.RGBA, 128, P | A | H, 1.0),
+    (96, 24, texture.RGBA, 50, A | H, 1.0),
 ]
 
 
@@ -116,18 +118,30 @@ def test_pack_512_chain_and_repacks(gpu):
 
 
 def test_unsupported_configurations(gpu):
-    # DXT1A transparency inside a cluster, qualities below uber under 255,
-    # superfast
+    # qualities below uber under 255, superfast
     blocks, desc = texture_blocks(32, 3, texture.RGBA)
-    assert (blocks.reshape(-1, 4)[:, 3] < 128).any()
     q = qdxt.Qdxt1(blocks, desc)
     out = torch.zeros(len(blocks), dtype=torch.int64, device="cuda")
     s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    for p in (qdxt.params(128, 4, P | A | H), qdxt.params(128, 3, P | H), qdxt.params(255, 0, P | H)):
+    for p in (qdxt.params(128, 3, P | H), qdxt.params(255, 0, P | H)):
         assert lib().crn2_qdxt1_pack(q.h, C.byref(p), 1.0, C.c_void_p(out.data_ptr()), s) == CRN2_EUNSUPPORTED
-    # without alpha blocks the threshold is off, and alpha below it is fine
-    q.pack(qdxt.params(128, 4, P | H))
     q.close()
+
+
+def test_transparent_clusters(gpu):
+    # DXT1A inside multi-block clusters: fully transparent blocks, clusters
+    # with one opaque colour, and mixed ones (crn_qdxt1.cpp:590-626)
+    img = texture.synth(64, 64, 9, texture.RGBA)
+    img[:16, :32, 3] = 0                       # transparent area
+    img[16:32, :32, :3] = (40, 90, 200)        # one colour ...
+    img[16:32, :16, 3] = 10                    # ... partly transparent
+    blocks, desc = level_blocks(texture.mip_chain(img))
+    for ql, flags in ((100, P | A | H), (30, A | H), (200, P | A)):
+        exp, _, _ = oracle.qdxt1(blocks, desc, ql, 4, flags, 128, 1.0)
+        q = qdxt.Qdxt1(blocks, desc, bool(flags & H))
+        got = q.pack(qdxt.params(ql, 4, flags, 128)).cpu().numpy().view(np.uint64)
+        q.close()
+        assert np.array_equal(got, exp), (ql, flags, np.nonzero(got != exp)[0][:5])
 
 
 # ------------------------------------------------------------------ qdxt5
