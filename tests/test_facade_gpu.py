@@ -338,11 +338,11 @@ def test_crn_compress_generates_mips_like_generate_mipmaps(gpu, case):
     assert raw == exp
 
 
-@pytest.mark.parametrize("fmt", [0, 2, 7])
-def test_crn_compress_clustered_dds_matches_the_oracle(gpu, fmt):
+@pytest.mark.parametrize("fmt,kind", [(0, texture.RGB), (0, texture.RGBA), (2, texture.RGBA), (7, texture.NORMAL)])
+def test_crn_compress_clustered_dds_matches_the_oracle(gpu, fmt, kind):
     # DDS below quality 255: qdxt1 / qdxt5 (crn_dds_comp.cpp:158-205) with the
-    # elements interleaved as qdxt_pack does (crn_mipmapped_texture.cpp:2536-2553)
-    kind = texture.RGB if fmt == 0 else (texture.RGBA if fmt == 2 else texture.NORMAL)
+    # elements interleaved as qdxt_pack does (crn_mipmapped_texture.cpp:2536-2553);
+    # DXT1 of an RGBA source: transparent texels below the threshold of 128
     img = texture.synth(96, 64, 13 + fmt, kind)
     chain = texture.mip_chain(img, 4)
     p = comp_params(96, 64, 4, fmt, 1)
