@@ -483,20 +483,46 @@ def qdxt_leg(size):
     import torch
     import oracle
     from crunch2_amd import qdxt, texture
+    import threading
     blocks, desc = qdxt.level_blocks(texture.mip_chain(texture.synth(size, size, 77, texture.RGBA)))
     n = len(blocks)
     b = torch.from_numpy(blocks).cuda()
+    torch.cuda.synchronize()
     P, H = qdxt.PERCEPTUAL, qdxt.HIERARCHICAL
+    s_colour, s_alpha = torch.cuda.Stream(), torch.cuda.Stream()
 
-    def step():
-        q1 = qdxt.Qdxt1(b, desc)
-        c = q1.pack(qdxt.params(128, 4, P | H), 0.75)
-        q5 = qdxt.Qdxt5(b, desc, 3)
-        a = q5.pack(qdxt.params5(128, 4, qdxt.BOTH_BLOCK_TYPES))
-        info = (q1.info(), q5.info())
-        q1.close()
-        q5.close()
-        return c, a, info
+    def colour(res):
+        try:
+            with torch.cuda.stream(s_colour):
+                q1 = qdxt.Qdxt1(b, desc)
+                res["c"] = q1.pack(qdxt.params(128, 4, P | H), 0.75)
+                res["i"] = q1.info()
+                q1.close()
+                s_colour.synchronize()
+        except BaseException as e:          # re-raised by the step
+            res["e"] = e
+
+    def step(concurrent=True):
+        # colour and alpha are independent (qdxt_pack runs them one after the
+        # other): one host thread and stream each, so that their serial tree
+        # rounds overlap, as crn2_compress_dds_clustered runs them
+        res = {}
+        t = threading.Thread(target=colour, args=(res,))
+        t.start()
+        if not concurrent:
+            t.join()
+        try:
+            with torch.cuda.stream(s_alpha):
+                q5 = qdxt.Qdxt5(b, desc, 3)
+                a = q5.pack(qdxt.params5(128, 4, qdxt.BOTH_BLOCK_TYPES))
+                i5 = q5.info()
+                q5.close()
+                s_alpha.synchronize()
+        finally:
+            t.join()
+        if "e" in res:
+            raise res["e"]
+        return res["c"], a, (res["i"], i5)
 
     step()
     torch.cuda.synchronize()
@@ -506,8 +532,12 @@ def qdxt_leg(size):
         c, a, info = step()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    step(concurrent=False)
+    torch.cuda.synchronize()
+    serial_ms = (time.perf_counter() - t0) * 1e3
     qdxt.profile(True)
-    step()
+    step(concurrent=False)        # launch times of the splits without the other stream's work beside them
     torch.cuda.synchronize()
     sms, launches, passes = qdxt.profile(False)
     got_c = c.cpu().numpy().view(np.uint64)
@@ -526,6 +556,7 @@ def qdxt_leg(size):
     return {"workload": "clustered DXT5 (qdxt1 colour + qdxt5 alpha) of one %dx%d RGBA8 texture with its %d-level "
                         "mip chain (%d blocks), quality 128, uber, init + pack" % (size, size, len(desc) // 3, n),
             "mtexels_s": round(texels / (ms * 1e-3) / 1e6, 2), "ms": round(ms, 2), "bit_exact": exact,
+            "how": "colour and alpha on two streams and host threads at once", "serial_ms": round(serial_ms, 2),
             "clusters": {"colour_endpoint": info[0]["endpoint_clusters"], "colour_selector":
                          info[0]["selector_clusters"], "alpha_endpoint": info[1]["endpoint_clusters"],
                          "alpha_selector": info[1]["selector_clusters"]},

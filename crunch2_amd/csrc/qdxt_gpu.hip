@@ -46,7 +46,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "crn2_internal.h"
@@ -1630,7 +1632,25 @@ std::mutex g_qprof_mu;
 
 int dim_slot(uint32_t d) { return d == 2 ? 0 : (d == 6 ? 1 : 2); }
 
-// Device buffers that grow as the trees do.
+// Temporaries of one call live in stream order on the call's stream:
+// freeing one waits for nothing, where hipFree waits for the whole device and
+// would stall the work that other streams (other textures, the other
+// channels of one texture) have in flight.
+template <class T>
+hipError_t tmp_alloc(T** p, size_t bytes, hipStream_t st)
+{
+    return hipMallocAsync(reinterpret_cast<void**>(p), bytes, st);
+}
+
+inline void tmp_free(void* p, hipStream_t st)
+{
+    if (p) (void)hipFreeAsync(p, st);
+}
+
+// Device buffers that grow as the trees do, stream-ordered on the stream of
+// the call that grows them.  A handle's buffers outlive that call: every call
+// on a handle has finished its work on its stream when it returns, and the
+// handle's destroy waits for the stream of its last call.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -1640,22 +1660,19 @@ struct DevBuf {
         size_t nb = bytes ? bytes : 4096;
         while (nb < need) nb *= 2;
         void* q = nullptr;
-        if (hipMalloc(&q, nb) != hipSuccess) return CRN2_ENOMEM;
+        if (tmp_alloc(&q, nb, st) != hipSuccess) return CRN2_ENOMEM;
         if (keep && bytes && hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) {
-            (void)hipFree(q);
+            tmp_free(q, st);
             return CRN2_EHIP;
         }
-        if (p) {
-            (void)hipStreamSynchronize(st);
-            (void)hipFree(p);
-        }
+        tmp_free(p, st);
         p = q;
         bytes = nb;
         return CRN2_OK;
     }
-    void release()
+    void release(hipStream_t st)
     {
-        if (p) (void)hipFree(p);
+        tmp_free(p, st);
         p = nullptr;
         bytes = 0;
     }
@@ -1808,12 +1825,7 @@ struct Forest {
 
     void release()
     {
-        pool.release();
-        cen.release();
-        tw.release();
-        jobs.release();
-        res.release();
-        varbuf.release();
+        for (DevBuf* b : {&pool, &cen, &tw, &jobs, &res, &varbuf}) b->release(st);
     }
     VqArgs args()
     {
@@ -2128,21 +2140,20 @@ extern "C" int crn2_qdxt1_init(const void* blocks, uint32_t nblocks, uint32_t nl
     uint32_t tsize = 1;
     while (tsize < 2 * nblocks) tsize <<= 1;
     auto fail = [&](int code) {
-        if (dch) (void)hipFree(dch);
-        if (lerr) (void)hipFree(lerr);
-        if (table) (void)hipFree(table);
-        if (count) (void)hipFree(count);
+        if (dch) tmp_free(dch, st);
+        if (lerr) tmp_free(lerr, st);
+        if (table) tmp_free(table, st);
+        if (count) tmp_free(count, st);
         crn2_qdxt1_destroy(q);
         return code;
     };
-    if (hipMalloc((void**)&q->vecs, (size_t)nblocks * 6 * 4) != hipSuccess ||
-        hipMalloc((void**)&q->weights, (size_t)nblocks * 4) != hipSuccess ||
-        hipMalloc((void**)&q->enc, nblocks) != hipSuccess)
+    if (tmp_alloc(&q->vecs, (size_t)nblocks * 6 * 4, st) != hipSuccess ||
+        tmp_alloc(&q->weights, (size_t)nblocks * 4, st) != hipSuccess || tmp_alloc(&q->enc, nblocks, st) != hipSuccess)
         return fail(CRN2_ENOMEM);
     if (hier) {
         const uint32_t nc = (uint32_t)chunks.size();
-        if (hipMalloc((void**)&dch, nc * sizeof(TChunk)) != hipSuccess ||
-            hipMalloc((void**)&lerr, (size_t)nc * 9 * 4) != hipSuccess)
+        if (tmp_alloc(&dch, nc * sizeof(TChunk), st) != hipSuccess ||
+            tmp_alloc(&lerr, (size_t)nc * 9 * 4, st) != hipSuccess)
             return fail(CRN2_ENOMEM);
         if (hipMemcpyAsync(dch, chunks.data(), nc * sizeof(TChunk), hipMemcpyHostToDevice, st) != hipSuccess)
             return fail(CRN2_EHIP);
@@ -2155,7 +2166,7 @@ extern "C" int crn2_qdxt1_init(const void* blocks, uint32_t nblocks, uint32_t nl
                            q->vecs, q->weights, q->enc);
     }
     // distinct dxt_fast selector words
-    if (hipMalloc((void**)&table, (size_t)tsize * 8) != hipSuccess || hipMalloc((void**)&count, 4) != hipSuccess)
+    if (tmp_alloc(&table, (size_t)tsize * 8, st) != hipSuccess || tmp_alloc(&count, 4, st) != hipSuccess)
         return fail(CRN2_ENOMEM);
     if (hipMemsetAsync(table, 0, (size_t)tsize * 8, st) != hipSuccess || hipMemsetAsync(count, 0, 4, st) != hipSuccess)
         return fail(CRN2_EHIP);
@@ -2175,10 +2186,10 @@ extern "C" int crn2_qdxt1_init(const void* blocks, uint32_t nblocks, uint32_t nl
     std::vector<Tree*> ts = {&q->tree};
     if (rc == CRN2_OK) rc = q->F.grow(ts);
     if (rc != CRN2_OK) return fail(rc);
-    (void)hipFree(dch);
-    (void)hipFree(lerr);
-    (void)hipFree(table);
-    (void)hipFree(count);
+    tmp_free(dch, st);
+    tmp_free(lerr, st);
+    tmp_free(table, st);
+    tmp_free(count, st);
     dch = nullptr;
     lerr = nullptr;
     table = nullptr;
@@ -2224,6 +2235,7 @@ extern "C" int crn2_qdxt1_pack(crn2_qdxt1* q, const crn2_qdxt1_params* p, float 
     if (quality < 1.0f && p->dxt_quality != 4) return CRN2_EUNSUPPORTED;
     if (quality >= 1.0f && p->dxt_quality == 0) return CRN2_EUNSUPPORTED;
     hipStream_t st = static_cast<hipStream_t>(stream);
+    q->st = st;
     q->F.st = st;
     const float eq = powf(quality, 1.8f * quality_power_mul), sq = powf(quality, 1.65f * quality_power_mul);
     const uint32_t cbs = q->tree.leaves;
@@ -2258,16 +2270,15 @@ extern "C" int crn2_qdxt1_pack(crn2_qdxt1* q, const crn2_qdxt1_params* p, float 
     float* sv = nullptr;
     uint32_t* sw = nullptr;
     auto cleanup = [&]() {
-        (void)hipStreamSynchronize(st);
         for (void* x : {(void*)idx, (void*)px, (void*)tofs, (void*)bcl, (void*)rec, (void*)sel, (void*)sv, (void*)sw,
                         (void*)sofs_d})
-            if (x) (void)hipFree(x);
+            tmp_free(x, st);
     };
-    if (hipMalloc((void**)&idx, (size_t)n * 4) != hipSuccess || hipMalloc((void**)&px, (size_t)n * 64) != hipSuccess ||
-        hipMalloc((void**)&tofs, ((size_t)nc + 1) * 4) != hipSuccess ||
-        hipMalloc((void**)&bcl, (size_t)n * 4) != hipSuccess ||
-        hipMalloc((void**)&rec, (size_t)nc * sizeof(Dxt1Rec)) != hipSuccess ||
-        hipMalloc((void**)&sel, (size_t)n * 16) != hipSuccess) {
+    if (tmp_alloc(&idx, (size_t)n * 4, st) != hipSuccess || tmp_alloc(&px, (size_t)n * 64, st) != hipSuccess ||
+        tmp_alloc(&tofs, ((size_t)nc + 1) * 4, st) != hipSuccess ||
+        tmp_alloc(&bcl, (size_t)n * 4, st) != hipSuccess ||
+        tmp_alloc(&rec, (size_t)nc * sizeof(Dxt1Rec), st) != hipSuccess ||
+        tmp_alloc(&sel, (size_t)n * 16, st) != hipSuccess) {
         cleanup();
         return CRN2_ENOMEM;
     }
@@ -2295,7 +2306,7 @@ extern "C" int crn2_qdxt1_pack(crn2_qdxt1* q, const crn2_qdxt1_params* p, float 
     }
     hipLaunchKernelGGL(cluster_blocks_kernel, dim3(grid(n, 256)), dim3(256), 0, st, idx, bcl, n, rec, sel, out);
     // selector clusters (crn_qdxt1.cpp:843-874, 1006-1020)
-    if (hipMalloc((void**)&sv, (size_t)n * 64) != hipSuccess || hipMalloc((void**)&sw, (size_t)n * 4) != hipSuccess) {
+    if (tmp_alloc(&sv, (size_t)n * 64, st) != hipSuccess || tmp_alloc(&sw, (size_t)n * 4, st) != hipSuccess) {
         cleanup();
         return CRN2_ENOMEM;
     }
@@ -2312,7 +2323,7 @@ extern "C" int crn2_qdxt1_pack(crn2_qdxt1* q, const crn2_qdxt1_params* p, float 
         q->selector_clusters = (uint32_t)scl.size();
         std::vector<uint32_t> sofs;
         rc = S.lists(scl, idx, sofs);
-        if (rc == CRN2_OK && hipMalloc((void**)&sofs_d, sofs.size() * 4) != hipSuccess) rc = CRN2_ENOMEM;
+        if (rc == CRN2_OK && tmp_alloc(&sofs_d, sofs.size() * 4, st) != hipSuccess) rc = CRN2_ENOMEM;
         if (rc == CRN2_OK &&
             hipMemcpyAsync(sofs_d, sofs.data(), sofs.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
             rc = CRN2_EHIP;
@@ -2329,11 +2340,14 @@ extern "C" int crn2_qdxt1_pack(crn2_qdxt1* q, const crn2_qdxt1_params* p, float 
 extern "C" int crn2_qdxt1_destroy(crn2_qdxt1* q)
 {
     if (!q) return CRN2_OK;
+    // the handle's work is done once the stream of its last call is idle;
+    // its buffers then go back in order on the null stream
     if (q->st) (void)hipStreamSynchronize(q->st);
+    q->F.st = nullptr;
     q->F.release();
-    if (q->vecs) (void)hipFree(q->vecs);
-    if (q->weights) (void)hipFree(q->weights);
-    if (q->enc) (void)hipFree(q->enc);
+    tmp_free(q->vecs, nullptr);
+    tmp_free(q->weights, nullptr);
+    tmp_free(q->enc, nullptr);
     delete q;
     return CRN2_OK;
 }
@@ -2362,7 +2376,7 @@ extern "C" int crn2_vq_clusters_device(const float* vecs, const uint32_t* weight
     std::vector<uint32_t> o;
     if (rc == CRN2_OK) {
         t.retrieve(max_clusters ? max_clusters : t.leaves, cl);
-        if (hipMalloc((void**)&didx, (size_t)n * 4) != hipSuccess) rc = CRN2_ENOMEM;
+        if (tmp_alloc(&didx, (size_t)n * 4, F.st) != hipSuccess) rc = CRN2_ENOMEM;
     }
     if (rc == CRN2_OK) rc = F.lists(cl, didx, o);
     if (rc == CRN2_OK && hipMemcpy(idx, didx, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = CRN2_EHIP;
@@ -2371,7 +2385,7 @@ extern "C" int crn2_vq_clusters_device(const float* vecs, const uint32_t* weight
         *nclusters = (uint32_t)cl.size();
         if (codebook_size) *codebook_size = t.leaves;
     }
-    if (didx) (void)hipFree(didx);
+    if (didx) tmp_free(didx, F.st);
     F.release();
     return rc;
 }
@@ -2391,14 +2405,14 @@ extern "C" int crn2_vq_threaded_clusters_device(const float* vecs, const uint32_
     int rc = threaded_clusters(F, n, max_clusters, cl);
     uint32_t* didx = nullptr;
     std::vector<uint32_t> o;
-    if (rc == CRN2_OK && hipMalloc((void**)&didx, (size_t)n * 4) != hipSuccess) rc = CRN2_ENOMEM;
+    if (rc == CRN2_OK && tmp_alloc(&didx, (size_t)n * 4, F.st) != hipSuccess) rc = CRN2_ENOMEM;
     if (rc == CRN2_OK) rc = F.lists(cl, didx, o);
     if (rc == CRN2_OK && hipMemcpy(idx, didx, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = CRN2_EHIP;
     if (rc == CRN2_OK) {
         memcpy(ofs, o.data(), o.size() * 4);
         *nclusters = (uint32_t)cl.size();
     }
-    if (didx) (void)hipFree(didx);
+    if (didx) tmp_free(didx, F.st);
     F.release();
     return rc;
 }
@@ -2454,25 +2468,23 @@ extern "C" int crn2_qdxt5_init(const void* blocks, uint32_t nblocks, uint32_t nl
     uint32_t tsize = 1;
     while (tsize < 2 * nblocks) tsize <<= 1;
     auto release = [&]() {
-        (void)hipStreamSynchronize(st);
-        if (dch) (void)hipFree(dch);
-        if (lerr) (void)hipFree(lerr);
-        if (table) (void)hipFree(table);
-        if (count) (void)hipFree(count);
+        if (dch) tmp_free(dch, st);
+        if (lerr) tmp_free(lerr, st);
+        if (table) tmp_free(table, st);
+        if (count) tmp_free(count, st);
     };
     auto fail = [&](int code) {
         release();
         crn2_qdxt5_destroy(q);
         return code;
     };
-    if (hipMalloc((void**)&q->vecs, (size_t)nblocks * 2 * 4) != hipSuccess ||
-        hipMalloc((void**)&q->weights, (size_t)nblocks * 4) != hipSuccess ||
-        hipMalloc((void**)&q->enc, nblocks) != hipSuccess)
+    if (tmp_alloc(&q->vecs, (size_t)nblocks * 2 * 4, st) != hipSuccess ||
+        tmp_alloc(&q->weights, (size_t)nblocks * 4, st) != hipSuccess || tmp_alloc(&q->enc, nblocks, st) != hipSuccess)
         return fail(CRN2_ENOMEM);
     if (hier) {
         const uint32_t nc = (uint32_t)chunks.size();
-        if (hipMalloc((void**)&dch, nc * sizeof(TChunk)) != hipSuccess ||
-            hipMalloc((void**)&lerr, (size_t)nc * 9 * 4) != hipSuccess)
+        if (tmp_alloc(&dch, nc * sizeof(TChunk), st) != hipSuccess ||
+            tmp_alloc(&lerr, (size_t)nc * 9 * 4, st) != hipSuccess)
             return fail(CRN2_ENOMEM);
         if (hipMemcpyAsync(dch, chunks.data(), nc * sizeof(TChunk), hipMemcpyHostToDevice, st) != hipSuccess)
             return fail(CRN2_EHIP);
@@ -2484,7 +2496,7 @@ extern "C" int crn2_qdxt5_init(const void* blocks, uint32_t nblocks, uint32_t nl
         hipLaunchKernelGGL(qdxt5_block_train_kernel, dim3(grid(nblocks, 256)), dim3(256), 0, st, q->blocks, nblocks,
                            comp_index, q->vecs, q->weights, q->enc);
     }
-    if (hipMalloc((void**)&table, (size_t)tsize * 8) != hipSuccess || hipMalloc((void**)&count, 4) != hipSuccess)
+    if (tmp_alloc(&table, (size_t)tsize * 8, st) != hipSuccess || tmp_alloc(&count, 4, st) != hipSuccess)
         return fail(CRN2_ENOMEM);
     if (hipMemsetAsync(table, 0, (size_t)tsize * 8, st) != hipSuccess || hipMemsetAsync(count, 0, 4, st) != hipSuccess)
         return fail(CRN2_EHIP);
@@ -2537,6 +2549,7 @@ extern "C" int crn2_qdxt5_pack(crn2_qdxt5* q, const crn2_qdxt5_params* p, void* 
     if (p->dxt_quality == 0) return CRN2_EUNSUPPORTED;
     const int use_both = p->flags & 1;
     hipStream_t st = static_cast<hipStream_t>(stream);
+    q->st = st;
     q->F.st = st;
     const float quality = p->quality_level / (float)255;
     const float eq = powf(quality, 2.1f), sq = powf(quality, 1.65f);
@@ -2566,15 +2579,14 @@ extern "C" int crn2_qdxt5_pack(crn2_qdxt5* q, const crn2_qdxt5_params* p, void* 
     uint8_t *vals = nullptr, *sel = nullptr, *kind = nullptr;
     float* sv = nullptr;
     auto cleanup = [&]() {
-        (void)hipStreamSynchronize(st);
         for (void* x : {(void*)idx, (void*)vofs, (void*)bcl, (void*)res, (void*)sw, (void*)sofs_d, (void*)vals,
                         (void*)sel, (void*)kind, (void*)sv})
-            if (x) (void)hipFree(x);
+            tmp_free(x, st);
     };
-    if (hipMalloc((void**)&idx, (size_t)n * 4) != hipSuccess ||
-        hipMalloc((void**)&vofs, ((size_t)nc + 1) * 4) != hipSuccess ||
-        hipMalloc((void**)&bcl, (size_t)n * 4) != hipSuccess || hipMalloc((void**)&res, (size_t)nc * 4) != hipSuccess ||
-        hipMalloc((void**)&vals, (size_t)n * 16) != hipSuccess || hipMalloc((void**)&sel, (size_t)n * 16) != hipSuccess) {
+    if (tmp_alloc(&idx, (size_t)n * 4, st) != hipSuccess ||
+        tmp_alloc(&vofs, ((size_t)nc + 1) * 4, st) != hipSuccess ||
+        tmp_alloc(&bcl, (size_t)n * 4, st) != hipSuccess || tmp_alloc(&res, (size_t)nc * 4, st) != hipSuccess ||
+        tmp_alloc(&vals, (size_t)n * 16, st) != hipSuccess || tmp_alloc(&sel, (size_t)n * 16, st) != hipSuccess) {
         cleanup();
         return CRN2_ENOMEM;
     }
@@ -2603,8 +2615,8 @@ extern "C" int crn2_qdxt5_pack(crn2_qdxt5* q, const crn2_qdxt5_params* p, void* 
         return rc;
     }
     // selector clusters per block type (crn_qdxt5.cpp:700-841)
-    if (hipMalloc((void**)&sv, (size_t)n * 64) != hipSuccess || hipMalloc((void**)&sw, (size_t)n * 4) != hipSuccess ||
-        hipMalloc((void**)&kind, n) != hipSuccess) {
+    if (tmp_alloc(&sv, (size_t)n * 64, st) != hipSuccess || tmp_alloc(&sw, (size_t)n * 4, st) != hipSuccess ||
+        tmp_alloc(&kind, n, st) != hipSuccess) {
         cleanup();
         return CRN2_ENOMEM;
     }
@@ -2642,7 +2654,7 @@ extern "C" int crn2_qdxt5_pack(crn2_qdxt5* q, const crn2_qdxt5_params* p, void* 
         q->selector_clusters = (uint32_t)scl.size();
         std::vector<uint32_t> sofs;
         rc = S.lists(scl, idx, sofs);
-        if (rc == CRN2_OK && hipMalloc((void**)&sofs_d, sofs.size() * 4) != hipSuccess) rc = CRN2_ENOMEM;
+        if (rc == CRN2_OK && tmp_alloc(&sofs_d, sofs.size() * 4, st) != hipSuccess) rc = CRN2_ENOMEM;
         if (rc == CRN2_OK &&
             hipMemcpyAsync(sofs_d, sofs.data(), sofs.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
             rc = CRN2_EHIP;
@@ -2659,11 +2671,14 @@ extern "C" int crn2_qdxt5_pack(crn2_qdxt5* q, const crn2_qdxt5_params* p, void* 
 extern "C" int crn2_qdxt5_destroy(crn2_qdxt5* q)
 {
     if (!q) return CRN2_OK;
+    // the handle's work is done once the stream of its last call is idle;
+    // its buffers then go back in order on the null stream
     if (q->st) (void)hipStreamSynchronize(q->st);
+    q->F.st = nullptr;
     q->F.release();
-    if (q->vecs) (void)hipFree(q->vecs);
-    if (q->weights) (void)hipFree(q->weights);
-    if (q->enc) (void)hipFree(q->enc);
+    tmp_free(q->vecs, nullptr);
+    tmp_free(q->weights, nullptr);
+    tmp_free(q->enc, nullptr);
     delete q;
     return CRN2_OK;
 }
@@ -2745,7 +2760,8 @@ extern "C" int crn2_compress_dds_clustered(const uint8_t* const* levels, uint32_
         }
     const uint32_t n = (uint32_t)(blocks.size() / 16);
     const uint32_t nl = (uint32_t)lcount.size();
-    hipStream_t st = nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return CRN2_EHIP;
     void* dblocks = nullptr;
     unsigned long long *c = nullptr, *a0 = nullptr, *a1 = nullptr;
     auto release = [&]() {
@@ -2757,40 +2773,84 @@ extern "C" int crn2_compress_dds_clustered(const uint8_t* const* levels, uint32_
         release();
         return CRN2_ENOMEM;
     }
-    int rc = CRN2_OK;
-    // colour (qdxt_pack, :2528-2544: codebook curve 0.75 for DXT5's colour)
-    if (dxt1 || dxt5) {
-        crn2_qdxt1* q = nullptr;
-        rc = hipMalloc((void**)&c, (size_t)n * 8) == hipSuccess ? CRN2_OK : CRN2_ENOMEM;
-        if (rc == CRN2_OK) rc = crn2_qdxt1_init(dblocks, n, nl, desc.data(), hier, &q, st);
-        if (rc == CRN2_OK) {
-            crn2_qdxt1_params p;
-            p.quality_level = quality_level;
-            p.dxt_quality = dxt_quality;
-            p.flags = perceptual | ((dxt1 && both) ? CRN2_QDXT_ALPHA_BLOCKS : 0) | (hier ? CRN2_QDXT_HIERARCHICAL : 0);
-            p.dxt1a_alpha_threshold = alpha_threshold;
-            rc = crn2_qdxt1_pack(q, &p, dxt5 ? 0.75f : 1.0f, c, st);
-        }
-        if (q) crn2_qdxt1_destroy(q);
-    }
-    // alpha channels (qdxt_pack_init :2318-2365)
+    // the colour (qdxt_pack, :2528-2544: codebook curve 0.75 for DXT5's
+    // colour) and each alpha channel (qdxt_pack_init :2318-2365) are
+    // independent: one job each, on its own stream and host thread, so that
+    // their serial tree rounds overlap
     const uint32_t comps[2] = {dxn ? (format == CRN2_FMT_DXN_XY ? 0u : 1u) : 3u,
                                format == CRN2_FMT_DXN_XY ? 1u : 0u};
     const int nalpha = dxn ? 2 : ((dxt5 || dxt5a) ? 1 : 0);
+    struct Job {
+        int alpha;            // -1: colour; else the alpha channel's index
+        unsigned long long* dst;
+        int rc;
+    };
+    std::vector<Job> jobs;
+    int rc = CRN2_OK;
+    if (dxt1 || dxt5) {
+        rc = hipMalloc((void**)&c, (size_t)n * 8) == hipSuccess ? CRN2_OK : CRN2_ENOMEM;
+        jobs.push_back({-1, c, CRN2_OK});
+    }
     for (int k = 0; k < nalpha && rc == CRN2_OK; k++) {
         unsigned long long*& dst = k ? a1 : a0;
-        crn2_qdxt5* q = nullptr;
         rc = hipMalloc((void**)&dst, (size_t)n * 8) == hipSuccess ? CRN2_OK : CRN2_ENOMEM;
-        if (rc == CRN2_OK) rc = crn2_qdxt5_init(dblocks, n, nl, desc.data(), hier, comps[k], &q, st);
-        if (rc == CRN2_OK) {
-            crn2_qdxt5_params p;
-            p.quality_level = quality_level;
-            p.dxt_quality = dxt_quality;
-            p.flags = both ? CRN2_QDXT5_BOTH_BLOCK_TYPES : 0;
-            rc = crn2_qdxt5_pack(q, &p, dst, st);
-        }
-        if (q) crn2_qdxt5_destroy(q);
+        jobs.push_back({k, dst, CRN2_OK});
     }
+    if (rc != CRN2_OK) {
+        release();
+        return rc;
+    }
+    auto run = [&](Job& jb) {
+        hipStream_t st = nullptr;
+        if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            jb.rc = CRN2_EHIP;
+            return;
+        }
+        if (jb.alpha < 0) {
+            crn2_qdxt1* q = nullptr;
+            jb.rc = crn2_qdxt1_init(dblocks, n, nl, desc.data(), hier, &q, st);
+            if (jb.rc == CRN2_OK) {
+                crn2_qdxt1_params p;
+                p.quality_level = quality_level;
+                p.dxt_quality = dxt_quality;
+                p.flags = perceptual | ((dxt1 && both) ? CRN2_QDXT_ALPHA_BLOCKS : 0) |
+                          (hier ? CRN2_QDXT_HIERARCHICAL : 0);
+                p.dxt1a_alpha_threshold = alpha_threshold;
+                jb.rc = crn2_qdxt1_pack(q, &p, dxt5 ? 0.75f : 1.0f, jb.dst, st);
+            }
+            if (q) crn2_qdxt1_destroy(q);
+        } else {
+            crn2_qdxt5* q = nullptr;
+            jb.rc = crn2_qdxt5_init(dblocks, n, nl, desc.data(), hier, comps[jb.alpha], &q, st);
+            if (jb.rc == CRN2_OK) {
+                crn2_qdxt5_params p;
+                p.quality_level = quality_level;
+                p.dxt_quality = dxt_quality;
+                p.flags = both ? CRN2_QDXT5_BOTH_BLOCK_TYPES : 0;
+                jb.rc = crn2_qdxt5_pack(q, &p, jb.dst, st);
+            }
+            if (q) crn2_qdxt5_destroy(q);
+        }
+        if (hipStreamSynchronize(st) != hipSuccess && jb.rc == CRN2_OK) jb.rc = CRN2_EHIP;
+        (void)hipStreamDestroy(st);
+    };
+    {
+        // a helper thread that cannot start runs its job here instead
+        std::vector<std::thread> helpers;
+        std::vector<size_t> inline_jobs;
+        for (size_t k = 1; k < jobs.size(); k++) {
+            try {
+                helpers.emplace_back(run, std::ref(jobs[k]));
+            } catch (...) {
+                inline_jobs.push_back(k);
+            }
+        }
+        if (!jobs.empty()) run(jobs[0]);
+        for (size_t k : inline_jobs) run(jobs[k]);
+        for (std::thread& t : helpers) t.join();
+    }
+    for (const Job& jb : jobs)
+        if (jb.rc != CRN2_OK && rc == CRN2_OK) rc = jb.rc;
     std::vector<unsigned long long> hc, ha0, ha1;
     if (rc == CRN2_OK && c) {
         hc.resize(n);
