@@ -599,10 +599,12 @@ int crn2_compress_dds(const uint8_t* const* levels, uint32_t width, uint32_t hei
  * qdxt_pack, crn_mipmapped_texture.cpp:2310-2560): every face's levels
  * through crn2_qdxt1_* (DXT1, DXT5's colour with the 0.75 codebook curve)
  * and crn2_qdxt5_* (DXT5's alpha, DXT5A, DXN's two channels), written by
- * crn2_write_dds.  flags: bit0 perceptual, bit1 both block types, bit2
- * hierarchical.  Formats DXT1, DXT5, DXT5A, DXN_XY, DXN_YX
- * (CRN2_EUNSUPPORTED otherwise, and where crn2_qdxt1_pack is).  *out is
- * freed with crn2_free. */
+ * crn2_write_dds.  The colour and each alpha channel run at once, each on
+ * a stream and host thread of its own (qdxt_pack runs them one after the
+ * other; the results are the same).  flags: bit0 perceptual, bit1 both
+ * block types, bit2 hierarchical.  Formats DXT1, DXT5, DXT5A, DXN_XY,
+ * DXN_YX (CRN2_EUNSUPPORTED otherwise, and where crn2_qdxt1_pack is).
+ * *out is freed with crn2_free. */
 int crn2_compress_dds_clustered(const uint8_t* const* levels, uint32_t width, uint32_t height, uint32_t nlevels,
                                 uint32_t faces, uint32_t format, uint32_t quality_level, int dxt_quality, int flags,
                                 uint32_t alpha_threshold, uint8_t** out, uint32_t* out_size);
@@ -691,7 +693,12 @@ int crn2_mip_gamma_tables(float gamma, float* srgb_to_linear, uint8_t* linear_to
  * Endpoint caching is off in the endpoint search.  Supported: quality 255
  * at any dxt quality but superfast; below 255, uber quality with or without
  * 3-colour blocks (DXT1A transparency included) -- CRN2_EUNSUPPORTED
- * otherwise. */
+ * otherwise.  Streams: init and pack return with their work on `stream`
+ * done; the handle's device buffers are stream-ordered allocations
+ * (hipMallocAsync) that never make a call wait for other streams, so
+ * handles on different streams run concurrently.  Calls on one handle must
+ * not overlap; destroy waits for the stream of the handle's last call, so
+ * call it before destroying that stream. */
 #define CRN2_QDXT_PERCEPTUAL 1
 #define CRN2_QDXT_ALPHA_BLOCKS 2   /* qdxt1_params::m_use_alpha_blocks */
 #define CRN2_QDXT_HIERARCHICAL 4   /* qdxt1_params::m_hierarchical */
@@ -724,7 +731,7 @@ int crn2_qdxt1_destroy(crn2_qdxt1* q);
  * distinct dxt_fast alpha selector words; pack (qdxt5::pack, :843-935)
  * writes one 8-byte DXT5 alpha block per block (first, second, 48 bits of
  * 3-bit selectors, texel 0 lowest) to out_blocks (device).  dxt_quality
- * fast .. uber (superfast: CRN2_EUNSUPPORTED). */
+ * fast .. uber (superfast: CRN2_EUNSUPPORTED).  Streams as crn2_qdxt1. */
 #define CRN2_QDXT5_BOTH_BLOCK_TYPES 1   /* qdxt5_params::m_use_both_block_types */
 typedef struct crn2_qdxt5_params {
     uint32_t quality_level;   /* 0..255 */
