@@ -1099,64 +1099,96 @@ __global__ void __launch_bounds__(256) selector_vec_kernel(const unsigned long l
     w[b] = d < 1 ? 1 : (d > 2048 ? 2048 : d);
 }
 
-// optimize_selectors_task (crn_qdxt1.cpp:714-840), one wave per selector
-// cluster: lane = texel * 4 + selector value, summing that value's error over
-// the group's blocks (exact integer sums).
-__global__ void __launch_bounds__(64) selector_opt_kernel(const uint32_t* __restrict__ blocks,
-                                                          const uint32_t* __restrict__ idx,
-                                                          const uint32_t* __restrict__ ofs,
-                                                          unsigned long long* __restrict__ blk, int perceptual,
-                                                          uint32_t thresh)
+// optimize_selectors_task (crn_qdxt1.cpp:714-840), one workgroup per
+// selector cluster: 16 block slots x 16 texels, each thread summing the
+// errors of all four selector values of its texel over its slot's blocks,
+// then the slots' sums added per (type, texel, value).  The sums are exact
+// integers, so their order does not matter.
+constexpr uint32_t kSelSlots = 16;
+
+__global__ void __launch_bounds__(256) selector_opt_kernel(const uint32_t* __restrict__ blocks,
+                                                           const uint32_t* __restrict__ idx,
+                                                           const uint32_t* __restrict__ ofs,
+                                                           unsigned long long* __restrict__ blk, int perceptual,
+                                                           uint32_t thresh)
 {
-    const uint32_t c = blockIdx.x, me = lane();
+    __shared__ unsigned long long part[kSelSlots][2][16][4];
+    __shared__ uint32_t pcnt[kSelSlots][2];
+    __shared__ uint32_t words[2];
+    __shared__ uint32_t counts[2];
+    const uint32_t c = blockIdx.x, tid = threadIdx.x;
     const uint32_t b0 = ofs[c], m = ofs[c + 1] - b0;
     if (m <= 1) return;
-    const uint32_t t = me >> 2, s = me & 3;
-    unsigned long long err[2] = {0, 0};
+    const uint32_t slot = tid >> 4, t = tid & 15;
+    unsigned long long err[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     uint32_t cnt[2] = {0, 0};
-    for (uint32_t j = 0; j < m; j++) {
+#pragma unroll 2
+    for (uint32_t j = slot; j < m; j += kSelSlots) {
+        const uint32_t b = idx[b0 + j];
+        const unsigned long long x = blk[b];
+        const uint32_t p = blocks[(size_t)b * 16 + t];
+        const uint32_t lo = (uint32_t)(x & 0xFFFF), hi = (uint32_t)((x >> 16) & 0xFFFF);
+        // a 3-colour block with a transparent texel keeps its selectors; the
+        // slot's 16 lanes are 16 consecutive lanes of one wave
+        const unsigned long long ta = __ballot(thresh > 0 && (p >> 24) < thresh);
+        const bool transparent = ((ta >> (lane() & ~15u)) & 0xFFFFull) != 0;
+        if (lo <= hi && transparent) continue;
+        const int typ = lo <= hi ? 1 : 0;
+        int col[4][3];
+        block_colors(lo, hi, col);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const unsigned long long e =
+                cdist(perceptual, ch(p, 0), ch(p, 1), ch(p, 2), col[v][0], col[v][1], col[v][2]);
+            err[0][v] += typ ? 0ull : e;
+            err[1][v] += typ ? e : 0ull;
+        }
+        cnt[typ]++;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        part[slot][0][t][v] = err[0][v];
+        part[slot][1][t][v] = err[1][v];
+    }
+    if (t == 0) {
+        pcnt[slot][0] = cnt[0];
+        pcnt[slot][1] = cnt[1];
+    }
+    __syncthreads();
+    if (tid < 32) {
+        // the first value of least error among the allowed ones (ascending)
+        const uint32_t typ = tid >> 4, tt = tid & 15;
+        uint32_t best_v = 0;
+        unsigned long long best_e = 0xFFFFFFFFFFull;
+        for (uint32_t v = 0; v < (typ ? 3u : 4u); v++) {
+            unsigned long long e = 0;
+            for (uint32_t k = 0; k < kSelSlots; k++) e += part[k][typ][tt][v];
+            if (e < best_e) {
+                best_e = e;
+                best_v = v;
+            }
+        }
+        uint32_t n = 0;
+        for (uint32_t k = 0; k < kSelSlots; k++) n += pcnt[k][typ];
+        // the 16 texels of a type are 16 consecutive lanes of wave 0
+        uint32_t word = best_v << (2 * tt);
+        for (uint32_t o = 1; o < 16; o <<= 1) word |= (uint32_t)__shfl_xor((int)word, (int)o);
+        if (tt == 0) {
+            words[typ] = word;
+            counts[typ] = n;
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < m; j += blockDim.x) {
         const uint32_t b = idx[b0 + j];
         const unsigned long long x = blk[b];
         const uint32_t lo = (uint32_t)(x & 0xFFFF), hi = (uint32_t)((x >> 16) & 0xFFFF);
-        int typ = 0;
-        if (lo <= hi) {
-            typ = 1;
-            if (thresh > 0) {
-                bool ha = false;
-                for (int k = 0; k < 16; k++) ha |= (blocks[(size_t)b * 16 + k] >> 24) < thresh;
-                if (ha) continue;
-            }
-        }
-        int col[4][3];
-        block_colors(lo, hi, col);
-        const uint32_t p = blocks[(size_t)b * 16 + t];
-        err[typ] += cdist(perceptual, ch(p, 0), ch(p, 1), ch(p, 2), col[s][0], col[s][1], col[s][2]);
-        cnt[typ]++;
-    }
-    for (int typ = 0; typ < 2; typ++) {
-        if (cnt[typ] <= 1) continue;
-        // the first value of least error among the allowed ones (s ascending)
-        const uint32_t base = me & ~3u;
-        uint32_t best_s = 0;
-        unsigned long long best_e = 0xFFFFFFFFFFull;
-        for (uint32_t k = 0; k < (typ ? 3u : 4u); k++) {
-            const unsigned long long e = __shfl(err[typ], (int)(base + k));
-            if (e < best_e) {
-                best_e = e;
-                best_s = k;
-            }
-        }
-        uint32_t word = 0;
-        for (uint32_t tt = 0; tt < 16; tt++) word |= (uint32_t)__shfl((int)best_s, (int)(tt * 4)) << (2 * tt);
-        for (uint32_t j = me; j < m; j += 64) {
-            const uint32_t b = idx[b0 + j];
-            const unsigned long long x = blk[b];
-            const uint32_t lo = (uint32_t)(x & 0xFFFF), hi = (uint32_t)((x >> 16) & 0xFFFF);
-            bool mine = typ == 0 ? lo > hi : lo <= hi;
-            if (mine && typ == 1 && thresh > 0)
-                for (int k = 0; k < 16; k++) mine &= (blocks[(size_t)b * 16 + k] >> 24) >= thresh;
-            if (mine) blk[b] = (x & 0xFFFFFFFFull) | ((unsigned long long)word << 32);
-        }
+        const uint32_t typ = lo <= hi ? 1u : 0u;
+        if (counts[typ] <= 1) continue;
+        bool mine = true;
+        if (typ == 1 && thresh > 0)
+            for (int k = 0; k < 16; k++) mine &= (blocks[(size_t)b * 16 + k] >> 24) >= thresh;
+        if (mine) blk[b] = (x & 0xFFFFFFFFull) | ((unsigned long long)words[typ] << 32);
     }
 }
 
@@ -1541,60 +1573,83 @@ __global__ void __launch_bounds__(256) alpha_selector_vec_kernel(const unsigned 
     kind[b] = absolute ? 2 : (a6 ? 1 : 0);
 }
 
-// optimize_selectors_task of qdxt5 (crn_qdxt5.cpp:578-700), one wave per
-// selector cluster: (texel, value) pair p = lane + 64 r (texel p >> 3, value
-// p & 7) summing the squared error over the group's blocks.
-__global__ void __launch_bounds__(64) alpha_selector_opt_kernel(const uint32_t* __restrict__ blocks,
-                                                                const uint32_t* __restrict__ idx,
-                                                                const uint32_t* __restrict__ ofs,
-                                                                unsigned long long* __restrict__ blk, uint32_t comp)
+// optimize_selectors_task of qdxt5 (crn_qdxt5.cpp:578-700), one workgroup
+// per selector cluster: 16 block slots x 16 texels, each thread summing the
+// squared errors of all eight values of its texel over its slot's blocks, by
+// block type, then the slots' sums added (exact integers).
+__global__ void __launch_bounds__(256) alpha_selector_opt_kernel(const uint32_t* __restrict__ blocks,
+                                                                 const uint32_t* __restrict__ idx,
+                                                                 const uint32_t* __restrict__ ofs,
+                                                                 unsigned long long* __restrict__ blk, uint32_t comp)
 {
-    const uint32_t c = blockIdx.x, me = lane();
+    __shared__ unsigned long long part[kSelSlots][2][16][8];
+    __shared__ uint32_t pcnt[kSelSlots][2];
+    __shared__ unsigned long long words[2];
+    __shared__ uint32_t counts[2];
+    const uint32_t c = blockIdx.x, tid = threadIdx.x;
     const uint32_t b0 = ofs[c], m = ofs[c + 1] - b0;
     if (m <= 1) return;
-    unsigned long long err[2][2] = {{0, 0}, {0, 0}};   // [type][r]
-    uint32_t cnt[2] = {0, 0};
-    for (uint32_t j = 0; j < m; j++) {
+    const uint32_t slot = tid >> 4, t = tid & 15;
+    unsigned long long err[2][8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) err[0][v] = err[1][v] = 0;
+    uint32_t cnt1 = 0, cnt = 0;
+#pragma unroll 2
+    for (uint32_t j = slot; j < m; j += kSelSlots) {
         const uint32_t b = idx[b0 + j];
         const unsigned long long x = blk[b];
+        const int a = ch(blocks[(size_t)b * 16 + t], (int)comp);
         const uint32_t lo = (uint32_t)(x & 255), hi = (uint32_t)((x >> 8) & 255);
-        const int typ = lo <= hi ? 1 : 0;
+        const bool typ = lo <= hi;
         uint32_t v[8];
         alpha_values(lo, hi, v);
-        for (int r = 0; r < 2; r++) {
-            const uint32_t p = me + 64 * r, t = p >> 3, s = p & 7;
-            const int e = ch(blocks[(size_t)b * 16 + t], (int)comp) - (int)v[s];
-            err[typ][r] += (unsigned long long)(e * e);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int e = a - (int)v[k];
+            const unsigned long long e2 = (unsigned long long)(e * e);
+            err[0][k] += typ ? 0ull : e2;
+            err[1][k] += typ ? e2 : 0ull;
         }
-        cnt[typ]++;
+        cnt1 += typ ? 1u : 0u;
+        cnt++;
     }
-    for (int typ = 0; typ < 2; typ++) {
-        if (cnt[typ] <= 1) continue;
-        uint32_t best_s[2];
-        for (int r = 0; r < 2; r++) {
-            const uint32_t base = me & ~7u;
-            unsigned long long be = 0xFFFFFFFFFFull;
-            best_s[r] = 0;
-            for (uint32_t k = 0; k < 8; k++) {
-                const unsigned long long e = __shfl(err[typ][r], (int)(base + k));
-                if (e < be) {
-                    be = e;
-                    best_s[r] = k;
-                }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        part[slot][0][t][k] = err[0][k];
+        part[slot][1][t][k] = err[1][k];
+    }
+    if (t == 0) {
+        pcnt[slot][0] = cnt - cnt1;
+        pcnt[slot][1] = cnt1;
+    }
+    __syncthreads();
+    if (tid < 32) {
+        const uint32_t typ = tid >> 4, tt = tid & 15;
+        uint32_t best_v = 0;
+        unsigned long long best_e = 0xFFFFFFFFFFull;
+        for (uint32_t k = 0; k < 8; k++) {
+            unsigned long long e = 0;
+            for (uint32_t q = 0; q < kSelSlots; q++) e += part[q][typ][tt][k];
+            if (e < best_e) {
+                best_e = e;
+                best_v = k;
             }
         }
-        unsigned long long word = 0;
-        for (uint32_t t = 0; t < 16; t++) {
-            const uint32_t r = t >> 3, src = (t & 7) * 8;
-            const uint32_t s = (uint32_t)__shfl((int)best_s[r], (int)src);
-            word |= (unsigned long long)s << (3 * t);
+        uint32_t n = 0;
+        for (uint32_t q = 0; q < kSelSlots; q++) n += pcnt[q][typ];
+        unsigned long long word = (unsigned long long)best_v << (3 * tt);
+        for (uint32_t o = 1; o < 16; o <<= 1) word |= __shfl_xor(word, (int)o);
+        if (tt == 0) {
+            words[typ] = word;
+            counts[typ] = n;
         }
-        for (uint32_t j = me; j < m; j += 64) {
-            const uint32_t b = idx[b0 + j];
-            const unsigned long long x = blk[b];
-            const bool a6 = (uint32_t)(x & 255) <= (uint32_t)((x >> 8) & 255);
-            if ((int)a6 == typ) blk[b] = (x & 0xFFFFull) | (word << 16);
-        }
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < m; j += blockDim.x) {
+        const uint32_t b = idx[b0 + j];
+        const unsigned long long x = blk[b];
+        const uint32_t typ = (uint32_t)(x & 255) <= (uint32_t)((x >> 8) & 255) ? 1u : 0u;
+        if (counts[typ] > 1) blk[b] = (x & 0xFFFFull) | (words[typ] << 16);
     }
 }
 
@@ -2328,7 +2383,7 @@ extern "C" int crn2_qdxt1_pack(crn2_qdxt1* q, const crn2_qdxt1_params* p, float 
             hipMemcpyAsync(sofs_d, sofs.data(), sofs.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
             rc = CRN2_EHIP;
         if (rc == CRN2_OK && !scl.empty())
-            hipLaunchKernelGGL(selector_opt_kernel, dim3((uint32_t)scl.size()), dim3(64), 0, st, q->blocks, idx,
+            hipLaunchKernelGGL(selector_opt_kernel, dim3((uint32_t)scl.size()), dim3(256), 0, st, q->blocks, idx,
                                sofs_d, out, perceptual, thresh);
     }
     if (rc == CRN2_OK && hipStreamSynchronize(st) != hipSuccess) rc = CRN2_EHIP;
@@ -2659,7 +2714,7 @@ extern "C" int crn2_qdxt5_pack(crn2_qdxt5* q, const crn2_qdxt5_params* p, void* 
             hipMemcpyAsync(sofs_d, sofs.data(), sofs.size() * 4, hipMemcpyHostToDevice, st) != hipSuccess)
             rc = CRN2_EHIP;
         if (rc == CRN2_OK)
-            hipLaunchKernelGGL(alpha_selector_opt_kernel, dim3((uint32_t)scl.size()), dim3(64), 0, st, q->blocks,
+            hipLaunchKernelGGL(alpha_selector_opt_kernel, dim3((uint32_t)scl.size()), dim3(256), 0, st, q->blocks,
                                idx, sofs_d, out, q->comp);
     }
     if (rc == CRN2_OK && hipStreamSynchronize(st) != hipSuccess) rc = CRN2_EHIP;
