@@ -432,6 +432,88 @@ __device__ inline void load_vec(const VqArgs& A, uint32_t vi, float (&v)[D])
     for (int c = 0; c < D; c++) v[c] = A.v[(size_t)vi * D + c];
 }
 
+// The list walks below keep U chunks of 64 vectors in flight per round:
+// their index and vector loads are issued together, so a pass over a long
+// list waits for memory once per U chunks instead of once per chunk.
+
+// Sum of the listed vectors' weights (integers: any order).
+__device__ inline unsigned long long list_weight(const VqArgs& A, const uint32_t* __restrict__ list, uint32_t n)
+{
+    constexpr uint32_t U = 8;
+    const uint32_t me = lane();
+    unsigned long long s = 0;
+    for (uint32_t j0 = 0; j0 < n; j0 += 64 * U) {
+        uint32_t vi[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t j = j0 + 64 * u + me;
+            vi[u] = j < n ? list[j] : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) s += vi[u] != 0xFFFFFFFFu ? A.w[vi[u]] : 0u;
+    }
+    return wave_sum_u64(s);
+}
+
+// Number of listed vectors for which is_left(v) holds.
+template <int D, class F>
+__device__ inline uint32_t list_count(const VqArgs& A, const uint32_t* __restrict__ list, uint32_t n, F&& is_left)
+{
+    constexpr uint32_t U = D >= 16 ? 4 : 8;
+    const uint32_t me = lane();
+    uint32_t cnt = 0;
+    for (uint32_t j0 = 0; j0 < n; j0 += 64 * U) {
+        uint32_t vi[U];
+        float v[U][D];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t j = j0 + 64 * u + me;
+            vi[u] = j < n ? list[j] : list[0];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) load_vec<D>(A, vi[u], v[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) cnt += (j0 + 64 * u + me < n && is_left(v[u])) ? 1u : 0u;
+    }
+    return (uint32_t)wave_sum_u64(cnt);
+}
+
+// Stable partition of list[0, n) into out: the vectors for which is_left(v)
+// holds from out[0] on, the others from out[nl] on (nl = their count), each
+// side in list order.
+template <int D, class F>
+__device__ inline void list_partition(const VqArgs& A, const uint32_t* __restrict__ list, uint32_t n, uint32_t nl,
+                                      uint32_t* __restrict__ out, F&& is_left)
+{
+    constexpr uint32_t U = D >= 16 ? 4 : 8;
+    const uint32_t me = lane();
+    const unsigned long long below = me ? (~0ull >> (64 - me)) : 0ull;
+    uint32_t lpos = 0, rpos = nl;
+    for (uint32_t j0 = 0; j0 < n; j0 += 64 * U) {
+        uint32_t vi[U];
+        float v[U][D];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t j = j0 + 64 * u + me;
+            vi[u] = j < n ? list[j] : list[0];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) load_vec<D>(A, vi[u], v[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const bool ok = j0 + 64 * u + me < n;
+            const bool left = ok && is_left(v[u]);
+            const unsigned long long lm = __ballot(left), rm = __ballot(ok && !left);
+            if (ok) {
+                if (left) out[lpos + (uint32_t)__popcll(lm & below)] = vi[u];
+                else out[rpos + (uint32_t)__popcll(rm & below)] = vi[u];
+            }
+            lpos += (uint32_t)__popcll(lm);
+            rpos += (uint32_t)__popcll(rm);
+        }
+    }
+}
+
 // acc + x[0] + x[1] + ... + x[cnt - 1], added one after another; x is 16-byte
 // aligned LDS.  Reads 16 values ahead (four 128-bit loads) while adding.
 __device__ inline float serial_sum(float acc, const float* x, uint32_t cnt)
@@ -893,27 +975,9 @@ __global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* _
         return;
     }
     // the final round's sides (from the centres before its update), stable
-    uint32_t* out = A.pool + sp.child_seg;
-    uint32_t lpos = 0, rpos = nl;
-    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
-        const uint32_t j = j0 + me;
-        bool left = false;
-        uint32_t vi = 0;
-        if (j < n) {
-            float v[D];
-            vi = list[j];
-            load_vec<D>(A, vi, v);
-            left = (double)vsqd<D>(plc, v) < (double)vsqd<D>(prc, v);
-        }
-        const unsigned long long lm = __ballot(j < n && left), rm = __ballot(j < n && !left);
-        const unsigned long long below = me ? (~0ull >> (64 - me)) : 0ull;
-        if (j < n) {
-            if (left) out[lpos + (uint32_t)__popcll(lm & below)] = vi;
-            else out[rpos + (uint32_t)__popcll(rm & below)] = vi;
-        }
-        lpos += (uint32_t)__popcll(lm);
-        rpos += (uint32_t)__popcll(rm);
-    }
+    list_partition<D>(A, list, n, nl, A.pool + sp.child_seg, [&](const float (&v)[D]) {
+        return (double)vsqd<D>(plc, v) < (double)vsqd<D>(prc, v);
+    });
     if (me < D) {
         A.cen[(size_t)sp.left * D + me] = lc[me];
         A.cen[(size_t)sp.right * D + me] = rc[me];
@@ -940,9 +1004,7 @@ __global__ void __launch_bounds__(64) vq_root_kernel(VqArgs A, uint32_t seg, uin
     __shared__ VqLds<D> L;
     const uint32_t* list = A.pool + seg;
     const uint32_t me = lane();
-    unsigned long long tw0 = 0;
-    for (uint32_t j = me; j < n; j += 64) tw0 += A.w[list[j]];
-    tw0 = wave_sum_u64(tw0);
+    const unsigned long long tw0 = list_weight(A, list, n);
     PassStats ps;
     const float acc = side_pass<D>(L, A, list, n, [&](const float (&)[D]) { return 0u; }, ps, exact_sums(A, tw0));
     const unsigned long long tw = wave_sum_u64(ps.w[0]);
@@ -973,9 +1035,7 @@ __global__ void __launch_bounds__(64) vq_division_kernel(VqArgs A, const DivJob*
     const DivJob jb = jobs[blockIdx.x];
     const uint32_t* list = A.pool + jb.seg;
     const uint32_t n = jb.n, me = lane();
-    unsigned long long tw0 = 0;
-    for (uint32_t j = me; j < n; j += 64) tw0 += A.w[list[j]];
-    tw0 = wave_sum_u64(tw0);
+    const unsigned long long tw0 = list_weight(A, list, n);
     PassStats ps;
     const float acc = side_pass<D>(L, A, list, n, [&](const float (&)[D]) { return 0u; }, ps, exact_sums(A, tw0));
     const double tw = (double)wave_sum_u64(ps.w[0]);
@@ -991,38 +1051,14 @@ __global__ void __launch_bounds__(64) vq_division_kernel(VqArgs A, const DivJob*
         covariance<D>(L, A, list, n, cen, 0.0f, oo, 1);
         pca_axis<D>(L, axis);
     }
-    uint32_t nl = 0;
-    for (uint32_t j = me; j < n; j += 64) {
-        float v[D], d[D];
-        load_vec<D>(A, list[j], v);
+    auto is_left = [&](const float (&v)[D]) {
+        float d[D];
 #pragma unroll
         for (int c = 0; c < D; c++) d[c] = v[c] - cen[c];
-        nl += vdot<D>(d, axis) < 0.0f ? 1u : 0u;
-    }
-    nl = (uint32_t)wave_sum_u64(nl);
-    uint32_t* out = A.pool + jb.out_seg;
-    uint32_t lpos = 0, rpos = nl;
-    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
-        const uint32_t j = j0 + me;
-        bool left = false;
-        uint32_t vi = 0;
-        if (j < n) {
-            float v[D], d[D];
-            vi = list[j];
-            load_vec<D>(A, vi, v);
-#pragma unroll
-            for (int c = 0; c < D; c++) d[c] = v[c] - cen[c];
-            left = vdot<D>(d, axis) < 0.0f;
-        }
-        const unsigned long long lm = __ballot(j < n && left), rm = __ballot(j < n && !left);
-        const unsigned long long below = me ? (~0ull >> (64 - me)) : 0ull;
-        if (j < n) {
-            if (left) out[lpos + (uint32_t)__popcll(lm & below)] = vi;
-            else out[rpos + (uint32_t)__popcll(rm & below)] = vi;
-        }
-        lpos += (uint32_t)__popcll(lm);
-        rpos += (uint32_t)__popcll(rm);
-    }
+        return vdot<D>(d, axis) < 0.0f;
+    };
+    const uint32_t nl = list_count<D>(A, list, n, is_left);
+    list_partition<D>(A, list, n, nl, A.pool + jb.out_seg, is_left);
     if (me == 0) nleft[blockIdx.x] = nl;
 }
 
