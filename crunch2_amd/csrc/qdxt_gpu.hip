@@ -572,29 +572,43 @@ __device__ float side_pass(VqLds<D>& L, const VqArgs& A, const uint32_t* __restr
         float pl[D], pr[D];
 #pragma unroll
         for (int c = 0; c < D; c++) pl[c] = pr[c] = 0.0f;
-#pragma unroll 4
-        for (uint32_t j = me; j < n; j += 64) {
-            float v[D];
-            const uint32_t vi = list[j];
-            load_vec<D>(A, vi, v);
-            const uint32_t w = A.w[vi];
-            const uint32_t sd = side_of(v);
-            const float wf = (float)w;
-            const double t = vdot<D>(v, v) * wf;
+        constexpr uint32_t U = D >= 16 ? 4 : 8;
+        for (uint32_t j0 = 0; j0 < n; j0 += 64 * U) {
+            uint32_t vi[U], wu[U];
+            float vu[U][D];
 #pragma unroll
-            for (int c = 0; c < D; c++) {
-                const float p = v[c] * wf;
-                pl[c] += sd == 0 ? p : 0.0f;
-                pr[c] += sd == 0 ? 0.0f : p;
+            for (uint32_t u = 0; u < U; u++) {
+                const uint32_t j = j0 + 64 * u + me;
+                vi[u] = j < n ? list[j] : list[0];
             }
-            if (sd == 0) {
-                st.w[0] += w;
-                st.tt[0] += t;
-                st.cnt[0]++;
-            } else {
-                st.w[1] += w;
-                st.tt[1] += t;
-                st.cnt[1]++;
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                load_vec<D>(A, vi[u], vu[u]);
+                wu[u] = A.w[vi[u]];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++) {
+                if (j0 + 64 * u + me >= n) continue;
+                const float (&v)[D] = vu[u];
+                const uint32_t w = wu[u];
+                const uint32_t sd = side_of(v);
+                const float wf = (float)w;
+                const double t = vdot<D>(v, v) * wf;
+#pragma unroll
+                for (int c = 0; c < D; c++) {
+                    const float p = v[c] * wf;
+                    pl[c] += sd == 0 ? p : 0.0f;
+                    pr[c] += sd == 0 ? 0.0f : p;
+                }
+                if (sd == 0) {
+                    st.w[0] += w;
+                    st.tt[0] += t;
+                    st.cnt[0]++;
+                } else {
+                    st.w[1] += w;
+                    st.tt[1] += t;
+                    st.cnt[1]++;
+                }
             }
         }
 #pragma unroll
@@ -608,7 +622,7 @@ __device__ float side_pass(VqLds<D>& L, const VqArgs& A, const uint32_t* __restr
     constexpr int R = VqLds<D>::kR;
     constexpr uint32_t kB = 64u * R;   // vectors per batch
     float v[R][D];
-    uint32_t w[R];
+    uint32_t w[R], vn[R];   // vn: the indices of the batch after the one loaded
 #pragma unroll
     for (int r = 0; r < R; r++) {
         w[r] = 0;
@@ -618,6 +632,8 @@ __device__ float side_pass(VqLds<D>& L, const VqArgs& A, const uint32_t* __restr
             load_vec<D>(A, vi, v[r]);
             w[r] = A.w[vi];
         }
+        const uint32_t jn = kB + j;
+        vn[r] = jn < n ? list[jn] : 0u;
     }
     const float* src = me < D ? &L.a[me][0] : &L.b[me < 2 * D ? me - D : 0][0];
     for (uint32_t j0 = 0; j0 < n; j0 += kB) {
@@ -645,15 +661,17 @@ __device__ float side_pass(VqLds<D>& L, const VqArgs& A, const uint32_t* __restr
                 }
             }
         }
-        // the next batch loads while this one is summed
+        // the next batch's vectors (their indices are in vn already) and the
+        // indices of the batch after it load while this one is summed
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const uint32_t jn = j0 + kB + (uint32_t)r * 64u + me;
             if (jn < n) {
-                const uint32_t vi = list[jn];
-                load_vec<D>(A, vi, v[r]);
-                w[r] = A.w[vi];
+                load_vec<D>(A, vn[r], v[r]);
+                w[r] = A.w[vn[r]];
             }
+            const uint32_t jnn = jn + kB;
+            vn[r] = jnn < n ? list[jnn] : 0u;
         }
         __syncthreads();
         const uint32_t cnt = n - j0 < kB ? n - j0 : kB;
@@ -737,7 +755,7 @@ __device__ void covariance(VqLds<D>& L, const VqArgs& A, const uint32_t* __restr
     constexpr int R = VqLds<D>::kR;
     constexpr uint32_t kB = 64u * R;
     float v[R][D];
-    uint32_t w[R];
+    uint32_t w[R], vn[R];   // vn: the indices of the batch after the one loaded
 #pragma unroll
     for (int r = 0; r < R; r++) {
         w[r] = 0;
@@ -747,6 +765,8 @@ __device__ void covariance(VqLds<D>& L, const VqArgs& A, const uint32_t* __restr
             load_vec<D>(A, vi, v[r]);
             w[r] = A.w[vi];
         }
+        const uint32_t jn = kB + j;
+        vn[r] = jn < n ? list[jn] : 0u;
     }
     for (uint32_t j0 = 0; j0 < n; j0 += kB) {
 #pragma unroll
@@ -766,10 +786,11 @@ __device__ void covariance(VqLds<D>& L, const VqArgs& A, const uint32_t* __restr
         for (int r = 0; r < R; r++) {
             const uint32_t jn = j0 + kB + (uint32_t)r * 64u + me;
             if (jn < n) {
-                const uint32_t vi = list[jn];
-                load_vec<D>(A, vi, v[r]);
-                w[r] = A.w[vi];
+                load_vec<D>(A, vn[r], v[r]);
+                w[r] = A.w[vn[r]];
             }
+            const uint32_t jnn = jn + kB;
+            vn[r] = jnn < n ? list[jnn] : 0u;
         }
         __syncthreads();
         const uint32_t cnt = n - j0 < kB ? n - j0 : kB;
