@@ -180,3 +180,39 @@ def test_qdxt5_pack_matches_oracle(gpu, case):
     bad = np.nonzero(got != exp)[0]
     assert not len(bad), (len(bad), bad[:5], got[bad[:3]], exp[bad[:3]])
     q.close()
+
+
+def test_colour_and_alpha_on_two_streams_at_once(gpu):
+    # handles on different streams run at once (stream-ordered device
+    # buffers, one host thread each) and give what the oracle gives
+    import threading
+    blocks, desc = texture_blocks(128, 41, texture.RGBA)
+    b = torch.from_numpy(blocks).cuda()
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    res = {}
+
+    def colour():
+        try:
+            with torch.cuda.stream(s1):
+                q = qdxt.Qdxt1(b, desc)
+                res["c"] = q.pack(qdxt.params(100, 4, P | H), 0.75)
+                q.close()
+                s1.synchronize()
+        except BaseException as e:
+            res["e"] = e
+
+    for _ in range(2):
+        t = threading.Thread(target=colour)
+        t.start()
+        with torch.cuda.stream(s2):
+            q5 = qdxt.Qdxt5(b, desc, 3)
+            a = q5.pack(qdxt.params5(100, 4, qdxt.BOTH_BLOCK_TYPES))
+            q5.close()
+            s2.synchronize()
+        t.join()
+        assert "e" not in res, res.get("e")
+        exp_c, _, _ = oracle.qdxt1(blocks, desc, 100, 4, P | H, 128, 0.75)
+        exp_a, _, _ = oracle.qdxt5(blocks, desc, 100, 4, oracle.QDXT5_BOTH | H, 3)
+        assert np.array_equal(res["c"].cpu().numpy().view(np.uint64), exp_c)
+        assert np.array_equal(a.cpu().numpy().view(np.uint64), exp_a)
