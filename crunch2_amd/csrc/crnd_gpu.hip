@@ -101,7 +101,8 @@ struct LevelJob {
     uint32_t crows;              // block rows per index chunk
     uint32_t nchunks;            // index chunks of the level
     uint32_t wg_idx;             // first workgroup of the job in crnd_index_chunk_kernel
-    uint32_t pad;
+    uint32_t need_x;             // 1: the even rows end in a partial group of pairs, the only place the
+                                 // walk reads X; 0: X is neither written nor read
     uint64_t blk_first;          // first block of the job in crnd_index_final_kernel
     uint64_t* ent;               // faces * gh * gw: (parent, index residues) after the chunk pass
     uint64_t* bnd;               // nchunks * gw: the chunks' last rows (resolved in place by crnd_index_carry_kernel)
@@ -455,6 +456,7 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
         guint8* O2 = (guint8*)(j.O2 + P0);
         guint8* E8 = (guint8*)(j.E8 + P0);
         guint8* X = (guint8*)(j.X + P0);
+        const bool need_x = j.need_x != 0;
         {
             constexpr uint32_t U = kTabU3a;
 #pragma unroll 1
@@ -484,7 +486,7 @@ __global__ void __launch_bounds__(kTabThreads) crnd_tables_kernel(const LevelJob
                         const uint32_t o0 = t0[u] >> 10;
                         *(guint32*)(O2 + 4 * r[u]) = (o0 + (ta[u] >> 10)) | ((o1[u] + (tb[u] >> 10)) << 8) |
                                                      ((o0 + ta8[u]) << 16) | ((o1[u] + tb8[u]) << 24);
-                        *(guint16*)(X + 2 * r[u]) = (uint16_t)(t0[u] & 0x3FFu);
+                        if (need_x) *(guint16*)(X + 2 * r[u]) = (uint16_t)(t0[u] & 0x3FFu);
                     }
                 }
             }
@@ -599,13 +601,15 @@ __device__ inline void producer_load(const LevelJob& j, uint32_t tsegs, uint32_t
             r.v[b][i] = po[64 * i + lane];
             r.v[b][4 + i] = pe[64 * i + lane];
         }
-        r.v[b][8] = px[lane];
-        r.v[b][9] = px[64 + lane];
+        if (j.need_x) {
+            r.v[b][8] = px[lane];
+            r.v[b][9] = px[64 + lane];
+        }
     }
 }
 
 template <uint32_t RS>
-__device__ inline void producer_store(WalkLds<RS>& sh, uint32_t s0, uint32_t lane, const SegVecs<RS>& r)
+__device__ inline void producer_store(WalkLds<RS>& sh, uint32_t s0, uint32_t lane, const SegVecs<RS>& r, bool need_x)
 {
     constexpr uint32_t kBatch = RS / 2, kRingSegs = RS;
 #pragma unroll
@@ -619,8 +623,10 @@ __device__ inline void producer_store(WalkLds<RS>& sh, uint32_t s0, uint32_t lan
             po[64 * i + lane] = r.v[b][i];
             pe[64 * i + lane] = r.v[b][4 + i];
         }
-        px[lane] = r.v[b][8];
-        px[64 + lane] = r.v[b][9];
+        if (need_x) {
+            px[lane] = r.v[b][8];
+            px[64 + lane] = r.v[b][9];
+        }
     }
 }
 
@@ -631,6 +637,7 @@ __device__ void walk_producer(const LevelJob& j, WalkLds<RS>& sh, uint32_t lane)
 {
     constexpr uint32_t kBatch = RS / 2, kRingSegs = RS;
     const uint32_t tsegs = j.tlen / kSeg;
+    const bool need_x = j.need_x != 0;
     SegVecs<RS> r;
     uint32_t s = 0;
     producer_load(j, tsegs, s, lane, r);
@@ -642,7 +649,7 @@ __device__ void walk_producer(const LevelJob& j, WalkLds<RS>& sh, uint32_t lane)
             if (spin >= kSpinCap) return;
             __builtin_amdgcn_s_sleep(2);
         }
-        producer_store(sh, s, lane, r);
+        producer_store(sh, s, lane, r, need_x);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         lds_put(&sh.ready, s + kBatch);
         s += kBatch;
@@ -1709,6 +1716,7 @@ static int build_plan(std::vector<PlanPart>& parts, crn2_batch** out)
             }
             j.tex = (uint32_t)texs.size();
             j.gw = gw;
+            j.need_x = ((gw >> 1) % kGroup) != 0 ? 1u : 0u;
             j.gh = gh;
             j.faces = h.faces;
             j.tlen = (uint32_t)align_up((size_t)j.src_size * 8 + 1024, kTabPerWG);
