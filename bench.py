@@ -24,7 +24,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes the walk kernel moves per copy of the 4096^2 DXT1 + DXT5 pair
 # (its FETCH_SIZE + WRITE_SIZE, scaled by a device copy of known size), for
 # the default inputs (GPU dxt_hc + the project's writer)
-WALK_TRAFFIC_PER_COPY = 462.3e6
+WALK_TRAFFIC_PER_COPY = 370.3e6
 STAGES = ("crnd_tables_kernel", "crnd_walk_kernel", "crnd_sym_kernel<even>", "crnd_sym_kernel<odd>",
           "crnd_index_{chunk,carry}_kernel", "crnd_index_final_kernel (+ block write)")
 

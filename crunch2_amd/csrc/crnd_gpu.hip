@@ -188,6 +188,8 @@ __device__ inline uint32_t decode_top(const LdsTables& t, gcuint16* sorted, uint
 //          bits 11-26: the odd-row flags of those pairs, two per pair (the
 //          block below the pair's first / second block has reference code 0)
 //   X[q],  bits 0-7: bits taken by one even-row pair; bits 8-9: its two flags
+//          (only for a level whose even rows end in a partial group of
+//          pairs, LevelJob::need_x: the walk reads X nowhere else)
 // One workgroup computes kTabPerWG consecutive positions in three LDS passes:
 //   1. per position: the reference group symbol and its code length, and the
 //      code lengths of one block's endpoint deltas and of its selectors
