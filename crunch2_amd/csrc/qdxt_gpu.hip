@@ -1581,6 +1581,45 @@ __global__ void __launch_bounds__(64) qdxt5_search_kernel(const uint8_t* __restr
     if (me == 0) res[c] = f | sc << 8;
 }
 
+// dxt_fast::compress_alpha_block (crn_dxt_fast.cpp:788-826) over one
+// cluster's values, qdxt5's superfast endpoint pass (crn_qdxt5.cpp:532-541):
+// low = the largest value, high = the smallest, each selector from the
+// value's position between them.  One wave per cluster.
+__global__ void __launch_bounds__(64) alpha_fast_kernel(const uint8_t* __restrict__ vals,
+                                                        const uint32_t* __restrict__ vofs, uint32_t* __restrict__ res,
+                                                        uint8_t* __restrict__ sel_all)
+{
+    const uint32_t c = blockIdx.x, me = lane();
+    const uint32_t o = vofs[c], n = vofs[c + 1] - o;
+    int mn = 255, mx = 0;
+    for (uint32_t j = me; j < n; j += 64) {
+        const int v = vals[o + j];
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    for (int k = 32; k; k >>= 1) {
+        const int omn = __shfl_xor(mn, k), omx = __shfl_xor(mx, k);
+        mn = omn < mn ? omn : mn;
+        mx = omx > mx ? omx : mx;
+    }
+    const int dist = mx - mn, bias = mn * 7 - (dist >> 1), dist4 = dist * 4, dist2 = dist * 2;
+    for (uint32_t j = me; j < n; j += 64) {
+        int a = (int)vals[o + j] * 7 - bias;
+        int t = (dist4 - a) >> 31;
+        int ind = t & 4;
+        a -= dist4 & t;
+        t = (dist2 - a) >> 31;
+        ind += t & 2;
+        a -= dist2 & t;
+        t = (dist - a) >> 31;
+        ind += t & 1;
+        ind = -ind & 7;
+        ind ^= (2 > ind) ? 1 : 0;
+        sel_all[o + j] = (uint8_t)ind;
+    }
+    if (me == 0) res[c] = (uint32_t)mx | ((uint32_t)mn << 8);
+}
+
 __global__ void __launch_bounds__(256) alpha_blocks_kernel(const uint32_t* __restrict__ idx,
                                                            const uint32_t* __restrict__ block_cluster, uint32_t n,
                                                            const uint32_t* __restrict__ res,
@@ -2658,7 +2697,6 @@ extern "C" int crn2_qdxt5_pack(crn2_qdxt5* q, const crn2_qdxt5_params* p, void* 
     if (!q || !p || !out_blocks || p->quality_level > 255 || p->dxt_quality < 0 || p->dxt_quality > 4 ||
         (p->flags & ~1))
         return CRN2_EINVAL;
-    if (p->dxt_quality == 0) return CRN2_EUNSUPPORTED;
     const int use_both = p->flags & 1;
     hipStream_t st = static_cast<hipStream_t>(stream);
     q->st = st;
@@ -2719,7 +2757,11 @@ extern "C" int crn2_qdxt5_pack(crn2_qdxt5* q, const crn2_qdxt5_params* p, void* 
     }
     hipLaunchKernelGGL(gather_values_kernel, dim3(grid((uint64_t)n * 16, 256)), dim3(256), 0, st, q->blocks, idx, n,
                        q->comp, vals);
-    hipLaunchKernelGGL(qdxt5_search_kernel, dim3(nc), dim3(64), 0, st, vals, vofs, p->dxt_quality, use_both, res, sel);
+    if (p->dxt_quality == 0)
+        hipLaunchKernelGGL(alpha_fast_kernel, dim3(nc), dim3(64), 0, st, vals, vofs, res, sel);
+    else
+        hipLaunchKernelGGL(qdxt5_search_kernel, dim3(nc), dim3(64), 0, st, vals, vofs, p->dxt_quality, use_both, res,
+                           sel);
     hipLaunchKernelGGL(alpha_blocks_kernel, dim3(grid(n, 256)), dim3(256), 0, st, idx, bcl, n, res, sel, out);
     if (quality >= 1.0f) {
         rc = hipStreamSynchronize(st) == hipSuccess ? CRN2_OK : CRN2_EHIP;

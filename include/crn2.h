@@ -731,7 +731,8 @@ int crn2_qdxt1_destroy(crn2_qdxt1* q);
  * distinct dxt_fast alpha selector words; pack (qdxt5::pack, :843-935)
  * writes one 8-byte DXT5 alpha block per block (first, second, 48 bits of
  * 3-bit selectors, texel 0 lowest) to out_blocks (device).  dxt_quality
- * fast .. uber (superfast: CRN2_EUNSUPPORTED).  Streams as crn2_qdxt1. */
+ * superfast .. uber (superfast: dxt_fast's alpha compressor over each
+ * endpoint cluster, crn_qdxt5.cpp:532-541).  Streams as crn2_qdxt1. */
 #define CRN2_QDXT5_BOTH_BLOCK_TYPES 1   /* qdxt5_params::m_use_both_block_types */
 typedef struct crn2_qdxt5_params {
     uint32_t quality_level;   /* 0..255 */

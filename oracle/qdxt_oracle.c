@@ -1232,7 +1232,7 @@ typedef struct {
 int orc_qdxt5(const uint8_t* blocks, uint32_t nblocks, uint32_t nlevels, const uint32_t* level_desc,
               const orc_qdxt5_params* p, uint64_t* out, uint64_t* mid, orc_qdxt1_stats* st)
 {
-    if (!nblocks || p->quality_level > 255 || p->dxt_quality < 1 || p->dxt_quality > 4 || p->comp > 3) return 0;
+    if (!nblocks || p->quality_level > 255 || p->dxt_quality < 0 || p->dxt_quality > 4 || p->comp > 3) return 0;
     const int use_both = p->flags & 1, hier = (p->flags >> 2) & 1;
     float* vecs = (float*)malloc(sizeof(float) * 2 * (size_t)nblocks);
     uint32_t* w = (uint32_t*)malloc(sizeof(uint32_t) * nblocks);
@@ -1274,7 +1274,19 @@ int orc_qdxt5(const uint8_t* blocks, uint32_t nblocks, uint32_t nlevels, const u
             for (int t = 0; t < 16; t++) vals[16 * j + t] = blocks[64 * (size_t)idx[ofs[c] + j] + 4 * t + p->comp];
         uint64_t err;
         uint8_t f, s, bt, ro;
-        orc_dxt5a_compute(vals, 16 * m, p->dxt_quality, use_both, sel, &err, &f, &s, &bt, &ro);
+        if (p->dxt_quality == 0) {
+            /* superfast: dxt_fast::compress_alpha_block over the cluster's
+               texels in place of the optimizer (crn_qdxt5.cpp:532-541) */
+            uint8_t* cp = (uint8_t*)malloc(64 * (size_t)m);
+            for (uint32_t j = 0; j < m; j++) memcpy(cp + 64 * (size_t)j, blocks + 64 * (size_t)idx[ofs[c] + j], 64);
+            uint32_t lo, hi;
+            fast_alpha(16 * m, cp, p->comp, &lo, &hi, sel);
+            free(cp);
+            f = (uint8_t)lo;
+            s = (uint8_t)hi;
+        } else {
+            orc_dxt5a_compute(vals, 16 * m, p->dxt_quality, use_both, sel, &err, &f, &s, &bt, &ro);
+        }
         for (uint32_t j = 0; j < m; j++) {
             uint64_t word = 0;
             for (int t = 0; t < 16; t++) word |= (uint64_t)sel[16 * j + t] << (3 * t);

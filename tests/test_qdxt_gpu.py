@@ -162,6 +162,8 @@ CASES5 = [
     (128, 33, texture.NORMAL, 0, 200, 3, 1),
     (96, 34, texture.NORMAL, 1, 128, 2, 1),
     (64, 35, texture.RGBA, 3, 255, 4, 1),
+    (96, 36, texture.RGBA, 3, 128, 0, 1),       # superfast: dxt_fast's alpha compressor per cluster
+    (64, 37, texture.NORMAL, 1, 255, 0, 0),
 ]
 
 

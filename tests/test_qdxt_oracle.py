@@ -296,6 +296,31 @@ def test_qdxt5_quality_255_is_the_block_search():
                         (word << np.uint64(16)))).all()
 
 
+def _palette_of(lo, hi):
+    lo, hi = int(lo), int(hi)
+    if lo > hi:
+        return [lo, hi] + [(lo * (8 - j) + hi * (j - 1)) // 7 for j in range(2, 8)]
+    return [lo, hi] + [(lo * (6 - j) + hi * (j - 1)) // 5 for j in range(2, 6)] + [0, 255]
+
+
+def test_qdxt5_superfast_at_quality_255():
+    # superfast (crn_qdxt5.cpp:532-541): dxt_fast's alpha compressor on each
+    # block alone -- low = the block's largest value, high = its smallest,
+    # every texel on a palette value within one step of it
+    blocks, desc = texture_blocks(64, 5, texture.RGBA)
+    out, _, _ = oracle.qdxt5(blocks, desc, 255, 0, 5, 3)
+    vals = blocks.reshape(-1, 16, 4)[:, :, 3].astype(np.int64)
+    lo = (out & np.uint64(255)).astype(np.int64)
+    hi = ((out >> np.uint64(8)) & np.uint64(255)).astype(np.int64)
+    assert (lo == vals.max(1)).all() and (hi == vals.min(1)).all()
+    for b in range(0, len(blocks), 7):
+        pal = _palette_of(lo[b], hi[b])
+        step = (lo[b] - hi[b]) / 7.0
+        for t in range(16):
+            s = int((int(out[b]) >> (16 + 3 * t)) & 7)
+            assert abs(pal[s] - vals[b, t]) <= step + 1, (b, t, s, pal, vals[b, t])
+
+
 @pytest.mark.parametrize("q,flags,comp", [(128, 5, 3), (60, 4, 3), (180, 5, 1)])
 def test_qdxt5_selector_optimisation_brute_force(q, flags, comp):
     blocks, desc = texture_blocks(64, 7, texture.RGBA if comp == 3 else texture.NORMAL)
