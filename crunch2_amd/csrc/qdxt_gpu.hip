@@ -372,6 +372,14 @@ __device__ inline float wave_sum_f(float v)
     return v;
 }
 
+// Waves per node in the split and division kernels.  Every wave runs the
+// whole node (the same loads, LDS stores of the same values and in-order
+// sums, so each holds every result); only the covariance entries are split
+// between them, which for D = 16 (136 entries) takes one round of lanes
+// instead of three.
+template <int D>
+constexpr int kVqWaves = D >= 16 ? 3 : 1;
+
 template <int D>
 struct VqLds {
     static constexpr int kE = D * (D + 1) / 2;
@@ -733,12 +741,15 @@ __device__ void covariance(VqLds<D>& L, const VqArgs& A, const uint32_t* __restr
                            const float (&cen)[D], float oo_f, double oo_d, int mode)
 {
     constexpr int kE = VqLds<D>::kE;
-    const uint32_t me = lane();
+    constexpr int W = kVqWaves<D>;
+    const uint32_t me = lane(), wv = threadIdx.x >> 6;
     int ex[3], ey[3];
     float acc[3] = {0.0f, 0.0f, 0.0f};
+    // entry e = lane + 64 (wave + W r): with W waves the rounds per wave drop
+    // to ceil(kE / (64 W))
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        int e = (int)me + 64 * r, x = 0;
+        int e = (int)me + 64 * ((int)wv + W * r), x = 0;
         if (e < kE) {
             int row = D;
             while (e >= row) {
@@ -815,8 +826,12 @@ __device__ void covariance(VqLds<D>& L, const VqArgs& A, const uint32_t* __restr
 template <int D>
 __device__ void pca_axis(VqLds<D>& L, float (&axis)[D])
 {
+    // every wave of the node runs the iteration; only wave 0 stores (the
+    // last step reads L.axis[k] before storing it, which another wave's store
+    // would overtake)
     const uint32_t me = lane();
-    if (me < D) {
+    const bool owner = me < D && threadIdx.x < 64;
+    if (owner) {
         const float t = (float)me * (1.0f / (float)(D - 1));
         L.axis[me] = .75f + (1.25f - .75f) * t;
         L.prev[me] = L.axis[me];
@@ -830,12 +845,12 @@ __device__ void pca_axis(VqLds<D>& L, float (&axis)[D])
         float x = (float)sum;
         if (max_sum != 0.0f) x *= (float)(1.0f / max_sum);
         __syncthreads();
-        if (me < D) L.x[me] = L.prev[me] - x;   // delta
+        if (owner) L.x[me] = L.prev[me] - x;   // delta
         __syncthreads();
         float nrm = L.x[0] * L.x[0];
         for (int i = 1; i < D; i++) nrm += L.x[i] * L.x[i];
         __syncthreads();
-        if (me < D) {
+        if (owner) {
             L.prev[me] = L.axis[me];
             L.axis[me] = x;
         }
@@ -890,7 +905,7 @@ __device__ void furthest(const VqArgs& A, const uint32_t* __restrict__ list, uin
 
 // split_node (crn_clusterizer.h:722-871) of one node per wave.
 template <int D>
-__global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* __restrict__ splits,
+__global__ void __launch_bounds__(192) vq_split_kernel(VqArgs A, const VqSplit* __restrict__ splits,
                                                       VqResult* __restrict__ results)
 {
     __shared__ VqLds<D> L;
@@ -991,7 +1006,7 @@ __global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* _
         r.lvar = r.rvar = 0.0f;
         if (me == 0) {
             results[blockIdx.x] = r;
-            if (A.passes) atomicAdd(A.passes, (unsigned long long)n * npass);
+            if (A.passes && threadIdx.x == 0) atomicAdd(A.passes, (unsigned long long)n * npass);
         }
         return;
     }
@@ -1004,7 +1019,7 @@ __global__ void __launch_bounds__(64) vq_split_kernel(VqArgs A, const VqSplit* _
         A.cen[(size_t)sp.right * D + me] = rc[me];
     }
     if (me == 0) {
-        if (A.passes) atomicAdd(A.passes, (unsigned long long)n * (npass + 1));
+        if (A.passes && threadIdx.x == 0) atomicAdd(A.passes, (unsigned long long)n * (npass + 1));
         A.tw[sp.left] = lw;
         A.tw[sp.right] = rw;
         r.split = 1;
@@ -1049,7 +1064,7 @@ struct DivJob {
 };
 
 template <int D>
-__global__ void __launch_bounds__(64) vq_division_kernel(VqArgs A, const DivJob* __restrict__ jobs,
+__global__ void __launch_bounds__(192) vq_division_kernel(VqArgs A, const DivJob* __restrict__ jobs,
                                                          uint32_t* __restrict__ nleft)
 {
     __shared__ VqLds<D> L;
@@ -2085,9 +2100,9 @@ struct Forest {
             hipEvent_t e0 = nullptr, e1 = nullptr;
             const bool prof = g_qprof.on && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
             if (prof) (void)hipEventRecord(e0, st);
-            if (d == 2) hipLaunchKernelGGL(vq_split_kernel<2>, g, dim3(64), 0, st, a, js, rr);
-            else if (d == 6) hipLaunchKernelGGL(vq_split_kernel<6>, g, dim3(64), 0, st, a, js, rr);
-            else hipLaunchKernelGGL(vq_split_kernel<16>, g, dim3(64), 0, st, a, js, rr);
+            if (d == 2) hipLaunchKernelGGL(vq_split_kernel<2>, g, dim3(64 * kVqWaves<2>), 0, st, a, js, rr);
+            else if (d == 6) hipLaunchKernelGGL(vq_split_kernel<6>, g, dim3(64 * kVqWaves<6>), 0, st, a, js, rr);
+            else hipLaunchKernelGGL(vq_split_kernel<16>, g, dim3(64 * kVqWaves<16>), 0, st, a, js, rr);
             if (prof) (void)hipEventRecord(e1, st);
             rs.resize(sp.size());
             if (hipMemcpyAsync(rs.data(), res.p, sp.size() * sizeof(VqResult), hipMemcpyDeviceToHost, st) !=
@@ -2123,9 +2138,9 @@ struct Forest {
         const DivJob* jp = (const DivJob*)jobs.p;
         uint32_t* o = (uint32_t*)res.p;
         const dim3 g((uint32_t)js.size());
-        if (d == 2) hipLaunchKernelGGL(vq_division_kernel<2>, g, dim3(64), 0, st, a, jp, o);
-        else if (d == 6) hipLaunchKernelGGL(vq_division_kernel<6>, g, dim3(64), 0, st, a, jp, o);
-        else hipLaunchKernelGGL(vq_division_kernel<16>, g, dim3(64), 0, st, a, jp, o);
+        if (d == 2) hipLaunchKernelGGL(vq_division_kernel<2>, g, dim3(64 * kVqWaves<2>), 0, st, a, jp, o);
+        else if (d == 6) hipLaunchKernelGGL(vq_division_kernel<6>, g, dim3(64 * kVqWaves<6>), 0, st, a, jp, o);
+        else hipLaunchKernelGGL(vq_division_kernel<16>, g, dim3(64 * kVqWaves<16>), 0, st, a, jp, o);
         nl.resize(js.size());
         if (hipMemcpyAsync(nl.data(), res.p, js.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)

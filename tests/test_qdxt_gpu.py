@@ -43,7 +43,8 @@ def _vectors(d, n, seed):
 
 
 @pytest.mark.parametrize("d,n,max_size", [(6, 5000, 1000), (6, 3000, 65535), (16, 4000, 700), (2, 2000, 300),
-                                          (6, 1, 10), (6, 2, 10)])
+                                          (6, 1, 10), (6, 2, 10),
+                                          (16, 100000, 3000)])   # large 16-D nodes: in-order sums, three waves
 def test_vq_clusters_match_oracle(gpu, d, n, max_size):
     v, w = _vectors(d, n, d * 7 + n)
     q = oracle.VQ(v, w, max_size)
@@ -54,7 +55,7 @@ def test_vq_clusters_match_oracle(gpu, d, n, max_size):
     q.close()
 
 
-@pytest.mark.parametrize("d,n,maxc", [(16, 6000, 500), (16, 3000, 100), (6, 2500, 300)])
+@pytest.mark.parametrize("d,n,maxc", [(16, 6000, 500), (16, 3000, 100), (6, 2500, 300), (16, 100000, 1000)])
 def test_vq_threaded_clusters_match_oracle(gpu, d, n, maxc):
     v, w = _vectors(d, n, 3 * n + maxc)
     _same_clusters(qdxt.vq_threaded_clusters(v, w, maxc), oracle.vq_threaded_clusters(v, w, maxc), (d, n, maxc))
